@@ -1,0 +1,222 @@
+"""ctypes front end of the FP64 CPU oracle (liborc.so, built from aldic_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by
+the CPU-baseline leg of ``bench.py``, where it is the checker and the CPU baseline.
+The product (the HIP library and its Python mirror) never imports this module.
+
+Array conventions follow aldic_oracle.c: images (ny, nx) C-ordered (x fastest),
+1-based pixel coordinates, interleaved nodal vectors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def build(force=False):
+    so = os.path.join(_HERE, "liborc.so")
+    src = os.path.join(_HERE, "aldic_oracle.c")
+    if force or not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-B" if force else "liborc.so"], cwd=_HERE)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(so):
+            build()
+        L = C.CDLL(so)
+        I, D, V = C.c_int, C.c_double, None
+        L.orc_normalize.argtypes = [f64p, I, I, i32p, f64p, f64p]
+        L.orc_gradient.argtypes = [f64p, I, I, C.c_void_p, f64p, f64p]
+        L.orc_bicubic.argtypes = [f64p, I, I, f64p, f64p, C.c_long, f64p]
+        L.orc_ic_compose.argtypes = [f64p, f64p, C.POINTER(I)]
+        L.orc_icgn6.argtypes = [f64p, f64p, f64p, f64p, I, I, I, f64p, f64p, i32p, D,
+                                f64p, f64p, f64p, i32p, f64p]
+        L.orc_icgn2.argtypes = [f64p, f64p, f64p, f64p, I, I, I, f64p, i32p, f64p, D,
+                                f64p, f64p, f64p, f64p, D, f64p, i32p]
+        L.orc_search_grid.argtypes = [I, I, i32p, I, I, i32p, i32p]
+        L.orc_ncc_full.argtypes = [f64p, I, f64p, I, I, f64p, C.POINTER(I)]
+        L.orc_findpeak.argtypes = [f64p, I, I, f64p]
+        L.orc_search.argtypes = [f64p, f64p, I, I, I, I, I, I, I, I, i32p, f64p, f64p, f64p,
+                                 f64p, f64p, f64p, f64p, i32p]
+        L.orc_subpb2_assemble.argtypes = [f64p, I, i32p, I, D, D, D, f64p, f64p, f64p, f64p,
+                                          i32p, i32p, f64p, f64p]
+        L.orc_pcg.argtypes = [I, i32p, i32p, f64p, f64p, f64p, D, I]
+        L.orc_pcg.restype = I
+        L.orc_nodal_strain.argtypes = [f64p, I, i32p, I, f64p, I, f64p]
+        for fn in ("orc_normalize", "orc_gradient", "orc_bicubic", "orc_ic_compose", "orc_icgn6",
+                   "orc_icgn2", "orc_search_grid", "orc_ncc_full", "orc_findpeak", "orc_search",
+                   "orc_subpb2_assemble", "orc_nodal_strain"):
+            getattr(L, fn).restype = V
+        _LIB = L
+    return _LIB
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def normalize(img, roi=None):
+    img = _f64(img)
+    ny, nx = img.shape
+    roi = _i32(roi if roi is not None else [1, nx, 1, ny])
+    out = np.empty_like(img)
+    ms = np.zeros(2)
+    lib().orc_normalize(img, nx, ny, roi, out, ms)
+    return out, ms
+
+
+def gradient(f, mask=None):
+    f = _f64(f)
+    ny, nx = f.shape
+    gx = np.empty_like(f)
+    gy = np.empty_like(f)
+    m = None
+    if mask is not None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+    lib().orc_gradient(f, nx, ny, m.ctypes.data if m is not None else None, gx, gy)
+    return gx, gy
+
+
+def bicubic(img, X, Y):
+    img = _f64(img)
+    ny, nx = img.shape
+    X = _f64(X).ravel()
+    Y = _f64(Y).ravel()
+    Z = np.empty_like(X)
+    lib().orc_bicubic(img, nx, ny, X, Y, X.size, Z)
+    return Z
+
+
+def ic_compose(P, D):
+    P = _f64(P).copy()
+    ok = C.c_int(0)
+    lib().orc_ic_compose(P, _f64(D), C.byref(ok))
+    return P, ok.value
+
+
+def _ws(ws, S):
+    if np.isscalar(ws):
+        return np.full(S, int(ws), np.int32)
+    return _i32(ws)
+
+
+def icgn6(f, g, gx, gy, coords, U0, ws, tol=1e-2):
+    f, g, gx, gy = map(_f64, (f, g, gx, gy))
+    ny, nx = f.shape
+    coords = _f64(coords).reshape(-1, 2)
+    S = coords.shape[0]
+    U = np.empty(2 * S)
+    F = np.empty(4 * S)
+    H = np.empty((S, 21))
+    it = np.empty(S, np.int32)
+    z = np.empty(S)
+    lib().orc_icgn6(f, g, gx, gy, nx, ny, S, coords, _f64(U0), _ws(ws, S), tol, U, F, H, it, z)
+    return U, F, H, it, z
+
+
+def icgn2(f, g, gx, gy, coords, ws, H21, mu, udual, vdual, Uold, Fold, tol=1e-2):
+    f, g, gx, gy = map(_f64, (f, g, gx, gy))
+    ny, nx = f.shape
+    coords = _f64(coords).reshape(-1, 2)
+    S = coords.shape[0]
+    U = np.empty(2 * S)
+    it = np.empty(S, np.int32)
+    lib().orc_icgn2(f, g, gx, gy, nx, ny, S, coords, _ws(ws, S), _f64(H21), mu, _f64(udual),
+                    _f64(vdual), _f64(Uold), _f64(Fold), tol, U, it)
+    return U, it
+
+
+def search_grid(nx, ny, grid, w, step, s):
+    out = np.zeros(4, np.int32)
+    lib().orc_search_grid(nx, ny, _i32(grid), w, step, _i32(s), out)
+    return [int(v) for v in out]
+
+
+def ncc_full(T, D):
+    T = _f64(T)
+    D = _f64(D)
+    tw = T.shape[0]
+    Dy, Dx = D.shape
+    Cm = np.empty((tw - 1 + Dy, tw - 1 + Dx))
+    ok = C.c_int(0)
+    lib().orc_ncc_full(T, tw, D, Dx, Dy, Cm, C.byref(ok))
+    return Cm, ok.value
+
+
+def findpeak(fmat):
+    """findpeak on a MATLAB-shaped (nr, nc) matrix; returns (xpeak=col, ypeak=row, max)."""
+    fmat = np.asarray(fmat, dtype=np.float64)
+    nr, nc = fmat.shape
+    col_major = np.ascontiguousarray(fmat.T)  # dim 1 (rows) fastest
+    out = np.zeros(3)
+    lib().orc_findpeak(col_major, nr, nc, out)
+    return out[0], out[1], out[2]
+
+
+def search(f, g, grid, w, step, s):
+    f = _f64(f)
+    g = _f64(g)
+    ny, nx = f.shape
+    s = _i32(s if np.ndim(s) else [s, s])
+    x1, y1, Nx, Ny = search_grid(nx, ny, grid, w, step, s)
+    n = Nx * Ny
+    outs = [np.empty(n) for _ in range(7)]
+    st = np.empty(n, np.int32)
+    lib().orc_search(f, g, nx, ny, x1, y1, Nx, Ny, w, step, s, *outs, st)
+    x, y, u, v, cm, pce, ppe = [o.reshape(Ny, Nx) for o in outs]
+    return dict(x=x, y=y, u=u, v=v, ccmax=cm, pce=pce, ppe=ppe, status=st.reshape(Ny, Nx),
+                x1=x1, y1=y1, Nx=Nx, Ny=Ny)
+
+
+def subpb2_assemble(coords, elems, beta, mu, U, F, udual, vdual, alpha=0.0):
+    coords = _f64(coords).reshape(-1, 2)
+    elems = _i32(elems).reshape(-1, 4)
+    nn = coords.shape[0]
+    E = elems.shape[0]
+    ri = np.empty(E * 64, np.int32)
+    ci = np.empty(E * 64, np.int32)
+    val = np.empty(E * 64)
+    b = np.empty(2 * nn)
+    lib().orc_subpb2_assemble(coords, nn, elems, E, beta, mu, alpha, _f64(U), _f64(F),
+                              _f64(udual), _f64(vdual), ri, ci, val, b)
+    return ri, ci, val, b
+
+
+def subpb2(coords, elems, beta, mu, U, F, udual, vdual, alpha=0.0, rtol=1e-14, maxit=20000):
+    """Subpb2.m: assemble (sparse() sums the triplets) and solve in FP64."""
+    import scipy.sparse as sp
+    ri, ci, val, b = subpb2_assemble(coords, elems, beta, mu, U, F, udual, vdual, alpha)
+    n = b.size
+    A = sp.csr_matrix((val, (ri, ci)), shape=(n, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    x = np.zeros(n)
+    it = lib().orc_pcg(n, _i32(A.indptr), _i32(A.indices), _f64(A.data), b, x, rtol, maxit)
+    return x, it
+
+
+def nodal_strain(coords, elems, U, swap_fix=False):
+    coords = _f64(coords).reshape(-1, 2)
+    elems = _i32(elems).reshape(-1, 4)
+    nn = coords.shape[0]
+    F = np.empty(4 * nn)
+    lib().orc_nodal_strain(coords, nn, elems, elems.shape[0], _f64(U), int(swap_fix), F)
+    return F
