@@ -1,0 +1,273 @@
+"""ctypes binding of lib/libaldic.so (the C ABI declared in include/aldic.h).
+
+The library must have been built (``python 2d_aldic_amd/build.py``); loading fails
+loudly otherwise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libaldic.so")
+
+ALDIC_U8, ALDIC_U16, ALDIC_F32, ALDIC_F64 = 0, 1, 2, 3
+_DTYPES = {np.dtype(np.uint8): ALDIC_U8, np.dtype(np.uint16): ALDIC_U16,
+           np.dtype(np.float32): ALDIC_F32, np.dtype(np.float64): ALDIC_F64}
+
+EXPORTS = (
+    "aldic_create", "aldic_destroy", "aldic_last_error", "aldic_set_images",
+    "aldic_set_normalized", "aldic_set_gradients", "aldic_get_images", "aldic_search_grid",
+    "aldic_integer_search", "aldic_icgn6", "aldic_icgn2", "aldic_set_mesh", "aldic_subpb2",
+    "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
+    "aldic_frame_run", "aldic_frame_get", "aldic_frame_times",
+)
+
+
+class FrameOpts(C.Structure):
+    _fields_ = [("ws", C.c_int), ("step", C.c_int), ("s", C.c_int * 2), ("grid", C.c_int * 4),
+                ("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
+                ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run 2d_aldic_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+    P = C.c_void_p
+    L.aldic_create.argtypes = [C.POINTER(P), ip]
+    L.aldic_destroy.argtypes = [P]
+    L.aldic_destroy.restype = None
+    L.aldic_last_error.argtypes = [P]
+    L.aldic_last_error.restype = C.c_char_p
+    L.aldic_set_images.argtypes = [P, vp, vp, ip, ip, ip, vp, vp]
+    L.aldic_set_normalized.argtypes = [P, vp, vp, ip, ip]
+    L.aldic_set_gradients.argtypes = [P, vp, vp]
+    L.aldic_get_images.argtypes = [P, vp, vp, vp, vp]
+    L.aldic_search_grid.argtypes = [ip, ip, vp, ip, ip, vp, vp]
+    L.aldic_integer_search.argtypes = [P, vp, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.aldic_icgn6.argtypes = [P, ip, vp, vp, vp, ip, dp, vp, vp, vp, vp, vp]
+    L.aldic_icgn2.argtypes = [P, ip, vp, vp, ip, vp, dp, vp, vp, vp, dp, vp, vp]
+    L.aldic_set_mesh.argtypes = [P, vp, ip, vp, ip]
+    L.aldic_subpb2.argtypes = [P, dp, dp, dp, vp, vp, vp, vp, dp, ip, vp, vp]
+    L.aldic_nodal_strain.argtypes = [P, vp, ip, vp]
+    L.aldic_bicubic.argtypes = [P, vp, vp, C.c_long, vp]
+    L.aldic_frame_stage.argtypes = [P, vp, vp, ip, ip, ip, vp, vp]
+    L.aldic_frame_setup.argtypes = [P, C.POINTER(FrameOpts), C.POINTER(ip)]
+    L.aldic_frame_run.argtypes = [P]
+    L.aldic_frame_get.argtypes = [P, vp, vp, vp, vp, vp, vp, vp]
+    L.aldic_frame_times.argtypes = [P, vp, ip]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class AldicError(RuntimeError):
+    pass
+
+
+class Context:
+    """One library context on one HIP device; owns the device-side state."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.aldic_create(C.byref(h), int(device))
+        if rc != 0:
+            raise AldicError(f"aldic_create failed ({rc}); is a GPU visible?")
+        self.h = h
+        self.nx = self.ny = 0
+        self.n_nodes = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.aldic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            msg = self.L.aldic_last_error(self.h).decode(errors="replace")
+            raise AldicError(f"{what} failed ({rc}): {msg}")
+
+    # -- images --------------------------------------------------------------------
+    def set_images(self, f, g, mask=None, roi=None):
+        """Raw pair, (ny, nx) C-ordered (x fastest), uint8/uint16/float32/float64."""
+        f = np.ascontiguousarray(f)
+        g = np.ascontiguousarray(g, dtype=f.dtype)
+        ny, nx = f.shape
+        m = None if mask is None else _c(mask, np.uint8)
+        r = None if roi is None else _c(roi, np.int32)
+        self._ck(self.L.aldic_set_images(self.h, _p(f), _p(g), nx, ny, _DTYPES[f.dtype], _p(m),
+                                         _p(r)), "aldic_set_images")
+        self.nx, self.ny = nx, ny
+
+    def set_normalized(self, f, g):
+        f = _c(f, np.float64)
+        g = _c(g, np.float64)
+        ny, nx = f.shape
+        self._ck(self.L.aldic_set_normalized(self.h, _p(f), _p(g), nx, ny), "aldic_set_normalized")
+        self.nx, self.ny = nx, ny
+
+    def set_gradients(self, gx, gy):
+        gx = _c(gx, np.float64)
+        gy = _c(gy, np.float64)
+        self._ck(self.L.aldic_set_gradients(self.h, _p(gx), _p(gy)), "aldic_set_gradients")
+
+    def get_images(self):
+        out = [np.empty((self.ny, self.nx)) for _ in range(4)]
+        self._ck(self.L.aldic_get_images(self.h, *[_p(o) for o in out]), "aldic_get_images")
+        return out
+
+    # -- search ---------------------------------------------------------------------
+    def search_grid(self, grid, ws, step, s):
+        out = np.zeros(4, np.int32)
+        rc = self.L.aldic_search_grid(self.nx, self.ny, _p(_c(grid, np.int32)), int(ws),
+                                      int(step), _p(_c(s, np.int32)), _p(out))
+        if rc != 0:
+            raise AldicError("aldic_search_grid: bad arguments")
+        return [int(v) for v in out]
+
+    def integer_search(self, grid, ws, step, s, qfactors=False):
+        grid = _c(grid, np.int32)
+        s = _c(s, np.int32)
+        x1, y1, Nx, Ny = self.search_grid(grid, ws, step, s)
+        S = Nx * Ny
+        outs = [np.empty(S) for _ in range(5)]
+        qf = np.empty((S, 2)) if qfactors else None
+        st = np.empty(S, np.int32)
+        self._ck(self.L.aldic_integer_search(self.h, _p(grid), int(ws), int(step), _p(s),
+                                             *[_p(o) for o in outs], _p(qf), _p(st)),
+                 "aldic_integer_search")
+        x, y, u, v, cm = [o.reshape(Ny, Nx) for o in outs]
+        return dict(x=x, y=y, u=u, v=v, ccmax=cm, qf=qf, status=st.reshape(Ny, Nx), x1=x1,
+                    y1=y1, Nx=Nx, Ny=Ny)
+
+    # -- IC-GN -----------------------------------------------------------------------
+    def icgn6(self, coords, U0, ws, tol=1e-2):
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        S = coords.shape[0]
+        U0 = _c(U0, np.float64)
+        wsa = None if np.isscalar(ws) else _c(ws, np.int32)
+        U = np.empty(2 * S)
+        F = np.empty(4 * S)
+        H = np.empty((S, 21))
+        it = np.empty(S, np.int32)
+        z = np.empty(S)
+        self._ck(self.L.aldic_icgn6(self.h, S, _p(coords), _p(U0), _p(wsa),
+                                    int(ws) if wsa is None else 0, float(tol), _p(U), _p(F),
+                                    _p(H), _p(it), _p(z)), "aldic_icgn6")
+        return U, F, H, it, z
+
+    def icgn2(self, coords, ws, H21, mu, vdual, Uold, Fold, tol=1e-2):
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        S = coords.shape[0]
+        wsa = None if np.isscalar(ws) else _c(ws, np.int32)
+        U = np.empty(2 * S)
+        it = np.empty(S, np.int32)
+        self._ck(self.L.aldic_icgn2(self.h, S, _p(coords), _p(wsa), int(ws) if wsa is None else 0,
+                                    _p(_c(H21, np.float64)), float(mu), _p(_c(vdual, np.float64)),
+                                    _p(_c(Uold, np.float64)), _p(_c(Fold, np.float64)),
+                                    float(tol), _p(U), _p(it)), "aldic_icgn2")
+        return U, it
+
+    # -- FEM ---------------------------------------------------------------------------
+    def set_mesh(self, coords, elems):
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        elems = _c(elems, np.int32).reshape(-1, 4)
+        self._ck(self.L.aldic_set_mesh(self.h, _p(coords), coords.shape[0], _p(elems),
+                                       elems.shape[0]), "aldic_set_mesh")
+        self.n_nodes = coords.shape[0]
+
+    def subpb2(self, beta, mu, U, F, udual, vdual, alpha=0.0, rtol=1e-12, maxit=5000):
+        n = self.n_nodes
+        Uh = np.empty(2 * n)
+        st = np.zeros(2)
+        self._ck(self.L.aldic_subpb2(self.h, float(beta), float(mu), float(alpha),
+                                     _p(_c(U, np.float64)), _p(_c(F, np.float64)),
+                                     _p(_c(udual, np.float64)), _p(_c(vdual, np.float64)),
+                                     float(rtol), int(maxit), _p(Uh), _p(st)), "aldic_subpb2")
+        return Uh, st
+
+    def nodal_strain(self, U, swap_fix=False):
+        F = np.empty(4 * self.n_nodes)
+        self._ck(self.L.aldic_nodal_strain(self.h, _p(_c(U, np.float64)), int(bool(swap_fix)),
+                                           _p(F)), "aldic_nodal_strain")
+        return F
+
+    def bicubic(self, X, Y):
+        X = _c(X, np.float64).ravel()
+        Y = _c(Y, np.float64).ravel()
+        Z = np.empty_like(X)
+        self._ck(self.L.aldic_bicubic(self.h, _p(X), _p(Y), X.size, _p(Z)), "aldic_bicubic")
+        return Z
+
+    # -- frame ---------------------------------------------------------------------------
+    def frame_stage(self, f, g, mask=None, roi=None):
+        f = np.ascontiguousarray(f)
+        g = np.ascontiguousarray(g, dtype=f.dtype)
+        ny, nx = f.shape
+        m = None if mask is None else _c(mask, np.uint8)
+        r = None if roi is None else _c(roi, np.int32)
+        self._ck(self.L.aldic_frame_stage(self.h, _p(f), _p(g), nx, ny, _DTYPES[f.dtype], _p(m),
+                                          _p(r)), "aldic_frame_stage")
+        self.nx, self.ny = nx, ny
+
+    def frame_setup(self, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
+                    cg_maxit=2000, cg_rtol=1e-10):
+        o = FrameOpts()
+        o.ws, o.step = int(ws), int(step)
+        o.s[0], o.s[1] = (int(s), int(s)) if np.isscalar(s) else (int(s[0]), int(s[1]))
+        for k in range(4):
+            o.grid[k] = int(grid[k])
+        o.tol = tol
+        o.mu = mu
+        o.beta = beta if beta is not None else 1e-2 * step * step * mu
+        o.n_admm = n_admm
+        o.cg_maxit = cg_maxit
+        o.cg_rtol = cg_rtol
+        n = C.c_int(0)
+        self._ck(self.L.aldic_frame_setup(self.h, C.byref(o), C.byref(n)), "aldic_frame_setup")
+        self.frame_nodes = n.value
+        self.frame_opts = o
+        return n.value
+
+    def frame_run(self):
+        self._ck(self.L.aldic_frame_run(self.h), "aldic_frame_run")
+
+    def frame_get(self):
+        S = self.frame_nodes
+        U, F, U6, F6 = np.empty(2 * S), np.empty(4 * S), np.empty(2 * S), np.empty(4 * S)
+        i6, i2 = np.empty(S, np.int32), np.empty(S, np.int32)
+        u0 = np.empty(2 * S)
+        self._ck(self.L.aldic_frame_get(self.h, _p(U), _p(F), _p(U6), _p(F6), _p(i6), _p(i2),
+                                        _p(u0)), "aldic_frame_get")
+        return dict(U=U, F=F, U6=U6, F6=F6, iters6=i6, iters2=i2, U0=u0)
+
+    def frame_times(self):
+        ms = np.zeros(8)
+        self._ck(self.L.aldic_frame_times(self.h, _p(ms), 8), "aldic_frame_times")
+        keys = ("prep", "search", "icgn6", "post6", "subpb2", "icgn2", "strain_dual", "frame")
+        return dict(zip(keys, ms.tolist()))

@@ -1,0 +1,151 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/aldic.h"
+
+namespace aldic {
+
+// One (node, element) incidence: the element, the node's local index in it and the
+// CSR slots of (node, element node b) for b = 0..3.
+struct N2E {
+    int e, a;
+    int slot[4];
+};
+
+// Device buffer that only grows.
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Q4 mesh in node-row (CSR-like) form: for node i, the elements around it and the
+// node's local index in each; plus the scalar node-to-node sparsity pattern (the
+// u- and v-blocks of the global matrix share it).
+struct MeshDev {
+    int nn = 0, E = 0;
+    int nnz = 0;              // scalar pattern entries
+    DBuf coords;              // double2[nn]
+    DBuf elems;               // int4[E]
+    DBuf n2e_ptr, n2e;        // int[nn+1], int2[...] = (element, local index)
+    DBuf rowptr, col;         // scalar CSR pattern, int[nn+1], int[nnz]
+    DBuf aval, mval, diag;    // double[nnz] (beta K + mu M (+alpha mu K)), double[nnz] M, double[nn]
+    DBuf gvec;                // double2[nn]: sum over elements of int grad N_i dA
+    double beta = -1, mu = -1, alpha = -1;  // values aval was assembled for
+};
+
+struct FrameState {
+    bool ready = false;
+    aldic_frame_opts o{};
+    int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
+    DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
+        bad, tmp;
+    std::vector<double> hcoords;
+    std::vector<int> helems;
+    float ms[8] = {0};
+};
+
+}  // namespace aldic
+
+struct aldic_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int nx = 0, ny = 0;
+    bool have_images = false;
+    aldic::DBuf f, g, gx, gy;        // FP32 normalised images and gradients
+    aldic::DBuf rawf, rawg, mask;    // staged raw input
+    int raw_dtype = 0;
+    int roi[4] = {0, 0, 0, 0};
+    bool have_mask = false;
+    aldic::DBuf stats;               // double[8] image statistics
+    aldic::DBuf scratch[8];          // per-call work buffers
+    aldic::MeshDev mesh;
+    bool have_mesh = false;
+    aldic::FrameState fr;
+    hipEvent_t ev[10] = {};
+};
+
+namespace aldic {
+
+// ---- kernels' host launchers (all enqueue on ctx->stream) ---------------------
+hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
+                        hipStream_t st);
+hipError_t launch_prep(const void *rawf, const void *rawg, int dtype, int nx, int ny,
+                       const uint8_t *mask, const double *acc, double n_roi, float *f, float *g,
+                       float *gx, float *gy, hipStream_t st);
+hipError_t launch_gradient(const float *f, int nx, int ny, float *gx, float *gy, hipStream_t st);
+
+struct IcgnArgs {
+    const float *f, *g, *gx, *gy;
+    int nx, ny, S;
+    const double *coords;   // S x 2
+    const int *ws;          // S or nullptr
+    int ws_all;
+    double tol;
+    // 6-DOF
+    const double *U0;       // 2S
+    double *U, *F, *H21, *zncc;
+    int *iters;
+    // 2-DOF
+    const double *Hin, *vdual, *Uold, *Fold;
+    double mu;
+    const int *list;        // subset indices to process (nullptr = all S)
+    int nlist;
+    const int *nlist_dev;   // when set, the list length is read from the device
+};
+hipError_t launch_icgn6(const IcgnArgs &a, int ppl, hipStream_t st);
+hipError_t launch_icgn2(const IcgnArgs &a, int ppl, hipStream_t st);
+int icgn_ppl_for(int ws);   // pixels per lane class that holds a (ws+1)^2 window, 0 if none
+
+struct SearchArgs {
+    const float *f, *g;
+    int nx, ny, x1, y1, Nx, Ny, ws, step, sx, sy;
+    double *x, *y, *u, *v, *ccmax, *qf;
+    int *status;
+    double *U0;             // optional interleaved output (2S), may be nullptr
+};
+hipError_t launch_search(const SearchArgs &a, hipStream_t st);
+size_t search_lds_bytes(int ws, int sx, int sy);
+
+hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const double *Y,
+                          long n, double *Z, hipStream_t st);
+
+// FEM
+hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st);
+hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, const double *F,
+                   const double *udual, const double *vdual, double *b, hipStream_t st);
+hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
+                   DBuf *work, double *stats_dev, hipStream_t st);
+hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F,
+                      hipStream_t st);
+
+// ADMM helpers
+hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);
+hipError_t launch_badpoint(const int *iters, int S, int use_mean_rule, double *U, double *F,
+                           int *bad, double *tmp, hipStream_t st);
+hipError_t launch_fill(int Nx, int Ny, const int *bad, double *U, double *F, int sweeps,
+                       hipStream_t st);
+
+}  // namespace aldic

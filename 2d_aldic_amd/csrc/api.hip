@@ -1,0 +1,831 @@
+// C-ABI entry points (include/aldic.h): argument checks, host<->device staging,
+// launch bucketing, mesh set-up and the device-resident frame driver.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "aldic_internal.h"
+
+using namespace aldic;
+
+namespace {
+
+int fail(aldic_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hipfail(aldic_ctx *c, hipError_t e, const char *what)
+{
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    return fail(c, ALDIC_ERR_HIP, buf);
+}
+
+#define CK(x)                                              \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return hipfail(ctx, e_, #x); \
+    } while (0)
+
+size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case ALDIC_U8: return 1;
+    case ALDIC_U16: return 2;
+    case ALDIC_F32: return 4;
+    case ALDIC_F64: return 8;
+    default: return 0;
+    }
+}
+
+template <class T>
+hipError_t up(DBuf &b, const T *h, size_t n, hipStream_t st)
+{
+    hipError_t e = b.ensure(sizeof(T) * n);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(b.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st);
+}
+
+template <class T>
+hipError_t down(T *h, const void *d, size_t n, hipStream_t st)
+{
+    if (!h) return hipSuccess;
+    return hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, st);
+}
+
+void clip_roi(const int *roi, int nx, int ny, int out[4])
+{
+    out[0] = roi ? std::max(1, roi[0]) : 1;
+    out[1] = roi ? std::min(nx, roi[1]) : nx;
+    out[2] = roi ? std::max(1, roi[2]) : 1;
+    out[3] = roi ? std::min(ny, roi[3]) : ny;
+}
+
+// Normalise + gradient from the staged raw pair (device resident).
+hipError_t run_prep(aldic_ctx *c)
+{
+    const size_t np = (size_t)c->nx * c->ny;
+    hipError_t e;
+    if ((e = c->f.ensure(4 * np)) || (e = c->g.ensure(4 * np)) || (e = c->gx.ensure(4 * np)) ||
+        (e = c->gy.ensure(4 * np)) || (e = c->stats.ensure(8 * sizeof(double))))
+        return e;
+    if ((e = hipMemsetAsync(c->stats.p, 0, 4 * sizeof(double), c->stream))) return e;
+    double *acc = c->stats.as<double>();
+    if ((e = launch_stats(c->rawf.p, c->raw_dtype, c->nx, c->ny, c->roi, acc, c->stream))) return e;
+    if ((e = launch_stats(c->rawg.p, c->raw_dtype, c->nx, c->ny, c->roi, acc + 2, c->stream)))
+        return e;
+    const double nroi = (double)(c->roi[1] - c->roi[0] + 1) * (c->roi[3] - c->roi[2] + 1);
+    return launch_prep(c->rawf.p, c->rawg.p, c->raw_dtype, c->nx, c->ny,
+                       c->have_mask ? c->mask.as<uint8_t>() : nullptr, acc, nroi, c->f.as<float>(),
+                       c->g.as<float>(), c->gx.as<float>(), c->gy.as<float>(), c->stream);
+}
+
+int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
+              const uint8_t *mask, const int *roi)
+{
+    if (!f || !g || nx < 8 || ny < 8) return fail(ctx, ALDIC_ERR_ARG, "images: bad arguments");
+    const size_t es = dtype_size(dtype);
+    if (!es) return fail(ctx, ALDIC_ERR_ARG, "images: unknown dtype");
+    const size_t np = (size_t)nx * ny;
+    CK(ctx->rawf.ensure(es * np));
+    CK(ctx->rawg.ensure(es * np));
+    CK(hipMemcpyAsync(ctx->rawf.p, f, es * np, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->rawg.p, g, es * np, hipMemcpyHostToDevice, ctx->stream));
+    ctx->have_mask = mask != nullptr;
+    if (mask) CK(up(ctx->mask, mask, np, ctx->stream));
+    ctx->nx = nx;
+    ctx->ny = ny;
+    ctx->raw_dtype = dtype;
+    clip_roi(roi, nx, ny, ctx->roi);
+    if (ctx->roi[1] - ctx->roi[0] < 1 || ctx->roi[3] - ctx->roi[2] < 1)
+        return fail(ctx, ALDIC_ERR_ARG, "images: empty ROI");
+    return ALDIC_OK;
+}
+
+// Pixels-per-lane class of each subset; launches each class once.
+int icgn_dispatch(aldic_ctx *ctx, bool six, IcgnArgs a, const int *ws_host)
+{
+    const int S = a.S;
+    std::vector<int> cls(S);
+    for (int j = 0; j < S; ++j) {
+        const int w = ws_host ? ws_host[j] : a.ws_all;
+        cls[j] = icgn_ppl_for(w);
+        if (w < 2 || cls[j] == 0)
+            return fail(ctx, ALDIC_ERR_ARG, "icgn: subset size must be 2..64");
+    }
+    const int classes[3] = {8, 18, 72};
+    std::vector<int> all(S);
+    DBuf &lst = ctx->scratch[7];
+    bool single = std::all_of(cls.begin(), cls.end(), [&](int v) { return v == cls[0]; });
+    int *iters = a.iters;
+    std::vector<int> pending;
+    if (single) {
+        a.list = nullptr;
+        a.nlist = S;
+        CK(six ? launch_icgn6(a, cls[0], ctx->stream) : launch_icgn2(a, cls[0], ctx->stream));
+    } else {
+        for (int ci = 0; ci < 3; ++ci) {
+            std::vector<int> l;
+            for (int j = 0; j < S; ++j)
+                if (cls[j] == classes[ci]) l.push_back(j);
+            if (l.empty()) continue;
+            CK(up(lst, l.data(), l.size(), ctx->stream));
+            a.list = lst.as<int>();
+            a.nlist = (int)l.size();
+            CK(six ? launch_icgn6(a, classes[ci], ctx->stream)
+                   : launch_icgn2(a, classes[ci], ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));  // list buffer is reused
+        }
+    }
+    // windows enlarged by the mask rule beyond their class: relaunch one class up
+    for (int round = 0; round < 3; ++round) {
+        std::vector<int> it(S);
+        CK(down(it.data(), iters, S, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        std::vector<int> l;
+        for (int j = 0; j < S; ++j)
+            if (it[j] == -2) l.push_back(j);
+        if (l.empty()) break;
+        int next = 0;
+        for (int j : l) {
+            const int cur = cls[j];
+            next = cur == 8 ? 18 : 72;
+            cls[j] = next;
+        }
+        if (round == 2 || next == 0) {
+            // beyond the largest class: reported as an indexing error
+            for (int j : l) it[j] = -1;
+            CK(hipMemcpyAsync(iters, it.data(), sizeof(int) * S, hipMemcpyHostToDevice,
+                              ctx->stream));
+            break;
+        }
+        for (int ci = 0; ci < 3; ++ci) {
+            std::vector<int> l2;
+            for (int j : l)
+                if (cls[j] == classes[ci]) l2.push_back(j);
+            if (l2.empty()) continue;
+            CK(up(lst, l2.data(), l2.size(), ctx->stream));
+            a.list = lst.as<int>();
+            a.nlist = (int)l2.size();
+            CK(six ? launch_icgn6(a, classes[ci], ctx->stream)
+                   : launch_icgn2(a, classes[ci], ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    return ALDIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aldic_create(aldic_ctx **out, int device)
+{
+    if (!out) return ALDIC_ERR_ARG;
+    *out = nullptr;
+    aldic_ctx *ctx = new (std::nothrow) aldic_ctx();
+    if (!ctx) return ALDIC_ERR_ARG;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
+    if (e != hipSuccess) {
+        delete ctx;
+        return ALDIC_ERR_HIP;
+    }
+    *out = ctx;
+    return ALDIC_OK;
+}
+
+void aldic_destroy(aldic_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DBuf *bufs[] = {&ctx->f, &ctx->g, &ctx->gx, &ctx->gy, &ctx->rawf, &ctx->rawg, &ctx->mask,
+                    &ctx->stats, &ctx->mesh.coords, &ctx->mesh.elems, &ctx->mesh.n2e_ptr,
+                    &ctx->mesh.n2e, &ctx->mesh.rowptr, &ctx->mesh.col, &ctx->mesh.aval,
+                    &ctx->mesh.mval, &ctx->mesh.diag, &ctx->mesh.gvec};
+    for (DBuf *b : bufs) b->release();
+    for (auto &b : ctx->scratch) b.release();
+    FrameState &F = ctx->fr;
+    DBuf *fb[] = {&F.coords, &F.U0, &F.U6, &F.F6, &F.H21, &F.it6, &F.zncc, &F.Ucur, &F.Fcur,
+                  &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp};
+    for (DBuf *b : fb) b->release();
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *aldic_last_error(const aldic_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int aldic_set_images(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
+                     const uint8_t *mask, const int *roi)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    int r = stage_raw(ctx, f, g, nx, ny, dtype, mask, roi);
+    if (r) return r;
+    CK(run_prep(ctx));
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->have_images = true;
+    return ALDIC_OK;
+}
+
+int aldic_set_normalized(aldic_ctx *ctx, const double *f, const double *g, int nx, int ny)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!f || !g || nx < 8 || ny < 8) return fail(ctx, ALDIC_ERR_ARG, "set_normalized: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    const size_t np = (size_t)nx * ny;
+    std::vector<float> tmp(np);
+    CK(ctx->f.ensure(4 * np));
+    CK(ctx->g.ensure(4 * np));
+    CK(ctx->gx.ensure(4 * np));
+    CK(ctx->gy.ensure(4 * np));
+    for (size_t i = 0; i < np; ++i) tmp[i] = (float)f[i];
+    CK(hipMemcpy(ctx->f.p, tmp.data(), 4 * np, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < np; ++i) tmp[i] = (float)g[i];
+    CK(hipMemcpy(ctx->g.p, tmp.data(), 4 * np, hipMemcpyHostToDevice));
+    ctx->nx = nx;
+    ctx->ny = ny;
+    CK(launch_gradient(ctx->f.as<float>(), nx, ny, ctx->gx.as<float>(), ctx->gy.as<float>(),
+                       ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->have_images = true;
+    return ALDIC_OK;
+}
+
+int aldic_set_gradients(aldic_ctx *ctx, const double *gx, const double *gy)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "set_gradients: no images");
+    if (!gx || !gy) return fail(ctx, ALDIC_ERR_ARG, "set_gradients: NULL");
+    CK(hipSetDevice(ctx->device));
+    const size_t np = (size_t)ctx->nx * ctx->ny;
+    std::vector<float> tmp(np);
+    const int nx = ctx->nx, ny = ctx->ny;
+    const double *src[2] = {gx, gy};
+    DBuf *dst[2] = {&ctx->gx, &ctx->gy};
+    for (int k = 0; k < 2; ++k) {
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const bool in = x >= 3 && x <= nx - 4 && y >= 3 && y <= ny - 4;
+                tmp[(size_t)y * nx + x] = in ? (float)src[k][(size_t)y * nx + x] : 0.f;
+            }
+        CK(hipMemcpy(dst[k]->p, tmp.data(), 4 * np, hipMemcpyHostToDevice));
+    }
+    return ALDIC_OK;
+}
+
+int aldic_get_images(aldic_ctx *ctx, double *f, double *g, double *gx, double *gy)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "get_images: no images");
+    CK(hipSetDevice(ctx->device));
+    const size_t np = (size_t)ctx->nx * ctx->ny;
+    std::vector<float> tmp(np);
+    double *dst[4] = {f, g, gx, gy};
+    DBuf *src[4] = {&ctx->f, &ctx->g, &ctx->gx, &ctx->gy};
+    for (int k = 0; k < 4; ++k) {
+        if (!dst[k]) continue;
+        CK(hipMemcpy(tmp.data(), src[k]->p, 4 * np, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < np; ++i) dst[k][i] = tmp[i];
+    }
+    return ALDIC_OK;
+}
+
+int aldic_search_grid(int nx, int ny, const int *grid, int ws, int step, const int *s, int *out)
+{
+    if (!grid || !s || !out || ws < 2 || step < 1 || s[0] < 0 || s[1] < 0) return ALDIC_ERR_ARG;
+    int gx1 = grid[0], gx2 = grid[1], gy1 = grid[2], gy2 = grid[3];
+    // funIntegerSearch.m:14-25
+    while (gx1 - s[0] - 0.5 * ws < 1) ++gx1;
+    while (gy1 - s[1] - 0.5 * ws < 1) ++gy1;
+    while (gx2 + 1.5 * ws + s[0] > nx) --gx2;
+    while (gy2 + 1.5 * ws + s[1] > ny) --gy2;
+    // :90-91 XList = gridx(1) : step : gridx(2) - step
+    const int Nx = gx2 - step >= gx1 ? (gx2 - step - gx1) / step + 1 : 0;
+    const int Ny = gy2 - step >= gy1 ? (gy2 - step - gy1) / step + 1 : 0;
+    out[0] = gx1;
+    out[1] = gy1;
+    out[2] = Nx;
+    out[3] = Ny;
+    return ALDIC_OK;
+}
+
+int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, const int *s,
+                         double *x, double *y, double *u, double *v, double *ccmax, double *qf,
+                         int *status)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "integer_search: no images");
+    int gr[4];
+    if (aldic_search_grid(ctx->nx, ctx->ny, grid, ws, step, s, gr))
+        return fail(ctx, ALDIC_ERR_ARG, "integer_search: bad grid arguments");
+    const int S = gr[2] * gr[3];
+    if (S <= 0) return ALDIC_OK;
+    if (search_lds_bytes(ws, s[0], s[1]) > 160 * 1024 - 256)
+        return fail(ctx, ALDIC_ERR_ARG, "integer_search: window too large for LDS");
+    CK(hipSetDevice(ctx->device));
+    DBuf *sc = ctx->scratch;
+    for (int k = 0; k < 6; ++k) CK(sc[k].ensure(sizeof(double) * 2 * (size_t)S));
+    SearchArgs a{};
+    a.f = ctx->f.as<float>();
+    a.g = ctx->g.as<float>();
+    a.nx = ctx->nx;
+    a.ny = ctx->ny;
+    a.x1 = gr[0];
+    a.y1 = gr[1];
+    a.Nx = gr[2];
+    a.Ny = gr[3];
+    a.ws = ws;
+    a.step = step;
+    a.sx = s[0];
+    a.sy = s[1];
+    a.x = sc[0].as<double>();
+    a.y = sc[1].as<double>();
+    a.u = sc[2].as<double>();
+    a.v = sc[3].as<double>();
+    a.ccmax = sc[4].as<double>();
+    a.qf = qf ? sc[5].as<double>() : nullptr;
+    CK(ctx->scratch[6].ensure(sizeof(int) * (size_t)S));
+    a.status = ctx->scratch[6].as<int>();
+    a.U0 = nullptr;
+    CK(launch_search(a, ctx->stream));
+    CK(down(x, a.x, S, ctx->stream));
+    CK(down(y, a.y, S, ctx->stream));
+    CK(down(u, a.u, S, ctx->stream));
+    CK(down(v, a.v, S, ctx->stream));
+    CK(down(ccmax, a.ccmax, S, ctx->stream));
+    if (qf) CK(down(qf, a.qf, 2 * (size_t)S, ctx->stream));
+    CK(down(status, a.status, S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, const int *ws,
+                int ws_all, double tol, double *U, double *F, double *H21, int *iters,
+                double *zncc)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "icgn6: no images");
+    if (S < 0 || (S > 0 && (!coords || !U0 || !U || !F || !H21 || !iters)))
+        return fail(ctx, ALDIC_ERR_ARG, "icgn6: bad arguments");
+    if (S == 0) return ALDIC_OK;
+    CK(hipSetDevice(ctx->device));
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], coords, 2 * (size_t)S, ctx->stream));
+    CK(up(sc[1], U0, 2 * (size_t)S, ctx->stream));
+    CK(sc[2].ensure(sizeof(double) * 2 * S));
+    CK(sc[3].ensure(sizeof(double) * 4 * S));
+    CK(sc[4].ensure(sizeof(double) * 21 * (size_t)S));
+    CK(sc[5].ensure(sizeof(double) * (size_t)S + sizeof(int) * (size_t)S));
+    if (ws) CK(up(sc[6], ws, S, ctx->stream));
+    IcgnArgs a{};
+    a.f = ctx->f.as<float>();
+    a.g = ctx->g.as<float>();
+    a.gx = ctx->gx.as<float>();
+    a.gy = ctx->gy.as<float>();
+    a.nx = ctx->nx;
+    a.ny = ctx->ny;
+    a.S = S;
+    a.coords = sc[0].as<double>();
+    a.U0 = sc[1].as<double>();
+    a.ws = ws ? sc[6].as<int>() : nullptr;
+    a.ws_all = ws_all;
+    a.tol = tol;
+    a.U = sc[2].as<double>();
+    a.F = sc[3].as<double>();
+    a.H21 = sc[4].as<double>();
+    a.zncc = sc[5].as<double>();
+    a.iters = reinterpret_cast<int *>(sc[5].as<double>() + S);
+    int r = icgn_dispatch(ctx, true, a, ws);
+    if (r) return r;
+    CK(down(U, a.U, 2 * (size_t)S, ctx->stream));
+    CK(down(F, a.F, 4 * (size_t)S, ctx->stream));
+    CK(down(H21, a.H21, 21 * (size_t)S, ctx->stream));
+    CK(down(iters, a.iters, S, ctx->stream));
+    if (zncc) CK(down(zncc, a.zncc, S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_icgn2(aldic_ctx *ctx, int S, const double *coords, const int *ws, int ws_all,
+                const double *H21, double mu, const double *vdual, const double *Uold,
+                const double *Fold, double tol, double *U, int *iters)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "icgn2: no images");
+    if (S < 0 || (S > 0 && (!coords || !H21 || !vdual || !Uold || !Fold || !U || !iters)))
+        return fail(ctx, ALDIC_ERR_ARG, "icgn2: bad arguments");
+    if (S == 0) return ALDIC_OK;
+    CK(hipSetDevice(ctx->device));
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], coords, 2 * (size_t)S, ctx->stream));
+    CK(up(sc[1], H21, 21 * (size_t)S, ctx->stream));
+    CK(up(sc[2], vdual, 2 * (size_t)S, ctx->stream));
+    CK(up(sc[3], Uold, 2 * (size_t)S, ctx->stream));
+    CK(up(sc[4], Fold, 4 * (size_t)S, ctx->stream));
+    CK(sc[5].ensure(sizeof(double) * 2 * (size_t)S + sizeof(int) * (size_t)S));
+    if (ws) CK(up(sc[6], ws, S, ctx->stream));
+    IcgnArgs a{};
+    a.f = ctx->f.as<float>();
+    a.g = ctx->g.as<float>();
+    a.gx = ctx->gx.as<float>();
+    a.gy = ctx->gy.as<float>();
+    a.nx = ctx->nx;
+    a.ny = ctx->ny;
+    a.S = S;
+    a.coords = sc[0].as<double>();
+    a.Hin = sc[1].as<double>();
+    a.vdual = sc[2].as<double>();
+    a.Uold = sc[3].as<double>();
+    a.Fold = sc[4].as<double>();
+    a.mu = mu;
+    a.ws = ws ? sc[6].as<int>() : nullptr;
+    a.ws_all = ws_all;
+    a.tol = tol;
+    a.U = sc[5].as<double>();
+    a.iters = reinterpret_cast<int *>(sc[5].as<double>() + 2 * (size_t)S);
+    int r = icgn_dispatch(ctx, false, a, ws);
+    if (r) return r;
+    CK(down(U, a.U, 2 * (size_t)S, ctx->stream));
+    CK(down(iters, a.iters, S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elems, int E)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (nn <= 0 || E < 0 || !coords || (E > 0 && !elems))
+        return fail(ctx, ALDIC_ERR_ARG, "set_mesh: bad arguments");
+    for (int e = 0; e < 4 * E; ++e)
+        if (elems[e] < 0 || elems[e] >= nn) return fail(ctx, ALDIC_ERR_ARG, "set_mesh: node index");
+    CK(hipSetDevice(ctx->device));
+    MeshDev &m = ctx->mesh;
+    // node -> element incidences
+    std::vector<int> cnt(nn + 1, 0);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 4; ++a) cnt[elems[4 * e + a] + 1]++;
+    for (int i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
+    std::vector<N2E> n2e(cnt[nn]);
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 4; ++a) {
+            N2E r{};
+            r.e = e;
+            r.a = a;
+            n2e[fillp[elems[4 * e + a]]++] = r;
+        }
+    // scalar pattern
+    std::vector<int> rowptr(nn + 1, 0), col;
+    col.reserve((size_t)nn * 9);
+    for (int i = 0; i < nn; ++i) {
+        std::vector<int> nb;
+        for (int q = cnt[i]; q < cnt[i + 1]; ++q)
+            for (int b = 0; b < 4; ++b) nb.push_back(elems[4 * n2e[q].e + b]);
+        if (nb.empty()) nb.push_back(i);
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        for (int q = cnt[i]; q < cnt[i + 1]; ++q)
+            for (int b = 0; b < 4; ++b) {
+                const int c = elems[4 * n2e[q].e + b];
+                n2e[q].slot[b] = rowptr[i] + (int)(std::lower_bound(nb.begin(), nb.end(), c) - nb.begin());
+            }
+        col.insert(col.end(), nb.begin(), nb.end());
+        rowptr[i + 1] = (int)col.size();
+    }
+    m.nn = nn;
+    m.E = E;
+    m.nnz = (int)col.size();
+    CK(up(m.coords, coords, 2 * (size_t)nn, ctx->stream));
+    if (E > 0) CK(up(m.elems, elems, 4 * (size_t)E, ctx->stream));
+    else CK(m.elems.ensure(16));
+    CK(up(m.n2e_ptr, cnt.data(), cnt.size(), ctx->stream));
+    if (!n2e.empty()) CK(up(m.n2e, n2e.data(), n2e.size(), ctx->stream));
+    else CK(m.n2e.ensure(sizeof(N2E)));
+    CK(up(m.rowptr, rowptr.data(), rowptr.size(), ctx->stream));
+    CK(up(m.col, col.data(), col.size(), ctx->stream));
+    CK(m.aval.ensure(sizeof(double) * col.size()));
+    CK(m.mval.ensure(sizeof(double) * col.size()));
+    CK(m.diag.ensure(sizeof(double) * nn));
+    CK(m.gvec.ensure(sizeof(double) * 2 * nn));
+    CK(hipStreamSynchronize(ctx->stream));
+    m.beta = m.mu = m.alpha = -1;
+    ctx->have_mesh = true;
+    return ALDIC_OK;
+}
+
+int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const double *U,
+                 const double *F, const double *udual, const double *vdual, double rtol,
+                 int maxit, double *Uhat, double *stats)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_mesh) return fail(ctx, ALDIC_ERR_STATE, "subpb2: no mesh");
+    if (!U || !F || !udual || !vdual || !Uhat || maxit < 1 || !(rtol > 0))
+        return fail(ctx, ALDIC_ERR_ARG, "subpb2: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    MeshDev &m = ctx->mesh;
+    const size_t nn = m.nn;
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], U, 2 * nn, ctx->stream));
+    CK(up(sc[1], F, 4 * nn, ctx->stream));
+    CK(up(sc[2], udual, 4 * nn, ctx->stream));
+    CK(up(sc[3], vdual, 2 * nn, ctx->stream));
+    CK(sc[4].ensure(sizeof(double) * 2 * nn));
+    CK(sc[5].ensure(sizeof(double) * 2 * nn));
+    if (m.beta != beta || m.mu != mu || m.alpha != alpha) CK(fem_assemble(m, beta, mu, alpha, ctx->stream));
+    CK(fem_rhs(m, beta, mu, sc[0].as<double>(), sc[1].as<double>(), sc[2].as<double>(),
+               sc[3].as<double>(), sc[4].as<double>(), ctx->stream));
+    static thread_local DBuf work[7];
+    double st[2] = {0, 0};
+    CK(fem_pcg(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream));
+    CK(down(Uhat, sc[5].p, 2 * nn, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (stats) { stats[0] = st[0]; stats[1] = st[1]; }
+    return ALDIC_OK;
+}
+
+int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_mesh) return fail(ctx, ALDIC_ERR_STATE, "nodal_strain: no mesh");
+    if (!U || !F) return fail(ctx, ALDIC_ERR_ARG, "nodal_strain: NULL");
+    CK(hipSetDevice(ctx->device));
+    const size_t nn = ctx->mesh.nn;
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], U, 2 * nn, ctx->stream));
+    CK(sc[1].ensure(sizeof(double) * 4 * nn));
+    CK(fem_strain(ctx->mesh, sc[0].as<double>(), swap_fix, sc[1].as<double>(), ctx->stream));
+    CK(down(F, sc[1].p, 4 * nn, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, double *Z)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "bicubic: no images");
+    if (n < 0 || (n > 0 && (!X || !Y || !Z))) return fail(ctx, ALDIC_ERR_ARG, "bicubic: bad arguments");
+    if (n == 0) return ALDIC_OK;
+    CK(hipSetDevice(ctx->device));
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], X, n, ctx->stream));
+    CK(up(sc[1], Y, n, ctx->stream));
+    CK(sc[2].ensure(sizeof(double) * n));
+    CK(launch_bicubic(ctx->g.as<float>(), ctx->nx, ctx->ny, sc[0].as<double>(), sc[1].as<double>(),
+                      n, sc[2].as<double>(), ctx->stream));
+    CK(down(Z, sc[2].p, n, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+// ---- device-resident frame ------------------------------------------------------------
+int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
+                      const uint8_t *mask, const int *roi)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    CK(hipSetDevice(ctx->device));
+    int r = stage_raw(ctx, f, g, nx, ny, dtype, mask, roi);
+    if (r) return r;
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->fr.ready = false;
+    return ALDIC_OK;
+}
+
+int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
+{
+    if (!ctx || !o) return ALDIC_ERR_ARG;
+    if (!ctx->rawf.p) return fail(ctx, ALDIC_ERR_STATE, "frame_setup: no staged images");
+    if (o->n_admm < 0 || o->cg_maxit < 1 || !(o->cg_rtol > 0))
+        return fail(ctx, ALDIC_ERR_ARG, "frame_setup: bad options");
+    if (icgn_ppl_for(o->ws) == 0) return fail(ctx, ALDIC_ERR_ARG, "frame_setup: ws must be 2..64");
+    if (search_lds_bytes(o->ws, o->s[0], o->s[1]) > 160 * 1024 - 256)
+        return fail(ctx, ALDIC_ERR_ARG, "frame_setup: search window too large for LDS");
+    CK(hipSetDevice(ctx->device));
+    FrameState &F = ctx->fr;
+    F.o = *o;
+    int gr[4];
+    if (aldic_search_grid(ctx->nx, ctx->ny, o->grid, o->ws, o->step, o->s, gr))
+        return fail(ctx, ALDIC_ERR_ARG, "frame_setup: bad grid");
+    F.x1 = gr[0];
+    F.y1 = gr[1];
+    F.Nx = gr[2];
+    F.Ny = gr[3];
+    F.S = F.Nx * F.Ny;
+    if (F.Nx < 2 || F.Ny < 2) return fail(ctx, ALDIC_ERR_ARG, "frame_setup: fewer than 2x2 subsets");
+    const int S = F.S;
+    // MeshSetUp.m:23-39 -- node (k, l) at the subset centre, x fastest; Q4 elements
+    F.hcoords.resize(2 * (size_t)S);
+    for (int l = 0; l < F.Ny; ++l)
+        for (int k = 0; k < F.Nx; ++k) {
+            F.hcoords[2 * ((size_t)l * F.Nx + k)] = F.x1 + k * o->step + 0.5 * o->ws;
+            F.hcoords[2 * ((size_t)l * F.Nx + k) + 1] = F.y1 + l * o->step + 0.5 * o->ws;
+        }
+    F.helems.clear();
+    for (int l = 0; l + 1 < F.Ny; ++l)
+        for (int k = 0; k + 1 < F.Nx; ++k) {
+            const int n0 = l * F.Nx + k;
+            F.helems.insert(F.helems.end(), {n0, n0 + 1, n0 + F.Nx + 1, n0 + F.Nx});
+        }
+    int r = aldic_set_mesh(ctx, F.hcoords.data(), S, F.helems.data(), (int)F.helems.size() / 4);
+    if (r) return r;
+    CK(up(F.coords, F.hcoords.data(), F.hcoords.size(), ctx->stream));
+    const size_t sd = sizeof(double) * S;
+    DBuf *v2[] = {&F.U0, &F.U6, &F.Ucur, &F.U1, &F.U2, &F.vdual};
+    for (DBuf *b : v2) CK(b->ensure(2 * sd));
+    DBuf *v4[] = {&F.F6, &F.Fcur, &F.F1, &F.F2, &F.udual};
+    for (DBuf *b : v4) CK(b->ensure(4 * sd));
+    CK(F.H21.ensure(21 * sd));
+    CK(F.zncc.ensure(sd));
+    CK(F.it6.ensure(sizeof(int) * S));
+    CK(F.it2.ensure(sizeof(int) * S));
+    CK(F.bad.ensure(sizeof(int) * S));
+    CK(F.tmp.ensure(2 * sd));
+    CK(ctx->scratch[5].ensure(2 * sd + 64));
+    CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    F.ready = true;
+    if (n_nodes) *n_nodes = S;
+    return ALDIC_OK;
+}
+
+int aldic_frame_run(aldic_ctx *ctx)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    FrameState &F = ctx->fr;
+    if (!F.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_run: frame_setup first");
+    CK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const aldic_frame_opts &o = F.o;
+    const int S = F.S;
+    static thread_local DBuf work[7];
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t *ev = ctx->ev;
+
+    CK(hipEventRecord(ev[0], st));
+    CK(run_prep(ctx));
+    ctx->have_images = true;
+    CK(hipEventRecord(ev[1], st));
+    // integer search -> U0
+    SearchArgs sa{};
+    sa.f = ctx->f.as<float>();
+    sa.g = ctx->g.as<float>();
+    sa.nx = ctx->nx;
+    sa.ny = ctx->ny;
+    sa.x1 = F.x1;
+    sa.y1 = F.y1;
+    sa.Nx = F.Nx;
+    sa.Ny = F.Ny;
+    sa.ws = o.ws;
+    sa.step = o.step;
+    sa.sx = o.s[0];
+    sa.sy = o.s[1];
+    double *sbuf = F.tmp.as<double>();
+    CK(ctx->scratch[4].ensure(sizeof(double) * 4 * S + sizeof(int) * S));
+    double *s4 = ctx->scratch[4].as<double>();
+    sa.x = s4;
+    sa.y = s4 + S;
+    sa.u = s4 + 2 * (size_t)S;
+    sa.v = s4 + 3 * (size_t)S;
+    sa.ccmax = sbuf;
+    sa.qf = nullptr;
+    sa.status = reinterpret_cast<int *>(s4 + 4 * (size_t)S);
+    sa.U0 = F.U0.as<double>();
+    CK(launch_search(sa, st));
+    // search NaNs (constant templates) are filled like Init's inpaint_nans
+    CK(hipEventRecord(ev[2], st));
+
+    // 6-DOF IC-GN (LocalICGN)
+    IcgnArgs a{};
+    a.f = ctx->f.as<float>();
+    a.g = ctx->g.as<float>();
+    a.gx = ctx->gx.as<float>();
+    a.gy = ctx->gy.as<float>();
+    a.nx = ctx->nx;
+    a.ny = ctx->ny;
+    a.S = S;
+    a.coords = F.coords.as<double>();
+    a.ws = nullptr;
+    a.ws_all = o.ws;
+    a.tol = o.tol;
+    a.U0 = F.U0.as<double>();
+    a.U = F.U6.as<double>();
+    a.F = F.F6.as<double>();
+    a.H21 = F.H21.as<double>();
+    a.zncc = F.zncc.as<double>();
+    a.iters = F.it6.as<int>();
+    a.nlist = S;
+    const int ppl = icgn_ppl_for(o.ws);
+    CK(launch_icgn6(a, ppl, st));
+    CK(hipEventRecord(ev[3], st));
+    // LocalICGN post-processing: bad-point rule and refill
+    CK(hipMemcpyAsync(F.U1.p, F.U6.p, sizeof(double) * 2 * S, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(F.F1.p, F.F6.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
+    CK(launch_badpoint(F.it6.as<int>(), S, 1, F.U1.as<double>(), F.F1.as<double>(),
+                       F.bad.as<int>(), nullptr, st));
+    CK(launch_fill(F.Nx, F.Ny, F.bad.as<int>(), F.U1.as<double>(), F.F1.as<double>(), 12, st));
+    CK(hipEventRecord(ev[4], st));
+
+    // step 1 global step with zero duals
+    CK(hipMemsetAsync(F.udual.p, 0, sizeof(double) * 4 * S, st));
+    CK(hipMemsetAsync(F.vdual.p, 0, sizeof(double) * 2 * S, st));
+    double *b = ctx->scratch[5].as<double>();
+    auto global_step = [&](float &t_sub2, float &t_str) -> int {
+        CK(hipEventRecord(ev[5], st));
+        CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F.F1.as<double>(),
+                   F.udual.as<double>(), F.vdual.as<double>(), b, st));
+        CK(fem_pcg(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st));
+        CK(hipEventRecord(ev[6], st));
+        CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F.F2.as<double>(), st));
+        // dual update, FEM branch: udual = F2 - F1, vdual = U2 - U1 (main_ALDIC.m:340,440)
+        CK(launch_sub(F.F2.as<double>(), F.F1.as<double>(), F.udual.as<double>(), 4L * S, st));
+        CK(launch_sub(F.U2.as<double>(), F.U1.as<double>(), F.vdual.as<double>(), 2L * S, st));
+        CK(hipEventRecord(ev[7], st));
+        CK(hipEventSynchronize(ev[7]));
+        float t1 = 0, t2 = 0;
+        CK(hipEventElapsedTime(&t1, ev[5], ev[6]));
+        CK(hipEventElapsedTime(&t2, ev[6], ev[7]));
+        t_sub2 += t1;
+        t_str += t2;
+        return ALDIC_OK;
+    };
+    int r = global_step(acc[4], acc[6]);
+    if (r) return r;
+
+    for (int step = 0; step < o.n_admm; ++step) {
+        // Subpb1: 2-DOF IC-GN from (U2, F2)
+        CK(hipEventRecord(ev[8], st));
+        IcgnArgs a2 = a;
+        a2.U0 = nullptr;
+        a2.Hin = F.H21.as<double>();
+        a2.vdual = F.vdual.as<double>();
+        a2.Uold = F.U2.as<double>();
+        a2.Fold = F.F2.as<double>();
+        a2.mu = o.mu;
+        a2.U = F.U1.as<double>();
+        a2.iters = F.it2.as<int>();
+        a2.F = nullptr;
+        a2.H21 = nullptr;
+        a2.zncc = nullptr;
+        CK(launch_icgn2(a2, ppl, st));
+        CK(launch_badpoint(F.it2.as<int>(), S, 0, F.U1.as<double>(), nullptr, F.bad.as<int>(),
+                           nullptr, st));
+        CK(launch_fill(F.Nx, F.Ny, F.bad.as<int>(), F.U1.as<double>(), nullptr, 12, st));
+        // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
+        CK(hipMemcpyAsync(F.F1.p, F.F2.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
+        CK(hipEventRecord(ev[9], st));
+        CK(hipEventSynchronize(ev[9]));
+        float t = 0;
+        CK(hipEventElapsedTime(&t, ev[8], ev[9]));
+        acc[5] += t;
+        r = global_step(acc[4], acc[6]);
+        if (r) return r;
+    }
+    CK(hipEventRecord(ev[9], st));
+    CK(hipEventSynchronize(ev[9]));
+    float t;
+    CK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] = t;
+    CK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] = t;
+    CK(hipEventElapsedTime(&t, ev[2], ev[3])); acc[2] = t;
+    CK(hipEventElapsedTime(&t, ev[3], ev[4])); acc[3] = t;
+    CK(hipEventElapsedTime(&t, ev[0], ev[9])); acc[7] = t;
+    for (int k = 0; k < 8; ++k) F.ms[k] = acc[k];
+    return ALDIC_OK;
+}
+
+int aldic_frame_get(aldic_ctx *ctx, double *U, double *Fo, double *U6, double *F6, int *iters6,
+                    int *iters2, double *u0)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    FrameState &F = ctx->fr;
+    if (!F.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_get: no frame");
+    CK(hipSetDevice(ctx->device));
+    const size_t S = F.S;
+    CK(down(U, F.U2.p, 2 * S, ctx->stream));
+    CK(down(Fo, F.F2.p, 4 * S, ctx->stream));
+    CK(down(U6, F.U6.p, 2 * S, ctx->stream));
+    CK(down(F6, F.F6.p, 4 * S, ctx->stream));
+    CK(down(iters6, F.it6.p, S, ctx->stream));
+    CK(down(iters2, F.it2.p, S, ctx->stream));
+    CK(down(u0, F.U0.p, 2 * S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_frame_times(aldic_ctx *ctx, double *ms, int n)
+{
+    if (!ctx || !ms || n < 0 || n > 8) return ALDIC_ERR_ARG;
+    for (int k = 0; k < n; ++k) ms[k] = ctx->fr.ms[k];
+    return ALDIC_OK;
+}
+
+}  // extern "C"

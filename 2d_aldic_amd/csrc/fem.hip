@@ -1,0 +1,504 @@
+// Q4 finite-element global step and nodal strain on the device.
+//
+//   Subpb2.m:76-195                  element matrices with 3x3 Gauss points, solve
+//   funGlobal_NodalStrainAvg.m:32-144 2x2 Gauss-point gradients, extrapolation,
+//                                     element averaging
+//
+// The global matrix of Subpb2 couples u with u and v with v only (DN'DN and N'N are
+// block-diagonal in the displacement component), with the same scalar node matrix
+// A = beta K + mu M + alpha mu K in both blocks.  It is assembled once per
+// (mesh, beta, mu, alpha) into a scalar CSR over nodes: one thread per node gathers
+// the elements around it (node -> element list) in a fixed order, so the assembly is
+// atomic-free and deterministic.  The right-hand side is
+//   b_u(i) = beta (G_i . (F11 - W11, F12 - W12)_i) + mu (M (U - V)_u)_i
+//   b_v(i) = beta (G_i . (F21 - W21, F22 - W22)_i) + mu (M (U - V)_v)_i
+// with G_i = sum_e int_e grad N_i dA (the reference's diag(DN' (F-W)') uses the
+// node's own F, Subpb2.m:88-94,149).  The solve is Jacobi-preconditioned CG in FP64
+// on interleaved (u, v) vectors, two kernels per iteration; scalars stay on the
+// device and the dot products finish in the last block (fixed order).
+#include "aldic_internal.h"
+#include "common.h"
+
+namespace aldic {
+
+__device__ __forceinline__ void q4_dN(double ksi, double eta, double dNk[4], double dNe[4])
+{
+    dNk[0] = -(1 - eta) * 0.25; dNe[0] = -(1 - ksi) * 0.25;
+    dNk[1] = (1 - eta) * 0.25;  dNe[1] = -(1 + ksi) * 0.25;
+    dNk[2] = (1 + eta) * 0.25;  dNe[2] = (1 + ksi) * 0.25;
+    dNk[3] = -(1 + eta) * 0.25; dNe[3] = (1 - ksi) * 0.25;
+}
+
+// x-derivative and y-derivative of the 4 shape functions at (ksi, eta); returns det J.
+__device__ __forceinline__ double q4_grad(const double px[4], const double py[4], double ksi,
+                                          double eta, double ddx[4], double ddy[4])
+{
+    double dNk[4], dNe[4];
+    q4_dN(ksi, eta, dNk, dNe);
+    double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        J11 += dNk[b] * px[b]; J12 += dNk[b] * py[b];
+        J21 += dNe[b] * px[b]; J22 += dNe[b] * py[b];
+    }
+    const double det = J11 * J22 - J12 * J21;
+    const double i11 = J22 / det, i12 = -J12 / det, i21 = -J21 / det, i22 = J11 / det;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        ddx[b] = i11 * dNk[b] + i12 * dNe[b];
+        ddy[b] = i21 * dNk[b] + i22 * dNe[b];
+    }
+    return det;
+}
+
+__global__ __launch_bounds__(256) void k_assemble(int nn, const double2 *coords, const int4 *elems,
+                                                  const int *n2e_ptr, const N2E *n2e,
+                                                  double beta, double mu, double alpha,
+                                                  const int *rowptr, const int *col, double *aval,
+                                                  double *mval, double *diag, double2 *gvec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) { aval[k] = 0.0; mval[k] = 0.0; }
+    const double gp[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
+    const double gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    double gx = 0.0, gy = 0.0;
+    for (int q = n2e_ptr[i]; q < n2e_ptr[i + 1]; ++q) {
+        const N2E r = n2e[q];
+        const int4 el = elems[r.e];
+        const int nd[4] = {el.x, el.y, el.z, el.w};
+        double px[4], py[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) { px[b] = coords[nd[b]].x; py[b] = coords[nd[b]].y; }
+        double Kr[4] = {0, 0, 0, 0}, Mr[4] = {0, 0, 0, 0};
+        for (int ti = 0; ti < 3; ++ti)
+            for (int tj = 0; tj < 3; ++tj) {
+                const double ksi = gp[ti], eta = gp[tj];
+                double ddx[4], ddy[4];
+                const double det = q4_grad(px, py, ksi, eta, ddx, ddy);
+                const double wJ = det * gw[ti] * gw[tj];
+                const double N[4] = {(1 - ksi) * (1 - eta) * 0.25, (1 + ksi) * (1 - eta) * 0.25,
+                                     (1 + ksi) * (1 + eta) * 0.25, (1 - ksi) * (1 + eta) * 0.25};
+                const int a = r.a;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    Kr[b] += wJ * (ddx[a] * ddx[b] + ddy[a] * ddy[b]);
+                    Mr[b] += wJ * (N[a] * N[b]);
+                }
+                gx += wJ * ddx[a];
+                gy += wJ * ddy[a];
+            }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            aval[r.slot[b]] += beta * Kr[b] + mu * Mr[b] + alpha * mu * Kr[b];
+            mval[r.slot[b]] += Mr[b];
+        }
+    }
+    double d = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+        if (col[k] == i) d = aval[k];
+    diag[i] = d;
+    gvec[i] = make_double2(gx, gy);
+}
+
+__global__ __launch_bounds__(256) void k_rhs(int nn, const int *rowptr, const int *col,
+                                             const double *mval, const double2 *gvec, double beta,
+                                             double mu, const double2 *U, const double4 *F,
+                                             const double4 *udual, const double2 *vdual,
+                                             double2 *b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double su = 0.0, sv = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const int c = col[k];
+        const double m = mval[k];
+        const double2 u = U[c], v = vdual[c];
+        su += m * (u.x - v.x);
+        sv += m * (u.y - v.y);
+    }
+    const double4 f = F[i], w = udual[i];  // [F11 F21 F12 F22]
+    const double2 g = gvec[i];
+    b[i] = make_double2(beta * (g.x * (f.x - w.x) + g.y * (f.z - w.z)) + mu * su,
+                        beta * (g.x * (f.y - w.y) + g.y * (f.w - w.w)) + mu * sv);
+}
+
+// ---- Jacobi-PCG ---------------------------------------------------------------------
+// Device scalar block (doubles): 0 rz, 1 alpha, 2 beta, 3 |b|, 4 rr, 5 done, 6 iters,
+// 7 spare; the counter for last-block reductions is a separate uint.
+struct PcgScal {
+    double rz, alpha, beta, bn, rr, done, iters, rtol;
+};
+
+__device__ __forceinline__ void block_reduce2(double &a, double &b, double *red)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[2 * wid] = a; red[2 * wid + 1] = b; }
+    __syncthreads();
+    a = red[0] + red[2] + red[4] + red[6];
+    b = red[1] + red[3] + red[5] + red[7];
+}
+
+// Sums the per-block partials in block order (deterministic) in the last block.
+__device__ __forceinline__ bool last_block(unsigned *counter)
+{
+    __shared__ bool amlast;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) amlast = atomicInc(counter, 0xffffffffu) == gridDim.x - 1;
+    __syncthreads();
+    if (amlast) __threadfence();  // acquire: see the other blocks' partials (per-XCD L2s)
+    return amlast;
+}
+
+__global__ __launch_bounds__(256) void k_pcg_init(int nn, const double2 *b, const double *diag,
+                                                  double2 *x, double2 *r, double2 *z,
+                                                  double2 *p, double2 *part, unsigned *counter,
+                                                  PcgScal *s)
+{
+    __shared__ double red[8];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, bb = 0.0;
+    if (i < nn) {
+        const double2 bi = b[i];
+        const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
+        const double2 zi = make_double2(bi.x * dinv, bi.y * dinv);
+        x[i] = make_double2(0.0, 0.0);
+        r[i] = bi;
+        z[i] = zi;
+        p[i] = zi;
+        rz = bi.x * zi.x + bi.y * zi.y;
+        bb = bi.x * bi.x + bi.y * bi.y;
+    }
+    block_reduce2(rz, bb, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rz, bb);
+    if (!last_block(counter)) return;
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (unsigned k = 0; k < gridDim.x; ++k) { a += part[k].x; c += part[k].y; }
+        s->rz = a;
+        s->bn = sqrt(c);
+        s->rr = c;
+        s->beta = 0.0;
+        s->iters = 0.0;
+        s->done = (c == 0.0) ? 1.0 : 0.0;
+        *counter = 0;
+    }
+}
+
+// p_new = z + beta p_old (computed on the fly for the neighbours), Ap, p.Ap
+__global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, const int *col,
+                                                  const double *aval, const double2 *z,
+                                                  const double2 *pold, double2 *pnew,
+                                                  double2 *Ap, double2 *part, unsigned *counter,
+                                                  PcgScal *s, int first)
+{
+    __shared__ double red[8];
+    if (s->done != 0.0) return;
+    const double beta = first ? 0.0 : s->beta;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double pap = 0.0, dummy = 0.0;
+    if (i < nn) {
+        double au = 0.0, av = 0.0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const int c = col[k];
+            const double2 zc = z[c], pc = pold[c];
+            const double a = aval[k];
+            au += a * (first ? pc.x : zc.x + beta * pc.x);
+            av += a * (first ? pc.y : zc.y + beta * pc.y);
+        }
+        const double2 zi = z[i], pi = pold[i];
+        const double2 pn = first ? pi : make_double2(zi.x + beta * pi.x, zi.y + beta * pi.y);
+        pnew[i] = pn;
+        Ap[i] = make_double2(au, av);
+        pap = pn.x * au + pn.y * av;
+    }
+    block_reduce2(pap, dummy, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(pap, 0.0);
+    if (!last_block(counter)) return;
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (unsigned k = 0; k < gridDim.x; ++k) a += part[k].x;
+        s->alpha = s->rz / a;
+        *counter = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, const double2 *p,
+                                                    const double2 *Ap, double2 *x, double2 *r,
+                                                    double2 *z, double2 *part,
+                                                    unsigned *counter, PcgScal *s)
+{
+    __shared__ double red[8];
+    if (s->done != 0.0) return;
+    const double alpha = s->alpha;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < nn) {
+        const double2 pi = p[i], api = Ap[i];
+        double2 xi = x[i], ri = r[i];
+        xi.x += alpha * pi.x; xi.y += alpha * pi.y;
+        ri.x -= alpha * api.x; ri.y -= alpha * api.y;
+        x[i] = xi;
+        r[i] = ri;
+        const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
+        const double2 zi = make_double2(ri.x * dinv, ri.y * dinv);
+        z[i] = zi;
+        rz = ri.x * zi.x + ri.y * zi.y;
+        rr = ri.x * ri.x + ri.y * ri.y;
+    }
+    block_reduce2(rz, rr, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rz, rr);
+    if (!last_block(counter)) return;
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (unsigned k = 0; k < gridDim.x; ++k) { a += part[k].x; c += part[k].y; }
+        s->beta = a / s->rz;
+        s->rz = a;
+        s->rr = c;
+        s->iters += 1.0;
+        if (sqrt(c) <= s->rtol * s->bn) s->done = 1.0;
+        *counter = 0;
+    }
+}
+
+hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st)
+{
+    const int nb = (m.nn + 255) / 256;
+    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(256), 0, st, m.nn, m.coords.as<double2>(),
+                       m.elems.as<int4>(), m.n2e_ptr.as<int>(), m.n2e.as<N2E>(), beta, mu, alpha,
+                       m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(),
+                       m.mval.as<double>(), m.diag.as<double>(), m.gvec.as<double2>());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) { m.beta = beta; m.mu = mu; m.alpha = alpha; }
+    return e;
+}
+
+hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, const double *F,
+                   const double *udual, const double *vdual, double *b, hipStream_t st)
+{
+    const int nb = (m.nn + 255) / 256;
+    hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(256), 0, st, m.nn, m.rowptr.as<int>(),
+                       m.col.as<int>(), m.mval.as<double>(), m.gvec.as<double2>(), beta, mu,
+                       (const double2 *)U, (const double4 *)F, (const double4 *)udual,
+                       (const double2 *)vdual, (double2 *)b);
+    return hipGetLastError();
+}
+
+size_t fem_n2e_bytes(int entries) { return sizeof(N2E) * (size_t)entries; }
+
+// work[0..5]: r, z, pA, pB, Ap, part (double2 each) ; work[6]: scalars + counter
+hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
+                   DBuf *work, double *stats_dev, hipStream_t st)
+{
+    const int nn = m.nn;
+    const int nb = (nn + 255) / 256;
+    const size_t vb = sizeof(double2) * (size_t)nn;
+    hipError_t e;
+    for (int k = 0; k < 5; ++k)
+        if ((e = work[k].ensure(vb)) != hipSuccess) return e;
+    if ((e = work[5].ensure(sizeof(double2) * (size_t)nb)) != hipSuccess) return e;
+    if ((e = work[6].ensure(sizeof(PcgScal) + 64)) != hipSuccess) return e;
+    PcgScal *s = work[6].as<PcgScal>();
+    unsigned *counter = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(s) + sizeof(PcgScal));
+    double2 *r = work[0].as<double2>(), *z = work[1].as<double2>(), *pA = work[2].as<double2>(),
+            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>(),
+            *part = work[5].as<double2>();
+    if ((e = hipMemsetAsync(counter, 0, sizeof(unsigned), st)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&s->rtol, &rtol, sizeof(double), hipMemcpyHostToDevice, st)) !=
+        hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, st, nn, (const double2 *)b,
+                       m.diag.as<double>(), (double2 *)x, r, z, pA, part, counter, s);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int chunk = 8;
+    for (int it = 0; it < maxit; it += chunk) {
+        for (int q = 0; q < chunk && it + q < maxit; ++q) {
+            const int k = it + q;
+            double2 *pold = (k & 1) ? pB : pA;
+            double2 *pnew = (k & 1) ? pA : pB;
+            hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(256), 0, st, nn, m.rowptr.as<int>(),
+                               m.col.as<int>(), m.aval.as<double>(), z, pold, pnew, Ap, part,
+                               counter, s, k == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, st, nn, m.diag.as<double>(),
+                               pnew, Ap, (double2 *)x, r, z, part, counter, s);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        double done = 0.0;
+        if ((e = hipMemcpyAsync(&done, &s->done, sizeof(double), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        if (done != 0.0) break;
+    }
+    if (stats_dev) {
+        // {iterations, relative residual}
+        PcgScal hs;
+        if ((e = hipMemcpyAsync(&hs, s, sizeof(PcgScal), hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        stats_dev[0] = hs.iters;
+        stats_dev[1] = hs.bn > 0 ? sqrt(hs.rr) / hs.bn : 0.0;
+    }
+    return hipSuccess;
+}
+
+// ---- nodal strain (funGlobal_NodalStrainAvg.m) ------------------------------------
+__global__ __launch_bounds__(256) void k_strain(int nn, const double2 *coords, const int4 *elems,
+                                                const int *n2e_ptr, const N2E *n2e,
+                                                const double2 *U, int swap_fix, double4 *F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const double g = 0.57735026918962573;  // 1/sqrt(3)
+    const double gk[4] = {-g, g, g, -g}, ge[4] = {-g, -g, g, g};
+    const double s3h = 0.86602540378443865;  // sqrt(3)/2
+    // extrapolation matrix of :105-108 (row a = node, column p = Gauss point)
+    const double Mx[4][4] = {{1 + s3h, -0.5, 1 - s3h, -0.5},
+                             {-0.5, 1 + s3h, -0.5, 1 - s3h},
+                             {1 - s3h, -0.5, 1 + s3h, -0.5},
+                             {-0.5, 1 - s3h, -0.5, 1 + s3h}};
+    double acc[4] = {0, 0, 0, 0};
+    int cnt = 0;
+    for (int q = n2e_ptr[i]; q < n2e_ptr[i + 1]; ++q) {
+        const N2E r = n2e[q];
+        const int4 el = elems[r.e];
+        const int nd[4] = {el.x, el.y, el.z, el.w};
+        double px[4], py[4], ue[4], ve[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            px[b] = coords[nd[b]].x; py[b] = coords[nd[b]].y;
+            ue[b] = U[nd[b]].x; ve[b] = U[nd[b]].y;
+        }
+        double nv[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            double ddx[4], ddy[4];
+            q4_grad(px, py, gk[p], ge[p], ddx, ddy);
+            double ux = 0, uy = 0, vx = 0, vy = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                ux += ddx[b] * ue[b]; uy += ddy[b] * ue[b];
+                vx += ddx[b] * ve[b]; vy += ddy[b] * ve[b];
+            }
+            const double m = Mx[r.a][p];
+            nv[0] += m * ux; nv[1] += m * uy; nv[2] += m * vx; nv[3] += m * vy;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] += nv[c];
+        ++cnt;
+    }
+    double4 o;
+    const double d = (double)cnt;  // 0 -> NaN like the reference's 0/0
+    o.x = acc[0] / d;
+    o.y = (swap_fix ? acc[2] : acc[1]) / d;
+    o.z = (swap_fix ? acc[1] : acc[2]) / d;
+    o.w = acc[3] / d;
+    F[i] = o;
+}
+
+hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F, hipStream_t st)
+{
+    const int nb = (m.nn + 255) / 256;
+    hipLaunchKernelGGL(k_strain, dim3(nb), dim3(256), 0, st, m.nn, m.coords.as<double2>(),
+                       m.elems.as<int4>(), m.n2e_ptr.as<int>(), m.n2e.as<N2E>(),
+                       (const double2 *)U, swap_fix, (double4 *)F);
+    return hipGetLastError();
+}
+
+// ---- ADMM helpers --------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sub(const double *a, const double *b, double *o, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) o[i] = a[i] - b[i];
+}
+
+hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out, n);
+    return hipGetLastError();
+}
+
+// LocalICGN.m:119-139 (use_mean_rule = 1) or Subpb1.m:167-182 (0): mark bad subsets
+// (status < 0, > 99, and with the mean rule status > mean + 0.15 std of the good
+// ones, std with N-1) and set their U (and F) to NaN.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_badpoint(const int *it, int S, int mean_rule,
+                                                   double *U, double *F, int *bad)
+{
+    __shared__ double red[3][16];
+    double c = 0, s = 0, s2 = 0;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        const int v = it[i];
+        if (v >= 0 && v <= 99) { c += 1; s += v; s2 += (double)v * v; }
+    }
+    c = wave_sum(c); s = wave_sum(s); s2 = wave_sum(s2);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wid] = c; red[1][wid] = s; red[2][wid] = s2; }
+    __syncthreads();
+    double C = 0, Sm = 0, S2 = 0;
+    for (int k = 0; k < 16; ++k) { C += red[0][k]; Sm += red[1][k]; S2 += red[2][k]; }
+    const double mean = C > 0 ? Sm / C : 0.0;
+    const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
+    const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        const int v = it[i];
+        const bool b = v < 0 || v > 99 || (mean_rule && C > 0 && (double)v > thr);
+        bad[i] = b ? 1 : 0;
+        if (b) {
+            U[2 * i] = NAN; U[2 * i + 1] = NAN;
+            if (F) { F[4 * i] = NAN; F[4 * i + 1] = NAN; F[4 * i + 2] = NAN; F[4 * i + 3] = NAN; }
+        }
+    }
+}
+
+hipError_t launch_badpoint(const int *iters, int S, int use_mean_rule, double *U, double *F,
+                           int *bad, double *tmp, hipStream_t st)
+{
+    (void)tmp;
+    hipLaunchKernelGGL(k_badpoint, dim3(1), dim3(1024), 0, st, iters, S, use_mean_rule, U, F, bad);
+    return hipGetLastError();
+}
+
+// Spring fill of bad nodes on the Nx x Ny node grid: each bad node takes the mean of
+// its non-NaN 4-neighbours (the inpaint_nans method-4 spring stencil).  Red-black
+// sweeps, so every update reads only the other colour: deterministic, race-free.
+template <int NC>
+__global__ __launch_bounds__(256) void k_fill(int Nx, int Ny, const int *bad, double *V, int color)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nx * Ny || !bad[i]) return;
+    const int x = i % Nx, y = i / Nx;
+    if (((x + y) & 1) != color) return;
+    const int nb[4] = {x > 0 ? i - 1 : -1, x < Nx - 1 ? i + 1 : -1, y > 0 ? i - Nx : -1,
+                       y < Ny - 1 ? i + Nx : -1};
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        double s = 0.0;
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (nb[k] < 0) continue;
+            const double v = V[(size_t)NC * nb[k] + c];
+            if (isnan(v)) continue;
+            s += v;
+            ++n;
+        }
+        if (n > 0) V[(size_t)NC * i + c] = s / n;
+    }
+}
+
+hipError_t launch_fill(int Nx, int Ny, const int *bad, double *U, double *F, int sweeps,
+                       hipStream_t st)
+{
+    const int n = Nx * Ny;
+    const int nb = (n + 255) / 256;
+    for (int k = 0; k < 2 * sweeps; ++k) {
+        hipLaunchKernelGGL(k_fill<2>, dim3(nb), dim3(256), 0, st, Nx, Ny, bad, U, k & 1);
+        if (F) hipLaunchKernelGGL(k_fill<4>, dim3(nb), dim3(256), 0, st, Nx, Ny, bad, F, k & 1);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace aldic

@@ -1,0 +1,548 @@
+// Batched inverse-compositional Gauss-Newton (IC-GN) for gfx950.
+//
+//   6-DOF: funICGN.m:1-322 (per subset of LocalICGN.m:91-107)
+//   2-DOF: funICGN_Subpb1.m:1-362 (per subset of Subpb1.m:119-152)
+//
+// Layout: WPS wavefronts work on one subset (WPS = 1 for windows up to 33 x 33,
+// 4 for larger ones); each lane keeps PPL pixels of the (w+1)^2 window in registers:
+// the normalised reference value f_n, the gradients fx, fy and the offsets dx, dy.
+// Pixel p of the window (x fastest) sits in slot k = p / (64 * WPS) of thread
+// p % (64 * WPS).  The Hessian, its inverse (FP64), mean/norm of f and the constant
+// sums sum J_k and sum J_k f_n are formed once per subset.  Each iteration then makes
+// ONE pass over the pixels: warp, Keys-cubic gather of g (16 loads through L1/L2),
+// and the nine sums  S_g, S_gg, S_{g J_k} (k = 1..6), S_{g f_n}, from which
+//   b_k  = bottomf * (S_{J_k f_n} - (S_{g J_k} - gbar S_{J_k}) / bottomg)
+//   zncc = S_{g f_n} / bottomg
+// follow exactly (the identity sum J_k r with r = f_n - (g - gbar)/bottomg).  The
+// convergence test, the 6x6 solve and the inverse-compositional update run in FP64
+// on every lane with bitwise-identical inputs (butterfly all-reduce), so the loop
+// is uniform across the wave.
+#include "aldic_internal.h"
+#include "common.h"
+
+namespace aldic {
+
+constexpr int kMaxIter = 100;
+
+// funICGN.m:34-65 window set-up with the reference's error cases and mask rule.
+__device__ __forceinline__ int win_corner(double x0, double y0, int w, int nx, int ny, int &xa,
+                                          int &ya)
+{
+    const double xs = x0 - 0.5 * w, ys = y0 - 0.5 * w;
+    if (xs != floor(xs) || ys != floor(ys)) return -1;
+    xa = (int)xs;
+    ya = (int)ys;
+    if (xa < 4 || ya < 4 || xa + w > nx - 3 || ya + w > ny - 3) return -1;
+    return 0;
+}
+
+template <int WPS>
+__device__ __forceinline__ float block_sum1(float v, float *red, int wid)
+{
+    v = wave_sum(v);
+    if constexpr (WPS > 1) {
+        if ((threadIdx.x & 63) == 0) red[wid] = v;
+        __syncthreads();
+        v = 0.f;
+#pragma unroll
+        for (int q = 0; q < WPS; ++q) v += red[q];
+        __syncthreads();
+    }
+    return v;
+}
+
+template <int WPS>
+__device__ __forceinline__ double block_sumd(double v, double *red, int wid)
+{
+    v = wave_sum(v);
+    if constexpr (WPS > 1) {
+        if ((threadIdx.x & 63) == 0) red[wid] = v;
+        __syncthreads();
+        v = 0.0;
+#pragma unroll
+        for (int q = 0; q < WPS; ++q) v += red[q];
+        __syncthreads();
+    }
+    return v;
+}
+
+template <int WPS, int N>
+__device__ __forceinline__ void block_sumN(float (&v)[N], float *red, int wid)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+    if constexpr (WPS > 1) {
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int i = 0; i < N; ++i) red[i * WPS + wid] = v[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            float s = 0.f;
+#pragma unroll
+            for (int q = 0; q < WPS; ++q) s += red[i * WPS + q];
+            v[i] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse of a symmetric positive definite 6x6 (row-major, FP64) by Cholesky.
+// Returns false when a pivot is not positive.
+__device__ __forceinline__ bool spd_inverse6(const double *H, double *Hi)
+{
+    double L[21];  // packed lower, row-major
+#define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
+    for (int r = 0; r < 6; ++r) {
+        for (int c = 0; c <= r; ++c) {
+            double s = H[r * 6 + c];
+            for (int k = 0; k < c; ++k) s -= LI(r, k) * LI(c, k);
+            if (r == c) {
+                if (!(s > 0.0)) return false;
+                LI(r, r) = sqrt(s);
+            } else {
+                LI(r, c) = s / LI(c, c);
+            }
+        }
+    }
+    // columns of H^-1: solve L L^T x = e_j
+    for (int j = 0; j < 6; ++j) {
+        double y[6];
+        for (int r = 0; r < 6; ++r) {
+            double s = (r == j) ? 1.0 : 0.0;
+            for (int k = 0; k < r; ++k) s -= LI(r, k) * y[k];
+            y[r] = s / LI(r, r);
+        }
+        for (int r = 5; r >= 0; --r) {
+            double s = y[r];
+            for (int k = r + 1; k < 6; ++k) s -= LI(k, r) * Hi[k * 6 + j];
+            Hi[r * 6 + j] = s / LI(r, r);
+        }
+    }
+#undef LI
+    return true;
+}
+
+// funICGN.m:268-286: P <- P o dP^-1.  Returns false when det(dP) == 0.
+__device__ __forceinline__ bool ic_compose(double P[6], const double D[6])
+{
+    const double det = (1.0 + D[0]) * (1.0 + D[3]) - D[1] * D[2];
+    if (det == 0.0) return false;
+    const double t1 = (-D[0] - D[0] * D[3] + D[1] * D[2]) / det;
+    const double t2 = -D[1] / det;
+    const double t3 = -D[2] / det;
+    const double t4 = (-D[3] - D[0] * D[3] + D[1] * D[2]) / det;
+    const double t5 = (-D[4] - D[3] * D[4] + D[2] * D[5]) / det;
+    const double t6 = (-D[5] - D[0] * D[5] + D[1] * D[4]) / det;
+    const double a11 = 1.0 + P[0], a12 = P[2], a13 = P[4];
+    const double a21 = P[1], a22 = 1.0 + P[3], a23 = P[5];
+    P[0] = a11 * (1.0 + t1) + a12 * t2 - 1.0;
+    P[2] = a11 * t3 + a12 * (1.0 + t4);
+    P[4] = a11 * t5 + a12 * t6 + a13;
+    P[1] = a21 * (1.0 + t1) + a22 * t2;
+    P[3] = a21 * t3 + a22 * (1.0 + t4) - 1.0;
+    P[5] = a21 * t5 + a22 * t6 + a23;
+    return true;
+}
+
+// Keys-cubic sample of g at (X + ax, Y + ay), X, Y 1-based integers, the sample known
+// to lie in [3, nx-2] x [3, ny-2] so every tap is inside the image.
+__device__ __forceinline__ float sample_g(const float *__restrict__ g, int nx, int X, int Y,
+                                          float ax, float ay)
+{
+    const float fx = floorf(ax), fy = floorf(ay);
+    float wx0, wx1, wx2, wx3, wy0, wy1, wy2, wy3;
+    keys_weights(ax - fx, wx0, wx1, wx2, wx3);
+    keys_weights(ay - fy, wy0, wy1, wy2, wy3);
+    const int c0 = X + (int)fx - 2, r0 = Y + (int)fy - 2;  // 0-based first tap
+    const float *q = g + (size_t)r0 * nx + c0;
+    float r[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const float *row = q + (size_t)b * nx;
+        r[b] = wx0 * row[0] + wx1 * row[1] + wx2 * row[2] + wx3 * row[3];
+    }
+    return wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3];
+}
+
+// Shared body for both modes.  SIX = true: 6-DOF funICGN; false: 2-DOF funICGN_Subpb1.
+template <int PPL, int WPS, bool SIX>
+__global__ __launch_bounds__(256, 2) void k_icgn(IcgnArgs a)
+{
+    constexpr int SPB = 4 / WPS;          // subsets per block
+    constexpr int T = 64 * WPS;           // threads per subset
+    __shared__ double s_hinv[SPB][36];
+    __shared__ double s_cst[SPB][12];
+    __shared__ float s_red[16 * WPS];
+    __shared__ double s_redd[WPS];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int sub_in_block = wave / WPS;
+    const int wid = wave % WPS;            // wave index inside the subset
+    const int tid = wid * 64 + lane;       // thread index inside the subset
+    const int slot = blockIdx.x * SPB + sub_in_block;
+    const int nlist = a.nlist_dev ? *a.nlist_dev : a.nlist;
+    if (slot >= nlist) return;             // uniform per subset
+    const int j = a.list ? a.list[slot] : slot;
+    const int nx = a.nx, ny = a.ny;
+    const double x0 = a.coords[2 * j], y0 = a.coords[2 * j + 1];
+    int w = a.ws ? a.ws[j] : a.ws_all;
+    const double tol = a.tol;
+
+    double P[6];
+    if (SIX) {
+        P[0] = P[1] = P[2] = P[3] = 0.0;
+        P[4] = a.U0[2 * j];
+        P[5] = a.U0[2 * j + 1];
+    } else {
+        P[0] = a.Fold[4 * j]; P[1] = a.Fold[4 * j + 1];
+        P[2] = a.Fold[4 * j + 2]; P[3] = a.Fold[4 * j + 3];
+        P[4] = a.Uold[2 * j]; P[5] = a.Uold[2 * j + 1];
+    }
+
+    // H21 is written once the Hessian exists; the early exits (-1, 102) leave zeros.
+    auto finish = [&](int status, bool zero_h, float zc) {
+        if (tid != 0) return;
+        const bool failed = status == -1;
+        if (SIX) {
+            a.U[2 * j] = failed ? NAN : P[4];
+            a.U[2 * j + 1] = failed ? NAN : P[5];
+            for (int q = 0; q < 4; ++q) a.F[4 * j + q] = failed ? NAN : P[q];
+            if (zero_h)
+                for (int q = 0; q < 21; ++q) a.H21[21 * (size_t)j + q] = 0.0;
+            if (a.zncc) a.zncc[j] = (double)zc;
+        } else {
+            a.U[2 * j] = failed ? NAN : P[4];
+            a.U[2 * j + 1] = failed ? NAN : P[5];
+        }
+        a.iters[j] = status;
+    };
+
+    // ---- window and mask rule (funICGN.m:53-65) -----------------------------------
+    int xa, ya;
+    if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
+    {
+        const int n = (w + 1) * (w + 1);
+        float cnt = 0.f;
+        for (int p = tid; p < n; p += T) {
+            const int i = p % (w + 1), jj = p / (w + 1);
+            cnt += a.gx[(size_t)(ya - 1 + jj) * nx + (xa - 1 + i)] == 0.f ? 1.f : 0.f;
+        }
+        const float n0 = block_sum1<WPS>(cnt, s_red, wid);
+        if (!((double)n0 < 0.5 * n)) { finish(102, true, NAN); return; }
+        if ((double)n0 > 0.1 * n) {
+            w = (int)floor(sqrt(2.0) * w + 0.5);  // MATLAB round (funICGN.m:58)
+            if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
+        }
+    }
+    const int w1 = w + 1, n = w1 * w1;
+    if (n > PPL * T) { if (tid == 0) a.iters[j] = -2; return; }
+
+    // ---- reference window ------------------------------------------------------
+    // Set-up sums read the window straight from memory (L2-resident) in a rolled
+    // loop, so only the per-pixel arrays used by the iterations occupy registers.
+    auto px_at = [&](int p, float &fval, float &gxv, float &gyv, double &dx, double &dy) {
+        const int i = p % w1, r = p / w1;
+        const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
+        fval = a.f[o];
+        gxv = a.gx[o];
+        gyv = a.gy[o];
+        dx = (double)i - 0.5 * w;
+        dy = (double)r - 0.5 * w;
+    };
+    // mean and norm of f (funICGN.m:93-94), FP64
+    double sf = 0.0;
+#pragma unroll 1
+    for (int p = tid; p < n; p += T) sf += (double)a.f[(size_t)(ya - 1 + p / w1) * nx + (xa - 1 + p % w1)];
+    sf = block_sumd<WPS>(sf, s_redd, wid);
+    const double meanf = sf / n;
+    double ssf = 0.0;
+#pragma unroll 1
+    for (int p = tid; p < n; p += T) {
+        const double d = (double)a.f[(size_t)(ya - 1 + p / w1) * nx + (xa - 1 + p % w1)] - meanf;
+        ssf += d * d;
+    }
+    ssf = block_sumd<WPS>(ssf, s_redd, wid);
+    const double bottomf = sqrt(ssf);
+
+    // Hessian (funICGN.m:67-79) and the constant sums sum J_k, sum J_k f_n (FP64).
+    double *cst = s_cst[sub_in_block];  // [0..5] sum J_k, [6..11] sum J_k f_n
+    double *hinv = s_hinv[sub_in_block];
+    bool singular = false;
+    {
+        double h[21];
+        double sJ[6], sJf[6];
+#pragma unroll
+        for (int q = 0; q < 21; ++q) h[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sJ[q] = sJf[q] = 0.0;
+#pragma unroll 1
+        for (int p = tid; p < n; p += T) {
+            float fval, gxv, gyv;
+            double dx, dy;
+            px_at(p, fval, gxv, gyv, dx, dy);
+            const double fx = gxv, fy = gyv, fn = ((double)fval - meanf) / bottomf;
+            const double J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
+            if (SIX) {
+                int q = 0;
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+#pragma unroll
+                    for (int rr = c; rr < 6; ++rr) h[q++] += J[rr] * J[c];
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) { sJ[q] += J[q]; sJf[q] += J[q] * fn; }
+        }
+        if (SIX) {
+#pragma unroll
+            for (int q = 0; q < 21; ++q) h[q] = block_sumd<WPS>(h[q], s_redd, wid);
+            if (tid == 0)
+                for (int q = 0; q < 21; ++q) a.H21[21 * (size_t)j + q] = h[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double v1 = block_sumd<WPS>(sJ[q], s_redd, wid);
+            const double v2 = block_sumd<WPS>(sJf[q], s_redd, wid);
+            cst[q] = v1;       // every lane writes the same value and reads back its own
+            cst[6 + q] = v2;
+        }
+        if (SIX) {
+            double H[36], Hi[36];
+            int q = 0;
+            for (int c = 0; c < 6; ++c)
+                for (int r = c; r < 6; ++r) { H[r * 6 + c] = h[q]; H[c * 6 + r] = h[q]; ++q; }
+            singular = !spd_inverse6(H, Hi);
+            if (!singular)
+                for (int e = 0; e < 36; ++e) hinv[e] = Hi[e];  // same value from every lane
+        }
+    }
+
+    // Per-pixel registers for the iterations: pixel p = tid + k*T sits at column i, row
+    // r (p = r*w1 + i); the walk over k advances (i, r) by T without divisions.
+    const int di = T % w1, dr = T / w1;
+    const int i0 = tid % w1, r0 = tid / w1;
+    float fv[PPL], fxv[PPL], fyv[PPL];
+    {
+        int i = i0, r = r0;
+        const float mf = (float)meanf;
+        const double ib = 1.0 / bottomf;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            if (r < w1) {
+                const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
+                fv[k] = (float)(((double)a.f[o] - meanf) * ib);  // f_n
+                fxv[k] = a.gx[o];
+                fyv[k] = a.gy[o];
+            } else {
+                fv[k] = fxv[k] = fyv[k] = 0.f;
+            }
+            i += di; r += dr;
+            if (i >= w1) { i -= w1; ++r; }
+        }
+        (void)mf;
+    }
+
+    // 2-DOF system matrix
+    double t00 = 0, t01 = 0, t11 = 0;
+    const double bf2 = ssf;  // bottomf^2
+    if (!SIX) {
+        // tempH (funICGN_Subpb1.m:281-283)
+        const double *hin = a.Hin + 21 * (size_t)j;
+        const double mu = a.mu;
+        const double h00 = hin[18] * 2.0 / bf2 + mu, h01 = hin[19] * 2.0 / bf2,
+                     h11 = hin[20] * 2.0 / bf2 + mu;
+        const double dmax = h00 > h11 ? h00 : h11;
+        t00 = h00 + 1e-3 * dmax;
+        t01 = h01;
+        t11 = h11 + 1e-3 * dmax;
+    }
+
+    // ---- Gauss-Newton loop (funICGN.m:107-294 / funICGN_Subpb1.m:94-337) ----------
+    const float Xa = (float)xa, Ya = (float)ya;  // window pixel (i, r) is X = Xa + i, Y = Ya + r
+    const float fhalf = 0.5f * (float)w;
+    double nrmNew = 1.0, nrmAbs = 1.0, nrmInit = 0.0;
+    float zc = NAN;
+    int it = 0;
+    while (it <= kMaxIter && nrmNew > tol && nrmAbs > tol) {
+        ++it;
+        bool finite = true;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) finite = finite && isfinite(P[q]);
+        if (!finite) { nrmNew = NAN; break; }
+        const float p0 = (float)P[0], p1 = (float)P[1], p2 = (float)P[2], p3 = (float)P[3],
+                    p4 = (float)P[4], p5 = (float)P[5];
+        // out-of-image test (funICGN.m:118-121).  The warp is affine, so over the
+        // window's pixel box the extremes of u and v sit at its four corners; u = X + ux
+        // with X the exact integer pixel, so u < 3 <=> ux < 3 - X.
+        bool oob = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int ci = (c & 1) ? w : 0, cr = (c & 2) ? w : 0;
+            const float dx = (float)ci - fhalf, dy = (float)cr - fhalf;
+            const float ux = p0 * dx + p2 * dy + p4;
+            const float uy = p1 * dx + p3 * dy + p5;
+            const float X = Xa + (float)ci, Y = Ya + (float)cr;
+            // written as "not inside" so a non-finite corner also counts as outside
+            const bool in = (ux >= 3.f - X) && (ux <= (float)(nx - 2) - X) &&
+                            (uy >= 3.f - Y) && (uy <= (float)(ny - 2) - Y);
+            oob = oob || !in;
+        }
+        if (oob) { nrmNew = 1e6; break; }  // same on every lane
+
+        constexpr int NS = SIX ? 9 : 5;
+        float acc[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) acc[q] = 0.f;
+        {
+            int i = i0, r = r0;
+            // keep the walk inside the loop: hoisting per-pixel offsets out of it would
+            // cost 2*PPL registers
+            asm volatile("" : "+v"(i), "+v"(r));
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                if (r < w1) {
+                    const float dx = (float)i - fhalf, dy = (float)r - fhalf;
+                    const float ux = p0 * dx + p2 * dy + p4;
+                    const float uy = p1 * dx + p3 * dy + p5;
+                    const float gv = sample_g(a.g, nx, xa + i, ya + r, ux, uy);
+                    const float t1 = gv * fxv[k], t2 = gv * fyv[k];
+                    acc[0] += gv;
+                    acc[1] += gv * gv;
+                    if (SIX) {
+                        acc[2] += t1 * dx; acc[3] += t2 * dx;
+                        acc[4] += t1 * dy; acc[5] += t2 * dy;
+                        acc[6] += t1; acc[7] += t2;
+                        acc[8] += gv * fv[k];
+                    } else {
+                        acc[2] += t1; acc[3] += t2;
+                        acc[4] += gv * fv[k];
+                    }
+                }
+                i += di; r += dr;
+                if (i >= w1) { i -= w1; ++r; }
+            }
+        }
+        block_sumN<WPS, NS>(acc, s_red, wid);
+        const double Sg = acc[0], Sgg = acc[1];
+        const double gbar = Sg / n;
+        const double vg = Sgg - Sg * gbar;
+        const double bottomg = sqrt(vg > 0 ? vg : 0.0);
+        double b[6];
+        if (SIX) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                b[q] = bottomf * (cst[6 + q] - ((double)acc[2 + q] - gbar * cst[q]) / bottomg);
+            zc = (float)((double)acc[8] / bottomg);
+            nrmAbs = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2] + b[3] * b[3] + b[4] * b[4] +
+                          b[5] * b[5]);
+        } else {
+            const double b5 = bottomf * (cst[10] - ((double)acc[2] - gbar * cst[4]) / bottomg);
+            const double b6 = bottomf * (cst[11] - ((double)acc[3] - gbar * cst[5]) / bottomg);
+            zc = (float)((double)acc[4] / bottomg);
+            // tempb (funICGN_Subpb1.m:269)
+            b[0] = b5 * 2.0 / bf2 + a.mu * (P[4] - a.Uold[2 * j] - a.vdual[2 * j]);
+            b[1] = b6 * 2.0 / bf2 + a.mu * (P[5] - a.Uold[2 * j + 1] - a.vdual[2 * j + 1]);
+            nrmAbs = sqrt(b[0] * b[0] + b[1] * b[1]);
+        }
+        nrmNew = nrmAbs;
+        if (it == 1) nrmInit = nrmNew;
+        nrmNew = nrmInit > tol ? nrmNew / nrmInit : 0.0;
+        if (nrmNew < tol || nrmAbs < tol) break;
+
+        double D[6];
+        if (SIX) {
+            if (singular) {  // MATLAB's backslash on a singular H gives non-finite steps
+#pragma unroll
+                for (int q = 0; q < 6; ++q) P[q] = NAN;
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) s += hinv[r * 6 + c] * b[c];
+                D[r] = -s;
+            }
+        } else {
+            const double det = t00 * t11 - t01 * t01;
+            D[0] = D[1] = D[2] = D[3] = 0.0;
+            if (det == 0.0) { P[4] = P[5] = NAN; continue; }
+            D[4] = -(t11 * b[0] - t01 * b[1]) / det;
+            D[5] = -(-t01 * b[0] + t00 * b[1]) / det;
+        }
+        if (!ic_compose(P, D)) break;
+    }
+    const bool conv = (nrmNew < tol) || (nrmAbs < tol);
+    if (!conv || isnan(nrmNew)) it = 101;
+    finish(it, false, zc);
+}
+
+int icgn_ppl_for(int ws)
+{
+    const int n = (ws + 1) * (ws + 1);
+    if (n <= 8 * 64) return 8;
+    if (n <= 18 * 64) return 18;
+    if (n <= 18 * 256) return 72;   // 4 waves x 18 pixels
+    return 0;
+}
+
+template <bool SIX>
+static hipError_t launch_any(const IcgnArgs &a, int ppl, hipStream_t st)
+{
+    if (a.nlist <= 0) return hipSuccess;
+    if (ppl == 8) {
+        hipLaunchKernelGGL((k_icgn<8, 1, SIX>), dim3((a.nlist + 3) / 4), dim3(256), 0, st, a);
+    } else if (ppl == 18) {
+        hipLaunchKernelGGL((k_icgn<18, 1, SIX>), dim3((a.nlist + 3) / 4), dim3(256), 0, st, a);
+    } else if (ppl == 72) {
+        hipLaunchKernelGGL((k_icgn<18, 4, SIX>), dim3(a.nlist), dim3(256), 0, st, a);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_icgn6(const IcgnArgs &a, int ppl, hipStream_t st) { return launch_any<true>(a, ppl, st); }
+hipError_t launch_icgn2(const IcgnArgs &a, int ppl, hipStream_t st) { return launch_any<false>(a, ppl, st); }
+
+// ---- ba_interp2 equivalent: Keys cubic with clamp-to-edge (ba_interp2.cpp:60-170) --
+__global__ __launch_bounds__(256) void k_bicubic(const float *g, int nx, int ny, const double *X,
+                                                 const double *Y, long n, double *Z)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = X[i], y = Y[i];
+    if (!(isfinite(x) && isfinite(y))) { Z[i] = NAN; return; }
+    const double xf = floor(x), yf = floor(y);
+    float wx[4], wy[4];
+    keys_weights((float)(x - xf), wx[0], wx[1], wx[2], wx[3]);
+    keys_weights((float)(y - yf), wy[0], wy[1], wy[2], wy[3]);
+    const long c0 = (long)xf - 2, r0 = (long)yf - 2;
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        long r = r0 + b;
+        r = r < 0 ? 0 : (r >= ny ? ny - 1 : r);
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            long cc = c0 + c;
+            cc = cc < 0 ? 0 : (cc >= nx ? nx - 1 : cc);
+            s += wx[c] * g[r * nx + cc];
+        }
+        acc += wy[b] * s;
+    }
+    Z[i] = (double)acc;
+}
+
+hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const double *Y,
+                          long n, double *Z, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bicubic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nx, ny,
+                       X, Y, n, Z);
+    return hipGetLastError();
+}
+
+}  // namespace aldic

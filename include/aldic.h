@@ -1,0 +1,154 @@
+/*
+ * aldic.h -- C ABI of the MI355X (gfx950) AL-DIC correlation library.
+ *
+ * Every entry point replaces one function of the reference MATLAB path
+ * (jyang526843/2D_ALDIC) and is what a MEX gateway for that function binds; the
+ * reference file:line each one stands in for is given beside it.
+ *
+ * Conventions
+ *   - Images are row-major [y][x] (x fastest): the same memory as the reference's
+ *     column-major image after its load-time transpose (dim 1 = x, ReadImage.m:79),
+ *     so a MATLAB array passes through unchanged.  Coordinates are 1-based.
+ *   - Nodal vectors are interleaved like the reference: U = [u1 v1 u2 v2 ...] (2S),
+ *     F = [F11 F21 F12 F22 ...] (4S), F = grad u.  H21 holds the lower triangle of
+ *     each node's 6x6 Hessian in MATLAB column-major order (S x 21, funICGN.m:320).
+ *   - Per-subset status (the reference's ConvItPerEle): 1..100 = converged after
+ *     that many residual evaluations, 101 = not converged / left the image / NaN,
+ *     102 = more than half of the subset masked, -1 = indexing error (window or
+ *     its gradient outside the image).  Failed subsets return NaN like the
+ *     reference's try/catch.
+ *   - Buffers are caller-owned host memory; the library owns device memory.  Calls
+ *     are synchronous on return.  One context per host thread.
+ *   - Return value: 0 on success, a negative ALDIC_ERR_* code otherwise
+ *     (aldic_last_error() gives the message).  No C++ exception crosses the ABI.
+ */
+#ifndef ALDIC_H
+#define ALDIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALDIC_OK 0
+#define ALDIC_ERR_ARG -1   /* invalid argument (sizes, NULL, unsupported window) */
+#define ALDIC_ERR_HIP -2   /* HIP runtime error */
+#define ALDIC_ERR_STATE -3 /* call order (e.g. no images or mesh set) */
+
+enum aldic_dtype { ALDIC_U8 = 0, ALDIC_U16 = 1, ALDIC_F32 = 2, ALDIC_F64 = 3 };
+
+typedef struct aldic_ctx aldic_ctx;
+
+/* Context on HIP device `device`. */
+int aldic_create(aldic_ctx **ctx, int device);
+void aldic_destroy(aldic_ctx *ctx);
+const char *aldic_last_error(const aldic_ctx *ctx);
+
+/* funNormalizeImg.m:22-40 + funImgGradient.m:53-78 (called main_ALDIC.m Section 2).
+ * Uploads the raw reference/deformed pair, normalises each by its own ROI mean and
+ * N-1 standard deviation (roi = {x1, x2, y1, y2}, 1-based inclusive, clipped;
+ * NULL = whole image) and computes the 7-tap gradients of the reference, valid for
+ * 4 <= x <= nx-3, 4 <= y <= ny-3, times mask (nx*ny bytes, NULL = all ones). */
+int aldic_set_images(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
+                     const uint8_t *mask, const int *roi);
+
+/* The same with the pair already normalised by the caller (the reference's
+ * fNormalized/gNormalized, FP64).  Gradients are computed on the device unless
+ * aldic_set_gradients() supplies them afterwards. */
+int aldic_set_normalized(aldic_ctx *ctx, const double *f, const double *g, int nx, int ny);
+
+/* The reference's Df.DfDx / Df.DfDy (funImgGradient.m:72-78) as full-size nx*ny
+ * arrays: gx(x, y) == DfDx(x-3, y-3) inside the valid band, ignored outside. */
+int aldic_set_gradients(aldic_ctx *ctx, const double *gx, const double *gy);
+
+/* Device images back to the host (any pointer may be NULL), full size nx*ny. */
+int aldic_get_images(aldic_ctx *ctx, double *f, double *g, double *gx, double *gy);
+
+/* funIntegerSearch.m:14-25,90-91: trimmed start corner and subset counts for a
+ * whole-field search.  grid = {x1, x2, y1, y2}, s = {sx, sy}.
+ * out = {x1', y1', Nx, Ny}.  Host-only. */
+int aldic_search_grid(int nx, int ny, const int *grid, int ws, int step, const int *s, int *out);
+
+/* funIntegerSearchWholeField (funIntegerSearch.m:82-200, called by IntegerSearch.m:54
+ * with method 1) over the subsets aldic_search_grid() gives.  Outputs are Nx*Ny
+ * arrays, entry l*Nx + k = subset (k, l) (x fastest): centres x, y; displacement u, v
+ * (findpeak.m subpixel fit, 1e-3 px rounding); cc.max; status (0 ok, -1 constant
+ * template -> NaN).  qf (Nx*Ny x 2: PCE, PPE before the per-column normalisation of
+ * :174-177) may be NULL to skip the full-map quality factors. */
+int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, const int *s,
+                         double *x, double *y, double *u, double *v, double *ccmax, double *qf,
+                         int *status);
+
+/* LocalICGN.m:91-107 -> funICGN.m:1-322: 6-DOF IC-GN of S subsets centred at coords
+ * (S x 2) from U0 (2S), subset size ws[j] (NULL -> ws_all), stop tolerance tol.
+ * Outputs U (2S), F (4S), H21 (S x 21), iters (S); zncc (S, nullable) is the
+ * zero-normalised cross-correlation of the final residual evaluation. */
+int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, const int *ws,
+                int ws_all, double tol, double *U, double *F, double *H21, int *iters,
+                double *zncc);
+
+/* Subpb1.m:119-152 -> funICGN_Subpb1.m:1-362: 2-DOF IC-GN with F frozen at Fold and
+ * the penalty mu * (P(5:6) - Uold - vdual); H21 from LocalICGN.  udual is unused by
+ * the reference and is not taken. */
+int aldic_icgn2(aldic_ctx *ctx, int S, const double *coords, const int *ws, int ws_all,
+                const double *H21, double mu, const double *vdual, const double *Uold,
+                const double *Fold, double tol, double *U, int *iters);
+
+/* DICmesh of MeshSetUp.m:80-89: nn nodes (nn x 2 coordinates), E Q4 elements
+ * (E x 4, 0-based node indices, counter-clockwise).  Kept on the device for
+ * aldic_subpb2 / aldic_nodal_strain. */
+int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elems, int E);
+
+/* Subpb2.m:1-197 (called main_ALDIC.m:287,304,393): assembles
+ * (beta K + mu M + alpha mu K) Uhat = beta G (F - udual) + mu M (U - vdual) with
+ * 3x3 Gauss points and solves it with FP64 Jacobi-PCG to relative residual rtol
+ * (maxit iterations at most).  stats (nullable) = {iterations, relative residual}. */
+int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const double *U,
+                 const double *F, const double *udual, const double *vdual, double rtol,
+                 int maxit, double *Uhat, double *stats);
+
+/* funGlobal_NodalStrainAvg.m:1-147: nodal gradient by 2x2 Gauss points,
+ * extrapolation and element averaging.  swap_fix = 0 writes the reference's slot
+ * order [u_x u_y v_x v_y] (off-diagonals swapped relative to F's convention);
+ * swap_fix = 1 writes [u_x v_x u_y v_y]. */
+int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F);
+
+/* ba_interp2.cpp:318-335 (funICGN.m:125): Keys-cubic samples of the deformed image
+ * at (X[i], Y[i]) (X along dim 1), clamp-to-edge. */
+int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, double *Z);
+
+/* ---- one whole frame on the device (main_ALDIC.m:87-451, FEM branch) ----------
+ * Raw images are staged once; aldic_frame_run() then runs, without host
+ * round-trips: normalisation + gradients, integer search, 6-DOF IC-GN with the
+ * LocalICGN bad-point rule (LocalICGN.m:119-139) and refill, the step-1 Subpb2 +
+ * nodal strain + dual update at fixed beta, and n_admm iterations of
+ * (2-DOF IC-GN -> Subpb2 -> nodal strain -> dual update) (main_ALDIC.m:357-451).
+ * Smoothing (funSmoothDisp/funSmoothStrain) is off. */
+typedef struct {
+    int ws, step;
+    int s[2];          /* search radius */
+    int grid[4];       /* x1 x2 y1 y2 ROI for the search grid */
+    double tol;        /* IC-GN stop tolerance (1e-2) */
+    double mu, beta;   /* ADMM penalties */
+    int n_admm;        /* ADMM iterations after step 1 */
+    int cg_maxit;      /* PCG iterations per Subpb2 */
+    double cg_rtol;    /* PCG relative residual */
+} aldic_frame_opts;
+
+int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
+                      const uint8_t *mask, const int *roi);
+int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *opts, int *n_nodes);
+int aldic_frame_run(aldic_ctx *ctx);
+/* Results of the last frame: U, F (after the last global step), U6/F6/iters6 of the
+ * 6-DOF IC-GN before post-processing, iters of the last 2-DOF step; any may be NULL. */
+int aldic_frame_get(aldic_ctx *ctx, double *U, double *F, double *U6, double *F6, int *iters6,
+                    int *iters2, double *u0);
+/* Stage times of the last frame in ms: {prep, search, icgn6, post6, subpb2 total,
+ * icgn2 total, strain+dual total, frame}. n <= 8. */
+int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

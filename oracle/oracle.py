@@ -220,3 +220,104 @@ def nodal_strain(coords, elems, U, swap_fix=False):
     F = np.empty(4 * nn)
     lib().orc_nodal_strain(coords, nn, elems, elems.shape[0], _f64(U), int(swap_fix), F)
     return F
+
+
+# ---- whole frame (main_ALDIC.m:87-451, FEM branch, smoothing off) -------------------
+
+def badpoint(iters, mean_rule):
+    """LocalICGN.m:119-131 (mean_rule) / Subpb1.m:167-169: bad-subset mask."""
+    it = np.asarray(iters)
+    bad = (it < 0) | (it > 99)
+    good = ~bad
+    if mean_rule and good.any():
+        m = it[good].mean()
+        sd = it[good].std(ddof=1) if good.sum() > 1 else 0.0
+        bad = bad | (it > m + 0.15 * sd)
+    return bad
+
+
+def spring_fill(V, nc, bad, Nx, Ny, sweeps=12):
+    """Red-black sweeps: each bad node takes the mean of its non-NaN 4-neighbours
+    (neighbour order left, right, down, up, summed in that order)."""
+    G = V.copy().reshape(Ny, Nx, nc)
+    B = bad.reshape(Ny, Nx)
+    yy, xx = np.mgrid[0:Ny, 0:Nx]
+    for k in range(2 * sweeps):
+        sel = B & (((xx + yy) & 1) == (k & 1))
+        s = np.zeros_like(G)
+        n = np.zeros(G.shape, np.int64)
+        P = np.pad(G, ((1, 1), (1, 1), (0, 0)), constant_values=np.nan)
+        for (dy, dx) in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+            nb = P[1 + dy:1 + dy + Ny, 1 + dx:1 + dx + Nx]
+            ok = ~np.isnan(nb)
+            s = s + np.where(ok, nb, 0.0)
+            n = n + ok
+        upd = sel[:, :, None] & (n > 0)
+        G = np.where(upd, s / np.maximum(n, 1), G)
+    return G.reshape(-1)
+
+
+def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
+          cg_rtol=1e-10, fill_sweeps=12):
+    """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
+    refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1 2-DOF IC-GN with
+    refill, Subpb2, strain, duals).  Returns the stages' outputs and times."""
+    import time
+    import scipy.sparse as sp
+    T = {}
+    t0 = time.perf_counter()
+    f, _ = normalize(np.asarray(fraw, np.float64))
+    g, _ = normalize(np.asarray(graw, np.float64))
+    gx, gy = gradient(f)
+    T["prep"] = time.perf_counter() - t0
+    t = time.perf_counter()
+    r = search(f, g, grid, ws, step, [s, s] if np.isscalar(s) else s)
+    T["search"] = time.perf_counter() - t
+    Nx, Ny = r["Nx"], r["Ny"]
+    S = Nx * Ny
+    xy = np.stack([r["x"].ravel(), r["y"].ravel()], 1)
+    U0 = np.stack([r["u"].ravel(), r["v"].ravel()], 1).ravel()
+    t = time.perf_counter()
+    U6, F6, H, it6, z = icgn6(f, g, gx, gy, xy, U0, ws, tol)
+    T["icgn6"] = time.perf_counter() - t
+    t = time.perf_counter()
+    bad = badpoint(it6, True)
+    U1 = U6.copy(); F1 = F6.copy()
+    U1.reshape(-1, 2)[bad] = np.nan
+    F1.reshape(-1, 4)[bad] = np.nan
+    U1 = spring_fill(U1, 2, bad, Nx, Ny, fill_sweeps)
+    F1 = spring_fill(F1, 4, bad, Nx, Ny, fill_sweeps)
+    T["post6"] = time.perf_counter() - t
+    el = []
+    for l in range(Ny - 1):
+        for k in range(Nx - 1):
+            n0 = l * Nx + k
+            el.append([n0, n0 + 1, n0 + Nx + 1, n0 + Nx])
+    el = np.array(el, np.int32)
+    if beta is None:
+        beta = 1e-2 * step * step * mu
+    udual = np.zeros(4 * S)
+    vdual = np.zeros(2 * S)
+    T["subpb2"] = T["icgn2"] = 0.0
+
+    def global_step(U1, F1, udual, vdual):
+        t = time.perf_counter()
+        U2, _ = subpb2(xy, el, beta, mu, U1, F1, udual, vdual, rtol=cg_rtol)
+        F2 = nodal_strain(xy, el, U2)
+        T["subpb2"] += time.perf_counter() - t
+        return U2, F2, F2 - F1, U2 - U1
+
+    U2, F2, udual, vdual = global_step(U1, F1, udual, vdual)
+    it2 = None
+    for _ in range(n_admm):
+        t = time.perf_counter()
+        U1, it2 = icgn2(f, g, gx, gy, xy, ws, H, mu, udual, vdual, U2, F2, tol)
+        bad = badpoint(it2, False)
+        U1.reshape(-1, 2)[bad] = np.nan
+        U1 = spring_fill(U1, 2, bad, Nx, Ny, fill_sweeps)
+        F1 = F2
+        T["icgn2"] += time.perf_counter() - t
+        U2, F2, udual, vdual = global_step(U1, F1, udual, vdual)
+    T["frame"] = time.perf_counter() - t0
+    return dict(U=U2, F=F2, U6=U6, F6=F6, iters6=it6, iters2=it2, U0=U0, xy=xy, Nx=Nx, Ny=Ny,
+                times=T)

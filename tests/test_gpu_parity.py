@@ -1,0 +1,196 @@
+"""Parity of the HIP library (through the C ABI) with the FP64 oracle.
+
+Tolerances (FP32 device arithmetic against FP64):
+  * images / gradients / bicubic samples: 2e-6 relative to the data range;
+  * integer search: the integer peak is identical and the sub-pixel result within
+    1e-3 px (+1e-9; findpeak rounds to 1e-3, so an FP32 value next to a rounding
+    boundary may land one step away);
+  * IC-GN: the same status for >= 97 % of the subsets; where the status agrees and is
+    a convergence (1..100): |dU| <= 1e-3 px and |dF| <= 1e-5, RMS dU <= 2e-4 px;
+  * Subpb2 / nodal strain: 1e-9 relative (FP64 on both sides).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1(gpu_ctx, cfg1, cfg1_norm):
+    _flush("config1: set_images")
+    gpu_ctx.set_images(cfg1.f, cfg1.g)
+    return gpu_ctx
+
+
+def test_prep_matches_oracle(c1, cfg1_norm):
+    f, g, gx, gy = cfg1_norm
+    df, dg, dgx, dgy = c1.get_images()
+    assert np.max(np.abs(df - f)) < 2e-6 * np.ptp(f)
+    assert np.max(np.abs(dg - g)) < 2e-6 * np.ptp(g)
+    assert np.max(np.abs(dgx - gx)) < 2e-6 * np.ptp(gx)
+    assert np.max(np.abs(dgy - gy)) < 2e-6 * np.ptp(gy)
+    assert np.array_equal(dgx == 0, gx == 0)        # exact zeros drive the mask rule
+
+
+def test_bicubic_matches_oracle(c1, cfg1_norm):
+    _, g, _, _ = cfg1_norm
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-2, 260, 4000)
+    Y = rng.uniform(-2, 260, 4000)
+    Z = c1.bicubic(X, Y)
+    from oracle import bicubic
+    R = bicubic(g, X, Y)
+    assert np.max(np.abs(Z - R)) < 2e-6 * np.ptp(g)
+
+
+def test_search_matches_oracle_config1(c1, cfg1_norm, orc):
+    f, g, _, _ = cfg1_norm
+    r = c1.integer_search([1, 256, 1, 256], 20, 10, [8, 8])
+    o = orc.search(f, g, [1, 256, 1, 256], 20, 10, [8, 8])
+    assert (r["Nx"], r["Ny"]) == (o["Nx"], o["Ny"]) == (19, 19)
+    assert np.array_equal(r["x"], o["x"]) and np.array_equal(r["y"], o["y"])
+    assert np.array_equal(r["status"], o["status"])
+    assert np.max(np.abs(r["u"] - o["u"])) <= 1e-3 + 1e-9
+    assert np.max(np.abs(r["v"] - o["v"])) <= 1e-3 + 1e-9
+    assert np.mean(r["u"] == o["u"]) > 0.95
+    assert np.max(np.abs(r["ccmax"] - o["ccmax"])) < 1e-4
+
+
+def _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.97):
+    same = it == ito
+    assert same.mean() >= frac, f"status agreement {same.mean():.3f}"
+    ok = same & (it >= 1) & (it <= 100)
+    assert ok.sum() > 0
+    dU = np.abs(U.reshape(-1, 2)[ok] - Uo.reshape(-1, 2)[ok])
+    dF = np.abs(F.reshape(-1, 4)[ok] - Fo.reshape(-1, 4)[ok])
+    assert dU.max() <= 1e-3, dU.max()
+    assert dF.max() <= 1e-5, dF.max()
+    assert np.sqrt(np.mean(dU ** 2)) <= 2e-4
+    return ok
+
+
+def test_icgn6_matches_oracle_config1(c1, cfg1_norm, orc, cfg1):
+    f, g, gx, gy = cfg1_norm
+    o = orc.search(f, g, [1, 256, 1, 256], 20, 10, [8, 8])
+    xy = np.stack([o["x"].ravel(), o["y"].ravel()], 1)
+    U0 = np.stack([o["u"].ravel(), o["v"].ravel()], 1).ravel()
+    U, F, H, it, z = c1.icgn6(xy, U0, 20)
+    Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gx, gy, xy, U0, 20)
+    ok = _cmp_icgn(U, F, it, Uo, Fo, ito)
+    assert np.max(np.abs(H - Ho) / (np.abs(Ho).max(1, keepdims=True) + 1e-30)) < 1e-5
+    assert np.max(np.abs(z[ok] - zo[ok])) < 1e-5
+    ut, vt = cfg1.ufun(xy[:, 0], xy[:, 1])
+    assert np.sqrt(np.mean((U[0::2][ok] - ut[ok]) ** 2)) < 0.02
+
+
+def test_icgn6_status_codes_gpu(c1, cfg1_norm, orc):
+    f, g, gx, gy = cfg1_norm
+    xy = np.array([[12.0, 100.0], [100.0, 100.0], [100.0, 100.0], [101.5, 100.0]])
+    U0 = np.array([0, 0, np.nan, 0, 300.0, 0, 0, 0])
+    U, F, H, it, z = c1.icgn6(xy, U0, 20)
+    assert list(it) == [-1, 101, 101, -1]
+    assert np.isnan(U[0]) and np.isnan(U[6]) and np.all(H[0] == 0)
+
+
+def test_icgn6_masked_window_gpu(gpu_ctx, cfg1, orc, cfg1_norm):
+    f, g, gx, gy = cfg1_norm
+    mask = np.ones((256, 256), np.uint8)
+    mask[80:121, 80:121] = 0                       # >= 50 % of the window at (100,100)
+    mask[140:146, 140:161] = 0                     # 10-50 %: window grows to round(sqrt2*20)
+    gpu_ctx.set_images(cfg1.f, cfg1.g, mask=mask)
+    gxm, gym = orc.gradient(f, mask)
+    xy = np.array([[100.0, 100.0], [150.0, 150.0], [60.0, 200.0]])
+    U0 = np.array([2.4, -1.6, 2.4, -1.6, 2.4, -1.6])
+    U, F, H, it, z = gpu_ctx.icgn6(xy, U0, 20)
+    Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gxm, gym, xy, U0, 20)
+    assert it[0] == ito[0] == 102
+    assert list(it) == list(ito)
+    assert np.allclose(U[2:], Uo[2:], atol=1e-3)
+    gpu_ctx.set_images(cfg1.f, cfg1.g)
+
+
+def test_icgn6_uneven_sizes_and_wide_windows(gpu_ctx, speckle, orc):
+    p = speckle.small_pair(n=200, u0=1.2, v0=-0.8, ux=0.003, vy=0.002, seed=21)
+    gpu_ctx.set_images(p.f, p.g)
+    f, _ = orc.normalize(p.f.astype(float))
+    g, _ = orc.normalize(p.g.astype(float))
+    gx, gy = orc.gradient(f)
+    c = np.arange(50, 151, 10, dtype=float)
+    X, Y = np.meshgrid(c, c)
+    xy = np.stack([X.ravel(), Y.ravel()], 1)
+    S = len(xy)
+    ws = np.array([16, 20, 32, 40, 64, 48][k % 6] for k in range(S)) if False else \
+        np.array([[16, 20, 32, 40, 64, 48][k % 6] for k in range(S)], np.int32)
+    U0 = np.tile([1.0, -1.0], S)
+    U, F, H, it, z = gpu_ctx.icgn6(xy, U0, ws)
+    Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gx, gy, xy, U0, ws)
+    _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.95)
+
+
+def test_icgn2_matches_oracle(gpu_ctx, cfg1, cfg1_norm, orc):
+    gpu_ctx.set_images(cfg1.f, cfg1.g)
+    f, g, gx, gy = cfg1_norm
+    o = orc.search(f, g, [1, 256, 1, 256], 20, 10, [8, 8])
+    xy = np.stack([o["x"].ravel(), o["y"].ravel()], 1)
+    S = len(xy)
+    U0 = np.stack([o["u"].ravel(), o["v"].ravel()], 1).ravel()
+    Uo6, Fo6, Ho, ito6, _ = orc.icgn6(f, g, gx, gy, xy, U0, 20)
+    rng = np.random.default_rng(5)
+    Uold = Uo6 + rng.normal(scale=0.05, size=2 * S)
+    Fold = np.where(np.isnan(Fo6), 0, Fo6)
+    Uold = np.where(np.isnan(Uold), 0, Uold)
+    vdual = rng.normal(scale=0.01, size=2 * S)
+    U, it = gpu_ctx.icgn2(xy, 20, Ho, 1e-3, vdual, Uold, Fold)
+    Uo, ito = orc.icgn2(f, g, gx, gy, xy, 20, Ho, 1e-3, np.zeros(4 * S), vdual, Uold, Fold)
+    same = it == ito
+    assert same.mean() >= 0.97
+    ok = same & (it >= 1) & (it <= 100)
+    dU = np.abs(U.reshape(-1, 2)[ok] - Uo.reshape(-1, 2)[ok])
+    assert dU.max() <= 1e-3 and np.sqrt(np.mean(dU ** 2)) <= 2e-4
+
+
+def _grid_mesh(nx, ny, h, x0=20.0, y0=30.0):
+    X, Y = np.meshgrid(x0 + h * np.arange(nx), y0 + h * np.arange(ny))
+    coords = np.stack([X.ravel(), Y.ravel()], 1)
+    el = []
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            n0 = j * nx + i
+            el.append([n0, n0 + 1, n0 + nx + 1, n0 + nx])
+    return coords, np.array(el, np.int32)
+
+
+def test_subpb2_and_strain_match_oracle(gpu_ctx, orc):
+    coords, el = _grid_mesh(23, 17, 10.0)
+    n = len(coords)
+    gpu_ctx.set_mesh(coords, el)
+    rng = np.random.default_rng(7)
+    U = rng.normal(size=2 * n)
+    F = rng.normal(scale=0.01, size=4 * n)
+    W = rng.normal(scale=0.01, size=4 * n)
+    V = rng.normal(scale=0.1, size=2 * n)
+    for beta in (1e-4, 2.56e-3, 0.1):
+        Uh, st = gpu_ctx.subpb2(beta, 1e-3, U, F, W, V, rtol=1e-13)
+        Uo, _ = orc.subpb2(coords, el, beta, 1e-3, U, F, W, V)
+        assert np.max(np.abs(Uh - Uo)) <= 1e-9 * np.max(np.abs(Uo)), (beta, st)
+    Fg = gpu_ctx.nodal_strain(Uh)
+    Fr = orc.nodal_strain(coords, el, Uh)
+    assert np.max(np.abs(Fg - Fr)) < 1e-12 * max(1.0, np.abs(Fr).max())
+    Fg2 = gpu_ctx.nodal_strain(Uh, swap_fix=True)
+    assert np.allclose(Fg2.reshape(-1, 4)[:, [0, 2, 1, 3]], Fg.reshape(-1, 4), atol=0)
+
+
+def test_subpb2_patch_test_gpu(gpu_ctx):
+    coords, el = _grid_mesh(31, 19, 16.0)
+    n = len(coords)
+    gpu_ctx.set_mesh(coords, el)
+    U = np.empty(2 * n)
+    U[0::2] = 0.5 + 0.01 * coords[:, 0] - 0.02 * coords[:, 1]
+    U[1::2] = -1.2 + 0.03 * coords[:, 0] + 0.005 * coords[:, 1]
+    F = np.tile([0.01, 0.03, -0.02, 0.005], n)
+    Uh, st = gpu_ctx.subpb2(2.56e-3, 1e-3, U, F, np.zeros(4 * n), np.zeros(2 * n), rtol=1e-13)
+    assert np.allclose(Uh, U, atol=1e-8)

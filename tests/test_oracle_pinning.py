@@ -1,0 +1,371 @@
+"""Pins the FP64 oracle with checks that do not restate its formulas: closed forms,
+invariants, independent numpy/scipy machinery and brute force on tiny inputs.
+
+The reference ships no kernel-level golden vectors (it has no tests; MATLAB is absent
+here), so these checks are what anchors the oracle."""
+import numpy as np
+import pytest
+
+
+# ---- a1 funNormalizeImg --------------------------------------------------------------
+
+def test_normalize_roi_mean_std(orc):
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 256, (40, 50)).astype(float)
+    out, ms = orc.normalize(img, [5, 30, 3, 20])
+    roi = out[2:20, 4:30]          # rows y 3..20, cols x 5..30
+    assert abs(roi.mean()) < 1e-12
+    assert abs(roi.std(ddof=1) - 1.0) < 1e-12
+    assert np.allclose(out, (img - img[2:20, 4:30].mean()) / img[2:20, 4:30].std(ddof=1))
+
+
+def test_normalize_roi_clipped(orc):
+    img = np.arange(30 * 20, dtype=float).reshape(20, 30)
+    a, _ = orc.normalize(img, [-5, 100, 0, 999])
+    b, _ = orc.normalize(img, [1, 30, 1, 20])
+    assert np.array_equal(a, b)
+
+
+# ---- a2 funImgGradient ----------------------------------------------------------------
+
+def test_gradient_exact_on_polynomials(orc):
+    ny, nx = 30, 40
+    y, x = np.mgrid[1:ny + 1, 1:nx + 1].astype(float)
+    f = 0.3 * x**2 - 0.2 * x * y + 0.05 * y**3 + 1e-4 * x**6
+    gx, gy = orc.gradient(f)
+    ex = 0.6 * x - 0.2 * y + 6e-4 * x**5
+    ey = -0.2 * x + 0.15 * y**2
+    band = (slice(3, ny - 3), slice(3, nx - 3))   # 4 <= x, y <= n-3 (1-based)
+    assert np.allclose(gx[band], ex[band], rtol=1e-10, atol=1e-8)
+    assert np.allclose(gy[band], ey[band], rtol=1e-10, atol=1e-8)
+    # outside the valid band the reference has no value (cropped): stored as 0
+    assert np.all(gx[:3] == 0) and np.all(gx[:, -3:] == 0)
+
+
+def test_gradient_mask_and_flat_zero(orc):
+    f = np.zeros((20, 20))
+    f[:, 12:] = np.arange(8)[None, :] * 2.0
+    mask = np.ones((20, 20), np.uint8)
+    mask[5:8, 5:8] = 0
+    gx, gy = orc.gradient(f, mask)
+    assert np.all(gx[3:17, 3:8] == 0.0)            # flat region -> exact zeros
+    assert np.all(gx[5:8, 5:8] == 0.0)
+    assert gx[10, 16] == pytest.approx(2.0)          # linear ramp slope 2
+
+
+# ---- a11 ba_interp2 (Keys cubic) ---------------------------------------------------------
+
+def keys_kernel(t):
+    """Keys (1981) cubic convolution kernel, a = -1/2, piecewise form."""
+    t = np.abs(t)
+    return np.where(t <= 1, 1.5 * t**3 - 2.5 * t**2 + 1,
+                    np.where(t < 2, -0.5 * t**3 + 2.5 * t**2 - 4 * t + 2, 0.0))
+
+
+def test_bicubic_nodes_and_polynomial_reproduction(orc):
+    ny, nx = 24, 30
+    y, x = np.mgrid[1:ny + 1, 1:nx + 1].astype(float)
+    img = 1.5 + 0.25 * x - 0.7 * y + 0.01 * x * y + 0.02 * x**2 - 0.03 * y**2
+    rng = np.random.default_rng(1)
+    X = rng.uniform(4, nx - 4, 500)
+    Y = rng.uniform(4, ny - 4, 500)
+    Z = orc.bicubic(img, X, Y)
+    exact = 1.5 + 0.25 * X - 0.7 * Y + 0.01 * X * Y + 0.02 * X**2 - 0.03 * Y**2
+    assert np.allclose(Z, exact, atol=1e-11)    # Keys reproduces quadratics
+    Xi = rng.integers(3, nx - 2, 50).astype(float)
+    Yi = rng.integers(3, ny - 2, 50).astype(float)
+    Zi = orc.bicubic(img, Xi, Yi)
+    assert np.array_equal(Zi, img[Yi.astype(int) - 1, Xi.astype(int) - 1])
+
+
+def test_bicubic_matches_kernel_sum_with_clamp(orc):
+    rng = np.random.default_rng(2)
+    ny, nx = 12, 15
+    img = rng.normal(size=(ny, nx))
+    X = rng.uniform(-1.5, nx + 2.5, 300)
+    Y = rng.uniform(-1.5, ny + 2.5, 300)
+    Z = orc.bicubic(img, X, Y)
+    ref = np.empty_like(Z)
+    for k in range(X.size):
+        xf, yf = np.floor(X[k]), np.floor(Y[k])
+        s = 0.0
+        for a in range(-1, 3):
+            for b in range(-1, 3):
+                cx = int(np.clip(xf + a, 1, nx)) - 1
+                cy = int(np.clip(yf + b, 1, ny)) - 1
+                s += keys_kernel(X[k] - (xf + a)) * keys_kernel(Y[k] - (yf + b)) * img[cy, cx]
+        ref[k] = s
+    assert np.allclose(Z, ref, atol=1e-12)
+
+
+# ---- a4 normxcorr2 definition / a6 findpeak --------------------------------------------
+
+def ncc_bruteforce(T, D):
+    """Independent restatement with scipy.signal.correlate2d and explicit padding."""
+    from scipy.signal import correlate2d
+    n = T.size
+    Tm = T - T.mean()
+    num = correlate2d(D, Tm, mode="full")
+    ones = np.ones_like(T)
+    s1 = correlate2d(D, ones, mode="full")
+    s2 = correlate2d(D * D, ones, mode="full")
+    var = np.maximum(s2 - s1 * s1 / n, 0)
+    den = np.sqrt((Tm**2).sum() * var)
+    return np.where(den > 0, num / np.where(den > 0, den, 1), 0.0)
+
+
+def test_ncc_full_matches_bruteforce_and_embedded_peak(orc):
+    rng = np.random.default_rng(3)
+    D = rng.normal(size=(13, 11))
+    T = D[4:9, 2:7].copy()                  # 5x5 template embedded at offset (4, 2)
+    C, ok = orc.ncc_full(T, D)
+    assert ok == 0
+    ref = ncc_bruteforce(T, D)
+    assert C.shape == ref.shape == (17, 15)
+    assert np.allclose(C, ref, atol=1e-12)
+    assert np.all(np.abs(C) <= 1 + 1e-12)
+    # full index of corner offset (4, 2) in (row=y, col=x): (4 + 4, 2 + 4)
+    assert C[8, 6] == pytest.approx(1.0, abs=1e-12)
+    assert np.unravel_index(np.argmax(C), C.shape) == (8, 6)
+
+
+def test_ncc_constant_template_raises(orc):
+    _, ok = orc.ncc_full(np.ones((5, 5)), np.random.default_rng(0).normal(size=(9, 9)))
+    assert ok == -1
+
+
+def test_findpeak_paraboloid_and_rules(orc):
+    nr, nc = 9, 11
+    r, c = np.mgrid[1:nr + 1, 1:nc + 1].astype(float)
+    r0, c0 = 4.3172, 6.7461
+    f = 1.0 - 0.05 * (r - r0) ** 2 - 0.03 * (c - c0) ** 2 + 0.01 * (r - r0) * (c - c0)
+    xp, yp, mf = orc.findpeak(f)
+    # a quadratic surface is fitted exactly; offsets are rounded to 1e-3 (findpeak.m:58-59)
+    assert abs(xp - c0) <= 5e-4 + 1e-12 and abs(yp - r0) <= 5e-4 + 1e-12
+    assert xp == pytest.approx(round(c0, 3)) and yp == pytest.approx(round(r0, 3))
+    # least squares of the 3x3 neighbourhood equals numpy lstsq
+    ri, ci = 4, 7
+    u = f[ri - 2:ri + 1, ci - 2:ci + 1].ravel(order="F")
+    x = np.array([-1, -1, -1, 0, 0, 0, 1, 1, 1.0])
+    y = np.array([-1, 0, 1, -1, 0, 1, -1, 0, 1.0])
+    A = np.linalg.lstsq(np.stack([np.ones(9), x, y, x * y, x * x, y * y], 1), u, rcond=None)[0]
+    xo, yo = round(c0 - ci, 3), round(r0 - ri, 3)
+    assert mf == pytest.approx(A @ [1, xo, yo, xo * yo, xo * xo, yo * yo], abs=1e-12)
+    # edge peak -> integer location and raw maximum
+    g = np.zeros((5, 5)); g[0, 3] = 2.0
+    assert orc.findpeak(g) == (4.0, 1.0, 2.0)
+    # ties -> first in column-major order
+    h = np.zeros((5, 5)); h[3, 1] = 1.0; h[1, 2] = 1.0
+    xp, yp, _ = orc.findpeak(h)
+    assert (round(xp), round(yp)) == (2, 4)
+
+
+def test_search_integer_shift(orc, speckle):
+    p = speckle.small_pair(n=128, u0=0.0, v0=0.0, seed=5)
+    f = p.f.astype(float)
+    g = np.zeros_like(f)
+    a, b = 3, -2                              # g(x) = f(x - a, y - b): u = a, v = b
+    g[max(b, 0):128 + min(b, 0), max(a, 0):128 + min(a, 0)] = \
+        f[max(-b, 0):128 - max(b, 0), max(-a, 0):128 - max(a, 0)]
+    r = orc.search(f, g, [1, 128, 1, 128], 20, 10, [6, 6])
+    assert (r["status"] == 0).all()
+    assert np.all(np.round(r["u"]) == a) and np.all(np.round(r["v"]) == b)
+    assert np.max(np.abs(r["u"] - a)) < 0.1 and np.max(np.abs(r["v"] - b)) < 0.1
+    # cc.max is the fitted paraboloid value at the sub-pixel peak (findpeak.m:65)
+    assert np.all(np.abs(r["ccmax"] - 1) < 0.01)
+
+
+def test_search_grid_trim_counts(orc):
+    # the subset counts of the three uniform workloads under the reference rule
+    assert orc.search_grid(256, 256, [1, 256, 1, 256], 20, 10, [8, 8])[2:] == [19, 19]
+    assert orc.search_grid(2048, 2048, [1, 2048, 1, 2048], 32, 16, [8, 8])[2:] == [122, 122]
+    assert orc.search_grid(4096, 4096, [1, 4096, 1, 4096], 32, 8, [24, 24])[2:] == [497, 497]
+
+
+# ---- a10 / a13 IC-GN ---------------------------------------------------------------------
+
+def _norm(orc, p):
+    f, _ = orc.normalize(p.f.astype(float))
+    g, _ = orc.normalize(p.g.astype(float))
+    gx, gy = orc.gradient(f)
+    return f, g, gx, gy
+
+
+def _nodes(n, w, step):
+    c = np.arange(w // 2 + 8, n - w // 2 - 8, step, dtype=float)
+    X, Y = np.meshgrid(c, c)
+    return np.stack([X.ravel(), Y.ravel()], 1)
+
+
+def test_icgn6_zero_deformation(orc, speckle):
+    p = speckle.small_pair(n=96, u0=0, v0=0, seed=7)
+    f, _, gx, gy = _norm(orc, p)
+    xy = _nodes(96, 20, 12)
+    U, F, H, it, z = orc.icgn6(f, f, gx, gy, xy, np.zeros(2 * len(xy)), 20)
+    assert np.all(it == 1)                       # b = 0 at P0 -> stops at once
+    assert np.all(U == 0) and np.all(F == 0)
+    assert np.allclose(z, 1.0)
+
+
+def test_icgn6_recovers_translation_and_affine(orc, speckle):
+    p = speckle.small_pair(n=128, u0=1.37, v0=-0.62, ux=0.004, vy=-0.003, uy=0.002,
+                           vx=-0.001, seed=11)
+    f, g, gx, gy = _norm(orc, p)
+    xy = _nodes(128, 20, 12)
+    U0 = np.zeros(2 * len(xy))
+    U0[0::2] = 1.0
+    U0[1::2] = -1.0
+    U, F, H, it, z = orc.icgn6(f, g, gx, gy, xy, U0, 20, tol=1e-4)
+    ut, vt = p.ufun(xy[:, 0], xy[:, 1])
+    assert np.all((it >= 2) & (it <= 100))
+    assert np.sqrt(np.mean((U[0::2] - ut) ** 2)) < 0.01
+    assert np.sqrt(np.mean((U[1::2] - vt) ** 2)) < 0.01
+    # F = [u_x v_x u_y v_y]
+    Fm = F.reshape(-1, 4).mean(0)
+    assert np.allclose(Fm, [0.004, -0.001, 0.002, -0.003], atol=5e-4)
+
+
+def test_icgn6_hessian_symmetric_psd_and_packing(orc, cfg1_norm):
+    f, g, gx, gy = cfg1_norm
+    xy = np.array([[100.0, 120.0], [60.0, 60.0]])
+    U, F, H, it, z = orc.icgn6(f, g, gx, gy, xy, np.zeros(4), 20)
+    for h21 in H:
+        M = np.zeros((6, 6))
+        k = 0
+        for c in range(6):
+            for r in range(c, 6):
+                M[r, c] = M[c, r] = h21[k]
+                k += 1
+        assert np.all(np.linalg.eigvalsh(M) > 0)
+    # H(5,5) = sum fx^2 and H(6,6) = sum fy^2 over the 21x21 window, packed at 18 and 20
+    x0, y0 = 100, 120
+    win = (slice(y0 - 10 - 1, y0 + 10), slice(x0 - 10 - 1, x0 + 10))
+    assert H[0, 18] == pytest.approx((gx[win] ** 2).sum(), rel=1e-12)
+    assert H[0, 20] == pytest.approx((gy[win] ** 2).sum(), rel=1e-12)
+    assert H[0, 19] == pytest.approx((gx[win] * gy[win]).sum(), rel=1e-12)
+
+
+def test_ic_compose_is_matrix_product(orc):
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        P = rng.normal(scale=0.05, size=6)
+        D = rng.normal(scale=0.05, size=6)
+        Pn, ok = orc.ic_compose(P, D)
+        W = lambda q: np.array([[1 + q[0], q[2], q[4]], [q[1], 1 + q[3], q[5]], [0, 0, 1]])
+        M = W(P) @ np.linalg.inv(W(D))
+        assert ok == 0
+        assert np.allclose(Pn, [M[0, 0] - 1, M[1, 0], M[0, 1], M[1, 1] - 1, M[0, 2], M[1, 2]],
+                           atol=1e-14)
+
+
+def test_icgn6_status_codes(orc, cfg1_norm):
+    f, g, gx, gy = cfg1_norm
+    xy = np.array([[12.0, 100.0],      # window reaches x < 4 -> indexing error (-1)
+                   [100.0, 100.0],     # NaN initial guess -> 101
+                   [100.0, 100.0],     # huge guess -> leaves the image -> 101
+                   [101.5, 100.0]])    # x0 - w/2 not an integer -> -1
+    U0 = np.array([0, 0, np.nan, 0, 300.0, 0, 0, 0])
+    U, F, H, it, z = orc.icgn6(f, g, gx, gy, xy, U0, 20)
+    assert list(it) == [-1, 101, 101, -1]
+    assert np.isnan(U[0]) and np.isnan(U[6])
+    # >= 50% masked window -> 102 with zero Hessian and U = U0
+    gx2 = gx.copy()
+    gx2[80:121, 80:121] = 0.0
+    U, F, H, it, z = orc.icgn6(f, g, gx2, gy, np.array([[100.0, 100.0]]), np.array([1.0, 2.0]), 20)
+    assert it[0] == 102 and np.all(H == 0) and list(U) == [1.0, 2.0]
+
+
+def test_icgn2_matches_translation(orc, speckle):
+    p = speckle.small_pair(n=128, u0=0.83, v0=0.41, seed=13)
+    f, g, gx, gy = _norm(orc, p)
+    xy = _nodes(128, 20, 12)
+    S = len(xy)
+    U6, F6, H, it6, _ = orc.icgn6(f, g, gx, gy, xy, np.zeros(2 * S), 20, tol=1e-4)
+    Uold = np.zeros(2 * S)
+    U2, it2 = orc.icgn2(f, g, gx, gy, xy, 20, H, 0.0, np.zeros(4 * S), np.zeros(2 * S), Uold,
+                        np.zeros(4 * S), tol=1e-4)
+    assert np.all((it2 >= 1) & (it2 <= 100))
+    assert np.max(np.abs(U2 - U6)) < 0.02
+    assert abs(np.mean(U2[0::2]) - 0.83) < 0.01 and abs(np.mean(U2[1::2]) - 0.41) < 0.01
+    # Reference quirk: the penalty step -tempH\tempb (funICGN_Subpb1.m:285) goes through
+    # the inverse-compositional update (:311-329), which negates it, so a penalty that
+    # dominates the image term pushes U away from Uold + vdual until it leaves the image.
+    vd = np.tile([0.1, -0.2], S)
+    U3, it3 = orc.icgn2(f, g, gx, gy, xy, 20, H, 1e6, np.zeros(4 * S), vd, Uold, np.zeros(4 * S))
+    assert np.all(it3 == 101)
+    # the loop test runs before the first evaluation: tol >= 1 returns Uold after 0 steps
+    U4, it4 = orc.icgn2(f, g, gx, gy, xy[:1], 20, H[:1], 1e6, np.zeros(4), np.zeros(2),
+                        np.array([0.5, 0.5]), np.zeros(4), tol=1e9)
+    assert it4[0] == 0 and np.allclose(U4, [0.5, 0.5])
+
+
+# ---- a14 Subpb2 / a15 nodal strain ----------------------------------------------------
+
+def grid_mesh(nx, ny, h, x0=10.0, y0=12.0):
+    X, Y = np.meshgrid(x0 + h * np.arange(nx), y0 + h * np.arange(ny))
+    coords = np.stack([X.ravel(), Y.ravel()], 1)
+    el = []
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            n0 = j * nx + i
+            el.append([n0, n0 + 1, n0 + nx + 1, n0 + nx])
+    return coords, np.array(el, np.int32)
+
+
+def test_subpb2_element_matrices_closed_form(orc):
+    h = 8.0
+    coords, el = grid_mesh(2, 2, h)
+    beta, mu = 0.7, 0.3
+    z2, z4 = np.zeros(8), np.zeros(16)
+    ri, ci, val, b = orc.subpb2_assemble(coords, el, beta, mu, z2, z4, z4, z2)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, (ri, ci)), shape=(8, 8)).toarray()
+    # bilinear square: K = 1/6 [[4 -1 -2 -1] ...], M = h^2/36 [[4 2 1 2] ...]
+    K = np.array([[4, -1, -2, -1], [-1, 4, -1, -2], [-2, -1, 4, -1], [-1, -2, -1, 4]]) / 6.0
+    M = h * h / 36.0 * np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]])
+    loc = beta * K + mu * M                  # element-local node order el[0..3]
+    glob = np.zeros((4, 4))
+    for a in range(4):
+        for b2 in range(4):
+            glob[el[0][a], el[0][b2]] = loc[a, b2]
+    assert np.allclose(A, np.kron(glob, np.eye(2)), atol=1e-12)
+
+
+def test_subpb2_patch_test_and_scipy(orc):
+    coords, el = grid_mesh(7, 5, 10.0)
+    n = len(coords)
+    a, bx, by, c, dx, dy = 0.5, 0.01, -0.02, -1.2, 0.03, 0.005
+    U = np.empty(2 * n)
+    U[0::2] = a + bx * coords[:, 0] + by * coords[:, 1]
+    U[1::2] = c + dx * coords[:, 0] + dy * coords[:, 1]
+    F = np.tile([bx, dx, by, dy], n)           # [u_x v_x u_y v_y]
+    for beta in (1e-4, 0.1, 5.0):
+        Uh, it = orc.subpb2(coords, el, beta, 1e-3, U, F, np.zeros(4 * n), np.zeros(2 * n))
+        assert np.allclose(Uh, U, atol=1e-9)
+    # random data: PCG solution == scipy direct solve
+    rng = np.random.default_rng(9)
+    U = rng.normal(size=2 * n)
+    F = rng.normal(scale=0.01, size=4 * n)
+    W = rng.normal(scale=0.01, size=4 * n)
+    V = rng.normal(scale=0.1, size=2 * n)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    ri, ci, val, b = orc.subpb2_assemble(coords, el, 0.05, 1e-3, U, F, W, V)
+    A = sp.csr_matrix((val, (ri, ci)), shape=(2 * n, 2 * n))
+    ref = spl.spsolve(A.tocsc(), b)
+    Uh, it = orc.subpb2(coords, el, 0.05, 1e-3, U, F, W, V)
+    assert np.allclose(Uh, ref, rtol=1e-10, atol=1e-10)
+    assert np.allclose(A.toarray(), A.toarray().T, atol=1e-15)
+
+
+def test_nodal_strain_linear_field_swapped_slots(orc):
+    coords, el = grid_mesh(6, 4, 7.0)
+    n = len(coords)
+    ux, uy, vx, vy = 0.011, -0.004, 0.007, 0.002
+    U = np.empty(2 * n)
+    U[0::2] = 1.0 + ux * coords[:, 0] + uy * coords[:, 1]
+    U[1::2] = -2.0 + vx * coords[:, 0] + vy * coords[:, 1]
+    F = orc.nodal_strain(coords, el, U)
+    assert np.allclose(F.reshape(-1, 4), [ux, uy, vx, vy], atol=1e-13)   # reference order
+    Ff = orc.nodal_strain(coords, el, U, swap_fix=True)
+    assert np.allclose(Ff.reshape(-1, 4), [ux, vx, uy, vy], atol=1e-13)
