@@ -137,7 +137,7 @@ hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStr
 hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, const double *F,
                    const double *udual, const double *vdual, double *b, hipStream_t st);
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
-                   DBuf *work, double *stats_dev, hipStream_t st);
+                   DBuf *work, double *stats_host, hipStream_t st, int poll);
 hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F,
                       hipStream_t st);
 

@@ -547,7 +547,7 @@ int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const dou
                sc[3].as<double>(), sc[4].as<double>(), ctx->stream));
     static thread_local DBuf work[7];
     double st[2] = {0, 0};
-    CK(fem_pcg(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream));
+    CK(fem_pcg(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream, 8));
     CK(down(Uhat, sc[5].p, 2 * nn, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     if (stats) { stats[0] = st[0]; stats[1] = st[1]; }
@@ -743,7 +743,7 @@ int aldic_frame_run(aldic_ctx *ctx)
         CK(hipEventRecord(ev[5], st));
         CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F.F1.as<double>(),
                    F.udual.as<double>(), F.vdual.as<double>(), b, st));
-        CK(fem_pcg(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st));
+        CK(fem_pcg(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
         CK(hipEventRecord(ev[6], st));
         CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F.F2.as<double>(), st));
         // dual update, FEM branch: udual = F2 - F1, vdual = U2 - U1 (main_ALDIC.m:340,440)

@@ -124,11 +124,52 @@ __global__ __launch_bounds__(256) void k_rhs(int nn, const int *rowptr, const in
 }
 
 // ---- Jacobi-PCG ---------------------------------------------------------------------
-// Device scalar block (doubles): 0 rz, 1 alpha, 2 beta, 3 |b|, 4 rr, 5 done, 6 iters,
-// 7 spare; the counter for last-block reductions is a separate uint.
-struct PcgScal {
-    double rz, alpha, beta, bn, rr, done, iters, rtol;
+// Every reduction is left as per-block partials; the NEXT kernel reduces them in its
+// prologue (each block, same fixed order, so every block computes bitwise-identical
+// scalars).  Kernel boundaries make the partials visible across the per-XCD L2s, so
+// no fence or atomic is needed and the solve is deterministic.
+//   init      : x = 0, r = b, z = D^-1 r, p = z; partials r.z -> rz[0], r.r -> rr[0], b.b
+//   spmv(k)   : beta_k = rz[k]/rz[k-1]; stop when |r| <= rtol |b|; p = z + beta p (on
+//               the fly for the neighbours), Ap, partials p.Ap
+//   update(k) : alpha = rz[k]/p.Ap; x += alpha p; r -= alpha Ap; z = D^-1 r;
+//               partials r.z -> rz[k+1], r.r -> rr[k+1]
+// Block 0 of each spmv records {done, iterations} for the host's convergence poll.
+struct PcgPart {
+    double *pap, *rz0, *rz1, *rr0, *rr1, *bb;
+    int nb;
+    double rtol2;
+    double *stat;  // [0] done, [1] iterations, [2] relative residual
 };
+
+constexpr int kPcgThreads = 256;
+constexpr int kPcgMaxBlocks = 512;
+
+// Sum of one partial array, identical in every block (fixed order).
+__device__ __forceinline__ double part_sum(const double *p, int nb)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x & 63; k < nb; k += 64) s += p[k];
+    return wave_sum(s);
+}
+
+template <int N>
+__device__ __forceinline__ void part_sums(const double *const (&ps)[N], int nb, double (&out)[N])
+{
+    __shared__ double sh[4];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int base = 0; base < N; base += 4) {
+        if (base + w < N) {
+            const double v = part_sum(ps[base + w], nb);
+            if ((threadIdx.x & 63) == 0) sh[w] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (base + q < N) out[base + q] = sh[q];
+        __syncthreads();
+    }
+}
 
 __device__ __forceinline__ void block_reduce2(double &a, double &b, double *red)
 {
@@ -141,27 +182,13 @@ __device__ __forceinline__ void block_reduce2(double &a, double &b, double *red)
     b = red[1] + red[3] + red[5] + red[7];
 }
 
-// Sums the per-block partials in block order (deterministic) in the last block.
-__device__ __forceinline__ bool last_block(unsigned *counter)
-{
-    __shared__ bool amlast;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) amlast = atomicInc(counter, 0xffffffffu) == gridDim.x - 1;
-    __syncthreads();
-    if (amlast) __threadfence();  // acquire: see the other blocks' partials (per-XCD L2s)
-    return amlast;
-}
-
 __global__ __launch_bounds__(256) void k_pcg_init(int nn, const double2 *b, const double *diag,
                                                   double2 *x, double2 *r, double2 *z,
-                                                  double2 *p, double2 *part, unsigned *counter,
-                                                  PcgScal *s)
+                                                  double2 *p, PcgPart P)
 {
     __shared__ double red[8];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, bb = 0.0;
-    if (i < nn) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         const double2 bi = b[i];
         const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
         const double2 zi = make_double2(bi.x * dinv, bi.y * dinv);
@@ -169,74 +196,88 @@ __global__ __launch_bounds__(256) void k_pcg_init(int nn, const double2 *b, cons
         r[i] = bi;
         z[i] = zi;
         p[i] = zi;
-        rz = bi.x * zi.x + bi.y * zi.y;
-        bb = bi.x * bi.x + bi.y * bi.y;
+        rz += bi.x * zi.x + bi.y * zi.y;
+        bb += bi.x * bi.x + bi.y * bi.y;
     }
     block_reduce2(rz, bb, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rz, bb);
-    if (!last_block(counter)) return;
     if (threadIdx.x == 0) {
-        double a = 0.0, c = 0.0;
-        for (unsigned k = 0; k < gridDim.x; ++k) { a += part[k].x; c += part[k].y; }
-        s->rz = a;
-        s->bn = sqrt(c);
-        s->rr = c;
-        s->beta = 0.0;
-        s->iters = 0.0;
-        s->done = (c == 0.0) ? 1.0 : 0.0;
-        *counter = 0;
+        P.rz0[blockIdx.x] = rz;
+        P.rr0[blockIdx.x] = bb;
+        P.bb[blockIdx.x] = bb;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
 }
 
-// p_new = z + beta p_old (computed on the fly for the neighbours), Ap, p.Ap
 __global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, const int *col,
                                                   const double *aval, const double2 *z,
                                                   const double2 *pold, double2 *pnew,
-                                                  double2 *Ap, double2 *part, unsigned *counter,
-                                                  PcgScal *s, int first)
+                                                  double2 *Ap, PcgPart P, int k)
 {
     __shared__ double red[8];
-    if (s->done != 0.0) return;
-    const double beta = first ? 0.0 : s->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *rzk = (k & 1) ? P.rz1 : P.rz0, *rzm = (k & 1) ? P.rz0 : P.rz1;
+    const double *rrk = (k & 1) ? P.rr1 : P.rr0;
+    double s[4];
+    const double *ps[4] = {rzk, rzm, rrk, P.bb};
+    part_sums<4>(ps, P.nb, s);
+    const double rr = s[2], bb = s[3];
+    // stop at the tolerance, on b == 0, on a NaN residual, or on a breakdown
+    // (r.z of the previous step zero, or a non-finite beta)
+    bool done = !(rr > P.rtol2 * bb);
+    if (k > 0 && !(s[1] != 0.0 && isfinite(s[0] / s[1]))) done = true;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.stat[0] = done ? 1.0 : 0.0;
+        P.stat[1] = (double)k;
+        P.stat[2] = bb > 0 ? sqrt(rr / bb) : 0.0;
+    }
+    if (done) return;
+    const bool first = k == 0;
+    const double beta = first ? 0.0 : s[0] / s[1];
     double pap = 0.0, dummy = 0.0;
-    if (i < nn) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         double au = 0.0, av = 0.0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            const int c = col[k];
-            const double2 zc = z[c], pc = pold[c];
-            const double a = aval[k];
-            au += a * (first ? pc.x : zc.x + beta * pc.x);
-            av += a * (first ? pc.y : zc.y + beta * pc.y);
+        for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+            const int c = col[q];
+            const double2 pc = pold[c];
+            const double a = aval[q];
+            if (first) {
+                au += a * pc.x;
+                av += a * pc.y;
+            } else {
+                const double2 zc = z[c];
+                au += a * (zc.x + beta * pc.x);
+                av += a * (zc.y + beta * pc.y);
+            }
         }
         const double2 zi = z[i], pi = pold[i];
         const double2 pn = first ? pi : make_double2(zi.x + beta * pi.x, zi.y + beta * pi.y);
         pnew[i] = pn;
         Ap[i] = make_double2(au, av);
-        pap = pn.x * au + pn.y * av;
+        pap += pn.x * au + pn.y * av;
     }
     block_reduce2(pap, dummy, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(pap, 0.0);
-    if (!last_block(counter)) return;
-    if (threadIdx.x == 0) {
-        double a = 0.0;
-        for (unsigned k = 0; k < gridDim.x; ++k) a += part[k].x;
-        s->alpha = s->rz / a;
-        *counter = 0;
-    }
+    if (threadIdx.x == 0) P.pap[blockIdx.x] = pap;
 }
 
 __global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, const double2 *p,
                                                     const double2 *Ap, double2 *x, double2 *r,
-                                                    double2 *z, double2 *part,
-                                                    unsigned *counter, PcgScal *s)
+                                                    double2 *z, PcgPart P, int k)
 {
     __shared__ double red[8];
-    if (s->done != 0.0) return;
-    const double alpha = s->alpha;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *rzk = (k & 1) ? P.rz1 : P.rz0;
+    const double *rrk = (k & 1) ? P.rr1 : P.rr0;
+    double *rzn = (k & 1) ? P.rz0 : P.rz1, *rrn = (k & 1) ? P.rr0 : P.rr1;
+    double s[4];
+    const double *ps[4] = {rzk, P.pap, rrk, P.bb};
+    part_sums<4>(ps, P.nb, s);
+    if (!(s[2] > P.rtol2 * s[3])) return;    // the spmv of this step stopped as well
+    if (k > 0 && !(s[0] != 0.0 && isfinite(s[0]))) return;
+    const double alpha = s[0] / s[1];
+    if (!(s[1] > 0.0) || !isfinite(alpha)) {  // breakdown: p.Ap <= 0 -> report r = 0, stop
+        if (threadIdx.x == 0) { rzn[blockIdx.x] = 0.0; rrn[blockIdx.x] = 0.0; }
+        return;
+    }
     double rz = 0.0, rr = 0.0;
-    if (i < nn) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         const double2 pi = p[i], api = Ap[i];
         double2 xi = x[i], ri = r[i];
         xi.x += alpha * pi.x; xi.y += alpha * pi.y;
@@ -246,22 +287,11 @@ __global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, 
         const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
         const double2 zi = make_double2(ri.x * dinv, ri.y * dinv);
         z[i] = zi;
-        rz = ri.x * zi.x + ri.y * zi.y;
-        rr = ri.x * ri.x + ri.y * ri.y;
+        rz += ri.x * zi.x + ri.y * zi.y;
+        rr += ri.x * ri.x + ri.y * ri.y;
     }
     block_reduce2(rz, rr, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rz, rr);
-    if (!last_block(counter)) return;
-    if (threadIdx.x == 0) {
-        double a = 0.0, c = 0.0;
-        for (unsigned k = 0; k < gridDim.x; ++k) { a += part[k].x; c += part[k].y; }
-        s->beta = a / s->rz;
-        s->rz = a;
-        s->rr = c;
-        s->iters += 1.0;
-        if (sqrt(c) <= s->rtol * s->bn) s->done = 1.0;
-        *counter = 0;
-    }
+    if (threadIdx.x == 0) { rzn[blockIdx.x] = rz; rrn[blockIdx.x] = rr; }
 }
 
 hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st)
@@ -287,60 +317,64 @@ hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, co
     return hipGetLastError();
 }
 
-size_t fem_n2e_bytes(int entries) { return sizeof(N2E) * (size_t)entries; }
+int pcg_blocks(int nn)
+{
+    int nb = (nn + kPcgThreads - 1) / kPcgThreads;
+    return nb < 1 ? 1 : (nb > kPcgMaxBlocks ? kPcgMaxBlocks : nb);
+}
 
-// work[0..5]: r, z, pA, pB, Ap, part (double2 each) ; work[6]: scalars + counter
+// work[0..4]: r, z, pA, pB, Ap (double2 each); work[5]: partials + status.
+// poll > 0: check convergence on the host every `poll` iterations (synchronises);
+// poll == 0: enqueue exactly maxit iterations (kernels after convergence return at
+// once), no host synchronisation -- usable inside stream capture.
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
-                   DBuf *work, double *stats_dev, hipStream_t st)
+                   DBuf *work, double *stats_host, hipStream_t st, int poll)
 {
     const int nn = m.nn;
-    const int nb = (nn + 255) / 256;
+    const int nb = pcg_blocks(nn);
     const size_t vb = sizeof(double2) * (size_t)nn;
     hipError_t e;
     for (int k = 0; k < 5; ++k)
         if ((e = work[k].ensure(vb)) != hipSuccess) return e;
-    if ((e = work[5].ensure(sizeof(double2) * (size_t)nb)) != hipSuccess) return e;
-    if ((e = work[6].ensure(sizeof(PcgScal) + 64)) != hipSuccess) return e;
-    PcgScal *s = work[6].as<PcgScal>();
-    unsigned *counter = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(s) + sizeof(PcgScal));
+    if ((e = work[5].ensure(sizeof(double) * (6 * (size_t)nb + 8))) != hipSuccess) return e;
+    double *pp = work[5].as<double>();
+    PcgPart P;
+    P.pap = pp; P.rz0 = pp + nb; P.rz1 = pp + 2 * nb; P.rr0 = pp + 3 * nb; P.rr1 = pp + 4 * nb;
+    P.bb = pp + 5 * nb; P.stat = pp + 6 * nb; P.nb = nb; P.rtol2 = rtol * rtol;
     double2 *r = work[0].as<double2>(), *z = work[1].as<double2>(), *pA = work[2].as<double2>(),
-            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>(),
-            *part = work[5].as<double2>();
-    if ((e = hipMemsetAsync(counter, 0, sizeof(unsigned), st)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(&s->rtol, &rtol, sizeof(double), hipMemcpyHostToDevice, st)) !=
-        hipSuccess)
-        return e;
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, st, nn, (const double2 *)b,
-                       m.diag.as<double>(), (double2 *)x, r, z, pA, part, counter, s);
+            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>();
+    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
+                       m.diag.as<double>(), (double2 *)x, r, z, pA, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int chunk = 8;
+    const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
         for (int q = 0; q < chunk && it + q < maxit; ++q) {
             const int k = it + q;
             double2 *pold = (k & 1) ? pB : pA;
             double2 *pnew = (k & 1) ? pA : pB;
-            hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(256), 0, st, nn, m.rowptr.as<int>(),
-                               m.col.as<int>(), m.aval.as<double>(), z, pold, pnew, Ap, part,
-                               counter, s, k == 0 ? 1 : 0);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, st, nn, m.diag.as<double>(),
-                               pnew, Ap, (double2 *)x, r, z, part, counter, s);
+            hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(kPcgThreads), 0, st, nn,
+                               m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(), z, pold,
+                               pnew, Ap, P, k);
+            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kPcgThreads), 0, st, nn,
+                               m.diag.as<double>(), pnew, Ap, (double2 *)x, r, z, P, k);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (poll <= 0) break;
         double done = 0.0;
-        if ((e = hipMemcpyAsync(&done, &s->done, sizeof(double), hipMemcpyDeviceToHost, st)) !=
+        if ((e = hipMemcpyAsync(&done, P.stat, sizeof(double), hipMemcpyDeviceToHost, st)) !=
             hipSuccess)
             return e;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
         if (done != 0.0) break;
     }
-    if (stats_dev) {
-        // {iterations, relative residual}
-        PcgScal hs;
-        if ((e = hipMemcpyAsync(&hs, s, sizeof(PcgScal), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    if (stats_host) {
+        // status of the last spmv launch: {done, iterations, relative residual}
+        double hs[3];
+        if ((e = hipMemcpyAsync(hs, P.stat, sizeof hs, hipMemcpyDeviceToHost, st)) != hipSuccess)
             return e;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-        stats_dev[0] = hs.iters;
-        stats_dev[1] = hs.bn > 0 ? sqrt(hs.rr) / hs.bn : 0.0;
+        stats_host[0] = hs[1];
+        stats_host[1] = hs[2];
     }
     return hipSuccess;
 }
@@ -464,8 +498,9 @@ hipError_t launch_badpoint(const int *iters, int S, int use_mean_rule, double *U
 // Spring fill of bad nodes on the Nx x Ny node grid: each bad node takes the mean of
 // its non-NaN 4-neighbours (the inpaint_nans method-4 spring stencil).  Red-black
 // sweeps, so every update reads only the other colour: deterministic, race-free.
-template <int NC>
-__global__ __launch_bounds__(256) void k_fill(int Nx, int Ny, const int *bad, double *V, int color)
+// U (2 components) and, when given, F (4 components) are filled in the same launch.
+__global__ __launch_bounds__(256) void k_fill(int Nx, int Ny, const int *bad, double *U,
+                                              double *F, int color)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nx * Ny || !bad[i]) return;
@@ -473,19 +508,21 @@ __global__ __launch_bounds__(256) void k_fill(int Nx, int Ny, const int *bad, do
     if (((x + y) & 1) != color) return;
     const int nb[4] = {x > 0 ? i - 1 : -1, x < Nx - 1 ? i + 1 : -1, y > 0 ? i - Nx : -1,
                        y < Ny - 1 ? i + Nx : -1};
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
+    const int ncomp = F ? 6 : 2;
+    for (int c = 0; c < ncomp; ++c) {
+        double *V = c < 2 ? U : F;
+        const int NC = c < 2 ? 2 : 4, cc = c < 2 ? c : c - 2;
         double s = 0.0;
         int n = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (nb[k] < 0) continue;
-            const double v = V[(size_t)NC * nb[k] + c];
+            const double v = V[(size_t)NC * nb[k] + cc];
             if (isnan(v)) continue;
             s += v;
             ++n;
         }
-        if (n > 0) V[(size_t)NC * i + c] = s / n;
+        if (n > 0) V[(size_t)NC * i + cc] = s / n;
     }
 }
 
@@ -494,10 +531,8 @@ hipError_t launch_fill(int Nx, int Ny, const int *bad, double *U, double *F, int
 {
     const int n = Nx * Ny;
     const int nb = (n + 255) / 256;
-    for (int k = 0; k < 2 * sweeps; ++k) {
-        hipLaunchKernelGGL(k_fill<2>, dim3(nb), dim3(256), 0, st, Nx, Ny, bad, U, k & 1);
-        if (F) hipLaunchKernelGGL(k_fill<4>, dim3(nb), dim3(256), 0, st, Nx, Ny, bad, F, k & 1);
-    }
+    for (int k = 0; k < 2 * sweeps; ++k)
+        hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, st, Nx, Ny, bad, U, F, k & 1);
     return hipGetLastError();
 }
 

@@ -123,6 +123,55 @@ __device__ __forceinline__ bool spd_inverse6(const double *H, double *Hi)
     return true;
 }
 
+// Same from the packed lower triangle h21 (MATLAB column-major order, as in H21),
+// writing H^-1 (row-major) to hi.  Every lane of the caller runs it with identical
+// inputs; the factor stays in registers, H and H^-1 live in LDS.
+__device__ __forceinline__ bool spd_inverse6_lds(const double *h21, double *hi)
+{
+    // column-major packed index of (r, c), r >= c
+    auto hp = [&](int r, int c) { return h21[c * 6 - c * (c - 1) / 2 + (r - c)]; };
+    double L[21];
+#define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+#pragma unroll
+        for (int c = 0; c <= r; ++c) {
+            double s = hp(r, c);
+#pragma unroll
+            for (int k = 0; k < c; ++k) s -= LI(r, k) * LI(c, k);
+            if (r == c) {
+                if (!(s > 0.0)) return false;
+                LI(r, r) = sqrt(s);
+            } else {
+                LI(r, c) = s / LI(c, c);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double y[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double s = (r == j) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < r; ++k) s -= LI(r, k) * y[k];
+            y[r] = s / LI(r, r);
+        }
+        double xcol[6];
+#pragma unroll
+        for (int r = 5; r >= 0; --r) {
+            double s = y[r];
+#pragma unroll
+            for (int k = r + 1; k < 6; ++k) s -= LI(k, r) * xcol[k];
+            xcol[r] = s / LI(r, r);
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) hi[r * 6 + j] = xcol[r];
+    }
+#undef LI
+    return true;
+}
+
 // funICGN.m:268-286: P <- P o dP^-1.  Returns false when det(dP) == 0.
 __device__ __forceinline__ bool ic_compose(double P[6], const double D[6])
 {
@@ -167,12 +216,14 @@ __device__ __forceinline__ float sample_g(const float *__restrict__ g, int nx, i
 
 // Shared body for both modes.  SIX = true: 6-DOF funICGN; false: 2-DOF funICGN_Subpb1.
 template <int PPL, int WPS, bool SIX>
-__global__ __launch_bounds__(256, 2) void k_icgn(IcgnArgs a)
+__global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
 {
     constexpr int SPB = 4 / WPS;          // subsets per block
     constexpr int T = 64 * WPS;           // threads per subset
+    constexpr int NCS = 2 + 6 + 6 + (SIX ? 21 : 0);
     __shared__ double s_hinv[SPB][36];
     __shared__ double s_cst[SPB][12];
+    __shared__ double s_cs[SPB][NCS];
     __shared__ float s_red[16 * WPS];
     __shared__ double s_redd[WPS];
 
@@ -219,128 +270,99 @@ __global__ __launch_bounds__(256, 2) void k_icgn(IcgnArgs a)
         a.iters[j] = status;
     };
 
-    // ---- window and mask rule (funICGN.m:53-65) -----------------------------------
+    // ---- window, mask rule and per-subset constants (funICGN.m:34-94) -------------
+    // One unrolled pass loads the window into registers and accumulates, per lane in
+    // FP32 (<= PPL terms), the exact zeros of DfDx, sum f, sum f^2, sum J_k,
+    // sum J_k f and (6-DOF) the 21 Hessian entries; lanes are combined in FP64.
+    // Pixel p = tid + k*T sits at column i, row r (p = r*w1 + i); the walk over k
+    // advances (i, r) by T without divisions.
     int xa, ya;
     if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
-    {
-        const int n = (w + 1) * (w + 1);
-        float cnt = 0.f;
-        for (int p = tid; p < n; p += T) {
-            const int i = p % (w + 1), jj = p / (w + 1);
-            cnt += a.gx[(size_t)(ya - 1 + jj) * nx + (xa - 1 + i)] == 0.f ? 1.f : 0.f;
-        }
-        const float n0 = block_sum1<WPS>(cnt, s_red, wid);
-        if (!((double)n0 < 0.5 * n)) { finish(102, true, NAN); return; }
-        if ((double)n0 > 0.1 * n) {
-            w = (int)floor(sqrt(2.0) * w + 0.5);  // MATLAB round (funICGN.m:58)
-            if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
-        }
-    }
-    const int w1 = w + 1, n = w1 * w1;
-    if (n > PPL * T) { if (tid == 0) a.iters[j] = -2; return; }
-
-    // ---- reference window ------------------------------------------------------
-    // Set-up sums read the window straight from memory (L2-resident) in a rolled
-    // loop, so only the per-pixel arrays used by the iterations occupy registers.
-    auto px_at = [&](int p, float &fval, float &gxv, float &gyv, double &dx, double &dy) {
-        const int i = p % w1, r = p / w1;
-        const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
-        fval = a.f[o];
-        gxv = a.gx[o];
-        gyv = a.gy[o];
-        dx = (double)i - 0.5 * w;
-        dy = (double)r - 0.5 * w;
-    };
-    // mean and norm of f (funICGN.m:93-94), FP64
-    double sf = 0.0;
-#pragma unroll 1
-    for (int p = tid; p < n; p += T) sf += (double)a.f[(size_t)(ya - 1 + p / w1) * nx + (xa - 1 + p % w1)];
-    sf = block_sumd<WPS>(sf, s_redd, wid);
-    const double meanf = sf / n;
-    double ssf = 0.0;
-#pragma unroll 1
-    for (int p = tid; p < n; p += T) {
-        const double d = (double)a.f[(size_t)(ya - 1 + p / w1) * nx + (xa - 1 + p % w1)] - meanf;
-        ssf += d * d;
-    }
-    ssf = block_sumd<WPS>(ssf, s_redd, wid);
-    const double bottomf = sqrt(ssf);
-
-    // Hessian (funICGN.m:67-79) and the constant sums sum J_k, sum J_k f_n (FP64).
-    double *cst = s_cst[sub_in_block];  // [0..5] sum J_k, [6..11] sum J_k f_n
-    double *hinv = s_hinv[sub_in_block];
-    bool singular = false;
-    {
-        double h[21];
-        double sJ[6], sJf[6];
-#pragma unroll
-        for (int q = 0; q < 21; ++q) h[q] = 0.0;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) sJ[q] = sJf[q] = 0.0;
-#pragma unroll 1
-        for (int p = tid; p < n; p += T) {
-            float fval, gxv, gyv;
-            double dx, dy;
-            px_at(p, fval, gxv, gyv, dx, dy);
-            const double fx = gxv, fy = gyv, fn = ((double)fval - meanf) / bottomf;
-            const double J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
-            if (SIX) {
-                int q = 0;
-#pragma unroll
-                for (int c = 0; c < 6; ++c)
-#pragma unroll
-                    for (int rr = c; rr < 6; ++rr) h[q++] += J[rr] * J[c];
-            }
-#pragma unroll
-            for (int q = 0; q < 6; ++q) { sJ[q] += J[q]; sJf[q] += J[q] * fn; }
-        }
-        if (SIX) {
-#pragma unroll
-            for (int q = 0; q < 21; ++q) h[q] = block_sumd<WPS>(h[q], s_redd, wid);
-            if (tid == 0)
-                for (int q = 0; q < 21; ++q) a.H21[21 * (size_t)j + q] = h[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const double v1 = block_sumd<WPS>(sJ[q], s_redd, wid);
-            const double v2 = block_sumd<WPS>(sJf[q], s_redd, wid);
-            cst[q] = v1;       // every lane writes the same value and reads back its own
-            cst[6 + q] = v2;
-        }
-        if (SIX) {
-            double H[36], Hi[36];
-            int q = 0;
-            for (int c = 0; c < 6; ++c)
-                for (int r = c; r < 6; ++r) { H[r * 6 + c] = h[q]; H[c * 6 + r] = h[q]; ++q; }
-            singular = !spd_inverse6(H, Hi);
-            if (!singular)
-                for (int e = 0; e < 36; ++e) hinv[e] = Hi[e];  // same value from every lane
-        }
-    }
-
-    // Per-pixel registers for the iterations: pixel p = tid + k*T sits at column i, row
-    // r (p = r*w1 + i); the walk over k advances (i, r) by T without divisions.
-    const int di = T % w1, dr = T / w1;
-    const int i0 = tid % w1, r0 = tid / w1;
+    constexpr int NH = SIX ? 21 : 0;
+    constexpr int NC = 2 + 6 + 6 + NH;   // sum f, sum f^2, sum J, sum J f, H
     float fv[PPL], fxv[PPL], fyv[PPL];
-    {
+    double *cs = s_cs[sub_in_block];  // reduced sums, same value from every lane
+    int w1 = w + 1, n = w1 * w1;
+    int di = 0, dr = 0, i0 = 0, r0 = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (n > PPL * T) { if (tid == 0) a.iters[j] = -2; return; }
+        di = T % w1; dr = T / w1; i0 = tid % w1; r0 = tid / w1;
+        float acc[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q] = 0.f;
+        float zeros = 0.f;
+        const float half = 0.5f * (float)w;
         int i = i0, r = r0;
-        const float mf = (float)meanf;
-        const double ib = 1.0 / bottomf;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
             if (r < w1) {
                 const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
-                fv[k] = (float)(((double)a.f[o] - meanf) * ib);  // f_n
-                fxv[k] = a.gx[o];
-                fyv[k] = a.gy[o];
+                const float fval = a.f[o], fx = a.gx[o], fy = a.gy[o];
+                fv[k] = fval; fxv[k] = fx; fyv[k] = fy;
+                zeros += fx == 0.f ? 1.f : 0.f;
+                const float dx = (float)i - half, dy = (float)r - half;
+                const float J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
+                acc[0] += fval;
+                acc[1] += fval * fval;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) { acc[2 + q] += J[q]; acc[8 + q] += J[q] * fval; }
+                if (SIX) {
+                    int q = 14;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+#pragma unroll
+                        for (int rr = c; rr < 6; ++rr) acc[q++] += J[rr] * J[c];
+                }
             } else {
                 fv[k] = fxv[k] = fyv[k] = 0.f;
             }
             i += di; r += dr;
             if (i >= w1) { i -= w1; ++r; }
         }
-        (void)mf;
+        if (pass == 0) {  // mask rule on the original window
+            const float n0 = block_sum1<WPS>(zeros, s_red, wid);
+            if (!((double)n0 < 0.5 * n)) { finish(102, true, NAN); return; }
+            if ((double)n0 > 0.1 * n) {
+                w = (int)floor(sqrt(2.0) * w + 0.5);  // MATLAB round (funICGN.m:58)
+                if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
+                w1 = w + 1;
+                n = w1 * w1;
+                continue;  // reload the enlarged window
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) cs[q] = block_sumd<WPS>((double)acc[q], s_redd, wid);
+        break;
+    }
+    const double meanf = cs[0] / n;
+    double ssf = cs[1] - cs[0] * meanf;  // sum (f - meanf)^2
+    ssf = ssf > 0 ? ssf : 0.0;
+    const double bottomf = sqrt(ssf);
+    double *cst = s_cst[sub_in_block];  // [0..5] sum J_k, [6..11] sum J_k f_n
+    double *hinv = s_hinv[sub_in_block];
+    bool singular = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        cst[q] = cs[2 + q];  // every lane writes the same value and reads back its own
+        cst[6 + q] = (cs[8 + q] - meanf * cs[2 + q]) / bottomf;
+    }
+    if (SIX) {
+        if (tid == 0)
+            for (int e = 0; e < 21; ++e) a.H21[21 * (size_t)j + e] = cs[14 + e];
+        singular = !spd_inverse6_lds(cs + 14, hinv);
+    }
+    {
+        const float mf = (float)meanf;
+        const float ib = (float)(1.0 / bottomf);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) fv[k] = (fv[k] - mf) * ib;  // f_n (0 in empty slots)
+        int i = i0, r = r0;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            if (!(r < w1)) fv[k] = 0.f;
+            i += di; r += dr;
+            if (i >= w1) { i -= w1; ++r; }
+        }
     }
 
     // 2-DOF system matrix

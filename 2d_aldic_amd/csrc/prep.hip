@@ -26,21 +26,21 @@ __device__ __forceinline__ double load_raw(const void *p, int dtype, size_t i)
     }
 }
 
-// acc[0..1] = sum, sum of squares of image 0 over the ROI; acc[2..3] image 1.  For
-// integer inputs every partial sum is an integer below 2^53, so the FP64 atomics
-// give the exact (order-independent) result.
+// acc[0..1] = sum, sum of squares of one image over the ROI.  For integer inputs every
+// partial sum is an integer below 2^53, so the FP64 atomics give the exact
+// (order-independent) result.  Grid: (x blocks, rows); no divisions.
 __global__ __launch_bounds__(256) void k_stats(const void *raw, int dtype, int nx, int x1,
                                                int x2, int y1, int y2, double *acc)
 {
-    const int w = x2 - x1 + 1;
-    const long n = (long)w * (y2 - y1 + 1);
     double s = 0.0, s2 = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        const int x = x1 - 1 + (int)(i % w), y = y1 - 1 + (int)(i / w);
-        const double v = load_raw(raw, dtype, (size_t)y * nx + x);
-        s += v;
-        s2 += v * v;
+    for (int y = y1 - 1 + blockIdx.y; y < y2; y += gridDim.y) {
+        const size_t row = (size_t)y * nx;
+        for (int x = x1 - 1 + blockIdx.x * blockDim.x + threadIdx.x; x < x2;
+             x += gridDim.x * blockDim.x) {
+            const double v = load_raw(raw, dtype, row + x);
+            s += v;
+            s2 += v * v;
+        }
     }
     s = wave_sum(s);
     s2 = wave_sum(s2);
@@ -121,10 +121,11 @@ __global__ __launch_bounds__(256) void k_gradient(const float *f, int nx, int ny
 hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
                         hipStream_t st)
 {
-    const long n = (long)(roi[1] - roi[0] + 1) * (roi[3] - roi[2] + 1);
-    int blocks = (int)((n + 255) / 256);
-    if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL(k_stats, dim3(blocks), dim3(256), 0, st, raw, dtype, nx, roi[0], roi[1],
+    const int w = roi[1] - roi[0] + 1, h = roi[3] - roi[2] + 1;
+    int bx = (w + 255) / 256;
+    if (bx > 8) bx = 8;
+    int by = h < 512 ? h : 512;
+    hipLaunchKernelGGL(k_stats, dim3(bx, by), dim3(256), 0, st, raw, dtype, nx, roi[0], roi[1],
                        roi[2], roi[3], acc);
     return hipGetLastError();
 }

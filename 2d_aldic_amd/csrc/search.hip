@@ -35,11 +35,23 @@ static SearchGeom search_geom(int ws, int sx, int sy)
     G.nstrip = (G.nrx + 3) / 4;
     G.DW = G.nstrip * 4 + G.tw;
     G.DH = G.tw + G.nry - 1;
+    // Split the template rows into K bands so the 256 threads stay busy: pick the K
+    // with the fewest (rounds of 256 items) x (rows per band + 2), keeping the band
+    // partials within a 64 KB LDS budget when possible.
     const int base = G.nry * G.nstrip;
-    int K = (256 + base - 1) / base;
-    if (K < 1) K = 1;
-    if (K > G.tw) K = G.tw;
-    G.rows_per = (G.tw + K - 1) / K;
+    const int fixed = G.tw * G.tw + (G.tw + G.nry - 1) * G.DW + 2 * G.nry * G.DW;
+    int bestK = 1;
+    long best = -1;
+    for (int K = 1; K <= G.tw; ++K) {
+        const int rows = (G.tw + K - 1) / K;
+        const int Keff = (G.tw + rows - 1) / rows;
+        const long lds = 4L * (fixed + (long)Keff * G.nrx * G.nry);
+        if (K > 1 && lds > 64 * 1024) break;
+        const long rounds = (base * (long)Keff + 255) / 256;
+        const long cost = rounds * (rows + 2);
+        if (best < 0 || cost < best) { best = cost; bestK = K; }
+    }
+    G.rows_per = (G.tw + bestK - 1) / bestK;
     G.K = (G.tw + G.rows_per - 1) / G.rows_per;
     G.off_T = 0;
     G.off_D = G.off_T + G.tw * G.tw;

@@ -794,6 +794,7 @@ int orc_pcg(int n, const int *rowptr, const int *col, const double *val, const d
             for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s += val[k] * p[col[k]];
             Ap[i] = s; pAp += p[i] * s;
         }
+        if (!(pAp > 0) || rz == 0) break; /* breakdown (converged to round-off) */
         double a = rz / pAp, rn = 0;
         for (int i = 0; i < n; ++i) { x[i] += a * p[i]; r[i] -= a * Ap[i]; rn += r[i] * r[i]; }
         if (sqrt(rn) <= rtol * bn) break;
