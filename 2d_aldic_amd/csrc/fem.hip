@@ -129,8 +129,9 @@ __global__ __launch_bounds__(256) void k_rhs(int nn, const int *rowptr, const in
 // scalars).  Kernel boundaries make the partials visible across the per-XCD L2s, so
 // no fence or atomic is needed and the solve is deterministic.
 //   init      : x = 0, r = b, z = D^-1 r, p = z; partials r.z -> rz[0], r.r -> rr[0], b.b
-//   spmv(k)   : beta_k = rz[k]/rz[k-1]; stop when |r| <= rtol |b|; p = z + beta p (on
-//               the fly for the neighbours), Ap, partials p.Ap
+//   spmv(k)   : beta_k = rz[k]/rz[k-1]; stop when |r| <= rtol |b| (the stop is sticky:
+//               block 0 records it and every later kernel returns at once); p = z + beta
+//               p (on the fly for the neighbours), Ap, partials p.Ap
 //   update(k) : alpha = rz[k]/p.Ap; x += alpha p; r -= alpha Ap; z = D^-1 r;
 //               partials r.z -> rz[k+1], r.r -> rr[k+1]
 // Block 0 of each spmv records {done, iterations} for the host's convergence poll.
@@ -214,6 +215,8 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, con
                                                   double2 *Ap, PcgPart P, int k)
 {
     __shared__ double red[8];
+    // sticky stop: once a step has finished the solve, later partials are stale
+    if (P.stat[0] != 0.0) return;
     const double *rzk = (k & 1) ? P.rz1 : P.rz0, *rzm = (k & 1) ? P.rz0 : P.rz1;
     const double *rrk = (k & 1) ? P.rr1 : P.rr0;
     double s[4];
@@ -264,13 +267,11 @@ __global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, 
 {
     __shared__ double red[8];
     const double *rzk = (k & 1) ? P.rz1 : P.rz0;
-    const double *rrk = (k & 1) ? P.rr1 : P.rr0;
     double *rzn = (k & 1) ? P.rz0 : P.rz1, *rrn = (k & 1) ? P.rr0 : P.rr1;
-    double s[4];
-    const double *ps[4] = {rzk, P.pap, rrk, P.bb};
-    part_sums<4>(ps, P.nb, s);
-    if (!(s[2] > P.rtol2 * s[3])) return;    // the spmv of this step stopped as well
-    if (k > 0 && !(s[0] != 0.0 && isfinite(s[0]))) return;
+    if (P.stat[0] != 0.0) return;            // the spmv of this step finished the solve
+    double s[2];
+    const double *ps[2] = {rzk, P.pap};
+    part_sums<2>(ps, P.nb, s);
     const double alpha = s[0] / s[1];
     if (!(s[1] > 0.0) || !isfinite(alpha)) {  // breakdown: p.Ap <= 0 -> report r = 0, stop
         if (threadIdx.x == 0) { rzn[blockIdx.x] = 0.0; rrn[blockIdx.x] = 0.0; }
