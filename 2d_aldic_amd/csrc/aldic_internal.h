@@ -53,6 +53,12 @@ struct MeshDev {
     DBuf aval, mval, diag;    // double[nnz] (beta K + mu M (+alpha mu K)), double[nnz] M, double[nn]
     DBuf gvec;                // double2[nn]: sum over elements of int grad N_i dA
     double beta = -1, mu = -1, alpha = -1;  // values aval was assembled for
+    // Structured grid (nodes x-fastest on a uniform gx x gy lattice, standard Q4
+    // elements): the mass matrix is M1x (x) M1y, used as the CG preconditioner.
+    bool structured = false;
+    int gx = 0, gy = 0;
+    double hx = 0, hy = 0;
+    DBuf thomas;              // double[2*gx + 2*gy]: Thomas factors of M1x, then M1y
 };
 
 struct FrameState {
@@ -90,6 +96,7 @@ struct aldic_ctx {
 namespace aldic {
 
 // ---- kernels' host launchers (all enqueue on ctx->stream) ---------------------
+constexpr int kStatParts = 256;  // per-block partials written by launch_stats
 hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
                         hipStream_t st);
 hipError_t launch_prep(const void *rawf, const void *rawg, int dtype, int nx, int ny,
@@ -143,9 +150,8 @@ hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F
 
 // ADMM helpers
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);
-hipError_t launch_badpoint(const int *iters, int S, int use_mean_rule, double *U, double *F,
-                           int *bad, double *tmp, hipStream_t st);
-hipError_t launch_fill(int Nx, int Ny, const int *bad, double *U, double *F, int sweeps,
-                       hipStream_t st);
+hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
+                          const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
+                          int sweeps, hipStream_t st);
 
 }  // namespace aldic

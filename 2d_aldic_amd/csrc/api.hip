@@ -72,12 +72,12 @@ hipError_t run_prep(aldic_ctx *c)
     const size_t np = (size_t)c->nx * c->ny;
     hipError_t e;
     if ((e = c->f.ensure(4 * np)) || (e = c->g.ensure(4 * np)) || (e = c->gx.ensure(4 * np)) ||
-        (e = c->gy.ensure(4 * np)) || (e = c->stats.ensure(8 * sizeof(double))))
+        (e = c->gy.ensure(4 * np)) || (e = c->stats.ensure(4 * kStatParts * sizeof(double))))
         return e;
-    if ((e = hipMemsetAsync(c->stats.p, 0, 4 * sizeof(double), c->stream))) return e;
-    double *acc = c->stats.as<double>();
+    double *acc = c->stats.as<double>();  // partials of f, then of g
     if ((e = launch_stats(c->rawf.p, c->raw_dtype, c->nx, c->ny, c->roi, acc, c->stream))) return e;
-    if ((e = launch_stats(c->rawg.p, c->raw_dtype, c->nx, c->ny, c->roi, acc + 2, c->stream)))
+    if ((e = launch_stats(c->rawg.p, c->raw_dtype, c->nx, c->ny, c->roi, acc + 2 * kStatParts,
+                          c->stream)))
         return e;
     const double nroi = (double)(c->roi[1] - c->roi[0] + 1) * (c->roi[3] - c->roi[2] + 1);
     return launch_prep(c->rawf.p, c->rawg.p, c->raw_dtype, c->nx, c->ny,
@@ -210,7 +210,7 @@ void aldic_destroy(aldic_ctx *ctx)
     DBuf *bufs[] = {&ctx->f, &ctx->g, &ctx->gx, &ctx->gy, &ctx->rawf, &ctx->rawg, &ctx->mask,
                     &ctx->stats, &ctx->mesh.coords, &ctx->mesh.elems, &ctx->mesh.n2e_ptr,
                     &ctx->mesh.n2e, &ctx->mesh.rowptr, &ctx->mesh.col, &ctx->mesh.aval,
-                    &ctx->mesh.mval, &ctx->mesh.diag, &ctx->mesh.gvec};
+                    &ctx->mesh.mval, &ctx->mesh.diag, &ctx->mesh.gvec, &ctx->mesh.thomas};
     for (DBuf *b : bufs) b->release();
     for (auto &b : ctx->scratch) b.release();
     FrameState &F = ctx->fr;
@@ -520,6 +520,50 @@ int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elem
     CK(m.gvec.ensure(sizeof(double) * 2 * nn));
     CK(hipStreamSynchronize(ctx->stream));
     m.beta = m.mu = m.alpha = -1;
+    // structured lattice? (x-fastest uniform grid with the standard Q4 connectivity)
+    m.structured = false;
+    {
+        int gx = 1;
+        while (gx < nn && coords[2 * gx + 1] == coords[1] && coords[2 * gx] > coords[2 * (gx - 1)]) ++gx;
+        const int gy = gx > 0 ? nn / gx : 0;
+        bool ok = gx >= 2 && gy >= 2 && gx * gy == nn && E == (gx - 1) * (gy - 1);
+        const double hx = ok ? coords[2] - coords[0] : 0, hy = ok ? coords[2 * gx + 1] - coords[1] : 0;
+        ok = ok && hx > 0 && hy > 0;
+        for (int i = 0; ok && i < nn; ++i) {
+            const double ex = coords[0] + hx * (i % gx), ey = coords[1] + hy * (i / gx);
+            ok = std::fabs(coords[2 * i] - ex) <= 1e-9 * std::fabs(hx) * gx &&
+                 std::fabs(coords[2 * i + 1] - ey) <= 1e-9 * std::fabs(hy) * gy;
+        }
+        for (int e = 0; ok && e < E; ++e) {
+            const int l = e / (gx - 1), k = e % (gx - 1), n0 = l * gx + k;
+            ok = elems[4 * e] == n0 && elems[4 * e + 1] == n0 + 1 &&
+                 elems[4 * e + 2] == n0 + gx + 1 && elems[4 * e + 3] == n0 + gx;
+        }
+        if (ok) {
+            // Thomas factors of h/6 tridiag(1, 4, 1) with end diagonals 2 (1-D mass)
+            std::vector<double> th(2 * (size_t)gx + 2 * (size_t)gy);
+            auto factor = [](int n, double h, double *c, double *inv) {
+                const double e = h / 6.0;
+                double cprev = 0.0;
+                for (int i = 0; i < n; ++i) {
+                    const double d = (i == 0 || i == n - 1) ? 2.0 * h / 6.0 : 4.0 * h / 6.0;
+                    const double den = d - (i > 0 ? e * cprev : 0.0);
+                    inv[i] = 1.0 / den;
+                    c[i] = e / den;
+                    cprev = c[i];
+                }
+            };
+            factor(gx, hx, th.data(), th.data() + gx);
+            factor(gy, hy, th.data() + 2 * gx, th.data() + 2 * gx + gy);
+            CK(up(m.thomas, th.data(), th.size(), ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+            m.structured = true;
+            m.gx = gx;
+            m.gy = gy;
+            m.hx = hx;
+            m.hy = hy;
+        }
+    }
     ctx->have_mesh = true;
     return ALDIC_OK;
 }
@@ -545,7 +589,7 @@ int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const dou
     if (m.beta != beta || m.mu != mu || m.alpha != alpha) CK(fem_assemble(m, beta, mu, alpha, ctx->stream));
     CK(fem_rhs(m, beta, mu, sc[0].as<double>(), sc[1].as<double>(), sc[2].as<double>(),
                sc[3].as<double>(), sc[4].as<double>(), ctx->stream));
-    static thread_local DBuf work[7];
+    static thread_local DBuf work[8];
     double st[2] = {0, 0};
     CK(fem_pcg(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream, 8));
     CK(down(Uhat, sc[5].p, 2 * nn, ctx->stream));
@@ -667,7 +711,7 @@ int aldic_frame_run(aldic_ctx *ctx)
     hipStream_t st = ctx->stream;
     const aldic_frame_opts &o = F.o;
     const int S = F.S;
-    static thread_local DBuf work[7];
+    static thread_local DBuf work[8];
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t *ev = ctx->ev;
 
@@ -728,11 +772,8 @@ int aldic_frame_run(aldic_ctx *ctx)
     CK(launch_icgn6(a, ppl, st));
     CK(hipEventRecord(ev[3], st));
     // LocalICGN post-processing: bad-point rule and refill
-    CK(hipMemcpyAsync(F.U1.p, F.U6.p, sizeof(double) * 2 * S, hipMemcpyDeviceToDevice, st));
-    CK(hipMemcpyAsync(F.F1.p, F.F6.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
-    CK(launch_badpoint(F.it6.as<int>(), S, 1, F.U1.as<double>(), F.F1.as<double>(),
-                       F.bad.as<int>(), nullptr, st));
-    CK(launch_fill(F.Nx, F.Ny, F.bad.as<int>(), F.U1.as<double>(), F.F1.as<double>(), 12, st));
+    CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, F.U6.as<double>(), F.F6.as<double>(),
+                      F.U1.as<double>(), F.F1.as<double>(), F.bad.as<int>(), 12, st));
     CK(hipEventRecord(ev[4], st));
 
     // step 1 global step with zero duals
@@ -777,9 +818,8 @@ int aldic_frame_run(aldic_ctx *ctx)
         a2.H21 = nullptr;
         a2.zncc = nullptr;
         CK(launch_icgn2(a2, ppl, st));
-        CK(launch_badpoint(F.it2.as<int>(), S, 0, F.U1.as<double>(), nullptr, F.bad.as<int>(),
-                           nullptr, st));
-        CK(launch_fill(F.Nx, F.Ny, F.bad.as<int>(), F.U1.as<double>(), nullptr, 12, st));
+        CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
+                          nullptr, F.bad.as<int>(), 12, st));
         // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         CK(hipMemcpyAsync(F.F1.p, F.F2.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
         CK(hipEventRecord(ev[9], st));

@@ -16,6 +16,8 @@
 // node's own F, Subpb2.m:88-94,149).  The solve is Jacobi-preconditioned CG in FP64
 // on interleaved (u, v) vectors, two kernels per iteration; scalars stay on the
 // device and the dot products finish in the last block (fixed order).
+#include <algorithm>
+
 #include "aldic_internal.h"
 #include "common.h"
 
@@ -137,7 +139,8 @@ __global__ __launch_bounds__(256) void k_rhs(int nn, const int *rowptr, const in
 // Block 0 of each spmv records {done, iterations} for the host's convergence poll.
 struct PcgPart {
     double *pap, *rz0, *rz1, *rr0, *rr1, *bb;
-    int nb;
+    int nb;        // partials of pap and bb (node-parallel kernels)
+    int nrz, nrr;  // partials of rz and rr (equal to nb on the Jacobi path)
     double rtol2;
     double *stat;  // [0] done, [1] iterations, [2] relative residual
 };
@@ -154,14 +157,15 @@ __device__ __forceinline__ double part_sum(const double *p, int nb)
 }
 
 template <int N>
-__device__ __forceinline__ void part_sums(const double *const (&ps)[N], int nb, double (&out)[N])
+__device__ __forceinline__ void part_sums(const double *const (&ps)[N], const int (&cnt)[N],
+                                          double (&out)[N])
 {
     __shared__ double sh[4];
     const int w = threadIdx.x >> 6;
 #pragma unroll
     for (int base = 0; base < N; base += 4) {
         if (base + w < N) {
-            const double v = part_sum(ps[base + w], nb);
+            const double v = part_sum(ps[base + w], cnt[base + w]);
             if ((threadIdx.x & 63) == 0) sh[w] = v;
         }
         __syncthreads();
@@ -220,8 +224,9 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, con
     const double *rzk = (k & 1) ? P.rz1 : P.rz0, *rzm = (k & 1) ? P.rz0 : P.rz1;
     const double *rrk = (k & 1) ? P.rr1 : P.rr0;
     double s[4];
-    const double *ps[4] = {rzk, rzm, rrk, P.bb};
-    part_sums<4>(ps, P.nb, s);
+    const double *ps[4] = {rzk, rzm, k == 0 ? P.bb : rrk, P.bb};
+    const int cnt[4] = {P.nrz, P.nrz, k == 0 ? P.nb : P.nrr, P.nb};
+    part_sums<4>(ps, cnt, s);
     const double rr = s[2], bb = s[3];
     // stop at the tolerance, on b == 0, on a NaN residual, or on a breakdown
     // (r.z of the previous step zero, or a non-finite beta)
@@ -271,7 +276,8 @@ __global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, 
     if (P.stat[0] != 0.0) return;            // the spmv of this step finished the solve
     double s[2];
     const double *ps[2] = {rzk, P.pap};
-    part_sums<2>(ps, P.nb, s);
+    const int cnt[2] = {P.nrz, P.nb};
+    part_sums<2>(ps, cnt, s);
     const double alpha = s[0] / s[1];
     if (!(s[1] > 0.0) || !isfinite(alpha)) {  // breakdown: p.Ap <= 0 -> report r = 0, stop
         if (threadIdx.x == 0) { rzn[blockIdx.x] = 0.0; rrn[blockIdx.x] = 0.0; }
@@ -293,6 +299,127 @@ __global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, 
     }
     block_reduce2(rz, rr, red);
     if (threadIdx.x == 0) { rzn[blockIdx.x] = rz; rrn[blockIdx.x] = rr; }
+}
+
+// ---- tensor-mass preconditioner for structured grids ------------------------------
+// On a uniform gx x gy lattice with bilinear elements the consistent mass matrix is the
+// Kronecker product M = M1x (x) M1y of the 1-D mass matrices h/6 tridiag(1, 4, 1) (end
+// diagonals 2).  P = mu M is applied exactly by tridiagonal (Thomas) solves along x,
+// then along y; P^-1 A = I + (beta/mu) M^-1 K, whose condition number is
+// 1 + 24 beta / (mu h^2) at most, so CG needs a handful of iterations where Jacobi
+// needs dozens.  Thomas factors (per line, shared by all lines): c'_i and 1/denominator_i.
+struct GridP {
+    int gx, gy;
+    const double *cx, *ix, *cy, *iy;  // Thomas factors along x and y
+    double ex, ey;                    // off-diagonals hx/6, hy/6
+    double inv_mu;
+};
+
+// Row pass: thread per (row, component).  With update = true it first applies
+// x += alpha p, r -= alpha Ap along the row (step k of CG) and emits r.r partials.
+template <bool UPDATE>
+__global__ __launch_bounds__(256) void k_pcgs_rows(GridP G, const double *p, const double *Ap,
+                                                   double *x, double *r, double *zt, PcgPart P,
+                                                   int k)
+{
+    __shared__ double red[8];
+    const double *rzk = (k & 1) ? P.rz1 : P.rz0;
+    double *rrn = (k & 1) ? P.rr0 : P.rr1;
+    double alpha = 0.0;
+    if (UPDATE) {
+        if (P.stat[0] != 0.0) return;
+        double s[2];
+        const double *ps[2] = {rzk, P.pap};
+        const int cnt[2] = {P.nrz, P.nb};
+        part_sums<2>(ps, cnt, s);
+        alpha = s[0] / s[1];
+        if (!(s[1] > 0.0) || !isfinite(alpha)) {  // breakdown: report r = 0, stop
+            if (threadIdx.x == 0) rrn[blockIdx.x] = 0.0;
+            return;
+        }
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (t < 2 * G.gy) {
+        const int row = t >> 1, c = t & 1;
+        const size_t base = (size_t)row * G.gx * 2 + c;
+        double prev = 0.0;
+        for (int i = 0; i < G.gx; ++i) {  // forward sweep (and the CG update)
+            const size_t o = base + 2 * (size_t)i;
+            double ri = r[o];
+            if (UPDATE) {
+                x[o] += alpha * p[o];
+                ri -= alpha * Ap[o];
+                r[o] = ri;
+                rr += ri * ri;
+            }
+            prev = (ri - G.ex * prev) * G.ix[i];
+            zt[o] = prev;
+        }
+        double nxt = 0.0;
+        for (int i = G.gx - 1; i >= 0; --i) {  // back substitution
+            const size_t o = base + 2 * (size_t)i;
+            nxt = zt[o] - G.cx[i] * nxt;
+            zt[o] = nxt;
+        }
+    }
+    if (UPDATE) {
+        double dummy = 0.0;
+        block_reduce2(rr, dummy, red);
+        if (threadIdx.x == 0) rrn[blockIdx.x] = rr;
+    }
+}
+
+// Column pass: thread per (column, component): z = (1/mu) M1y^-1 zt, r.z partials; with
+// init = true also p = z.
+template <bool INIT>
+__global__ __launch_bounds__(256) void k_pcgs_cols(GridP G, const double *r, const double *zt,
+                                                   double *z, double *p, PcgPart P, int k)
+{
+    __shared__ double red[8];
+    if (!INIT && P.stat[0] != 0.0) return;
+    double *rzn = INIT ? P.rz0 : ((k & 1) ? P.rz0 : P.rz1);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0;
+    if (t < 2 * G.gx) {
+        const int col = t >> 1, c = t & 1;
+        const size_t stride = (size_t)G.gx * 2;
+        const size_t base = 2 * (size_t)col + c;
+        double prev = 0.0;
+        for (int j = 0; j < G.gy; ++j) {
+            const size_t o = base + stride * j;
+            prev = (zt[o] - G.ey * prev) * G.iy[j];
+            z[o] = prev;
+        }
+        double nxt = 0.0;
+        for (int j = G.gy - 1; j >= 0; --j) {
+            const size_t o = base + stride * j;
+            nxt = z[o] - G.cy[j] * nxt;
+            const double zz = nxt * G.inv_mu;
+            z[o] = zz;
+            if (INIT) p[o] = zz;
+            rz += r[o] * zz;
+        }
+    }
+    double dummy = 0.0;
+    block_reduce2(rz, dummy, red);
+    if (threadIdx.x == 0) rzn[blockIdx.x] = rz;
+}
+
+__global__ __launch_bounds__(256) void k_pcgs_init(int nn, const double2 *b, double2 *x,
+                                                   double2 *r, PcgPart P)
+{
+    __shared__ double red[8];
+    double bb = 0.0, dummy = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        const double2 bi = b[i];
+        x[i] = make_double2(0.0, 0.0);
+        r[i] = bi;
+        bb += bi.x * bi.x + bi.y * bi.y;
+    }
+    block_reduce2(bb, dummy, red);
+    if (threadIdx.x == 0) P.bb[blockIdx.x] = bb;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
 }
 
 hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st)
@@ -324,28 +451,49 @@ int pcg_blocks(int nn)
     return nb < 1 ? 1 : (nb > kPcgMaxBlocks ? kPcgMaxBlocks : nb);
 }
 
-// work[0..4]: r, z, pA, pB, Ap (double2 each); work[5]: partials + status.
+// work[0..5]: r, z, pA, pB, Ap, zt (double2 each); work[6]: partials + status.
 // poll > 0: check convergence on the host every `poll` iterations (synchronises);
 // poll == 0: enqueue exactly maxit iterations (kernels after convergence return at
 // once), no host synchronisation -- usable inside stream capture.
+// Structured grids with mu > 0 use the tensor-mass preconditioner, others Jacobi.
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
                    DBuf *work, double *stats_host, hipStream_t st, int poll)
 {
     const int nn = m.nn;
     const int nb = pcg_blocks(nn);
+    const bool tens = m.structured && m.mu > 0.0;
+    const int nbr = tens ? (2 * m.gy + kPcgThreads - 1) / kPcgThreads : nb;
+    const int nbc = tens ? (2 * m.gx + kPcgThreads - 1) / kPcgThreads : nb;
+    const int npart = std::max(nb, std::max(nbr, nbc));
     const size_t vb = sizeof(double2) * (size_t)nn;
     hipError_t e;
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < 6; ++k)
         if ((e = work[k].ensure(vb)) != hipSuccess) return e;
-    if ((e = work[5].ensure(sizeof(double) * (6 * (size_t)nb + 8))) != hipSuccess) return e;
-    double *pp = work[5].as<double>();
+    if ((e = work[6].ensure(sizeof(double) * (6 * (size_t)npart + 8))) != hipSuccess) return e;
+    double *pp = work[6].as<double>();
     PcgPart P;
-    P.pap = pp; P.rz0 = pp + nb; P.rz1 = pp + 2 * nb; P.rr0 = pp + 3 * nb; P.rr1 = pp + 4 * nb;
-    P.bb = pp + 5 * nb; P.stat = pp + 6 * nb; P.nb = nb; P.rtol2 = rtol * rtol;
+    P.pap = pp; P.rz0 = pp + npart; P.rz1 = pp + 2 * npart; P.rr0 = pp + 3 * npart;
+    P.rr1 = pp + 4 * npart; P.bb = pp + 5 * npart; P.stat = pp + 6 * npart;
+    P.nb = nb; P.nrz = nbc; P.nrr = nbr; P.rtol2 = rtol * rtol;
     double2 *r = work[0].as<double2>(), *z = work[1].as<double2>(), *pA = work[2].as<double2>(),
-            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>();
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
-                       m.diag.as<double>(), (double2 *)x, r, z, pA, P);
+            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>(), *zt = work[5].as<double2>();
+    GridP G{};
+    if (tens) {
+        const double *th = m.thomas.as<double>();
+        G.gx = m.gx; G.gy = m.gy;
+        G.cx = th; G.ix = th + m.gx; G.cy = th + 2 * m.gx; G.iy = th + 2 * m.gx + m.gy;
+        G.ex = m.hx / 6.0; G.ey = m.hy / 6.0; G.inv_mu = 1.0 / m.mu;
+        hipLaunchKernelGGL(k_pcgs_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
+                           (double2 *)x, r, P);
+        hipLaunchKernelGGL(k_pcgs_rows<false>, dim3(nbr), dim3(kPcgThreads), 0, st, G,
+                           (const double *)nullptr, (const double *)nullptr, (double *)x,
+                           (double *)r, (double *)zt, P, 0);
+        hipLaunchKernelGGL(k_pcgs_cols<true>, dim3(nbc), dim3(kPcgThreads), 0, st, G,
+                           (const double *)r, (const double *)zt, (double *)z, (double *)pA, P, 0);
+    } else {
+        hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
+                           m.diag.as<double>(), (double2 *)x, r, z, pA, P);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
@@ -356,8 +504,17 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
             hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(kPcgThreads), 0, st, nn,
                                m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(), z, pold,
                                pnew, Ap, P, k);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kPcgThreads), 0, st, nn,
-                               m.diag.as<double>(), pnew, Ap, (double2 *)x, r, z, P, k);
+            if (tens) {
+                hipLaunchKernelGGL(k_pcgs_rows<true>, dim3(nbr), dim3(kPcgThreads), 0, st, G,
+                                   (const double *)pnew, (const double *)Ap, (double *)x,
+                                   (double *)r, (double *)zt, P, k);
+                hipLaunchKernelGGL(k_pcgs_cols<false>, dim3(nbc), dim3(kPcgThreads), 0, st, G,
+                                   (const double *)r, (const double *)zt, (double *)z,
+                                   (double *)nullptr, P, k);
+            } else {
+                hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kPcgThreads), 0, st, nn,
+                                   m.diag.as<double>(), pnew, Ap, (double2 *)x, r, z, P, k);
+            }
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
@@ -456,84 +613,91 @@ hipError_t launch_sub(const double *a, const double *b, double *out, long n, hip
     return hipGetLastError();
 }
 
-// LocalICGN.m:119-139 (use_mean_rule = 1) or Subpb1.m:167-182 (0): mark bad subsets
-// (status < 0, > 99, and with the mean rule status > mean + 0.15 std of the good
-// ones, std with N-1) and set their U (and F) to NaN.  One block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_badpoint(const int *it, int S, int mean_rule,
-                                                   double *U, double *F, int *bad)
-{
-    __shared__ double red[3][16];
-    double c = 0, s = 0, s2 = 0;
-    for (int i = threadIdx.x; i < S; i += blockDim.x) {
-        const int v = it[i];
-        if (v >= 0 && v <= 99) { c += 1; s += v; s2 += (double)v * v; }
-    }
-    c = wave_sum(c); s = wave_sum(s); s2 = wave_sum(s2);
-    const int wid = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][wid] = c; red[1][wid] = s; red[2][wid] = s2; }
-    __syncthreads();
-    double C = 0, Sm = 0, S2 = 0;
-    for (int k = 0; k < 16; ++k) { C += red[0][k]; Sm += red[1][k]; S2 += red[2][k]; }
-    const double mean = C > 0 ? Sm / C : 0.0;
-    const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
-    const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
-    for (int i = threadIdx.x; i < S; i += blockDim.x) {
-        const int v = it[i];
-        const bool b = v < 0 || v > 99 || (mean_rule && C > 0 && (double)v > thr);
-        bad[i] = b ? 1 : 0;
-        if (b) {
-            U[2 * i] = NAN; U[2 * i + 1] = NAN;
-            if (F) { F[4 * i] = NAN; F[4 * i + 1] = NAN; F[4 * i + 2] = NAN; F[4 * i + 3] = NAN; }
-        }
-    }
-}
+// Bad-point rule and refill in one single-workgroup launch.
+//   LocalICGN.m:119-139 (mean_rule = 1) / Subpb1.m:167-182 (mean_rule = 0): a subset is
+//   bad when its status is < 0 or > 99, and with the mean rule also when it exceeds
+//   mean + 0.15 std of the good ones (std with N-1); bad U (and F) become NaN.
+//   Refill: every bad node takes the mean of its non-NaN 4-neighbours on the Nx x Ny
+//   node grid (the inpaint_nans method-4 spring stencil), `sweeps` red-black sweeps.
+// One workgroup of 1024 threads does everything (statistics, marking, sweeps separated
+// by workgroup barriers), so the step costs one launch; red-black order makes it
+// deterministic.  When Usrc (Fsrc) is given, it is first copied to U (F).
+constexpr int kFillThreads = 1024;
 
-hipError_t launch_badpoint(const int *iters, int S, int use_mean_rule, double *U, double *F,
-                           int *bad, double *tmp, hipStream_t st)
+__device__ __forceinline__ void fill_node(int i, int Nx, int Ny, double *V, int NC)
 {
-    (void)tmp;
-    hipLaunchKernelGGL(k_badpoint, dim3(1), dim3(1024), 0, st, iters, S, use_mean_rule, U, F, bad);
-    return hipGetLastError();
-}
-
-// Spring fill of bad nodes on the Nx x Ny node grid: each bad node takes the mean of
-// its non-NaN 4-neighbours (the inpaint_nans method-4 spring stencil).  Red-black
-// sweeps, so every update reads only the other colour: deterministic, race-free.
-// U (2 components) and, when given, F (4 components) are filled in the same launch.
-__global__ __launch_bounds__(256) void k_fill(int Nx, int Ny, const int *bad, double *U,
-                                              double *F, int color)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nx * Ny || !bad[i]) return;
     const int x = i % Nx, y = i / Nx;
-    if (((x + y) & 1) != color) return;
     const int nb[4] = {x > 0 ? i - 1 : -1, x < Nx - 1 ? i + 1 : -1, y > 0 ? i - Nx : -1,
                        y < Ny - 1 ? i + Nx : -1};
-    const int ncomp = F ? 6 : 2;
-    for (int c = 0; c < ncomp; ++c) {
-        double *V = c < 2 ? U : F;
-        const int NC = c < 2 ? 2 : 4, cc = c < 2 ? c : c - 2;
+    for (int c = 0; c < NC; ++c) {
         double s = 0.0;
         int n = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (nb[k] < 0) continue;
-            const double v = V[(size_t)NC * nb[k] + cc];
+            const double v = V[(size_t)NC * nb[k] + c];
             if (isnan(v)) continue;
             s += v;
             ++n;
         }
-        if (n > 0) V[(size_t)NC * i + cc] = s / n;
+        if (n > 0) V[(size_t)NC * i + c] = s / n;
     }
 }
 
-hipError_t launch_fill(int Nx, int Ny, const int *bad, double *U, double *F, int sweeps,
-                       hipStream_t st)
+__global__ __launch_bounds__(kFillThreads) void k_badfill(const int *it, int S, int Nx, int Ny,
+                                                          int mean_rule, const double *Usrc,
+                                                          const double *Fsrc, double *U,
+                                                          double *F, int *bad, int sweeps)
 {
-    const int n = Nx * Ny;
-    const int nb = (n + 255) / 256;
-    for (int k = 0; k < 2 * sweeps; ++k)
-        hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, st, Nx, Ny, bad, U, F, k & 1);
+    __shared__ double red[3][kFillThreads / 64];
+    const int t = threadIdx.x;
+    double c = 0, s = 0, s2 = 0;
+    for (int i = t; i < S; i += kFillThreads) {
+        const int v = it[i];
+        if (v >= 0 && v <= 99) { c += 1; s += v; s2 += (double)v * v; }
+    }
+    c = wave_sum(c); s = wave_sum(s); s2 = wave_sum(s2);
+    const int wid = t >> 6;
+    if ((t & 63) == 0) { red[0][wid] = c; red[1][wid] = s; red[2][wid] = s2; }
+    __syncthreads();
+    double C = 0, Sm = 0, S2 = 0;
+    for (int k = 0; k < kFillThreads / 64; ++k) { C += red[0][k]; Sm += red[1][k]; S2 += red[2][k]; }
+    const double mean = C > 0 ? Sm / C : 0.0;
+    const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
+    const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
+    for (int i = t; i < S; i += kFillThreads) {
+        const int v = it[i];
+        const bool b = v < 0 || v > 99 || (mean_rule && C > 0 && (double)v > thr);
+        bad[i] = b ? 1 : 0;
+        double2 u = Usrc ? reinterpret_cast<const double2 *>(Usrc)[i]
+                         : reinterpret_cast<const double2 *>(U)[i];
+        if (b) u = make_double2(NAN, NAN);
+        reinterpret_cast<double2 *>(U)[i] = u;
+        if (F) {
+            double4 f = Fsrc ? reinterpret_cast<const double4 *>(Fsrc)[i]
+                             : reinterpret_cast<const double4 *>(F)[i];
+            if (b) f = make_double4(NAN, NAN, NAN, NAN);
+            reinterpret_cast<double4 *>(F)[i] = f;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < 2 * sweeps; ++k) {
+        const int color = k & 1;
+        for (int i = t; i < S; i += kFillThreads) {
+            if (!bad[i] || (((i % Nx) + (i / Nx)) & 1) != color) continue;
+            fill_node(i, Nx, Ny, U, 2);
+            if (F) fill_node(i, Nx, Ny, F, 4);
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
+                          const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
+                          int sweeps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_badfill, dim3(1), dim3(kFillThreads), 0, st, iters, S, Nx, Ny,
+                       use_mean_rule, Usrc, Fsrc, U, F, bad, sweeps);
     return hipGetLastError();
 }
 

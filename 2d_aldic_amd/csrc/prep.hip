@@ -26,17 +26,16 @@ __device__ __forceinline__ double load_raw(const void *p, int dtype, size_t i)
     }
 }
 
-// acc[0..1] = sum, sum of squares of one image over the ROI.  For integer inputs every
-// partial sum is an integer below 2^53, so the FP64 atomics give the exact
-// (order-independent) result.  Grid: (x blocks, rows); no divisions.
+// Per-block partial sums over the ROI: part[2b] = sum, part[2b+1] = sum of squares
+// (FP64; exact for integer input since every partial stays below 2^53).  Block b takes
+// rows y1 + b, y1 + b + kStatParts, ...; the prep kernel reduces the partials.
 __global__ __launch_bounds__(256) void k_stats(const void *raw, int dtype, int nx, int x1,
-                                               int x2, int y1, int y2, double *acc)
+                                               int x2, int y1, int y2, double *part)
 {
     double s = 0.0, s2 = 0.0;
-    for (int y = y1 - 1 + blockIdx.y; y < y2; y += gridDim.y) {
+    for (int y = y1 - 1 + blockIdx.x; y < y2; y += gridDim.x) {
         const size_t row = (size_t)y * nx;
-        for (int x = x1 - 1 + blockIdx.x * blockDim.x + threadIdx.x; x < x2;
-             x += gridDim.x * blockDim.x) {
+        for (int x = x1 - 1 + threadIdx.x; x < x2; x += blockDim.x) {
             const double v = load_raw(raw, dtype, row + x);
             s += v;
             s2 += v * v;
@@ -49,17 +48,16 @@ __global__ __launch_bounds__(256) void k_stats(const void *raw, int dtype, int n
     if (lane == 0) { red[0][wid] = s; red[1][wid] = s2; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0, b = 0;
-        for (int k = 0; k < (int)(blockDim.x / kWave); ++k) { a += red[0][k]; b += red[1][k]; }
-        atomicAdd(acc, a);
-        atomicAdd(acc + 1, b);
+        part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
     }
 }
 
-__device__ __forceinline__ void mean_std(const double *acc, double n, double &mean, double &sd)
+__device__ __forceinline__ void mean_std(double sum, double sum2, double n, double &mean,
+                                         double &sd)
 {
-    mean = acc[0] / n;
-    double ss = acc[1] - acc[0] * mean;  // sum (x - mean)^2
+    mean = sum / n;
+    double ss = sum2 - sum * mean;  // sum (x - mean)^2
     if (ss < 0) ss = 0;
     sd = sqrt(ss / (n - 1.0));
 }
@@ -74,10 +72,22 @@ __global__ __launch_bounds__(256) void k_prep(const void *rawf, const void *rawg
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;  // 0-based
     const int y = blockIdx.y;
+    // reduce the per-block partials of both images (same order in every block)
+    __shared__ double st4[4];
+    if (threadIdx.x < 64) {
+        double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        for (int k = threadIdx.x; k < kStatParts; k += 64) {
+            a0 += acc[2 * k]; a1 += acc[2 * k + 1];
+            b0 += acc[2 * kStatParts + 2 * k]; b1 += acc[2 * kStatParts + 2 * k + 1];
+        }
+        a0 = wave_sum(a0); a1 = wave_sum(a1); b0 = wave_sum(b0); b1 = wave_sum(b1);
+        if (threadIdx.x == 0) { st4[0] = a0; st4[1] = a1; st4[2] = b0; st4[3] = b1; }
+    }
+    __syncthreads();
     if (x >= nx) return;
     double mf, sf, mg, sg;
-    mean_std(acc, n, mf, sf);
-    mean_std(acc + 2, n, mg, sg);
+    mean_std(st4[0], st4[1], n, mf, sf);
+    mean_std(st4[2], st4[3], n, mg, sg);
     const size_t i = (size_t)y * nx + x;
     f[i] = (float)((load_raw(rawf, dtype, i) - mf) / sf);
     g[i] = (float)((load_raw(rawg, dtype, i) - mg) / sg);
@@ -121,12 +131,8 @@ __global__ __launch_bounds__(256) void k_gradient(const float *f, int nx, int ny
 hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
                         hipStream_t st)
 {
-    const int w = roi[1] - roi[0] + 1, h = roi[3] - roi[2] + 1;
-    int bx = (w + 255) / 256;
-    if (bx > 8) bx = 8;
-    int by = h < 512 ? h : 512;
-    hipLaunchKernelGGL(k_stats, dim3(bx, by), dim3(256), 0, st, raw, dtype, nx, roi[0], roi[1],
-                       roi[2], roi[3], acc);
+    hipLaunchKernelGGL(k_stats, dim3(kStatParts), dim3(256), 0, st, raw, dtype, nx, roi[0],
+                       roi[1], roi[2], roi[3], acc);
     return hipGetLastError();
 }
 

@@ -194,3 +194,26 @@ def test_subpb2_patch_test_gpu(gpu_ctx):
     F = np.tile([0.01, 0.03, -0.02, 0.005], n)
     Uh, st = gpu_ctx.subpb2(2.56e-3, 1e-3, U, F, np.zeros(4 * n), np.zeros(2 * n), rtol=1e-12)
     assert np.allclose(Uh, U, atol=1e-8)
+
+
+def test_subpb2_distorted_mesh_jacobi_path(gpu_ctx, orc):
+    # interior nodes moved off the lattice: not a structured grid, so the library takes
+    # the Jacobi-preconditioned path; the FP64 results must agree all the same
+    coords, el = _grid_mesh(15, 11, 12.0)
+    rng = np.random.default_rng(3)
+    inner = (coords[:, 0] > coords[:, 0].min()) & (coords[:, 0] < coords[:, 0].max()) & \
+            (coords[:, 1] > coords[:, 1].min()) & (coords[:, 1] < coords[:, 1].max())
+    coords = coords.copy()
+    coords[inner] += rng.uniform(-2.0, 2.0, size=(inner.sum(), 2))
+    n = len(coords)
+    gpu_ctx.set_mesh(coords, el)
+    U = rng.normal(size=2 * n)
+    F = rng.normal(scale=0.01, size=4 * n)
+    W = rng.normal(scale=0.01, size=4 * n)
+    V = rng.normal(scale=0.1, size=2 * n)
+    Uh, st = gpu_ctx.subpb2(2.56e-3, 1e-3, U, F, W, V, rtol=1e-12)
+    Uo, _ = orc.subpb2(coords, el, 2.56e-3, 1e-3, U, F, W, V)
+    assert np.max(np.abs(Uh - Uo)) <= 1e-9 * np.max(np.abs(Uo)), st
+    Fg = gpu_ctx.nodal_strain(Uh)
+    Fr = orc.nodal_strain(coords, el, Uh)
+    assert np.max(np.abs(Fg - Fr)) < 1e-12 * max(1.0, np.abs(Fr).max())

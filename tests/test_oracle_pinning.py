@@ -369,3 +369,20 @@ def test_nodal_strain_linear_field_swapped_slots(orc):
     assert np.allclose(F.reshape(-1, 4), [ux, uy, vx, vy], atol=1e-13)   # reference order
     Ff = orc.nodal_strain(coords, el, U, swap_fix=True)
     assert np.allclose(Ff.reshape(-1, 4), [ux, vx, uy, vy], atol=1e-13)
+
+
+def test_q4_mass_is_kronecker_of_1d_mass(orc):
+    # the identity behind the library's structured-grid CG preconditioner: on a uniform
+    # lattice the consistent Q4 mass matrix is M1y (x) M1x (x-fastest node order)
+    import scipy.sparse as sp
+    gx, gy, h = 6, 4, 9.0
+    coords, el = grid_mesh(gx, gy, h)
+    n = len(coords)
+    z2, z4 = np.zeros(2 * n), np.zeros(4 * n)
+    ri, ci, val, b = orc.subpb2_assemble(coords, el, 0.0, 1.0, z2, z4, z4, z2)
+    M = sp.csr_matrix((val, (ri, ci)), shape=(2 * n, 2 * n)).toarray()[0::2, 0::2]
+    def M1(k):
+        A = np.diag(np.full(k, 4.0)) + np.diag(np.ones(k - 1), 1) + np.diag(np.ones(k - 1), -1)
+        A[0, 0] = A[-1, -1] = 2.0
+        return A * h / 6
+    assert np.allclose(M, np.kron(M1(gy), M1(gx)), atol=1e-12)
