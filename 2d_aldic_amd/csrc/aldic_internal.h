@@ -54,11 +54,13 @@ struct MeshDev {
     DBuf gvec;                // double2[nn]: sum over elements of int grad N_i dA
     double beta = -1, mu = -1, alpha = -1;  // values aval was assembled for
     // Structured grid (nodes x-fastest on a uniform gx x gy lattice, standard Q4
-    // elements): the mass matrix is M1x (x) M1y, used as the CG preconditioner.
+    // elements): K and M are Kronecker sums/products of the 1-D matrices, solved
+    // directly by fast diagonalisation (fem_solve).
     bool structured = false;
     int gx = 0, gy = 0;
     double hx = 0, hy = 0;
-    DBuf thomas;              // double[2*gx + 2*gy]: Thomas factors of M1x, then M1y
+    DBuf fdV;                 // double[gx*gx + gy*gy]: M-orthonormal eigenbases Vx, Vy
+    DBuf fdlam;               // double[gx + gy]: generalised eigenvalues of (K1, M1)
 };
 
 struct FrameState {
@@ -66,7 +68,7 @@ struct FrameState {
     aldic_frame_opts o{};
     int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
     DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
-        bad, tmp;
+        bad, tmp, blist;
     std::vector<double> hcoords;
     std::vector<int> helems;
     float ms[8] = {0};
@@ -145,6 +147,11 @@ hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, co
                    const double *udual, const double *vdual, double *b, hipStream_t st);
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
                    DBuf *work, double *stats_host, hipStream_t st, int poll);
+// Solves the assembled system: fast diagonalisation (direct, FP64 GEMMs) on structured
+// grids, Jacobi-PCG otherwise.  stats_host (nullable) = {iterations (0 = direct),
+// relative residual estimate}.
+hipError_t fem_solve(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
+                     DBuf *work, double *stats_host, hipStream_t st, int poll);
 hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F,
                       hipStream_t st);
 
@@ -152,6 +159,6 @@ hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
-                          int sweeps, hipStream_t st);
+                          int *list, int sweeps, hipStream_t st);
 
 }  // namespace aldic

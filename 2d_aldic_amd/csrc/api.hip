@@ -210,12 +210,13 @@ void aldic_destroy(aldic_ctx *ctx)
     DBuf *bufs[] = {&ctx->f, &ctx->g, &ctx->gx, &ctx->gy, &ctx->rawf, &ctx->rawg, &ctx->mask,
                     &ctx->stats, &ctx->mesh.coords, &ctx->mesh.elems, &ctx->mesh.n2e_ptr,
                     &ctx->mesh.n2e, &ctx->mesh.rowptr, &ctx->mesh.col, &ctx->mesh.aval,
-                    &ctx->mesh.mval, &ctx->mesh.diag, &ctx->mesh.gvec, &ctx->mesh.thomas};
+                    &ctx->mesh.mval, &ctx->mesh.diag, &ctx->mesh.gvec, &ctx->mesh.fdV, &ctx->mesh.fdlam};
     for (DBuf *b : bufs) b->release();
     for (auto &b : ctx->scratch) b.release();
     FrameState &F = ctx->fr;
     DBuf *fb[] = {&F.coords, &F.U0, &F.U6, &F.F6, &F.H21, &F.it6, &F.zncc, &F.Ucur, &F.Fcur,
-                  &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp};
+                  &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
+                  &F.blist};
     for (DBuf *b : fb) b->release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -540,22 +541,24 @@ int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elem
                  elems[4 * e + 2] == n0 + gx + 1 && elems[4 * e + 3] == n0 + gx;
         }
         if (ok) {
-            // Thomas factors of h/6 tridiag(1, 4, 1) with end diagonals 2 (1-D mass)
-            std::vector<double> th(2 * (size_t)gx + 2 * (size_t)gy);
-            auto factor = [](int n, double h, double *c, double *inv) {
-                const double e = h / 6.0;
-                double cprev = 0.0;
-                for (int i = 0; i < n; ++i) {
-                    const double d = (i == 0 || i == n - 1) ? 2.0 * h / 6.0 : 4.0 * h / 6.0;
-                    const double den = d - (i > 0 ? e * cprev : 0.0);
-                    inv[i] = 1.0 / den;
-                    c[i] = e / den;
-                    cprev = c[i];
+            // closed-form generalised eigenbases of the 1-D FE matrices (DCT-I vectors,
+            // M1-orthonormal): V[i][k] = cos(pi k i / (n-1)) / sqrt(m_k w_k)
+            std::vector<double> V((size_t)gx * gx + (size_t)gy * gy), lam(gx + gy);
+            auto basis = [](int n, double h, double *Vo, double *lo) {
+                const double pi = 3.14159265358979323846;
+                for (int k = 0; k < n; ++k) {
+                    const double th = pi * k / (n - 1), c = std::cos(th);
+                    const double mk = h / 6.0 * (4.0 + 2.0 * c);
+                    const double wk = (k == 0 || k == n - 1) ? (n - 1.0) : 0.5 * (n - 1.0);
+                    const double sc = 1.0 / std::sqrt(mk * wk);
+                    lo[k] = (2.0 / h) * (1.0 - c) / mk;
+                    for (int i = 0; i < n; ++i) Vo[(size_t)i * n + k] = std::cos(th * i) * sc;
                 }
             };
-            factor(gx, hx, th.data(), th.data() + gx);
-            factor(gy, hy, th.data() + 2 * gx, th.data() + 2 * gx + gy);
-            CK(up(m.thomas, th.data(), th.size(), ctx->stream));
+            basis(gx, hx, V.data(), lam.data());
+            basis(gy, hy, V.data() + (size_t)gx * gx, lam.data() + gx);
+            CK(up(m.fdV, V.data(), V.size(), ctx->stream));
+            CK(up(m.fdlam, lam.data(), lam.size(), ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
             m.structured = true;
             m.gx = gx;
@@ -591,7 +594,7 @@ int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const dou
                sc[3].as<double>(), sc[4].as<double>(), ctx->stream));
     static thread_local DBuf work[8];
     double st[2] = {0, 0};
-    CK(fem_pcg(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream, 8));
+    CK(fem_solve(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream, 8));
     CK(down(Uhat, sc[5].p, 2 * nn, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     if (stats) { stats[0] = st[0]; stats[1] = st[1]; }
@@ -693,6 +696,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.it6.ensure(sizeof(int) * S));
     CK(F.it2.ensure(sizeof(int) * S));
     CK(F.bad.ensure(sizeof(int) * S));
+    CK(F.blist.ensure(2 * sizeof(int) * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
     CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
@@ -773,7 +777,8 @@ int aldic_frame_run(aldic_ctx *ctx)
     CK(hipEventRecord(ev[3], st));
     // LocalICGN post-processing: bad-point rule and refill
     CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, F.U6.as<double>(), F.F6.as<double>(),
-                      F.U1.as<double>(), F.F1.as<double>(), F.bad.as<int>(), 12, st));
+                      F.U1.as<double>(), F.F1.as<double>(), F.bad.as<int>(),
+                      F.blist.as<int>(), 12, st));
     CK(hipEventRecord(ev[4], st));
 
     // step 1 global step with zero duals
@@ -784,7 +789,7 @@ int aldic_frame_run(aldic_ctx *ctx)
         CK(hipEventRecord(ev[5], st));
         CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F.F1.as<double>(),
                    F.udual.as<double>(), F.vdual.as<double>(), b, st));
-        CK(fem_pcg(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
+        CK(fem_solve(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
         CK(hipEventRecord(ev[6], st));
         CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F.F2.as<double>(), st));
         // dual update, FEM branch: udual = F2 - F1, vdual = U2 - U1 (main_ALDIC.m:340,440)
@@ -819,7 +824,7 @@ int aldic_frame_run(aldic_ctx *ctx)
         a2.zncc = nullptr;
         CK(launch_icgn2(a2, ppl, st));
         CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
-                          nullptr, F.bad.as<int>(), 12, st));
+                          nullptr, F.bad.as<int>(), F.blist.as<int>(), 12, st));
         // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         CK(hipMemcpyAsync(F.F1.p, F.F2.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
         CK(hipEventRecord(ev[9], st));

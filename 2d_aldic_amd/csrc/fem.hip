@@ -301,125 +301,112 @@ __global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, 
     if (threadIdx.x == 0) { rzn[blockIdx.x] = rz; rrn[blockIdx.x] = rr; }
 }
 
-// ---- tensor-mass preconditioner for structured grids ------------------------------
-// On a uniform gx x gy lattice with bilinear elements the consistent mass matrix is the
-// Kronecker product M = M1x (x) M1y of the 1-D mass matrices h/6 tridiag(1, 4, 1) (end
-// diagonals 2).  P = mu M is applied exactly by tridiagonal (Thomas) solves along x,
-// then along y; P^-1 A = I + (beta/mu) M^-1 K, whose condition number is
-// 1 + 24 beta / (mu h^2) at most, so CG needs a handful of iterations where Jacobi
-// needs dozens.  Thomas factors (per line, shared by all lines): c'_i and 1/denominator_i.
-struct GridP {
-    int gx, gy;
-    const double *cx, *ix, *cy, *iy;  // Thomas factors along x and y
-    double ex, ey;                    // off-diagonals hx/6, hy/6
-    double inv_mu;
+// ---- fast diagonalisation for structured grids ------------------------------------
+// On a uniform gx x gy lattice with bilinear elements (x-fastest numbering) the Q4
+// matrices are Kronecker forms of the 1-D FE matrices K1 = 1/h tridiag(-1, 2, -1) and
+// M1 = h/6 tridiag(1, 4, 1) (end diagonals halved):  M = M1y (x) M1x,
+// K = K1y (x) M1x + M1y (x) K1x.  With V^T M1 V = I and V^T K1 V = diag(lambda) (the
+// DCT-I basis, computed in closed form on the host), the system
+// (cK K + mu M) X = B, with X, B the gy x gx nodal arrays, has the exact solution
+//   X = Vy [ (Vy^T B Vx) ./ (mu + cK (lambda_y,i + lambda_x,j)) ] Vx^T
+// (Lynch, Rice & Thomas 1964).  Four FP64 GEMM passes replace the whole CG loop; u and
+// v ride along as the interleaved column pairs of the gy x 2gx nodal array.
+struct GemmArgs {
+    int M, N, K;
+    const double *A; long sam, sak, boA;   // A(m,k) = A[m*sam + k*sak + batch*boA]
+    const double *B; long sbk, sbn, boB;   // B(k,n)
+    double *C; long scm, scn, boC;         // C(m,n)
+    const double *lamy, *lamx;             // scale C(m,n) /= mu + cK (lamy[m] + lamx[n])
+    double mu, cK;
 };
 
-// Row pass: thread per (row, component).  With update = true it first applies
-// x += alpha p, r -= alpha Ap along the row (step k of CG) and emits r.r partials.
-template <bool UPDATE>
-__global__ __launch_bounds__(256) void k_pcgs_rows(GridP G, const double *p, const double *Ap,
-                                                   double *x, double *r, double *zt, PcgPart P,
-                                                   int k)
+constexpr int kGT = 64, kGK = 16;
+
+template <bool SCALE>
+__global__ __launch_bounds__(256) void k_gemm64(GemmArgs g)
 {
-    __shared__ double red[8];
-    const double *rzk = (k & 1) ? P.rz1 : P.rz0;
-    double *rrn = (k & 1) ? P.rr0 : P.rr1;
-    double alpha = 0.0;
-    if (UPDATE) {
-        if (P.stat[0] != 0.0) return;
-        double s[2];
-        const double *ps[2] = {rzk, P.pap};
-        const int cnt[2] = {P.nrz, P.nb};
-        part_sums<2>(ps, cnt, s);
-        alpha = s[0] / s[1];
-        if (!(s[1] > 0.0) || !isfinite(alpha)) {  // breakdown: report r = 0, stop
-            if (threadIdx.x == 0) rrn[blockIdx.x] = 0.0;
-            return;
+    __shared__ double As[kGK][kGT + 1];
+    __shared__ double Bs[kGK][kGT + 1];
+    const int t = threadIdx.x, tm = t / 16, tn = t % 16;
+    const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
+    const long bz = blockIdx.z;
+    const double *A = g.A + bz * g.boA;
+    const double *B = g.B + bz * g.boB;
+    double *C = g.C + bz * g.boC;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < g.K; k0 += kGK) {
+        for (int e = t; e < kGK * kGT; e += 256) {
+            const int mm = e / kGK, kk = e % kGK;  // A tile: 64 m x 16 k
+            const int gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < g.M && gk < g.K) ? A[gm * g.sam + gk * g.sak] : 0.0;
+            const int kb = e / kGT, nb = e % kGT;  // B tile: 16 k x 64 n
+            const int gk2 = k0 + kb, gn = n0 + nb;
+            Bs[kb][nb] = (gk2 < g.K && gn < g.N) ? B[gk2 * g.sbk + gn * g.sbn] : 0.0;
         }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGK; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
     }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double rr = 0.0;
-    if (t < 2 * G.gy) {
-        const int row = t >> 1, c = t & 1;
-        const size_t base = (size_t)row * G.gx * 2 + c;
-        double prev = 0.0;
-        for (int i = 0; i < G.gx; ++i) {  // forward sweep (and the CG update)
-            const size_t o = base + 2 * (size_t)i;
-            double ri = r[o];
-            if (UPDATE) {
-                x[o] += alpha * p[o];
-                ri -= alpha * Ap[o];
-                r[o] = ri;
-                rr += ri * ri;
-            }
-            prev = (ri - G.ex * prev) * G.ix[i];
-            zt[o] = prev;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gm = m0 + tm + 16 * i;
+        if (gm >= g.M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + tn + 16 * j;
+            if (gn >= g.N) continue;
+            double v = acc[i][j];
+            if (SCALE) v /= g.mu + g.cK * (g.lamy[gm] + g.lamx[gn]);
+            C[gm * g.scm + gn * g.scn] = v;
         }
-        double nxt = 0.0;
-        for (int i = G.gx - 1; i >= 0; --i) {  // back substitution
-            const size_t o = base + 2 * (size_t)i;
-            nxt = zt[o] - G.cx[i] * nxt;
-            zt[o] = nxt;
-        }
-    }
-    if (UPDATE) {
-        double dummy = 0.0;
-        block_reduce2(rr, dummy, red);
-        if (threadIdx.x == 0) rrn[blockIdx.x] = rr;
     }
 }
 
-// Column pass: thread per (column, component): z = (1/mu) M1y^-1 zt, r.z partials; with
-// init = true also p = z.
-template <bool INIT>
-__global__ __launch_bounds__(256) void k_pcgs_cols(GridP G, const double *r, const double *zt,
-                                                   double *z, double *p, PcgPart P, int k)
+static hipError_t gemm(const GemmArgs &g, int batch, bool scale, hipStream_t st)
 {
-    __shared__ double red[8];
-    if (!INIT && P.stat[0] != 0.0) return;
-    double *rzn = INIT ? P.rz0 : ((k & 1) ? P.rz0 : P.rz1);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0.0;
-    if (t < 2 * G.gx) {
-        const int col = t >> 1, c = t & 1;
-        const size_t stride = (size_t)G.gx * 2;
-        const size_t base = 2 * (size_t)col + c;
-        double prev = 0.0;
-        for (int j = 0; j < G.gy; ++j) {
-            const size_t o = base + stride * j;
-            prev = (zt[o] - G.ey * prev) * G.iy[j];
-            z[o] = prev;
-        }
-        double nxt = 0.0;
-        for (int j = G.gy - 1; j >= 0; --j) {
-            const size_t o = base + stride * j;
-            nxt = z[o] - G.cy[j] * nxt;
-            const double zz = nxt * G.inv_mu;
-            z[o] = zz;
-            if (INIT) p[o] = zz;
-            rz += r[o] * zz;
-        }
-    }
-    double dummy = 0.0;
-    block_reduce2(rz, dummy, red);
-    if (threadIdx.x == 0) rzn[blockIdx.x] = rz;
+    dim3 grid((g.N + kGT - 1) / kGT, (g.M + kGT - 1) / kGT, batch);
+    if (scale) hipLaunchKernelGGL(k_gemm64<true>, grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL(k_gemm64<false>, grid, dim3(256), 0, st, g);
+    return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_pcgs_init(int nn, const double2 *b, double2 *x,
-                                                   double2 *r, PcgPart P)
+static hipError_t fem_fd_solve(const MeshDev &m, const double *b, double *x, DBuf *work,
+                               hipStream_t st)
 {
-    __shared__ double red[8];
-    double bb = 0.0, dummy = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const double2 bi = b[i];
-        x[i] = make_double2(0.0, 0.0);
-        r[i] = bi;
-        bb += bi.x * bi.x + bi.y * bi.y;
-    }
-    block_reduce2(bb, dummy, red);
-    if (threadIdx.x == 0) P.bb[blockIdx.x] = bb;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
+    const int gx = m.gx, gy = m.gy;
+    const size_t nv = 2 * (size_t)gx * gy;
+    hipError_t e;
+    if ((e = work[0].ensure(sizeof(double) * nv)) != hipSuccess) return e;
+    if ((e = work[1].ensure(sizeof(double) * nv)) != hipSuccess) return e;
+    double *T1 = work[0].as<double>(), *T2 = work[1].as<double>();
+    const double *Vx = m.fdV.as<double>(), *Vy = Vx + (size_t)gx * gx;
+    const double *lx = m.fdlam.as<double>(), *ly = lx + gx;
+    const long w2 = 2L * gx;
+    GemmArgs g{};
+    // T1 = Vy^T B
+    g = GemmArgs{gy, 2 * gx, gy, Vy, 1, gy, 0, b, w2, 1, 0, T1, w2, 1, 0, nullptr, nullptr, 0, 0};
+    if ((e = gemm(g, 1, false, st)) != hipSuccess) return e;
+    // T2 = (T1 Vx) ./ D, per component
+    g = GemmArgs{gy, gx, gx, T1, w2, 2, 1, Vx, gx, 1, 0, T2, w2, 2, 1, ly, lx, m.mu,
+                 m.beta + m.alpha * m.mu};
+    if ((e = gemm(g, 2, true, st)) != hipSuccess) return e;
+    // T1 = T2 Vx^T, per component
+    g = GemmArgs{gy, gx, gx, T2, w2, 2, 1, Vx, 1, gx, 0, T1, w2, 2, 1, nullptr, nullptr, 0, 0};
+    if ((e = gemm(g, 2, false, st)) != hipSuccess) return e;
+    // X = Vy T1
+    g = GemmArgs{gy, 2 * gx, gy, Vy, gy, 1, 0, T1, w2, 1, 0, x, w2, 1, 0, nullptr, nullptr, 0, 0};
+    return gemm(g, 1, false, st);
 }
 
 hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st)
@@ -451,49 +438,29 @@ int pcg_blocks(int nn)
     return nb < 1 ? 1 : (nb > kPcgMaxBlocks ? kPcgMaxBlocks : nb);
 }
 
-// work[0..5]: r, z, pA, pB, Ap, zt (double2 each); work[6]: partials + status.
+// work[0..4]: r, z, pA, pB, Ap (double2 each); work[5]: partials + status.
 // poll > 0: check convergence on the host every `poll` iterations (synchronises);
 // poll == 0: enqueue exactly maxit iterations (kernels after convergence return at
 // once), no host synchronisation -- usable inside stream capture.
-// Structured grids with mu > 0 use the tensor-mass preconditioner, others Jacobi.
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
                    DBuf *work, double *stats_host, hipStream_t st, int poll)
 {
     const int nn = m.nn;
     const int nb = pcg_blocks(nn);
-    const bool tens = m.structured && m.mu > 0.0;
-    const int nbr = tens ? (2 * m.gy + kPcgThreads - 1) / kPcgThreads : nb;
-    const int nbc = tens ? (2 * m.gx + kPcgThreads - 1) / kPcgThreads : nb;
-    const int npart = std::max(nb, std::max(nbr, nbc));
     const size_t vb = sizeof(double2) * (size_t)nn;
     hipError_t e;
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < 5; ++k)
         if ((e = work[k].ensure(vb)) != hipSuccess) return e;
-    if ((e = work[6].ensure(sizeof(double) * (6 * (size_t)npart + 8))) != hipSuccess) return e;
-    double *pp = work[6].as<double>();
+    if ((e = work[5].ensure(sizeof(double) * (6 * (size_t)nb + 8))) != hipSuccess) return e;
+    double *pp = work[5].as<double>();
     PcgPart P;
-    P.pap = pp; P.rz0 = pp + npart; P.rz1 = pp + 2 * npart; P.rr0 = pp + 3 * npart;
-    P.rr1 = pp + 4 * npart; P.bb = pp + 5 * npart; P.stat = pp + 6 * npart;
-    P.nb = nb; P.nrz = nbc; P.nrr = nbr; P.rtol2 = rtol * rtol;
+    P.pap = pp; P.rz0 = pp + nb; P.rz1 = pp + 2 * nb; P.rr0 = pp + 3 * nb; P.rr1 = pp + 4 * nb;
+    P.bb = pp + 5 * nb; P.stat = pp + 6 * nb; P.nb = nb; P.nrz = nb; P.nrr = nb;
+    P.rtol2 = rtol * rtol;
     double2 *r = work[0].as<double2>(), *z = work[1].as<double2>(), *pA = work[2].as<double2>(),
-            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>(), *zt = work[5].as<double2>();
-    GridP G{};
-    if (tens) {
-        const double *th = m.thomas.as<double>();
-        G.gx = m.gx; G.gy = m.gy;
-        G.cx = th; G.ix = th + m.gx; G.cy = th + 2 * m.gx; G.iy = th + 2 * m.gx + m.gy;
-        G.ex = m.hx / 6.0; G.ey = m.hy / 6.0; G.inv_mu = 1.0 / m.mu;
-        hipLaunchKernelGGL(k_pcgs_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
-                           (double2 *)x, r, P);
-        hipLaunchKernelGGL(k_pcgs_rows<false>, dim3(nbr), dim3(kPcgThreads), 0, st, G,
-                           (const double *)nullptr, (const double *)nullptr, (double *)x,
-                           (double *)r, (double *)zt, P, 0);
-        hipLaunchKernelGGL(k_pcgs_cols<true>, dim3(nbc), dim3(kPcgThreads), 0, st, G,
-                           (const double *)r, (const double *)zt, (double *)z, (double *)pA, P, 0);
-    } else {
-        hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
-                           m.diag.as<double>(), (double2 *)x, r, z, pA, P);
-    }
+            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>();
+    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
+                       m.diag.as<double>(), (double2 *)x, r, z, pA, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
@@ -504,17 +471,8 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
             hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(kPcgThreads), 0, st, nn,
                                m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(), z, pold,
                                pnew, Ap, P, k);
-            if (tens) {
-                hipLaunchKernelGGL(k_pcgs_rows<true>, dim3(nbr), dim3(kPcgThreads), 0, st, G,
-                                   (const double *)pnew, (const double *)Ap, (double *)x,
-                                   (double *)r, (double *)zt, P, k);
-                hipLaunchKernelGGL(k_pcgs_cols<false>, dim3(nbc), dim3(kPcgThreads), 0, st, G,
-                                   (const double *)r, (const double *)zt, (double *)z,
-                                   (double *)nullptr, P, k);
-            } else {
-                hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kPcgThreads), 0, st, nn,
-                                   m.diag.as<double>(), pnew, Ap, (double2 *)x, r, z, P, k);
-            }
+            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kPcgThreads), 0, st, nn,
+                               m.diag.as<double>(), pnew, Ap, (double2 *)x, r, z, P, k);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
@@ -535,6 +493,17 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
         stats_host[1] = hs[2];
     }
     return hipSuccess;
+}
+
+hipError_t fem_solve(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
+                     DBuf *work, double *stats_host, hipStream_t st, int poll)
+{
+    if (m.structured && m.mu > 0.0) {
+        hipError_t e = fem_fd_solve(m, b, x, work + 6, st);
+        if (stats_host) { stats_host[0] = 0.0; stats_host[1] = 0.0; }
+        return e;
+    }
+    return fem_pcg(m, b, x, rtol, maxit, work, stats_host, st, poll);
 }
 
 // ---- nodal strain (funGlobal_NodalStrainAvg.m) ------------------------------------
@@ -619,9 +588,10 @@ hipError_t launch_sub(const double *a, const double *b, double *out, long n, hip
 //   mean + 0.15 std of the good ones (std with N-1); bad U (and F) become NaN.
 //   Refill: every bad node takes the mean of its non-NaN 4-neighbours on the Nx x Ny
 //   node grid (the inpaint_nans method-4 spring stencil), `sweeps` red-black sweeps.
-// One workgroup of 1024 threads does everything (statistics, marking, sweeps separated
-// by workgroup barriers), so the step costs one launch; red-black order makes it
-// deterministic.  When Usrc (Fsrc) is given, it is first copied to U (F).
+// One workgroup of 1024 threads does everything: statistics, marking, one list of bad
+// nodes per colour (an update reads only the other colour, so the order inside a list
+// does not matter and the result is deterministic), then the sweeps over the lists,
+// separated by workgroup barriers.  When Usrc (Fsrc) is given it is copied to U (F).
 constexpr int kFillThreads = 1024;
 
 __device__ __forceinline__ void fill_node(int i, int Nx, int Ny, double *V, int NC)
@@ -629,15 +599,17 @@ __device__ __forceinline__ void fill_node(int i, int Nx, int Ny, double *V, int 
     const int x = i % Nx, y = i / Nx;
     const int nb[4] = {x > 0 ? i - 1 : -1, x < Nx - 1 ? i + 1 : -1, y > 0 ? i - Nx : -1,
                        y < Ny - 1 ? i + Nx : -1};
+    double v[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (int c = 0; c < NC; ++c) v[k][c] = nb[k] < 0 ? NAN : V[(size_t)NC * nb[k] + c];
     for (int c = 0; c < NC; ++c) {
         double s = 0.0;
         int n = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (nb[k] < 0) continue;
-            const double v = V[(size_t)NC * nb[k] + c];
-            if (isnan(v)) continue;
-            s += v;
+            if (isnan(v[k][c])) continue;
+            s += v[k][c];
             ++n;
         }
         if (n > 0) V[(size_t)NC * i + c] = s / n;
@@ -647,10 +619,13 @@ __device__ __forceinline__ void fill_node(int i, int Nx, int Ny, double *V, int 
 __global__ __launch_bounds__(kFillThreads) void k_badfill(const int *it, int S, int Nx, int Ny,
                                                           int mean_rule, const double *Usrc,
                                                           const double *Fsrc, double *U,
-                                                          double *F, int *bad, int sweeps)
+                                                          double *F, int *bad, int *list,
+                                                          int sweeps)
 {
     __shared__ double red[3][kFillThreads / 64];
+    __shared__ int cnt[2];
     const int t = threadIdx.x;
+    if (t < 2) cnt[t] = 0;
     double c = 0, s = 0, s2 = 0;
     for (int i = t; i < S; i += kFillThreads) {
         const int v = it[i];
@@ -679,12 +654,21 @@ __global__ __launch_bounds__(kFillThreads) void k_badfill(const int *it, int S, 
             if (b) f = make_double4(NAN, NAN, NAN, NAN);
             reinterpret_cast<double4 *>(F)[i] = f;
         }
+        if (b) {
+            const int color = ((i % Nx) + (i / Nx)) & 1;
+            const int slot = atomicAdd(&cnt[color], 1);
+            list[color * (size_t)S + slot] = i;
+        }
     }
     __syncthreads();
+    const int n0 = cnt[0], n1 = cnt[1];
+    if (n0 + n1 == 0) return;
     for (int k = 0; k < 2 * sweeps; ++k) {
         const int color = k & 1;
-        for (int i = t; i < S; i += kFillThreads) {
-            if (!bad[i] || (((i % Nx) + (i / Nx)) & 1) != color) continue;
+        const int n = color ? n1 : n0;
+        const int *l = list + color * (size_t)S;
+        for (int q = t; q < n; q += kFillThreads) {
+            const int i = l[q];
             fill_node(i, Nx, Ny, U, 2);
             if (F) fill_node(i, Nx, Ny, F, 4);
         }
@@ -694,10 +678,10 @@ __global__ __launch_bounds__(kFillThreads) void k_badfill(const int *it, int S, 
 
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
-                          int sweeps, hipStream_t st)
+                          int *list, int sweeps, hipStream_t st)
 {
     hipLaunchKernelGGL(k_badfill, dim3(1), dim3(kFillThreads), 0, st, iters, S, Nx, Ny,
-                       use_mean_rule, Usrc, Fsrc, U, F, bad, sweeps);
+                       use_mean_rule, Usrc, Fsrc, U, F, bad, list, sweeps);
     return hipGetLastError();
 }
 
