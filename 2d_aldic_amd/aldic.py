@@ -241,19 +241,54 @@ def _node_grid(coordinatesFEM):
     return xs.size, ys.size, ix, iy
 
 
-def _spring_refill(coordinatesFEM, U, F=None):
-    """Refill NaN nodes on the node grid with the springs stencil (the reference uses
-    scatteredInterpolant 'natural', LocalICGN.m:144-161; equal for isolated holes)."""
+def axis_fill(V, nc, bad, Nx, Ny):
+    """Fill bad nodes of an Nx x Ny node grid (x fastest, nc interleaved components):
+    each takes the inverse-distance average of the nearest good node along the four grid
+    directions (left, right, down, up, summed in that order); 0 when none exists."""
+    G = np.asarray(V, dtype=np.float64).reshape(Ny, Nx, nc)
+    B = np.asarray(bad, dtype=bool).reshape(Ny, Nx)
+    yy, xx = np.mgrid[0:Ny, 0:Nx]
+    A = np.where(~B, xx, -1)
+    L = np.concatenate([np.full((Ny, 1), -1), np.maximum.accumulate(A, axis=1)[:, :-1]], axis=1)
+    A = np.where(~B, xx, Nx)
+    R = np.concatenate([np.minimum.accumulate(A[:, ::-1], axis=1)[:, ::-1][:, 1:],
+                        np.full((Ny, 1), Nx)], axis=1)
+    A = np.where(~B, yy, -1)
+    D = np.concatenate([np.full((1, Nx), -1), np.maximum.accumulate(A, axis=0)[:-1, :]], axis=0)
+    A = np.where(~B, yy, Ny)
+    Up = np.concatenate([np.minimum.accumulate(A[::-1, :], axis=0)[::-1, :][1:, :],
+                         np.full((1, Nx), Ny)], axis=0)
+    srcs = []
+    for ok, idx_y, idx_x, dist in ((L >= 0, yy, np.clip(L, 0, Nx - 1), xx - L),
+                                   (R < Nx, yy, np.clip(R, 0, Nx - 1), R - xx),
+                                   (D >= 0, np.clip(D, 0, Ny - 1), xx, yy - D),
+                                   (Up < Ny, np.clip(Up, 0, Ny - 1), xx, Up - yy)):
+        w = np.where(ok, 1.0 / np.maximum(dist, 1), 0.0)
+        srcs.append((ok, w, G[idx_y, idx_x]))
+    ws = np.zeros((Ny, Nx))
+    s = np.zeros((Ny, Nx, nc))
+    for ok, w, v in srcs:
+        ws = ws + w
+        s = s + np.where(ok[:, :, None], w[:, :, None] * v, 0.0)
+    fillv = np.where(ws[:, :, None] > 0, s / np.where(ws > 0, ws, 1.0)[:, :, None], 0.0)
+    out = np.where(B[:, :, None], fillv, G)
+    return out.reshape(-1)
+
+
+def _refill(coordinatesFEM, U, F=None):
+    """Refill NaN nodes on the node grid (axis_fill; the reference uses scatteredInterpolant
+    'natural', LocalICGN.m:144-161; equal for isolated holes: the 4-neighbour mean)."""
     Nx, Ny, ix, iy = _node_grid(coordinatesFEM)
     if Nx * Ny != coordinatesFEM.shape[0]:
         raise NotImplementedError("refill needs a full rectangular node grid")
+    order = iy * Nx + ix
+    bad = np.zeros(Nx * Ny, bool)
+    bad[order] = np.isnan(U[0::2]) | np.isnan(U[1::2])
     def fill(vec, nc):
-        out = vec.copy()
-        for c in range(nc):
-            g = np.full((Ny, Nx), np.nan)
-            g[iy, ix] = vec[c::nc]
-            out[c::nc] = inpaint_nans_springs(g)[iy, ix]
-        return out
+        g = np.empty(Nx * Ny * nc)
+        g.reshape(-1, nc)[order] = vec.reshape(-1, nc)
+        out = axis_fill(g, nc, bad, Nx, Ny).reshape(-1, nc)
+        return out[order].ravel()
     U = fill(U, 2)
     if F is not None:
         F = fill(F, 4)
@@ -286,7 +321,7 @@ def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="Gauss
     U[1::2][bad] = np.nan
     for c in range(4):
         F[c::4][bad] = np.nan
-    U, F = _spring_refill(coords, U, F)
+    U, F = _refill(coords, U, F)
     return U, F, H, LocalTime, it, LocalICGNBadPtNum
 
 
@@ -304,7 +339,7 @@ def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, bet
     HPar may be the reference's cell(21) (list of 21 node vectors) or an (S, 21) array.
     With ALSolveStep == 1 the reference passes a zero Hessian (:69-77).  The gridfit
     smoothing of every node (:186-199) is not applied (smoothing='off'); failed nodes
-    are refilled by the springs stencil instead."""
+    are refilled by axis_fill instead."""
     if smoothing != "off":
         raise NotImplementedError("gridfit smoothing of Subpb1 is not provided")
     ctx = _ensure_images(ImgRef, ImgDef, Df)
@@ -322,7 +357,7 @@ def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, bet
     U = U.copy()
     U[0::2][bad] = np.nan
     U[1::2][bad] = np.nan
-    U, _ = _spring_refill(coords, U)
+    U, _ = _refill(coords, U)
     return U, HPar, ALSub1Time, it.astype(np.int64), int(bad.sum())
 
 

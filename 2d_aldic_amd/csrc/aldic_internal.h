@@ -68,7 +68,7 @@ struct FrameState {
     aldic_frame_opts o{};
     int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
     DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
-        bad, tmp, blist;
+        bad, tmp, part;
     std::vector<double> hcoords;
     std::vector<int> helems;
     float ms[8] = {0};
@@ -159,6 +159,6 @@ hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
-                          int *list, int sweeps, hipStream_t st);
+                          double *part, hipStream_t st);
 
 }  // namespace aldic

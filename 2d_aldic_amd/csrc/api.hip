@@ -216,7 +216,7 @@ void aldic_destroy(aldic_ctx *ctx)
     FrameState &F = ctx->fr;
     DBuf *fb[] = {&F.coords, &F.U0, &F.U6, &F.F6, &F.H21, &F.it6, &F.zncc, &F.Ucur, &F.Fcur,
                   &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
-                  &F.blist};
+                  &F.part};
     for (DBuf *b : fb) b->release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -696,7 +696,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.it6.ensure(sizeof(int) * S));
     CK(F.it2.ensure(sizeof(int) * S));
     CK(F.bad.ensure(sizeof(int) * S));
-    CK(F.blist.ensure(2 * sizeof(int) * S));
+    CK(F.part.ensure(3 * 256 * sizeof(double)));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
     CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
@@ -778,7 +778,7 @@ int aldic_frame_run(aldic_ctx *ctx)
     // LocalICGN post-processing: bad-point rule and refill
     CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, F.U6.as<double>(), F.F6.as<double>(),
                       F.U1.as<double>(), F.F1.as<double>(), F.bad.as<int>(),
-                      F.blist.as<int>(), 12, st));
+                      F.part.as<double>(), st));
     CK(hipEventRecord(ev[4], st));
 
     // step 1 global step with zero duals
@@ -824,7 +824,7 @@ int aldic_frame_run(aldic_ctx *ctx)
         a2.zncc = nullptr;
         CK(launch_icgn2(a2, ppl, st));
         CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
-                          nullptr, F.bad.as<int>(), F.blist.as<int>(), 12, st));
+                          nullptr, F.bad.as<int>(), F.part.as<double>(), st));
         // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         CK(hipMemcpyAsync(F.F1.p, F.F2.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
         CK(hipEventRecord(ev[9], st));

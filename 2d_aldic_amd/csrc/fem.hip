@@ -320,50 +320,54 @@ struct GemmArgs {
     double mu, cK;
 };
 
-constexpr int kGT = 64, kGK = 16;
+constexpr int kGK = 16;
 
-template <bool SCALE>
+// C tile of TILE x TILE per 256-thread block, each thread (TILE/16)^2 outputs; K in
+// chunks of 16 through LDS.  The tile shrinks for small problems so the grid still
+// spreads over the CUs (a 122 x 244 product is 128 blocks at TILE 16).
+template <int TILE, bool SCALE>
 __global__ __launch_bounds__(256) void k_gemm64(GemmArgs g)
 {
-    __shared__ double As[kGK][kGT + 1];
-    __shared__ double Bs[kGK][kGT + 1];
+    constexpr int R = TILE / 16;
+    __shared__ double As[kGK][TILE + 1];
+    __shared__ double Bs[kGK][TILE + 1];
     const int t = threadIdx.x, tm = t / 16, tn = t % 16;
-    const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
+    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
     const long bz = blockIdx.z;
     const double *A = g.A + bz * g.boA;
     const double *B = g.B + bz * g.boB;
     double *C = g.C + bz * g.boC;
-    double acc[4][4] = {};
+    double acc[R][R] = {};
     for (int k0 = 0; k0 < g.K; k0 += kGK) {
-        for (int e = t; e < kGK * kGT; e += 256) {
-            const int mm = e / kGK, kk = e % kGK;  // A tile: 64 m x 16 k
+        for (int e = t; e < kGK * TILE; e += 256) {
+            const int mm = e / kGK, kk = e % kGK;  // A tile: TILE m x 16 k
             const int gm = m0 + mm, gk = k0 + kk;
             As[kk][mm] = (gm < g.M && gk < g.K) ? A[gm * g.sam + gk * g.sak] : 0.0;
-            const int kb = e / kGT, nb = e % kGT;  // B tile: 16 k x 64 n
+            const int kb = e / TILE, nb = e % TILE;  // B tile: 16 k x TILE n
             const int gk2 = k0 + kb, gn = n0 + nb;
             Bs[kb][nb] = (gk2 < g.K && gn < g.N) ? B[gk2 * g.sbk + gn * g.sbn] : 0.0;
         }
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kGK; ++kk) {
-            double a[4], b[4];
+            double a[R], b[R];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+            for (int i = 0; i < R; ++i) a[i] = As[kk][tm + 16 * i];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+            for (int j = 0; j < R; ++j) b[j] = Bs[kk][tn + 16 * j];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < R; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+                for (int j = 0; j < R; ++j) acc[i][j] += a[i] * b[j];
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < R; ++i) {
         const int gm = m0 + tm + 16 * i;
         if (gm >= g.M) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < R; ++j) {
             const int gn = n0 + tn + 16 * j;
             if (gn >= g.N) continue;
             double v = acc[i][j];
@@ -373,11 +377,20 @@ __global__ __launch_bounds__(256) void k_gemm64(GemmArgs g)
     }
 }
 
+template <int TILE>
+static void gemm_launch(const GemmArgs &g, int batch, bool scale, hipStream_t st)
+{
+    dim3 grid((g.N + TILE - 1) / TILE, (g.M + TILE - 1) / TILE, batch);
+    if (scale) hipLaunchKernelGGL((k_gemm64<TILE, true>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((k_gemm64<TILE, false>), grid, dim3(256), 0, st, g);
+}
+
 static hipError_t gemm(const GemmArgs &g, int batch, bool scale, hipStream_t st)
 {
-    dim3 grid((g.N + kGT - 1) / kGT, (g.M + kGT - 1) / kGT, batch);
-    if (scale) hipLaunchKernelGGL(k_gemm64<true>, grid, dim3(256), 0, st, g);
-    else hipLaunchKernelGGL(k_gemm64<false>, grid, dim3(256), 0, st, g);
+    auto blocks = [&](int T) { return (long)((g.N + T - 1) / T) * ((g.M + T - 1) / T) * batch; };
+    if (blocks(64) >= 256) gemm_launch<64>(g, batch, scale, st);
+    else if (blocks(32) >= 256) gemm_launch<32>(g, batch, scale, st);
+    else gemm_launch<16>(g, batch, scale, st);
     return hipGetLastError();
 }
 
@@ -582,65 +595,52 @@ hipError_t launch_sub(const double *a, const double *b, double *out, long n, hip
     return hipGetLastError();
 }
 
-// Bad-point rule and refill in one single-workgroup launch.
+// Bad-point rule and refill (three grid launches, no host round trip).
 //   LocalICGN.m:119-139 (mean_rule = 1) / Subpb1.m:167-182 (mean_rule = 0): a subset is
 //   bad when its status is < 0 or > 99, and with the mean rule also when it exceeds
 //   mean + 0.15 std of the good ones (std with N-1); bad U (and F) become NaN.
-//   Refill: every bad node takes the mean of its non-NaN 4-neighbours on the Nx x Ny
-//   node grid (the inpaint_nans method-4 spring stencil), `sweeps` red-black sweeps.
-// One workgroup of 1024 threads does everything: statistics, marking, one list of bad
-// nodes per colour (an update reads only the other colour, so the order inside a list
-// does not matter and the result is deterministic), then the sweeps over the lists,
-// separated by workgroup barriers.  When Usrc (Fsrc) is given it is copied to U (F).
-constexpr int kFillThreads = 1024;
-
-__device__ __forceinline__ void fill_node(int i, int Nx, int Ny, double *V, int NC)
+//   Refill (the reference interpolates the bad nodes from the good ones with
+//   scatteredInterpolant 'natural', LocalICGN.m:144-161): every bad node takes the
+//   inverse-distance average of the nearest good node along each of the four grid
+//   directions (left, right, down, up, summed in that order); an isolated bad node gets
+//   the mean of its four neighbours.  A node with no good node in its row or column
+//   gets 0.
+// k_itstats: per-block partials {count, sum, sum of squares} of the good statuses.
+__global__ __launch_bounds__(256) void k_itstats(const int *it, int S, double *part)
 {
-    const int x = i % Nx, y = i / Nx;
-    const int nb[4] = {x > 0 ? i - 1 : -1, x < Nx - 1 ? i + 1 : -1, y > 0 ? i - Nx : -1,
-                       y < Ny - 1 ? i + Nx : -1};
-    double v[4][4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        for (int c = 0; c < NC; ++c) v[k][c] = nb[k] < 0 ? NAN : V[(size_t)NC * nb[k] + c];
-    for (int c = 0; c < NC; ++c) {
-        double s = 0.0;
-        int n = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (isnan(v[k][c])) continue;
-            s += v[k][c];
-            ++n;
-        }
-        if (n > 0) V[(size_t)NC * i + c] = s / n;
-    }
-}
-
-__global__ __launch_bounds__(kFillThreads) void k_badfill(const int *it, int S, int Nx, int Ny,
-                                                          int mean_rule, const double *Usrc,
-                                                          const double *Fsrc, double *U,
-                                                          double *F, int *bad, int *list,
-                                                          int sweeps)
-{
-    __shared__ double red[3][kFillThreads / 64];
-    __shared__ int cnt[2];
-    const int t = threadIdx.x;
-    if (t < 2) cnt[t] = 0;
+    __shared__ double red[3][4];
     double c = 0, s = 0, s2 = 0;
-    for (int i = t; i < S; i += kFillThreads) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x) {
         const int v = it[i];
         if (v >= 0 && v <= 99) { c += 1; s += v; s2 += (double)v * v; }
     }
     c = wave_sum(c); s = wave_sum(s); s2 = wave_sum(s2);
-    const int wid = t >> 6;
-    if ((t & 63) == 0) { red[0][wid] = c; red[1][wid] = s; red[2][wid] = s2; }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = c; red[1][w] = s; red[2][w] = s2; }
     __syncthreads();
-    double C = 0, Sm = 0, S2 = 0;
-    for (int k = 0; k < kFillThreads / 64; ++k) { C += red[0][k]; Sm += red[1][k]; S2 += red[2][k]; }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) part[3 * blockIdx.x + q] = red[q][0] + red[q][1] + red[q][2] + red[q][3];
+}
+
+// k_mark: threshold from the partials (same order in every block), mark, copy, NaN.
+__global__ __launch_bounds__(256) void k_mark(const int *it, int S, const double *part, int np,
+                                              int mean_rule, const double *Usrc,
+                                              const double *Fsrc, double *U, double *F, int *bad)
+{
+    __shared__ double tot[3];
+    if (threadIdx.x < 64) {
+        double a[3] = {0, 0, 0};
+        for (int k = threadIdx.x; k < np; k += 64)
+            for (int q = 0; q < 3; ++q) a[q] += part[3 * k + q];
+        for (int q = 0; q < 3; ++q) a[q] = wave_sum(a[q]);
+        if (threadIdx.x == 0) { tot[0] = a[0]; tot[1] = a[1]; tot[2] = a[2]; }
+    }
+    __syncthreads();
+    const double C = tot[0], Sm = tot[1], S2 = tot[2];
     const double mean = C > 0 ? Sm / C : 0.0;
     const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
     const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
-    for (int i = t; i < S; i += kFillThreads) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x) {
         const int v = it[i];
         const bool b = v < 0 || v > 99 || (mean_rule && C > 0 && (double)v > thr);
         bad[i] = b ? 1 : 0;
@@ -654,34 +654,61 @@ __global__ __launch_bounds__(kFillThreads) void k_badfill(const int *it, int S, 
             if (b) f = make_double4(NAN, NAN, NAN, NAN);
             reinterpret_cast<double4 *>(F)[i] = f;
         }
-        if (b) {
-            const int color = ((i % Nx) + (i / Nx)) & 1;
-            const int slot = atomicAdd(&cnt[color], 1);
-            list[color * (size_t)S + slot] = i;
-        }
     }
-    __syncthreads();
-    const int n0 = cnt[0], n1 = cnt[1];
-    if (n0 + n1 == 0) return;
-    for (int k = 0; k < 2 * sweeps; ++k) {
-        const int color = k & 1;
-        const int n = color ? n1 : n0;
-        const int *l = list + color * (size_t)S;
-        for (int q = t; q < n; q += kFillThreads) {
-            const int i = l[q];
-            fill_node(i, Nx, Ny, U, 2);
-            if (F) fill_node(i, Nx, Ny, F, 4);
-        }
-        __syncthreads();
+}
+
+// k_axisfill: thread per node; bad nodes read only good nodes, so one pass suffices.
+__global__ __launch_bounds__(256) void k_axisfill(int Nx, int Ny, const int *bad, double *U,
+                                                  double *F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nx * Ny || !bad[i]) return;
+    const int x = i % Nx, y = i / Nx;
+    int src[4];
+    double w[4];
+    int q = x - 1;
+    while (q >= 0 && bad[y * Nx + q]) --q;
+    src[0] = q >= 0 ? y * Nx + q : -1; w[0] = q >= 0 ? 1.0 / (x - q) : 0.0;
+    q = x + 1;
+    while (q < Nx && bad[y * Nx + q]) ++q;
+    src[1] = q < Nx ? y * Nx + q : -1; w[1] = q < Nx ? 1.0 / (q - x) : 0.0;
+    q = y - 1;
+    while (q >= 0 && bad[q * Nx + x]) --q;
+    src[2] = q >= 0 ? q * Nx + x : -1; w[2] = q >= 0 ? 1.0 / (y - q) : 0.0;
+    q = y + 1;
+    while (q < Ny && bad[q * Nx + x]) ++q;
+    src[3] = q < Ny ? q * Nx + x : -1; w[3] = q < Ny ? 1.0 / (q - y) : 0.0;
+    double ws = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ws += w[k];
+    for (int c = 0; c < 2; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (src[k] >= 0) s += w[k] * U[2 * (size_t)src[k] + c];
+        U[2 * (size_t)i + c] = ws > 0 ? s / ws : 0.0;
     }
+    if (F)
+        for (int c = 0; c < 4; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (src[k] >= 0) s += w[k] * F[4 * (size_t)src[k] + c];
+            F[4 * (size_t)i + c] = ws > 0 ? s / ws : 0.0;
+        }
 }
 
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
-                          int *list, int sweeps, hipStream_t st)
+                          double *part, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_badfill, dim3(1), dim3(kFillThreads), 0, st, iters, S, Nx, Ny,
-                       use_mean_rule, Usrc, Fsrc, U, F, bad, list, sweeps);
+    int nb = (S + 255) / 256;
+    if (nb > 256) nb = 256;
+    const int nbf = (S + 255) / 256;
+    hipLaunchKernelGGL(k_itstats, dim3(nb), dim3(256), 0, st, iters, S, part);
+    hipLaunchKernelGGL(k_mark, dim3(nbf), dim3(256), 0, st, iters, S, (const double *)part, nb,
+                       use_mean_rule, Usrc, Fsrc, U, F, bad);
+    hipLaunchKernelGGL(k_axisfill, dim3(nbf), dim3(256), 0, st, Nx, Ny, (const int *)bad, U, F);
     return hipGetLastError();
 }
 

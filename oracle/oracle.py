@@ -236,32 +236,47 @@ def badpoint(iters, mean_rule):
     return bad
 
 
-def spring_fill(V, nc, bad, Nx, Ny, sweeps=12):
-    """Red-black sweeps: each bad node takes the mean of its non-NaN 4-neighbours
-    (neighbour order left, right, down, up, summed in that order)."""
-    G = V.copy().reshape(Ny, Nx, nc)
-    B = bad.reshape(Ny, Nx)
+def axis_fill(V, nc, bad, Nx, Ny):
+    """Fill bad nodes of an Nx x Ny node grid (x fastest, nc interleaved components):
+    each takes the inverse-distance average of the nearest good node along the four grid
+    directions (left, right, down, up, summed in that order); 0 when none exists."""
+    G = np.asarray(V, dtype=np.float64).reshape(Ny, Nx, nc)
+    B = np.asarray(bad, dtype=bool).reshape(Ny, Nx)
     yy, xx = np.mgrid[0:Ny, 0:Nx]
-    for k in range(2 * sweeps):
-        sel = B & (((xx + yy) & 1) == (k & 1))
-        s = np.zeros_like(G)
-        n = np.zeros(G.shape, np.int64)
-        P = np.pad(G, ((1, 1), (1, 1), (0, 0)), constant_values=np.nan)
-        for (dy, dx) in ((0, -1), (0, 1), (-1, 0), (1, 0)):
-            nb = P[1 + dy:1 + dy + Ny, 1 + dx:1 + dx + Nx]
-            ok = ~np.isnan(nb)
-            s = s + np.where(ok, nb, 0.0)
-            n = n + ok
-        upd = sel[:, :, None] & (n > 0)
-        G = np.where(upd, s / np.maximum(n, 1), G)
-    return G.reshape(-1)
+    A = np.where(~B, xx, -1)
+    L = np.concatenate([np.full((Ny, 1), -1), np.maximum.accumulate(A, axis=1)[:, :-1]], axis=1)
+    A = np.where(~B, xx, Nx)
+    R = np.concatenate([np.minimum.accumulate(A[:, ::-1], axis=1)[:, ::-1][:, 1:],
+                        np.full((Ny, 1), Nx)], axis=1)
+    A = np.where(~B, yy, -1)
+    D = np.concatenate([np.full((1, Nx), -1), np.maximum.accumulate(A, axis=0)[:-1, :]], axis=0)
+    A = np.where(~B, yy, Ny)
+    Up = np.concatenate([np.minimum.accumulate(A[::-1, :], axis=0)[::-1, :][1:, :],
+                         np.full((1, Nx), Ny)], axis=0)
+    srcs = []
+    for ok, idx_y, idx_x, dist in ((L >= 0, yy, np.clip(L, 0, Nx - 1), xx - L),
+                                   (R < Nx, yy, np.clip(R, 0, Nx - 1), R - xx),
+                                   (D >= 0, np.clip(D, 0, Ny - 1), xx, yy - D),
+                                   (Up < Ny, np.clip(Up, 0, Ny - 1), xx, Up - yy)):
+        w = np.where(ok, 1.0 / np.maximum(dist, 1), 0.0)
+        srcs.append((ok, w, G[idx_y, idx_x]))
+    ws = np.zeros((Ny, Nx))
+    s = np.zeros((Ny, Nx, nc))
+    for ok, w, v in srcs:
+        ws = ws + w
+        s = s + np.where(ok[:, :, None], w[:, :, None] * v, 0.0)
+    fillv = np.where(ws[:, :, None] > 0, s / np.where(ws > 0, ws, 1.0)[:, :, None], 0.0)
+    out = np.where(B[:, :, None], fillv, G)
+    return out.reshape(-1)
 
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-          cg_rtol=1e-10, fill_sweeps=12):
+          cg_rtol=1e-10):
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
     refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1 2-DOF IC-GN with
-    refill, Subpb2, strain, duals).  Returns the stages' outputs and times."""
+    refill, Subpb2, strain, duals).  The refill of bad nodes (the reference's
+    scatteredInterpolant 'natural', LocalICGN.m:144-161) is axis_fill; parity unpinned.
+    Returns the stages' outputs and times."""
     import time
     import scipy.sparse as sp
     T = {}
@@ -285,8 +300,8 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     U1 = U6.copy(); F1 = F6.copy()
     U1.reshape(-1, 2)[bad] = np.nan
     F1.reshape(-1, 4)[bad] = np.nan
-    U1 = spring_fill(U1, 2, bad, Nx, Ny, fill_sweeps)
-    F1 = spring_fill(F1, 4, bad, Nx, Ny, fill_sweeps)
+    U1 = axis_fill(U1, 2, bad, Nx, Ny)
+    F1 = axis_fill(F1, 4, bad, Nx, Ny)
     T["post6"] = time.perf_counter() - t
     el = []
     for l in range(Ny - 1):
@@ -314,7 +329,7 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
         U1, it2 = icgn2(f, g, gx, gy, xy, ws, H, mu, udual, vdual, U2, F2, tol)
         bad = badpoint(it2, False)
         U1.reshape(-1, 2)[bad] = np.nan
-        U1 = spring_fill(U1, 2, bad, Nx, Ny, fill_sweeps)
+        U1 = axis_fill(U1, 2, bad, Nx, Ny)
         F1 = F2
         T["icgn2"] += time.perf_counter() - t
         U2, F2, udual, vdual = global_step(U1, F1, udual, vdual)

@@ -217,3 +217,27 @@ def test_subpb2_distorted_mesh_jacobi_path(gpu_ctx, orc):
     Fg = gpu_ctx.nodal_strain(Uh)
     Fr = orc.nodal_strain(coords, el, Uh)
     assert np.max(np.abs(Fg - Fr)) < 1e-12 * max(1.0, np.abs(Fr).max())
+
+
+def test_frame_matches_oracle_frame_config1(gpu_ctx, cfg1, orc):
+    # whole device frame (prep, search, LocalICGN + bad-point refill, 1 + 3 ADMM global
+    # steps) against the FP64 restatement of the same frame
+    gpu_ctx.frame_stage(cfg1.f, cfg1.g)
+    S = gpu_ctx.frame_setup(20, 10, 8, [1, 256, 1, 256], tol=1e-2, mu=1e-3, beta=0.1,
+                            n_admm=3)
+    assert S == 361
+    gpu_ctx.frame_run()
+    out = gpu_ctx.frame_get()
+    o = orc.frame(cfg1.f, cfg1.g, 20, 10, 8, [1, 256, 1, 256], tol=1e-2, mu=1e-3, beta=0.1,
+                  n_admm=3)
+    assert np.max(np.abs(out["U0"] - o["U0"])) <= 1e-3 + 1e-9
+    same = out["iters6"] == o["iters6"]
+    assert same.mean() >= 0.97
+    if same.all():
+        # identical statuses -> identical bad-point sets -> the refilled frames agree
+        assert np.max(np.abs(out["U"] - o["U"])) < 1e-4
+        assert np.max(np.abs(out["F"] - o["F"])) < 1e-5
+    ut, vt = cfg1.ufun(o["xy"][:, 0], o["xy"][:, 1])
+    assert np.sqrt(np.mean((out["U"][0::2] - ut) ** 2)) < 0.05
+    t = gpu_ctx.frame_times()
+    assert t["frame"] > 0

@@ -386,3 +386,25 @@ def test_q4_mass_is_kronecker_of_1d_mass(orc):
         A[0, 0] = A[-1, -1] = 2.0
         return A * h / 6
     assert np.allclose(M, np.kron(M1(gy), M1(gx)), atol=1e-12)
+
+
+def test_axis_fill_rules(orc, pkg):
+    # isolated bad node -> mean of its 4 neighbours; distances weight longer gaps less;
+    # good nodes untouched; the host glue's copy computes the same values
+    Nx, Ny = 7, 5
+    rng = np.random.default_rng(2)
+    V = rng.normal(size=(Nx * Ny * 2))
+    bad = np.zeros(Nx * Ny, bool)
+    bad[2 * Nx + 3] = True                         # isolated
+    bad[[1 * Nx + 1, 1 * Nx + 2]] = True           # a pair in row 1
+    out = orc.axis_fill(V, 2, bad, Nx, Ny)
+    G = V.reshape(Ny, Nx, 2)
+    exp = (G[2, 2] + G[2, 4] + G[1, 3] + G[3, 3]) / 4
+    assert np.allclose(out.reshape(Ny, Nx, 2)[2, 3], exp, atol=1e-15)
+    # node (1,1): left (1,0) d=1, right (1,3) d=2, down (0,1) d=1, up (2,1) d=1
+    w = np.array([1, 0.5, 1, 1])
+    v = np.stack([G[1, 0], G[1, 3], G[0, 1], G[2, 1]])
+    assert np.allclose(out.reshape(Ny, Nx, 2)[1, 1], (w[:, None] * v).sum(0) / w.sum())
+    keep = ~bad
+    assert np.array_equal(out.reshape(-1, 2)[keep], V.reshape(-1, 2)[keep])
+    assert np.array_equal(pkg.aldic.axis_fill(V, 2, bad, Nx, Ny), out)
