@@ -98,12 +98,12 @@ struct aldic_ctx {
 namespace aldic {
 
 // ---- kernels' host launchers (all enqueue on ctx->stream) ---------------------
-constexpr int kStatParts = 256;  // per-block partials written by launch_stats
+constexpr int kStatParts = 1024;  // per-block partials written by launch_stats
 hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
                         hipStream_t st);
 hipError_t launch_prep(const void *rawf, const void *rawg, int dtype, int nx, int ny,
-                       const uint8_t *mask, const double *acc, double n_roi, float *f, float *g,
-                       float *gx, float *gy, hipStream_t st);
+                       const uint8_t *mask, const double *acc, double n_roi, double *ms,
+                       float *f, float *g, float *gx, float *gy, hipStream_t st);
 hipError_t launch_gradient(const float *f, int nx, int ny, float *gx, float *gy, hipStream_t st);
 
 struct IcgnArgs {

@@ -72,7 +72,8 @@ hipError_t run_prep(aldic_ctx *c)
     const size_t np = (size_t)c->nx * c->ny;
     hipError_t e;
     if ((e = c->f.ensure(4 * np)) || (e = c->g.ensure(4 * np)) || (e = c->gx.ensure(4 * np)) ||
-        (e = c->gy.ensure(4 * np)) || (e = c->stats.ensure(4 * kStatParts * sizeof(double))))
+        (e = c->gy.ensure(4 * np)) ||
+        (e = c->stats.ensure((4 * kStatParts + 4) * sizeof(double))))
         return e;
     double *acc = c->stats.as<double>();  // partials of f, then of g
     if ((e = launch_stats(c->rawf.p, c->raw_dtype, c->nx, c->ny, c->roi, acc, c->stream))) return e;
@@ -81,8 +82,9 @@ hipError_t run_prep(aldic_ctx *c)
         return e;
     const double nroi = (double)(c->roi[1] - c->roi[0] + 1) * (c->roi[3] - c->roi[2] + 1);
     return launch_prep(c->rawf.p, c->rawg.p, c->raw_dtype, c->nx, c->ny,
-                       c->have_mask ? c->mask.as<uint8_t>() : nullptr, acc, nroi, c->f.as<float>(),
-                       c->g.as<float>(), c->gx.as<float>(), c->gy.as<float>(), c->stream);
+                       c->have_mask ? c->mask.as<uint8_t>() : nullptr, acc, nroi,
+                       acc + 4 * kStatParts, c->f.as<float>(), c->g.as<float>(),
+                       c->gx.as<float>(), c->gy.as<float>(), c->stream);
 }
 
 int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,

@@ -19,27 +19,31 @@
 namespace aldic {
 
 struct SearchGeom {
-    int tw, sx, sy, nrx, nry, nstrip, DW, DH, K, rows_per;
-    int off_T, off_D, off_V, off_V2, off_P;  // float offsets into LDS
+    int tw, twp, sx, sy, nrx, nry, nstrip, nyp, DW, DH, K, rows_per;
+    int off_T, off_D, off_V, off_V2, off_P;  // float offsets into LDS (multiples of 4)
     int total;                               // floats
 };
+
+static int round4(int v) { return (v + 3) & ~3; }
 
 static SearchGeom search_geom(int ws, int sx, int sy)
 {
     SearchGeom G;
     G.tw = ws + 1;
+    G.twp = round4(G.tw);          // template row stride, zero-padded to a float4 multiple
     G.sx = sx;
     G.sy = sy;
     G.nrx = 2 * sx + 3;
     G.nry = 2 * sy + 3;
-    G.nstrip = (G.nrx + 3) / 4;
-    G.DW = G.nstrip * 4 + G.tw;
-    G.DH = G.tw + G.nry - 1;
+    G.nstrip = (G.nrx + 3) / 4;    // 4 x-lags per work item
+    G.nyp = (G.nry + 1) / 2;       // 2 y-lags per work item
+    G.DW = G.nstrip * 4 + G.twp;   // window row stride (multiple of 4)
+    G.DH = G.tw + G.nry;           // one spare zero row for an odd lag count
     // Split the template rows into K bands so the 256 threads stay busy: pick the K
     // with the fewest (rounds of 256 items) x (rows per band + 2), keeping the band
     // partials within a 64 KB LDS budget when possible.
-    const int base = G.nry * G.nstrip;
-    const int fixed = G.tw * G.tw + (G.tw + G.nry - 1) * G.DW + 2 * G.nry * G.DW;
+    const int base = G.nyp * G.nstrip;
+    const int fixed = G.tw * G.twp + G.DH * G.DW + 2 * round4(G.nry * G.DW);
     int bestK = 1;
     long best = -1;
     for (int K = 1; K <= G.tw; ++K) {
@@ -54,10 +58,10 @@ static SearchGeom search_geom(int ws, int sx, int sy)
     G.rows_per = (G.tw + bestK - 1) / bestK;
     G.K = (G.tw + G.rows_per - 1) / G.rows_per;
     G.off_T = 0;
-    G.off_D = G.off_T + G.tw * G.tw;
+    G.off_D = G.off_T + G.tw * G.twp;
     G.off_V = G.off_D + G.DH * G.DW;
-    G.off_V2 = G.off_V + G.nry * G.DW;
-    G.off_P = G.off_V2 + G.nry * G.DW;
+    G.off_V2 = G.off_V + round4(G.nry * G.DW);
+    G.off_P = G.off_V2 + round4(G.nry * G.DW);
     G.total = G.off_P + G.K * G.nrx * G.nry;
     return G;
 }
@@ -79,7 +83,7 @@ __device__ __forceinline__ double block_sum256(double v, double *red)
 
 __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
 {
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];
     __shared__ double red[4];
     __shared__ float rv[4];
     __shared__ int ri[4];
@@ -96,17 +100,19 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
     const int t = threadIdx.x;
 
     // template and its mean
+    const int twp = G.twp;
     double s = 0.0;
-    for (int p = t; p < tw * tw; p += 256) {
-        const float v = a.f[(size_t)(jj - 1 + p / tw) * nx + (ii - 1 + p % tw)];
+    for (int p = t; p < tw * twp; p += 256) {
+        const int r = p / twp, c = p % twp;
+        const float v = c < tw ? a.f[(size_t)(jj - 1 + r) * nx + (ii - 1 + c)] : 0.f;
         Tm[p] = v;
         s += v;
     }
     s = block_sum256(s, red);
     const double mt = s / (tw * tw);
     double st = 0.0;
-    for (int p = t; p < tw * tw; p += 256) {
-        const float d = (float)((double)Tm[p] - mt);
+    for (int p = t; p < tw * twp; p += 256) {
+        const float d = (p % twp) < tw ? (float)((double)Tm[p] - mt) : 0.f;  // padding stays 0
         Tm[p] = d;
         st += (double)d * d;
     }
@@ -148,37 +154,45 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
         V2[p] = s2;
     }
 
-    // numerator: item -> (ly fastest, strip, band)
-    const int nitems = G.nry * G.nstrip * G.K;
+    // numerator: item -> (y-lag pair fastest, strip of 4 x-lags, band of template rows).
+    // Per 4 template columns one float4 of T' and one float4 from each of the two window
+    // rows feed 32 FMAs (the 8-wide window slides in registers).
+    const int nitems = G.nyp * G.nstrip * G.K;
     for (int it = t; it < nitems; it += 256) {
-        const int ly = it % G.nry;
-        const int rest = it / G.nry;
+        const int lyp = it % G.nyp;
+        const int rest = it / G.nyp;
         const int strip = rest % G.nstrip, band = rest / G.nstrip;
-        const int lx0 = strip * 4;
+        const int lx0 = strip * 4, ly = 2 * lyp;
         const int j0 = band * G.rows_per;
         const int j1 = min(tw, j0 + G.rows_per);
-        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+        float c0[4] = {0.f, 0.f, 0.f, 0.f}, c1[4] = {0.f, 0.f, 0.f, 0.f};
         for (int j = j0; j < j1; ++j) {
-            const float *drow = Dp + (ly + j) * G.DW + lx0;
-            const float *trow = Tm + j * tw;
-            float d0 = drow[0], d1 = drow[1], d2 = drow[2], d3 = drow[3];
-            for (int i = 0; i < tw; ++i) {
-                const float tv = trow[i];
-                c0 += tv * d0;
-                c1 += tv * d1;
-                c2 += tv * d2;
-                c3 += tv * d3;
-                d0 = d1;
-                d1 = d2;
-                d2 = d3;
-                d3 = drow[i + 4];
+            const float4 *trow = reinterpret_cast<const float4 *>(Tm + j * twp);
+            const float4 *d0 = reinterpret_cast<const float4 *>(Dp + (ly + j) * G.DW + lx0);
+            const float4 *d1 = reinterpret_cast<const float4 *>(Dp + (ly + 1 + j) * G.DW + lx0);
+            float4 a0 = d0[0], b0 = d1[0];
+            for (int q = 0; q < twp / 4; ++q) {
+                const float4 tv = trow[q];
+                const float4 a1 = d0[q + 1], b1 = d1[q + 1];
+                c0[0] += tv.x * a0.x + tv.y * a0.y + tv.z * a0.z + tv.w * a0.w;
+                c0[1] += tv.x * a0.y + tv.y * a0.z + tv.z * a0.w + tv.w * a1.x;
+                c0[2] += tv.x * a0.z + tv.y * a0.w + tv.z * a1.x + tv.w * a1.y;
+                c0[3] += tv.x * a0.w + tv.y * a1.x + tv.z * a1.y + tv.w * a1.z;
+                c1[0] += tv.x * b0.x + tv.y * b0.y + tv.z * b0.z + tv.w * b0.w;
+                c1[1] += tv.x * b0.y + tv.y * b0.z + tv.z * b0.w + tv.w * b1.x;
+                c1[2] += tv.x * b0.z + tv.y * b0.w + tv.z * b1.x + tv.w * b1.y;
+                c1[3] += tv.x * b0.w + tv.y * b1.x + tv.z * b1.y + tv.w * b1.z;
+                a0 = a1;
+                b0 = b1;
             }
         }
-        float *pp = Pn + band * G.nrx * G.nry + ly * G.nrx;
-        if (lx0 + 0 < G.nrx) pp[lx0 + 0] = c0;
-        if (lx0 + 1 < G.nrx) pp[lx0 + 1] = c1;
-        if (lx0 + 2 < G.nrx) pp[lx0 + 2] = c2;
-        if (lx0 + 3 < G.nrx) pp[lx0 + 3] = c3;
+        float *pp = Pn + band * G.nrx * G.nry;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            if (lx0 + l >= G.nrx) continue;
+            pp[ly * G.nrx + lx0 + l] = c0[l];
+            if (ly + 1 < G.nry) pp[(ly + 1) * G.nrx + lx0 + l] = c1[l];
+        }
     }
     __syncthreads();
 
