@@ -37,7 +37,9 @@ static SearchGeom search_geom(int ws, int sx, int sy)
     G.nry = 2 * sy + 3;
     G.nstrip = (G.nrx + 3) / 4;    // 4 x-lags per work item
     G.nyp = (G.nry + 1) / 2;       // 2 y-lags per work item
-    G.DW = G.nstrip * 4 + G.twp;   // window row stride (multiple of 4)
+    G.DW = G.nstrip * 4 + G.twp;   // window row stride: a multiple of 4 floats, and an
+    if ((G.DW / 4) % 2 == 0) G.DW += 4;  // odd number of float4s so that neighbouring lanes'
+                                         // rows fall on different LDS banks
     G.DH = G.tw + G.nry;           // one spare zero row for an odd lag count
     // Split the template rows into K bands so the 256 threads stay busy: pick the K
     // with the fewest (rounds of 256 items) x (rows per band + 2), keeping the band
@@ -162,14 +164,14 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
         const int lyp = it % G.nyp;
         const int rest = it / G.nyp;
         const int strip = rest % G.nstrip, band = rest / G.nstrip;
-        const int lx0 = strip * 4, ly = 2 * lyp;
+        const int lx0 = strip * 4, ly = lyp, ly2 = lyp + G.nyp;  // the item's two y-lags
         const int j0 = band * G.rows_per;
         const int j1 = min(tw, j0 + G.rows_per);
         float c0[4] = {0.f, 0.f, 0.f, 0.f}, c1[4] = {0.f, 0.f, 0.f, 0.f};
         for (int j = j0; j < j1; ++j) {
             const float4 *trow = reinterpret_cast<const float4 *>(Tm + j * twp);
             const float4 *d0 = reinterpret_cast<const float4 *>(Dp + (ly + j) * G.DW + lx0);
-            const float4 *d1 = reinterpret_cast<const float4 *>(Dp + (ly + 1 + j) * G.DW + lx0);
+            const float4 *d1 = reinterpret_cast<const float4 *>(Dp + (ly2 + j) * G.DW + lx0);
             float4 a0 = d0[0], b0 = d1[0];
             for (int q = 0; q < twp / 4; ++q) {
                 const float4 tv = trow[q];
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
         for (int l = 0; l < 4; ++l) {
             if (lx0 + l >= G.nrx) continue;
             pp[ly * G.nrx + lx0 + l] = c0[l];
-            if (ly + 1 < G.nry) pp[(ly + 1) * G.nrx + lx0 + l] = c1[l];
+            if (ly2 < G.nry) pp[ly2 * G.nrx + lx0 + l] = c1[l];
         }
     }
     __syncthreads();

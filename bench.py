@@ -33,6 +33,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of nbytes (hipMemcpy's copy kernel): the reference
+    point for scaling the FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: {nbytes} bytes device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +56,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -58,6 +76,8 @@ def main():
     n_admm = 3
 
     ctx = pkg.Context(0)
+    if args.copy_calib:
+        copy_calibration(256 << 20)
     ctx.frame_stage(pair.f, pair.g)
     S = ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
                         cg_maxit=2000, cg_rtol=1e-10)
