@@ -7,132 +7,241 @@
 //   findpeak.m:1-68            first max in column-major order (x-lag fastest),
 //                              3x3 quadratic fit off the crop edge, 1e-3 rounding
 //
-// One 256-thread workgroup per subset.  LDS holds the mean-free template T'
-// ((w+1)^2), the search window D with a one-pixel zero border (the partial-overlap
-// lags of the crop edge), vertical box sums of D and D^2 and the lag map.  Each
-// work item accumulates 4 neighbouring x-lags over a band of template rows, sliding
-// a 4-wide register window along the D row so every T' value feeds 4 FMAs; bands
-// are summed in a fixed order (deterministic argmax).
+// One 256-thread workgroup per subset.  LDS holds the mean-free template T', the
+// search window D with a one-pixel zero border (the partial-overlap lags of the crop
+// edge), the per-band numerator partials and the window box sums.
+//
+// Numerator: a work item is (task group, band of template rows); each of its 64 lanes
+// owns X neighbouring x-lags of one y-lag and slides an X+3 register window along the
+// D row, so one D load feeds X FMAs and one T' float4 (the same address in every lane,
+// an LDS broadcast) feeds 4X.  Lanes are numbered y-lag fastest (task = s nry + ly)
+// and the D row stride is R = k (mod 64) with k = X nry^-1 (odd): the word lane L
+// reads is then k L (mod 64), so the 64 lanes of a ds_read_b32 hit 64 different banks.
+// Box sums of D and D^2 per lag run as FP64 running sums in wave 0, beside the
+// numerator.  Bands are added in a fixed order, so the argmax is deterministic.
 #include "aldic_internal.h"
 #include "common.h"
 
 namespace aldic {
 
+namespace {
+
 struct SearchGeom {
-    int tw, twp, sx, sy, nrx, nry, nstrip, nyp, DW, DH, K, rows_per;
-    int off_T, off_D, off_V, off_V2, off_P;  // float offsets into LDS (multiples of 4)
-    int total;                               // floats
+    int tw, twp, nq, rem;         // template width, padded width (x4), full 4-col steps, rest
+    int sx, sy, nrx, nry, nl;     // search radii, lag counts, lags
+    int X, nstrip, ngroups, nb, rows_per, nitems;
+    int R, DH, RW, VW;            // window row stride / rows / used columns; box-sum columns
+    int off_T, off_D, off_P, off_V, off_S;  // float offsets into LDS (multiples of 4)
+    int total;                    // floats
 };
 
-static int round4(int v) { return (v + 3) & ~3; }
+int round4(int v) { return (v + 3) & ~3; }
 
-static SearchGeom search_geom(int ws, int sx, int sy)
+// Lays out one geometry for lag block width X (odd); returns false if it does not fit.
+bool geom_for(int ws, int sx, int sy, int X, SearchGeom &G, long &cost)
 {
-    SearchGeom G;
     G.tw = ws + 1;
-    G.twp = round4(G.tw);          // template row stride, zero-padded to a float4 multiple
+    G.twp = round4(G.tw);
+    G.nq = G.tw / 4;
+    G.rem = G.tw % 4;
     G.sx = sx;
     G.sy = sy;
     G.nrx = 2 * sx + 3;
     G.nry = 2 * sy + 3;
-    G.nstrip = (G.nrx + 3) / 4;    // 4 x-lags per work item
-    G.nyp = (G.nry + 1) / 2;       // 2 y-lags per work item
-    G.DW = G.nstrip * 4 + G.twp;   // window row stride: a multiple of 4 floats, and an
-    if ((G.DW / 4) % 2 == 0) G.DW += 4;  // odd number of float4s so that neighbouring lanes'
-                                         // rows fall on different LDS banks
-    G.DH = G.tw + G.nry;           // one spare zero row for an odd lag count
-    // Split the template rows into K bands so the 256 threads stay busy: pick the K
-    // with the fewest (rounds of 256 items) x (rows per band + 2), keeping the band
-    // partials within a 64 KB LDS budget when possible.
-    const int base = G.nyp * G.nstrip;
-    const int fixed = G.tw * G.twp + G.DH * G.DW + 2 * round4(G.nry * G.DW);
-    int bestK = 1;
+    G.nl = G.nrx * G.nry;
+    G.X = X;
+    G.nstrip = (G.nrx + X - 1) / X;
+    const int ntask = G.nstrip * G.nry;
+    G.ngroups = (ntask + 63) / 64;
+    // bands: fewest (rounds of 4 waves) x (rows per band)
     long best = -1;
-    for (int K = 1; K <= G.tw; ++K) {
-        const int rows = (G.tw + K - 1) / K;
-        const int Keff = (G.tw + rows - 1) / rows;
-        const long lds = 4L * (fixed + (long)Keff * G.nrx * G.nry);
-        if (K > 1 && lds > 64 * 1024) break;
-        const long rounds = (base * (long)Keff + 255) / 256;
-        const long cost = rounds * (rows + 2);
-        if (best < 0 || cost < best) { best = cost; bestK = K; }
+    int bestnb = 1;
+    for (int nb = 1; nb <= G.tw && nb <= 32; ++nb) {
+        const int rows = (G.tw + nb - 1) / nb;
+        const int nbe = (G.tw + rows - 1) / rows;
+        const long c = (long)((G.ngroups * nbe + 3) / 4) * (rows * G.tw + X) * X;
+        if (best < 0 || c < best) { best = c; bestnb = nbe; }
     }
-    G.rows_per = (G.tw + bestK - 1) / bestK;
-    G.K = (G.tw + G.rows_per - 1) / G.rows_per;
+    G.rows_per = (G.tw + bestnb - 1) / bestnb;
+    G.nb = (G.tw + G.rows_per - 1) / G.rows_per;
+    G.nitems = G.ngroups * G.nb;
+    cost = best;
+    // window: rows ly + j, columns X*s + (0 .. X + tw - 2)
+    G.DH = G.nry + G.tw - 1;
+    G.RW = X * G.nstrip + G.tw - 1;
+    G.VW = G.nrx + G.tw - 1;
+    int inv = 1;
+    while ((inv * G.nry) % 64 != 1) inv += 2;  // nry is odd
+    const int c = (X * inv) % 64;
+    G.R = G.RW + (((c - G.RW) % 64) + 64) % 64;
     G.off_T = 0;
     G.off_D = G.off_T + G.tw * G.twp;
-    G.off_V = G.off_D + G.DH * G.DW;
-    G.off_V2 = G.off_V + round4(G.nry * G.DW);
-    G.off_P = G.off_V2 + round4(G.nry * G.DW);
-    G.total = G.off_P + G.K * G.nrx * G.nry;
-    return G;
+    G.off_P = round4(G.off_D + G.DH * G.R);
+    G.off_V = round4(G.off_P + G.nb * G.nl);
+    G.off_S = round4(G.off_V + 2 * G.nry * G.VW);
+    G.total = G.off_S + 2 * G.nl;
+    return sizeof(float) * (size_t)G.total <= 160 * 1024 - 1024;
 }
+
+SearchGeom search_geom(int ws, int sx, int sy, bool &ok)
+{
+    SearchGeom best{};
+    long bc = -1;
+    ok = false;
+    for (int X : {7, 5, 9, 3}) {
+        SearchGeom G;
+        long c;
+        if (!geom_for(ws, sx, sy, X, G, c)) continue;
+        if (bc < 0 || c < bc) { bc = c; best = G; }
+        ok = true;
+    }
+    return best;
+}
+
+}  // namespace
 
 size_t search_lds_bytes(int ws, int sx, int sy)
 {
-    return sizeof(float) * (size_t)search_geom(ws, sx, sy).total + 64;
+    bool ok;
+    const SearchGeom G = search_geom(ws, sx, sy, ok);
+    return ok ? sizeof(float) * (size_t)G.total + 64 : (size_t)1 << 30;
 }
 
-__device__ __forceinline__ double block_sum256(double v, double *red)
+// One 4-column step of the register-window correlation.  w is a 12-entry ring of D
+// values indexed by column mod 12 (the caller keeps every block start a multiple of 12
+// columns); O is the step's column offset inside the block, NC its real columns.
+template <int X, int O, int NC>
+__device__ __forceinline__ void corr_step(float (&w)[12], float (&acc)[X], const float *dr,
+                                          const float4 *t4)
 {
-    v = wave_sum(v);
-    const int wid = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[wid] = v;
-    __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) w[(O + X - 1 + k) % 12] = dr[O + X - 1 + k];
+    const float4 tv = t4[O / 4];
+    const float tc[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int x = 0; x < X; ++x) acc[x] = fmaf(tc[c], w[(O + x + c) % 12], acc[x]);
 }
 
+template <int X, int O>
+__device__ __forceinline__ void corr_rest(int rem, float (&w)[12], float (&acc)[X],
+                                          const float *dr, const float4 *t4)
+{
+    if (rem == 1) corr_step<X, O, 1>(w, acc, dr, t4);
+    else if (rem == 2) corr_step<X, O, 2>(w, acc, dr, t4);
+    else if (rem == 3) corr_step<X, O, 3>(w, acc, dr, t4);
+}
+
+// acc[x] += sum_i T'[i] * D[x + i] over one template row (tw = 4 nq + rem columns).
+template <int X>
+__device__ __forceinline__ void corr_row(float (&acc)[X], const float *dr, const float *tr,
+                                         int nq, int rem)
+{
+    float w[12];
+#pragma unroll
+    for (int k = 0; k < X - 1; ++k) w[k] = dr[k];
+    const float4 *t4 = reinterpret_cast<const float4 *>(tr);
+    int q = 0;
+    for (; q + 3 <= nq; q += 3) {
+        corr_step<X, 0, 4>(w, acc, dr, t4);
+        corr_step<X, 4, 4>(w, acc, dr, t4);
+        corr_step<X, 8, 4>(w, acc, dr, t4);
+        dr += 12;
+        t4 += 3;
+    }
+    const int tail = nq - q;
+    if (tail == 0) {
+        corr_rest<X, 0>(rem, w, acc, dr, t4);
+    } else if (tail == 1) {
+        corr_step<X, 0, 4>(w, acc, dr, t4);
+        corr_rest<X, 4>(rem, w, acc, dr, t4);
+    } else {
+        corr_step<X, 0, 4>(w, acc, dr, t4);
+        corr_step<X, 4, 4>(w, acc, dr, t4);
+        corr_rest<X, 8>(rem, w, acc, dr, t4);
+    }
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int X>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
 {
     extern __shared__ __align__(16) float lds[];
-    __shared__ double red[4];
+    __shared__ double redd[4][2];
+    __shared__ float redf[4][2];
     __shared__ float rv[4];
     __shared__ int ri[4];
     float *Tm = lds + G.off_T;
     float *Dp = lds + G.off_D;
-    float *V = lds + G.off_V;
-    float *V2 = lds + G.off_V2;
     float *Pn = lds + G.off_P;
+    float *V1 = lds + G.off_V;
+    float *V2 = V1 + G.nry * G.VW;
+    float *S1 = lds + G.off_S;
+    float *S2 = S1 + G.nl;
 
-    const int q = blockIdx.x;
+    // subsets in contiguous runs per XCD (workgroups are dealt to the 8 XCDs in turn),
+    // so neighbouring windows share an L2
+    const int S = a.Nx * a.Ny;
+    const int b = blockIdx.x, xcd = b & 7, per = S >> 3, rm = S & 7;
+    const int q = xcd * per + min(xcd, rm) + (b >> 3);
     const int k = q % a.Nx, l = q / a.Nx;
     const int ii = a.x1 + k * a.step, jj = a.y1 + l * a.step;  // 1-based corner
-    const int tw = G.tw, nx = a.nx;
-    const int t = threadIdx.x;
+    const int tw = G.tw, twp = G.twp, nx = a.nx;
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
 
-    // template and its mean
-    const int twp = G.twp;
-    double s = 0.0;
-    for (int p = t; p < tw * twp; p += 256) {
-        const int r = p / twp, c = p % twp;
-        const float v = c < tw ? a.f[(size_t)(jj - 1 + r) * nx + (ii - 1 + c)] : 0.f;
-        Tm[p] = v;
-        s += v;
+    // template (raw) and its statistics; window with a zero border:
+    //   Dp[r][c] = g(ii - sx - 1 + c, jj - sy - 1 + r)
+    double s1 = 0.0, s2 = 0.0;
+    float mn = INFINITY, mx = -INFINITY;
+    for (int r = wv; r < tw; r += 4) {
+        const float *src = a.f + (size_t)(jj - 1 + r) * nx + (ii - 1);
+        for (int c = lane; c < twp; c += 64) {
+            float v = 0.f;
+            if (c < tw) {
+                v = src[c];
+                s1 += v;
+                s2 += (double)v * v;
+                mn = fminf(mn, v);
+                mx = fmaxf(mx, v);
+            }
+            Tm[r * twp + c] = v;
+        }
     }
-    s = block_sum256(s, red);
-    const double mt = s / (tw * tw);
-    double st = 0.0;
-    for (int p = t; p < tw * twp; p += 256) {
-        const float d = (p % twp) < tw ? (float)((double)Tm[p] - mt) : 0.f;  // padding stays 0
-        Tm[p] = d;
-        st += (double)d * d;
-    }
-    st = block_sum256(st, red);
-
-    // window with a zero border: Dp[r][c] = g(ii - sx - 1 + c, jj - sy - 1 + r)
     const int Dx = tw + 2 * G.sx, Dy = tw + 2 * G.sy;
-    for (int p = t; p < G.DH * G.DW; p += 256) {
-        const int r = p / G.DW, c = p % G.DW;
-        float v = 0.f;
-        if (r >= 1 && r <= Dy && c >= 1 && c <= Dx)
-            v = a.g[(size_t)(jj - G.sy - 2 + r) * nx + (ii - G.sx - 2 + c)];
-        Dp[p] = v;
+    for (int r = wv; r < G.DH; r += 4) {
+        const bool rin = r >= 1 && r <= Dy;
+        const float *src = a.g + (size_t)(jj - G.sy - 2 + r) * nx + (ii - G.sx - 2);
+        for (int c = lane; c < G.RW; c += 64)
+            Dp[r * G.R + c] = (rin && c >= 1 && c <= Dx) ? src[c] : 0.f;
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, m, 64));
+        mx = fmaxf(mx, __shfl_xor(mx, m, 64));
+    }
+    if (lane == 0) {
+        redd[wv][0] = s1;
+        redd[wv][1] = s2;
+        redf[wv][0] = mn;
+        redf[wv][1] = mx;
     }
     __syncthreads();
+    s1 = (redd[0][0] + redd[1][0]) + (redd[2][0] + redd[3][0]);
+    s2 = (redd[0][1] + redd[1][1]) + (redd[2][1] + redd[3][1]);
+    mn = fminf(fminf(redf[0][0], redf[1][0]), fminf(redf[2][0], redf[3][0]));
+    mx = fmaxf(fmaxf(redf[0][1], redf[1][1]), fmaxf(redf[2][1], redf[3][1]));
 
     const double xc = (ii + ii + a.ws) / 2.0, yc = (jj + jj + a.ws) / 2.0;
-    if (st == 0.0) {  // constant template: normxcorr2 raises (funIntegerSearch.m:153-157)
+    if (mn == mx) {  // constant template: normxcorr2 raises (funIntegerSearch.m:153-157)
         if (t == 0) {
             a.x[q] = xc; a.y[q] = yc;
             a.u[q] = a.v[q] = a.ccmax[q] = NAN;
@@ -142,91 +251,93 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
         }
         return;
     }
+    const double n = (double)tw * tw;
+    const double mt = s1 / n;
+    const double st = fmax(s2 - s1 * mt, 0.0);
+    for (int r = wv; r < tw; r += 4)
+        for (int c = lane; c < tw; c += 64) Tm[r * twp + c] = (float)((double)Tm[r * twp + c] - mt);
+    __syncthreads();
 
-    // vertical box sums over tw rows, for every lag row and column
-    for (int p = t; p < G.nry * G.DW; p += 256) {
-        const int ly = p / G.DW, c = p % G.DW;
-        float s1 = 0.f, s2 = 0.f;
-        for (int j = 0; j < tw; ++j) {
-            const float d = Dp[(ly + j) * G.DW + c];
-            s1 += d;
-            s2 += d * d;
-        }
-        V[p] = s1;
-        V2[p] = s2;
-    }
-
-    // numerator: item -> (y-lag pair fastest, strip of 4 x-lags, band of template rows).
-    // Per 4 template columns one float4 of T' and one float4 from each of the two window
-    // rows feed 32 FMAs (the 8-wide window slides in registers).
-    const int nitems = G.nyp * G.nstrip * G.K;
-    for (int it = t; it < nitems; it += 256) {
-        const int lyp = it % G.nyp;
-        const int rest = it / G.nyp;
-        const int strip = rest % G.nstrip, band = rest / G.nstrip;
-        const int lx0 = strip * 4, ly = lyp, ly2 = lyp + G.nyp;  // the item's two y-lags
-        const int j0 = band * G.rows_per;
-        const int j1 = min(tw, j0 + G.rows_per);
-        float c0[4] = {0.f, 0.f, 0.f, 0.f}, c1[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int j = j0; j < j1; ++j) {
-            const float4 *trow = reinterpret_cast<const float4 *>(Tm + j * twp);
-            const float4 *d0 = reinterpret_cast<const float4 *>(Dp + (ly + j) * G.DW + lx0);
-            const float4 *d1 = reinterpret_cast<const float4 *>(Dp + (ly2 + j) * G.DW + lx0);
-            float4 a0 = d0[0], b0 = d1[0];
-            for (int q = 0; q < twp / 4; ++q) {
-                const float4 tv = trow[q];
-                const float4 a1 = d0[q + 1], b1 = d1[q + 1];
-                c0[0] += tv.x * a0.x + tv.y * a0.y + tv.z * a0.z + tv.w * a0.w;
-                c0[1] += tv.x * a0.y + tv.y * a0.z + tv.z * a0.w + tv.w * a1.x;
-                c0[2] += tv.x * a0.z + tv.y * a0.w + tv.z * a1.x + tv.w * a1.y;
-                c0[3] += tv.x * a0.w + tv.y * a1.x + tv.z * a1.y + tv.w * a1.z;
-                c1[0] += tv.x * b0.x + tv.y * b0.y + tv.z * b0.z + tv.w * b0.w;
-                c1[1] += tv.x * b0.y + tv.y * b0.z + tv.z * b0.w + tv.w * b1.x;
-                c1[2] += tv.x * b0.z + tv.y * b0.w + tv.z * b1.x + tv.w * b1.y;
-                c1[3] += tv.x * b0.w + tv.y * b1.x + tv.z * b1.y + tv.w * b1.z;
-                a0 = a1;
-                b0 = b1;
+    if (wv == 0) {
+        // vertical running sums of D and D^2 over tw rows, one column per lane (FP64)
+        for (int c = lane; c < G.VW; c += 64) {
+            double v1 = 0.0, v2 = 0.0;
+            for (int j = 0; j < tw; ++j) {
+                const double d = Dp[j * G.R + c];
+                v1 += d;
+                v2 += d * d;
+            }
+            V1[c] = (float)v1;
+            V2[c] = (float)v2;
+            for (int ly = 1; ly < G.nry; ++ly) {
+                const double dout = Dp[(ly - 1) * G.R + c], din = Dp[(ly - 1 + tw) * G.R + c];
+                v1 += din - dout;
+                v2 += din * din - dout * dout;
+                V1[ly * G.VW + c] = (float)v1;
+                V2[ly * G.VW + c] = (float)v2;
             }
         }
-        float *pp = Pn + band * G.nrx * G.nry;
+        wave_sync_lds();
+        // horizontal running sums of the vertical sums -> box sums per lag
+        for (int idx = lane; idx < 2 * G.nry; idx += 64) {
+            const int qn = idx >= G.nry, ly = idx - qn * G.nry;
+            const float *vr = (qn ? V2 : V1) + ly * G.VW;
+            float *so = (qn ? S2 : S1) + ly * G.nrx;
+            double h = 0.0;
+            for (int i = 0; i < tw; ++i) h += vr[i];
+            so[0] = (float)h;
+            for (int lx = 1; lx < G.nrx; ++lx) {
+                h += (double)vr[lx - 1 + tw] - (double)vr[lx - 1];
+                so[lx] = (float)h;
+            }
+        }
+    }
+
+    // numerator items; wave 0 (box sums) takes the last, shortest band
+    for (int it = G.nitems - 1 - wv; it >= 0; it -= 4) {
+        const int grp = it % G.ngroups, band = it / G.ngroups;
+        const int tau = grp * 64 + lane;
+        int s = tau / G.nry;
+        const int ly = tau - s * G.nry;
+        const bool act = s < G.nstrip;
+        if (!act) s = G.nstrip - 1;  // in-range reads, result dropped
+        const int j0 = band * G.rows_per, j1 = min(tw, j0 + G.rows_per);
+        float acc[X];
 #pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            if (lx0 + l >= G.nrx) continue;
-            pp[ly * G.nrx + lx0 + l] = c0[l];
-            if (ly2 < G.nry) pp[ly2 * G.nrx + lx0 + l] = c1[l];
+        for (int x = 0; x < X; ++x) acc[x] = 0.f;
+        for (int j = j0; j < j1; ++j)
+            corr_row<X>(acc, Dp + (ly + j) * G.R + X * s, Tm + j * twp, G.nq, G.rem);
+        if (act) {
+            float *pp = Pn + band * G.nl + ly * G.nrx;
+#pragma unroll
+            for (int x = 0; x < X; ++x)
+                if (X * s + x < G.nrx) pp[X * s + x] = acc[x];
         }
     }
     __syncthreads();
 
     // NCC per lag, then argmax (ties -> smallest column-major index)
-    const int nl = G.nrx * G.nry;
-    const float n = (float)(tw * tw);
-    const float stf = (float)st;
+    const int nl = G.nl;
     float best = -INFINITY;
     int bi = 0x7fffffff;
     for (int p = t; p < nl; p += 256) {
-        const int ly = p / G.nrx, lx = p % G.nrx;
-        float num = 0.f;
-        for (int b = 0; b < G.K; ++b) num += Pn[b * nl + p];
-        float s1 = 0.f, s2 = 0.f;
-        const float *v1 = V + ly * G.DW + lx;
-        const float *v2 = V2 + ly * G.DW + lx;
-        for (int i = 0; i < tw; ++i) { s1 += v1[i]; s2 += v2[i]; }
-        float var = s2 - s1 * s1 / n;
-        var = var > 0.f ? var : 0.f;
-        const float den = sqrtf(stf * var);
-        const float c = den > 0.f ? num / den : 0.f;
-        Pn[p] = c;  // band 0 slot now holds the NCC value of lag p
+        double num = 0.0;
+        for (int bb = 0; bb < G.nb; ++bb) num += Pn[bb * nl + p];
+        const double b1 = S1[p], b2 = S2[p];
+        double var = b2 - b1 * b1 / n;
+        var = var > 0.0 ? var : 0.0;
+        const double den = sqrt(st * var);
+        const float c = den > 0.0 ? (float)(num / den) : 0.f;
         if (c > best || (c == best && p < bi)) { best = c; bi = p; }
+        Pn[p] = c;  // band 0 slot now holds the NCC value of lag p (only this thread read it)
     }
-    // wave argmax
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
         const float ob = __shfl_xor(best, m, 64);
         const int oi = __shfl_xor(bi, m, 64);
         if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
     }
-    if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
+    if (lane == 0) { rv[wv] = best; ri[wv] = bi; }
     __syncthreads();
     if (t != 0) return;
     for (int w = 1; w < 4; ++w)
@@ -268,21 +379,33 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
     if (a.U0) { a.U0[2 * q] = u; a.U0[2 * q + 1] = v; }
 }
 
+template <int X>
+hipError_t launch_x(const SearchArgs &a, const SearchGeom &G, int S, hipStream_t st)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)k_search<X>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_search<X>, dim3(S), dim3(256), sizeof(float) * (size_t)G.total, st, a,
+                       G);
+    return hipGetLastError();
+}
+
 hipError_t launch_search(const SearchArgs &a, hipStream_t st)
 {
     const int S = a.Nx * a.Ny;
     if (S <= 0) return hipSuccess;
-    const SearchGeom G = search_geom(a.ws, a.sx, a.sy);
-    const size_t lds = sizeof(float) * (size_t)G.total;
-    if (lds > 160 * 1024 - 256) return hipErrorInvalidValue;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)k_search,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
-        attr_set = true;
+    bool ok;
+    const SearchGeom G = search_geom(a.ws, a.sx, a.sy, ok);
+    if (!ok) return hipErrorInvalidValue;
+    switch (G.X) {
+    case 3: return launch_x<3>(a, G, S, st);
+    case 5: return launch_x<5>(a, G, S, st);
+    case 7: return launch_x<7>(a, G, S, st);
+    default: return launch_x<9>(a, G, S, st);
     }
-    hipLaunchKernelGGL(k_search, dim3(S), dim3(256), lds, st, a, G);
-    return hipGetLastError();
 }
 
 }  // namespace aldic
