@@ -68,7 +68,7 @@ struct FrameState {
     aldic_frame_opts o{};
     int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
     DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
-        bad, tmp, part;
+        bad, tmp, part, cc;
     std::vector<double> hcoords;
     std::vector<int> helems;
     float ms[8] = {0};
@@ -98,7 +98,10 @@ struct aldic_ctx {
 namespace aldic {
 
 // ---- kernels' host launchers (all enqueue on ctx->stream) ---------------------
-constexpr int kStatParts = 1024;  // per-block partials written by launch_stats
+constexpr int kStatParts = 1024;
+// Largest image: FP32 planes are addressed with 32-bit byte offsets in the kernels,
+// and row offsets (4 nx bytes) with 24-bit multiplies, so nx < 2^22.
+constexpr size_t kMaxPixels = (size_t)1 << 29;  // per-block partials written by launch_stats
 hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
                         hipStream_t st);
 hipError_t launch_prep(const void *rawf, const void *rawg, int dtype, int nx, int ny,
@@ -123,6 +126,10 @@ struct IcgnArgs {
     const int *list;        // subset indices to process (nullptr = all S)
     int nlist;
     const int *nlist_dev;   // when set, the list length is read from the device
+    // per-subset constants shared by the 6-DOF and 2-DOF runs on the same windows
+    // (8 doubles per subset); cc_mode 0: unused, 1: written by 6-DOF, 2: read by 2-DOF
+    double *cc;
+    int cc_mode;
 };
 hipError_t launch_icgn6(const IcgnArgs &a, int ppl, hipStream_t st);
 hipError_t launch_icgn2(const IcgnArgs &a, int ppl, hipStream_t st);

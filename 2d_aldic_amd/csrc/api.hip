@@ -91,6 +91,8 @@ int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int 
               const uint8_t *mask, const int *roi)
 {
     if (!f || !g || nx < 8 || ny < 8) return fail(ctx, ALDIC_ERR_ARG, "images: bad arguments");
+    if ((size_t)nx * ny > kMaxPixels || nx >= (1 << 22))
+        return fail(ctx, ALDIC_ERR_ARG, "images: more than 2^29 pixels or 2^22 columns");
     const size_t es = dtype_size(dtype);
     if (!es) return fail(ctx, ALDIC_ERR_ARG, "images: unknown dtype");
     const size_t np = (size_t)nx * ny;
@@ -218,7 +220,7 @@ void aldic_destroy(aldic_ctx *ctx)
     FrameState &F = ctx->fr;
     DBuf *fb[] = {&F.coords, &F.U0, &F.U6, &F.F6, &F.H21, &F.it6, &F.zncc, &F.Ucur, &F.Fcur,
                   &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
-                  &F.part};
+                  &F.part, &F.cc};
     for (DBuf *b : fb) b->release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -245,6 +247,8 @@ int aldic_set_normalized(aldic_ctx *ctx, const double *f, const double *g, int n
 {
     if (!ctx) return ALDIC_ERR_ARG;
     if (!f || !g || nx < 8 || ny < 8) return fail(ctx, ALDIC_ERR_ARG, "set_normalized: bad arguments");
+    if ((size_t)nx * ny > kMaxPixels || nx >= (1 << 22))
+        return fail(ctx, ALDIC_ERR_ARG, "set_normalized: more than 2^29 pixels or 2^22 columns");
     CK(hipSetDevice(ctx->device));
     const size_t np = (size_t)nx * ny;
     std::vector<float> tmp(np);
@@ -699,6 +703,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.it2.ensure(sizeof(int) * S));
     CK(F.bad.ensure(sizeof(int) * S));
     CK(F.part.ensure(3 * 256 * sizeof(double)));
+    CK(F.cc.ensure(8 * sizeof(double) * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
     CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
@@ -774,6 +779,8 @@ int aldic_frame_run(aldic_ctx *ctx)
     a.zncc = F.zncc.as<double>();
     a.iters = F.it6.as<int>();
     a.nlist = S;
+    a.cc = F.cc.as<double>();
+    a.cc_mode = 1;
     const int ppl = icgn_ppl_for(o.ws);
     CK(launch_icgn6(a, ppl, st));
     CK(hipEventRecord(ev[3], st));
@@ -824,6 +831,7 @@ int aldic_frame_run(aldic_ctx *ctx)
         a2.F = nullptr;
         a2.H21 = nullptr;
         a2.zncc = nullptr;
+        a2.cc_mode = 2;
         CK(launch_icgn2(a2, ppl, st));
         CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
                           nullptr, F.bad.as<int>(), F.part.as<double>(), st));

@@ -25,6 +25,14 @@ __device__ __forceinline__ double wave_sum(double v)
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// Orders this wavefront's LDS writes before its later LDS reads from other lanes.
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Keys (1981) cubic convolution weights, a = -1/2, for the taps at
 // floor-1, floor, floor+1, floor+2 of a sample with fractional part d.
 template <class T>

@@ -204,11 +204,15 @@ __device__ __forceinline__ float sample_g(const float *__restrict__ g, int nx, i
     keys_weights(ax - fx, wx0, wx1, wx2, wx3);
     keys_weights(ay - fy, wy0, wy1, wy2, wy3);
     const int c0 = X + (int)fx - 2, r0 = Y + (int)fy - 2;  // 0-based first tap
-    const float *q = g + (size_t)r0 * nx + c0;
+    // 32-bit byte offsets from the uniform image base (images hold < 2^29 pixels,
+    // rows < 2^24: the 24-bit multiply is exact)
+    const char *gb = reinterpret_cast<const char *>(g);
+    const unsigned rs = 4u * (unsigned)nx;
+    const unsigned o = __umul24((unsigned)r0, rs) + ((unsigned)c0 << 2);
     float r[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const float *row = q + (size_t)b * nx;
+        const float *row = reinterpret_cast<const float *>(gb + (o + (unsigned)b * rs));
         r[b] = wx0 * row[0] + wx1 * row[1] + wx2 * row[2] + wx3 * row[3];
     }
     return wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3];
@@ -255,6 +259,7 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
     // H21 is written once the Hessian exists; the early exits (-1, 102) leave zeros.
     auto finish = [&](int status, bool zero_h, float zc) {
         if (tid != 0) return;
+        if (SIX && a.cc_mode == 1 && zero_h) a.cc[8 * (size_t)j + 7] = (double)status;
         const bool failed = status == -1;
         if (SIX) {
             a.U[2 * j] = failed ? NAN : P[4];
@@ -276,16 +281,59 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
     // sum J_k f and (6-DOF) the 21 Hessian entries; lanes are combined in FP64.
     // Pixel p = tid + k*T sits at column i, row r (p = r*w1 + i); the walk over k
     // advances (i, r) by T without divisions.
-    int xa, ya;
-    if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
+    int xa = 0, ya = 0;
     constexpr int NH = SIX ? 21 : 0;
     constexpr int NC = 2 + 6 + 6 + NH;   // sum f, sum f^2, sum J, sum J f, H
     float fv[PPL], fxv[PPL], fyv[PPL];
     double *cs = s_cs[sub_in_block];  // reduced sums, same value from every lane
+    double *cst = s_cst[sub_in_block];  // [0..5] sum J_k, [6..11] sum J_k f_n
+    double *hinv = s_hinv[sub_in_block];
     int w1 = w + 1, n = w1 * w1;
     int di = 0, dr = 0, i0 = 0, r0 = 0;
+    double meanf = 0.0, ssf = 0.0;
+    bool singular = false;
+    bool cached = false;
+    if (!SIX && a.cc_mode == 2) {
+        // constants of this subset kept by the 6-DOF run on the same window:
+        // {mean f, sum (f - mean f)^2, S_J5, S_J6, S_J5fn, S_J6fn, w, status}
+        const double *cc = a.cc + 8 * (size_t)j;
+        const int st0 = (int)cc[7];
+        if (st0 == -1 || st0 == 102) { finish(st0, true, NAN); return; }
+        if (st0 == 0) {
+            w = (int)cc[6];
+            if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
+            w1 = w + 1;
+            n = w1 * w1;
+            if (n > PPL * T) { if (tid == 0) a.iters[j] = -2; return; }
+            di = T % w1; dr = T / w1; i0 = tid % w1; r0 = tid / w1;
+            int i = i0, r = r0;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                if (r < w1) {
+                    const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
+                    fv[k] = a.f[o]; fxv[k] = a.gx[o]; fyv[k] = a.gy[o];
+                } else {
+                    fv[k] = fxv[k] = fyv[k] = 0.f;
+                }
+                i += di; r += dr;
+                if (i >= w1) { i -= w1; ++r; }
+            }
+            meanf = cc[0];
+            ssf = cc[1];
+            cst[4] = cc[2]; cst[5] = cc[3]; cst[10] = cc[4]; cst[11] = cc[5];
+            cached = true;
+        }
+    }
+    if (!cached) {
+    if (win_corner(x0, y0, w, nx, ny, xa, ya)) { finish(-1, true, NAN); return; }
     for (int pass = 0; pass < 2; ++pass) {
-        if (n > PPL * T) { if (tid == 0) a.iters[j] = -2; return; }
+        if (n > PPL * T) {
+            if (tid == 0) {
+                a.iters[j] = -2;
+                if (SIX && a.cc_mode == 1) a.cc[8 * (size_t)j + 7] = -2.0;
+            }
+            return;
+        }
         di = T % w1; dr = T / w1; i0 = tid % w1; r0 = tid / w1;
         float acc[NC];
 #pragma unroll
@@ -334,23 +382,31 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
         for (int q = 0; q < NC; ++q) cs[q] = block_sumd<WPS>((double)acc[q], s_redd, wid);
         break;
     }
-    const double meanf = cs[0] / n;
-    double ssf = cs[1] - cs[0] * meanf;  // sum (f - meanf)^2
+    meanf = cs[0] / n;
+    ssf = cs[1] - cs[0] * meanf;  // sum (f - meanf)^2
     ssf = ssf > 0 ? ssf : 0.0;
-    const double bottomf = sqrt(ssf);
-    double *cst = s_cst[sub_in_block];  // [0..5] sum J_k, [6..11] sum J_k f_n
-    double *hinv = s_hinv[sub_in_block];
-    bool singular = false;
+    {
+        const double bottomf = sqrt(ssf);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        cst[q] = cs[2 + q];  // every lane writes the same value and reads back its own
-        cst[6 + q] = (cs[8 + q] - meanf * cs[2 + q]) / bottomf;
+        for (int q = 0; q < 6; ++q) {
+            cst[q] = cs[2 + q];  // every lane writes the same value and reads back its own
+            cst[6 + q] = (cs[8 + q] - meanf * cs[2 + q]) / bottomf;
+        }
     }
     if (SIX) {
-        if (tid == 0)
+        if (tid == 0) {
             for (int e = 0; e < 21; ++e) a.H21[21 * (size_t)j + e] = cs[14 + e];
+            if (a.cc_mode == 1) {
+                double *cc = a.cc + 8 * (size_t)j;
+                cc[0] = meanf; cc[1] = ssf;
+                cc[2] = cst[4]; cc[3] = cst[5]; cc[4] = cst[10]; cc[5] = cst[11];
+                cc[6] = (double)w; cc[7] = 0.0;
+            }
+        }
         singular = !spd_inverse6_lds(cs + 14, hinv);
     }
+    }  // !cached
+    const double bottomf = sqrt(ssf);
     {
         const float mf = (float)meanf;
         const float ib = (float)(1.0 / bottomf);

@@ -163,13 +163,6 @@ __device__ __forceinline__ void corr_row(float (&acc)[X], const float *dr, const
     }
 }
 
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int X>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
 {

@@ -131,6 +131,32 @@ def test_icgn6_uneven_sizes_and_wide_windows(gpu_ctx, speckle, orc):
     _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.95)
 
 
+@pytest.mark.parametrize("case", ["moving_window", "large_stretch"])
+def test_icgn6_far_initial_guess_and_large_stretch(gpu_ctx, speckle, orc, case):
+    """IC-GN with the initial guess ~2.5 px off (the warped window moves by more than a
+    pixel between iterations) and with a 30 % stretch of a 33 x 33 window."""
+    if case == "moving_window":
+        p = speckle.small_pair(n=160, u0=1.7, v0=-1.1, ux=0.01, vy=-0.008, seed=31)
+        du = np.array([2.6, -2.2])
+    else:
+        p = speckle.small_pair(n=160, u0=0.6, v0=-0.4, ux=0.3, vy=0.3, seed=32)
+        du = np.array([0.0, 0.0])
+    _flush(f"icgn6 case {case}")
+    gpu_ctx.set_images(p.f, p.g)
+    f, _ = orc.normalize(p.f.astype(float))
+    g, _ = orc.normalize(p.g.astype(float))
+    gx, gy = orc.gradient(f)
+    c = np.arange(64, 97, 4, dtype=float)
+    X, Y = np.meshgrid(c, c)
+    xy = np.stack([X.ravel(), Y.ravel()], 1)
+    ut, vt = p.ufun(xy[:, 0], xy[:, 1])
+    U0 = (np.stack([ut, vt], 1) + du).ravel()
+    U, F, H, it, z = gpu_ctx.icgn6(xy, U0, 32)
+    Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gx, gy, xy, U0, 32)
+    ok = _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.95)
+    assert ok.mean() > 0.5
+
+
 def test_icgn2_matches_oracle(gpu_ctx, cfg1, cfg1_norm, orc):
     gpu_ctx.set_images(cfg1.f, cfg1.g)
     f, g, gx, gy = cfg1_norm
