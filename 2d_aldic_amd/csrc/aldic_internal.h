@@ -68,9 +68,11 @@ struct FrameState {
     aldic_frame_opts o{};
     int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
     DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
-        bad, tmp, part, cc;
+        bad, tmp, part, cc, rl, F2b;
+    int f2cur = 0;  // which of F2 / F2b holds the last global step's F
     std::vector<double> hcoords;
     std::vector<int> helems;
+    std::vector<hipEvent_t> ev;  // stage-boundary events of frame_run
     float ms[8] = {0};
 };
 
@@ -126,6 +128,7 @@ struct IcgnArgs {
     const int *list;        // subset indices to process (nullptr = all S)
     int nlist;
     const int *nlist_dev;   // when set, the list length is read from the device
+    double *Ucopy, *Fcopy;  // 6-DOF: optional second copy of U and F
     // per-subset constants shared by the 6-DOF and 2-DOF runs on the same windows
     // (8 doubles per subset); cc_mode 0: unused, 1: written by 6-DOF, 2: read by 2-DOF
     double *cc;
@@ -134,6 +137,8 @@ struct IcgnArgs {
 hipError_t launch_icgn6(const IcgnArgs &a, int ppl, hipStream_t st);
 hipError_t launch_icgn2(const IcgnArgs &a, int ppl, hipStream_t st);
 int icgn_ppl_for(int ws);   // pixels per lane class that holds a (ws+1)^2 window, 0 if none
+// list = indices j (ascending) with it[j] == code, *count = their number; one workgroup
+hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count, hipStream_t st);
 
 struct SearchArgs {
     const float *f, *g;
@@ -159,8 +164,10 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
 // relative residual estimate}.
 hipError_t fem_solve(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
                      DBuf *work, double *stats_host, hipStream_t st, int poll);
-hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F,
-                      hipStream_t st);
+// Nodal strain; with udual set also udual = F - F1, vdual = U - U1 (ADMM dual update).
+hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F, hipStream_t st,
+                      const double *F1 = nullptr, const double *U1 = nullptr,
+                      double *udual = nullptr, double *vdual = nullptr);
 
 // ADMM helpers
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);

@@ -220,8 +220,10 @@ void aldic_destroy(aldic_ctx *ctx)
     FrameState &F = ctx->fr;
     DBuf *fb[] = {&F.coords, &F.U0, &F.U6, &F.F6, &F.H21, &F.it6, &F.zncc, &F.Ucur, &F.Fcur,
                   &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
-                  &F.part, &F.cc};
+                  &F.part, &F.cc, &F.rl, &F.F2b};
     for (DBuf *b : fb) b->release();
+    for (auto &e : F.ev) (void)hipEventDestroy(e);
+    F.ev.clear();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -549,7 +551,8 @@ int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elem
         if (ok) {
             // closed-form generalised eigenbases of the 1-D FE matrices (DCT-I vectors,
             // M1-orthonormal): V[i][k] = cos(pi k i / (n-1)) / sqrt(m_k w_k)
-            std::vector<double> V((size_t)gx * gx + (size_t)gy * gy), lam(gx + gy);
+            // fdV = [Vx, Vy, Vx^T, Vy^T] (row-major)
+            std::vector<double> V(2 * ((size_t)gx * gx + (size_t)gy * gy)), lam(gx + gy);
             auto basis = [](int n, double h, double *Vo, double *lo) {
                 const double pi = 3.14159265358979323846;
                 for (int k = 0; k < n; ++k) {
@@ -563,6 +566,12 @@ int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elem
             };
             basis(gx, hx, V.data(), lam.data());
             basis(gy, hy, V.data() + (size_t)gx * gx, lam.data() + gx);
+            const size_t oT = (size_t)gx * gx + (size_t)gy * gy;
+            for (int i = 0; i < gx; ++i)
+                for (int k = 0; k < gx; ++k) V[oT + (size_t)k * gx + i] = V[(size_t)i * gx + k];
+            for (int i = 0; i < gy; ++i)
+                for (int k = 0; k < gy; ++k)
+                    V[oT + (size_t)gx * gx + (size_t)k * gy + i] = V[(size_t)gx * gx + (size_t)i * gy + k];
             CK(up(m.fdV, V.data(), V.size(), ctx->stream));
             CK(up(m.fdlam, lam.data(), lam.size(), ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
@@ -666,6 +675,11 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(hipSetDevice(ctx->device));
     FrameState &F = ctx->fr;
     F.o = *o;
+    for (size_t nev = 8 + 5 * (size_t)(o->n_admm + 1); F.ev.size() < nev;) {
+        hipEvent_t e;
+        CK(hipEventCreate(&e));
+        F.ev.push_back(e);
+    }
     int gr[4];
     if (aldic_search_grid(ctx->nx, ctx->ny, o->grid, o->ws, o->step, o->s, gr))
         return fail(ctx, ALDIC_ERR_ARG, "frame_setup: bad grid");
@@ -704,6 +718,8 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.bad.ensure(sizeof(int) * S));
     CK(F.part.ensure(3 * 256 * sizeof(double)));
     CK(F.cc.ensure(8 * sizeof(double) * S));
+    CK(F.rl.ensure(sizeof(int) * (S + 1)));
+    CK(F.F2b.ensure(sizeof(double) * 4 * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
     CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
@@ -724,12 +740,23 @@ int aldic_frame_run(aldic_ctx *ctx)
     const int S = F.S;
     static thread_local DBuf work[8];
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t *ev = ctx->ev;
-
-    CK(hipEventRecord(ev[0], st));
+    // Stage boundaries are events in one pool; the host waits once, at the end, so the
+    // stream never drains between stages.  spans: (stage, first event, second event).
+    std::vector<hipEvent_t> &ev = F.ev;
+    int ne = 0;
+    std::vector<int> spans;
+    auto mark = [&]() -> int {
+        if (ne >= (int)ev.size()) return -1;
+        return hipEventRecord(ev[ne], st) == hipSuccess ? ne++ : -1;
+    };
+    auto span = [&](int stage, int e0, int e1) {
+        spans.push_back(stage); spans.push_back(e0); spans.push_back(e1);
+    };
+    const int e_start = mark();
     CK(run_prep(ctx));
     ctx->have_images = true;
-    CK(hipEventRecord(ev[1], st));
+    const int e_prep = mark();
+    span(0, e_start, e_prep);
     // integer search -> U0
     SearchArgs sa{};
     sa.f = ctx->f.as<float>();
@@ -757,7 +784,8 @@ int aldic_frame_run(aldic_ctx *ctx)
     sa.U0 = F.U0.as<double>();
     CK(launch_search(sa, st));
     // search NaNs (constant templates) are filled like Init's inpaint_nans
-    CK(hipEventRecord(ev[2], st));
+    const int e_search = mark();
+    span(1, e_prep, e_search);
 
     // 6-DOF IC-GN (LocalICGN)
     IcgnArgs a{};
@@ -781,78 +809,101 @@ int aldic_frame_run(aldic_ctx *ctx)
     a.nlist = S;
     a.cc = F.cc.as<double>();
     a.cc_mode = 1;
+    a.Ucopy = F.U1.as<double>();  // refilled in place by the bad-point step
+    a.Fcopy = F.F1.as<double>();
     const int ppl = icgn_ppl_for(o.ws);
     CK(launch_icgn6(a, ppl, st));
-    CK(hipEventRecord(ev[3], st));
-    // LocalICGN post-processing: bad-point rule and refill
-    CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, F.U6.as<double>(), F.F6.as<double>(),
-                      F.U1.as<double>(), F.F1.as<double>(), F.bad.as<int>(),
-                      F.part.as<double>(), st));
-    CK(hipEventRecord(ev[4], st));
+    // windows the mask rule enlarged beyond the class (status -2, funICGN.m:56-60) run
+    // again one class up, over a list built on the device
+    const int ppl_big = icgn_ppl_for((int)std::floor(std::sqrt(2.0) * o.ws + 0.5));
+    const bool relaunch = ppl_big != 0 && ppl_big != ppl;
+    IcgnArgs ar = a;
+    if (relaunch) {
+        int *rl = F.rl.as<int>();
+        CK(launch_list_eq(F.it6.as<int>(), S, -2, rl, rl + S, st));
+        ar.list = rl;
+        ar.nlist = S;
+        ar.nlist_dev = rl + S;
+        CK(launch_icgn6(ar, ppl_big, st));
+    }
+    const int e_icgn6 = mark();
+    span(2, e_search, e_icgn6);
+    // LocalICGN post-processing: bad-point rule and refill (U1, F1 hold copies of U6, F6)
+    CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, nullptr, nullptr, F.U1.as<double>(),
+                      F.F1.as<double>(), F.bad.as<int>(), F.part.as<double>(), st));
+    const int e_post6 = mark();
+    span(3, e_icgn6, e_post6);
 
-    // step 1 global step with zero duals
-    CK(hipMemsetAsync(F.udual.p, 0, sizeof(double) * 4 * S, st));
-    CK(hipMemsetAsync(F.vdual.p, 0, sizeof(double) * 2 * S, st));
+    // Global steps.  F1 is F6 refilled for step 1 and the previous F2 afterwards
+    // (FSubpb1 = FSubpb2, main_ALDIC.m:364): F2 alternates between two buffers instead of
+    // being copied.  Step 1 runs with zero duals (nullptr).
     double *b = ctx->scratch[5].as<double>();
-    auto global_step = [&](float &t_sub2, float &t_str) -> int {
-        CK(hipEventRecord(ev[5], st));
-        CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F.F1.as<double>(),
-                   F.udual.as<double>(), F.vdual.as<double>(), b, st));
+    double *F2buf[2] = {F.F2.as<double>(), F.F2b.as<double>()};
+    int cur = 0;
+    const double *F1p = F.F1.as<double>();
+    auto global_step = [&](bool zero_duals) -> int {
+        const int g0 = mark();
+        CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F1p,
+                   zero_duals ? nullptr : F.udual.as<double>(),
+                   zero_duals ? nullptr : F.vdual.as<double>(), b, st));
         CK(fem_solve(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
-        CK(hipEventRecord(ev[6], st));
-        CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F.F2.as<double>(), st));
-        // dual update, FEM branch: udual = F2 - F1, vdual = U2 - U1 (main_ALDIC.m:340,440)
-        CK(launch_sub(F.F2.as<double>(), F.F1.as<double>(), F.udual.as<double>(), 4L * S, st));
-        CK(launch_sub(F.U2.as<double>(), F.U1.as<double>(), F.vdual.as<double>(), 2L * S, st));
-        CK(hipEventRecord(ev[7], st));
-        CK(hipEventSynchronize(ev[7]));
-        float t1 = 0, t2 = 0;
-        CK(hipEventElapsedTime(&t1, ev[5], ev[6]));
-        CK(hipEventElapsedTime(&t2, ev[6], ev[7]));
-        t_sub2 += t1;
-        t_str += t2;
+        const int g1 = mark();
+        span(4, g0, g1);
+        // nodal strain and the dual update of the FEM branch (main_ALDIC.m:340,440):
+        // udual = F2 - F1, vdual = U2 - U1
+        CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F2buf[cur], st, F1p, F.U1.as<double>(),
+                      F.udual.as<double>(), F.vdual.as<double>()));
+        span(6, g1, mark());
         return ALDIC_OK;
     };
-    int r = global_step(acc[4], acc[6]);
+    int r = global_step(true);
     if (r) return r;
 
     for (int step = 0; step < o.n_admm; ++step) {
         // Subpb1: 2-DOF IC-GN from (U2, F2)
-        CK(hipEventRecord(ev[8], st));
+        const int i0 = mark();
         IcgnArgs a2 = a;
         a2.U0 = nullptr;
         a2.Hin = F.H21.as<double>();
         a2.vdual = F.vdual.as<double>();
         a2.Uold = F.U2.as<double>();
-        a2.Fold = F.F2.as<double>();
+        a2.Fold = F2buf[cur];
         a2.mu = o.mu;
         a2.U = F.U1.as<double>();
         a2.iters = F.it2.as<int>();
         a2.F = nullptr;
         a2.H21 = nullptr;
         a2.zncc = nullptr;
+        a2.Ucopy = nullptr;
+        a2.Fcopy = nullptr;
         a2.cc_mode = 2;
         CK(launch_icgn2(a2, ppl, st));
+        if (relaunch) {
+            IcgnArgs ar2 = a2;
+            ar2.list = ar.list;
+            ar2.nlist = ar.nlist;
+            ar2.nlist_dev = ar.nlist_dev;
+            CK(launch_icgn2(ar2, ppl_big, st));
+        }
         CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
                           nullptr, F.bad.as<int>(), F.part.as<double>(), st));
-        // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
-        CK(hipMemcpyAsync(F.F1.p, F.F2.p, sizeof(double) * 4 * S, hipMemcpyDeviceToDevice, st));
-        CK(hipEventRecord(ev[9], st));
-        CK(hipEventSynchronize(ev[9]));
-        float t = 0;
-        CK(hipEventElapsedTime(&t, ev[8], ev[9]));
-        acc[5] += t;
-        r = global_step(acc[4], acc[6]);
+        F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
+        cur ^= 1;
+        span(5, i0, mark());
+        r = global_step(false);
         if (r) return r;
     }
-    CK(hipEventRecord(ev[9], st));
-    CK(hipEventSynchronize(ev[9]));
-    float t;
-    CK(hipEventElapsedTime(&t, ev[0], ev[1])); acc[0] = t;
-    CK(hipEventElapsedTime(&t, ev[1], ev[2])); acc[1] = t;
-    CK(hipEventElapsedTime(&t, ev[2], ev[3])); acc[2] = t;
-    CK(hipEventElapsedTime(&t, ev[3], ev[4])); acc[3] = t;
-    CK(hipEventElapsedTime(&t, ev[0], ev[9])); acc[7] = t;
+    F.f2cur = cur;
+    const int e_end = mark();
+    span(7, e_start, e_end);
+    if (e_end < 0) return fail(ctx, ALDIC_ERR_HIP, "frame_run: event pool exhausted");
+    CK(hipEventSynchronize(ev[e_end]));
+    for (size_t k = 0; k + 2 < spans.size(); k += 3) {
+        float t = 0;
+        if (spans[k + 1] < 0 || spans[k + 2] < 0) continue;
+        CK(hipEventElapsedTime(&t, ev[spans[k + 1]], ev[spans[k + 2]]));
+        acc[spans[k]] += t;
+    }
     for (int k = 0; k < 8; ++k) F.ms[k] = acc[k];
     return ALDIC_OK;
 }
@@ -866,7 +917,7 @@ int aldic_frame_get(aldic_ctx *ctx, double *U, double *Fo, double *U6, double *F
     CK(hipSetDevice(ctx->device));
     const size_t S = F.S;
     CK(down(U, F.U2.p, 2 * S, ctx->stream));
-    CK(down(Fo, F.F2.p, 4 * S, ctx->stream));
+    CK(down(Fo, F.f2cur ? F.F2b.p : F.F2.p, 4 * S, ctx->stream));
     CK(down(U6, F.U6.p, 2 * S, ctx->stream));
     CK(down(F6, F.F6.p, 4 * S, ctx->stream));
     CK(down(iters6, F.it6.p, S, ctx->stream));

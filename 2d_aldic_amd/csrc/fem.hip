@@ -115,11 +115,12 @@ __global__ __launch_bounds__(256) void k_rhs(int nn, const int *rowptr, const in
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
         const int c = col[k];
         const double m = mval[k];
-        const double2 u = U[c], v = vdual[c];
+        const double2 u = U[c], v = vdual ? vdual[c] : make_double2(0.0, 0.0);
         su += m * (u.x - v.x);
         sv += m * (u.y - v.y);
     }
-    const double4 f = F[i], w = udual[i];  // [F11 F21 F12 F22]
+    const double4 f = F[i];  // [F11 F21 F12 F22]
+    const double4 w = udual ? udual[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     const double2 g = gvec[i];
     b[i] = make_double2(beta * (g.x * (f.x - w.x) + g.y * (f.z - w.z)) + mu * su,
                         beta * (g.x * (f.y - w.y) + g.y * (f.w - w.w)) + mu * sv);
@@ -394,8 +395,130 @@ static hipError_t gemm(const GemmArgs &g, int batch, bool scale, hipStream_t st)
     return hipGetLastError();
 }
 
+static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DBuf *work,
+                                hipStream_t st);
+
+// Fused pass of the fast-diagonalisation solve for grids whose gy x gx component fits
+// in LDS: workgroup (row strip of kFdRows rows, component) computes
+//   out[m, :] = ((L B)[m, :] R) (./ (mu + cK (lamy[m] + lamx[n])) when SCALE)
+// with L(m, k) = Lsrc[k * gy + m] and R(j, n) = Rsrc[j * gx + n].  B and out are the
+// interleaved gy x 2gx nodal arrays.  Every load of B and R is issued up front into
+// registers (one wave per SIMD, so each lane can hold 2 x 64 doubles): one memory
+// round trip per kernel.  B, then R, goes to LDS; the strip of L and the intermediate
+// row block are kept transposed (row index fastest); each FMA step reads one B or R
+// word and one broadcast strip word.
+constexpr int kFdRows = 2;
+constexpr int kFdMax = 128;           // gx, gy limit of the fused path
+constexpr int kFdPer = kFdMax * kFdMax / 256;  // words of a component per thread
+
+template <bool SCALE>
+__global__ __launch_bounds__(256, 1) void k_fd_pass(int gy, int gx, const double *Lsrc,
+                                                    const double *B, const double *Rsrc,
+                                                    double *out, const double *lamy,
+                                                    const double *lamx, double mu, double cK)
+{
+    extern __shared__ __align__(16) double sm[];
+    const int big = ((gy > gx ? gy : gx) * gx + 1) & ~1;  // keeps Lt, Tt 16-byte aligned
+    double *Ms = sm;                        // B (gy x gx), later R (gx x gx)
+    double *Lt = Ms + big;                  // Lt[k][r] = L(m0 + r, k), k < gy
+    double *Tt = Lt + gy * kFdRows;         // Tt[j][r] = (L B)(m0 + r, j), j < gx
+    const int comp = blockIdx.y, m0 = blockIdx.x * kFdRows;
+    const int t = threadIdx.x;
+    const int w2 = 2 * gx;
+    // every load issued before the first use: L strip word, B (component comp, walked as
+    // e = t + 256 i over the gy x gx array without divisions), R (contiguous)
+    double vl = 0.0;
+    if (t < gy * kFdRows) {
+        const int k = t / kFdRows, r = t % kFdRows;
+        if (m0 + r < gy) vl = Lsrc[(long)k * gy + m0 + r];
+    }
+    const int nb = gy * gx, nr = gx * gx;
+    const int dk = 256 / gx, dc = 256 % gx;
+    double vb[kFdPer], vr[kFdPer];
+    {
+        int k = t / gx, c = t % gx;
+#pragma unroll
+        for (int i = 0; i < kFdPer; ++i) {
+            vb[i] = (t + 256 * i < nb) ? B[(long)k * w2 + 2 * c + comp] : 0.0;
+            k += dk;
+            c += dc;
+            if (c >= gx) { c -= gx; ++k; }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kFdPer; ++i) vr[i] = (t + 256 * i < nr) ? Rsrc[t + 256 * i] : 0.0;
+    if (t < gy * kFdRows) Lt[t] = vl;
+#pragma unroll
+    for (int i = 0; i < kFdPer; ++i)
+        if (t + 256 * i < nb) Ms[t + 256 * i] = vb[i];
+    __syncthreads();
+    // thread: column c, row h of the strip (the row is the same in a whole wave, so its
+    // strip words are LDS broadcasts)
+    const int c = t & 127, h = t >> 7;
+    if (c < gx) {
+        double acc = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < gy; ++k) acc += Lt[k * kFdRows + h] * Ms[k * gx + c];
+        Tt[c * kFdRows + h] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kFdPer; ++i)
+        if (t + 256 * i < nr) Ms[t + 256 * i] = vr[i];
+    __syncthreads();
+    if (c < gx) {
+        double acc = 0.0;
+#pragma unroll 8
+        for (int j = 0; j < gx; ++j) acc += Tt[j * kFdRows + h] * Ms[j * gx + c];
+        const int m = m0 + h;
+        if (m < gy) {
+            if (SCALE) acc /= mu + cK * (lamy[m] + lamx[c]);
+            out[(long)m * w2 + 2 * c + comp] = acc;
+        }
+    }
+}
+
+static size_t fd_pass_lds(int gy, int gx)
+{
+    return sizeof(double) * ((((size_t)(gy > gx ? gy : gx) * gx + 1) & ~(size_t)1) +
+                             (size_t)(gy + gx) * kFdRows);
+}
+
 static hipError_t fem_fd_solve(const MeshDev &m, const double *b, double *x, DBuf *work,
                                hipStream_t st)
+{
+    if (m.gx <= kFdMax && m.gy <= kFdMax && fd_pass_lds(m.gy, m.gx) <= 160 * 1024 - 1024) {
+        const int gx = m.gx, gy = m.gy;
+        hipError_t e;
+        if ((e = work[0].ensure(sizeof(double) * 2 * (size_t)gx * gy)) != hipSuccess) return e;
+        double *T2 = work[0].as<double>();
+        const double *Vx = m.fdV.as<double>(), *Vy = Vx + (size_t)gx * gx;
+        const double *VxT = Vy + (size_t)gy * gy, *VyT = VxT + (size_t)gx * gx;
+        const double *lx = m.fdlam.as<double>(), *ly = lx + gx;
+        const size_t lds = fd_pass_lds(gy, gx);
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void *)k_fd_pass<true>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+            (void)hipFuncSetAttribute((const void *)k_fd_pass<false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+            attr = true;
+        }
+        const dim3 grid((gy + kFdRows - 1) / kFdRows, 2);
+        // T2 = (Vy^T B Vx) ./ D:  L(m, k) = Vy[k][m],  R(j, n) = Vx[j][n]
+        hipLaunchKernelGGL(k_fd_pass<true>, grid, dim3(256), lds, st, gy, gx, Vy, b, Vx, T2, ly,
+                           lx, m.mu, m.beta + m.alpha * m.mu);
+        // X = Vy T2 Vx^T:  L(m, k) = Vy[m][k] = VyT[k][m],  R(j, n) = Vx[n][j] = VxT[j][n]
+        hipLaunchKernelGGL(k_fd_pass<false>, grid, dim3(256), lds, st, gy, gx, VyT,
+                           (const double *)T2, VxT, x, nullptr, nullptr, 0.0, 0.0);
+        return hipGetLastError();
+    }
+    return fem_fd_solve4(m, b, x, work, st);
+}
+
+// Four separate GEMM passes (grids too large for the fused path).
+static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DBuf *work,
+                                hipStream_t st)
 {
     const int gx = m.gx, gy = m.gy;
     const size_t nv = 2 * (size_t)gx * gy;
@@ -520,9 +643,13 @@ hipError_t fem_solve(const MeshDev &m, const double *b, double *x, double rtol, 
 }
 
 // ---- nodal strain (funGlobal_NodalStrainAvg.m) ------------------------------------
+// With udual set, also the ADMM dual update of the FEM branch (main_ALDIC.m:340,440):
+// udual = F - F1, vdual = U - U1 at the same node.
 __global__ __launch_bounds__(256) void k_strain(int nn, const double2 *coords, const int4 *elems,
                                                 const int *n2e_ptr, const N2E *n2e,
-                                                const double2 *U, int swap_fix, double4 *F)
+                                                const double2 *U, int swap_fix, double4 *F,
+                                                const double4 *F1, const double2 *U1,
+                                                double4 *udual, double2 *vdual)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nn) return;
@@ -571,14 +698,22 @@ __global__ __launch_bounds__(256) void k_strain(int nn, const double2 *coords, c
     o.z = (swap_fix ? acc[1] : acc[2]) / d;
     o.w = acc[3] / d;
     F[i] = o;
+    if (udual) {
+        const double4 f1 = F1[i];
+        udual[i] = make_double4(o.x - f1.x, o.y - f1.y, o.z - f1.z, o.w - f1.w);
+        const double2 u = U[i], u1 = U1[i];
+        vdual[i] = make_double2(u.x - u1.x, u.y - u1.y);
+    }
 }
 
-hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F, hipStream_t st)
+hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F, hipStream_t st,
+                      const double *F1, const double *U1, double *udual, double *vdual)
 {
     const int nb = (m.nn + 255) / 256;
     hipLaunchKernelGGL(k_strain, dim3(nb), dim3(256), 0, st, m.nn, m.coords.as<double2>(),
                        m.elems.as<int4>(), m.n2e_ptr.as<int>(), m.n2e.as<N2E>(),
-                       (const double2 *)U, swap_fix, (double4 *)F);
+                       (const double2 *)U, swap_fix, (double4 *)F, (const double4 *)F1,
+                       (const double2 *)U1, (double4 *)udual, (double2 *)vdual);
     return hipGetLastError();
 }
 
@@ -702,6 +837,7 @@ hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
                           double *part, hipStream_t st)
 {
+    if (S <= 0) return hipSuccess;
     int nb = (S + 255) / 256;
     if (nb > 256) nb = 256;
     const int nbf = (S + 255) / 256;

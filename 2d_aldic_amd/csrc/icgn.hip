@@ -17,6 +17,8 @@
 // convergence test, the 6x6 solve and the inverse-compositional update run in FP64
 // on every lane with bitwise-identical inputs (butterfly all-reduce), so the loop
 // is uniform across the wave.
+#include <algorithm>
+
 #include "aldic_internal.h"
 #include "common.h"
 
@@ -220,7 +222,7 @@ __device__ __forceinline__ float sample_g(const float *__restrict__ g, int nx, i
 
 // Shared body for both modes.  SIX = true: 6-DOF funICGN; false: 2-DOF funICGN_Subpb1.
 template <int PPL, int WPS, bool SIX>
-__global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
+__device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
 {
     constexpr int SPB = 4 / WPS;          // subsets per block
     constexpr int T = 64 * WPS;           // threads per subset
@@ -236,9 +238,6 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
     const int sub_in_block = wave / WPS;
     const int wid = wave % WPS;            // wave index inside the subset
     const int tid = wid * 64 + lane;       // thread index inside the subset
-    const int slot = blockIdx.x * SPB + sub_in_block;
-    const int nlist = a.nlist_dev ? *a.nlist_dev : a.nlist;
-    if (slot >= nlist) return;             // uniform per subset
     const int j = a.list ? a.list[slot] : slot;
     const int nx = a.nx, ny = a.ny;
     const double x0 = a.coords[2 * j], y0 = a.coords[2 * j + 1];
@@ -265,6 +264,12 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
             a.U[2 * j] = failed ? NAN : P[4];
             a.U[2 * j + 1] = failed ? NAN : P[5];
             for (int q = 0; q < 4; ++q) a.F[4 * j + q] = failed ? NAN : P[q];
+            if (a.Ucopy) {
+                a.Ucopy[2 * j] = failed ? NAN : P[4];
+                a.Ucopy[2 * j + 1] = failed ? NAN : P[5];
+            }
+            if (a.Fcopy)
+                for (int q = 0; q < 4; ++q) a.Fcopy[4 * j + q] = failed ? NAN : P[q];
             if (zero_h)
                 for (int q = 0; q < 21; ++q) a.H21[21 * (size_t)j + q] = 0.0;
             if (a.zncc) a.zncc[j] = (double)zc;
@@ -556,6 +561,77 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
     finish(it, false, zc);
 }
 
+// Subset slot = blockIdx.x * SPB + wave / WPS.  LOOP: the grid is capped and steps over
+// a list whose length is only known on the device.
+template <int PPL, int WPS, bool SIX, bool LOOP>
+__global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
+{
+    constexpr int SPB = 4 / WPS;
+    const int nlist = a.nlist_dev ? *a.nlist_dev : a.nlist;
+    const int sub = (threadIdx.x >> 6) / WPS;
+    if constexpr (!LOOP) {
+        const int slot = blockIdx.x * SPB + sub;
+        if (slot < nlist) icgn_subset<PPL, WPS, SIX>(a, slot);  // uniform per subset
+    } else {
+        for (int base = blockIdx.x * SPB; base < nlist; base += gridDim.x * SPB) {
+            if (base + sub < nlist) icgn_subset<PPL, WPS, SIX>(a, base + sub);
+            if constexpr (WPS > 1) __syncthreads();  // the subset's LDS is free for the next
+        }
+    }
+}
+
+// Indices j with it[j] == code, in increasing order, and their count (one workgroup):
+// each wave owns a contiguous run, counts it with ballots, then writes at its offset.
+__global__ __launch_bounds__(1024) void k_list_eq(const int *it, int S, int code, int *list,
+                                                  int *count)
+{
+    __shared__ int wc[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int per = ((S + 15) / 16 + 511) & ~511;
+    const int b0 = min(S, w * per), b1 = min(S, b0 + per);
+    int c = 0;
+    for (int base = b0; base < b1; base += 512) {  // 8 loads in flight per lane
+        int v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = base + 64 * k + lane;
+            v[k] = i < b1 ? it[i] : code + 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += __popcll(__ballot(v[k] == code));
+    }
+    if (lane == 0) wc[w] = c;
+    __syncthreads();
+    int o = 0;
+    for (int k = 0; k < w; ++k) o += wc[k];
+    for (int base = b0; base < b1; base += 512) {
+        int v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = base + 64 * k + lane;
+            v[k] = i < b1 ? it[i] : code + 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool m = v[k] == code;
+            const unsigned long long bal = __ballot(m);
+            if (m) list[o + __popcll(bal & ((1ull << lane) - 1ull))] = base + 64 * k + lane;
+            o += __popcll(bal);
+        }
+    }
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < 16; ++k) tot += wc[k];
+        *count = tot;
+    }
+}
+
+hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_list_eq, dim3(1), dim3(1024), 0, st, it, S, code, list, count);
+    return hipGetLastError();
+}
+
 int icgn_ppl_for(int ws)
 {
     const int n = (ws + 1) * (ws + 1);
@@ -569,12 +645,29 @@ template <bool SIX>
 static hipError_t launch_any(const IcgnArgs &a, int ppl, hipStream_t st)
 {
     if (a.nlist <= 0) return hipSuccess;
+    // a device-side length (a.nlist its upper bound) gets a capped, looping grid
+    constexpr int cap = 512;
+    const bool loop = a.nlist_dev != nullptr;
     if (ppl == 8) {
-        hipLaunchKernelGGL((k_icgn<8, 1, SIX>), dim3((a.nlist + 3) / 4), dim3(256), 0, st, a);
+        if (loop)
+            hipLaunchKernelGGL((k_icgn<8, 1, SIX, true>), dim3(std::min(cap, (a.nlist + 3) / 4)),
+                               dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_icgn<8, 1, SIX, false>), dim3((a.nlist + 3) / 4), dim3(256), 0,
+                               st, a);
     } else if (ppl == 18) {
-        hipLaunchKernelGGL((k_icgn<18, 1, SIX>), dim3((a.nlist + 3) / 4), dim3(256), 0, st, a);
+        if (loop)
+            hipLaunchKernelGGL((k_icgn<18, 1, SIX, true>),
+                               dim3(std::min(cap, (a.nlist + 3) / 4)), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_icgn<18, 1, SIX, false>), dim3((a.nlist + 3) / 4), dim3(256), 0,
+                               st, a);
     } else if (ppl == 72) {
-        hipLaunchKernelGGL((k_icgn<18, 4, SIX>), dim3(a.nlist), dim3(256), 0, st, a);
+        if (loop)
+            hipLaunchKernelGGL((k_icgn<18, 4, SIX, true>), dim3(std::min(cap, a.nlist)),
+                               dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_icgn<18, 4, SIX, false>), dim3(a.nlist), dim3(256), 0, st, a);
     } else {
         return hipErrorInvalidValue;
     }
