@@ -570,7 +570,11 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
     const int nlist = a.nlist_dev ? *a.nlist_dev : a.nlist;
     const int sub = (threadIdx.x >> 6) / WPS;
     if constexpr (!LOOP) {
-        const int slot = blockIdx.x * SPB + sub;
+        // workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous run of
+        // subsets so that overlapping windows meet in the same L2
+        const int nb = gridDim.x, b = blockIdx.x, xcd = b & 7, per = nb >> 3, rm = nb & 7;
+        const int blk = xcd * per + min(xcd, rm) + (b >> 3);
+        const int slot = blk * SPB + sub;
         if (slot < nlist) icgn_subset<PPL, WPS, SIX>(a, slot);  // uniform per subset
     } else {
         for (int base = blockIdx.x * SPB; base < nlist; base += gridDim.x * SPB) {
