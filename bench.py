@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
 def log(msg):
@@ -100,14 +102,30 @@ def main():
     value = conv / (ms_per_step / 1e3)
     log(f"[bench] {ms_per_step:.3f} ms/frame, stages {stage}")
 
-    # roofline of the dominant kernel (algorithmic bytes per launch / event-timed duration)
+    # Roofline of the dominant kernel: the integer search (one k_search launch per frame,
+    # event-timed on the library's stream).  It is FP32-FMA bound: algorithmic FLOPs =
+    # the NCC numerator, 2 (2s+3)^2 (w+1)^2 per subset.  Its compulsory HBM bytes
+    # (8 N_px + 32 S) are reported beside it; "traffic" is the measured HBM bytes per
+    # launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB
+    # device copy reads on the same counters), kept in profiles/hbm_traffic.json.
     npx = 2048 * 2048
-    algo = {"icgn6": 16 * npx + 124 * S, "search": 8 * npx + 32 * S}
-    dom = max(algo, key=lambda k: stage[k])
-    achieved = algo[dom] / (stage[dom] / 1e3) / 1e9
-    roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
-                algorithmic_bytes=algo[dom], avg_ms=round(stage[dom], 5))
+    lags = (2 * srad + 3) ** 2
+    flops = 2.0 * lags * (ws + 1) ** 2 * S
+    algo_bytes = 8 * npx + 32 * S
+    t_s = stage["search"] / 1e3
+    achieved = flops / t_s / 1e12
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh)["k_search"]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        pass
+    roof = dict(kernel="k_search", bound="mfma", engine="VALU FP32 FMA (packed FP32 off)",
+                achieved=round(achieved, 3), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=traffic,
+                algorithmic_flops=flops, avg_ms=round(stage["search"], 5),
+                hbm=dict(algorithmic_bytes=algo_bytes,
+                         achieved_GBs=round(algo_bytes / t_s / 1e9, 2), peak_GBs=HBM_PEAK_GBS))
 
     cpu = None
     acc = {}
