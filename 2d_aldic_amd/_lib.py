@@ -22,14 +22,15 @@ EXPORTS = (
     "aldic_set_normalized", "aldic_set_gradients", "aldic_get_images", "aldic_search_grid",
     "aldic_integer_search", "aldic_icgn6", "aldic_icgn2", "aldic_set_mesh", "aldic_subpb2",
     "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
-    "aldic_frame_run", "aldic_frame_get", "aldic_frame_times",
+    "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
 )
 
 
 class FrameOpts(C.Structure):
     _fields_ = [("ws", C.c_int), ("step", C.c_int), ("s", C.c_int * 2), ("grid", C.c_int * 4),
                 ("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
-                ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double)]
+                ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
+                ("no_subpb1_fit", C.c_int)]
 
 
 _lib = None
@@ -67,6 +68,7 @@ def load():
     L.aldic_frame_run.argtypes = [P]
     L.aldic_frame_get.argtypes = [P, vp, vp, vp, vp, vp, vp, vp]
     L.aldic_frame_times.argtypes = [P, vp, ip]
+    L.aldic_gridfit.argtypes = [P, ip, ip, vp, vp, vp, dp, ip, vp, vp]
     _lib = L
     return L
 
@@ -224,6 +226,19 @@ class Context:
         self._ck(self.L.aldic_bicubic(self.h, _p(X), _p(Y), X.size, _p(Z)), "aldic_bicubic")
         return Z
 
+    def gridfit(self, xnodes, ynodes, z, rtol=1e-12, maxit=4000):
+        """gridfit.m with Subpb1's options for data on the nodes: z (ny, nx) with NaN
+        where there is no datum -> zgrid (ny, nx), and {iterations, relative residual}."""
+        xnodes = _c(xnodes, np.float64).ravel()
+        ynodes = _c(ynodes, np.float64).ravel()
+        z = _c(z, np.float64).reshape(ynodes.size, xnodes.size)
+        out = np.empty_like(z)
+        st = np.zeros(2)
+        self._ck(self.L.aldic_gridfit(self.h, xnodes.size, ynodes.size, _p(xnodes), _p(ynodes),
+                                      _p(z), C.c_double(rtol), int(maxit), _p(out), _p(st)),
+                 "aldic_gridfit")
+        return out, st
+
     # -- frame ---------------------------------------------------------------------------
     def frame_stage(self, f, g, mask=None, roi=None):
         f = np.ascontiguousarray(f)
@@ -236,7 +251,7 @@ class Context:
         self.nx, self.ny = nx, ny
 
     def frame_setup(self, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-                    cg_maxit=2000, cg_rtol=1e-10):
+                    cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True):
         o = FrameOpts()
         o.ws, o.step = int(ws), int(step)
         o.s[0], o.s[1] = (int(s), int(s)) if np.isscalar(s) else (int(s[0]), int(s[1]))
@@ -248,6 +263,7 @@ class Context:
         o.n_admm = n_admm
         o.cg_maxit = cg_maxit
         o.cg_rtol = cg_rtol
+        o.no_subpb1_fit = 0 if subpb1_fit else 1
         n = C.c_int(0)
         self._ck(self.L.aldic_frame_setup(self.h, C.byref(o), C.byref(n)), "aldic_frame_setup")
         self.frame_nodes = n.value

@@ -63,12 +63,28 @@ struct MeshDev {
     DBuf fdlam;               // double[gx + gy]: generalised eigenvalues of (K1, M1)
 };
 
+// gridfit ('springs') of nodal data on a rectilinear node grid, data on the nodes
+struct GridfitDev {
+    int gx = 0, gy = 0;
+    double lam2 = 0.0;
+    DBuf w;        // squared stiffnesses [x springs | y springs | (dx/xs)^2 | (dy/ys)^2]
+    MeshDev csr;   // 9-point pattern and values for the PCG path
+};
+hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const double *ynodes,
+                         int gy, hipStream_t st);
+// U: ncomp (1 or 2) interleaved values per node (node = iy * gx + ix), the initial
+// guess on entry and the fit on return; bad[node] != 0: the node carries no data.
+// stats_dev (nullable, device): per component {iterations, relative residual}.
+hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int ncomp, double rtol,
+                         int maxit, DBuf *work, double *stats_dev, hipStream_t st, int poll);
+
 struct FrameState {
     bool ready = false;
     aldic_frame_opts o{};
     int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
     DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
-        bad, tmp, part, cc, rl, F2b;
+        bad, tmp, part, cc, rl, F2b, gfstat;
+    GridfitDev gf;  // Subpb1's gridfit over the node grid
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
     std::vector<double> hcoords;
     std::vector<int> helems;
@@ -94,6 +110,7 @@ struct aldic_ctx {
     aldic::MeshDev mesh;
     bool have_mesh = false;
     aldic::FrameState fr;
+    aldic::GridfitDev gf;            // aldic_gridfit
     hipEvent_t ev[10] = {};
 };
 

@@ -222,6 +222,11 @@ void aldic_destroy(aldic_ctx *ctx)
                   &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
                   &F.part, &F.cc, &F.rl, &F.F2b};
     for (DBuf *b : fb) b->release();
+    for (aldic::GridfitDev *g : {&F.gf, &ctx->gf}) {
+        g->w.release();
+        g->csr.rowptr.release(); g->csr.col.release(); g->csr.aval.release(); g->csr.diag.release();
+    }
+    F.gfstat.release();
     for (auto &e : F.ev) (void)hipEventDestroy(e);
     F.ev.clear();
     for (auto &e : ctx->ev)
@@ -632,6 +637,43 @@ int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
     return ALDIC_OK;
 }
 
+int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const double *ynodes,
+                  const double *z, double rtol, int maxit, double *out, double *stats)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (gx < 2 || gy < 2 || !xnodes || !ynodes || !z || !out || !(rtol > 0) || maxit < 1)
+        return fail(ctx, ALDIC_ERR_ARG, "gridfit: bad arguments");
+    for (int i = 0; i + 1 < gx; ++i)
+        if (!(xnodes[i + 1] > xnodes[i])) return fail(ctx, ALDIC_ERR_ARG, "gridfit: xnodes not increasing");
+    for (int j = 0; j + 1 < gy; ++j)
+        if (!(ynodes[j + 1] > ynodes[j])) return fail(ctx, ALDIC_ERR_ARG, "gridfit: ynodes not increasing");
+    const size_t n = (size_t)gx * gy;
+    std::vector<double> u(n);
+    std::vector<int> bad(n);
+    double sum = 0.0;
+    size_t cnt = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (std::isfinite(z[i])) { sum += z[i]; ++cnt; }
+    if (cnt < 3) return fail(ctx, ALDIC_ERR_ARG, "gridfit: insufficient data (fewer than 3 points)");
+    for (size_t i = 0; i < n; ++i) {
+        bad[i] = std::isfinite(z[i]) ? 0 : 1;
+        u[i] = bad[i] ? sum / cnt : z[i];  // starting guess at the empty nodes
+    }
+    CK(hipSetDevice(ctx->device));
+    CK(gridfit_setup(ctx->gf, xnodes, gx, ynodes, gy, ctx->stream));
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], u.data(), n, ctx->stream));
+    CK(up(sc[1], bad.data(), n, ctx->stream));
+    CK(sc[2].ensure(4 * sizeof(double)));
+    static thread_local DBuf work[8];
+    CK(gridfit_solve(ctx->gf, sc[1].as<int>(), sc[0].as<double>(), 1, rtol, maxit, work,
+                     sc[2].as<double>(), ctx->stream, 8));
+    CK(down(out, sc[0].p, n, ctx->stream));
+    CK(down(stats, sc[2].p, 2, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
 int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, double *Z)
 {
     if (!ctx) return ALDIC_ERR_ARG;
@@ -719,10 +761,20 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.part.ensure(3 * 256 * sizeof(double)));
     CK(F.cc.ensure(8 * sizeof(double) * S));
     CK(F.rl.ensure(sizeof(int) * (S + 1)));
+    CK(F.gfstat.ensure(4 * sizeof(double)));
     CK(F.F2b.ensure(sizeof(double) * 4 * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
     CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
+    if (!o->no_subpb1_fit) {
+        // Subpb1's gridfit grid: xnodes = unique node x, ynodes = unique node y
+        if (F.Nx < 2 || F.Ny < 2)
+            return fail(ctx, ALDIC_ERR_ARG, "frame_setup: gridfit needs at least 2 x 2 nodes");
+        std::vector<double> xn(F.Nx), yn(F.Ny);
+        for (int k = 0; k < F.Nx; ++k) xn[k] = F.hcoords[2 * (size_t)k];
+        for (int l = 0; l < F.Ny; ++l) yn[l] = F.hcoords[2 * (size_t)l * F.Nx + 1];
+        CK(gridfit_setup(F.gf, xn.data(), F.Nx, yn.data(), F.Ny, ctx->stream));
+    }
     CK(hipStreamSynchronize(ctx->stream));
     F.ready = true;
     if (n_nodes) *n_nodes = S;
@@ -887,6 +939,11 @@ int aldic_frame_run(aldic_ctx *ctx)
         }
         CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
                           nullptr, F.bad.as<int>(), F.part.as<double>(), st));
+        // Subpb1.m:185-199: gridfit of u and v over the good nodes, written to every node
+        // (the refilled values above are only its starting guess)
+        if (!o.no_subpb1_fit)
+            CK(gridfit_solve(F.gf, F.bad.as<int>(), F.U1.as<double>(), 2, 1e-10, 4000, work,
+                             nullptr, st, 8));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         cur ^= 1;
         span(5, i0, mark());

@@ -118,13 +118,24 @@ int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
  * at (X[i], Y[i]) (X along dim 1), clamp-to-edge. */
 int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, double *Z);
 
+/* gridfit.m as Subpb1 calls it (Subpb1.m:185-189: 'regularizer','springs', other
+ * options default) for data on the nodes of the gx x gy grid xnodes x ynodes (both
+ * strictly increasing): z[iy*gx + ix] is the datum at node (ix, iy), NaN where there
+ * is none.  out[iy*gx + ix] receives zgrid(iy, ix) (gridfit.m:279, the least-squares fit
+ * of gridfit.m:651-666).  FP64 conjugate gradients to rtol (relative residual of the
+ * normal equations, e.g. 1e-12) in at most maxit iterations; stats (nullable) =
+ * {iterations, relative residual}. */
+int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const double *ynodes,
+                  const double *z, double rtol, int maxit, double *out, double *stats);
+
 /* ---- one whole frame on the device (main_ALDIC.m:87-451, FEM branch) ----------
  * Raw images are staged once; aldic_frame_run() then runs, without host
  * round-trips: normalisation + gradients, integer search, 6-DOF IC-GN with the
  * LocalICGN bad-point rule (LocalICGN.m:119-139) and refill, the step-1 Subpb2 +
  * nodal strain + dual update at fixed beta, and n_admm iterations of
- * (2-DOF IC-GN -> Subpb2 -> nodal strain -> dual update) (main_ALDIC.m:357-451).
- * Smoothing (funSmoothDisp/funSmoothStrain) is off. */
+ * (2-DOF IC-GN with Subpb1's bad-point rule and gridfit refit of every node
+ * (Subpb1.m:165-199) -> Subpb2 -> nodal strain -> dual update)
+ * (main_ALDIC.m:357-451).  Smoothing (funSmoothDisp/funSmoothStrain) is off. */
 typedef struct {
     int ws, step;
     int s[2];          /* search radius */
@@ -134,6 +145,7 @@ typedef struct {
     int n_admm;        /* ADMM iterations after step 1 */
     int cg_maxit;      /* PCG iterations per Subpb2 */
     double cg_rtol;    /* PCG relative residual */
+    int no_subpb1_fit; /* 1: skip Subpb1's gridfit refit (bad nodes are only refilled) */
 } aldic_frame_opts;
 
 int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,

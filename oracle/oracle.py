@@ -222,6 +222,99 @@ def nodal_strain(coords, elems, U, swap_fix=False):
     return F
 
 
+# ---- gridfit (func/gridfit.m), the options Subpb1 uses ------------------------------
+
+def gridfit_system(x, y, z, xnodes, ynodes, smoothness=1.0):
+    """Least-squares system of gridfit.m with interp 'triangle', regularizer 'springs',
+    autoscale on, no mask, no tiling (gridfit.m:295-306 defaults; Subpb1.m:188-189 sets
+    'regularizer','springs').  Returns (A, Areg, rhs, lam) with the fit minimising
+    |A z - rhs|^2 + lam^2 |Areg z|^2, lam = smoothness * |A|_1 / |Areg|_1 (:651-654).
+    Grid index of node (ix, iy) is iy + ny * ix (MATLAB column-major (ny, nx))."""
+    import scipy.sparse as sp
+    x = np.asarray(x, np.float64).ravel()
+    y = np.asarray(y, np.float64).ravel()
+    z = np.asarray(z, np.float64).ravel()
+    xnodes = np.asarray(xnodes, np.float64).ravel()
+    ynodes = np.asarray(ynodes, np.float64).ravel()
+    nx, ny = xnodes.size, ynodes.size
+    dx, dy = np.diff(xnodes), np.diff(ynodes)
+    xscale, yscale = dx.mean(), dy.mean()                       # :361-365
+    n = x.size
+    # :460-469 cell of each point (histc), the last node counted in the last cell
+    indx = np.searchsorted(xnodes, x, side="right")             # 1-based histc bin
+    indy = np.searchsorted(ynodes, y, side="right")
+    indx[indx == nx] = nx - 1
+    indy[indy == ny] = ny - 1
+    ind = (indy - 1) + ny * (indx - 1)                          # 0-based grid index
+    # :496-511 triangle interpolation inside the cell
+    tx = np.clip((x - xnodes[indx - 1]) / dx[indx - 1], 0.0, 1.0)
+    ty = np.clip((y - ynodes[indy - 1]) / dy[indy - 1], 0.0, 1.0)
+    L = np.where(tx > ty, ny, 1)
+    t1, t2 = np.minimum(tx, ty), np.maximum(tx, ty)
+    rows = np.repeat(np.arange(n), 3)
+    cols = np.stack([ind, ind + ny + 1, ind + L], 1).ravel()
+    vals = np.stack([1 - t2, t1, t2 - t1], 1).ravel()
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, nx * ny))
+    # :540-569 zero-rest-length springs: y neighbours, x neighbours, both diagonals
+    blocks = []
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny - 1))        # ind = j + ny i
+    ii, jj = i.ravel(), j.ravel()
+    k = jj + ny * ii
+    st = 1.0 / (dy[jj] / yscale)
+    blocks.append((k, k + 1, st))
+    i, j = np.meshgrid(np.arange(nx - 1), np.arange(ny))
+    ii, jj = i.ravel(), j.ravel()
+    k = jj + ny * ii
+    st = 1.0 / (dx[ii] / xscale)
+    blocks.append((k, k + ny, st))
+    i, j = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1))
+    ii, jj = i.ravel(), j.ravel()
+    k = jj + ny * ii
+    st = 1.0 / np.sqrt((dx[ii] / xscale) ** 2 + (dy[jj] / yscale) ** 2)
+    blocks.append((k, k + ny + 1, st))
+    blocks.append((k + 1, k + ny, st))
+    r0, rr, cc, vv = 0, [], [], []
+    for a, b, st in blocks:
+        m = a.size
+        rr += [np.arange(r0, r0 + m)] * 2
+        cc += [a, b]
+        vv += [-st, st]
+        r0 += m
+    Areg = sp.csr_matrix((np.concatenate(vv), (np.concatenate(rr), np.concatenate(cc))),
+                         shape=(r0, nx * ny))
+    NA = abs(A).sum(axis=0).max()                               # norm(A,1)
+    NR = abs(Areg).sum(axis=0).max()
+    return A, Areg, z, smoothness * NA / NR
+
+
+def gridfit(x, y, z, xnodes, ynodes, smoothness=1.0):
+    """gridfit.m (Subpb1's options): zgrid (ny x nx).  MATLAB's backslash gives the
+    least-squares solution of [A; lam Areg] z = [rhs; 0]; here through the normal
+    equations (the regularised system is well conditioned), FP64 sparse direct."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    A, Areg, rhs, lam = gridfit_system(x, y, z, xnodes, ynodes, smoothness)
+    N = (A.T @ A + lam * lam * (Areg.T @ Areg)).tocsc()
+    zz = spl.spsolve(N, A.T @ rhs)
+    return zz.reshape(np.asarray(xnodes).size, np.asarray(ynodes).size).T
+
+
+def subpb1_smooth(U, coords):
+    """Subpb1.m:185-199: gridfit ('springs') of u and v over the nodes whose u is not
+    NaN, evaluated back at every node."""
+    coords = np.asarray(coords, np.float64).reshape(-1, 2)
+    U = np.asarray(U, np.float64).copy()
+    xn, yn = np.unique(coords[:, 0]), np.unique(coords[:, 1])
+    ok = ~np.isnan(U[0::2])
+    ix = np.searchsorted(xn, coords[:, 0])
+    iy = np.searchsorted(yn, coords[:, 1])
+    zu = gridfit(coords[ok, 0], coords[ok, 1], U[0::2][ok], xn, yn)
+    zv = gridfit(coords[ok, 0], coords[ok, 1], U[1::2][ok], xn, yn)
+    U[0::2] = zu[iy, ix]
+    U[1::2] = zv[iy, ix]
+    return U
+
+
 # ---- whole frame (main_ALDIC.m:87-451, FEM branch, smoothing off) -------------------
 
 def badpoint(iters, mean_rule):
@@ -271,12 +364,13 @@ def axis_fill(V, nc, bad, Nx, Ny):
 
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-          cg_rtol=1e-10):
+          cg_rtol=1e-10, subpb1_fit=True):
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
-    refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1 2-DOF IC-GN with
-    refill, Subpb2, strain, duals).  The refill of bad nodes (the reference's
-    scatteredInterpolant 'natural', LocalICGN.m:144-161) is axis_fill; parity unpinned.
-    Returns the stages' outputs and times."""
+    refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1: 2-DOF IC-GN, bad
+    points, gridfit of every node; Subpb2, strain, duals).  The LocalICGN refill of bad
+    nodes (the reference's scatteredInterpolant 'natural', LocalICGN.m:144-161) is
+    axis_fill; parity unpinned.  subpb1_fit=False replaces Subpb1's gridfit with the
+    same axis_fill of its bad nodes.  Returns the stages' outputs and times."""
     import time
     import scipy.sparse as sp
     T = {}
@@ -329,7 +423,10 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
         U1, it2 = icgn2(f, g, gx, gy, xy, ws, H, mu, udual, vdual, U2, F2, tol)
         bad = badpoint(it2, False)
         U1.reshape(-1, 2)[bad] = np.nan
-        U1 = axis_fill(U1, 2, bad, Nx, Ny)
+        if subpb1_fit:
+            U1 = subpb1_smooth(U1, xy)              # Subpb1.m:185-199
+        else:
+            U1 = axis_fill(U1, 2, bad, Nx, Ny)
         F1 = F2
         T["icgn2"] += time.perf_counter() - t
         U2, F2, udual, vdual = global_step(U1, F1, udual, vdual)

@@ -297,3 +297,27 @@ def test_frame_matches_oracle_frame_config1(gpu_ctx, cfg1, orc):
     assert np.sqrt(np.mean((out["U"][0::2] - ut) ** 2)) < 0.05
     t = gpu_ctx.frame_times()
     assert t["frame"] > 0
+
+
+@pytest.mark.parametrize("case", ["uniform_122x100", "uniform_140x131", "nonuniform"])
+def test_gridfit_matches_oracle(gpu_ctx, orc, case):
+    # Subpb1's gridfit (FP64 Jacobi-PCG on the normal equations) against the oracle's
+    # sparse direct least squares
+    _flush(f"gridfit case {case}")
+    rng = np.random.default_rng(11)
+    if case == "uniform_122x100":
+        xn, yn = 17.0 + 16 * np.arange(122), 17.0 + 16 * np.arange(100)
+    elif case == "uniform_140x131":
+        xn, yn = 9.0 + 8 * np.arange(140), 9.0 + 8 * np.arange(131)
+    else:
+        xn = np.cumsum(rng.uniform(5, 15, 23))
+        yn = np.cumsum(rng.uniform(5, 15, 19))
+    X, Y = np.meshgrid(xn, yn)
+    z = 2.0 * np.sin(2 * np.pi * X / 256.0) + 0.3 * np.cos(Y / 40.0) + 0.01 * rng.normal(size=X.shape)
+    z[rng.uniform(size=z.shape) < 0.05] = np.nan
+    z[5:9, 6:10] = np.nan                            # a hole
+    ok = np.isfinite(z)
+    zg, st = gpu_ctx.gridfit(xn, yn, z, rtol=1e-13)
+    zo = orc.gridfit(X[ok], Y[ok], z[ok], xn, yn)
+    assert np.max(np.abs(zg - zo)) <= 1e-9 * np.max(np.abs(zo)), (st, np.max(np.abs(zg - zo)))
+    assert 1 <= st[0] < 4000 and st[1] <= 1e-12, st

@@ -408,3 +408,72 @@ def test_axis_fill_rules(orc, pkg):
     keep = ~bad
     assert np.array_equal(out.reshape(-1, 2)[keep], V.reshape(-1, 2)[keep])
     assert np.array_equal(pkg.aldic.axis_fill(V, 2, bad, Nx, Ny), out)
+
+
+# ---- gridfit (Subpb1's smoothing) ---------------------------------------------------
+
+def _grid_nodes(nx, ny, hx, hy, x0=10.0, y0=20.0):
+    xn = x0 + hx * np.arange(nx)
+    yn = y0 + hy * np.arange(ny)
+    X, Y = np.meshgrid(xn, yn)
+    return xn, yn, X.ravel(), Y.ravel()
+
+
+def test_gridfit_spring_energy_and_norm(orc):
+    # |Areg z|^2 against an explicit loop over the springs of a uniform grid (autoscale
+    # makes every axis spring 1 and every diagonal 1/sqrt(2) whatever hx, hy are);
+    # |Areg|_1 = 4 + 2 sqrt(2) with an interior node, 3 + sqrt(2) for two columns
+    xn, yn, x, y = _grid_nodes(6, 5, 16.0, 12.0)
+    A, Areg, rhs, lam = orc.gridfit_system(x, y, np.zeros_like(x), xn, yn)
+    rng = np.random.default_rng(4)
+    zg = rng.normal(size=(5, 6))                    # zgrid (ny, nx)
+    zvec = zg.T.ravel()                             # index iy + ny * ix
+    e = 0.0
+    for iy in range(5):
+        for ix in range(6):
+            if ix + 1 < 6: e += (zg[iy, ix + 1] - zg[iy, ix]) ** 2
+            if iy + 1 < 5: e += (zg[iy + 1, ix] - zg[iy, ix]) ** 2
+            if ix + 1 < 6 and iy + 1 < 5:
+                e += 0.5 * (zg[iy + 1, ix + 1] - zg[iy, ix]) ** 2
+                e += 0.5 * (zg[iy + 1, ix] - zg[iy, ix + 1]) ** 2
+    assert np.isclose(np.sum((Areg @ zvec) ** 2), e, rtol=1e-13)
+    assert np.isclose(1.0 / lam, 4 + 2 * np.sqrt(2), rtol=1e-14)
+    xn2, yn2, x2, y2 = _grid_nodes(2, 5, 16.0, 16.0)
+    lam2 = orc.gridfit_system(x2, y2, np.zeros_like(x2), xn2, yn2)[3]
+    assert np.isclose(1.0 / lam2, 3 + np.sqrt(2), rtol=1e-14)
+
+
+def test_gridfit_triangle_interpolation_reproduces_linear(orc):
+    # off-node data: each row of A holds the barycentric weights of the point's
+    # triangle, so a linear field on the nodes is reproduced exactly at the points;
+    # on-node data select exactly one node
+    xn, yn, X, Y = _grid_nodes(5, 4, 10.0, 10.0)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(xn[0], xn[-1], 50)
+    y = rng.uniform(yn[0], yn[-1], 50)
+    A = orc.gridfit_system(x, y, np.zeros(50), xn, yn)[0]
+    lin = lambda px, py: 0.3 + 0.02 * px - 0.05 * py
+    Xg, Yg = np.meshgrid(xn, yn)
+    zvec = lin(Xg, Yg).T.ravel()
+    assert np.allclose(A @ zvec, lin(x, y), atol=1e-12)
+    A2 = orc.gridfit_system(X, Y, np.zeros_like(X), xn, yn)[0].toarray()
+    assert np.array_equal((A2 != 0).sum(1), np.ones(X.size))
+    assert np.allclose(A2 @ zvec, lin(X, Y), atol=1e-12)
+
+
+def test_gridfit_least_squares_and_invariances(orc):
+    # the fit is the least-squares solution of [A; lam Areg] (dense QR); constants pass
+    # through unchanged; scaling all coordinates leaves it unchanged (autoscale)
+    xn, yn, X, Y = _grid_nodes(7, 6, 16.0, 16.0)
+    rng = np.random.default_rng(6)
+    keep = rng.uniform(size=X.size) > 0.15
+    z = np.sin(X / 30.0) + 0.1 * rng.normal(size=X.size)
+    zg = orc.gridfit(X[keep], Y[keep], z[keep], xn, yn)
+    A, Areg, rhs, lam = orc.gridfit_system(X[keep], Y[keep], z[keep], xn, yn)
+    M = np.vstack([A.toarray(), lam * Areg.toarray()])
+    ref = np.linalg.lstsq(M, np.concatenate([rhs, np.zeros(Areg.shape[0])]), rcond=None)[0]
+    assert np.allclose(zg.T.ravel(), ref, atol=1e-12)
+    assert np.allclose(orc.gridfit(X[keep], Y[keep], np.full(keep.sum(), 2.5), xn, yn), 2.5,
+                       atol=1e-13)
+    zs = orc.gridfit(3 * X[keep], 3 * Y[keep], z[keep], 3 * xn, 3 * yn)
+    assert np.allclose(zs, zg, atol=1e-12)
