@@ -68,15 +68,16 @@ struct GridfitDev {
     int gx = 0, gy = 0;
     double lam2 = 0.0;
     DBuf w;        // squared stiffnesses [x springs | y springs | (dx/xs)^2 | (dy/ys)^2]
-    MeshDev csr;   // 9-point pattern and values for the PCG path
+    MeshDev csr;   // 9-point pattern, values of N and the inverse of its diagonal
 };
 hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const double *ynodes,
                          int gy, hipStream_t st);
-// U: ncomp (1 or 2) interleaved values per node (node = iy * gx + ix), the initial
-// guess on entry and the fit on return; bad[node] != 0: the node carries no data.
-// stats_dev (nullable, device): per component {iterations, relative residual}.
-hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int ncomp, double rtol,
-                         int maxit, DBuf *work, double *stats_dev, hipStream_t st, int poll);
+// U: two interleaved values per node (node = iy * gx + ix), the initial guess on entry
+// and the fit of both on return (one CG on the pair); bad[node] != 0: the node carries
+// no data.  stats_dev (nullable, device): {iterations, relative residual of the pair}.
+// work[0..1] are used.  poll > 0: a host check for convergence every poll iterations.
+hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
+                         DBuf *work, double *stats_dev, hipStream_t st, int poll);
 
 struct FrameState {
     bool ready = false;

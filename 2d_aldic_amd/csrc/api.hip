@@ -648,7 +648,7 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
     for (int j = 0; j + 1 < gy; ++j)
         if (!(ynodes[j + 1] > ynodes[j])) return fail(ctx, ALDIC_ERR_ARG, "gridfit: ynodes not increasing");
     const size_t n = (size_t)gx * gy;
-    std::vector<double> u(n);
+    std::vector<double> u(2 * n, 0.0);  // (z, 0) pairs: the solver fits two components
     std::vector<int> bad(n);
     double sum = 0.0;
     size_t cnt = 0;
@@ -657,20 +657,21 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
     if (cnt < 3) return fail(ctx, ALDIC_ERR_ARG, "gridfit: insufficient data (fewer than 3 points)");
     for (size_t i = 0; i < n; ++i) {
         bad[i] = std::isfinite(z[i]) ? 0 : 1;
-        u[i] = bad[i] ? sum / cnt : z[i];  // starting guess at the empty nodes
+        u[2 * i] = bad[i] ? sum / cnt : z[i];  // starting guess at the empty nodes
     }
     CK(hipSetDevice(ctx->device));
     CK(gridfit_setup(ctx->gf, xnodes, gx, ynodes, gy, ctx->stream));
     DBuf *sc = ctx->scratch;
-    CK(up(sc[0], u.data(), n, ctx->stream));
+    CK(up(sc[0], u.data(), 2 * n, ctx->stream));
     CK(up(sc[1], bad.data(), n, ctx->stream));
     CK(sc[2].ensure(4 * sizeof(double)));
     static thread_local DBuf work[8];
-    CK(gridfit_solve(ctx->gf, sc[1].as<int>(), sc[0].as<double>(), 1, rtol, maxit, work,
+    CK(gridfit_solve(ctx->gf, sc[1].as<int>(), sc[0].as<double>(), rtol, maxit, work,
                      sc[2].as<double>(), ctx->stream, 8));
-    CK(down(out, sc[0].p, n, ctx->stream));
+    CK(down(u.data(), sc[0].p, 2 * n, ctx->stream));
     CK(down(stats, sc[2].p, 2, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n; ++i) out[i] = u[2 * i];
     return ALDIC_OK;
 }
 
@@ -942,7 +943,7 @@ int aldic_frame_run(aldic_ctx *ctx)
         // Subpb1.m:185-199: gridfit of u and v over the good nodes, written to every node
         // (the refilled values above are only its starting guess)
         if (!o.no_subpb1_fit)
-            CK(gridfit_solve(F.gf, F.bad.as<int>(), F.U1.as<double>(), 2, 1e-10, 4000, work,
+            CK(gridfit_solve(F.gf, F.bad.as<int>(), F.U1.as<double>(), 1e-10, 4000, work,
                              nullptr, st, 8));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         cur ^= 1;

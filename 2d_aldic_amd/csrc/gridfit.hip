@@ -13,8 +13,8 @@
 // 1/sqrt((dx/xscale)^2 + (dy/yscale)^2) (gridfit.m:540-569; L weights = stiffness^2).
 //
 // The FP64 CSR matrix of N (9-point pattern) is assembled on the device from the data
-// mask and solved by the Jacobi-PCG of the FEM step (two kernels per iteration, no host
-// round trip between convergence polls).
+// mask and solved by Jacobi-preconditioned conjugate gradients with one kernel per
+// iteration (no host round trip between convergence polls).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -26,44 +26,190 @@ namespace aldic {
 
 namespace {
 
-// CSR values of N = D + lam2 L for general weights; pattern: node itself then its
-// existing neighbours in the order the host built them (rowptr, col)
-__global__ __launch_bounds__(256) void k_gridfit_csr(int gx, int gy, const double *w,
-                                                     double lam2, const int *bad,
-                                                     const int *rowptr, const int *col,
-                                                     double *aval, double *diag, int ncomp,
-                                                     const double *U, double2 *b)
+constexpr int kGfThreads = 256;
+constexpr int kGfMaxBlocks = 512;
+
+// Sum of one partial array, identical in every workgroup (fixed order).
+__device__ __forceinline__ double gf_part(const double *p, int nb)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= gx * gy) return;
-    const double *wx = w, *wy = wx + (gx - 1), *axs = wy + (gy - 1), *ays = axs + (gx - 1);
-    const int ix = i % gx, iy = i / gx;
-    const bool gd = !bad[i];
-    double dsum = 0.0;
-    for (int k = rowptr[i] + 1; k < rowptr[i + 1]; ++k) {
-        const int j = col[k], jx = j % gx, jy = j / gx;
-        double wt;
-        if (jy == iy) wt = wx[min(ix, jx)];
-        else if (jx == ix) wt = wy[min(iy, jy)];
-        else wt = 1.0 / (axs[min(ix, jx)] + ays[min(iy, jy)]);
-        aval[k] = -lam2 * wt;
-        dsum += wt;
-    }
-    const double d = (gd ? 1.0 : 0.0) + lam2 * dsum;
-    aval[rowptr[i]] = d;
-    diag[i] = d;
-    const double u = gd ? U[(size_t)i * ncomp] : 0.0;
-    const double v = gd && ncomp > 1 ? U[(size_t)i * ncomp + 1] : 0.0;
-    b[i] = make_double2(u, v);
+    double s = 0.0;
+    for (int k = threadIdx.x & 63; k < nb; k += 64) s += p[k];
+    return wave_sum(s);
 }
 
-__global__ __launch_bounds__(256) void k_gridfit_csr_out(int n, int ncomp, const double2 *x,
-                                                         double *U)
+// block sums of three values, written by thread 0 to o0[b], o1[b], o2[b]
+__device__ __forceinline__ void gf_block3(double a, double b, double c, double *o0, double *o1,
+                                          double *o2)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    U[(size_t)i * ncomp] = x[i].x;
-    if (ncomp > 1) U[(size_t)i * ncomp + 1] = x[i].y;
+    __shared__ double red[3][kGfThreads / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = a; red[1][w] = b; red[2][w] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int q = 0; q < kGfThreads / 64; ++q) { s0 += red[0][q]; s1 += red[1][q]; s2 += red[2][q]; }
+        o0[blockIdx.x] = s0;
+        if (o1) o1[blockIdx.x] = s1;
+        if (o2) o2[blockIdx.x] = s2;
+    }
+}
+
+// Conjugate gradients with one global reduction per iteration (Chronopoulos & Gear
+// 1989), Jacobi preconditioner M = diag(N), on the pair of systems N z = b (the u and
+// v components, one shared scalar recurrence), one kernel per iteration.  Kernel i
+// forms u_{i+1} = M^-1 r_{i+1} at its neighbours from the previous kernel's r_i, w_i,
+// s_{i-1} (r_{i+1} = r_i - alpha_i (w_i + beta_i s_{i-1})), so w_{i+1} = N u_{i+1}
+// needs no second grid-wide step.
+struct GfCg {
+    int n;
+    const int *rowptr, *col;
+    double *aval, *dinv;       // values of N, inverse of its diagonal (M^-1)
+    double2 *x;                // in: starting guess, out: solution (interleaved u, v)
+    double2 *r[2], *w[2], *s[2], *p;
+    double *G[2], *D[2], *RR[2], *BB;  // per-workgroup partials (gamma, delta, r.r, b.b)
+    double *hist;              // [parity][2] = {alpha_i, gamma_i}, written by workgroup 0
+    double *stat;              // {done, iterations, relative residual}
+    int nb;
+    double rtol2;
+};
+
+// K0: rows of N from the data mask (squared spring stiffnesses w), b = D d with the
+// data d = x at the data nodes, r_0 = b - N x_0; p and s start at 0.
+__global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, int gx, int gy, const double *w,
+                                                       double lam2, const int *bad)
+{
+    const double *wx = w, *wy = wx + (gx - 1), *axs = wy + (gy - 1), *ays = axs + (gx - 1);
+    double bb = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
+        const int ix = i % gx, iy = i / gx;
+        const bool gd = !bad[i];
+        double dsum = 0.0;
+        const double2 xi = c.x[i];
+        double au = 0.0, av = 0.0;
+        for (int k = c.rowptr[i] + 1; k < c.rowptr[i + 1]; ++k) {
+            const int j = c.col[k], jx = j % gx, jy = j / gx;
+            double wt;
+            if (jy == iy) wt = wx[min(ix, jx)];
+            else if (jx == ix) wt = wy[min(iy, jy)];
+            else wt = 1.0 / (axs[min(ix, jx)] + ays[min(iy, jy)]);
+            const double av_ = -lam2 * wt;
+            c.aval[k] = av_;
+            dsum += wt;
+            const double2 xj = c.x[j];
+            au += av_ * xj.x;
+            av += av_ * xj.y;
+        }
+        const double d = (gd ? 1.0 : 0.0) + lam2 * dsum;
+        c.aval[c.rowptr[i]] = d;
+        c.dinv[i] = 1.0 / d;
+        au += d * xi.x;
+        av += d * xi.y;
+        const double2 b = gd ? xi : make_double2(0.0, 0.0);
+        c.r[0][i] = make_double2(b.x - au, b.y - av);
+        c.p[i] = make_double2(0.0, 0.0);
+        c.s[0][i] = make_double2(0.0, 0.0);
+        bb += b.x * b.x + b.y * b.y;
+    }
+    gf_block3(bb, 0.0, 0.0, c.BB, nullptr, nullptr);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { c.stat[0] = 0.0; c.stat[1] = 0.0; c.stat[2] = 1.0; }
+}
+
+// K1: w_0 = N u_0 with u_0 = M^-1 r_0; partials gamma_0, delta_0, r_0.r_0
+__global__ __launch_bounds__(kGfThreads) void k_gf_cg1(GfCg c)
+{
+    double g = 0.0, dl = 0.0, rr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
+        double wu = 0.0, wv = 0.0;
+        for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+            const int j = c.col[k];
+            const double2 rj = c.r[0][j];
+            const double dinv = c.dinv[j];
+            wu += c.aval[k] * (rj.x * dinv);
+            wv += c.aval[k] * (rj.y * dinv);
+        }
+        const double2 ri = c.r[0][i];
+        const double dinv = c.dinv[i];
+        const double uu = ri.x * dinv, uv = ri.y * dinv;
+        c.w[0][i] = make_double2(wu, wv);
+        g += ri.x * uu + ri.y * uv;
+        dl += wu * uu + wv * uv;
+        rr += ri.x * ri.x + ri.y * ri.y;
+    }
+    gf_block3(g, dl, rr, c.G[0], c.D[0], c.RR[0]);
+}
+
+// iteration i (reads parity i & 1, writes (i + 1) & 1)
+__global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
+{
+    if (c.stat[0] != 0.0) return;  // sticky stop: later partials are stale
+    const int cur = i & 1, nxt = cur ^ 1;
+    __shared__ double sh[4];
+    {
+        const int w = threadIdx.x >> 6;
+        const double *ps[4] = {c.G[cur], c.D[cur], c.RR[cur], c.BB};
+        const double v = gf_part(ps[w], c.nb);
+        if ((threadIdx.x & 63) == 0) sh[w] = v;
+        __syncthreads();
+    }
+    const double gam = sh[0], del = sh[1], rr = sh[2], bb = sh[3];
+    double alpha, beta;
+    if (i == 0) {
+        beta = 0.0;
+        alpha = gam / del;
+    } else {
+        const double *h = c.hist + 2 * nxt;  // {alpha_{i-1}, gamma_{i-1}}
+        beta = gam / h[1];
+        alpha = gam / (del - beta * gam / h[0]);
+    }
+    // stop at the tolerance, on b == 0, or on a breakdown (non-finite or non-positive
+    // step: the system is symmetric positive definite)
+    const bool done = !(rr > c.rtol2 * bb) || !(alpha > 0.0) || !isfinite(alpha) ||
+                      !isfinite(beta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.stat[0] = done ? 1.0 : 0.0;
+        c.stat[1] = (double)i;
+        c.stat[2] = bb > 0 ? sqrt(rr / bb) : 0.0;
+        c.hist[2 * cur] = alpha;
+        c.hist[2 * cur + 1] = gam;
+    }
+    if (done) return;
+    double g = 0.0, dl = 0.0, rn = 0.0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < c.n; q += gridDim.x * blockDim.x) {
+        double wu = 0.0, wv = 0.0;
+        double2 rnew_i = make_double2(0.0, 0.0), snew_i = rnew_i;
+        for (int k = c.rowptr[q]; k < c.rowptr[q + 1]; ++k) {
+            const int j = c.col[k];
+            const double2 rj = c.r[cur][j], wj = c.w[cur][j], sj = c.s[cur][j];
+            const double2 sn = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
+            const double2 rnj = make_double2(rj.x - alpha * sn.x, rj.y - alpha * sn.y);
+            const double dinv = c.dinv[j];
+            wu += c.aval[k] * (rnj.x * dinv);
+            wv += c.aval[k] * (rnj.y * dinv);
+            if (j == q) { rnew_i = rnj; snew_i = sn; }
+        }
+        const double dinv = c.dinv[q];
+        const double2 ri = c.r[cur][q];
+        // p_i = u_i + beta p_{i-1}, x_{i+1} = x_i + alpha p_i
+        double2 pi = c.p[q];
+        pi.x = ri.x * dinv + beta * pi.x;
+        pi.y = ri.y * dinv + beta * pi.y;
+        c.p[q] = pi;
+        double2 xi = c.x[q];
+        xi.x += alpha * pi.x;
+        xi.y += alpha * pi.y;
+        c.x[q] = xi;
+        c.r[nxt][q] = rnew_i;
+        c.s[nxt][q] = snew_i;
+        c.w[nxt][q] = make_double2(wu, wv);
+        const double uu = rnew_i.x * dinv, uv = rnew_i.y * dinv;
+        g += rnew_i.x * uu + rnew_i.y * uv;
+        dl += wu * uu + wv * uv;
+        rn += rnew_i.x * rnew_i.x + rnew_i.y * rnew_i.y;
+    }
+    gf_block3(g, dl, rn, c.G[nxt], c.D[nxt], c.RR[nxt]);
 }
 
 }  // namespace
@@ -151,32 +297,56 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
     return hipStreamSynchronize(st);
 }
 
-hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int ncomp, double rtol,
-                         int maxit, DBuf *work, double *stats_dev, hipStream_t st, int poll)
+hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
+                         DBuf *work, double *stats_dev, hipStream_t st, int poll)
 {
-    if (ncomp < 1 || ncomp > 2) return hipErrorInvalidValue;
-    const int n = g.gx * g.gy, nb = (n + 255) / 256;
+    const int n = g.gx * g.gy;
+    const int nb = std::min(kGfMaxBlocks, (n + kGfThreads - 1) / kGfThreads);
     hipError_t e;
-    if ((e = work[7].ensure(2 * sizeof(double2) * (size_t)n)) != hipSuccess) return e;
-    double2 *b = work[7].as<double2>(), *x = b + n;
-    MeshDev &m = g.csr;
-    hipLaunchKernelGGL(k_gridfit_csr, dim3(nb), dim3(256), 0, st, g.gx, g.gy,
-                       (const double *)g.w.p, g.lam2, bad, m.rowptr.as<int>(), m.col.as<int>(),
-                       m.aval.as<double>(), m.diag.as<double>(), ncomp, (const double *)U, b);
+    const size_t vb = sizeof(double2) * (size_t)n;
+    if ((e = work[0].ensure(7 * vb)) != hipSuccess) return e;
+    if ((e = work[1].ensure(sizeof(double) * (7 * (size_t)nb + 8))) != hipSuccess) return e;
+    GfCg c{};
+    c.n = n;
+    c.rowptr = g.csr.rowptr.as<int>();
+    c.col = g.csr.col.as<int>();
+    c.aval = g.csr.aval.as<double>();
+    c.dinv = g.csr.diag.as<double>();
+    c.x = reinterpret_cast<double2 *>(U);
+    double2 *v = work[0].as<double2>();
+    c.r[0] = v; c.r[1] = v + n; c.w[0] = v + 2 * n; c.w[1] = v + 3 * n;
+    c.s[0] = v + 4 * n; c.s[1] = v + 5 * n; c.p = v + 6 * n;
+    double *ps = work[1].as<double>();
+    c.G[0] = ps; c.G[1] = ps + nb; c.D[0] = ps + 2 * nb; c.D[1] = ps + 3 * nb;
+    c.RR[0] = ps + 4 * nb; c.RR[1] = ps + 5 * nb; c.BB = ps + 6 * nb;
+    c.hist = ps + 7 * nb;
+    c.stat = ps + 7 * nb + 4;
+    c.nb = nb;
+    c.rtol2 = rtol * rtol;
+    hipLaunchKernelGGL(k_gf_cg0, dim3(nb), dim3(kGfThreads), 0, st, c, g.gx, g.gy,
+                       (const double *)g.w.p, g.lam2, bad);
+    hipLaunchKernelGGL(k_gf_cg1, dim3(nb), dim3(kGfThreads), 0, st, c);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    double hs[2] = {0, 0};
-    if ((e = fem_pcg(m, (const double *)b, (double *)x, rtol, maxit, work, hs, st, poll)) !=
-        hipSuccess)
-        return e;
-    hipLaunchKernelGGL(k_gridfit_csr_out, dim3(nb), dim3(256), 0, st, n, ncomp,
-                       (const double2 *)x, U);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (stats_dev) {
-        const double s4[4] = {hs[0], hs[1], hs[0], hs[1]};
-        e = hipMemcpyAsync(stats_dev, s4, sizeof s4, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const int chunk = poll > 0 ? poll : maxit;
+    for (int it = 0; it < maxit; it += chunk) {
+        for (int q = 0; q < chunk && it + q < maxit; ++q)
+            hipLaunchKernelGGL(k_gf_cgit, dim3(nb), dim3(kGfThreads), 0, st, c, it + q);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (poll <= 0) break;
+        double done = 0.0;
+        if ((e = hipMemcpyAsync(&done, c.stat, sizeof(double), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        if (done != 0.0) break;
     }
-    return e;
+    if (stats_dev) {
+        // {iterations, relative residual} from the last iteration kernel's record
+        if ((e = hipMemcpyAsync(stats_dev, c.stat + 1, 2 * sizeof(double),
+                                hipMemcpyDeviceToDevice, st)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace aldic
