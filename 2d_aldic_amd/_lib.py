@@ -23,6 +23,7 @@ EXPORTS = (
     "aldic_integer_search", "aldic_icgn6", "aldic_icgn2", "aldic_set_mesh", "aldic_subpb2",
     "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
+    "aldic_frame_counters",
 )
 
 
@@ -69,6 +70,7 @@ def load():
     L.aldic_frame_get.argtypes = [P, vp, vp, vp, vp, vp, vp, vp]
     L.aldic_frame_times.argtypes = [P, vp, ip]
     L.aldic_gridfit.argtypes = [P, ip, ip, vp, vp, vp, dp, ip, vp, vp]
+    L.aldic_frame_counters.argtypes = [P, vp, ip]
     _lib = L
     return L
 
@@ -287,3 +289,10 @@ class Context:
         self._ck(self.L.aldic_frame_times(self.h, _p(ms), 8), "aldic_frame_times")
         keys = ("prep", "search", "icgn6", "post6", "subpb2", "icgn2", "strain_dual", "frame")
         return dict(zip(keys, ms.tolist()))
+
+    def frame_counters(self):
+        """{gridfit iteration budget of the next frame, frames re-run with polled
+        gridfits, most gridfit iterations in the last frame}."""
+        v = np.zeros(3, np.int32)
+        self._ck(self.L.aldic_frame_counters(self.h, _p(v), 3), "aldic_frame_counters")
+        return dict(zip(("gf_budget", "gf_reruns", "gf_most"), v.tolist()))

@@ -68,14 +68,17 @@ struct GridfitDev {
     int gx = 0, gy = 0;
     double lam2 = 0.0;
     DBuf w;        // squared stiffnesses [x springs | y springs | (dx/xs)^2 | (dy/ys)^2]
-    MeshDev csr;   // 9-point pattern, values of N and the inverse of its diagonal
+    DBuf a9, dinv; // 9-point stencil of N (slot-major), inverse of its diagonal
 };
 hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const double *ynodes,
                          int gy, hipStream_t st);
 // U: two interleaved values per node (node = iy * gx + ix), the initial guess on entry
 // and the fit of both on return (one CG on the pair); bad[node] != 0: the node carries
-// no data.  stats_dev (nullable, device): {iterations, relative residual of the pair}.
-// work[0..1] are used.  poll > 0: a host check for convergence every poll iterations.
+// no data.  stats_dev (nullable, device): {converged (1) or stopped on a breakdown (1)
+// or not (0), iterations, relative residual of the pair}.  work[0..1] are used.
+// poll > 0: a host check for convergence every poll iterations, up to maxit; poll <= 0:
+// exactly maxit iteration kernels (those after convergence return at once) and no host
+// wait, so the caller reads stats_dev to learn whether maxit sufficed.
 hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
                          DBuf *work, double *stats_dev, hipStream_t st, int poll);
 
@@ -87,6 +90,10 @@ struct FrameState {
         bad, tmp, part, cc, rl, F2b, gfstat;
     GridfitDev gf;  // Subpb1's gridfit over the node grid
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
+    double *gfh = nullptr;  // pinned: {converged, iterations, residual} per ADMM gridfit
+    int gf_budget = 16;     // iteration kernels per gridfit before a frame is re-run
+    int gf_reruns = 0;      // frames re-run with polled gridfits
+    int gf_most = 0;        // most gridfit iterations in the last frame
     std::vector<double> hcoords;
     std::vector<int> helems;
     std::vector<hipEvent_t> ev;  // stage-boundary events of frame_run

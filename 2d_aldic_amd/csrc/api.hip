@@ -224,9 +224,11 @@ void aldic_destroy(aldic_ctx *ctx)
     for (DBuf *b : fb) b->release();
     for (aldic::GridfitDev *g : {&F.gf, &ctx->gf}) {
         g->w.release();
-        g->csr.rowptr.release(); g->csr.col.release(); g->csr.aval.release(); g->csr.diag.release();
+        g->a9.release();
+        g->dinv.release();
     }
     F.gfstat.release();
+    if (F.gfh) (void)hipHostFree(F.gfh);
     for (auto &e : F.ev) (void)hipEventDestroy(e);
     F.ev.clear();
     for (auto &e : ctx->ev)
@@ -669,7 +671,7 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
     CK(gridfit_solve(ctx->gf, sc[1].as<int>(), sc[0].as<double>(), rtol, maxit, work,
                      sc[2].as<double>(), ctx->stream, 8));
     CK(down(u.data(), sc[0].p, 2 * n, ctx->stream));
-    CK(down(stats, sc[2].p, 2, ctx->stream));
+    CK(down(stats, sc[2].as<double>() + 1, 2, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < n; ++i) out[i] = u[2 * i];
     return ALDIC_OK;
@@ -762,7 +764,13 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.part.ensure(3 * 256 * sizeof(double)));
     CK(F.cc.ensure(8 * sizeof(double) * S));
     CK(F.rl.ensure(sizeof(int) * (S + 1)));
-    CK(F.gfstat.ensure(4 * sizeof(double)));
+    CK(F.gfstat.ensure(3 * sizeof(double) * (o->n_admm + 1)));
+    F.gf_budget = 16;
+    F.gf_reruns = 0;
+    F.gf_most = 0;
+    if (F.gfh) CK(hipHostFree(F.gfh));
+    F.gfh = nullptr;
+    CK(hipHostMalloc((void **)&F.gfh, 3 * sizeof(double) * (o->n_admm + 1), hipHostMallocDefault));
     CK(F.F2b.ensure(sizeof(double) * 4 * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
@@ -782,12 +790,12 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     return ALDIC_OK;
 }
 
-int aldic_frame_run(aldic_ctx *ctx)
+// One pass of the frame.  gf_poll false: each Subpb1 gridfit runs a fixed budget of
+// iteration kernels with no host wait and records whether it converged in F.gfh; true:
+// each polls the host for convergence.
+static int frame_pass(aldic_ctx *ctx, bool gf_poll)
 {
-    if (!ctx) return ALDIC_ERR_ARG;
     FrameState &F = ctx->fr;
-    if (!F.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_run: frame_setup first");
-    CK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const aldic_frame_opts &o = F.o;
     const int S = F.S;
@@ -943,8 +951,9 @@ int aldic_frame_run(aldic_ctx *ctx)
         // Subpb1.m:185-199: gridfit of u and v over the good nodes, written to every node
         // (the refilled values above are only its starting guess)
         if (!o.no_subpb1_fit)
-            CK(gridfit_solve(F.gf, F.bad.as<int>(), F.U1.as<double>(), 1e-10, 4000, work,
-                             nullptr, st, 8));
+            CK(gridfit_solve(F.gf, F.bad.as<int>(), F.U1.as<double>(), 1e-10,
+                             gf_poll ? 4000 : F.gf_budget, work,
+                             F.gfstat.as<double>() + 3 * step, st, gf_poll ? 8 : 0));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         cur ^= 1;
         span(5, i0, mark());
@@ -952,6 +961,9 @@ int aldic_frame_run(aldic_ctx *ctx)
         if (r) return r;
     }
     F.f2cur = cur;
+    if (!o.no_subpb1_fit && o.n_admm > 0)
+        CK(hipMemcpyAsync(F.gfh, F.gfstat.p, 3 * sizeof(double) * o.n_admm,
+                          hipMemcpyDeviceToHost, st));
     const int e_end = mark();
     span(7, e_start, e_end);
     if (e_end < 0) return fail(ctx, ALDIC_ERR_HIP, "frame_run: event pool exhausted");
@@ -963,6 +975,43 @@ int aldic_frame_run(aldic_ctx *ctx)
         acc[spans[k]] += t;
     }
     for (int k = 0; k < 8; ++k) F.ms[k] = acc[k];
+    return ALDIC_OK;
+}
+
+int aldic_frame_run(aldic_ctx *ctx)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    FrameState &F = ctx->fr;
+    if (!F.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_run: frame_setup first");
+    CK(hipSetDevice(ctx->device));
+    // The frame runs without a host round trip; should a gridfit have needed more than
+    // its iteration budget, the frame runs again with polled gridfits (it depends only
+    // on the staged images and options, so the second pass gives the polled result).
+    int r = frame_pass(ctx, false);
+    if (r) return r;
+    if (F.o.no_subpb1_fit || F.o.n_admm == 0) return ALDIC_OK;
+    bool ok = true;
+    int most = 0;
+    for (int k = 0; k < F.o.n_admm; ++k) {
+        ok = ok && F.gfh[3 * k] != 0.0;
+        most = std::max(most, (int)F.gfh[3 * k + 1]);
+    }
+    if (!ok) {
+        ++F.gf_reruns;
+        if ((r = frame_pass(ctx, true)) != ALDIC_OK) return r;
+        for (int k = 0; k < F.o.n_admm; ++k) most = std::max(most, (int)F.gfh[3 * k + 1]);
+    }
+    // the next frame's budget: the most iterations seen here and a margin
+    F.gf_most = most;
+    F.gf_budget = std::min(64, std::max(6, most + 3));
+    return ALDIC_OK;
+}
+
+int aldic_frame_counters(aldic_ctx *ctx, int *out, int n)
+{
+    if (!ctx || !out || n < 0 || n > 3) return ALDIC_ERR_ARG;
+    const int v[3] = {ctx->fr.gf_budget, ctx->fr.gf_reruns, ctx->fr.gf_most};
+    for (int k = 0; k < n; ++k) out[k] = v[k];
     return ALDIC_OK;
 }
 

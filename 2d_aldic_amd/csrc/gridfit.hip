@@ -12,9 +12,8 @@
 // axis springs of stiffness xscale/dx, yscale/dy and both diagonals of every cell with
 // 1/sqrt((dx/xscale)^2 + (dy/yscale)^2) (gridfit.m:540-569; L weights = stiffness^2).
 //
-// The FP64 CSR matrix of N (9-point pattern) is assembled on the device from the data
-// mask and solved by Jacobi-preconditioned conjugate gradients with one kernel per
-// iteration (no host round trip between convergence polls).
+// The FP64 9-point stencil of N is assembled on the device from the data mask and
+// solved by Jacobi-preconditioned conjugate gradients with one kernel per iteration.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -62,11 +61,12 @@ __device__ __forceinline__ void gf_block3(double a, double b, double c, double *
 // v components, one shared scalar recurrence), one kernel per iteration.  Kernel i
 // forms u_{i+1} = M^-1 r_{i+1} at its neighbours from the previous kernel's r_i, w_i,
 // s_{i-1} (r_{i+1} = r_i - alpha_i (w_i + beta_i s_{i-1})), so w_{i+1} = N u_{i+1}
-// needs no second grid-wide step.
+// needs no second grid-wide step.  N has the 9-point stencil of the node grid: its
+// values sit slot-major, a[k * n + node] for the neighbour (dx, dy) = (k % 3 - 1,
+// k / 3 - 1), so every load of an iteration is independent of the others.
 struct GfCg {
-    int n;
-    const int *rowptr, *col;
-    double *aval, *dinv;       // values of N, inverse of its diagonal (M^-1)
+    int n, gx, gy;
+    double *a9, *dinv;         // stencil values of N (0 outside the grid), 1 / diag(N)
     double2 *x;                // in: starting guess, out: solution (interleaved u, v)
     double2 *r[2], *w[2], *s[2], *p;
     double *G[2], *D[2], *RR[2], *BB;  // per-workgroup partials (gamma, delta, r.r, b.b)
@@ -76,35 +76,46 @@ struct GfCg {
     double rtol2;
 };
 
-// K0: rows of N from the data mask (squared spring stiffnesses w), b = D d with the
-// data d = x at the data nodes, r_0 = b - N x_0; p and s start at 0.
-__global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, int gx, int gy, const double *w,
-                                                       double lam2, const int *bad)
+__device__ __forceinline__ bool gf_in(int ix, int iy, int k, int gx, int gy)
 {
+    const int jx = ix + k % 3 - 1, jy = iy + k / 3 - 1;
+    return jx >= 0 && jx < gx && jy >= 0 && jy < gy;
+}
+
+// K0: stencil of N from the data mask (squared spring stiffnesses w), b = D d with the
+// data d = x at the data nodes, r_0 = b - N x_0; p and s start at 0.
+__global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, const double *w, double lam2,
+                                                       const int *bad)
+{
+    const int gx = c.gx, gy = c.gy;
     const double *wx = w, *wy = wx + (gx - 1), *axs = wy + (gy - 1), *ays = axs + (gx - 1);
     double bb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
         const int ix = i % gx, iy = i / gx;
-        const bool gd = !bad[i];
-        double dsum = 0.0;
-        const double2 xi = c.x[i];
-        double au = 0.0, av = 0.0;
-        for (int k = c.rowptr[i] + 1; k < c.rowptr[i + 1]; ++k) {
-            const int j = c.col[k], jx = j % gx, jy = j / gx;
-            double wt;
-            if (jy == iy) wt = wx[min(ix, jx)];
-            else if (jx == ix) wt = wy[min(iy, jy)];
-            else wt = 1.0 / (axs[min(ix, jx)] + ays[min(iy, jy)]);
-            const double av_ = -lam2 * wt;
-            c.aval[k] = av_;
-            dsum += wt;
-            const double2 xj = c.x[j];
-            au += av_ * xj.x;
-            av += av_ * xj.y;
+        double dsum = 0.0, au = 0.0, av = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (k == 4) continue;
+            double v = 0.0;
+            if (gf_in(ix, iy, k, gx, gy)) {
+                const int dx = k % 3 - 1, dy = k / 3 - 1;
+                double wt;
+                if (dy == 0) wt = wx[min(ix, ix + dx)];
+                else if (dx == 0) wt = wy[min(iy, iy + dy)];
+                else wt = 1.0 / (axs[min(ix, ix + dx)] + ays[min(iy, iy + dy)]);
+                v = -lam2 * wt;
+                dsum += wt;
+                const double2 xj = c.x[i + dy * gx + dx];
+                au += v * xj.x;
+                av += v * xj.y;
+            }
+            c.a9[k * c.n + i] = v;
         }
+        const bool gd = !bad[i];
         const double d = (gd ? 1.0 : 0.0) + lam2 * dsum;
-        c.aval[c.rowptr[i]] = d;
+        c.a9[4 * c.n + i] = d;
         c.dinv[i] = 1.0 / d;
+        const double2 xi = c.x[i];
         au += d * xi.x;
         av += d * xi.y;
         const double2 b = gd ? xi : make_double2(0.0, 0.0);
@@ -120,15 +131,19 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, int gx, int gy, c
 // K1: w_0 = N u_0 with u_0 = M^-1 r_0; partials gamma_0, delta_0, r_0.r_0
 __global__ __launch_bounds__(kGfThreads) void k_gf_cg1(GfCg c)
 {
+    const int gx = c.gx, gy = c.gy;
     double g = 0.0, dl = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
+        const int ix = i % gx, iy = i / gx;
         double wu = 0.0, wv = 0.0;
-        for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
-            const int j = c.col[k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (!gf_in(ix, iy, k, gx, gy)) continue;
+            const int j = i + (k / 3 - 1) * gx + (k % 3 - 1);
             const double2 rj = c.r[0][j];
-            const double dinv = c.dinv[j];
-            wu += c.aval[k] * (rj.x * dinv);
-            wv += c.aval[k] * (rj.y * dinv);
+            const double av = c.a9[k * c.n + i] * c.dinv[j];
+            wu += av * rj.x;
+            wv += av * rj.y;
         }
         const double2 ri = c.r[0][i];
         const double dinv = c.dinv[i];
@@ -144,8 +159,8 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg1(GfCg c)
 // iteration i (reads parity i & 1, writes (i + 1) & 1)
 __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
 {
-    if (c.stat[0] != 0.0) return;  // sticky stop: later partials are stale
     const int cur = i & 1, nxt = cur ^ 1;
+    const double stop = c.stat[0];  // sticky: later partials are stale
     __shared__ double sh[4];
     {
         const int w = threadIdx.x >> 6;
@@ -154,6 +169,7 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
         if ((threadIdx.x & 63) == 0) sh[w] = v;
         __syncthreads();
     }
+    if (stop != 0.0) return;
     const double gam = sh[0], del = sh[1], rr = sh[2], bb = sh[3];
     double alpha, beta;
     if (i == 0) {
@@ -176,19 +192,23 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
         c.hist[2 * cur + 1] = gam;
     }
     if (done) return;
+    const int gx = c.gx, gy = c.gy;
     double g = 0.0, dl = 0.0, rn = 0.0;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < c.n; q += gridDim.x * blockDim.x) {
+        const int ix = q % gx, iy = q / gx;
         double wu = 0.0, wv = 0.0;
         double2 rnew_i = make_double2(0.0, 0.0), snew_i = rnew_i;
-        for (int k = c.rowptr[q]; k < c.rowptr[q + 1]; ++k) {
-            const int j = c.col[k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (!gf_in(ix, iy, k, gx, gy)) continue;
+            const int j = q + (k / 3 - 1) * gx + (k % 3 - 1);
             const double2 rj = c.r[cur][j], wj = c.w[cur][j], sj = c.s[cur][j];
             const double2 sn = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
             const double2 rnj = make_double2(rj.x - alpha * sn.x, rj.y - alpha * sn.y);
-            const double dinv = c.dinv[j];
-            wu += c.aval[k] * (rnj.x * dinv);
-            wv += c.aval[k] * (rnj.y * dinv);
-            if (j == q) { rnew_i = rnj; snew_i = sn; }
+            const double av = c.a9[k * c.n + q] * c.dinv[j];
+            wu += av * rnj.x;
+            wv += av * rnj.y;
+            if (k == 4) { rnew_i = rnj; snew_i = sn; }
         }
         const double dinv = c.dinv[q];
         const double2 ri = c.r[cur][q];
@@ -262,38 +282,10 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
     if ((e = hipMemcpyAsync(g.w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice,
                             st)) != hipSuccess)
         return e;
-    {
-        // 9-point pattern: the node first, then its neighbours
-        std::vector<int> rp(gx * gy + 1), cl;
-        cl.reserve(9 * (size_t)gx * gy);
-        for (int iy = 0; iy < gy; ++iy)
-            for (int ix = 0; ix < gx; ++ix) {
-                const int i = iy * gx + ix;
-                rp[i] = (int)cl.size();
-                cl.push_back(i);
-                for (int dyy = -1; dyy <= 1; ++dyy)
-                    for (int dxx = -1; dxx <= 1; ++dxx) {
-                        const int jx = ix + dxx, jy = iy + dyy;
-                        if ((dxx || dyy) && jx >= 0 && jx < gx && jy >= 0 && jy < gy)
-                            cl.push_back(jy * gx + jx);
-                    }
-            }
-        rp[gx * gy] = (int)cl.size();
-        MeshDev &m = g.csr;
-        m.nn = gx * gy;
-        m.nnz = (int)cl.size();
-        if ((e = m.rowptr.ensure(sizeof(int) * rp.size())) != hipSuccess) return e;
-        if ((e = m.col.ensure(sizeof(int) * cl.size())) != hipSuccess) return e;
-        if ((e = m.aval.ensure(sizeof(double) * cl.size())) != hipSuccess) return e;
-        if ((e = m.diag.ensure(sizeof(double) * (size_t)m.nn)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(m.rowptr.p, rp.data(), sizeof(int) * rp.size(),
-                                hipMemcpyHostToDevice, st)) != hipSuccess)
-            return e;
-        if ((e = hipMemcpyAsync(m.col.p, cl.data(), sizeof(int) * cl.size(),
-                                hipMemcpyHostToDevice, st)) != hipSuccess)
-            return e;
-    }
-    // the host vectors above are copied before they go out of scope
+    const size_t n = (size_t)gx * gy;
+    if ((e = g.a9.ensure(9 * sizeof(double) * n)) != hipSuccess) return e;
+    if ((e = g.dinv.ensure(sizeof(double) * n)) != hipSuccess) return e;
+    // the host vector above is copied before it goes out of scope
     return hipStreamSynchronize(st);
 }
 
@@ -308,10 +300,10 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, 
     if ((e = work[1].ensure(sizeof(double) * (7 * (size_t)nb + 8))) != hipSuccess) return e;
     GfCg c{};
     c.n = n;
-    c.rowptr = g.csr.rowptr.as<int>();
-    c.col = g.csr.col.as<int>();
-    c.aval = g.csr.aval.as<double>();
-    c.dinv = g.csr.diag.as<double>();
+    c.gx = g.gx;
+    c.gy = g.gy;
+    c.a9 = g.a9.as<double>();
+    c.dinv = g.dinv.as<double>();
     c.x = reinterpret_cast<double2 *>(U);
     double2 *v = work[0].as<double2>();
     c.r[0] = v; c.r[1] = v + n; c.w[0] = v + 2 * n; c.w[1] = v + 3 * n;
@@ -323,10 +315,11 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, 
     c.stat = ps + 7 * nb + 4;
     c.nb = nb;
     c.rtol2 = rtol * rtol;
-    hipLaunchKernelGGL(k_gf_cg0, dim3(nb), dim3(kGfThreads), 0, st, c, g.gx, g.gy,
-                       (const double *)g.w.p, g.lam2, bad);
+    hipLaunchKernelGGL(k_gf_cg0, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
+                       g.lam2, bad);
     hipLaunchKernelGGL(k_gf_cg1, dim3(nb), dim3(kGfThreads), 0, st, c);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    // poll <= 0: maxit iteration kernels and no host wait (the caller reads stat)
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
         for (int q = 0; q < chunk && it + q < maxit; ++q)
@@ -341,8 +334,8 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, 
         if (done != 0.0) break;
     }
     if (stats_dev) {
-        // {iterations, relative residual} from the last iteration kernel's record
-        if ((e = hipMemcpyAsync(stats_dev, c.stat + 1, 2 * sizeof(double),
+        // {done, iterations, relative residual} from the last iteration kernel's record
+        if ((e = hipMemcpyAsync(stats_dev, c.stat, 3 * sizeof(double),
                                 hipMemcpyDeviceToDevice, st)) != hipSuccess)
             return e;
     }

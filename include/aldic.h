@@ -159,6 +159,11 @@ int aldic_frame_get(aldic_ctx *ctx, double *U, double *F, double *U6, double *F6
 /* Stage times of the last frame in ms: {prep, search, icgn6, post6, subpb2 total,
  * icgn2 total, strain+dual total, frame}. n <= 8. */
 int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
+/* Counters of the frame driver (no counterpart in the reference): {iteration kernels
+ * each Subpb1 gridfit of the next frame runs before its convergence is checked,
+ * frames run a second time because a gridfit needed more, largest gridfit iteration
+ * count of the last frame}. n <= 3. */
+int aldic_frame_counters(aldic_ctx *ctx, int *out, int n);
 
 #ifdef __cplusplus
 }

@@ -275,6 +275,35 @@ def test_frame_flat_patch_enlarged_windows(gpu_ctx, cfg1, orc):
     assert np.sqrt(np.mean((out["U"][0::2][good] - ut[good]) ** 2)) < 0.1
 
 
+def test_frame_gridfit_over_budget_reruns(gpu_ctx, cfg1, orc):
+    """A wide textureless patch leaves a block of bad nodes, so Subpb1's gridfit needs
+    more CG iterations than the first frame's budget: the frame runs again with polled
+    gridfits, and the next frame fits the budget without a second pass."""
+    f = cfg1.f.copy()
+    g = cfg1.g.copy()
+    f[70:170, 60:180] = 100
+    g[70:170, 62:182] = 100
+    _flush("frame with a large flat patch (gridfit re-run)")
+    gpu_ctx.frame_stage(f, g)
+    gpu_ctx.frame_setup(20, 10, 8, [1, 256, 1, 256], tol=1e-2, mu=1e-3, beta=0.1, n_admm=3)
+    gpu_ctx.frame_run()
+    c1 = gpu_ctx.frame_counters()
+    out = gpu_ctx.frame_get()
+    assert c1["gf_most"] > 16 and c1["gf_reruns"] == 1, c1
+    assert c1["gf_budget"] == min(64, c1["gf_most"] + 3)
+    gpu_ctx.frame_run()
+    c2 = gpu_ctx.frame_counters()
+    assert c2["gf_reruns"] == 1 and c2["gf_most"] == c1["gf_most"], (c1, c2)
+    out2 = gpu_ctx.frame_get()
+    assert np.array_equal(out["U"], out2["U"]) and np.array_equal(out["F"], out2["F"])
+    o = orc.frame(f, g, 20, 10, 8, [1, 256, 1, 256], tol=1e-2, mu=1e-3, beta=0.1, n_admm=3)
+    assert np.mean(o["iters6"] == 102) > 0.1
+    if np.array_equal(out["iters6"], o["iters6"]) and np.array_equal(out["iters2"], o["iters2"]):
+        assert np.max(np.abs(out["U"] - o["U"])) < 1e-4
+    ut, vt = cfg1.ufun(o["xy"][:, 0], o["xy"][:, 1])
+    assert np.sqrt(np.mean((out["U"][0::2] - ut) ** 2)) < 0.2
+
+
 def test_frame_matches_oracle_frame_config1(gpu_ctx, cfg1, orc):
     # whole device frame (prep, search, LocalICGN + bad-point refill, 1 + 3 ADMM global
     # steps) against the FP64 restatement of the same frame
