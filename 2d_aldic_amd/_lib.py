@@ -31,7 +31,7 @@ class FrameOpts(C.Structure):
     _fields_ = [("ws", C.c_int), ("step", C.c_int), ("s", C.c_int * 2), ("grid", C.c_int * 4),
                 ("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
                 ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
-                ("no_subpb1_fit", C.c_int)]
+                ("no_subpb1_fit", C.c_int), ("stage_times", C.c_int)]
 
 
 _lib = None
@@ -253,7 +253,7 @@ class Context:
         self.nx, self.ny = nx, ny
 
     def frame_setup(self, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-                    cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True):
+                    cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True, stage_times=True):
         o = FrameOpts()
         o.ws, o.step = int(ws), int(step)
         o.s[0], o.s[1] = (int(s), int(s)) if np.isscalar(s) else (int(s[0]), int(s[1]))
@@ -266,6 +266,7 @@ class Context:
         o.cg_maxit = cg_maxit
         o.cg_rtol = cg_rtol
         o.no_subpb1_fit = 0 if subpb1_fit else 1
+        o.stage_times = 1 if stage_times else 0
         n = C.c_int(0)
         self._ck(self.L.aldic_frame_setup(self.h, C.byref(o), C.byref(n)), "aldic_frame_setup")
         self.frame_nodes = n.value

@@ -810,6 +810,8 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         if (ne >= (int)ev.size()) return -1;
         return hipEventRecord(ev[ne], st) == hipSuccess ? ne++ : -1;
     };
+    // boundaries other than the search's and the frame's only with stage_times
+    auto smark = [&]() -> int { return o.stage_times ? mark() : -1; };
     auto span = [&](int stage, int e0, int e1) {
         spans.push_back(stage); spans.push_back(e0); spans.push_back(e1);
     };
@@ -887,12 +889,12 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         ar.nlist_dev = rl + S;
         CK(launch_icgn6(ar, ppl_big, st));
     }
-    const int e_icgn6 = mark();
+    const int e_icgn6 = smark();
     span(2, e_search, e_icgn6);
     // LocalICGN post-processing: bad-point rule and refill (U1, F1 hold copies of U6, F6)
     CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, nullptr, nullptr, F.U1.as<double>(),
                       F.F1.as<double>(), F.bad.as<int>(), F.part.as<double>(), st));
-    const int e_post6 = mark();
+    const int e_post6 = smark();
     span(3, e_icgn6, e_post6);
 
     // Global steps.  F1 is F6 refilled for step 1 and the previous F2 afterwards
@@ -903,18 +905,18 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     int cur = 0;
     const double *F1p = F.F1.as<double>();
     auto global_step = [&](bool zero_duals) -> int {
-        const int g0 = mark();
+        const int g0 = smark();
         CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F1p,
                    zero_duals ? nullptr : F.udual.as<double>(),
                    zero_duals ? nullptr : F.vdual.as<double>(), b, st));
         CK(fem_solve(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
-        const int g1 = mark();
+        const int g1 = smark();
         span(4, g0, g1);
         // nodal strain and the dual update of the FEM branch (main_ALDIC.m:340,440):
         // udual = F2 - F1, vdual = U2 - U1
         CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F2buf[cur], st, F1p, F.U1.as<double>(),
                       F.udual.as<double>(), F.vdual.as<double>()));
-        span(6, g1, mark());
+        span(6, g1, smark());
         return ALDIC_OK;
     };
     int r = global_step(true);
@@ -922,7 +924,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
 
     for (int step = 0; step < o.n_admm; ++step) {
         // Subpb1: 2-DOF IC-GN from (U2, F2)
-        const int i0 = mark();
+        const int i0 = smark();
         IcgnArgs a2 = a;
         a2.U0 = nullptr;
         a2.Hin = F.H21.as<double>();
@@ -956,7 +958,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
                              F.gfstat.as<double>() + 3 * step, st, gf_poll ? 8 : 0));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         cur ^= 1;
-        span(5, i0, mark());
+        span(5, i0, smark());
         r = global_step(false);
         if (r) return r;
     }

@@ -135,7 +135,10 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
  * nodal strain + dual update at fixed beta, and n_admm iterations of
  * (2-DOF IC-GN with Subpb1's bad-point rule and gridfit refit of every node
  * (Subpb1.m:165-199) -> Subpb2 -> nodal strain -> dual update)
- * (main_ALDIC.m:357-451).  Smoothing (funSmoothDisp/funSmoothStrain) is off. */
+ * (main_ALDIC.m:357-451).  Smoothing (funSmoothDisp/funSmoothStrain) is off.
+ * A gridfit runs a budget of CG iterations (the most the previous frame needed plus
+ * a margin) without a host wait; if one needs more, the frame runs a second time with
+ * gridfits that poll for convergence, so the result never depends on the budget. */
 typedef struct {
     int ws, step;
     int s[2];          /* search radius */
@@ -146,6 +149,9 @@ typedef struct {
     int cg_maxit;      /* PCG iterations per Subpb2 */
     double cg_rtol;    /* PCG relative residual */
     int no_subpb1_fit; /* 1: skip Subpb1's gridfit refit (bad nodes are only refilled) */
+    int stage_times;   /* 1: an event at every stage boundary (all of aldic_frame_times);
+                        * 0: events only around the search and the whole frame (each
+                        * event costs the stream a few microseconds) */
 } aldic_frame_opts;
 
 int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
@@ -157,7 +163,8 @@ int aldic_frame_run(aldic_ctx *ctx);
 int aldic_frame_get(aldic_ctx *ctx, double *U, double *F, double *U6, double *F6, int *iters6,
                     int *iters2, double *u0);
 /* Stage times of the last frame in ms: {prep, search, icgn6, post6, subpb2 total,
- * icgn2 total, strain+dual total, frame}. n <= 8. */
+ * icgn2 total, strain+dual total, frame}; with stage_times 0 only search and frame
+ * are measured (the others read 0). n <= 8. */
 int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
 /* Counters of the frame driver (no counterpart in the reference): {iteration kernels
  * each Subpb1 gridfit of the next frame runs before its convergence is checked,
