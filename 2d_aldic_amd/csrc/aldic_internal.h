@@ -113,7 +113,8 @@ struct aldic_ctx {
     int raw_dtype = 0;
     int roi[4] = {0, 0, 0, 0};
     bool have_mask = false;
-    aldic::DBuf stats;               // double[8] image statistics
+    bool images_from_raw = false;    // f, g normalised on the device from rawf, rawg
+    aldic::DBuf stats;               // ROI partial sums, then {mean_f, 1/std_f, mean_g, 1/std_g}
     aldic::DBuf scratch[8];          // per-call work buffers
     aldic::MeshDev mesh;
     bool have_mesh = false;
@@ -171,9 +172,17 @@ struct SearchArgs {
     double *x, *y, *u, *v, *ccmax, *qf;
     int *status;
     double *U0;             // optional interleaved output (2S), may be nullptr
+    // raw 8-bit pair and {mean_f, 1/std_f, mean_g, 1/std_g} of the normalisation that
+    // made f, g: when set (and the window fits) the exact int8 matrix-core search runs
+    const uint8_t *f8 = nullptr, *g8 = nullptr;
+    const double *ms = nullptr;
 };
+// Runs the int8 matrix-core search when a.f8, a.g8, a.ms are set and the geometry fits
+// (2s+3 <= 32 lags per axis, ws + 2 s + 3 <= 64), else the FP32 search on a.f, a.g.
 hipError_t launch_search(const SearchArgs &a, hipStream_t st);
 size_t search_lds_bytes(int ws, int sx, int sy);
+bool search_i8_supported(int ws, int sx, int sy);
+hipError_t launch_search_i8(const SearchArgs &a, hipStream_t st);
 
 hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const double *Y,
                           long n, double *Z, hipStream_t st);

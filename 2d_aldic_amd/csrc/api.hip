@@ -87,6 +87,15 @@ hipError_t run_prep(aldic_ctx *c)
                        c->gx.as<float>(), c->gy.as<float>(), c->stream);
 }
 
+// 8-bit raw images behind f, g: the search runs on them exactly (int8 matrix cores)
+void search_raw8(aldic_ctx *c, SearchArgs &a)
+{
+    if (!c->images_from_raw || c->raw_dtype != ALDIC_U8) return;
+    a.f8 = c->rawf.as<uint8_t>();
+    a.g8 = c->rawg.as<uint8_t>();
+    a.ms = c->stats.as<double>() + 4 * kStatParts;
+}
+
 int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
               const uint8_t *mask, const int *roi)
 {
@@ -96,6 +105,7 @@ int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int 
     const size_t es = dtype_size(dtype);
     if (!es) return fail(ctx, ALDIC_ERR_ARG, "images: unknown dtype");
     const size_t np = (size_t)nx * ny;
+    ctx->have_images = false;  // the normalised planes are made from the new pair later
     CK(ctx->rawf.ensure(es * np));
     CK(ctx->rawg.ensure(es * np));
     CK(hipMemcpyAsync(ctx->rawf.p, f, es * np, hipMemcpyHostToDevice, ctx->stream));
@@ -249,6 +259,7 @@ int aldic_set_images(aldic_ctx *ctx, const void *f, const void *g, int nx, int n
     CK(run_prep(ctx));
     CK(hipStreamSynchronize(ctx->stream));
     ctx->have_images = true;
+    ctx->images_from_raw = true;
     return ALDIC_OK;
 }
 
@@ -275,6 +286,7 @@ int aldic_set_normalized(aldic_ctx *ctx, const double *f, const double *g, int n
                        ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     ctx->have_images = true;
+    ctx->images_from_raw = false;
     return ALDIC_OK;
 }
 
@@ -374,6 +386,7 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
     CK(ctx->scratch[6].ensure(sizeof(int) * (size_t)S));
     a.status = ctx->scratch[6].as<int>();
     a.U0 = nullptr;
+    search_raw8(ctx, a);
     CK(launch_search(a, ctx->stream));
     CK(down(x, a.x, S, ctx->stream));
     CK(down(y, a.y, S, ctx->stream));
@@ -818,6 +831,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     const int e_start = mark();
     CK(run_prep(ctx));
     ctx->have_images = true;
+    ctx->images_from_raw = true;
     const int e_prep = mark();
     span(0, e_start, e_prep);
     // integer search -> U0
@@ -845,6 +859,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     sa.qf = nullptr;
     sa.status = reinterpret_cast<int *>(s4 + 4 * (size_t)S);
     sa.U0 = F.U0.as<double>();
+    search_raw8(ctx, sa);
     CK(launch_search(sa, st));
     // search NaNs (constant templates) are filled like Init's inpaint_nans
     const int e_search = mark();

@@ -21,6 +21,7 @@
 // numerator.  Bands are added in a fixed order, so the argmax is deterministic.
 #include "aldic_internal.h"
 #include "common.h"
+#include "search_peak.h"
 
 namespace aldic {
 
@@ -336,31 +337,8 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
     for (int w = 1; w < 4; ++w)
         if (rv[w] > best || (rv[w] == best && ri[w] < bi)) { best = rv[w]; bi = ri[w]; }
 
-    // findpeak.m:30-66 (row = x-lag, column = y-lag, both 1-based)
-    const int r = bi % G.nrx + 1, c = bi / G.nrx + 1;
-    double xp = c, yp = r, mf = best;
-    if (!(c == 1 || c == G.nry || r == 1 || r == G.nrx)) {
-        double s0 = 0, sxu = 0, syu = 0, sxy = 0, sxx = 0, syy = 0;
-        for (int dc = -1; dc <= 1; ++dc)
-            for (int dr = -1; dr <= 1; ++dr) {
-                const double u = Pn[(c - 1 + dc) * G.nrx + (r - 1 + dr)];
-                s0 += u; sxu += dc * u; syu += dr * u; sxy += dc * dr * u;
-                sxx += dc * dc * u; syy += dr * dr * u;
-            }
-        const double A1 = (5.0 * s0 - 3.0 * sxx - 3.0 * syy) / 9.0;
-        const double A2 = sxu / 6.0, A3 = syu / 6.0, A4 = sxy / 4.0;
-        const double A5 = sxx / 2.0 - s0 / 3.0, A6 = syy / 2.0 - s0 / 3.0;
-        const double den = A4 * A4 - 4.0 * A5 * A6;
-        double xo = (-A3 * A4 + 2.0 * A6 * A2) / den;
-        double yo = -1.0 / den * (A4 * A2 - 2.0 * A5 * A3);
-        if (!(fabs(xo) > 1.0 || fabs(yo) > 1.0) && !isnan(xo) && !isnan(yo)) {
-            xo = xo < 0 ? -floor(-1000.0 * xo + 0.5) / 1000.0 : floor(1000.0 * xo + 0.5) / 1000.0;
-            yo = yo < 0 ? -floor(-1000.0 * yo + 0.5) / 1000.0 : floor(1000.0 * yo + 0.5) / 1000.0;
-            xp = c + xo;
-            yp = r + yo;
-            mf = A1 + A2 * xo + A3 * yo + A4 * xo * yo + A5 * xo * xo + A6 * yo * yo;
-        }
-    }
+    double xp, yp, mf;
+    findpeak_fit(Pn, G.nrx, G.nry, bi, best, xp, yp, mf);
     const double u = yp - (G.sx + 2), v = xp - (G.sy + 2);
     a.x[q] = xc;
     a.y[q] = yc;
@@ -390,6 +368,8 @@ hipError_t launch_search(const SearchArgs &a, hipStream_t st)
 {
     const int S = a.Nx * a.Ny;
     if (S <= 0) return hipSuccess;
+    if (a.f8 && a.g8 && a.ms && search_i8_supported(a.ws, a.sx, a.sy))
+        return launch_search_i8(a, st);
     bool ok;
     const SearchGeom G = search_geom(a.ws, a.sx, a.sy, ok);
     if (!ok) return hipErrorInvalidValue;

@@ -2,9 +2,11 @@
 
 Tolerances (FP32 device arithmetic against FP64):
   * images / gradients / bicubic samples: 2e-6 relative to the data range;
-  * integer search: the integer peak is identical and the sub-pixel result within
-    1e-3 px (+1e-9; findpeak rounds to 1e-3, so an FP32 value next to a rounding
-    boundary may land one step away);
+  * integer search on 8-bit images (exact integer correlation on the int8 matrix
+    cores, FP64 NCC): same status, |du|, |dv| < 1e-9 px, |dccmax| < 1e-12;
+  * integer search on normalised float images (FP32 path): the integer peak is
+    identical and the sub-pixel result within 1e-3 px (+1e-9; findpeak rounds to
+    1e-3, so an FP32 value next to a rounding boundary may land one step away);
   * IC-GN: the same status for >= 97 % of the subsets; where the status agrees and is
     a convergence (1..100): |dU| <= 1e-3 px and |dF| <= 1e-5, RMS dU <= 2e-4 px;
   * Subpb2 / nodal strain: 1e-9 relative (FP64 on both sides).
@@ -19,7 +21,7 @@ def _flush(msg):
     print(msg, flush=True)
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture
 def c1(gpu_ctx, cfg1, cfg1_norm):
     _flush("config1: set_images")
     gpu_ctx.set_images(cfg1.f, cfg1.g)
@@ -47,9 +49,31 @@ def test_bicubic_matches_oracle(c1, cfg1_norm):
     assert np.max(np.abs(Z - R)) < 2e-6 * np.ptp(g)
 
 
+def _search_exact(r, o):
+    assert (r["Nx"], r["Ny"]) == (o["Nx"], o["Ny"])
+    assert np.array_equal(r["x"], o["x"]) and np.array_equal(r["y"], o["y"])
+    assert np.array_equal(r["status"], o["status"])
+    ok = r["status"] == 0
+    assert np.all(np.isnan(r["u"][~ok])) and np.all(np.isnan(r["ccmax"][~ok]))
+    assert np.max(np.abs(r["u"][ok] - o["u"][ok])) < 1e-9
+    assert np.max(np.abs(r["v"][ok] - o["v"][ok])) < 1e-9
+    assert np.max(np.abs(r["ccmax"][ok] - o["ccmax"][ok])) < 1e-12
+
+
 def test_search_matches_oracle_config1(c1, cfg1_norm, orc):
+    # 8-bit images staged raw: the exact int8 matrix-core search
     f, g, _, _ = cfg1_norm
     r = c1.integer_search([1, 256, 1, 256], 20, 10, [8, 8])
+    o = orc.search(f, g, [1, 256, 1, 256], 20, 10, [8, 8])
+    assert (r["Nx"], r["Ny"]) == (o["Nx"], o["Ny"]) == (19, 19)
+    _search_exact(r, o)
+
+
+def test_search_fp32_path_matches_oracle_config1(gpu_ctx, cfg1_norm, orc):
+    # normalised float images: the FP32 search
+    f, g, _, _ = cfg1_norm
+    gpu_ctx.set_normalized(f, g)
+    r = gpu_ctx.integer_search([1, 256, 1, 256], 20, 10, [8, 8])
     o = orc.search(f, g, [1, 256, 1, 256], 20, 10, [8, 8])
     assert (r["Nx"], r["Ny"]) == (o["Nx"], o["Ny"]) == (19, 19)
     assert np.array_equal(r["x"], o["x"]) and np.array_equal(r["y"], o["y"])
@@ -58,6 +82,46 @@ def test_search_matches_oracle_config1(c1, cfg1_norm, orc):
     assert np.max(np.abs(r["v"] - o["v"])) <= 1e-3 + 1e-9
     assert np.mean(r["u"] == o["u"]) > 0.95
     assert np.max(np.abs(r["ccmax"] - o["ccmax"])) < 1e-4
+
+
+@pytest.mark.parametrize("case", ["ws20_s8_peak_at_edge", "ws16_s4", "ws32_s3x7",
+                                  "ws40_s10", "flat_templates", "ws20_s15_fp32_fallback"])
+def test_search_8bit_cases(gpu_ctx, speckle, orc, case):
+    """Geometries of the int8 search (one or two 32-column MFMA steps per template row,
+    unequal radii, 23 x 23 lags), peaks one lag from the crop edge (the 3x3 fit reads
+    the edge lags, whose footprint leaves the window), constant templates (status -1),
+    and a radius beyond 32 lags that falls back to the FP32 search."""
+    _flush(f"search case {case}")
+    ws, sx, sy, u0, v0, step = 20, 8, 8, 7.6, -7.7, 12
+    if case == "ws16_s4":
+        ws, sx, sy, u0, v0 = 16, 4, 4, 2.4, -1.3
+    elif case == "ws32_s3x7":
+        ws, sx, sy, u0, v0 = 32, 3, 7, 1.7, -6.2
+    elif case == "ws40_s10":
+        ws, sx, sy, u0, v0 = 40, 10, 10, -9.3, 4.4
+    elif case == "ws20_s15_fp32_fallback":
+        ws, sx, sy, u0, v0 = 20, 15, 15, 11.2, -3.6
+    p = speckle.small_pair(n=208, u0=u0, v0=v0, ux=0.002, vy=-0.001, seed=31)
+    fr, gr = p.f.copy(), p.g.copy()
+    if case == "flat_templates":
+        fr[60:110, 40:100] = 77
+    gpu_ctx.set_images(fr, gr)
+    grid = [1, 208, 1, 208]
+    r = gpu_ctx.integer_search(grid, ws, step, [sx, sy])
+    f, _ = orc.normalize(fr.astype(float))
+    g, _ = orc.normalize(gr.astype(float))
+    o = orc.search(f, g, grid, ws, step, [sx, sy])
+    if case == "flat_templates":
+        assert np.sum(o["status"] == -1) >= 4
+    if case == "ws20_s15_fp32_fallback":
+        assert np.array_equal(r["status"], o["status"])
+        assert np.max(np.abs(r["u"] - o["u"])) <= 1e-3 + 1e-9
+        assert np.max(np.abs(r["v"] - o["v"])) <= 1e-3 + 1e-9
+        return
+    _search_exact(r, o)
+    if case == "ws20_s8_peak_at_edge":
+        # integer peaks at lag +-s with fits that used the edge lags
+        assert np.mean(np.abs(np.round(o["u"])) == sx) > 0.5
 
 
 def _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.97):
