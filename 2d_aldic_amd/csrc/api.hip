@@ -106,8 +106,9 @@ int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int 
     if (!es) return fail(ctx, ALDIC_ERR_ARG, "images: unknown dtype");
     const size_t np = (size_t)nx * ny;
     ctx->have_images = false;  // the normalised planes are made from the new pair later
-    CK(ctx->rawf.ensure(es * np));
-    CK(ctx->rawg.ensure(es * np));
+    // 64 spare bytes: the 8-bit search reads whole aligned dwords around each row segment
+    CK(ctx->rawf.ensure(es * np + 64));
+    CK(ctx->rawg.ensure(es * np + 64));
     CK(hipMemcpyAsync(ctx->rawf.p, f, es * np, hipMemcpyHostToDevice, ctx->stream));
     CK(hipMemcpyAsync(ctx->rawg.p, g, es * np, hipMemcpyHostToDevice, ctx->stream));
     ctx->have_mask = mask != nullptr;

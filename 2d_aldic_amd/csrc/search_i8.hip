@@ -52,6 +52,7 @@ struct I8Geom {
     int TS, WS;      // template / window row strides in bytes
     int tgrp, wgrp;  // dwords per template row / window row rewritten per subset
     int off_T, off_W, off_S, off_N, per_wave;  // byte offsets in a wave's LDS region
+    // (the N region first holds the vertical box sums, 2 nry VW ints)
 };
 
 int r16(int v) { return (v + 15) & ~15; }
@@ -78,7 +79,7 @@ bool i8_geom(int ws, int sx, int sy, I8Geom &G)
     G.off_W = r16(G.off_T + G.tw * G.TS);
     G.off_S = r16(G.off_W + G.DH * G.WS);
     G.off_N = r16(G.off_S + 8 * G.nry * G.nrx);
-    G.per_wave = r16(G.off_N + 8 * G.nry * G.nrx);
+    G.per_wave = r16(G.off_N + std::max(8 * G.nry * G.nrx, 8 * G.nry * G.VW));
     return kI8Waves * G.per_wave <= 64 * 1024;
 }
 
@@ -96,21 +97,90 @@ __device__ __forceinline__ int wave_scan(int x)
 
 __device__ __forceinline__ int wave_sum_i(int v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-    return v;
+    return __builtin_amdgcn_readlane(wave_scan(v), 63);
 }
 
-__device__ __forceinline__ unsigned pack4(const uint8_t *p, int k0, int k1)
+// Template statistics of one subset for the NCC of a lag (see the header comment).
+struct TplStats {
+    int tw, nrx, nry, st, r0, rl, c0, cl, k00, k0l, kl0, kll;
+    double dn, dst, dstn, d;
+};
+
+// num and den2 = n st * n var of lag p (NCC = num / sqrt(den2)); P, sw, sw2 its sums.
+// Every term is an exact integer below 2^53 in FP64 except the mean correction's
+// one rounding.
+__device__ __forceinline__ void ncc_terms(const TplStats &t, int p, double P, int isw, int isw2,
+                                          double &num, double &den2)
 {
-    // bytes p[k0 .. k1) shifted to int8 (b ^ 0x80), zero outside
-    unsigned v = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k >= k0 && k < k1) v |= (unsigned)(p[k] ^ 0x80u) << (8 * k);
-    return v;
+    const int ly = p / t.nrx, lx = p - ly * t.nrx;
+    const double sw = (double)isw, sw2 = (double)isw2;
+    const bool top = ly == 0, bot = ly == t.nry - 1, lft = lx == 0, rgt = lx == t.nrx - 1;
+    const int ro = (top || bot) ? 1 : 0, co = (lft || rgt) ? 1 : 0;
+    const int rs = top ? t.r0 : (bot ? t.rl : 0);
+    const int cs = lft ? t.c0 : (rgt ? t.cl : 0);
+    const int kc = (ro && co) ? (top ? (lft ? t.k00 : t.k0l) : (lft ? t.kl0 : t.kll)) : 0;
+    const double stin = (double)(t.st - rs - cs + kc);
+    const double nin = (double)((t.tw - ro) * (t.tw - co)), nout = t.dn - nin;
+    num = (t.dn * P - t.dst * sw) + t.d * (t.dn * stin - nin * t.dst);
+    const double var = (t.dn * sw2 - sw * sw) + 2.0 * t.d * sw * nout + t.d * t.d * nin * nout;
+    den2 = t.dstn * (var > 0.0 ? var : 0.0);
 }
 
+// The rows of one subset's template and window, fetched into registers ahead of use
+// (the next subset's loads are in flight while the current one is computed).  Item
+// k of a lane is dword (row, g) = divmod(lane + 64 k, dwords per row) of the template
+// (KT items) or window (KW items); each is two aligned dwords around the row's bytes.
+template <int KT, int KW> struct Prefetch {
+    unsigned t0[KT], t1[KT], w0[KW], w1[KW];
+    int ii, jj;  // 1-based corner of the subset they belong to
+};
+
+struct Walk {
+    int r, g;
+    __device__ __forceinline__ void next(int dr, int dg, int grp)
+    {
+        g += dg;
+        r += dr;
+        if (g >= grp) { g -= grp; ++r; }
+    }
+};
+
+template <int KT, int KW>
+__device__ __forceinline__ void issue_rows(const SearchArgs &a, const I8Geom &G, int q, int lane,
+                                           Walk wt, int djt, int dgt, Walk ww, int djw, int dgw,
+                                           Prefetch<KT, KW> &pf)
+{
+    const int kx = q % a.Nx, ky = q / a.Nx;
+    pf.ii = a.x1 + kx * a.step;
+    pf.jj = a.y1 + ky * a.step;
+    const int nx = a.nx;
+    const int tb = (pf.jj - 1) * nx + (pf.ii - 1);
+    const int nt = G.tw * G.tgrp;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        if (lane + 64 * k < nt) {
+            const int lo = (tb + wt.r * nx + 4 * wt.g) & ~3;
+            pf.t0[k] = *reinterpret_cast<const unsigned *>(a.f8 + lo);
+            pf.t1[k] = *reinterpret_cast<const unsigned *>(a.f8 + lo + 4);
+        }
+        wt.next(djt, dgt, G.tgrp);
+    }
+    // window row r (1 .. Dy) starts at image byte wb + r nx; its column 0 (the zero
+    // ring) may lie one byte before the image, so the offset can be -1
+    const int wb = (pf.jj - G.sy - 2) * nx + (pf.ii - G.sx - 2);
+    const int nw = G.Dy * G.wgrp;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+        if (lane + 64 * k < nw) {
+            const int lo = (wb + (1 + ww.r) * nx + 4 * ww.g) & ~3;
+            pf.w0[k] = *reinterpret_cast<const unsigned *>(a.g8 + (lo < 0 ? 0 : lo));
+            pf.w1[k] = *reinterpret_cast<const unsigned *>(a.g8 + lo + 4);
+        }
+        ww.next(djw, dgw, G.wgrp);
+    }
+}
+
+template <int KT, int KW>
 __global__ __launch_bounds__(64 * kI8Waves) void k_search_i8(SearchArgs a, I8Geom G)
 {
     extern __shared__ __align__(16) uint8_t lds8[];
@@ -131,6 +201,14 @@ __global__ __launch_bounds__(64 * kI8Waves) void k_search_i8(SearchArgs a, I8Geo
     const int lq = lane & 31, h = lane >> 5;
     const int o = kPL - lq + 16 * G.nk * h, b0 = o & ~3, s3 = o & 3;
     const int rowB = min(lq, nry - 1);
+    // staging items it = lane + 64 k: (row, dword) walked without divisions
+    Walk wt0, ww0;
+    wt0.r = lane / G.tgrp;
+    wt0.g = lane - wt0.r * G.tgrp;
+    const int djt = 64 / G.tgrp, dgt = 64 - djt * G.tgrp;
+    ww0.r = lane / G.wgrp;
+    ww0.g = lane - ww0.r * G.wgrp;
+    const int djw = 64 / G.wgrp, dgw = 64 - djw * G.wgrp;
 
     const int S = a.Nx * a.Ny;
     const int nb = gridDim.x, b = blockIdx.x;
@@ -147,45 +225,69 @@ __global__ __launch_bounds__(64 * kI8Waves) void k_search_i8(SearchArgs a, I8Geo
         q1 = S;
         qstep = nb * kI8Waves;
     }
+    Prefetch<KT, KW> pf;
+    if (q0 < q1) issue_rows(a, G, q0, lane, wt0, djt, dgt, ww0, djw, dgw, pf);
     for (int q = q0; q < q1; q += qstep) {
-        const int kx = q % a.Nx, ky = q / a.Nx;
-        const int ii = a.x1 + kx * a.step, jj = a.y1 + ky * a.step;  // 1-based corner
-        // template rows -> T(j, i) at Tl[j TS + kPL + i], and its integer statistics
-        // (|sums| <= 128^2 * 65^2 < 2^31)
+        const int ii = pf.ii, jj = pf.jj;  // 1-based corner
+        // template rows -> T(j, i) at Tl[j TS + kPL + i] (b ^ 0x80 per byte) and its
+        // integer statistics (|sums| <= 128^2 * 65^2 < 2^31)
         int st = 0, st2 = 0, r0 = 0, rl = 0, c0 = 0, cl = 0, k00 = 0, k0l = 0, kl0 = 0, kll = 0;
-        for (int it = lane; it < tw * G.tgrp; it += 64) {
-            const int j = it / G.tgrp, g4 = it - j * G.tgrp;
-            const uint8_t *src = a.f8 + (size_t)(jj - 1 + j) * nx + (ii - 1) + 4 * g4;
-            const int k1 = min(4, tw - 4 * g4);
-            const unsigned v = pack4(src, 0, k1);
-            *reinterpret_cast<unsigned *>(Tl + j * G.TS + kPL + 4 * g4) = v;
+        {
+            Walk w = wt0;
+            const int tb = (jj - 1) * nx + (ii - 1);
+            const int nt = tw * G.tgrp;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (k >= k1) break;
-                const int t = (int)(int8_t)(v >> (8 * k));
-                const int i = 4 * g4 + k;
-                st += t;
-                st2 += t * t;
-                if (j == 0) r0 += t;
-                if (j == tw - 1) rl += t;
-                if (i == 0) c0 += t;
-                if (i == tw - 1) cl += t;
-                if (j == 0 && i == 0) k00 = t;
-                if (j == 0 && i == tw - 1) k0l = t;
-                if (j == tw - 1 && i == 0) kl0 = t;
-                if (j == tw - 1 && i == tw - 1) kll = t;
+            for (int k = 0; k < KT; ++k) {
+                if (lane + 64 * k < nt) {
+                    const int j = w.r, g4 = w.g;
+                    const int off = tb + j * nx + 4 * g4;
+                    const int k1 = min(4, tw - 4 * g4);
+                    const unsigned m = k1 >= 4 ? 0xffffffffu : ((1u << (8 * k1)) - 1u);
+                    const unsigned v = (__builtin_amdgcn_alignbyte(pf.t1[k], pf.t0[k], (unsigned)(off & 3)) ^
+                                        0x80808080u) & m;
+                    *reinterpret_cast<unsigned *>(Tl + j * G.TS + kPL + 4 * g4) = v;
+                    const int s1 = __builtin_amdgcn_sdot4((int)v, 0x01010101, 0, false);
+                    st += s1;
+                    st2 = __builtin_amdgcn_sdot4((int)v, (int)v, st2, false);
+                    if (j == 0) r0 += s1;
+                    if (j == tw - 1) rl += s1;
+                    const int first = (int)(int8_t)(v & 0xffu);
+                    const int last = (int)(int8_t)(v >> (8 * ((tw - 1) & 3)));
+                    const bool gl = g4 == ((tw - 1) >> 2);
+                    if (g4 == 0) c0 += first;
+                    if (gl) cl += last;
+                    if (j == 0 && g4 == 0) k00 = first;
+                    if (j == 0 && gl) k0l = last;
+                    if (j == tw - 1 && g4 == 0) kl0 = first;
+                    if (j == tw - 1 && gl) kll = last;
+                }
+                w.next(djt, dgt, G.tgrp);
             }
         }
         // window rows 1 .. Dy, columns 1 .. Dx (row 0, row Dy + 1, column 0 and columns
         // past Dx stay zero): Wl[r WS + c] = g(ii - sx - 2 + c, jj - sy - 2 + r) - 128
-        for (int it = lane; it < G.Dy * G.wgrp; it += 64) {
-            const int r = 1 + it / G.wgrp, g4 = it - (r - 1) * G.wgrp;
-            const int c = 4 * g4;
-            const uint8_t *src =
-                a.g8 + (size_t)(jj - G.sy - 2 + r) * nx + (ii - G.sx - 2) + c;
-            const int k0 = c == 0 ? 1 : 0, k1 = min(4, G.Dx + 1 - c);
-            *reinterpret_cast<unsigned *>(Wl + r * G.WS + c) = pack4(src, k0, k1);
+        {
+            Walk w = ww0;
+            const int wb = (jj - G.sy - 2) * nx + (ii - G.sx - 2);
+            const int nw = G.Dy * G.wgrp;
+#pragma unroll
+            for (int k = 0; k < KW; ++k) {
+                if (lane + 64 * k < nw) {
+                    const int r = 1 + w.r, g4 = w.g;
+                    const int off = wb + r * nx + 4 * g4;  // >= -1
+                    const int k0 = g4 == 0 ? 1 : 0, k1 = min(4, G.Dx + 1 - 4 * g4);
+                    const unsigned m = (k1 >= 4 ? 0xffffffffu : ((1u << (8 * k1)) - 1u)) &
+                                       (k0 ? 0xffffff00u : 0xffffffffu);
+                    *reinterpret_cast<unsigned *>(Wl + r * G.WS + 4 * g4) =
+                        (__builtin_amdgcn_alignbyte(pf.w1[k], pf.w0[k], (unsigned)(off & 3)) ^
+                         0x80808080u) & m;
+                }
+                w.next(djw, dgw, G.wgrp);
+            }
         }
+        wave_sync_lds();
+        // the next subset's rows load while this one is computed
+        if (q + qstep < q1) issue_rows(a, G, q + qstep, lane, wt0, djt, dgt, ww0, djw, dgw, pf);
         st = wave_sum_i(st);
         st2 = wave_sum_i(st2);
         r0 = wave_sum_i(r0);
@@ -196,7 +298,6 @@ __global__ __launch_bounds__(64 * kI8Waves) void k_search_i8(SearchArgs a, I8Geo
         k0l = wave_sum_i(k0l);
         kl0 = wave_sum_i(kl0);
         kll = wave_sum_i(kll);
-        wave_sync_lds();
 
         const double xc = (ii + ii + a.ws) / 2.0, yc = (jj + jj + a.ws) / 2.0;
         const long long stn = n * st2 - (long long)st * st;
@@ -211,63 +312,96 @@ __global__ __launch_bounds__(64 * kI8Waves) void k_search_i8(SearchArgs a, I8Geo
             continue;
         }
 
-        // box sums of w and w^2 per lag: vertical running sums (a lane per window
-        // column), horizontal sums from a wave prefix scan
+        // box sums of w and w^2 per lag: vertical running sums over tw rows (a lane per
+        // window column) into V, then horizontal running sums over tw columns (a lane
+        // per lag row and quantity) into S
         {
-            const bool colok = lane < G.VW;
-            int v1 = 0, v2 = 0;
-            if (colok)
+            int *V1 = reinterpret_cast<int *>(Nc), *V2 = V1 + nry * G.VW;
+            if (lane < G.VW) {
+                int v1 = 0, v2 = 0;
                 for (int r = 0; r < tw; ++r) {
                     const int w = (int8_t)Wl[r * G.WS + lane];
                     v1 += w;
                     v2 += w * w;
                 }
-            for (int ly = 0; ly < nry; ++ly) {
-                if (ly > 0 && colok) {
+                V1[lane] = v1;
+                V2[lane] = v2;
+                for (int ly = 1; ly < nry; ++ly) {
                     const int wi = (int8_t)Wl[(ly + tw - 1) * G.WS + lane];
                     const int wo = (int8_t)Wl[(ly - 1) * G.WS + lane];
                     v1 += wi - wo;
                     v2 += wi * wi - wo * wo;
+                    V1[ly * G.VW + lane] = v1;
+                    V2[ly * G.VW + lane] = v2;
                 }
-                const int p1 = wave_scan(v1), p2 = wave_scan(v2);
-                const int hi = min(lane + tw - 1, 63), lo = max(lane - 1, 0);
-                const int a1 = __shfl(p1, hi, 64), a2 = __shfl(p2, hi, 64);
-                const int z1 = __shfl(p1, lo, 64), z2 = __shfl(p2, lo, 64);
-                if (lane < nrx) {
-                    S1[ly * nrx + lane] = a1 - (lane > 0 ? z1 : 0);
-                    S2[ly * nrx + lane] = a2 - (lane > 0 ? z2 : 0);
+            }
+            wave_sync_lds();
+            if (lane < 2 * nry) {
+                const int qn = lane >= nry ? 1 : 0, ly = lane - qn * nry;
+                const int *vr = (qn ? V2 : V1) + ly * G.VW;
+                int *so = (qn ? S2 : S1) + ly * nrx;
+                int hs = 0;
+                for (int i = 0; i < tw; ++i) hs += vr[i];
+                so[0] = hs;
+                for (int lx = 1; lx < nrx; ++lx) {
+                    hs += vr[lx - 1 + tw] - vr[lx - 1];
+                    so[lx] = hs;
                 }
             }
         }
 
-        // P = sum_j A_j B_j on the matrix cores
+        // P = sum_j A_j B_j on the matrix cores; row j + 1's operands load before row
+        // j's products issue
         v16i acc;
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[r] = 0;
-        for (int j = 0; j < tw; ++j) {
-            const unsigned *tq = reinterpret_cast<const unsigned *>(Tl + j * G.TS + b0);
-            const uint8_t *wr = Wl + (rowB + j) * G.WS + 16 * G.nk * h;
-            if (G.nk == 1) {
-                unsigned dw[5];
+        if (G.nk == 1) {
+            const unsigned *tq = reinterpret_cast<const unsigned *>(Tl + b0);
+            const uint8_t *wr = Wl + rowB * G.WS + 16 * h;
+            unsigned dw[5];
 #pragma unroll
-                for (int k = 0; k < 5; ++k) dw[k] = tq[k];
-                v4i A, B = *reinterpret_cast<const v4i *>(wr);
+            for (int k = 0; k < 5; ++k) dw[k] = tq[k];
+            v4i B = *reinterpret_cast<const v4i *>(wr);
+            for (int j = 0; j < tw; ++j) {
+                v4i A;
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     A[k] = (int)__builtin_amdgcn_alignbyte(dw[k + 1], dw[k], s3);
-                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B, acc, 0, 0, 0);
-            } else {
-                unsigned dw[9];
+                const v4i Bc = B;
+                if (j + 1 < tw) {
+                    tq += G.TS / 4;
+                    wr += G.WS;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) dw[k] = tq[k];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    v4i A, B = *reinterpret_cast<const v4i *>(wr + 16 * t);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        A[k] = (int)__builtin_amdgcn_alignbyte(dw[4 * t + k + 1], dw[4 * t + k], s3);
-                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B, acc, 0, 0, 0);
+                    for (int k = 0; k < 5; ++k) dw[k] = tq[k];
+                    B = *reinterpret_cast<const v4i *>(wr);
                 }
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, Bc, acc, 0, 0, 0);
+            }
+        } else {
+            const unsigned *tq = reinterpret_cast<const unsigned *>(Tl + b0);
+            const uint8_t *wr = Wl + rowB * G.WS + 32 * h;
+            unsigned dw[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) dw[k] = tq[k];
+            v4i B0 = *reinterpret_cast<const v4i *>(wr), B1 = *reinterpret_cast<const v4i *>(wr + 16);
+            for (int j = 0; j < tw; ++j) {
+                v4i A0, A1;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    A0[k] = (int)__builtin_amdgcn_alignbyte(dw[k + 1], dw[k], s3);
+                    A1[k] = (int)__builtin_amdgcn_alignbyte(dw[k + 5], dw[k + 4], s3);
+                }
+                const v4i C0 = B0, C1 = B1;
+                if (j + 1 < tw) {
+                    tq += G.TS / 4;
+                    wr += G.WS;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) dw[k] = tq[k];
+                    B0 = *reinterpret_cast<const v4i *>(wr);
+                    B1 = *reinterpret_cast<const v4i *>(wr + 16);
+                }
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, C0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, C1, acc, 0, 0, 0);
             }
         }
         wave_sync_lds();
@@ -281,37 +415,50 @@ __global__ __launch_bounds__(64 * kI8Waves) void k_search_i8(SearchArgs a, I8Geo
         }
         wave_sync_lds();
 
-        // NCC per lag in place, and the argmax (ties -> smallest column-major index,
-        // x-lag fastest)
+        // argmax of NCC without dividing: NCC = num / sqrt(den2) has the order of
+        // key / den2 with key = num |num|, compared as key_a den2_b > key_b den2_a
+        // (ties -> smallest column-major index, x-lag fastest)
+        TplStats ts;
+        ts.tw = tw; ts.nrx = nrx; ts.nry = nry;
+        ts.st = st; ts.r0 = r0; ts.rl = rl; ts.c0 = c0; ts.cl = cl;
+        ts.k00 = k00; ts.k0l = k0l; ts.kl0 = kl0; ts.kll = kll;
+        ts.dn = (double)n; ts.dst = (double)st; ts.dstn = (double)stn; ts.d = d;
         const int nl = nrx * nry;
-        double best = -INFINITY;
+        double bk = -INFINITY, bd = 1.0;
         int bi = 0x7fffffff;
         for (int p = lane; p < nl; p += 64) {
-            const int ly = p / nrx, lx = p - ly * nrx;
-            const long long P = (long long)Nc[p];
-            const long long sw = S1[p], sw2 = S2[p];
-            const bool top = ly == 0, bot = ly == nry - 1, lft = lx == 0, rgt = lx == nrx - 1;
-            const int ro = (top || bot) ? 1 : 0, co = (lft || rgt) ? 1 : 0;
-            const int rs = top ? r0 : (bot ? rl : 0);
-            const int cs = lft ? c0 : (rgt ? cl : 0);
-            const int kc = (ro && co) ? (top ? (lft ? k00 : k0l) : (lft ? kl0 : kll)) : 0;
-            const long long stin = st - rs - cs + kc;
-            const long long nin = (long long)(tw - ro) * (tw - co);
-            const double num =
-                (double)(n * P - (long long)st * sw) + d * (double)(n * stin - nin * st);
-            double var = (double)(n * sw2 - sw * sw) + 2.0 * d * (double)sw * (double)(n - nin) +
-                         d * d * (double)nin * (double)(n - nin);
-            var = var > 0.0 ? var : 0.0;
-            const double den = sqrt((double)stn * var);
-            const double c = den > 0.0 ? num / den : 0.0;
-            Nc[p] = c;
-            if (c > best) { best = c; bi = p; }
+            double num, den2;
+            ncc_terms(ts, p, Nc[p], S1[p], S2[p], num, den2);
+            double key = num * fabs(num);
+            if (!(den2 > 0.0)) { key = 0.0; den2 = 1.0; }
+            if (key * bd > bk * den2) { bk = key; bd = den2; bi = p; }
         }
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) {
-            const double ob = __shfl_xor(best, m, 64);
+            const double ok = __shfl_xor(bk, m, 64), od = __shfl_xor(bd, m, 64);
             const int oi = __shfl_xor(bi, m, 64);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            const double l = ok * bd, r = bk * od;
+            if (l > r || (l == r && oi < bi)) { bk = ok; bd = od; bi = oi; }
+        }
+        // exact NCC at the peak and its 3 x 3 neighbourhood (lane k: offset (k%3-1, k/3-1))
+        wave_sync_lds();
+        double best = 0.0;
+        {
+            const int rx = bi % nrx, cy = bi / nrx;
+            const int dxk = lane % 3 - 1, dyk = lane / 3 - 1;
+            const int px = rx + dxk, py = cy + dyk;
+            double c = 0.0;
+            const bool mine = lane < 9 && px >= 0 && px < nrx && py >= 0 && py < nry;
+            if (mine) {
+                double num, den2;
+                const int pp = py * nrx + px;
+                ncc_terms(ts, pp, Nc[pp], S1[pp], S2[pp], num, den2);
+                const double den = sqrt(den2);
+                c = den > 0.0 ? num / den : 0.0;
+            }
+            best = __shfl(c, 4, 64);
+            wave_sync_lds();
+            if (mine) Nc[py * nrx + px] = c;
         }
         wave_sync_lds();
         if (lane == 0) {
@@ -339,24 +486,36 @@ bool search_i8_supported(int ws, int sx, int sy)
     return i8_geom(ws, sx, sy, G);
 }
 
-hipError_t launch_search_i8(const SearchArgs &a, hipStream_t st)
+template <int KT, int KW>
+hipError_t launch_kt(const SearchArgs &a, const I8Geom &G, int S, hipStream_t st)
 {
-    const int S = a.Nx * a.Ny;
-    if (S <= 0) return hipSuccess;
-    I8Geom G;
-    if (!i8_geom(a.ws, a.sx, a.sy, G) || !a.f8 || !a.g8 || !a.ms) return hipErrorInvalidValue;
     const size_t lds = (size_t)kI8Waves * G.per_wave;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)k_search_i8,
+        (void)hipFuncSetAttribute((const void *)k_search_i8<KT, KW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
         attr_set = true;
     }
     // resident workgroups: LDS-limited, and enough to give every wave a few subsets
     const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / lds)));
     const int nwg = std::min((S + kI8Waves - 1) / kI8Waves, 256 * per_cu);
-    hipLaunchKernelGGL(k_search_i8, dim3(nwg), dim3(64 * kI8Waves), lds, st, a, G);
+    auto kern = k_search_i8<KT, KW>;
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(64 * kI8Waves), lds, st, a, G);
     return hipGetLastError();
+}
+
+hipError_t launch_search_i8(const SearchArgs &a, hipStream_t st)
+{
+    const int S = a.Nx * a.Ny;
+    if (S <= 0) return hipSuccess;
+    I8Geom G;
+    if (!i8_geom(a.ws, a.sx, a.sy, G) || !a.f8 || !a.g8 || !a.ms) return hipErrorInvalidValue;
+    // prefetch registers per lane: template and window dwords over 64 lanes
+    const int kt = (G.tw * G.tgrp + 63) / 64, kw = (G.Dy * G.wgrp + 63) / 64;
+    if (kt <= 3 && kw <= 6) return launch_kt<3, 6>(a, G, S, st);
+    if (kt <= 5 && kw <= 10) return launch_kt<5, 10>(a, G, S, st);
+    if (kt <= 8 && kw <= 16) return launch_kt<8, 16>(a, G, S, st);
+    return launch_kt<16, 24>(a, G, S, st);
 }
 
 }  // namespace aldic
