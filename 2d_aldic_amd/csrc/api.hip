@@ -824,7 +824,8 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         if (ne >= (int)ev.size()) return -1;
         return hipEventRecord(ev[ne], st) == hipSuccess ? ne++ : -1;
     };
-    // boundaries other than the search's and the frame's only with stage_times
+    // boundaries other than the search's, the 6-DOF IC-GN's and the frame's only with
+    // stage_times
     auto smark = [&]() -> int { return o.stage_times ? mark() : -1; };
     auto span = [&](int stage, int e0, int e1) {
         spans.push_back(stage); spans.push_back(e0); spans.push_back(e1);
@@ -905,7 +906,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         ar.nlist_dev = rl + S;
         CK(launch_icgn6(ar, ppl_big, st));
     }
-    const int e_icgn6 = smark();
+    const int e_icgn6 = mark();
     span(2, e_search, e_icgn6);
     // LocalICGN post-processing: bad-point rule and refill (U1, F1 hold copies of U6, F6)
     CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, nullptr, nullptr, F.U1.as<double>(),

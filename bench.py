@@ -10,7 +10,7 @@ strain, dual update).  The raw pair is resident in HBM before the timed region.
 value = converged subsets (6-DOF IC-GN status 1..100) per second of whole frame.
 The CPU baseline is the FP64 restatement of the same frame on the host cores.
 
-  python bench.py [--steps K] [--warmup W] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-stages]
 """
 from __future__ import annotations
 
@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32
+INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stages", action="store_true",
+                    help="skip the untimed per-stage breakdown frames")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
@@ -90,53 +92,69 @@ def main():
     for _ in range(args.warmup):
         ctx.frame_run()
     log("[bench] warmup done")
-    search_ms = 0.0
+    search_ms = icgn6_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.frame_run()
-        search_ms += ctx.frame_times()["search"]
+        t = ctx.frame_times()
+        search_ms += t["search"]
+        icgn6_ms += t["icgn6"]
     wall = time.perf_counter() - t0
     search_ms /= args.steps
+    icgn6_ms /= args.steps
     ms_per_step = wall * 1e3 / args.steps
     out = ctx.frame_get()
     counters = ctx.frame_counters()
-    ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
-                    cg_maxit=2000, cg_rtol=1e-10, stage_times=True)
     stage = {}
-    for _ in range(args.warmup + args.steps):
-        ctx.frame_run()
-        for k, v in ctx.frame_times().items():
-            stage[k] = stage.get(k, 0.0) + v
-    stage = {k: v / (args.warmup + args.steps) for k, v in stage.items()}
+    if not args.no_stages:
+        ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
+                        cg_maxit=2000, cg_rtol=1e-10, stage_times=True)
+        for _ in range(args.warmup + args.steps):
+            ctx.frame_run()
+            for k, v in ctx.frame_times().items():
+                stage[k] = stage.get(k, 0.0) + v
+        stage = {k: v / (args.warmup + args.steps) for k, v in stage.items()}
     it6 = out["iters6"]
     conv = int(np.sum((it6 >= 1) & (it6 <= 100)))
     value = conv / (ms_per_step / 1e3)
     log(f"[bench] {ms_per_step:.3f} ms/frame, stages {stage}")
 
-    # Roofline of the dominant kernel: the integer search (one k_search launch per frame,
-    # event-timed on the library's stream).  It is FP32-FMA bound: algorithmic FLOPs =
-    # the NCC numerator, 2 (2s+3)^2 (w+1)^2 per subset.  Its compulsory HBM bytes
-    # (8 N_px + 32 S) are reported beside it; "traffic" is the measured HBM bytes per
+    # Roofline of the dominant kernel, the 6-DOF IC-GN (one k_icgn launch per frame plus
+    # the device-listed relaunch of enlarged windows, event-timed on the library's
+    # stream over the timed frames).  Bandwidth class: achieved = compulsory HBM bytes
+    # per launch / time, with 16 N_px (f, fx, fy, g as FP32) + 124 S (2 words in; 6 + 21
+    # + 2 words out, 4 B each) per launch.  "traffic" is the measured HBM bytes per
     # launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB
     # device copy reads on the same counters), kept in profiles/hbm_traffic.json.
     npx = 2048 * 2048
-    lags = (2 * srad + 3) ** 2
-    flops = 2.0 * lags * (ws + 1) ** 2 * S
-    algo_bytes = 8 * npx + 32 * S
-    t_s = search_ms / 1e3
-    achieved = flops / t_s / 1e12
-    traffic = None
+    traffic = {}
     try:
         with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh)["k_search"]["hbm_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+            traffic = json.load(fh)
+    except (OSError, ValueError):
         pass
-    roof = dict(kernel="k_search", bound="mfma", engine="VALU FP32 FMA (packed FP32 off)",
-                achieved=round(achieved, 3), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=traffic,
-                algorithmic_flops=flops, avg_ms=round(search_ms, 5),
-                hbm=dict(algorithmic_bytes=algo_bytes,
-                         achieved_GBs=round(algo_bytes / t_s / 1e9, 2), peak_GBs=HBM_PEAK_GBS))
+
+    def measured(key):
+        return traffic.get(key, {}).get("hbm_bytes_per_launch")
+
+    icgn_bytes = 16 * npx + 124 * S
+    t_i = icgn6_ms / 1e3
+    roof = dict(kernel="k_icgn (6-DOF IC-GN, ws 32)", bound="hbm",
+                achieved=round(icgn_bytes / t_i / 1e9, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(icgn_bytes / t_i / 1e9 / HBM_PEAK_GBS, 5),
+                traffic=measured("k_icgn_6dof"), algorithmic_bytes=icgn_bytes,
+                avg_ms=round(icgn6_ms, 5),
+                note="VALU-bound (FP32 Keys gathers and sums, ~110 VALU per pixel pass)")
+    # The integer search beside it: int8 matrix cores, algorithmic ops = the NCC
+    # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
+    ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
+    t_s = search_ms / 1e3
+    search_bytes = npx * 2 + 32 * S  # the raw 8-bit pair once + outputs
+    search_roof = dict(kernel="k_search_i8", bound="mfma", engine="int8 MFMA (32x32x32)",
+                       achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS, unit="TOP/s",
+                       frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
+                       traffic=measured("k_search_i8"), algorithmic_ops=ops,
+                       algorithmic_bytes=search_bytes, avg_ms=round(search_ms, 5))
 
     cpu = None
     acc = {}
@@ -186,8 +204,9 @@ def main():
                    "stage_ms": {k: round(v, 4) for k, v in stage.items()},
                    "stage_ms_from": "separate untimed frames with an event at every stage boundary",
                    "gridfit": counters,
-                   "icgn6_subsets_per_s": round(conv / (stage["icgn6"] / 1e3), 1)},
+                   "icgn6_subsets_per_s": round(conv / (icgn6_ms / 1e3), 1)},
         "roofline": roof,
+        "search_roofline": search_roof,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": acc or None,
     }

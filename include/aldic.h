@@ -150,8 +150,8 @@ typedef struct {
     double cg_rtol;    /* PCG relative residual */
     int no_subpb1_fit; /* 1: skip Subpb1's gridfit refit (bad nodes are only refilled) */
     int stage_times;   /* 1: an event at every stage boundary (all of aldic_frame_times);
-                        * 0: events only around the search and the whole frame (each
-                        * event costs the stream a few microseconds) */
+                        * 0: events only around the search, the 6-DOF IC-GN and the
+                        * whole frame (each event costs the stream a few microseconds) */
 } aldic_frame_opts;
 
 int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
@@ -163,8 +163,8 @@ int aldic_frame_run(aldic_ctx *ctx);
 int aldic_frame_get(aldic_ctx *ctx, double *U, double *F, double *U6, double *F6, int *iters6,
                     int *iters2, double *u0);
 /* Stage times of the last frame in ms: {prep, search, icgn6, post6, subpb2 total,
- * icgn2 total, strain+dual total, frame}; with stage_times 0 only search and frame
- * are measured (the others read 0). n <= 8. */
+ * icgn2 total, strain+dual total, frame}; with stage_times 0 only search, icgn6 and
+ * frame are measured (the others read 0). n <= 8. */
 int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
 /* Counters of the frame driver (no counterpart in the reference): {iteration kernels
  * each Subpb1 gridfit of the next frame runs before its convergence is checked,
