@@ -115,6 +115,7 @@ struct aldic_ctx {
     bool have_mask = false;
     bool images_from_raw = false;    // f, g normalised on the device from rawf, rawg
     aldic::DBuf stats;               // ROI partial sums, then {mean_f, 1/std_f, mean_g, 1/std_g}
+    aldic::DBuf boxp;                // window box-sum planes of the exact search
     aldic::DBuf scratch[8];          // per-call work buffers
     aldic::MeshDev mesh;
     bool have_mesh = false;
@@ -172,17 +173,23 @@ struct SearchArgs {
     double *x, *y, *u, *v, *ccmax, *qf;
     int *status;
     double *U0;             // optional interleaved output (2S), may be nullptr
-    // raw 8-bit pair and {mean_f, 1/std_f, mean_g, 1/std_g} of the normalisation that
-    // made f, g: when set (and the window fits) the exact int8 matrix-core search runs
+    // raw 8- or 16-bit pair and {mean_f, 1/std_f, mean_g, 1/std_g} of the normalisation
+    // that made f, g: when set (and the window fits) the exact int8 matrix-core search
+    // runs; planes is its work buffer for the window box sums (large lag maps)
     const uint8_t *f8 = nullptr, *g8 = nullptr;
+    const uint16_t *f16 = nullptr, *g16 = nullptr;
     const double *ms = nullptr;
+    DBuf *planes = nullptr;
 };
-// Runs the int8 matrix-core search when a.f8, a.g8, a.ms are set and the geometry fits
-// (2s+3 <= 32 lags per axis, ws + 2 s + 3 <= 64), else the FP32 search on a.f, a.g.
+// Runs an exact int8 matrix-core search when raw pixels are given: k_search_i8 for 8-bit
+// images with 2s+3 <= 32 lags per axis and ws + 2s + 3 <= 64, k_search_mt for 16-bit
+// images or larger lag maps (ws <= 64); else the FP32 search on a.f, a.g.
 hipError_t launch_search(const SearchArgs &a, hipStream_t st);
 size_t search_lds_bytes(int ws, int sx, int sy);
 bool search_i8_supported(int ws, int sx, int sy);
 hipError_t launch_search_i8(const SearchArgs &a, hipStream_t st);
+bool search_mt_supported(int ws, int sx, int sy, int bits);
+hipError_t launch_search_mt(const SearchArgs &a, hipStream_t st);
 
 hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const double *Y,
                           long n, double *Z, hipStream_t st);

@@ -87,13 +87,21 @@ hipError_t run_prep(aldic_ctx *c)
                        c->gx.as<float>(), c->gy.as<float>(), c->stream);
 }
 
-// 8-bit raw images behind f, g: the search runs on them exactly (int8 matrix cores)
+// 8- or 16-bit raw images behind f, g: the search runs on them exactly (int8 matrix cores)
 void search_raw8(aldic_ctx *c, SearchArgs &a)
 {
-    if (!c->images_from_raw || c->raw_dtype != ALDIC_U8) return;
-    a.f8 = c->rawf.as<uint8_t>();
-    a.g8 = c->rawg.as<uint8_t>();
+    if (!c->images_from_raw) return;
+    if (c->raw_dtype == ALDIC_U8) {
+        a.f8 = c->rawf.as<uint8_t>();
+        a.g8 = c->rawg.as<uint8_t>();
+    } else if (c->raw_dtype == ALDIC_U16) {
+        a.f16 = c->rawf.as<uint16_t>();
+        a.g16 = c->rawg.as<uint16_t>();
+    } else {
+        return;
+    }
     a.ms = c->stats.as<double>() + 4 * kStatParts;
+    a.planes = &c->boxp;
 }
 
 int stage_raw(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,

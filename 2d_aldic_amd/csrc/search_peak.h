@@ -3,24 +3,18 @@
 
 namespace aldic {
 
-// ncc[(c - 1) * nrx + (r - 1)]: row r = x-lag, column c = y-lag (both 1-based); bi is the
-// index of the first maximum in that (column-major) order and best its value.  Returns
-// the 1-based peak (xp = column, yp = row) and the fitted maximum mf: off the map's edge
-// the least-squares quadratic of the 3x3 neighbourhood moves the peak by (xo, yo) when
-// both stay within one pixel, rounded to 1e-3 as findpeak.m:57-58 does.
-template <class T>
-__device__ inline void findpeak_fit(const T *ncc, int nrx, int nry, int bi, double best,
-                                    double &xp, double &yp, double &mf)
+// The fit of findpeak.m:37-66 on a 3x3 neighbourhood u[(dc + 1) * 3 + (dr + 1)]
+// (dc = column offset = y-lag, dr = row offset = x-lag) around the 1-based peak (r, c).
+__device__ inline void findpeak_fit9(const double *u9, int r, int c, double best, double &xp,
+                                     double &yp, double &mf)
 {
-    const int r = bi % nrx + 1, c = bi / nrx + 1;
     xp = c;
     yp = r;
     mf = best;
-    if (c == 1 || c == nry || r == 1 || r == nrx) return;
     double s0 = 0, sxu = 0, syu = 0, sxy = 0, sxx = 0, syy = 0;
     for (int dc = -1; dc <= 1; ++dc)
         for (int dr = -1; dr <= 1; ++dr) {
-            const double u = ncc[(c - 1 + dc) * nrx + (r - 1 + dr)];
+            const double u = u9[(dc + 1) * 3 + (dr + 1)];
             s0 += u; sxu += dc * u; syu += dr * u; sxy += dc * dr * u;
             sxx += dc * dc * u; syy += dr * dr * u;
         }
@@ -37,6 +31,27 @@ __device__ inline void findpeak_fit(const T *ncc, int nrx, int nry, int bi, doub
         yp = r + yo;
         mf = A1 + A2 * xo + A3 * yo + A4 * xo * yo + A5 * xo * xo + A6 * yo * yo;
     }
+}
+
+// ncc[(c - 1) * nrx + (r - 1)]: row r = x-lag, column c = y-lag (both 1-based); bi is the
+// index of the first maximum in that (column-major) order and best its value.  Returns
+// the 1-based peak (xp = column, yp = row) and the fitted maximum mf: off the map's edge
+// the least-squares quadratic of the 3x3 neighbourhood moves the peak by (xo, yo) when
+// both stay within one pixel, rounded to 1e-3 as findpeak.m:57-58 does.
+template <class T>
+__device__ inline void findpeak_fit(const T *ncc, int nrx, int nry, int bi, double best,
+                                    double &xp, double &yp, double &mf)
+{
+    const int r = bi % nrx + 1, c = bi / nrx + 1;
+    xp = c;
+    yp = r;
+    mf = best;
+    if (c == 1 || c == nry || r == 1 || r == nrx) return;
+    double u9[9];
+    for (int dc = -1; dc <= 1; ++dc)
+        for (int dr = -1; dr <= 1; ++dr)
+            u9[(dc + 1) * 3 + (dr + 1)] = ncc[(c - 1 + dc) * nrx + (r - 1 + dr)];
+    findpeak_fit9(u9, r, c, best, xp, yp, mf);
 }
 
 }  // namespace aldic

@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
@@ -60,6 +61,9 @@ def main():
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the untimed per-stage breakdown frames")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", type=int, default=2, choices=(2, 3),
+                    help="2: 2048^2 uint8 ws32/step16 s8 (default); 3: 4096^2 12-bit "
+                         "ws32/step8 s24, ~247k subsets (generation takes ~1.5 min)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -71,10 +75,19 @@ def main():
     pkg = importlib.import_module("2d_aldic_amd")
     sp = pkg.speckle
     t = time.perf_counter()
-    pair = sp.config2()
+    if args.workload == 3:
+        if not args.no_cpu:
+            log("[bench] workload 3: the FP64 CPU frame would take hours; CPU baseline skipped")
+            args.no_cpu = True
+        pair = sp.config3()
+        ws, step, srad, side = 32, 8, 24, 4096
+        wname = "config3: 4096x4096 12-bit speckle in uint16, 0.5% tension + shear band, ws32/step8"
+    else:
+        pair = sp.config2()
+        ws, step, srad, side = 32, 16, 8, 2048
+        wname = "config2: 2048x2048 uint8 speckle, sinusoidal field, ws32/step16"
     log(f"[bench] workload generated in {time.perf_counter() - t:.1f} s")
-    ws, step, srad = 32, 16, 8
-    grid = [1, 2048, 1, 2048]
+    grid = [1, side, 1, side]
     mu = 1e-3
     beta = 1e-2 * step * step * mu
     n_admm = 3
@@ -126,7 +139,7 @@ def main():
     # + 2 words out, 4 B each) per launch.  "traffic" is the measured HBM bytes per
     # launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB
     # device copy reads on the same counters), kept in profiles/hbm_traffic.json.
-    npx = 2048 * 2048
+    npx = side * side
     traffic = {}
     try:
         with open(TRAFFIC_FILE) as fh:
@@ -139,6 +152,8 @@ def main():
 
     icgn_bytes = 16 * npx + 124 * S
     t_i = icgn6_ms / 1e3
+    if args.workload != 2:
+        traffic = {}  # the measured table is for the default workload
     roof = dict(kernel="k_icgn (6-DOF IC-GN, ws 32)", bound="hbm",
                 achieved=round(icgn_bytes / t_i / 1e9, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(icgn_bytes / t_i / 1e9 / HBM_PEAK_GBS, 5),
@@ -149,12 +164,21 @@ def main():
     # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
     t_s = search_ms / 1e3
-    search_bytes = npx * 2 + 32 * S  # the raw 8-bit pair once + outputs
-    search_roof = dict(kernel="k_search_i8", bound="mfma", engine="int8 MFMA (32x32x32)",
-                       achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS, unit="TOP/s",
-                       frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
-                       traffic=measured("k_search_i8"), algorithmic_ops=ops,
-                       algorithmic_bytes=search_bytes, avg_ms=round(search_ms, 5))
+    if pair.f.dtype == np.uint8 and srad <= 14:
+        search_bytes = npx * 2 + 32 * S  # the raw 8-bit pair once + outputs
+        search_roof = dict(kernel="k_search_i8", bound="mfma", engine="int8 MFMA (32x32x32)",
+                           achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS,
+                           unit="TOP/s", frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
+                           traffic=measured("k_search_i8"), algorithmic_ops=ops,
+                           algorithmic_bytes=search_bytes, avg_ms=round(search_ms, 5))
+    else:
+        search_bytes = npx * 8 + 32 * S  # the FP32 pair once + outputs
+        search_roof = dict(kernel="k_search", bound="mfma",
+                           engine="VALU FP32 FMA (packed FP32 off)",
+                           achieved=round(ops / t_s / 1e12, 3), peak=FP32_PEAK_TFLOPS,
+                           unit="TFLOP/s", frac=round(ops / t_s / 1e12 / FP32_PEAK_TFLOPS, 5),
+                           traffic=None, algorithmic_ops=ops, algorithmic_bytes=search_bytes,
+                           avg_ms=round(search_ms, 5))
 
     cpu = None
     acc = {}
@@ -198,8 +222,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "config2: 2048x2048 uint8 speckle, sinusoidal field, ws32/step16 "
-                               f"({S} subsets), search 8, 3 ADMM iterations, FEM Q4",
+        "config": {"workload": f"{wname} ({S} subsets), search {srad}, 3 ADMM iterations, "
+                               "FEM Q4",
                    "subsets": S, "converged": conv, "frame_ms": round(ms_per_step, 4),
                    "stage_ms": {k: round(v, 4) for k, v in stage.items()},
                    "stage_ms_from": "separate untimed frames with an event at every stage boundary",

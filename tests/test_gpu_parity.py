@@ -85,12 +85,14 @@ def test_search_fp32_path_matches_oracle_config1(gpu_ctx, cfg1_norm, orc):
 
 
 @pytest.mark.parametrize("case", ["ws20_s8_peak_at_edge", "ws16_s4", "ws32_s3x7",
-                                  "ws40_s10", "flat_templates", "ws20_s15_fp32_fallback"])
+                                  "ws40_s10", "flat_templates", "ws20_s15_two_tiles",
+                                  "ws16_s20_float_images"])
 def test_search_8bit_cases(gpu_ctx, speckle, orc, case):
     """Geometries of the int8 search (one or two 32-column MFMA steps per template row,
     unequal radii, 23 x 23 lags), peaks one lag from the crop edge (the 3x3 fit reads
     the edge lags, whose footprint leaves the window), constant templates (status -1),
-    and a radius beyond 32 lags that falls back to the FP32 search."""
+    a 33 x 33 lag map (two lag tiles per axis, window box sums from the image planes),
+    and float images with a wide radius (the FP32 search)."""
     _flush(f"search case {case}")
     ws, sx, sy, u0, v0, step = 20, 8, 8, 7.6, -7.7, 12
     if case == "ws16_s4":
@@ -99,21 +101,26 @@ def test_search_8bit_cases(gpu_ctx, speckle, orc, case):
         ws, sx, sy, u0, v0 = 32, 3, 7, 1.7, -6.2
     elif case == "ws40_s10":
         ws, sx, sy, u0, v0 = 40, 10, 10, -9.3, 4.4
-    elif case == "ws20_s15_fp32_fallback":
+    elif case == "ws20_s15_two_tiles":
         ws, sx, sy, u0, v0 = 20, 15, 15, 11.2, -3.6
+    elif case == "ws16_s20_float_images":
+        ws, sx, sy, u0, v0 = 16, 20, 20, -13.4, 8.7
     p = speckle.small_pair(n=208, u0=u0, v0=v0, ux=0.002, vy=-0.001, seed=31)
     fr, gr = p.f.copy(), p.g.copy()
     if case == "flat_templates":
         fr[60:110, 40:100] = 77
-    gpu_ctx.set_images(fr, gr)
-    grid = [1, 208, 1, 208]
-    r = gpu_ctx.integer_search(grid, ws, step, [sx, sy])
     f, _ = orc.normalize(fr.astype(float))
     g, _ = orc.normalize(gr.astype(float))
+    if case == "ws16_s20_float_images":
+        gpu_ctx.set_normalized(f, g)
+    else:
+        gpu_ctx.set_images(fr, gr)
+    grid = [1, 208, 1, 208]
+    r = gpu_ctx.integer_search(grid, ws, step, [sx, sy])
     o = orc.search(f, g, grid, ws, step, [sx, sy])
     if case == "flat_templates":
         assert np.sum(o["status"] == -1) >= 4
-    if case == "ws20_s15_fp32_fallback":
+    if case == "ws16_s20_float_images":
         assert np.array_equal(r["status"], o["status"])
         assert np.max(np.abs(r["u"] - o["u"])) <= 1e-3 + 1e-9
         assert np.max(np.abs(r["v"] - o["v"])) <= 1e-3 + 1e-9
@@ -122,6 +129,39 @@ def test_search_8bit_cases(gpu_ctx, speckle, orc, case):
     if case == "ws20_s8_peak_at_edge":
         # integer peaks at lag +-s with fits that used the edge lags
         assert np.mean(np.abs(np.round(o["u"])) == sx) > 0.5
+
+
+@pytest.mark.parametrize("case", ["u12_ws20_s8", "u12_ws32_s24", "u16_full_range_ws16_s12",
+                                  "u12_flat_and_image_edges"])
+def test_search_16bit_cases(gpu_ctx, speckle, orc, case):
+    """16-bit images: two int8 digit planes per image, four int8 GEMMs per lag tile, the
+    window box sums from int64 image planes.  12-bit speckle (the 4096^2 workload's
+    format) at radius 8 and 24 (51 x 51 lags, 2 x 2 tiles), full 16-bit range, and
+    constant templates with subsets against the image border."""
+    _flush(f"search 16-bit case {case}")
+    n, ws, sx, sy, step, u0, v0, bits = 208, 20, 8, 8, 12, 5.3, -6.6, 12
+    if case == "u12_ws32_s24":
+        n, ws, sx, sy, step, u0, v0 = 240, 32, 24, 24, 16, 17.4, -9.2
+    elif case == "u16_full_range_ws16_s12":
+        ws, sx, sy, u0, v0, bits = 16, 12, 12, -10.6, 3.1, 16
+    elif case == "u12_flat_and_image_edges":
+        ws, sx, sy, step, u0, v0 = 24, 6, 6, 8, 2.2, 1.4
+    p = speckle.small_pair(n=n, u0=u0, v0=v0, ux=0.002, vy=-0.001, bits=bits, seed=47)
+    fr, gr = p.f.copy(), p.g.copy()
+    assert fr.dtype == np.uint16
+    if case == "u12_flat_and_image_edges":
+        fr[70:120, 50:110] = 1234
+    gpu_ctx.set_images(fr, gr)
+    grid = [1, n, 1, n]
+    r = gpu_ctx.integer_search(grid, ws, step, [sx, sy])
+    f, _ = orc.normalize(fr.astype(float))
+    g, _ = orc.normalize(gr.astype(float))
+    o = orc.search(f, g, grid, ws, step, [sx, sy])
+    if case == "u12_flat_and_image_edges":
+        assert np.sum(o["status"] == -1) >= 4
+        assert r["x1"] - ws // 2 <= sx + 2 and r["y1"] - ws // 2 <= sy + 2
+    _search_exact(r, o)
+    assert np.mean(r["status"] == 0) > 0.5
 
 
 def _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.97):
