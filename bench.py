@@ -164,13 +164,19 @@ def main():
     # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
     t_s = search_ms / 1e3
-    if pair.f.dtype == np.uint8 and srad <= 14:
-        search_bytes = npx * 2 + 32 * S  # the raw 8-bit pair once + outputs
-        search_roof = dict(kernel="k_search_i8", bound="mfma", engine="int8 MFMA (32x32x32)",
+    if pair.f.dtype in (np.uint8, np.uint16):
+        small = pair.f.dtype == np.uint8 and srad <= 14
+        bpp = pair.f.dtype.itemsize
+        # raw pair once + outputs (+ the window box-sum planes for the multi-tile kernel)
+        search_bytes = npx * 2 * bpp + 32 * S + (0 if small else npx * 36)
+        search_roof = dict(kernel="k_search_i8" if small else "k_search_mt", bound="mfma",
+                           engine="int8 MFMA (32x32x32)" + ("" if bpp == 1 else
+                                                            ", two digit planes (4 GEMMs)"),
                            achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS,
                            unit="TOP/s", frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
-                           traffic=measured("k_search_i8"), algorithmic_ops=ops,
-                           algorithmic_bytes=search_bytes, avg_ms=round(search_ms, 5))
+                           traffic=measured("k_search_i8") if small else None,
+                           algorithmic_ops=ops, algorithmic_bytes=search_bytes,
+                           avg_ms=round(search_ms, 5))
     else:
         search_bytes = npx * 8 + 32 * S  # the FP32 pair once + outputs
         search_roof = dict(kernel="k_search", bound="mfma",
