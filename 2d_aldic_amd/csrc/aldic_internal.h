@@ -190,6 +190,10 @@ bool search_i8_supported(int ws, int sx, int sy);
 hipError_t launch_search_i8(const SearchArgs &a, hipStream_t st);
 bool search_mt_supported(int ws, int sx, int sy, int bits);
 hipError_t launch_search_mt(const SearchArgs &a, hipStream_t st);
+// q-factors (PCE, PPE) of every subset into a.qf (2S, normalised per column over the
+// subsets), from the full NCC map; LDS need per workgroup (limit 150 KiB).
+size_t qfactor_lds_bytes(int ws, int sx, int sy);
+hipError_t launch_qfactors(const SearchArgs &a, hipStream_t st);
 
 hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const double *Y,
                           long n, double *Z, hipStream_t st);

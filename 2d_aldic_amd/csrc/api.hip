@@ -396,7 +396,10 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
     a.status = ctx->scratch[6].as<int>();
     a.U0 = nullptr;
     search_raw8(ctx, a);
+    if (qf && qfactor_lds_bytes(ws, s[0], s[1]) > 150 * 1024)
+        return fail(ctx, ALDIC_ERR_ARG, "integer_search: full NCC map too large for q-factors");
     CK(launch_search(a, ctx->stream));
+    if (qf) CK(launch_qfactors(a, ctx->stream));
     CK(down(x, a.x, S, ctx->stream));
     CK(down(y, a.y, S, ctx->stream));
     CK(down(u, a.u, S, ctx->stream));

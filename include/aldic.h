@@ -74,8 +74,12 @@ int aldic_search_grid(int nx, int ny, const int *grid, int ws, int step, const i
  * with method 1) over the subsets aldic_search_grid() gives.  Outputs are Nx*Ny
  * arrays, entry l*Nx + k = subset (k, l) (x fastest): centres x, y; displacement u, v
  * (findpeak.m subpixel fit, 1e-3 px rounding); cc.max; status (0 ok, -1 constant
- * template -> NaN).  qf (Nx*Ny x 2: PCE, PPE before the per-column normalisation of
- * :174-177) may be NULL to skip the full-map quality factors. */
+ * template -> NaN).  8- and 16-bit images staged with aldic_set_images are searched
+ * exactly (int8 matrix cores, integer sums, FP64 NCC); float images in FP32.  qf
+ * (Nx*Ny x 2: cc.qfactors, PCE and PPE of the full normxcorr2 map after the per-column
+ * normalisation of :174-177; a constant template gives Inf before it) may be NULL to
+ * skip the full-map quality factors; the full map must fit 150 KiB of LDS
+ * (4 (ws+1)^2 + 4 D^2 + 8 (ws + D)^2 bytes, D = ws + 1 + 2 s). */
 int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, const int *s,
                          double *x, double *y, double *u, double *v, double *ccmax, double *qf,
                          int *status);

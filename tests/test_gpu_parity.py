@@ -164,6 +164,57 @@ def test_search_16bit_cases(gpu_ctx, speckle, orc, case):
     assert np.mean(r["status"] == 0) > 0.5
 
 
+def _qf_norm(q):
+    # funIntegerSearch.m:174-177: (q - min q) / max(q - min q) per column
+    q = np.asarray(q, float).copy()
+    for k in range(2):
+        d = q[:, k] - np.min(q[:, k])
+        q[:, k] = d / np.max(d)
+    return q
+
+
+@pytest.mark.parametrize("case", ["u8_config1", "u12_ws16_s6", "float_images", "constant_template"])
+def test_search_qfactors_match_oracle(gpu_ctx, cfg1, speckle, orc, case):
+    """PCE and PPE on the full normxcorr2 map, then the per-column normalisation over the
+    subsets; a constant template gives [Inf, Inf], which turns the finite entries of the
+    normalised columns into 0 and its own into NaN (Inf / Inf), as in the reference."""
+    _flush(f"q-factors case {case}")
+    if case in ("u8_config1", "float_images", "constant_template"):
+        fr, gr = cfg1.f.copy(), cfg1.g.copy()
+        grid, ws, step, s = [1, 256, 1, 256], 20, 10, [8, 8]
+    else:
+        p = speckle.small_pair(n=160, u0=3.2, v0=-2.1, bits=12, seed=5)
+        fr, gr = p.f.copy(), p.g.copy()
+        grid, ws, step, s = [1, 160, 1, 160], 16, 12, [6, 6]
+    if case == "constant_template":
+        fr[40:80, 40:80] = 90
+    f, _ = orc.normalize(fr.astype(float))
+    g, _ = orc.normalize(gr.astype(float))
+    if case == "float_images":
+        gpu_ctx.set_normalized(f, g)
+    else:
+        gpu_ctx.set_images(fr, gr)
+    r = gpu_ctx.integer_search(grid, ws, step, s, qfactors=True)
+    o = orc.search(f, g, grid, ws, step, s)
+    raw = np.stack([o["pce"].ravel(), o["ppe"].ravel()], 1)
+    qo = _qf_norm(raw)
+    q = r["qf"]
+    assert q.shape == qo.shape
+    assert np.array_equal(np.isnan(q), np.isnan(qo))
+    ok = ~np.isnan(qo)
+    if case == "constant_template":
+        assert np.any(np.isinf(raw))
+        assert np.all(q[ok] == 0.0)
+        return
+    # PCE: FP64 sums in another order; PPE: the histogram counts agree except where a
+    # map value sits within rounding of a bin edge
+    tol_pce = 1e-9 if case != "float_images" else 1e-5
+    assert np.all(ok)
+    assert np.max(np.abs(q[:, 0] - qo[:, 0])) < tol_pce
+    assert np.mean(np.abs(q[:, 1] - qo[:, 1]) < 1e-9) > 0.95
+    assert np.max(np.abs(q[:, 1] - qo[:, 1])) < 0.05
+
+
 def _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.97):
     same = it == ito
     assert same.mean() >= frac, f"status agreement {same.mean():.3f}"
