@@ -315,6 +315,67 @@ def subpb1_smooth(U, coords):
     return U
 
 
+def gauss_kernel(size=3, sigma=0.5):
+    """fspecial('gaussian', size, sigma): exp(-(x^2+y^2)/(2 sigma^2)) over the centred
+    size x size grid, entries below eps * max set to 0, normalised to sum 1."""
+    r = (size - 1) / 2.0
+    x = np.arange(size) - r
+    h = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2.0 * sigma * sigma))
+    h[h < np.finfo(float).eps * h.max()] = 0.0
+    return h / h.sum()
+
+
+def nanconv_edge(Z, k):
+    """nanconv(Z, k, 'edge', 'nanout') (func/nanconv.m:1-...): NaNs count as 0 in the
+    sum, each output is divided by the kernel weight that fell on valid in-image
+    values, and NaN inputs stay NaN (conv2 'same')."""
+    from scipy.signal import convolve2d
+    Z = np.asarray(Z, np.float64)
+    n = np.isnan(Z)
+    a = np.where(n, 0.0, Z)
+    flat = convolve2d((~n).astype(np.float64), k, mode="same")
+    c = convolve2d(a, k, mode="same") / flat
+    c[n] = np.nan
+    return c
+
+
+def smooth_nodal(V, coords, ncomp, passes=2, size=3, sigma=0.5):
+    """funSmoothDisp.m / funSmoothStrain.m (default filter: sigma 0.5, size
+    2 ceil(2 sigma) + 1 = 3): each pass fits every component by gridfit ('springs') over
+    the node grid min:step:max (the unique node coordinates of a uniform mesh), blurs the
+    grid with nanconv(..., 'edge', 'nanout') and reads it back at the nodes; two passes
+    (SmoothTimes 1, 2)."""
+    coords = np.asarray(coords, np.float64).reshape(-1, 2)
+    V = np.asarray(V, np.float64).copy()
+    xn, yn = np.unique(coords[:, 0]), np.unique(coords[:, 1])
+    ix = np.searchsorted(xn, coords[:, 0])
+    iy = np.searchsorted(yn, coords[:, 1])
+    k = gauss_kernel(size, sigma)
+    for _ in range(passes):
+        for c in range(ncomp):
+            z = V[c::ncomp]
+            ok = ~np.isnan(z)
+            Z = gridfit(coords[ok, 0], coords[ok, 1], z[ok], xn, yn)   # (ny, nx)
+            V[c::ncomp] = nanconv_edge(Z, k)[iy, ix]
+    return V
+
+
+def beta_choice(beta_list, err1, err2):
+    """main_ALDIC.m:285-301 (FE branch): z-scores of the two error norms (std with N-1),
+    the first minimum of their sum, then the vertex of the parabola through the three
+    (log10 beta, ErrSum) points around it; a minimum at either end of the list (where
+    the reference's fit raises) keeps that list entry."""
+    e1 = np.asarray(err1, np.float64)
+    e2 = np.asarray(err2, np.float64)
+    es = (e1 - e1.mean()) / e1.std(ddof=1) + (e2 - e2.mean()) / e2.std(ddof=1)
+    i = int(np.argmin(es))
+    if i == 0 or i == len(beta_list) - 1:
+        return float(beta_list[i]), i
+    x = np.log10(np.asarray(beta_list[i - 1:i + 2], np.float64))
+    p = np.polyfit(x, es[i - 1:i + 2], 2)
+    return float(10.0 ** (-p[1] / 2.0 / p[0])), i
+
+
 # ---- whole frame (main_ALDIC.m:87-451, FEM branch, smoothing off) -------------------
 
 def badpoint(iters, mean_rule):
@@ -364,13 +425,20 @@ def axis_fill(V, nc, bad, Nx, Ny):
 
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-          cg_rtol=1e-10, subpb1_fit=True):
+          cg_rtol=1e-10, subpb1_fit=True, smooth=False, beta_sweep=False, admm_tol=0.0):
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
     refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1: 2-DOF IC-GN, bad
     points, gridfit of every node; Subpb2, strain, duals).  The LocalICGN refill of bad
     nodes (the reference's scatteredInterpolant 'natural', LocalICGN.m:144-161) is
     axis_fill; parity unpinned.  subpb1_fit=False replaces Subpb1's gridfit with the
-    same axis_fill of its bad nodes.  Returns the stages' outputs and times."""
+    same axis_fill of its bad nodes.  smooth=True adds main_ALDIC.m's smoothing:
+    funSmoothStrain of F1 after LocalICGN (:233) and of F2 after every global step
+    (:320, :407), funSmoothDisp of U1 after Subpb1 and of U2 after Subpb2 in the ADMM
+    steps (:381, :400).  beta_sweep=True picks beta as main_ALDIC.m:237,274-301 does
+    (FE branch) from the step-1 global step at the six candidates.  admm_tol > 0 stops
+    the ADMM steps once the RMS change of U2 or of Subpb1's U (before smoothing) between
+    steps falls below it (main_ALDIC.m:409-447).  Returns the stages' outputs and
+    times, the beta used and the number of ADMM steps run."""
     import time
     import scipy.sparse as sp
     T = {}
@@ -408,16 +476,35 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     udual = np.zeros(4 * S)
     vdual = np.zeros(2 * S)
     T["subpb2"] = T["icgn2"] = 0.0
+    if smooth:
+        F1 = smooth_nodal(F1, xy, 4)
+    if beta_sweep:
+        blist = np.array([1e-3, np.sqrt(1e-5), 1e-2, np.sqrt(1e-3), 1e-1, np.sqrt(1e-1)]) \
+            * step * step * mu
+        e1, e2 = [], []
+        for b_ in blist:
+            U2k, _ = subpb2(xy, el, b_, mu, U1, F1, udual, vdual, rtol=cg_rtol)
+            F2k = nodal_strain(xy, el, U2k)
+            e1.append(np.linalg.norm(U1 - U2k))
+            e2.append(np.linalg.norm(F1 - F2k))
+        beta, _ = beta_choice(blist, e1, e2)
 
-    def global_step(U1, F1, udual, vdual):
+    def global_step(U1, F1, udual, vdual, smooth_u):
         t = time.perf_counter()
         U2, _ = subpb2(xy, el, beta, mu, U1, F1, udual, vdual, rtol=cg_rtol)
+        if smooth_u:
+            U2 = smooth_nodal(U2, xy, 2)
         F2 = nodal_strain(xy, el, U2)
+        if smooth:
+            F2 = smooth_nodal(F2, xy, 4)
         T["subpb2"] += time.perf_counter() - t
         return U2, F2, F2 - F1, U2 - U1
 
-    U2, F2, udual, vdual = global_step(U1, F1, udual, vdual)
+    U2, F2, udual, vdual = global_step(U1, F1, udual, vdual, False)
     it2 = None
+    u1_prev, u2_prev = U1.copy(), U2.copy()
+    upd = upd2 = np.nan
+    steps = 0
     for _ in range(n_admm):
         t = time.perf_counter()
         U1, it2 = icgn2(f, g, gx, gy, xy, ws, H, mu, udual, vdual, U2, F2, tol)
@@ -428,8 +515,17 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
         else:
             U1 = axis_fill(U1, 2, bad, Nx, Ny)
         F1 = F2
+        u1_saved = U1.copy()
+        if smooth:
+            U1 = smooth_nodal(U1, xy, 2)
         T["icgn2"] += time.perf_counter() - t
-        U2, F2, udual, vdual = global_step(U1, F1, udual, vdual)
+        U2, F2, udual, vdual = global_step(U1, F1, udual, vdual, smooth)
+        steps += 1
+        upd = np.linalg.norm(u2_prev - U2) / np.sqrt(U2.size)
+        upd2 = np.linalg.norm(u1_prev - u1_saved) / np.sqrt(U2.size)
+        u1_prev, u2_prev = u1_saved, U2.copy()
+        if admm_tol > 0 and (upd < admm_tol or upd2 < admm_tol):
+            break
     T["frame"] = time.perf_counter() - t0
     return dict(U=U2, F=F2, U6=U6, F6=F6, iters6=it6, iters2=it2, U0=U0, xy=xy, Nx=Nx, Ny=Ny,
-                times=T)
+                times=T, beta=beta, admm_steps=steps, update=(upd, upd2))
