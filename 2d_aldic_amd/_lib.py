@@ -23,7 +23,7 @@ EXPORTS = (
     "aldic_integer_search", "aldic_icgn6", "aldic_icgn2", "aldic_set_mesh", "aldic_subpb2",
     "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
-    "aldic_frame_counters",
+    "aldic_frame_counters", "aldic_frame_scalars",
 )
 
 
@@ -31,7 +31,8 @@ class FrameOpts(C.Structure):
     _fields_ = [("ws", C.c_int), ("step", C.c_int), ("s", C.c_int * 2), ("grid", C.c_int * 4),
                 ("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
                 ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
-                ("no_subpb1_fit", C.c_int), ("stage_times", C.c_int)]
+                ("no_subpb1_fit", C.c_int), ("stage_times", C.c_int), ("smooth", C.c_int),
+                ("beta_sweep", C.c_int), ("admm_tol", C.c_double)]
 
 
 _lib = None
@@ -71,6 +72,7 @@ def load():
     L.aldic_frame_times.argtypes = [P, vp, ip]
     L.aldic_gridfit.argtypes = [P, ip, ip, vp, vp, vp, dp, ip, vp, vp]
     L.aldic_frame_counters.argtypes = [P, vp, ip]
+    L.aldic_frame_scalars.argtypes = [P, vp, ip]
     _lib = L
     return L
 
@@ -253,7 +255,8 @@ class Context:
         self.nx, self.ny = nx, ny
 
     def frame_setup(self, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-                    cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True, stage_times=True):
+                    cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True, stage_times=True,
+                    smooth=False, beta_sweep=False, admm_tol=0.0):
         o = FrameOpts()
         o.ws, o.step = int(ws), int(step)
         o.s[0], o.s[1] = (int(s), int(s)) if np.isscalar(s) else (int(s[0]), int(s[1]))
@@ -267,6 +270,9 @@ class Context:
         o.cg_rtol = cg_rtol
         o.no_subpb1_fit = 0 if subpb1_fit else 1
         o.stage_times = 1 if stage_times else 0
+        o.smooth = 1 if smooth else 0
+        o.beta_sweep = 1 if beta_sweep else 0
+        o.admm_tol = float(admm_tol)
         n = C.c_int(0)
         self._ck(self.L.aldic_frame_setup(self.h, C.byref(o), C.byref(n)), "aldic_frame_setup")
         self.frame_nodes = n.value
@@ -290,6 +296,13 @@ class Context:
         self._ck(self.L.aldic_frame_times(self.h, _p(ms), 8), "aldic_frame_times")
         keys = ("prep", "search", "icgn6", "post6", "subpb2", "icgn2", "strain_dual", "frame")
         return dict(zip(keys, ms.tolist()))
+
+    def frame_scalars(self):
+        """{beta used, ADMM steps run, RMS change of U2 and of Subpb1's U in the last
+        step}."""
+        v = np.zeros(4)
+        self._ck(self.L.aldic_frame_scalars(self.h, _p(v), 4), "aldic_frame_scalars")
+        return dict(beta=v[0], admm_steps=int(v[1]), update=(v[2], v[3]))
 
     def frame_counters(self):
         """{gridfit iteration budget of the next frame, frames re-run with polled

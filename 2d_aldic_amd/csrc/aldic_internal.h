@@ -72,25 +72,35 @@ struct GridfitDev {
 };
 hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const double *ynodes,
                          int gy, hipStream_t st);
-// U: two interleaved values per node (node = iy * gx + ix), the initial guess on entry
-// and the fit of both on return (one CG on the pair); bad[node] != 0: the node carries
-// no data.  stats_dev (nullable, device): {converged (1) or stopped on a breakdown (1)
+// U: np (1 or 2) interleaved value pairs per node (node = iy * gx + ix), the initial
+// guess on entry and the fit of all on return (one CG on them together); bad[node] != 0:
+// the node carries no data (bad may be null: every node does).  stats_dev (nullable, device): {converged (1) or stopped on a breakdown (1)
 // or not (0), iterations, relative residual of the pair}.  work[0..1] are used.
 // poll > 0: a host check for convergence every poll iterations, up to maxit; poll <= 0:
 // exactly maxit iteration kernels (those after convergence return at once) and no host
 // wait, so the caller reads stats_dev to learn whether maxit sufficed.
-hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
+hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, double rtol, int maxit,
                          DBuf *work, double *stats_dev, hipStream_t st, int poll);
+// nanconv(., fspecial('gaussian', 3, sigma), 'edge') of nc values per node on the gx x gy
+// node grid (out-of-place)
+hipError_t launch_blur3(const double *in, double *out, int gx, int gy, int nc, double sigma,
+                        hipStream_t st);
 
 struct FrameState {
     bool ready = false;
     aldic_frame_opts o{};
     int x1 = 0, y1 = 0, Nx = 0, Ny = 0, S = 0;
     DBuf coords, U0, U6, F6, H21, it6, zncc, Ucur, Fcur, U1, F1, U2, F2, udual, vdual, it2,
-        bad, tmp, part, cc, rl, F2b, gfstat;
+        bad, tmp, part, cc, rl, F2b, gfstat,
+        Us, Fs, U1s, U1old, U2old, errs;  // smoothing temporaries, stop-rule copies, norms
     GridfitDev gf;  // Subpb1's gridfit over the node grid
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
-    double *gfh = nullptr;  // pinned: {converged, iterations, residual} per ADMM gridfit
+    double *gfh = nullptr;  // pinned: {converged, iterations, residual} per frame gridfit
+    double *hostd = nullptr;  // pinned: norms read back by the beta sweep and stop rule
+    int gf_cap = 0;         // gridfit records a frame can hold
+    int gf_count = 0;       // gridfits the last frame pass ran
+    double beta_used = 0.0, upd = 0.0, upd2 = 0.0;  // beta, last RMS updates of U2 and U1
+    int admm_steps = 0;     // ADMM steps the last frame ran
     int gf_budget = 16;     // iteration kernels per gridfit before a frame is re-run
     int gf_reruns = 0;      // frames re-run with polled gridfits
     int gf_most = 0;        // most gridfit iterations in the last frame
@@ -216,6 +226,8 @@ hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F
 
 // ADMM helpers
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);
+// out[0] = sum (a - b)^2 (device scalar)
+hipError_t launch_sqdiff(const double *a, const double *b, long n, double *out, hipStream_t st);
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
                           double *part, hipStream_t st);

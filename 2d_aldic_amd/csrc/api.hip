@@ -58,6 +58,31 @@ hipError_t down(T *h, const void *d, size_t n, hipStream_t st)
     return hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, st);
 }
 
+// main_ALDIC.m:285-301 (FE branch): z-scores of both error norms (std with N - 1), the
+// first minimum of their sum, and the vertex of the parabola through the three
+// (log10 beta, sum) points around it; a minimum at an end of the list keeps that entry
+// (where the reference's fit raises and its catch takes the list value).
+double beta_choice(const double bl[6], const double e1[6], const double e2[6])
+{
+    double m1 = 0, m2 = 0;
+    for (int k = 0; k < 6; ++k) { m1 += e1[k] / 6; m2 += e2[k] / 6; }
+    double v1 = 0, v2 = 0;
+    for (int k = 0; k < 6; ++k) {
+        v1 += (e1[k] - m1) * (e1[k] - m1) / 5;
+        v2 += (e2[k] - m2) * (e2[k] - m2) / 5;
+    }
+    double es[6];
+    for (int k = 0; k < 6; ++k) es[k] = (e1[k] - m1) / std::sqrt(v1) + (e2[k] - m2) / std::sqrt(v2);
+    int i = 0;
+    for (int k = 1; k < 6; ++k)
+        if (es[k] < es[i]) i = k;
+    if (i == 0 || i == 5) return bl[i];
+    const double x0 = std::log10(bl[i - 1]), x1 = std::log10(bl[i]), x2 = std::log10(bl[i + 1]);
+    const double d01 = (es[i] - es[i - 1]) / (x1 - x0), d12 = (es[i + 1] - es[i]) / (x2 - x1);
+    const double p1 = (d12 - d01) / (x2 - x0), p2 = d01 - p1 * (x0 + x1);
+    return std::pow(10.0, -p2 / (2.0 * p1));
+}
+
 void clip_roi(const int *roi, int nx, int ny, int out[4])
 {
     out[0] = roi ? std::max(1, roi[0]) : 1;
@@ -248,6 +273,8 @@ void aldic_destroy(aldic_ctx *ctx)
     }
     F.gfstat.release();
     if (F.gfh) (void)hipHostFree(F.gfh);
+    if (F.hostd) (void)hipHostFree(F.hostd);
+    for (DBuf *bb : {&F.Us, &F.Fs, &F.U1s, &F.U1old, &F.U2old, &F.errs}) bb->release();
     for (auto &e : F.ev) (void)hipEventDestroy(e);
     F.ev.clear();
     for (auto &e : ctx->ev)
@@ -693,7 +720,7 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
     CK(up(sc[1], bad.data(), n, ctx->stream));
     CK(sc[2].ensure(4 * sizeof(double)));
     static thread_local DBuf work[8];
-    CK(gridfit_solve(ctx->gf, sc[1].as<int>(), sc[0].as<double>(), rtol, maxit, work,
+    CK(gridfit_solve(ctx->gf, sc[1].as<int>(), sc[0].as<double>(), 1, rtol, maxit, work,
                      sc[2].as<double>(), ctx->stream, 8));
     CK(down(u.data(), sc[0].p, 2 * n, ctx->stream));
     CK(down(stats, sc[2].as<double>() + 1, 2, ctx->stream));
@@ -789,19 +816,39 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.part.ensure(3 * 256 * sizeof(double)));
     CK(F.cc.ensure(8 * sizeof(double) * S));
     CK(F.rl.ensure(sizeof(int) * (S + 1)));
-    CK(F.gfstat.ensure(3 * sizeof(double) * (o->n_admm + 1)));
+    // gridfits per frame: Subpb1's refit per ADMM step, and with smoothing two per field
+    // smoothed (F1; F2 of every step; U1, U2 of every ADMM step)
+    F.gf_cap = 4 + 8 * (o->n_admm + 1);
+    CK(F.gfstat.ensure(3 * sizeof(double) * F.gf_cap));
     F.gf_budget = 16;
     F.gf_reruns = 0;
     F.gf_most = 0;
+    F.gf_count = 0;
     if (F.gfh) CK(hipHostFree(F.gfh));
     F.gfh = nullptr;
-    CK(hipHostMalloc((void **)&F.gfh, 3 * sizeof(double) * (o->n_admm + 1), hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&F.gfh, 3 * sizeof(double) * F.gf_cap, hipHostMallocDefault));
+    if (!F.hostd) CK(hipHostMalloc((void **)&F.hostd, 16 * sizeof(double), hipHostMallocDefault));
+    CK(F.errs.ensure(16 * sizeof(double)));
+    if (o->smooth) {
+        CK(F.Us.ensure(2 * sd));
+        CK(F.Fs.ensure(4 * sd));
+    }
+    if (o->beta_sweep) CK(F.Fs.ensure(4 * sd));
+    if (o->admm_tol > 0) {
+        CK(F.U1s.ensure(2 * sd));
+        CK(F.U1old.ensure(2 * sd));
+        CK(F.U2old.ensure(2 * sd));
+    }
+    F.beta_used = o->beta;
+    F.admm_steps = 0;
+    F.upd = F.upd2 = NAN;
     CK(F.F2b.ensure(sizeof(double) * 4 * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
     CK(fem_assemble(ctx->mesh, o->beta, o->mu, 0.0, ctx->stream));
-    if (!o->no_subpb1_fit) {
-        // Subpb1's gridfit grid: xnodes = unique node x, ynodes = unique node y
+    if (!o->no_subpb1_fit || o->smooth) {
+        // gridfit grid (Subpb1's refit and the smoothing passes): xnodes = unique node x,
+        // ynodes = unique node y (= min:step:max of funSmoothDisp.m on this mesh)
         if (F.Nx < 2 || F.Ny < 2)
             return fail(ctx, ALDIC_ERR_ARG, "frame_setup: gridfit needs at least 2 x 2 nodes");
         std::vector<double> xn(F.Nx), yn(F.Ny);
@@ -932,23 +979,85 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     double *F2buf[2] = {F.F2.as<double>(), F.F2b.as<double>()};
     int cur = 0;
     const double *F1p = F.F1.as<double>();
-    auto global_step = [&](bool zero_duals) -> int {
+    double *U1 = F.U1.as<double>(), *U2 = F.U2.as<double>();
+    const bool smooth = o.smooth != 0, stop = o.admm_tol > 0.0;
+    // every gridfit records {converged, iterations, residual} for the budget check
+    int ngf = 0;
+    auto gfit = [&](double *X, int np, const int *bad) -> hipError_t {
+        double *rec = ngf < F.gf_cap ? F.gfstat.as<double>() + 3 * ngf++ : nullptr;
+        const bool poll = gf_poll || !rec;
+        return gridfit_solve(F.gf, bad, X, np, 1e-10, poll ? 4000 : F.gf_budget, work, rec, st,
+                             poll ? 8 : 0);
+    };
+    // funSmoothDisp.m / funSmoothStrain.m: two passes of (gridfit of every node, 3x3
+    // Gaussian); nc values per node, tmp holds the first pass
+    auto smooth_field = [&](double *X, int nc, double *tmp) -> int {
+        CK(gfit(X, nc / 2, nullptr));
+        CK(launch_blur3(X, tmp, F.Nx, F.Ny, nc, 0.5, st));
+        CK(gfit(tmp, nc / 2, nullptr));
+        CK(launch_blur3(tmp, X, F.Nx, F.Ny, nc, 0.5, st));
+        return ALDIC_OK;
+    };
+    int r;
+    if (smooth && (r = smooth_field(F.F1.as<double>(), 4, F.Fs.as<double>()))) return r;
+
+    double beta = o.beta;
+    if (o.beta_sweep) {
+        // main_ALDIC.m:237,274-301 (FE branch): the step-1 global step at six candidates
+        const double fac[6] = {1e-3, std::sqrt(1e-5), 1e-2, std::sqrt(1e-3), 1e-1, std::sqrt(1e-1)};
+        double bl[6], e1[6], e2[6];
+        double *errs = F.errs.as<double>();
+        for (int k = 0; k < 6; ++k) {
+            bl[k] = fac[k] * o.step * o.step * o.mu;
+            CK(fem_assemble(ctx->mesh, bl[k], o.mu, 0.0, st));
+            CK(fem_rhs(ctx->mesh, bl[k], o.mu, U1, F1p, nullptr, nullptr, b, st));
+            CK(fem_solve(ctx->mesh, b, U2, o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
+            CK(fem_strain(ctx->mesh, U2, 0, F.Fs.as<double>(), st));
+            CK(launch_sqdiff(U1, U2, 2 * (long)S, errs + 2 * k, st));
+            CK(launch_sqdiff(F1p, F.Fs.as<double>(), 4 * (long)S, errs + 2 * k + 1, st));
+        }
+        CK(hipMemcpyAsync(F.hostd, errs, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        for (int k = 0; k < 6; ++k) {
+            e1[k] = std::sqrt(F.hostd[2 * k]);
+            e2[k] = std::sqrt(F.hostd[2 * k + 1]);
+        }
+        beta = beta_choice(bl, e1, e2);
+        CK(fem_assemble(ctx->mesh, beta, o.mu, 0.0, st));
+    }
+    F.beta_used = beta;
+
+    auto global_step = [&](bool zero_duals, bool smooth_u) -> int {
         const int g0 = smark();
-        CK(fem_rhs(ctx->mesh, o.beta, o.mu, F.U1.as<double>(), F1p,
-                   zero_duals ? nullptr : F.udual.as<double>(),
+        CK(fem_rhs(ctx->mesh, beta, o.mu, U1, F1p, zero_duals ? nullptr : F.udual.as<double>(),
                    zero_duals ? nullptr : F.vdual.as<double>(), b, st));
-        CK(fem_solve(ctx->mesh, b, F.U2.as<double>(), o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
+        CK(fem_solve(ctx->mesh, b, U2, o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
         const int g1 = smark();
         span(4, g0, g1);
         // nodal strain and the dual update of the FEM branch (main_ALDIC.m:340,440):
-        // udual = F2 - F1, vdual = U2 - U1
-        CK(fem_strain(ctx->mesh, F.U2.as<double>(), 0, F2buf[cur], st, F1p, F.U1.as<double>(),
-                      F.udual.as<double>(), F.vdual.as<double>()));
+        // udual = F2 - F1, vdual = U2 - U1 (after the smoothing passes when they run)
+        if (!smooth) {
+            CK(fem_strain(ctx->mesh, U2, 0, F2buf[cur], st, F1p, U1, F.udual.as<double>(),
+                          F.vdual.as<double>()));
+        } else {
+            int rr;
+            if (smooth_u && (rr = smooth_field(U2, 2, F.Us.as<double>()))) return rr;
+            CK(fem_strain(ctx->mesh, U2, 0, F2buf[cur], st));
+            if ((rr = smooth_field(F2buf[cur], 4, F.Fs.as<double>()))) return rr;
+            CK(launch_sub(F2buf[cur], F1p, F.udual.as<double>(), 4 * (long)S, st));
+            CK(launch_sub(U2, U1, F.vdual.as<double>(), 2 * (long)S, st));
+        }
         span(6, g1, smark());
         return ALDIC_OK;
     };
-    int r = global_step(true);
+    // the stop rule compares Subpb1's U (before smoothing; LocalICGN's for step 1) and U2
+    // with the previous step's
+    if (stop) CK(hipMemcpyAsync(F.U1old.p, U1, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
+    r = global_step(true, false);
     if (r) return r;
+    if (stop) CK(hipMemcpyAsync(F.U2old.p, U2, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
+    F.admm_steps = 0;
+    F.upd = F.upd2 = NAN;
 
     for (int step = 0; step < o.n_admm; ++step) {
         // Subpb1: 2-DOF IC-GN from (U2, F2)
@@ -957,10 +1066,10 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         a2.U0 = nullptr;
         a2.Hin = F.H21.as<double>();
         a2.vdual = F.vdual.as<double>();
-        a2.Uold = F.U2.as<double>();
+        a2.Uold = U2;
         a2.Fold = F2buf[cur];
         a2.mu = o.mu;
-        a2.U = F.U1.as<double>();
+        a2.U = U1;
         a2.iters = F.it2.as<int>();
         a2.F = nullptr;
         a2.H21 = nullptr;
@@ -976,24 +1085,36 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
             ar2.nlist_dev = ar.nlist_dev;
             CK(launch_icgn2(ar2, ppl_big, st));
         }
-        CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, F.U1.as<double>(),
-                          nullptr, F.bad.as<int>(), F.part.as<double>(), st));
+        CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, U1, nullptr,
+                          F.bad.as<int>(), F.part.as<double>(), st));
         // Subpb1.m:185-199: gridfit of u and v over the good nodes, written to every node
         // (the refilled values above are only its starting guess)
-        if (!o.no_subpb1_fit)
-            CK(gridfit_solve(F.gf, F.bad.as<int>(), F.U1.as<double>(), 1e-10,
-                             gf_poll ? 4000 : F.gf_budget, work,
-                             F.gfstat.as<double>() + 3 * step, st, gf_poll ? 8 : 0));
+        if (!o.no_subpb1_fit) CK(gfit(U1, 1, F.bad.as<int>()));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         cur ^= 1;
+        if (stop) CK(hipMemcpyAsync(F.U1s.p, U1, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
+        if (smooth && (r = smooth_field(U1, 2, F.Us.as<double>()))) return r;
         span(5, i0, smark());
-        r = global_step(false);
+        r = global_step(false, smooth);
         if (r) return r;
+        ++F.admm_steps;
+        if (stop) {
+            double *errs = F.errs.as<double>();
+            CK(launch_sqdiff(F.U2old.as<double>(), U2, 2 * (long)S, errs, st));
+            CK(launch_sqdiff(F.U1old.as<double>(), F.U1s.as<double>(), 2 * (long)S, errs + 1, st));
+            CK(hipMemcpyAsync(F.hostd, errs, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            CK(hipMemcpyAsync(F.U2old.p, U2, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
+            CK(hipMemcpyAsync(F.U1old.p, F.U1s.p, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
+            CK(hipStreamSynchronize(st));
+            F.upd = std::sqrt(F.hostd[0] / (2.0 * S));
+            F.upd2 = std::sqrt(F.hostd[1] / (2.0 * S));
+            if (F.upd < o.admm_tol || F.upd2 < o.admm_tol) break;
+        }
     }
     F.f2cur = cur;
-    if (!o.no_subpb1_fit && o.n_admm > 0)
-        CK(hipMemcpyAsync(F.gfh, F.gfstat.p, 3 * sizeof(double) * o.n_admm,
-                          hipMemcpyDeviceToHost, st));
+    F.gf_count = ngf;
+    if (ngf > 0)
+        CK(hipMemcpyAsync(F.gfh, F.gfstat.p, 3 * sizeof(double) * ngf, hipMemcpyDeviceToHost, st));
     const int e_end = mark();
     span(7, e_start, e_end);
     if (e_end < 0) return fail(ctx, ALDIC_ERR_HIP, "frame_run: event pool exhausted");
@@ -1019,21 +1140,30 @@ int aldic_frame_run(aldic_ctx *ctx)
     // on the staged images and options, so the second pass gives the polled result).
     int r = frame_pass(ctx, false);
     if (r) return r;
-    if (F.o.no_subpb1_fit || F.o.n_admm == 0) return ALDIC_OK;
+    if (F.gf_count == 0) return ALDIC_OK;
     bool ok = true;
     int most = 0;
-    for (int k = 0; k < F.o.n_admm; ++k) {
+    for (int k = 0; k < F.gf_count; ++k) {
         ok = ok && F.gfh[3 * k] != 0.0;
         most = std::max(most, (int)F.gfh[3 * k + 1]);
     }
     if (!ok) {
         ++F.gf_reruns;
         if ((r = frame_pass(ctx, true)) != ALDIC_OK) return r;
-        for (int k = 0; k < F.o.n_admm; ++k) most = std::max(most, (int)F.gfh[3 * k + 1]);
+        for (int k = 0; k < F.gf_count; ++k) most = std::max(most, (int)F.gfh[3 * k + 1]);
     }
     // the next frame's budget: the most iterations seen here and a margin
     F.gf_most = most;
     F.gf_budget = std::min(64, std::max(6, most + 3));
+    return ALDIC_OK;
+}
+
+int aldic_frame_scalars(aldic_ctx *ctx, double *out, int n)
+{
+    if (!ctx || !out || n < 0 || n > 4) return ALDIC_ERR_ARG;
+    const FrameState &F = ctx->fr;
+    const double v[4] = {F.beta_used, (double)F.admm_steps, F.upd, F.upd2};
+    for (int k = 0; k < n; ++k) out[k] = v[k];
     return ALDIC_OK;
 }
 

@@ -724,6 +724,32 @@ __global__ __launch_bounds__(256) void k_sub(const double *a, const double *b, d
     if (i < n) o[i] = a[i] - b[i];
 }
 
+// out[0] = sum (a - b)^2 over n values: one workgroup, fixed order (deterministic)
+__global__ __launch_bounds__(1024) void k_sqdiff(const double *a, const double *b, long n,
+                                                 double *out)
+{
+    __shared__ double red[16];
+    double s = 0.0;
+    for (long i = threadIdx.x; i < n; i += 1024) {
+        const double d = a[i] - b[i];
+        s = fma(d, d, s);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < 16; ++k) t += red[k];
+        out[0] = t;
+    }
+}
+
+hipError_t launch_sqdiff(const double *a, const double *b, long n, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sqdiff, dim3(1), dim3(1024), 0, st, a, b, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sub, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, out, n);

@@ -65,7 +65,7 @@ __device__ __forceinline__ void gf_block3(double a, double b, double c, double *
 // values sit slot-major, a[k * n + node] for the neighbour (dx, dy) = (k % 3 - 1,
 // k / 3 - 1), so every load of an iteration is independent of the others.
 struct GfCg {
-    int n, gx, gy;
+    int n, gx, gy;  // nodes (NP value pairs each), grid
     double *a9, *dinv;         // stencil values of N (0 outside the grid), 1 / diag(N)
     double2 *x;                // in: starting guess, out: solution (interleaved u, v)
     double2 *r[2], *w[2], *s[2], *p;
@@ -84,6 +84,7 @@ __device__ __forceinline__ bool gf_in(int ix, int iy, int k, int gx, int gy)
 
 // K0: stencil of N from the data mask (squared spring stiffnesses w), b = D d with the
 // data d = x at the data nodes, r_0 = b - N x_0; p and s start at 0.
+template <int NP>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, const double *w, double lam2,
                                                        const int *bad)
 {
@@ -92,7 +93,10 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, const double *w, 
     double bb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
         const int ix = i % gx, iy = i / gx;
-        double dsum = 0.0, au = 0.0, av = 0.0;
+        double dsum = 0.0;
+        double2 ax[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) ax[k] = make_double2(0.0, 0.0);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if (k == 4) continue;
@@ -105,58 +109,73 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, const double *w, 
                 else wt = 1.0 / (axs[min(ix, ix + dx)] + ays[min(iy, iy + dy)]);
                 v = -lam2 * wt;
                 dsum += wt;
-                const double2 xj = c.x[i + dy * gx + dx];
-                au += v * xj.x;
-                av += v * xj.y;
+                const int j = i + dy * gx + dx;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    const double2 xj = c.x[j * NP + q];
+                    ax[q].x += v * xj.x;
+                    ax[q].y += v * xj.y;
+                }
             }
             c.a9[k * c.n + i] = v;
         }
-        const bool gd = !bad[i];
+        const bool gd = !bad || !bad[i];
         const double d = (gd ? 1.0 : 0.0) + lam2 * dsum;
         c.a9[4 * c.n + i] = d;
         c.dinv[i] = 1.0 / d;
-        const double2 xi = c.x[i];
-        au += d * xi.x;
-        av += d * xi.y;
-        const double2 b = gd ? xi : make_double2(0.0, 0.0);
-        c.r[0][i] = make_double2(b.x - au, b.y - av);
-        c.p[i] = make_double2(0.0, 0.0);
-        c.s[0][i] = make_double2(0.0, 0.0);
-        bb += b.x * b.x + b.y * b.y;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const double2 xi = c.x[i * NP + q];
+            const double2 b = gd ? xi : make_double2(0.0, 0.0);
+            c.r[0][i * NP + q] = make_double2(b.x - (ax[q].x + d * xi.x), b.y - (ax[q].y + d * xi.y));
+            c.p[i * NP + q] = make_double2(0.0, 0.0);
+            c.s[0][i * NP + q] = make_double2(0.0, 0.0);
+            bb += b.x * b.x + b.y * b.y;
+        }
     }
     gf_block3(bb, 0.0, 0.0, c.BB, nullptr, nullptr);
     if (blockIdx.x == 0 && threadIdx.x == 0) { c.stat[0] = 0.0; c.stat[1] = 0.0; c.stat[2] = 1.0; }
 }
 
 // K1: w_0 = N u_0 with u_0 = M^-1 r_0; partials gamma_0, delta_0, r_0.r_0
+template <int NP>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cg1(GfCg c)
 {
     const int gx = c.gx, gy = c.gy;
     double g = 0.0, dl = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
         const int ix = i % gx, iy = i / gx;
-        double wu = 0.0, wv = 0.0;
+        double2 wa[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) wa[q] = make_double2(0.0, 0.0);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if (!gf_in(ix, iy, k, gx, gy)) continue;
             const int j = i + (k / 3 - 1) * gx + (k % 3 - 1);
-            const double2 rj = c.r[0][j];
             const double av = c.a9[k * c.n + i] * c.dinv[j];
-            wu += av * rj.x;
-            wv += av * rj.y;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const double2 rj = c.r[0][j * NP + q];
+                wa[q].x += av * rj.x;
+                wa[q].y += av * rj.y;
+            }
         }
-        const double2 ri = c.r[0][i];
         const double dinv = c.dinv[i];
-        const double uu = ri.x * dinv, uv = ri.y * dinv;
-        c.w[0][i] = make_double2(wu, wv);
-        g += ri.x * uu + ri.y * uv;
-        dl += wu * uu + wv * uv;
-        rr += ri.x * ri.x + ri.y * ri.y;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const double2 ri = c.r[0][i * NP + q];
+            const double uu = ri.x * dinv, uv = ri.y * dinv;
+            c.w[0][i * NP + q] = wa[q];
+            g += ri.x * uu + ri.y * uv;
+            dl += wa[q].x * uu + wa[q].y * uv;
+            rr += ri.x * ri.x + ri.y * ri.y;
+        }
     }
     gf_block3(g, dl, rr, c.G[0], c.D[0], c.RR[0]);
 }
 
 // iteration i (reads parity i & 1, writes (i + 1) & 1)
+template <int NP>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
 {
     const int cur = i & 1, nxt = cur ^ 1;
@@ -196,38 +215,51 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
     double g = 0.0, dl = 0.0, rn = 0.0;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < c.n; q += gridDim.x * blockDim.x) {
         const int ix = q % gx, iy = q / gx;
-        double wu = 0.0, wv = 0.0;
-        double2 rnew_i = make_double2(0.0, 0.0), snew_i = rnew_i;
+        double2 wa[NP], rnew[NP], snew[NP];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            wa[m] = make_double2(0.0, 0.0);
+            rnew[m] = wa[m];
+            snew[m] = wa[m];
+        }
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             if (!gf_in(ix, iy, k, gx, gy)) continue;
             const int j = q + (k / 3 - 1) * gx + (k % 3 - 1);
-            const double2 rj = c.r[cur][j], wj = c.w[cur][j], sj = c.s[cur][j];
-            const double2 sn = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
-            const double2 rnj = make_double2(rj.x - alpha * sn.x, rj.y - alpha * sn.y);
             const double av = c.a9[k * c.n + q] * c.dinv[j];
-            wu += av * rnj.x;
-            wv += av * rnj.y;
-            if (k == 4) { rnew_i = rnj; snew_i = sn; }
+#pragma unroll
+            for (int m = 0; m < NP; ++m) {
+                const double2 rj = c.r[cur][j * NP + m], wj = c.w[cur][j * NP + m],
+                              sj = c.s[cur][j * NP + m];
+                const double2 sn = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
+                const double2 rnj = make_double2(rj.x - alpha * sn.x, rj.y - alpha * sn.y);
+                wa[m].x += av * rnj.x;
+                wa[m].y += av * rnj.y;
+                if (k == 4) { rnew[m] = rnj; snew[m] = sn; }
+            }
         }
         const double dinv = c.dinv[q];
-        const double2 ri = c.r[cur][q];
-        // p_i = u_i + beta p_{i-1}, x_{i+1} = x_i + alpha p_i
-        double2 pi = c.p[q];
-        pi.x = ri.x * dinv + beta * pi.x;
-        pi.y = ri.y * dinv + beta * pi.y;
-        c.p[q] = pi;
-        double2 xi = c.x[q];
-        xi.x += alpha * pi.x;
-        xi.y += alpha * pi.y;
-        c.x[q] = xi;
-        c.r[nxt][q] = rnew_i;
-        c.s[nxt][q] = snew_i;
-        c.w[nxt][q] = make_double2(wu, wv);
-        const double uu = rnew_i.x * dinv, uv = rnew_i.y * dinv;
-        g += rnew_i.x * uu + rnew_i.y * uv;
-        dl += wu * uu + wv * uv;
-        rn += rnew_i.x * rnew_i.x + rnew_i.y * rnew_i.y;
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const int e = q * NP + m;
+            const double2 ri = c.r[cur][e];
+            // p_i = u_i + beta p_{i-1}, x_{i+1} = x_i + alpha p_i
+            double2 pi = c.p[e];
+            pi.x = ri.x * dinv + beta * pi.x;
+            pi.y = ri.y * dinv + beta * pi.y;
+            c.p[e] = pi;
+            double2 xi = c.x[e];
+            xi.x += alpha * pi.x;
+            xi.y += alpha * pi.y;
+            c.x[e] = xi;
+            c.r[nxt][e] = rnew[m];
+            c.s[nxt][e] = snew[m];
+            c.w[nxt][e] = wa[m];
+            const double uu = rnew[m].x * dinv, uv = rnew[m].y * dinv;
+            g += rnew[m].x * uu + rnew[m].y * uv;
+            dl += wa[m].x * uu + wa[m].y * uv;
+            rn += rnew[m].x * rnew[m].x + rnew[m].y * rnew[m].y;
+        }
     }
     gf_block3(g, dl, rn, c.G[nxt], c.D[nxt], c.RR[nxt]);
 }
@@ -289,13 +321,14 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
     return hipStreamSynchronize(st);
 }
 
-hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
-                         DBuf *work, double *stats_dev, hipStream_t st, int poll)
+template <int NP>
+static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
+                           DBuf *work, double *stats_dev, hipStream_t st, int poll)
 {
     const int n = g.gx * g.gy;
     const int nb = std::min(kGfMaxBlocks, (n + kGfThreads - 1) / kGfThreads);
     hipError_t e;
-    const size_t vb = sizeof(double2) * (size_t)n;
+    const size_t vb = sizeof(double2) * (size_t)n * NP;
     if ((e = work[0].ensure(7 * vb)) != hipSuccess) return e;
     if ((e = work[1].ensure(sizeof(double) * (7 * (size_t)nb + 8))) != hipSuccess) return e;
     GfCg c{};
@@ -306,8 +339,9 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, 
     c.dinv = g.dinv.as<double>();
     c.x = reinterpret_cast<double2 *>(U);
     double2 *v = work[0].as<double2>();
-    c.r[0] = v; c.r[1] = v + n; c.w[0] = v + 2 * n; c.w[1] = v + 3 * n;
-    c.s[0] = v + 4 * n; c.s[1] = v + 5 * n; c.p = v + 6 * n;
+    const size_t m = (size_t)n * NP;
+    c.r[0] = v; c.r[1] = v + m; c.w[0] = v + 2 * m; c.w[1] = v + 3 * m;
+    c.s[0] = v + 4 * m; c.s[1] = v + 5 * m; c.p = v + 6 * m;
     double *ps = work[1].as<double>();
     c.G[0] = ps; c.G[1] = ps + nb; c.D[0] = ps + 2 * nb; c.D[1] = ps + 3 * nb;
     c.RR[0] = ps + 4 * nb; c.RR[1] = ps + 5 * nb; c.BB = ps + 6 * nb;
@@ -315,15 +349,15 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, 
     c.stat = ps + 7 * nb + 4;
     c.nb = nb;
     c.rtol2 = rtol * rtol;
-    hipLaunchKernelGGL(k_gf_cg0, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
+    hipLaunchKernelGGL(k_gf_cg0<NP>, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
                        g.lam2, bad);
-    hipLaunchKernelGGL(k_gf_cg1, dim3(nb), dim3(kGfThreads), 0, st, c);
+    hipLaunchKernelGGL(k_gf_cg1<NP>, dim3(nb), dim3(kGfThreads), 0, st, c);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // poll <= 0: maxit iteration kernels and no host wait (the caller reads stat)
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
         for (int q = 0; q < chunk && it + q < maxit; ++q)
-            hipLaunchKernelGGL(k_gf_cgit, dim3(nb), dim3(kGfThreads), 0, st, c, it + q);
+            hipLaunchKernelGGL(k_gf_cgit<NP>, dim3(nb), dim3(kGfThreads), 0, st, c, it + q);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
         double done = 0.0;
@@ -340,6 +374,55 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, double rtol, 
             return e;
     }
     return hipSuccess;
+}
+
+hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, double rtol, int maxit,
+                         DBuf *work, double *stats_dev, hipStream_t st, int poll)
+{
+    if (np == 1) return solve_np<1>(g, bad, U, rtol, maxit, work, stats_dev, st, poll);
+    if (np == 2) return solve_np<2>(g, bad, U, rtol, maxit, work, stats_dev, st, poll);
+    return hipErrorInvalidValue;
+}
+
+namespace {
+
+// nanconv(Z, fspecial('gaussian', 3, sigma), 'edge') on the node grid (no NaNs): the 3x3
+// Gaussian is separable, w = [e, 1, e] / (1 + 2 e) with e = exp(-1 / (2 sigma^2)) per
+// axis, and the edge correction divides by the weight inside the grid, wx(ix) wy(iy).
+__global__ __launch_bounds__(256) void k_blur3(const double *in, double *out, int gx, int gy,
+                                               int nc, double e)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gx * gy * nc) return;
+    const int c = t % nc, i = t / nc, ix = i % gx, iy = i / gx;
+    const double w1 = 1.0 / (1.0 + 2.0 * e), w0 = e * w1;
+    const double wl[3] = {w0, w1, w0};
+    double sx = 0.0, sy = 0.0, acc = 0.0;
+    for (int d = -1; d <= 1; ++d) {
+        if (ix + d >= 0 && ix + d < gx) sx += wl[d + 1];
+        if (iy + d >= 0 && iy + d < gy) sy += wl[d + 1];
+    }
+    for (int dy = -1; dy <= 1; ++dy) {
+        if (iy + dy < 0 || iy + dy >= gy) continue;
+        double row = 0.0;
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (ix + dx < 0 || ix + dx >= gx) continue;
+            row += wl[dx + 1] * in[(size_t)(i + dy * gx + dx) * nc + c];
+        }
+        acc += wl[dy + 1] * row;
+    }
+    out[t] = acc / (sx * sy);
+}
+
+}  // namespace
+
+hipError_t launch_blur3(const double *in, double *out, int gx, int gy, int nc, double sigma,
+                        hipStream_t st)
+{
+    const int n = gx * gy * nc;
+    hipLaunchKernelGGL(k_blur3, dim3((n + 255) / 256), dim3(256), 0, st, in, out, gx, gy, nc,
+                       std::exp(-1.0 / (2.0 * sigma * sigma)));
+    return hipGetLastError();
 }
 
 }  // namespace aldic

@@ -156,6 +156,16 @@ typedef struct {
     int stage_times;   /* 1: an event at every stage boundary (all of aldic_frame_times);
                         * 0: events only around the search, the 6-DOF IC-GN and the
                         * whole frame (each event costs the stream a few microseconds) */
+    int smooth;        /* 1: main_ALDIC.m's smoothing passes (funSmoothStrain.m, funSmoothDisp.m:
+                        * two passes of gridfit 'springs' over the node grid + 3x3 Gaussian
+                        * sigma 0.5 with edge correction): F1 after LocalICGN (:233), F2 after
+                        * every global step (:320,:407), U1 and U2 in the ADMM steps (:381,:400) */
+    int beta_sweep;    /* 1: beta from the step-1 global step at the six candidates
+                        * [1e-3, 1e-5^.5, 1e-2, 1e-3^.5, 1e-1, 1e-1^.5] step^2 mu (main_ALDIC.m:
+                        * 237,274-301, FE branch); the beta field is then unused */
+    double admm_tol;   /* > 0: stop the ADMM steps once the RMS change of U2 or of Subpb1's U
+                        * (before smoothing) between steps is below it (main_ALDIC.m:409-447,
+                        * tol2 = 1e-4); n_admm stays the most steps (the reference: 4) */
 } aldic_frame_opts;
 
 int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
@@ -175,6 +185,11 @@ int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
  * frames run a second time because a gridfit needed more, largest gridfit iteration
  * count of the last frame}. n <= 3. */
 int aldic_frame_counters(aldic_ctx *ctx, int *out, int n);
+/* Scalars of the last frame (no counterpart in the reference's outputs; main_ALDIC.m
+ * prints them): {beta used, ADMM steps run, RMS change of U2 in the last step, RMS change
+ * of Subpb1's U in the last step (both measured only with admm_tol > 0, else NaN)}.
+ * n <= 4. */
+int aldic_frame_scalars(aldic_ctx *ctx, double *out, int n);
 
 #ifdef __cplusplus
 }

@@ -459,6 +459,36 @@ def test_frame_gridfit_over_budget_reruns(gpu_ctx, cfg1, orc):
     assert np.sqrt(np.mean((out["U"][0::2] - ut) ** 2)) < 0.2
 
 
+@pytest.mark.parametrize("case", ["smooth_sweep_stop", "smooth_only", "stop_early"])
+def test_frame_smoothing_beta_sweep_stop_rule(gpu_ctx, cfg1, orc, case):
+    """main_ALDIC.m's driver options: smoothing passes (gridfit + 3x3 Gaussian, two each)
+    on F1, F2, U1, U2; the six-point beta choice; the stop rule on the RMS updates."""
+    _flush(f"frame options {case}")
+    kw = dict(smooth=True, beta_sweep=True, admm_tol=1e-4)
+    if case == "smooth_only":
+        kw = dict(smooth=True)
+    elif case == "stop_early":
+        kw = dict(admm_tol=5e-3)
+    gpu_ctx.frame_stage(cfg1.f, cfg1.g)
+    gpu_ctx.frame_setup(20, 10, 8, [1, 256, 1, 256], tol=1e-2, mu=1e-3, beta=0.1, n_admm=4, **kw)
+    gpu_ctx.frame_run()
+    out = gpu_ctx.frame_get()
+    sc = gpu_ctx.frame_scalars()
+    o = orc.frame(cfg1.f, cfg1.g, 20, 10, 8, [1, 256, 1, 256], tol=1e-2, mu=1e-3, beta=0.1,
+                  n_admm=4, **kw)
+    assert abs(sc["beta"] - o["beta"]) <= 1e-5 * o["beta"], (sc["beta"], o["beta"])
+    assert sc["admm_steps"] == o["admm_steps"], (sc, o["admm_steps"])
+    if case == "stop_early":
+        assert o["admm_steps"] < 4
+    if np.array_equal(out["iters6"], o["iters6"]) and np.array_equal(out["iters2"], o["iters2"]):
+        assert np.max(np.abs(out["U"] - o["U"])) < 1e-4
+        assert np.max(np.abs(out["F"] - o["F"])) < 1e-5
+        if kw.get("admm_tol", 0) > 0:   # the device measures the updates for the stop rule
+            assert abs(sc["update"][0] - o["update"][0]) < 1e-6 + 1e-3 * o["update"][0]
+    ut, vt = cfg1.ufun(o["xy"][:, 0], o["xy"][:, 1])
+    assert np.sqrt(np.mean((out["U"][0::2] - ut) ** 2)) < 0.05
+
+
 def test_frame_matches_oracle_frame_config1(gpu_ctx, cfg1, orc):
     # whole device frame (prep, search, LocalICGN + bad-point refill, 1 + 3 ADMM global
     # steps) against the FP64 restatement of the same frame
