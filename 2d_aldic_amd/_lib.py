@@ -23,7 +23,8 @@ EXPORTS = (
     "aldic_integer_search", "aldic_icgn6", "aldic_icgn2", "aldic_set_mesh", "aldic_subpb2",
     "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
-    "aldic_frame_counters", "aldic_frame_scalars",
+    "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
+    "aldic_subpb2_quadtree",
 )
 
 
@@ -73,6 +74,8 @@ def load():
     L.aldic_gridfit.argtypes = [P, ip, ip, vp, vp, vp, dp, ip, vp, vp]
     L.aldic_frame_counters.argtypes = [P, vp, ip]
     L.aldic_frame_scalars.argtypes = [P, vp, ip]
+    L.aldic_set_quadtree.argtypes = [P, vp, ip, vp, ip, vp, ip]
+    L.aldic_subpb2_quadtree.argtypes = [P, dp, dp, vp, vp, dp, ip, vp, vp]
     _lib = L
     return L
 
@@ -215,6 +218,27 @@ class Context:
                                      _p(_c(U, np.float64)), _p(_c(F, np.float64)),
                                      _p(_c(udual, np.float64)), _p(_c(vdual, np.float64)),
                                      float(rtol), int(maxit), _p(Uh), _p(st)), "aldic_subpb2")
+        return Uh, st
+
+    def set_quadtree(self, coords, elems8, dirichlet=()):
+        """Quadtree mesh: coords (nn, 2), elems8 (E, 8) 0-based with -1 for an absent
+        midside node, Dirichlet node indices (0-based)."""
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        elems8 = _c(elems8, np.int32).reshape(-1, 8)
+        d = _c(dirichlet, np.int32).ravel()
+        self._ck(self.L.aldic_set_quadtree(self.h, _p(coords), coords.shape[0], _p(elems8),
+                                           elems8.shape[0], _p(d) if d.size else None, d.size),
+                 "aldic_set_quadtree")
+        self.n_quad_nodes = coords.shape[0]
+
+    def subpb2_quadtree(self, beta, mu, U, F, rtol=1e-12, maxit=5000):
+        n = self.n_quad_nodes
+        Uh = np.empty(2 * n)
+        st = np.zeros(2)
+        self._ck(self.L.aldic_subpb2_quadtree(self.h, float(beta), float(mu),
+                                              _p(_c(U, np.float64)), _p(_c(F, np.float64)),
+                                              float(rtol), int(maxit), _p(Uh), _p(st)),
+                 "aldic_subpb2_quadtree")
         return Uh, st
 
     def nodal_strain(self, U, swap_fix=False):

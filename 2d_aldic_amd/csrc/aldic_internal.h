@@ -17,6 +17,13 @@ struct N2E {
     int slot[4];
 };
 
+// The same for the quadtree's 8-node elements (corners 0..3, midside nodes 4..7 that
+// may be absent); slot[b] is unused where element node b is absent.
+struct N2E8 {
+    int e, a;
+    int slot[8];
+};
+
 // Device buffer that only grows.
 struct DBuf {
     void *p = nullptr;
@@ -61,6 +68,15 @@ struct MeshDev {
     double hx = 0, hy = 0;
     DBuf fdV;                 // double[gx*gx + gy*gy]: M-orthonormal eigenbases Vx, Vy
     DBuf fdlam;               // double[gx + gy]: generalised eigenvalues of (K1, M1)
+};
+
+// Quadtree mesh of the global step: node rows of an 8-node-element mesh in m (coords,
+// n2e_ptr, CSR pattern, aval / mval / diag / gvec), the elements (int[8 E], -1 = no
+// midside node) and their incidences, each node's role code (0 free, 1 Dirichlet =
+// kept at U, 2 fixed at 0) and cval = the matrix entries of free rows in fixed columns.
+struct QuadDev {
+    MeshDev m;
+    DBuf elems, n2e, code, cval;
 };
 
 // gridfit ('springs') of nodal data on a rectilinear node grid, data on the nodes
@@ -129,6 +145,8 @@ struct aldic_ctx {
     aldic::DBuf scratch[8];          // per-call work buffers
     aldic::MeshDev mesh;
     bool have_mesh = false;
+    aldic::QuadDev quad;             // aldic_set_quadtree / aldic_subpb2_quadtree
+    bool have_quad = false;
     aldic::FrameState fr;
     aldic::GridfitDev gf;            // aldic_gridfit
     hipEvent_t ev[10] = {};
@@ -223,6 +241,14 @@ hipError_t fem_solve(const MeshDev &m, const double *b, double *x, double rtol, 
 hipError_t fem_strain(const MeshDev &m, const double *U, int swap_fix, double *F, hipStream_t st,
                       const double *F1 = nullptr, const double *U1 = nullptr,
                       double *udual = nullptr, double *vdual = nullptr);
+
+// Quadtree global step: assemble beta K + mu M with the fixed nodes' rows made identity
+// rows; rhs b = beta G . F + mu M U at free nodes minus the fixed columns' share, 0 at
+// fixed nodes; quad_finish writes the fixed nodes' values into the solution.
+hipError_t quad_assemble(QuadDev &q, double beta, double mu, hipStream_t st);
+hipError_t quad_rhs(QuadDev &q, double beta, double mu, const double *U, const double *F,
+                    double *b, hipStream_t st);
+hipError_t quad_finish(QuadDev &q, const double *U, double *x, hipStream_t st);
 
 // ADMM helpers
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);

@@ -275,6 +275,10 @@ void aldic_destroy(aldic_ctx *ctx)
     if (F.gfh) (void)hipHostFree(F.gfh);
     if (F.hostd) (void)hipHostFree(F.hostd);
     for (DBuf *bb : {&F.Us, &F.Fs, &F.U1s, &F.U1old, &F.U2old, &F.errs}) bb->release();
+    aldic::QuadDev &Q = ctx->quad;
+    for (DBuf *bb : {&Q.m.coords, &Q.m.n2e_ptr, &Q.m.rowptr, &Q.m.col, &Q.m.aval, &Q.m.mval,
+                     &Q.m.diag, &Q.m.gvec, &Q.elems, &Q.n2e, &Q.code, &Q.cval, &ctx->boxp})
+        bb->release();
     for (auto &e : F.ev) (void)hipEventDestroy(e);
     F.ev.clear();
     for (auto &e : ctx->ev)
@@ -669,6 +673,125 @@ int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const dou
     static thread_local DBuf work[8];
     double st[2] = {0, 0};
     CK(fem_solve(m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st, ctx->stream, 8));
+    CK(down(Uhat, sc[5].p, 2 * nn, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (stats) { stats[0] = st[0]; stats[1] = st[1]; }
+    return ALDIC_OK;
+}
+
+int aldic_set_quadtree(aldic_ctx *ctx, const double *coords, int nn, const int *elems, int E,
+                       const int *dirichlet, int nd)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (nn <= 0 || E < 0 || nd < 0 || !coords || (E > 0 && !elems) || (nd > 0 && !dirichlet))
+        return fail(ctx, ALDIC_ERR_ARG, "set_quadtree: bad arguments");
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 8; ++a) {
+            const int v = elems[8 * e + a];
+            if (v >= nn || (v < 0 && (a < 4 || v != -1)))
+                return fail(ctx, ALDIC_ERR_ARG, "set_quadtree: node index");
+        }
+    for (int k = 0; k < nd; ++k)
+        if (dirichlet[k] < 0 || dirichlet[k] >= nn)
+            return fail(ctx, ALDIC_ERR_ARG, "set_quadtree: dirichlet node index");
+    CK(hipSetDevice(ctx->device));
+    aldic::QuadDev &q = ctx->quad;
+    aldic::MeshDev &m = q.m;
+    std::vector<int> cnt(nn + 1, 0);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 8; ++a)
+            if (elems[8 * e + a] >= 0) cnt[elems[8 * e + a] + 1]++;
+    for (int i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
+    std::vector<aldic::N2E8> n2e(cnt[nn]);
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 8; ++a) {
+            const int v = elems[8 * e + a];
+            if (v < 0) continue;
+            aldic::N2E8 r{};
+            r.e = e;
+            r.a = a;
+            n2e[fillp[v]++] = r;
+        }
+    std::vector<int> rowptr(nn + 1, 0), col;
+    col.reserve((size_t)nn * 13);
+    for (int i = 0; i < nn; ++i) {
+        std::vector<int> nb{i};
+        for (int p = cnt[i]; p < cnt[i + 1]; ++p)
+            for (int b = 0; b < 8; ++b)
+                if (elems[8 * n2e[p].e + b] >= 0) nb.push_back(elems[8 * n2e[p].e + b]);
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        for (int p = cnt[i]; p < cnt[i + 1]; ++p)
+            for (int b = 0; b < 8; ++b) {
+                const int c = elems[8 * n2e[p].e + b];
+                n2e[p].slot[b] = c < 0 ? -1
+                                       : rowptr[i] + (int)(std::lower_bound(nb.begin(), nb.end(), c) - nb.begin());
+            }
+        col.insert(col.end(), nb.begin(), nb.end());
+        rowptr[i + 1] = (int)col.size();
+    }
+    // roles (Subpb2Quadtree.m:165-172): the free DOFs are the nodes of the elements
+    // minus the first entry of their sorted list when no element lacks a midside node
+    // (the list is built assuming a 0 there), minus the Dirichlet nodes
+    std::vector<int> code(nn, 2);
+    bool absent = false;
+    int first = nn;
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 8; ++a) {
+            const int v = elems[8 * e + a];
+            if (v < 0) absent = true;
+            else { code[v] = 0; first = std::min(first, v); }
+        }
+    if (!absent && first < nn) code[first] = 2;
+    for (int k = 0; k < nd; ++k) code[dirichlet[k]] = 1;
+    m.nn = nn;
+    m.E = E;
+    m.nnz = (int)col.size();
+    m.structured = false;
+    CK(up(m.coords, coords, 2 * (size_t)nn, ctx->stream));
+    if (E > 0) CK(up(q.elems, elems, 8 * (size_t)E, ctx->stream));
+    else CK(q.elems.ensure(32));
+    CK(up(m.n2e_ptr, cnt.data(), cnt.size(), ctx->stream));
+    if (!n2e.empty()) CK(up(q.n2e, n2e.data(), n2e.size(), ctx->stream));
+    else CK(q.n2e.ensure(sizeof(aldic::N2E8)));
+    CK(up(m.rowptr, rowptr.data(), rowptr.size(), ctx->stream));
+    CK(up(m.col, col.data(), col.size(), ctx->stream));
+    CK(up(q.code, code.data(), code.size(), ctx->stream));
+    CK(m.aval.ensure(sizeof(double) * col.size()));
+    CK(m.mval.ensure(sizeof(double) * col.size()));
+    CK(q.cval.ensure(sizeof(double) * col.size()));
+    CK(m.diag.ensure(sizeof(double) * nn));
+    CK(m.gvec.ensure(sizeof(double) * 2 * nn));
+    CK(hipStreamSynchronize(ctx->stream));
+    m.beta = m.mu = m.alpha = -1;
+    ctx->have_quad = true;
+    return ALDIC_OK;
+}
+
+int aldic_subpb2_quadtree(aldic_ctx *ctx, double beta, double mu, const double *U, const double *F,
+                          double rtol, int maxit, double *Uhat, double *stats)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_quad) return fail(ctx, ALDIC_ERR_STATE, "subpb2_quadtree: no quadtree mesh");
+    if (!U || !F || !Uhat || maxit < 1 || !(rtol > 0))
+        return fail(ctx, ALDIC_ERR_ARG, "subpb2_quadtree: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    aldic::QuadDev &q = ctx->quad;
+    const size_t nn = q.m.nn;
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], U, 2 * nn, ctx->stream));
+    CK(up(sc[1], F, 4 * nn, ctx->stream));
+    CK(sc[4].ensure(sizeof(double) * 2 * nn));
+    CK(sc[5].ensure(sizeof(double) * 2 * nn));
+    if (q.m.beta != beta || q.m.mu != mu) CK(aldic::quad_assemble(q, beta, mu, ctx->stream));
+    CK(aldic::quad_rhs(q, beta, mu, sc[0].as<double>(), sc[1].as<double>(), sc[4].as<double>(),
+                       ctx->stream));
+    static thread_local DBuf work[8];
+    double st[2] = {0, 0};
+    CK(aldic::fem_pcg(q.m, sc[4].as<double>(), sc[5].as<double>(), rtol, maxit, work, st,
+                      ctx->stream, 8));
+    CK(aldic::quad_finish(q, sc[0].as<double>(), sc[5].as<double>(), ctx->stream));
     CK(down(Uhat, sc[5].p, 2 * nn, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     if (stats) { stats[0] = st[0]; stats[1] = st[1]; }

@@ -112,6 +112,24 @@ int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const dou
                  const double *F, const double *udual, const double *vdual, double rtol,
                  int maxit, double *Uhat, double *stats);
 
+/* The quadtree mesh of main_ALDIC_Quadtree.m (coordinatesFEM, elementsFEM, dirichlet of
+ * GenerateQuadtreeMesh.m): nn nodes (nn x 2), E elements (E x 8, 0-based: four corners
+ * counter-clockwise, then the midside nodes on the right, top, left and bottom edges,
+ * -1 where an edge has none), nd Dirichlet nodes (0-based).  Kept on the device for
+ * aldic_subpb2_quadtree. */
+int aldic_set_quadtree(aldic_ctx *ctx, const double *coords, int nn, const int *elems, int E,
+                       const int *dirichlet, int nd);
+
+/* Subpb2Quadtree.m:1-346 as main_ALDIC_Quadtree.m:181-212 calls it (alpha = 0, 2x2 Gauss
+ * points, duals multiplied by 0): solves (beta K + mu M) Uhat = beta G F + mu M U over
+ * the free nodes with FP64 Jacobi-PCG to relative residual rtol (maxit iterations at
+ * most); Uhat = U at the Dirichlet nodes and 0 at nodes outside every element and at the
+ * first (lowest) element node when every element has all four midside nodes (the
+ * reference's free-node list assumes a 0 entry there).  U: 2 nn, F: 4 nn
+ * ([F11 F21 F12 F22] per node); stats (nullable) = {iterations, relative residual}. */
+int aldic_subpb2_quadtree(aldic_ctx *ctx, double beta, double mu, const double *U,
+                          const double *F, double rtol, int maxit, double *Uhat, double *stats);
+
 /* funGlobal_NodalStrainAvg.m:1-147: nodal gradient by 2x2 Gauss points,
  * extrapolation and element averaging.  swap_fix = 0 writes the reference's slot
  * order [u_x u_y v_x v_y] (off-diagonals swapped relative to F's convention);

@@ -315,6 +315,100 @@ def subpb1_smooth(U, coords):
     return U
 
 
+def _q8_shape(ksi, eta, d):
+    """Shape functions and their (ksi, eta) derivatives of the quadtree element with
+    optional midside nodes 5..8 (Subpb2Quadtree.m:126-133 and its funDN* helpers):
+    N5..N8 = d_k/2 (1 +- ksi or eta)(1 - |other|) on the right, top, left, bottom edges,
+    corners bilinear minus half of their two neighbouring midside functions."""
+    d5, d6, d7, d8 = d
+    N5 = d5 * 0.5 * (1 + ksi) * (1 - abs(eta))
+    N6 = d6 * 0.5 * (1 + eta) * (1 - abs(ksi))
+    N7 = d7 * 0.5 * (1 - ksi) * (1 - abs(eta))
+    N8 = d8 * 0.5 * (1 - eta) * (1 - abs(ksi))
+    N = np.array([(1 - ksi) * (1 - eta) / 4 - 0.5 * (N7 + N8), (1 + ksi) * (1 - eta) / 4 - 0.5 * (N8 + N5),
+                  (1 + ksi) * (1 + eta) / 4 - 0.5 * (N5 + N6), (1 - ksi) * (1 + eta) / 4 - 0.5 * (N6 + N7),
+                  N5, N6, N7, N8])
+    se, sk = np.sign(-eta), np.sign(-ksi)
+    k5, e5 = d5 * 0.5 * (1 - abs(eta)), d5 * 0.5 * (1 + ksi) * se
+    k6, e6 = d6 * 0.5 * (1 + eta) * sk, d6 * 0.5 * (1 - abs(ksi))
+    k7, e7 = -d7 * 0.5 * (1 - abs(eta)), d7 * 0.5 * (1 - ksi) * se
+    k8, e8 = d8 * 0.5 * (1 - eta) * sk, -d8 * 0.5 * (1 - abs(ksi))
+    dk = np.array([-0.25 * (1 - eta) - 0.5 * (k7 + k8), 0.25 * (1 - eta) - 0.5 * (k8 + k5),
+                   0.25 * (1 + eta) - 0.5 * (k5 + k6), -0.25 * (1 + eta) - 0.5 * (k6 + k7),
+                   k5, k6, k7, k8])
+    de = np.array([-0.25 * (1 - ksi) - 0.5 * (e7 + e8), -0.25 * (1 + ksi) - 0.5 * (e8 + e5),
+                   0.25 * (1 + ksi) - 0.5 * (e5 + e6), 0.25 * (1 - ksi) - 0.5 * (e6 + e7),
+                   e5, e6, e7, e8])
+    return N, dk, de
+
+
+def subpb2_quadtree(coords, elems8, dirichlet, beta, mu, U, F):
+    """Subpb2Quadtree.m:1-346 with alpha = 0 and 2x2 Gauss points (as
+    main_ALDIC_Quadtree.m:181-212 calls it): elements (E x 8, 0-based, -1 = no midside
+    node); the duals are multiplied by 0 (:42-43); A = sum wJ (beta DN'DN + mu N'N),
+    b = sum wJ (beta diag(DN' F_node') + mu N'N U) with each DOF's own nodal F; U fixed
+    at the Dirichlet nodes; the free DOFs are the nodes of the elements minus the first
+    entry of their sorted list (:165-168, which assumes a 0 there, i.e. that some element
+    lacks a midside node) minus the Dirichlet nodes; every other node ends at 0."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    coords = np.asarray(coords, np.float64).reshape(-1, 2)
+    el = np.asarray(elems8, np.int64).reshape(-1, 8)
+    U = np.asarray(U, np.float64)
+    F = np.asarray(F, np.float64)
+    nn = coords.shape[0]
+    g = 1.0 / np.sqrt(3.0)
+    rows, cols, vals = [], [], []
+    b = np.zeros(2 * nn)
+    for e in range(el.shape[0]):
+        nd = el[e]
+        d = [1.0 if nd[k] >= 0 else 0.0 for k in range(4, 8)]
+        px = np.array([coords[n, 0] if n >= 0 else 0.0 for n in nd])
+        py = np.array([coords[n, 1] if n >= 0 else 0.0 for n in nd])
+        Ae = np.zeros((16, 16))
+        be = np.zeros(16)
+        idx = np.array([2 * n + c if n >= 0 else -1 for n in nd for c in (0, 1)])
+        for ksi in (-g, g):
+            for eta in (-g, g):
+                N, dk, de = _q8_shape(ksi, eta, d)
+                J = np.array([[dk @ px, dk @ py], [de @ px, de @ py]])
+                det = np.linalg.det(J)
+                Ji = np.linalg.inv(J)
+                dx = Ji[0, 0] * dk + Ji[0, 1] * de
+                dy = Ji[1, 0] * dk + Ji[1, 1] * de
+                Nm = np.zeros((2, 16)); Nm[0, 0::2] = N; Nm[1, 1::2] = N
+                DN = np.zeros((4, 16)); DN[0, 0::2] = dx; DN[1, 0::2] = dy; DN[2, 1::2] = dx; DN[3, 1::2] = dy
+                Ae += det * (beta * DN.T @ DN + mu * Nm.T @ Nm)
+                Ue = np.array([U[i] if i >= 0 else 0.0 for i in idx])
+                Fn = np.zeros((16, 4))   # [F11 F12 F21 F22] of each DOF's node
+                for k, n in enumerate(nd):
+                    if n >= 0:
+                        Fn[2 * k] = Fn[2 * k + 1] = [F[4 * n], F[4 * n + 2], F[4 * n + 1], F[4 * n + 3]]
+                be += det * (beta * np.einsum("rk,kr->k", DN, Fn) + mu * Nm.T @ Nm @ Ue)
+        ok = idx >= 0
+        ii, jj = np.meshgrid(idx[ok], idx[ok], indexing="ij")
+        rows.append(ii.ravel()); cols.append(jj.ravel()); vals.append(Ae[np.ix_(ok, ok)].ravel())
+        np.add.at(b, idx[ok], be[ok])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(2 * nn, 2 * nn))
+    involved = np.unique(el[:, :8])
+    has_absent = involved[0] < 0
+    involved = involved[involved >= 0]
+    if not has_absent:
+        involved = involved[1:]      # :166 drops the first sorted entry as if it were 0
+    dirichlet = np.asarray(dirichlet, np.int64).ravel()
+    free_nodes = np.setdiff1d(involved, dirichlet)
+    free = np.sort(np.concatenate([2 * free_nodes, 2 * free_nodes + 1]))
+    Uh = np.zeros(2 * nn)
+    if dirichlet.size:
+        dd = np.concatenate([2 * dirichlet, 2 * dirichlet + 1])
+        Uh[dd] = U[dd]
+    rhs = b - A @ Uh
+    if free.size:
+        Uh[free] = spl.spsolve(A[free][:, free].tocsc(), rhs[free])
+    return Uh
+
+
 def gauss_kernel(size=3, sigma=0.5):
     """fspecial('gaussian', size, sigma): exp(-(x^2+y^2)/(2 sigma^2)) over the centred
     size x size grid, entries below eps * max set to 0, normalised to sum 1."""

@@ -535,3 +535,52 @@ def test_gridfit_matches_oracle(gpu_ctx, orc, case):
     zo = orc.gridfit(X[ok], Y[ok], z[ok], xn, yn)
     assert np.max(np.abs(zg - zo)) <= 1e-9 * np.max(np.abs(zo)), (st, np.max(np.abs(zg - zo)))
     assert 1 <= st[0] < 4000 and st[1] <= 1e-12, st
+
+
+@pytest.mark.parametrize("case", ["hanging_dirichlet", "random_refine", "all_midside"])
+def test_subpb2_quadtree_matches_oracle(gpu_ctx, orc, case):
+    # 8-node quadtree elements (hanging midside nodes), Dirichlet nodes kept at U; in
+    # all_midside no element lacks a midside node, so the lowest element node is fixed
+    # at 0 as the reference's free-node list does
+    from meshes import quadtree_mesh
+    rng = np.random.default_rng(11)
+    if case == "hanging_dirichlet":
+        xy, el = quadtree_mesh()
+        bnd = np.flatnonzero((xy[:, 0] == xy[:, 0].min()) | (xy[:, 1] == xy[:, 1].max()))
+    elif case == "random_refine":
+        cells = [(x, y) for y in range(14) for x in range(14)]
+        pick = rng.choice(len(cells), size=60, replace=False)
+        xy, el = quadtree_mesh(nc=14, h=8.0, refine=[cells[k] for k in pick])
+        bnd = np.array([], np.int64)
+    else:
+        xy, el = quadtree_mesh(nc=3, refine=(), all_midside=True)
+        bnd = np.array([], np.int64)
+        assert np.all(el[:, 4:] >= 0)
+    _flush(f"subpb2_quadtree {case}: {len(xy)} nodes, {len(el)} elements, {bnd.size} dirichlet")
+    n = len(xy)
+    gpu_ctx.set_quadtree(xy, el, bnd)
+    U = rng.normal(size=2 * n)
+    F = rng.normal(scale=0.01, size=4 * n)
+    for beta in (1e-3, 0.1):
+        Uh, st = gpu_ctx.subpb2_quadtree(beta, 1e-3, U, F, rtol=1e-13)
+        Uo = orc.subpb2_quadtree(xy, el, bnd, beta, 1e-3, U, F)
+        assert np.max(np.abs(Uh - Uo)) <= 1e-9 * np.max(np.abs(Uo)), (beta, st)
+        if bnd.size:
+            assert np.array_equal(Uh.reshape(-1, 2)[bnd], U.reshape(-1, 2)[bnd])
+    if case == "all_midside":
+        first = el.min()
+        assert Uh[2 * first] == 0.0 and Uh[2 * first + 1] == 0.0
+
+
+def test_subpb2_quadtree_patch_test_gpu(gpu_ctx):
+    from meshes import quadtree_mesh
+    xy, el = quadtree_mesh(nc=6, h=12.0, refine=((1, 1), (2, 1), (4, 4), (4, 3), (0, 5)))
+    n = len(xy)
+    a = np.array([0.7, 0.004, -0.002, -1.1, 0.003, 0.005])
+    U = np.zeros(2 * n)
+    U[0::2] = a[0] + a[1] * xy[:, 0] + a[2] * xy[:, 1]
+    U[1::2] = a[3] + a[4] * xy[:, 0] + a[5] * xy[:, 1]
+    F = np.tile([a[1], a[4], a[2], a[5]], n)
+    gpu_ctx.set_quadtree(xy, el, [0, 1])
+    Uh, st = gpu_ctx.subpb2_quadtree(2.56e-3, 1e-3, U, F, rtol=1e-13)
+    assert np.allclose(Uh, U, atol=1e-8), st

@@ -6,6 +6,8 @@ here), so these checks are what anchors the oracle."""
 import numpy as np
 import pytest
 
+from meshes import quadtree_mesh
+
 
 # ---- a1 funNormalizeImg --------------------------------------------------------------
 
@@ -541,3 +543,41 @@ def test_smooth_nodal_keeps_constants_and_damps_noise(orc):
     N = rng.standard_normal(2 * S)
     Sm = orc.smooth_nodal(N, xy, 2)
     assert np.std(Sm) < 0.5 * np.std(N)
+
+
+# ---- a17: Subpb2Quadtree (8-node elements with hanging midside nodes) -----------------
+
+def test_subpb2_quadtree_without_hanging_nodes_equals_q4(orc):
+    # no midside nodes and no Dirichlet nodes: the Q4 system (2x2 Gauss integrates the
+    # bilinear stiffness and mass exactly on rectangles, as 3x3 does)
+    xy, el = quadtree_mesh(refine=())
+    rng = np.random.default_rng(8)
+    nn = xy.shape[0]
+    U = rng.standard_normal(2 * nn)
+    F = rng.standard_normal(4 * nn)
+    # every element lacks midside nodes, so no node is dropped as the ":166 zero"
+    Uq = orc.subpb2_quadtree(xy, el, [], 0.3, 1e-3, U, F)
+    Uo, _ = orc.subpb2(xy, el[:, :4].astype(np.int32), 0.3, 1e-3, U, F, np.zeros(4 * nn),
+                       np.zeros(2 * nn))
+    assert np.max(np.abs(Uq - Uo)) < 1e-8 * np.max(np.abs(Uo))
+
+
+def test_subpb2_quadtree_patch_test_with_hanging_nodes(orc):
+    # linear U with F = grad U is reproduced exactly, hanging nodes and Dirichlet nodes
+    # included (the midside functions vanish against linear fields at edge midpoints)
+    xy, el = quadtree_mesh()
+    assert np.any(el[:, 4:] >= 0)
+    nn = xy.shape[0]
+    a = np.array([0.7, 0.004, -0.002, -1.1, 0.003, 0.005])
+    U = np.zeros(2 * nn)
+    U[0::2] = a[0] + a[1] * xy[:, 0] + a[2] * xy[:, 1]
+    U[1::2] = a[3] + a[4] * xy[:, 0] + a[5] * xy[:, 1]
+    F = np.tile([a[1], a[4], a[2], a[5]], nn)   # [F11 F21 F12 F22]
+    dirichlet = np.where(np.isclose(xy[:, 0], xy[:, 0].min()))[0]
+    Uh = orc.subpb2_quadtree(xy, el, dirichlet, 0.2, 1e-3, U, F)
+    assert np.max(np.abs(Uh - U)) < 1e-9
+    # Dirichlet nodes keep U whatever the system says
+    U2 = U.copy()
+    U2[2 * dirichlet] += 5.0
+    Uh2 = orc.subpb2_quadtree(xy, el, dirichlet, 0.2, 1e-3, U2, F)
+    assert np.array_equal(Uh2[2 * dirichlet], U2[2 * dirichlet])
