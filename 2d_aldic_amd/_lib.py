@@ -24,7 +24,7 @@ EXPORTS = (
     "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
-    "aldic_subpb2_quadtree",
+    "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree",
 )
 
 
@@ -76,6 +76,7 @@ def load():
     L.aldic_frame_scalars.argtypes = [P, vp, ip]
     L.aldic_set_quadtree.argtypes = [P, vp, ip, vp, ip, vp, ip]
     L.aldic_subpb2_quadtree.argtypes = [P, dp, dp, vp, vp, dp, ip, vp, vp]
+    L.aldic_nodal_strain_quadtree.argtypes = [P, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -104,6 +105,8 @@ class Context:
         self.h = h
         self.nx = self.ny = 0
         self.n_nodes = 0
+        self.n_quad_nodes = 0
+        self.n_quad_elems = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -230,6 +233,7 @@ class Context:
                                            elems8.shape[0], _p(d) if d.size else None, d.size),
                  "aldic_set_quadtree")
         self.n_quad_nodes = coords.shape[0]
+        self.n_quad_elems = elems8.shape[0]
 
     def subpb2_quadtree(self, beta, mu, U, F, rtol=1e-12, maxit=5000):
         n = self.n_quad_nodes
@@ -240,6 +244,23 @@ class Context:
                                               float(rtol), int(maxit), _p(Uh), _p(st)),
                  "aldic_subpb2_quadtree")
         return Uh, st
+
+    def nodal_strain_quadtree(self, U, tri=None, w=None):
+        """Gauss-point strains of the quadtree mesh (4E x [u_x u_y v_x v_y]) and their
+        coordinates (4E x 2); with interpolation weights (tri, w: nn x 3) also the nodal
+        values (4 nn, interleaved per node), else None."""
+        n, ngp = self.n_quad_nodes, 4 * self.n_quad_elems
+        Fgp = np.empty((ngp, 4))
+        xy = np.empty((ngp, 2))
+        F = None
+        if tri is not None:
+            tri = _c(tri, np.int32).reshape(n, 3)
+            w = _c(w, np.float64).reshape(n, 3)
+            F = np.empty(4 * n)
+        self._ck(self.L.aldic_nodal_strain_quadtree(self.h, _p(_c(U, np.float64)), _p(tri), _p(w),
+                                                    _p(F), _p(Fgp), _p(xy)),
+                 "aldic_nodal_strain_quadtree")
+        return F, Fgp, xy
 
     def nodal_strain(self, U, swap_fix=False):
         F = np.empty(4 * self.n_nodes)

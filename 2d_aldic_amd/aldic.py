@@ -333,15 +333,16 @@ def funICGN(U0, x0, y0, Df, ImgRef, ImgDef, winsize, tol, ICGNmethod="GaussNewto
 
 
 def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta, HPar,
-           ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, smoothing="off"):
+           ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, smoothing="gridfit"):
     """Subpb1.m:1-199 -> [U, HPar, ALSub1Time, ConvItPerEle, LocalICGNBadPtNum].
 
     HPar may be the reference's cell(21) (list of 21 node vectors) or an (S, 21) array.
-    With ALSolveStep == 1 the reference passes a zero Hessian (:69-77).  The gridfit
-    smoothing of every node (:186-199) is not applied (smoothing='off'); failed nodes
-    are refilled by axis_fill instead."""
-    if smoothing != "off":
-        raise NotImplementedError("gridfit smoothing of Subpb1 is not provided")
+    With ALSolveStep == 1 the reference passes a zero Hessian (:69-77).  Failed nodes
+    (status < 0 or > 99) become NaN and u, v are then refitted at every node by gridfit
+    with the springs regulariser over the node grid (:165-199, device CG); with
+    smoothing='off' the failed nodes are refilled by axis_fill instead."""
+    if smoothing not in ("gridfit", "off"):
+        raise ValueError("smoothing must be 'gridfit' or 'off'")
     ctx = _ensure_images(ImgRef, ImgDef, Df)
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
     S = coords.shape[0]
@@ -357,7 +358,18 @@ def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, bet
     U = U.copy()
     U[0::2][bad] = np.nan
     U[1::2][bad] = np.nan
-    U, _ = _refill(coords, U)
+    if smoothing == "off":
+        U, _ = _refill(coords, U)
+    else:
+        xs = np.unique(coords[:, 0])
+        ys = np.unique(coords[:, 1])
+        ix = np.searchsorted(xs, coords[:, 0])
+        iy = np.searchsorted(ys, coords[:, 1])
+        for c in range(2):
+            z = np.full((ys.size, xs.size), np.nan)
+            z[iy, ix] = U[c::2]
+            zg, _ = ctx.gridfit(xs, ys, z)
+            U[c::2] = zg[iy, ix]
     return U, HPar, ALSub1Time, it.astype(np.int64), int(bad.sum())
 
 

@@ -249,6 +249,12 @@ hipError_t quad_assemble(QuadDev &q, double beta, double mu, hipStream_t st);
 hipError_t quad_rhs(QuadDev &q, double beta, double mu, const double *U, const double *F,
                     double *b, hipStream_t st);
 hipError_t quad_finish(QuadDev &q, const double *U, double *x, hipStream_t st);
+// Gauss-point strains of the quadtree mesh (midside nodes dropped): Fgp 4E x [u_x u_y v_x
+// v_y], gpxy (nullable) 4E x 2; and node values by given linear-interpolation weights.
+hipError_t quad_gp_strain(const QuadDev &q, const double *U, double *Fgp, double *gpxy,
+                          hipStream_t st);
+hipError_t quad_gp_gather(int nn, const int *tri, const double *w, const double *Fgp, double *F,
+                          hipStream_t st);
 
 // ADMM helpers
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);

@@ -798,6 +798,40 @@ int aldic_subpb2_quadtree(aldic_ctx *ctx, double beta, double mu, const double *
     return ALDIC_OK;
 }
 
+int aldic_nodal_strain_quadtree(aldic_ctx *ctx, const double *U, const int *tri, const double *w,
+                                double *F, double *Fgp, double *gpxy)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_quad) return fail(ctx, ALDIC_ERR_STATE, "nodal_strain_quadtree: no quadtree mesh");
+    if (!U || (!tri) != (!w) || (tri && !F) || (!F && !Fgp && !gpxy))
+        return fail(ctx, ALDIC_ERR_ARG, "nodal_strain_quadtree: bad arguments");
+    aldic::QuadDev &q = ctx->quad;
+    const size_t nn = q.m.nn, ngp = 4 * (size_t)q.m.E;
+    if (tri)
+        for (size_t k = 0; k < 3 * nn; ++k)
+            if (tri[k] < 0 || (size_t)tri[k] >= ngp)
+                return fail(ctx, ALDIC_ERR_ARG, "nodal_strain_quadtree: Gauss point index");
+    CK(hipSetDevice(ctx->device));
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], U, 2 * nn, ctx->stream));
+    CK(sc[1].ensure(sizeof(double) * 4 * (ngp ? ngp : 1)));
+    CK(sc[2].ensure(sizeof(double) * 2 * (ngp ? ngp : 1)));
+    CK(aldic::quad_gp_strain(q, sc[0].as<double>(), sc[1].as<double>(), sc[2].as<double>(),
+                             ctx->stream));
+    if (tri) {
+        CK(up(sc[3], tri, 3 * nn, ctx->stream));
+        CK(up(sc[4], w, 3 * nn, ctx->stream));
+        CK(sc[5].ensure(sizeof(double) * 4 * nn));
+        CK(aldic::quad_gp_gather((int)nn, sc[3].as<int>(), sc[4].as<double>(), sc[1].as<double>(),
+                                 sc[5].as<double>(), ctx->stream));
+        CK(down(F, sc[5].p, 4 * nn, ctx->stream));
+    }
+    if (Fgp && ngp) CK(down(Fgp, sc[1].p, 4 * ngp, ctx->stream));
+    if (gpxy && ngp) CK(down(gpxy, sc[2].p, 2 * ngp, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
 int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
 {
     if (!ctx) return ALDIC_ERR_ARG;

@@ -162,6 +162,116 @@ __global__ __launch_bounds__(256) void k_fix(int nn, const int *code, const doub
     else if (c != 0) x[i] = make_double2(0.0, 0.0);
 }
 
+// ---- nodal strain (funGlobalNodalStrainQuadtree.m:31-255) ---------------------------
+// The midside nodes are dropped (:33), so each element is a bilinear Q4 over its corners.
+// Gauss point (i, j) of element e sits at x_i = g_i (x3 - x1)/2 + (x3 + x1)/2,
+// y_j likewise (corners 1 and 3, :116-121), stored at 4 e + 2 i + j (:201-205); its
+// (ksi, eta) come from the element's map ksi = l1 x y + l2 x + l3 y + l4 (eta with m),
+// l = [x y, x, y, 1]^-1 [-1 1 1 -1]' over the corners (:102-107).  The strain there is
+// [u_x, u_y, v_x, v_y] (:184-197).
+
+// 4x4 solve with partial pivoting, two right-hand sides (overwritten with the solutions)
+__device__ __forceinline__ void solve4(double A[4][4], double b0[4], double b1[4])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int p = k;
+        double best = fabs(A[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < 4; ++r)
+            if (fabs(A[r][k]) > best) { best = fabs(A[r][k]); p = r; }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (r == p && p != k) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) { const double t = A[k][c]; A[k][c] = A[r][c]; A[r][c] = t; }
+                double t = b0[k]; b0[k] = b0[r]; b0[r] = t;
+                t = b1[k]; b1[k] = b1[r]; b1[r] = t;
+            }
+#pragma unroll
+        for (int r = k + 1; r < 4; ++r) {
+            const double f = A[r][k] / A[k][k];
+#pragma unroll
+            for (int c = k; c < 4; ++c) A[r][c] -= f * A[k][c];
+            b0[r] -= f * b0[k];
+            b1[r] -= f * b1[k];
+        }
+    }
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+#pragma unroll
+        for (int c = k + 1; c < 4; ++c) { b0[k] -= A[k][c] * b0[c]; b1[k] -= A[k][c] * b1[c]; }
+        b0[k] /= A[k][k];
+        b1[k] /= A[k][k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gp_strain(int E, const double2 *coords, const int *elems,
+                                                   const double2 *U, double4 *Fgp, double2 *gpxy)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int *nd = elems + 8 * e;
+    double2 p[4];
+    double2 u[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) { p[a] = coords[nd[a]]; u[a] = U[nd[a]]; }
+    double A[4][4], l[4] = {-1, 1, 1, -1}, m[4] = {-1, -1, 1, 1};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        A[a][0] = p[a].x * p[a].y; A[a][1] = p[a].x; A[a][2] = p[a].y; A[a][3] = 1.0;
+    }
+    solve4(A, l, m);
+    const double g = 0.57735026918962576;  // 1 / sqrt(3)
+    const double hx = 0.5 * (p[2].x - p[0].x), cx = 0.5 * (p[2].x + p[0].x);
+    const double hy = 0.5 * (p[2].y - p[0].y), cy = 0.5 * (p[2].y + p[0].y);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const double x = (i ? g : -g) * hx + cx, y = (j ? g : -g) * hy + cy;
+            const double ksi = l[0] * x * y + l[1] * x + l[2] * y + l[3];
+            const double eta = m[0] * x * y + m[1] * x + m[2] * y + m[3];
+            const double dk[4] = {-0.25 * (1 - eta), 0.25 * (1 - eta), 0.25 * (1 + eta), -0.25 * (1 + eta)};
+            const double de[4] = {-0.25 * (1 - ksi), -0.25 * (1 + ksi), 0.25 * (1 + ksi), 0.25 * (1 - ksi)};
+            double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                J11 += dk[a] * p[a].x; J12 += dk[a] * p[a].y;
+                J21 += de[a] * p[a].x; J22 += de[a] * p[a].y;
+            }
+            const double det = J11 * J22 - J12 * J21;
+            double ux = 0, uy = 0, vx = 0, vy = 0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double dx = (J22 * dk[a] - J12 * de[a]) / det;
+                const double dy = (-J21 * dk[a] + J11 * de[a]) / det;
+                ux += dx * u[a].x; uy += dy * u[a].x;
+                vx += dx * u[a].y; vy += dy * u[a].y;
+            }
+            const int q = 4 * e + 2 * i + j;
+            Fgp[q] = make_double4(ux, uy, vx, vy);
+            if (gpxy) gpxy[q] = make_double2(x, y);
+        }
+}
+
+// F at node i = sum_k w[3i + k] Fgp[tri[3i + k]] (linear interpolation weights of the
+// Gauss points, given by the caller)
+__global__ __launch_bounds__(256) void k_gp_gather(int nn, const int *tri, const double *w,
+                                                   const double4 *Fgp, double4 *F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double wk = w[3 * i + k];
+        const double4 f = Fgp[tri[3 * i + k]];
+        s.x += wk * f.x; s.y += wk * f.y; s.z += wk * f.z; s.w += wk * f.w;
+    }
+    F[i] = s;
+}
+
 }  // namespace
 
 hipError_t quad_assemble(QuadDev &q, double beta, double mu, hipStream_t st)
@@ -191,6 +301,26 @@ hipError_t quad_rhs(QuadDev &q, double beta, double mu, const double *U, const d
                        q.m.rowptr.as<int>(), q.m.col.as<int>(), q.code.as<int>(),
                        q.cval.as<double>(), reinterpret_cast<const double2 *>(U),
                        reinterpret_cast<double2 *>(b));
+    return hipGetLastError();
+}
+
+hipError_t quad_gp_strain(const QuadDev &q, const double *U, double *Fgp, double *gpxy,
+                          hipStream_t st)
+{
+    const int E = q.m.E;
+    if (E <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gp_strain, dim3((E + 255) / 256), dim3(256), 0, st, E,
+                       q.m.coords.as<double2>(), q.elems.as<int>(),
+                       reinterpret_cast<const double2 *>(U), reinterpret_cast<double4 *>(Fgp),
+                       reinterpret_cast<double2 *>(gpxy));
+    return hipGetLastError();
+}
+
+hipError_t quad_gp_gather(int nn, const int *tri, const double *w, const double *Fgp, double *F,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gp_gather, dim3((nn + 255) / 256), dim3(256), 0, st, nn, tri, w,
+                       reinterpret_cast<const double4 *>(Fgp), reinterpret_cast<double4 *>(F));
     return hipGetLastError();
 }
 

@@ -130,6 +130,18 @@ int aldic_set_quadtree(aldic_ctx *ctx, const double *coords, int nn, const int *
 int aldic_subpb2_quadtree(aldic_ctx *ctx, double beta, double mu, const double *U,
                           const double *F, double rtol, int maxit, double *Uhat, double *stats);
 
+/* funGlobalNodalStrainQuadtree.m:31-255 (GaussPtOrder 2, smoothness 0) on the quadtree
+ * mesh of aldic_set_quadtree: the midside nodes are dropped (:33); per element the 2x2
+ * Gauss-point strains [u_x u_y v_x v_y] (:184-197, the reference's slot order) at
+ * x_i = g_i (x3 - x1)/2 + (x3 + x1)/2, y_j likewise, row 4 e + 2 i + j (:116-121,
+ * :201-205) -> Fgp (4E x 4) and gpxy (4E x 2), both nullable.  The reference then
+ * evaluates scatteredInterpolant(gpxy, Fgp(:,k), 'linear') at the nodes (:246-255); here
+ * the caller passes that interpolation as weights: node i gets
+ * sum_k w[3i+k] Fgp[tri[3i+k]] -> F (4 nn, [F11 F21 F12 F22] slots holding
+ * [u_x u_y v_x v_y] as the reference stores them).  tri / w / F nullable together. */
+int aldic_nodal_strain_quadtree(aldic_ctx *ctx, const double *U, const int *tri, const double *w,
+                                double *F, double *Fgp, double *gpxy);
+
 /* funGlobal_NodalStrainAvg.m:1-147: nodal gradient by 2x2 Gauss points,
  * extrapolation and element averaging.  swap_fix = 0 writes the reference's slot
  * order [u_x u_y v_x v_y] (off-diagonals swapped relative to F's convention);

@@ -409,6 +409,52 @@ def subpb2_quadtree(coords, elems8, dirichlet, beta, mu, U, F):
     return Uh
 
 
+def gauss_strain_quadtree(coords, elems8, U):
+    """funGlobalNodalStrainQuadtree.m:31-205 with GaussPtOrder 2: the midside nodes are
+    dropped (:33, elementsFEM(:,5:8) = 0), so every element is the bilinear Q4 of its
+    corners.  Gauss points x_i = g_i (x3 - x1)/2 + (x3 + x1)/2 (and y_j) from corners 1
+    and 3 (:116-121), row 4 e + 2 i + j (:201-205); (ksi, eta) from the element's map
+    ksi = l . [x y, x, y, 1], l = linsolve([x y, x, y, 1] of the corners, [-1 1 1 -1]')
+    (eta with [-1 -1 1 1]', :102-107, :123-124); strain DN U = [u_x u_y v_x v_y]
+    (:150-197).  Returns (Fgp (4E, 4), gpxy (4E, 2))."""
+    coords = np.asarray(coords, np.float64).reshape(-1, 2)
+    el = np.asarray(elems8, np.int64).reshape(-1, 8)[:, :4]
+    U = np.asarray(U, np.float64).reshape(-1, 2)
+    E = el.shape[0]
+    gq = np.array([-1.0, 1.0]) / np.sqrt(3.0)
+    Fgp = np.zeros((4 * E, 4))
+    gxy = np.zeros((4 * E, 2))
+    for e in range(E):
+        p = coords[el[e]]
+        u = U[el[e]]
+        L = np.column_stack([p[:, 0] * p[:, 1], p[:, 0], p[:, 1], np.ones(4)])
+        lv = np.linalg.solve(L, [-1.0, 1.0, 1.0, -1.0])
+        mv = np.linalg.solve(L, [-1.0, -1.0, 1.0, 1.0])
+        xs = gq * 0.5 * (p[2, 0] - p[0, 0]) + 0.5 * (p[2, 0] + p[0, 0])
+        ys = gq * 0.5 * (p[2, 1] - p[0, 1]) + 0.5 * (p[2, 1] + p[0, 1])
+        for i in range(2):
+            for j in range(2):
+                x, y = xs[i], ys[j]
+                ksi = lv @ [x * y, x, y, 1.0]
+                eta = mv @ [x * y, x, y, 1.0]
+                dk = 0.25 * np.array([-(1 - eta), 1 - eta, 1 + eta, -(1 + eta)])
+                de = 0.25 * np.array([-(1 - ksi), -(1 + ksi), 1 + ksi, 1 - ksi])
+                J = np.array([[dk @ p[:, 0], dk @ p[:, 1]], [de @ p[:, 0], de @ p[:, 1]]])
+                D = np.linalg.inv(J) @ np.vstack([dk, de])     # rows d/dx, d/dy
+                q = 4 * e + 2 * i + j
+                Fgp[q] = [D[0] @ u[:, 0], D[1] @ u[:, 0], D[0] @ u[:, 1], D[1] @ u[:, 1]]
+                gxy[q] = (x, y)
+    return Fgp, gxy
+
+
+def gather_weights(Fgp, tri, w):
+    """Node values sum_k w[i, k] Fgp[tri[i, k]] (the linear interpolation of :246-255 in
+    weight form), interleaved per node."""
+    tri = np.asarray(tri, np.int64).reshape(-1, 3)
+    w = np.asarray(w, np.float64).reshape(-1, 3)
+    return np.einsum("nk,nkc->nc", w, np.asarray(Fgp)[tri]).ravel()
+
+
 def gauss_kernel(size=3, sigma=0.5):
     """fspecial('gaussian', size, sigma): exp(-(x^2+y^2)/(2 sigma^2)) over the centred
     size x size grid, entries below eps * max set to 0, normalised to sum 1."""

@@ -584,3 +584,108 @@ def test_subpb2_quadtree_patch_test_gpu(gpu_ctx):
     gpu_ctx.set_quadtree(xy, el, [0, 1])
     Uh, st = gpu_ctx.subpb2_quadtree(2.56e-3, 1e-3, U, F, rtol=1e-13)
     assert np.allclose(Uh, U, atol=1e-8), st
+
+
+def test_nodal_strain_quadtree_matches_oracle(gpu_ctx, orc):
+    # Gauss-point strains (midside nodes dropped) and their coordinates, then the node
+    # values by the same interpolation weights; FP64 on both sides
+    import importlib
+    from meshes import quadtree_mesh
+    qt = importlib.import_module("2d_aldic_amd.quadtree")
+    rng = np.random.default_rng(12)
+    cells = [(x, y) for y in range(10) for x in range(10)]
+    pick = rng.choice(len(cells), size=30, replace=False)
+    xy, el = quadtree_mesh(nc=10, h=8.0, refine=[cells[k] for k in pick])
+    _flush(f"nodal_strain_quadtree: {len(xy)} nodes, {len(el)} elements")
+    gpu_ctx.set_quadtree(xy, el, [])
+    U = rng.normal(size=2 * len(xy))
+    F0, Fgp, gxy = gpu_ctx.nodal_strain_quadtree(U)
+    assert F0 is None
+    Fo, go = orc.gauss_strain_quadtree(xy, el, U)
+    assert np.max(np.abs(gxy - go)) <= 1e-12 * np.abs(go).max()
+    assert np.max(np.abs(Fgp - Fo)) <= 1e-10 * np.abs(Fo).max()
+    tri, w = qt.linear_weights(gxy, xy)
+    F, Fgp2, _ = gpu_ctx.nodal_strain_quadtree(U, tri, w)
+    assert np.array_equal(Fgp2, Fgp)
+    assert np.max(np.abs(F - orc.gather_weights(Fo, tri, w))) <= 1e-10 * np.abs(Fo).max()
+
+
+def test_quadtree_mirror_one_based_mesh(gpu_ctx, orc):
+    # the MATLAB-shaped mirror: 1-based elementsFEM with 0 = no midside node, 1-based
+    # Dirichlet list
+    import importlib
+    from meshes import quadtree_mesh
+    qt = importlib.import_module("2d_aldic_amd.quadtree")
+    xy, el = quadtree_mesh()
+    d0 = np.flatnonzero(xy[:, 1] == xy[:, 1].min())
+    mesh = dict(coordinatesFEM=xy, elementsFEM=el + 1, dirichlet=d0 + 1)
+    rng = np.random.default_rng(13)
+    n = len(xy)
+    U = rng.normal(size=2 * n)
+    F = rng.normal(scale=0.01, size=4 * n)
+    Uh = qt.Subpb2Quadtree(mesh, 2, 0.05, 1e-3, U, F, 0 * F, 0 * U, 0)
+    Uo = orc.subpb2_quadtree(xy, el, d0, 0.05, 1e-3, U, F)
+    assert np.max(np.abs(Uh - Uo)) <= 1e-9 * np.abs(Uo).max()
+    Fn, Fgp, gxy = qt.funGlobalNodalStrainQuadtree(mesh, Uh, 2, 0)
+    Fo, _ = orc.gauss_strain_quadtree(xy, el, Uh)
+    tri, w = qt.linear_weights(gxy, xy)
+    assert np.max(np.abs(Fn - orc.gather_weights(Fo, tri, w))) <= 1e-10 * np.abs(Fo).max()
+
+
+def test_mirror_chain_config1(cfg1, orc):
+    """main_ALDIC.m's first frame through the MATLAB-shaped mirror (images (nx, ny),
+    1-based elementsFEM): funNormalizeImg, funImgGradient, IntegerSearch, MeshSetUp,
+    Init, LocalICGN (bad-point rule + refill), Subpb2, funGlobal_NodalStrainAvg and Subpb1
+    (gridfit refit), each against the oracle on the mirror's own normalised images."""
+    import importlib
+    al = importlib.import_module("2d_aldic_amd.aldic")
+    _flush("mirror chain config1")
+    Img = [cfg1.f.T.astype(np.float64), cfg1.g.T.astype(np.float64)]
+    ImgN, roi = al.funNormalizeImg(Img, dict(gridx=[1, 256], gridy=[1, 256]))
+    fo, _ = orc.normalize(cfg1.f.astype(np.float64))
+    assert np.max(np.abs(ImgN[0].T - fo)) < 2e-6 * np.ptp(fo)
+    fN, gN = ImgN
+    f = np.ascontiguousarray(fN.T, np.float64)
+    g = np.ascontiguousarray(gN.T, np.float64)
+    gx, gy = orc.gradient(f)
+    Df = al.funImgGradient(fN, gN)
+    assert np.max(np.abs(Df["DfDx"] - gx.T[3:-3, 3:-3])) < 2e-6 * np.ptp(gx)
+    assert np.max(np.abs(Df["DfDy"] - gy.T[3:-3, 3:-3])) < 2e-6 * np.ptp(gy)
+    DICpara = dict(winsize=20, winstepsize=10, SizeOfFFTSearchRegion=8, gridxyROIRange=roi,
+                   ImgSize=(256, 256))
+    DICpara, x0, y0, u, v, cc = al.IntegerSearch(fN, gN, None, DICpara)
+    o = orc.search(f, g, [1, 256, 1, 256], 20, 10, [8, 8])
+    assert np.array_equal(x0, o["x"]) and np.array_equal(y0, o["y"])
+    assert np.max(np.abs(u - o["u"])) <= 1e-3 + 1e-9 and np.max(np.abs(v - o["v"])) <= 1e-3 + 1e-9
+    mesh = al.MeshSetUp(x0, y0, DICpara)
+    coords = mesh["coordinatesFEM"]
+    U0 = al.Init(u, v, cc["max"], mesh["x0"], mesh["y0"], 0)
+    assert np.array_equal(U0.reshape(-1, 2), np.stack([u.ravel(), v.ravel()], 1))
+    U, F, H, _, it, nbad = al.LocalICGN(U0, coords, Df, fN, gN, DICpara, "GaussNewton", 1e-2)
+    Uo, Fo, Ho, ito, _ = orc.icgn6(f, g, gx, gy, coords, U0, 20)
+    assert np.array_equal(it, ito)
+    bad = orc.badpoint(ito, True)
+    ok = ~bad
+    assert np.max(np.abs(U.reshape(-1, 2)[ok] - Uo.reshape(-1, 2)[ok])) <= 1e-3
+    if bad.any():
+        Ub = Uo.copy()
+        Ub[np.repeat(bad, 2)] = np.nan
+        Uf = orc.axis_fill(np.where(np.isnan(Ub), 0.0, Ub), 2, bad, 19, 19)
+        assert np.max(np.abs(U - Uf)) <= 2e-3
+    zeros4, zeros2 = np.zeros(4 * len(coords)), np.zeros(2 * len(coords))
+    U2 = al.Subpb2(mesh, 2.56e-3, 1e-3, U, F, zeros4, zeros2)
+    U2o, _ = orc.subpb2(coords, mesh["elementsFEM"] - 1, 2.56e-3, 1e-3, U, F, zeros4, zeros2)
+    assert np.max(np.abs(U2 - U2o)) <= 1e-9 * np.abs(U2o).max()
+    F2 = al.funGlobal_NodalStrainAvg(coords, mesh["elementsFEM"], U2)
+    F2o = orc.nodal_strain(coords, mesh["elementsFEM"] - 1, U2)
+    assert np.max(np.abs(F2 - F2o)) < 1e-12 * max(1.0, np.abs(F2o).max())
+    udual, vdual = F2 - F, U2 - U
+    U1, _, _, it1, _ = al.Subpb1(U2, F2, udual, vdual, coords, Df, fN, gN, 1e-3, 2.56e-3,
+                                 np.zeros((len(coords), 21)), 1, DICpara)
+    U1o, it1o = orc.icgn2(f, g, gx, gy, coords, 20, np.zeros((len(coords), 21)), 1e-3, udual,
+                          vdual, U2, F2)
+    assert np.array_equal(it1, it1o)
+    U1o = U1o.copy()
+    U1o[np.repeat(orc.badpoint(it1o, False), 2)] = np.nan
+    U1s = orc.subpb1_smooth(U1o, coords)
+    assert np.max(np.abs(U1 - U1s)) <= 1e-3

@@ -581,3 +581,53 @@ def test_subpb2_quadtree_patch_test_with_hanging_nodes(orc):
     U2[2 * dirichlet] += 5.0
     Uh2 = orc.subpb2_quadtree(xy, el, dirichlet, 0.2, 1e-3, U2, F)
     assert np.array_equal(Uh2[2 * dirichlet], U2[2 * dirichlet])
+
+
+def test_gauss_strain_quadtree_bilinear_field(orc):
+    # U bilinear in x, y is reproduced exactly by every Q4 element, so the Gauss-point
+    # strain is the exact gradient at the Gauss point; the Gauss points sit at the
+    # element centre +- h / (2 sqrt 3), x outer, y inner; midside nodes are ignored
+    xy, el = quadtree_mesh()
+    a = np.array([0.3, 0.01, -0.02, 1e-4, -1.2, 0.004, 0.003, -2e-4])
+    X, Y = xy[:, 0], xy[:, 1]
+    U = np.column_stack([a[0] + a[1] * X + a[2] * Y + a[3] * X * Y,
+                         a[4] + a[5] * X + a[6] * Y + a[7] * X * Y]).ravel()
+    Fgp, gxy = orc.gauss_strain_quadtree(xy, el, U)
+    gx, gy = gxy[:, 0], gxy[:, 1]
+    exact = np.column_stack([a[1] + a[3] * gy, a[2] + a[3] * gx, a[5] + a[7] * gy, a[6] + a[7] * gx])
+    assert np.max(np.abs(Fgp - exact)) < 1e-12
+    c = xy[el[:, :4]].mean(axis=1)
+    h = xy[el[:, 2], 0] - xy[el[:, 0], 0]
+    d = h / (2.0 * np.sqrt(3.0))
+    want = np.stack([np.column_stack([c[:, 0] + sx * d, c[:, 1] + sy * d])
+                     for sx in (-1, 1) for sy in (-1, 1)], axis=1).reshape(-1, 2)
+    assert np.max(np.abs(gxy - want)) < 1e-12
+
+
+def test_linear_weights_reproduce_linear_fields(pkg):
+    # interpolation inside the hull equals scipy's LinearNDInterpolator on the same
+    # triangulation; outside it extrapolates a plane, so any linear field is exact at
+    # every query; weights sum to 1
+    import importlib
+    from scipy.interpolate import LinearNDInterpolator
+    qt = importlib.import_module("2d_aldic_amd.quadtree")
+    rng = np.random.default_rng(5)
+    P = rng.uniform(0, 100, size=(300, 2))
+    Q = rng.uniform(-20, 120, size=(400, 2))
+    tri, w = qt.linear_weights(P, Q)
+    assert np.allclose(w.sum(axis=1), 1.0, atol=1e-12)
+    f = 0.3 + 0.02 * P[:, 0] - 0.05 * P[:, 1]
+    fq = np.einsum("nk,nk->n", w, f[tri])
+    assert np.max(np.abs(fq - (0.3 + 0.02 * Q[:, 0] - 0.05 * Q[:, 1]))) < 1e-10
+    g = np.sin(P[:, 0] / 7.0) * P[:, 1]
+    ref = LinearNDInterpolator(P, g)(Q)
+    inside = ~np.isnan(ref)
+    assert inside.sum() > 100 and (~inside).sum() > 50
+    gq = np.einsum("nk,nk->n", w, g[tri])
+    assert np.max(np.abs(gq[inside] - ref[inside])) < 1e-9
+    # scattered_fill: bad nodes of a linear field come back exactly
+    bad = rng.random(300) < 0.2
+    V = np.column_stack([f, 2 * f]).ravel()
+    Vb = V.copy()
+    Vb[np.repeat(bad, 2)] = np.nan
+    assert np.allclose(qt.scattered_fill(P, Vb, 2, bad), V, atol=1e-10)
