@@ -17,6 +17,8 @@
 // convergence test, the 6x6 solve and the inverse-compositional update run in FP64
 // on every lane with bitwise-identical inputs (butterfly all-reduce), so the loop
 // is uniform across the wave.
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 
 #include "aldic_internal.h"
@@ -196,28 +198,59 @@ __device__ __forceinline__ bool ic_compose(double P[6], const double D[6])
     return true;
 }
 
-// Keys-cubic sample of g at (X + ax, Y + ay), X, Y 1-based integers, the sample known
-// to lie in [3, nx-2] x [3, ny-2] so every tap is inside the image.
-__device__ __forceinline__ float sample_g(const float *__restrict__ g, int nx, int X, int Y,
-                                          float ax, float ay)
+// Keys-cubic sample of g at window pixel `off` (0-based image index of the pixel at
+// the integer position X, Y) displaced by (ax, ay); the sample is known to lie in
+// [3, nx-2] x [3, ny-2] (1-based), so every tap is inside the image.  rows[b] is the
+// uniform byte address of g(-1 + b*nx - nx) -- the tap rows' bases -- so that every
+// tap row is one load at the same 32-bit per-lane byte offset.  The weights are kept
+// doubled (2 w, exact in binary) and the result scaled by 1/4 at the end.
+__device__ __forceinline__ float sample_g(const char *const (&rows)[4], int nx, int off, float ax,
+                                          float ay)
 {
-    const float fx = floorf(ax), fy = floorf(ay);
-    float wx0, wx1, wx2, wx3, wy0, wy1, wy2, wy3;
-    keys_weights(ax - fx, wx0, wx1, wx2, wx3);
-    keys_weights(ay - fy, wy0, wy1, wy2, wy3);
-    const int c0 = X + (int)fx - 2, r0 = Y + (int)fy - 2;  // 0-based first tap
-    // 32-bit byte offsets from the uniform image base (images hold < 2^29 pixels,
-    // rows < 2^24: the 24-bit multiply is exact)
-    const char *gb = reinterpret_cast<const char *>(g);
-    const unsigned rs = 4u * (unsigned)nx;
-    const unsigned o = __umul24((unsigned)r0, rs) + ((unsigned)c0 << 2);
+    const float tx = __builtin_amdgcn_fractf(ax), ty = __builtin_amdgcn_fractf(ay);
+    const int ix = (int)floorf(ax), iy = (int)floorf(ay);
+    const float tx2 = tx * tx, tx3 = tx2 * tx, ty2 = ty * ty, ty3 = ty2 * ty;
+    const float wx0 = -tx + 2.f * tx2 - tx3, wx1 = 2.f - 5.f * tx2 + 3.f * tx3;
+    const float wx2 = tx + 4.f * tx2 - 3.f * tx3, wx3 = tx3 - tx2;
+    const float wy0 = -ty + 2.f * ty2 - ty3, wy1 = 2.f - 5.f * ty2 + 3.f * ty3;
+    const float wy2 = ty + 4.f * ty2 - 3.f * ty3, wy3 = ty3 - ty2;
+    // images hold < 2^29 pixels: the byte offset fits in 32 bits
+    const unsigned ob = 4u * (unsigned)(off + ix + __mul24(iy, nx));
     float r[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const float *row = reinterpret_cast<const float *>(gb + (o + (unsigned)b * rs));
+        const float *row = reinterpret_cast<const float *>(rows[b] + ob);
         r[b] = wx0 * row[0] + wx1 * row[1] + wx2 * row[2] + wx3 * row[3];
     }
-    return wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3];
+    return 0.25f * (wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3]);
+}
+
+// One pixel of an IC-GN pass: warp its window offset, sample g, accumulate the sums
+// (S_g, S_gg, then S_{g J_k} and S_{g f_n}).  `in` = false counts nothing.
+template <bool SIX, int NS>
+__device__ __forceinline__ void icgn_pixel(const uint2 e, const char *const (&rows)[4], int nx,
+                                           float p0, float p1, float p2, float p3, float p4,
+                                           float p5, float fn, float fx, float fy, bool in,
+                                           float (&acc)[NS])
+{
+    const __half2 h = *reinterpret_cast<const __half2 *>(&e.x);
+    const float dx = __low2float(h), dy = __high2float(h);
+    const float ux = p0 * dx + p2 * dy + p4;
+    const float uy = p1 * dx + p3 * dy + p5;
+    float gv = sample_g(rows, nx, (int)e.y, ux, uy);
+    gv = in ? gv : 0.f;
+    const float t1 = gv * fx, t2 = gv * fy;
+    acc[0] += gv;
+    acc[1] += gv * gv;
+    if constexpr (SIX) {
+        acc[2] += t1 * dx; acc[3] += t2 * dx;
+        acc[4] += t1 * dy; acc[5] += t2 * dy;
+        acc[6] += t1; acc[7] += t2;
+        acc[8] += gv * fn;
+    } else {
+        acc[2] += t1; acc[3] += t2;
+        acc[4] += gv * fn;
+    }
 }
 
 // Shared body for both modes.  SIX = true: 6-DOF funICGN; false: 2-DOF funICGN_Subpb1.
@@ -232,6 +265,10 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     __shared__ double s_cs[SPB][NCS];
     __shared__ float s_red[16 * WPS];
     __shared__ double s_redd[WPS];
+    // per pixel slot of every thread: the offsets (dx, dy) from the window centre as
+    // two halves (exact: multiples of 1/2 below 1024) and the pixel's image index.
+    // Each thread writes and reads only its own entries.
+    __shared__ uint2 s_tab[SPB * PPL * T];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -412,15 +449,24 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     }
     }  // !cached
     const double bottomf = sqrt(ssf);
+    uint2 *tab = s_tab + sub_in_block * (PPL * T);
     {
         const float mf = (float)meanf;
         const float ib = (float)(1.0 / bottomf);
 #pragma unroll
         for (int k = 0; k < PPL; ++k) fv[k] = (fv[k] - mf) * ib;  // f_n (0 in empty slots)
+        const float half = 0.5f * (float)w;
         int i = i0, r = r0;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
-            if (!(r < w1)) fv[k] = 0.f;
+            const bool in = r < w1;
+            if (!in) fv[k] = 0.f;
+            const int ii = in ? i : 0, rr = in ? r : 0;  // empty slots: an in-window pixel
+            const __half2 h = __floats2half2_rn((float)ii - half, (float)rr - half);
+            uint2 e;
+            e.x = *reinterpret_cast<const unsigned *>(&h);
+            e.y = (unsigned)((ya - 1 + rr) * nx + (xa - 1 + ii));
+            tab[k * T + tid] = e;
             i += di; r += dr;
             if (i >= w1) { i -= w1; ++r; }
         }
@@ -478,32 +524,28 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
 #pragma unroll
         for (int q = 0; q < NS; ++q) acc[q] = 0.f;
         {
-            int i = i0, r = r0;
-            // keep the walk inside the loop: hoisting per-pixel offsets out of it would
-            // cost 2*PPL registers
-            asm volatile("" : "+v"(i), "+v"(r));
+            // the table is re-read from LDS every pass (kept opaque to the compiler, which
+            // would otherwise hold all of it in registers)
+            asm volatile("" ::: "memory");
+            const int lim = n - tid;  // slot k holds a pixel iff k * T < lim
+            const char *gb = reinterpret_cast<const char *>(a.g) - 4 * (1 + (size_t)nx);
+            const char *const rows[4] = {gb, gb + 4 * (size_t)nx, gb + 8 * (size_t)nx,
+                                         gb + 12 * (size_t)nx};
+            if (SIX && n > (PPL - 1) * T) {
+                // every slot holds a pixel on some lane (a 33 x 33 window in 18 slots): one
+                // branch-free run over the slots, so loads of later pixels can be issued
+                // ahead; only the last slot's empty lanes are masked out.  (The 2-DOF
+                // variant compiles this run into serialised loads and keeps the branches.)
 #pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                if (r < w1) {
-                    const float dx = (float)i - fhalf, dy = (float)r - fhalf;
-                    const float ux = p0 * dx + p2 * dy + p4;
-                    const float uy = p1 * dx + p3 * dy + p5;
-                    const float gv = sample_g(a.g, nx, xa + i, ya + r, ux, uy);
-                    const float t1 = gv * fxv[k], t2 = gv * fyv[k];
-                    acc[0] += gv;
-                    acc[1] += gv * gv;
-                    if (SIX) {
-                        acc[2] += t1 * dx; acc[3] += t2 * dx;
-                        acc[4] += t1 * dy; acc[5] += t2 * dy;
-                        acc[6] += t1; acc[7] += t2;
-                        acc[8] += gv * fv[k];
-                    } else {
-                        acc[2] += t1; acc[3] += t2;
-                        acc[4] += gv * fv[k];
-                    }
-                }
-                i += di; r += dr;
-                if (i >= w1) { i -= w1; ++r; }
+                for (int k = 0; k < PPL; ++k)
+                    icgn_pixel<SIX, NS>(tab[k * T + tid], rows, nx, p0, p1, p2, p3, p4, p5, fv[k],
+                                        fxv[k], fyv[k], k < PPL - 1 || k * T < lim, acc);
+            } else {
+#pragma unroll
+                for (int k = 0; k < PPL; ++k)
+                    if (k * T < lim)
+                        icgn_pixel<SIX, NS>(tab[k * T + tid], rows, nx, p0, p1, p2, p3, p4, p5,
+                                            fv[k], fxv[k], fyv[k], true, acc);
             }
         }
         block_sumN<WPS, NS>(acc, s_red, wid);

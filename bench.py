@@ -159,7 +159,7 @@ def main():
                 frac=round(icgn_bytes / t_i / 1e9 / HBM_PEAK_GBS, 5),
                 traffic=measured("k_icgn_6dof"), algorithmic_bytes=icgn_bytes,
                 avg_ms=round(icgn6_ms, 5),
-                note="VALU-bound (FP32 Keys gathers and sums, ~110 VALU per pixel pass)")
+                note="VALU- and latency-bound (FP32 Keys gathers and sums, 66 VALU per pixel pass)")
     # The integer search beside it: int8 matrix cores, algorithmic ops = the NCC
     # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
