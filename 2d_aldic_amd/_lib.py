@@ -24,7 +24,8 @@ EXPORTS = (
     "aldic_nodal_strain", "aldic_bicubic", "aldic_frame_stage", "aldic_frame_setup",
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
-    "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree",
+    "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
+    "aldic_qframe_run", "aldic_qframe_get",
 )
 
 
@@ -34,6 +35,11 @@ class FrameOpts(C.Structure):
                 ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
                 ("no_subpb1_fit", C.c_int), ("stage_times", C.c_int), ("smooth", C.c_int),
                 ("beta_sweep", C.c_int), ("admm_tol", C.c_double)]
+
+
+class QFrameOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
+                ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double)]
 
 
 _lib = None
@@ -77,6 +83,9 @@ def load():
     L.aldic_set_quadtree.argtypes = [P, vp, ip, vp, ip, vp, ip]
     L.aldic_subpb2_quadtree.argtypes = [P, dp, dp, vp, vp, dp, ip, vp, vp]
     L.aldic_nodal_strain_quadtree.argtypes = [P, vp, vp, vp, vp, vp, vp]
+    L.aldic_qframe_setup.argtypes = [P, C.POINTER(QFrameOpts), vp, vp, vp]
+    L.aldic_qframe_run.argtypes = [P, vp]
+    L.aldic_qframe_get.argtypes = [P, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -261,6 +270,37 @@ class Context:
                                                     _p(F), _p(Fgp), _p(xy)),
                  "aldic_nodal_strain_quadtree")
         return F, Fgp, xy
+
+    def qframe_setup(self, ws, tri, w, tol=1e-2, mu=1e-3, beta=None, n_admm=3, step=None,
+                     cg_maxit=5000, cg_rtol=1e-10):
+        """Quadtree frame on the mesh of set_quadtree: per-node windows ws (LocalICGN and
+        Subpb1), Gauss-point weights tri / w; beta defaults to 1e-3 step^2 mu
+        (main_ALDIC_Quadtree.m:202) with step = mean(ws) / 2."""
+        ws = _c(ws, np.int32).ravel()
+        if beta is None:
+            st = float(np.mean(ws)) / 2.0 if step is None else float(step)
+            beta = 1e-3 * st ** 2 * mu
+        o = QFrameOpts(float(tol), float(mu), float(beta), int(n_admm), int(cg_maxit),
+                       float(cg_rtol))
+        self._ck(self.L.aldic_qframe_setup(self.h, C.byref(o), _p(ws),
+                                           _p(_c(tri, np.int32)), _p(_c(w, np.float64))),
+                 "aldic_qframe_setup")
+        self.qframe_opts = o
+
+    def qframe_run(self, U0):
+        self._ck(self.L.aldic_qframe_run(self.h, _p(_c(U0, np.float64))), "aldic_qframe_run")
+
+    def qframe_get(self):
+        n = self.n_quad_nodes
+        U = np.empty(2 * n)
+        F = np.empty(4 * n)
+        it6 = np.empty(n, np.int32)
+        it2 = np.empty(n, np.int32)
+        t = np.empty(4)
+        self._ck(self.L.aldic_qframe_get(self.h, _p(U), _p(F), _p(it6), _p(it2), _p(t)),
+                 "aldic_qframe_get")
+        return dict(U=U, F=F, iters6=it6, iters2=it2, frame_ms=t[0], local_ms=t[1],
+                    global_ms=t[2], pcg_iters=int(t[3]))
 
     def nodal_strain(self, U, swap_fix=False):
         F = np.empty(4 * self.n_nodes)

@@ -102,6 +102,20 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, doubl
 hipError_t launch_blur3(const double *in, double *out, int gx, int gy, int nc, double sigma,
                         hipStream_t st);
 
+// Quadtree frame (main_ALDIC_Quadtree.m's Sections 4-5 on the device).
+struct QFrameState {
+    bool ready = false;
+    aldic_qframe_opts o{};
+    int S = 0;
+    DBuf ws, U0, U6, F6, H21, it6, U1, F1, U2, F2a, F2b, udual, vdual, it2, bad, part, known,
+        tri, w, Fgp, lists, rl, b, cc;
+    std::vector<int> boff, bn, bppl;  // 6-DOF launches: list offset, length and pixel class
+    int pcg_iters = 0;                // PCG iterations of the last frame's global steps
+    int cur_F2 = 0;                   // which of F2a / F2b holds the last F
+    double t_frame = 0, t_local = 0, t_global = 0;  // ms of the last frame
+    hipEvent_t ev[4] = {};
+};
+
 struct FrameState {
     bool ready = false;
     aldic_frame_opts o{};
@@ -148,6 +162,7 @@ struct aldic_ctx {
     aldic::QuadDev quad;             // aldic_set_quadtree / aldic_subpb2_quadtree
     bool have_quad = false;
     aldic::FrameState fr;
+    aldic::QFrameState qf;
     aldic::GridfitDev gf;            // aldic_gridfit
     hipEvent_t ev[10] = {};
 };
@@ -263,5 +278,17 @@ hipError_t launch_sqdiff(const double *a, const double *b, long n, double *out, 
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
                           double *part, hipStream_t st);
+// The bad-point rule alone (rule 0: status < 0 or > 99; 1: also > mean + 0.15 std of the
+// good ones; 2: also > max(mean + 0.15 std, 20)); bad U (and F) become NaN.
+hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
+                          const double *Fsrc, double *U, double *F, int *bad, double *part,
+                          hipStream_t st);
+// Refill of the bad nodes of an unstructured mesh from the node adjacency (rowptr/col):
+// sweeps over the nodes still unknown, each taking the inverse-distance average of its
+// known neighbours; nodes left unknown after `sweeps` get 0.  known: int[nn] scratch.
+hipError_t launch_meshfill(const MeshDev &m, const int *bad, double *U, double *F, int *known,
+                           int sweeps, hipStream_t st);
+// x[i] = src[i] at the Dirichlet-coded nodes of the quadtree (4 values per node)
+hipError_t quad_copy_dirichlet(const QuadDev &q, const double *src, double *x, hipStream_t st);
 
 }  // namespace aldic

@@ -279,6 +279,13 @@ void aldic_destroy(aldic_ctx *ctx)
     for (DBuf *bb : {&Q.m.coords, &Q.m.n2e_ptr, &Q.m.rowptr, &Q.m.col, &Q.m.aval, &Q.m.mval,
                      &Q.m.diag, &Q.m.gvec, &Q.elems, &Q.n2e, &Q.code, &Q.cval, &ctx->boxp})
         bb->release();
+    QFrameState &QF = ctx->qf;
+    for (DBuf *bb : {&QF.ws, &QF.U0, &QF.U6, &QF.F6, &QF.H21, &QF.it6, &QF.U1, &QF.F1, &QF.U2,
+                     &QF.F2a, &QF.F2b, &QF.udual, &QF.vdual, &QF.it2, &QF.bad, &QF.part,
+                     &QF.known, &QF.tri, &QF.w, &QF.Fgp, &QF.lists, &QF.rl, &QF.b, &QF.cc})
+        bb->release();
+    for (auto &e : QF.ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto &e : F.ev) (void)hipEventDestroy(e);
     F.ev.clear();
     for (auto &e : ctx->ev)
@@ -1321,6 +1328,225 @@ int aldic_frame_scalars(aldic_ctx *ctx, double *out, int n)
     const FrameState &F = ctx->fr;
     const double v[4] = {F.beta_used, (double)F.admm_steps, F.upd, F.upd2};
     for (int k = 0; k < n; ++k) out[k] = v[k];
+    return ALDIC_OK;
+}
+
+int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int *ws,
+                       const int *tri, const double *w)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_quad) return fail(ctx, ALDIC_ERR_STATE, "qframe_setup: no quadtree mesh");
+    if (!opts || !ws || !tri || !w || opts->n_admm < 0 || opts->cg_maxit < 1 ||
+        !(opts->cg_rtol > 0) || !(opts->tol > 0) || !(opts->mu > 0) || !(opts->beta > 0))
+        return fail(ctx, ALDIC_ERR_ARG, "qframe_setup: bad arguments");
+    aldic::QuadDev &q = ctx->quad;
+    QFrameState &Q = ctx->qf;
+    const int S = q.m.nn, ngp = 4 * q.m.E;
+    for (int k = 0; k < 3 * S; ++k)
+        if (tri[k] < 0 || tri[k] >= ngp)
+            return fail(ctx, ALDIC_ERR_ARG, "qframe_setup: Gauss point index");
+    // 6-DOF launches: nodes grouped by pixel class in increasing node order
+    std::vector<int> cls(S);
+    for (int i = 0; i < S; ++i) {
+        cls[i] = icgn_ppl_for(ws[i]);
+        if (ws[i] < 2 || cls[i] == 0) return fail(ctx, ALDIC_ERR_ARG, "qframe_setup: window size");
+    }
+    std::vector<int> lists;
+    Q.boff.clear(); Q.bn.clear(); Q.bppl.clear();
+    for (int c : {8, 18, 72}) {
+        const int off = (int)lists.size();
+        for (int i = 0; i < S; ++i)
+            if (cls[i] == c) lists.push_back(i);
+        if ((int)lists.size() > off) {
+            Q.boff.push_back(off);
+            Q.bn.push_back((int)lists.size() - off);
+            Q.bppl.push_back(c);
+        }
+    }
+    CK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t s8 = sizeof(double) * (size_t)S;
+    CK(up(Q.ws, ws, S, st));
+    CK(up(Q.lists, lists.data(), lists.size(), st));
+    CK(up(Q.tri, tri, 3 * (size_t)S, st));
+    CK(up(Q.w, w, 3 * (size_t)S, st));
+    for (DBuf *b : {&Q.U0, &Q.U6, &Q.U1, &Q.U2, &Q.vdual, &Q.b}) CK(b->ensure(2 * s8));
+    for (DBuf *b : {&Q.F6, &Q.F1, &Q.F2a, &Q.F2b, &Q.udual}) CK(b->ensure(4 * s8));
+    CK(Q.H21.ensure(21 * s8));
+    CK(Q.cc.ensure(8 * s8));
+    CK(Q.Fgp.ensure(sizeof(double) * 4 * (size_t)(ngp ? ngp : 1)));
+    CK(Q.it6.ensure(sizeof(int) * (size_t)S));
+    CK(Q.it2.ensure(sizeof(int) * (size_t)S));
+    CK(Q.bad.ensure(sizeof(int) * (size_t)S));
+    CK(Q.known.ensure(sizeof(int) * 2 * (size_t)S));
+    CK(Q.rl.ensure(sizeof(int) * ((size_t)S + 1)));
+    CK(Q.part.ensure(sizeof(double) * 3 * 256));
+    for (auto &e : Q.ev)
+        if (!e) CK(hipEventCreate(&e));
+    CK(aldic::quad_assemble(q, opts->beta, opts->mu, st));
+    CK(hipStreamSynchronize(st));
+    Q.o = *opts;
+    Q.S = S;
+    Q.ready = true;
+    return ALDIC_OK;
+}
+
+int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    QFrameState &Q = ctx->qf;
+    if (!Q.ready || !ctx->have_quad) return fail(ctx, ALDIC_ERR_STATE, "qframe_run: no setup");
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "qframe_run: no images");
+    if (!U0) return fail(ctx, ALDIC_ERR_ARG, "qframe_run: U0");
+    aldic::QuadDev &q = ctx->quad;
+    if (q.m.beta != Q.o.beta || q.m.mu != Q.o.mu) CK(aldic::quad_assemble(q, Q.o.beta, Q.o.mu, ctx->stream));
+    CK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int S = Q.S;
+    const aldic_qframe_opts &o = Q.o;
+    CK(up(Q.U0, U0, 2 * (size_t)S, st));
+    CK(hipEventRecord(Q.ev[0], st));
+    // LocalICGN: one launch per window class over its node list
+    IcgnArgs a{};
+    a.f = ctx->f.as<float>();
+    a.g = ctx->g.as<float>();
+    a.gx = ctx->gx.as<float>();
+    a.gy = ctx->gy.as<float>();
+    a.nx = ctx->nx;
+    a.ny = ctx->ny;
+    a.S = S;
+    a.coords = q.m.coords.as<double>();
+    a.ws = Q.ws.as<int>();
+    a.tol = o.tol;
+    a.U0 = Q.U0.as<double>();
+    a.U = Q.U6.as<double>();
+    a.F = Q.F6.as<double>();
+    a.H21 = Q.H21.as<double>();
+    a.iters = Q.it6.as<int>();
+    a.Ucopy = Q.U1.as<double>();  // bad-point step works on the copies
+    a.Fcopy = Q.F1.as<double>();
+    a.cc = Q.cc.as<double>();     // window constants kept for the 2-DOF runs
+    a.cc_mode = 1;
+    for (size_t k = 0; k < Q.bn.size(); ++k) {
+        IcgnArgs ak = a;
+        ak.list = Q.lists.as<int>() + Q.boff[k];
+        ak.nlist = Q.bn[k];
+        CK(launch_icgn6(ak, Q.bppl[k], st));
+    }
+    // windows the mask rule enlarged past their class (status -2) run again in the
+    // largest class
+    {
+        IcgnArgs ar = a;
+        int *rl = Q.rl.as<int>();
+        CK(launch_list_eq(Q.it6.as<int>(), S, -2, rl, rl + S, st));
+        ar.list = rl;
+        ar.nlist = S;
+        ar.nlist_dev = rl + S;
+        CK(launch_icgn6(ar, 72, st));
+    }
+    double *U1 = Q.U1.as<double>(), *F1 = Q.F1.as<double>(), *U2 = Q.U2.as<double>();
+    CK(launch_badmark(Q.it6.as<int>(), S, 1, nullptr, nullptr, U1, F1, Q.bad.as<int>(),
+                      Q.part.as<double>(), st));
+    CK(launch_meshfill(q.m, Q.bad.as<int>(), U1, F1, Q.known.as<int>(), 8, st));
+    CK(hipEventRecord(Q.ev[1], st));
+
+    double *F2buf[2] = {Q.F2a.as<double>(), Q.F2b.as<double>()};
+    int cur = 0;
+    const double *F1p = F1;
+    static thread_local DBuf work[8];
+    int pcg_total = 0;
+    float t_glob = 0.f;
+    auto global_step = [&]() -> int {
+        CK(hipEventRecord(Q.ev[2], st));
+        CK(aldic::quad_rhs(q, o.beta, o.mu, U1, F1p, Q.b.as<double>(), st));
+        double stt[2] = {0, 0};
+        CK(aldic::fem_pcg(q.m, Q.b.as<double>(), U2, o.cg_rtol, o.cg_maxit, work, stt, st, 8));
+        pcg_total += (int)stt[0];
+        CK(aldic::quad_finish(q, U1, U2, st));
+        CK(aldic::quad_gp_strain(q, U2, Q.Fgp.as<double>(), nullptr, st));
+        CK(aldic::quad_gp_gather(S, Q.tri.as<int>(), Q.w.as<double>(), Q.Fgp.as<double>(),
+                                 F2buf[cur], st));
+        // hole-edge nodes keep Subpb1's F (main_ALDIC_Quadtree.m:222,225,286,288)
+        CK(aldic::quad_copy_dirichlet(q, F1p, F2buf[cur], st));
+        CK(launch_sub(F2buf[cur], F1p, Q.udual.as<double>(), 4 * (long)S, st));
+        CK(launch_sub(U2, U1, Q.vdual.as<double>(), 2 * (long)S, st));
+        CK(hipEventRecord(Q.ev[3], st));
+        CK(hipEventSynchronize(Q.ev[3]));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, Q.ev[2], Q.ev[3]));
+        t_glob += ms;
+        return ALDIC_OK;
+    };
+    int r = global_step();
+    if (r) return r;
+    for (int step = 0; step < o.n_admm; ++step) {
+        // Subpb1Quadtree from (U2, F2) with the LocalICGN Hessian, on the same windows
+        IcgnArgs a2 = a;
+        a2.U0 = nullptr;
+        a2.Hin = Q.H21.as<double>();
+        a2.vdual = Q.vdual.as<double>();
+        a2.Uold = U2;
+        a2.Fold = F2buf[cur];
+        a2.mu = o.mu;
+        a2.U = U1;
+        a2.iters = Q.it2.as<int>();
+        a2.F = nullptr;
+        a2.H21 = nullptr;
+        a2.Ucopy = nullptr;
+        a2.Fcopy = nullptr;
+        a2.cc_mode = 2;
+        for (size_t k = 0; k < Q.bn.size(); ++k) {
+            IcgnArgs ak = a2;
+            ak.list = Q.lists.as<int>() + Q.boff[k];
+            ak.nlist = Q.bn[k];
+            CK(launch_icgn2(ak, Q.bppl[k], st));
+        }
+        {
+            IcgnArgs ar2 = a2;
+            int *rl = Q.rl.as<int>();
+            CK(launch_list_eq(Q.it2.as<int>(), S, -2, rl, rl + S, st));
+            ar2.list = rl;
+            ar2.nlist = S;
+            ar2.nlist_dev = rl + S;
+            CK(launch_icgn2(ar2, 72, st));
+        }
+        CK(launch_badmark(Q.it2.as<int>(), S, 2, nullptr, nullptr, U1, nullptr, Q.bad.as<int>(),
+                          Q.part.as<double>(), st));
+        CK(launch_meshfill(q.m, Q.bad.as<int>(), U1, nullptr, Q.known.as<int>(), 8, st));
+        F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC_Quadtree.m:258)
+        cur ^= 1;
+        if ((r = global_step())) return r;
+    }
+    Q.cur_F2 = cur;
+    float ms_frame = 0.f, ms_local = 0.f;
+    CK(hipEventElapsedTime(&ms_frame, Q.ev[0], Q.ev[3]));
+    CK(hipEventElapsedTime(&ms_local, Q.ev[0], Q.ev[1]));
+    Q.t_frame = ms_frame;
+    Q.t_local = ms_local;
+    Q.t_global = t_glob;
+    Q.pcg_iters = pcg_total;
+    return ALDIC_OK;
+}
+
+int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *iters2,
+                     double *times)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    QFrameState &Q = ctx->qf;
+    if (!Q.ready) return fail(ctx, ALDIC_ERR_STATE, "qframe_get: no setup");
+    CK(hipSetDevice(ctx->device));
+    const size_t S = Q.S;
+    if (U) CK(down(U, Q.U2.p, 2 * S, ctx->stream));
+    if (F) CK(down(F, (Q.cur_F2 ? Q.F2b : Q.F2a).p, 4 * S, ctx->stream));
+    if (iters6) CK(down(iters6, Q.it6.p, S, ctx->stream));
+    if (iters2) CK(down(iters2, Q.it2.p, S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (times) {
+        times[0] = Q.t_frame;
+        times[1] = Q.t_local;
+        times[2] = Q.t_global;
+        times[3] = Q.pcg_iters;
+    }
     return ALDIC_OK;
 }
 

@@ -784,6 +784,7 @@ __global__ __launch_bounds__(256) void k_itstats(const int *it, int S, double *p
 }
 
 // k_mark: threshold from the partials (same order in every block), mark, copy, NaN.
+// mean_rule 2 is the quadtree Subpb1's max(mean + 0.15 std, 20).
 __global__ __launch_bounds__(256) void k_mark(const int *it, int S, const double *part, int np,
                                               int mean_rule, const double *Usrc,
                                               const double *Fsrc, double *U, double *F, int *bad)
@@ -800,7 +801,8 @@ __global__ __launch_bounds__(256) void k_mark(const int *it, int S, const double
     const double C = tot[0], Sm = tot[1], S2 = tot[2];
     const double mean = C > 0 ? Sm / C : 0.0;
     const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
-    const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
+    double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
+    if (mean_rule == 2) thr = thr > 20.0 ? thr : 20.0;  // Subpb1Quadtree.m:180
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x) {
         const int v = it[i];
         const bool b = v < 0 || v > 99 || (mean_rule && C > 0 && (double)v > thr);
@@ -871,6 +873,20 @@ hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_
     hipLaunchKernelGGL(k_mark, dim3(nbf), dim3(256), 0, st, iters, S, (const double *)part, nb,
                        use_mean_rule, Usrc, Fsrc, U, F, bad);
     hipLaunchKernelGGL(k_axisfill, dim3(nbf), dim3(256), 0, st, Nx, Ny, (const int *)bad, U, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
+                          const double *Fsrc, double *U, double *F, int *bad, double *part,
+                          hipStream_t st)
+{
+    if (S <= 0) return hipSuccess;
+    int nb = (S + 255) / 256;
+    if (nb > 256) nb = 256;
+    const int nbf = (S + 255) / 256;
+    hipLaunchKernelGGL(k_itstats, dim3(nb), dim3(256), 0, st, iters, S, part);
+    hipLaunchKernelGGL(k_mark, dim3(nbf), dim3(256), 0, st, iters, S, (const double *)part, nb,
+                       rule, Usrc, Fsrc, U, F, bad);
     return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@
 // identity rows with zero right-hand side, their columns move into a coupling array
 // that the free rows' right-hand side subtracts (times U at Dirichlet nodes), so the
 // system stays symmetric positive definite and the fixed values are written exactly.
+#include <utility>
+
 #include "aldic_internal.h"
 #include "common.h"
 
@@ -272,6 +274,63 @@ __global__ __launch_bounds__(256) void k_gp_gather(int nn, const int *tri, const
     F[i] = s;
 }
 
+// ---- bad-node refill on the mesh -----------------------------------------------------
+// One sweep: a node not yet known takes the inverse-distance average of its known
+// neighbours (nodes sharing an element) and becomes known for the next sweep; nodes it
+// reads were known before this sweep, so no node reads a value written in the same sweep.
+__global__ __launch_bounds__(256) void k_fill_init(int nn, const int *bad, int *known)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) known[i] = bad[i] ? 0 : 1;
+}
+
+__global__ __launch_bounds__(256) void k_fill_sweep(int nn, const int *rowptr, const int *col,
+                                                    const double2 *xy, const int *kin, int *kout,
+                                                    double2 *U, double4 *F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    if (kin[i]) { kout[i] = 1; return; }
+    const double2 pi = xy[i];
+    double ws = 0.0;
+    double2 su = make_double2(0.0, 0.0);
+    double4 sf = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const int c = col[k];
+        if (c == i || !kin[c]) continue;
+        const double dx = xy[c].x - pi.x, dy = xy[c].y - pi.y;
+        const double w = 1.0 / sqrt(dx * dx + dy * dy);
+        ws += w;
+        su.x += w * U[c].x; su.y += w * U[c].y;
+        if (F) {
+            const double4 f = F[c];
+            sf.x += w * f.x; sf.y += w * f.y; sf.z += w * f.z; sf.w += w * f.w;
+        }
+    }
+    if (ws > 0.0) {
+        U[i] = make_double2(su.x / ws, su.y / ws);
+        if (F) F[i] = make_double4(sf.x / ws, sf.y / ws, sf.z / ws, sf.w / ws);
+        kout[i] = 1;
+    } else {
+        kout[i] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_rest(int nn, const int *known, double2 *U, double4 *F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn || known[i]) return;
+    U[i] = make_double2(0.0, 0.0);
+    if (F) F[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_copy_dir(int nn, const int *code, const double4 *src,
+                                                  double4 *x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn && code[i] == 1) x[i] = src[i];
+}
+
 }  // namespace
 
 hipError_t quad_assemble(QuadDev &q, double beta, double mu, hipStream_t st)
@@ -321,6 +380,31 @@ hipError_t quad_gp_gather(int nn, const int *tri, const double *w, const double 
 {
     hipLaunchKernelGGL(k_gp_gather, dim3((nn + 255) / 256), dim3(256), 0, st, nn, tri, w,
                        reinterpret_cast<const double4 *>(Fgp), reinterpret_cast<double4 *>(F));
+    return hipGetLastError();
+}
+
+hipError_t launch_meshfill(const MeshDev &m, const int *bad, double *U, double *F, int *known,
+                           int sweeps, hipStream_t st)
+{
+    const int nn = m.nn, nb = (nn + 255) / 256;
+    int *k0 = known, *k1 = known + nn;
+    hipLaunchKernelGGL(k_fill_init, dim3(nb), dim3(256), 0, st, nn, bad, k0);
+    for (int s = 0; s < sweeps; ++s) {
+        hipLaunchKernelGGL(k_fill_sweep, dim3(nb), dim3(256), 0, st, nn, m.rowptr.as<int>(),
+                           m.col.as<int>(), m.coords.as<double2>(), (const int *)k0, k1,
+                           reinterpret_cast<double2 *>(U), reinterpret_cast<double4 *>(F));
+        std::swap(k0, k1);
+    }
+    hipLaunchKernelGGL(k_fill_rest, dim3(nb), dim3(256), 0, st, nn, (const int *)k0,
+                       reinterpret_cast<double2 *>(U), reinterpret_cast<double4 *>(F));
+    return hipGetLastError();
+}
+
+hipError_t quad_copy_dirichlet(const QuadDev &q, const double *src, double *x, hipStream_t st)
+{
+    const int nn = q.m.nn;
+    hipLaunchKernelGGL(k_copy_dir, dim3((nn + 255) / 256), dim3(256), 0, st, nn, q.code.as<int>(),
+                       reinterpret_cast<const double4 *>(src), reinterpret_cast<double4 *>(x));
     return hipGetLastError();
 }
 

@@ -109,7 +109,9 @@ class SpecklePair:
     params: dict = field(default_factory=dict)
 
 
-def make_pair(nx, ny, n_dots, sigma, ufun, noise, bits=8, amp=(0.4, 1.0), seed=0):
+def make_pair(nx, ny, n_dots, sigma, ufun, noise, bits=8, amp=(0.4, 1.0), seed=0, hole=None):
+    """hole = (cx, cy, a): a black disc of radius a in the reference, carried into the
+    deformed image through the inverse map."""
     dots = draw_dots(n_dots, nx, ny, sigma, amp, seed)
     fr = _render(dots, nx, ny)
     full = float((1 << bits) - 1)
@@ -127,6 +129,11 @@ def make_pair(nx, ny, n_dots, sigma, ufun, noise, bits=8, amp=(0.4, 1.0), seed=0
     gn = gd * scale
     if noise > 0:
         gn = gn + rng.normal(0.0, noise, gn.shape)
+    if hole is not None:
+        hx, hy, ha = hole
+        y, x = np.mgrid[1 : ny + 1, 1 : nx + 1]
+        f[(x - hx) ** 2 + (y - hy) ** 2 < ha * ha] = 0
+        gn[(Xm - hx) ** 2 + (Ym - hy) ** 2 < ha * ha] = 0.0
     g = np.clip(np.rint(gn), 0, full).astype(dt)
     return SpecklePair(f, g, ufun, dict(nx=nx, ny=ny, n_dots=n_dots, sigma=sigma,
                                          noise=noise, bits=bits, seed=seed))
@@ -165,6 +172,148 @@ def config3(seed=0, n=4096, n_dots=600_000):
 
     p = make_pair(n, n, n_dots, (1.5, 3.0), ufun, 2.0, 12, seed=seed)
     p.params.update(ws=32, step=8, search=24, admm=3, name=f"config3-{n}-ws32-st8")
+    return p
+
+
+def kirsch_field(cx, cy, a, eps=0.01, nu=0.3):
+    """Plate with a circular hole of radius a under uniaxial tension along x, plane stress
+    (Kirsch): far-field strain eps; (u, v) of the reference point (X, Y).  Inside the
+    hole (masked) r is held at 0.9 a so the field stays smooth."""
+    kap = (3.0 - nu) / (1.0 + nu)
+    k = eps * (1.0 + nu) / 2.0  # sigma / (4 G)
+
+    def ufun(X, Y):
+        x = X - cx
+        y = Y - cy
+        r = np.maximum(np.hypot(x, y), 0.9 * a)
+        t = np.arctan2(y, x)
+        c2, s2 = np.cos(2 * t), np.sin(2 * t)
+        ur = k * (r * ((kap - 1) / 2 + c2) + a * a / r * (1 + (1 + kap) * c2) - a ** 4 / r ** 3 * c2)
+        ut = k * ((1 - kap) * a * a / r - r - a ** 4 / r ** 3) * s2
+        return ur * np.cos(t) - ut * np.sin(t), ur * np.sin(t) + ut * np.cos(t)
+
+    return ufun
+
+
+def quadtree_hole_mesh(nx, ny, cx, cy, a, margin=64, h_far=32, h_mid=16, h_near=8,
+                       band_near=64.0, band_mid=400.0):
+    """Quadtree of square cells over [margin, n - margin] with a circular hole (centre
+    (cx, cy), radius a): cells within band_near of the hole edge are refined to h_near,
+    cells within band_mid of it to h_mid, the rest stay h_far; cells entirely inside the
+    hole are dropped; neighbouring cells differ by at most one level (2:1), so an edge
+    has at most one hanging node, which becomes the coarse cell's midside node.
+    Returns (coords (nn, 2) 1-based pixels, elems (E, 8) 0-based with -1 for no midside
+    node (corners counter-clockwise, then right, top, left, bottom midpoints), dirichlet
+    = nodes inside the hole, ws = per-node window 2 x smallest incident cell clamped to
+    [16, 64])."""
+    n0x = (nx - 2 * margin) // h_far
+    n0y = (ny - 2 * margin) // h_far
+    x0 = y0 = margin
+
+    def target(x, y, h):
+        # distance range of the cell to the centre -> finest size it needs
+        dx = max(cx - (x + h), 0.0, x - cx)
+        dy = max(cy - (y + h), 0.0, y - cy)
+        rmin = math.hypot(dx, dy)
+        rmax = max(math.hypot(x - cx, y - cy), math.hypot(x + h - cx, y - cy),
+                   math.hypot(x - cx, y + h - cy), math.hypot(x + h - cx, y + h - cy))
+        if rmax < a:
+            return -1  # inside the hole
+        if rmin <= a + band_near:
+            return h_near
+        if rmin <= a + band_mid:
+            return h_mid
+        return h_far
+
+    leaves = set()
+    stack = [(x0 + i * h_far, y0 + j * h_far, h_far) for j in range(n0y) for i in range(n0x)]
+    while stack:
+        x, y, h = stack.pop()
+        t = target(x, y, h)
+        if t == -1:
+            continue
+        if h > t:
+            hh = h // 2
+            stack += [(x, y, hh), (x + hh, y, hh), (x, y + hh, hh), (x + hh, y + hh, hh)]
+        else:
+            leaves.add((x, y, h))
+
+    # 2:1 balance: split a leaf while a neighbour across an edge is more than twice finer
+    changed = True
+    while changed:
+        changed = False
+        fine = {}
+        for (x, y, h) in leaves:
+            fine.setdefault(h, set()).add((x, y))
+        split = []
+        for (x, y, h) in leaves:
+            if h < 4:
+                continue
+            # any leaf of size <= h/4 touching this cell's edges from outside?
+            hit = False
+            for hs in [v for v in fine if v <= h // 4]:
+                for k in range(0, h, hs):
+                    for (px, py) in ((x - hs, y + k), (x + h, y + k), (x + k, y - hs), (x + k, y + h)):
+                        if (px, py) in fine[hs]:
+                            hit = True
+                            break
+                    if hit:
+                        break
+                if hit:
+                    break
+            if hit:
+                split.append((x, y, h))
+        for (x, y, h) in split:
+            leaves.discard((x, y, h))
+            hh = h // 2
+            for c in ((x, y), (x + hh, y), (x, y + hh), (x + hh, y + hh)):
+                if target(c[0], c[1], hh) != -1:
+                    leaves.add((c[0], c[1], hh))
+            changed = True
+
+    corners = set()
+    for (x, y, h) in leaves:
+        corners.update({(x, y), (x + h, y), (x + h, y + h), (x, y + h)})
+    pts = sorted(corners, key=lambda p: (p[1], p[0]))
+    index = {p: i for i, p in enumerate(pts)}
+    elems = []
+    cell_of = []
+    for (x, y, h) in sorted(leaves, key=lambda c: (c[1], c[0], c[2])):
+        hh = h // 2
+        e = [index[(x, y)], index[(x + h, y)], index[(x + h, y + h)], index[(x, y + h)]]
+        for m in ((x + h, y + hh), (x + hh, y + h), (x, y + hh), (x + hh, y)):
+            e.append(index.get(m, -1) if hh * 2 == h else -1)
+        elems.append(e)
+        cell_of.append(h)
+    coords = np.array(pts, dtype=np.float64)
+    elems = np.array(elems, dtype=np.int64)
+    hmin = np.full(len(pts), 10 ** 9)
+    for e, h in zip(elems, cell_of):
+        for v in e[:4]:
+            hmin[v] = min(hmin[v], h)
+        for v in e[4:]:
+            if v >= 0:
+                hmin[v] = min(hmin[v], h // 2)
+    ws = np.clip(2 * hmin, 16, 64).astype(np.int32)
+    r = np.hypot(coords[:, 0] - cx, coords[:, 1] - cy)
+    dirichlet = np.flatnonzero(r < a)
+    return coords, elems, dirichlet, ws
+
+
+def config4(seed=0, n=2048, a=300.0, eps=0.01, band_near=96.0, band_mid=600.0, **mesh):
+    """2048x2048 uint8 speckle (200k dots, sigma 2), circular hole radius 300 at the
+    centre, Kirsch plate-with-hole field (far-field strain 1 %), noise 1; quadtree mesh
+    refined towards the hole edge (cells 8-32 px, windows 16-64 px)."""
+    c = (n + 1) / 2.0
+    ufun = kirsch_field(c, c, a, eps)
+    n_dots = int(200_000 * (n / 2048.0) ** 2)
+    p = make_pair(n, n, n_dots, 2.0, ufun, 1.0, 8, seed=seed, hole=(c, c, a))
+    coords, elems, dirichlet, ws = quadtree_hole_mesh(n, n, c, c, a, band_near=band_near,
+                                                      band_mid=band_mid, **mesh)
+    y, x = np.mgrid[1 : n + 1, 1 : n + 1]
+    mask = ((x - c) ** 2 + (y - c) ** 2 >= a * a).astype(np.uint8)
+    p.params.update(name=f"config4-{n}-hole{int(a)}-quadtree", coords=coords, elems=elems,
+                    dirichlet=dirichlet, ws=ws, mask=mask, hole=(c, c, a))
     return p
 
 

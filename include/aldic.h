@@ -221,6 +221,37 @@ int aldic_frame_counters(aldic_ctx *ctx, int *out, int n);
  * n <= 4. */
 int aldic_frame_scalars(aldic_ctx *ctx, double *out, int n);
 
+/* Quadtree frame: main_ALDIC_Quadtree.m:159-312 (Sections 4-5 of one frame, smoothing
+ * off) on the mesh of aldic_set_quadtree (its nodes are the subset centres, its
+ * Dirichlet nodes the hole-edge nodes) and the images of aldic_set_images:
+ *   LocalICGN (6-DOF, window ws[i] per node; LocalICGN.m:119-131 bad-point rule),
+ *   Subpb2Quadtree at beta with zero duals, funGlobalNodalStrainQuadtree, hole-edge F
+ *   kept from Subpb1 (:222,:225), udual = F2 - F1, vdual = U2 - U1, then n_admm times
+ *   Subpb1Quadtree (2-DOF on the same per-node windows, its winsizeList of
+ *   Subpb1Quadtree.m:50, with LocalICGN's Hessian; the max(mean + 0.15 std, 20) rule;
+ *   F1 = F2), Subpb2Quadtree, nodal strain, hole-edge F, duals (:251-312).
+ * Bad nodes are refilled from their mesh neighbours (inverse-distance average, sweeps
+ * outward; the reference uses scatteredInterpolant 'natural').  The nodal strain uses
+ * the Gauss-point interpolation weights tri / w (nn x 3) as in
+ * aldic_nodal_strain_quadtree. */
+typedef struct {
+    double tol;        /* IC-GN stop tolerance (1e-2) */
+    double mu, beta;   /* ADMM penalties */
+    int n_admm;        /* ADMM steps after step 1 (3) */
+    int cg_maxit;      /* PCG iterations per global step */
+    double cg_rtol;    /* PCG relative residual */
+} aldic_qframe_opts;
+int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int *ws,
+                       const int *tri, const double *w);
+/* One frame from the initial guess U0 (2 nn). */
+int aldic_qframe_run(aldic_ctx *ctx, const double *U0);
+/* Results of the last frame, any may be NULL: U (2 nn) and F (4 nn) after the last
+ * global step, iters6 of LocalICGN, iters2 of the last Subpb1, times = {frame ms,
+ * LocalICGN + bad-point step ms, global steps ms (Subpb2 + strain + duals, all steps),
+ * PCG iterations (all steps)}. */
+int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *iters2,
+                     double *times);
+
 #ifdef __cplusplus
 }
 #endif

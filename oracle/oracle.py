@@ -519,15 +519,50 @@ def beta_choice(beta_list, err1, err2):
 # ---- whole frame (main_ALDIC.m:87-451, FEM branch, smoothing off) -------------------
 
 def badpoint(iters, mean_rule):
-    """LocalICGN.m:119-131 (mean_rule) / Subpb1.m:167-169: bad-subset mask."""
+    """LocalICGN.m:119-131 (mean_rule 1) / Subpb1.m:167-169 (0) / Subpb1Quadtree.m:170-181
+    (2: threshold max(mean + 0.15 std, 20)): bad-subset mask."""
     it = np.asarray(iters)
     bad = (it < 0) | (it > 99)
     good = ~bad
     if mean_rule and good.any():
         m = it[good].mean()
         sd = it[good].std(ddof=1) if good.sum() > 1 else 0.0
-        bad = bad | (it > m + 0.15 * sd)
+        thr = m + 0.15 * sd
+        if mean_rule == 2:
+            thr = max(thr, 20.0)
+        bad = bad | (it > thr)
     return bad
+
+
+def mesh_fill(coords, elems8, V, nc, bad, sweeps=8):
+    """Refill of bad nodes on a mesh (the reference interpolates them with
+    scatteredInterpolant 'natural'; this rule is the library's): in each sweep every node
+    not yet known takes the inverse-distance average of its known neighbours (nodes
+    sharing an element), the known set growing after the sweep; after `sweeps` sweeps
+    the rest get 0."""
+    coords = np.asarray(coords, np.float64).reshape(-1, 2)
+    el = np.asarray(elems8, np.int64).reshape(-1, 8)
+    nn = coords.shape[0]
+    nbr = [set() for _ in range(nn)]
+    for e in el:
+        nd = [int(v) for v in e if v >= 0]
+        for a in nd:
+            nbr[a].update(nd)
+    V = np.asarray(V, np.float64).reshape(nn, nc).copy()
+    known = ~np.asarray(bad, bool)
+    for _ in range(sweeps):
+        new = known.copy()
+        out = V.copy()
+        for i in np.flatnonzero(~known):
+            js = [j for j in sorted(nbr[i]) if j != i and known[j]]
+            if not js:
+                continue
+            w = np.array([1.0 / np.hypot(*(coords[j] - coords[i])) for j in js])
+            out[i] = (w[:, None] * V[js]).sum(0) / w.sum()
+            new[i] = True
+        V, known = out, new
+    V[~known] = 0.0
+    return V.ravel()
 
 
 def axis_fill(V, nc, bad, Nx, Ny):

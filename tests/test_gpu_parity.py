@@ -689,3 +689,70 @@ def test_mirror_chain_config1(cfg1, orc):
     U1o[np.repeat(orc.badpoint(it1o, False), 2)] = np.nan
     U1s = orc.subpb1_smooth(U1o, coords)
     assert np.max(np.abs(U1 - U1s)) <= 1e-3
+
+
+def _qframe_oracle(orc, f, g, gx, gy, xy, el, dirichlet, ws, tri, w, U0, beta, mu, n_admm):
+    """main_ALDIC_Quadtree.m:159-312 restated from the oracle's pieces (smoothing off)."""
+    U6, F6, H, it6, _ = orc.icgn6(f, g, gx, gy, xy, U0, ws)
+    bad = orc.badpoint(it6, 1)
+    U1 = orc.mesh_fill(xy, el, np.where(np.repeat(bad, 2), np.nan, U6), 2, bad)
+    F1 = orc.mesh_fill(xy, el, np.where(np.repeat(bad, 4), np.nan, F6), 4, bad)
+
+    def global_step(U1, F1):
+        U2 = orc.subpb2_quadtree(xy, el, dirichlet, beta, mu, U1, F1)
+        Fgp, _ = orc.gauss_strain_quadtree(xy, el, U2)
+        F2 = orc.gather_weights(Fgp, tri, w).reshape(-1, 4)
+        F2[dirichlet] = F1.reshape(-1, 4)[dirichlet]
+        return U2, F2.ravel()
+
+    U2, F2 = global_step(U1, F1)
+    it2 = None
+    for _ in range(n_admm):
+        udual, vdual = F2 - F1, U2 - U1
+        U1, it2 = orc.icgn2(f, g, gx, gy, xy, ws, H, mu, udual, vdual, U2, F2)
+        bad = orc.badpoint(it2, 2)
+        U1 = orc.mesh_fill(xy, el, np.where(np.repeat(bad, 2), np.nan, U1), 2, bad)
+        F1 = F2
+        U2, F2 = global_step(U1, F1)
+    return dict(U=U2, F=F2, iters6=it6, iters2=it2)
+
+
+def test_qframe_plate_with_hole_matches_oracle(gpu_ctx, speckle, orc):
+    """The quadtree frame on a small plate with a hole (uneven windows 16-64 px, masked
+    hole, hanging nodes, hole-edge Dirichlet nodes) against the oracle's restatement."""
+    import importlib
+    qt = importlib.import_module("2d_aldic_amd.quadtree")
+    n, a = 384, 60.0
+    c = (n + 1) / 2.0
+    ufun = speckle.kirsch_field(c, c, a, 0.01)
+    p = speckle.make_pair(n, n, int(0.05 * n * n), 2.0, ufun, 1.0, 8, seed=5, hole=(c, c, a))
+    xy, el, dr, ws = speckle.quadtree_hole_mesh(n, n, c, c, a, margin=48, band_near=24,
+                                                band_mid=60)
+    yy, xx = np.mgrid[1:n + 1, 1:n + 1]
+    mask = ((xx - c) ** 2 + (yy - c) ** 2 >= a * a).astype(np.uint8)
+    _flush(f"qframe: {len(xy)} nodes, {len(el)} elements, windows {np.unique(ws)}")
+    gpu_ctx.set_images(p.f, p.g, mask=mask)
+    gpu_ctx.set_quadtree(xy, el, dr)
+    u, v = ufun(xy[:, 0], xy[:, 1])
+    U0 = np.column_stack([np.rint(u), np.rint(v)]).ravel()
+    _, _, gxy = gpu_ctx.nodal_strain_quadtree(U0)
+    tri, w = qt.linear_weights(gxy, xy)
+    mu = 1e-3
+    beta = 1e-3 * 16 ** 2 * mu
+    gpu_ctx.qframe_setup(ws, tri, w, mu=mu, beta=beta, n_admm=3, cg_rtol=1e-12)
+    gpu_ctx.qframe_run(U0)
+    r = gpu_ctx.qframe_get()
+    f, _ = orc.normalize(p.f.astype(np.float64))
+    g, _ = orc.normalize(p.g.astype(np.float64))
+    gx, gy = orc.gradient(f, mask)
+    o = _qframe_oracle(orc, f, g, gx, gy, xy, el, dr, ws, tri, w, U0, beta, mu, 3)
+    assert np.mean(r["iters6"] == o["iters6"]) >= 0.97
+    assert np.mean(r["iters2"] == o["iters2"]) >= 0.97
+    dU = np.abs(r["U"] - o["U"])
+    assert np.sqrt(np.mean(dU ** 2)) <= 2e-3, (np.sqrt(np.mean(dU ** 2)), dU.max())
+    far = np.hypot(xy[:, 0] - c, xy[:, 1] - c) > a + 16
+    ut, vt = ufun(xy[far, 0], xy[far, 1])
+    err = np.hypot(r["U"][0::2][far] - ut, r["U"][1::2][far] - vt)
+    assert np.sqrt(np.mean(err ** 2)) < 0.05, np.sqrt(np.mean(err ** 2))
+    assert r["frame_ms"] > 0 and r["pcg_iters"] > 0
+    gpu_ctx.set_images(p.f, p.g)

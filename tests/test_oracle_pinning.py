@@ -631,3 +631,56 @@ def test_linear_weights_reproduce_linear_fields(pkg):
     Vb = V.copy()
     Vb[np.repeat(bad, 2)] = np.nan
     assert np.allclose(qt.scattered_fill(P, Vb, 2, bad), V, atol=1e-10)
+
+
+def test_mesh_fill_rule(orc):
+    # an interior bad node of a uniform Q4 grid sees 8 neighbours placed symmetrically,
+    # so the inverse-distance average reproduces a linear field exactly; a bad block
+    # fills from the outside in, one ring per sweep; what no sweep reaches gets 0
+    from meshes import quadtree_mesh
+    xy, el = quadtree_mesh(nc=6, refine=())
+    nn = len(xy)
+    f = 0.5 + 0.01 * xy[:, 0] - 0.03 * xy[:, 1]
+    V = np.column_stack([f, -f]).ravel()
+    inner = np.flatnonzero((xy[:, 0] > xy[:, 0].min()) & (xy[:, 0] < xy[:, 0].max()) &
+                           (xy[:, 1] > xy[:, 1].min()) & (xy[:, 1] < xy[:, 1].max()))
+    bad = np.zeros(nn, bool)
+    bad[inner[len(inner) // 2]] = True
+    out = orc.mesh_fill(xy, el, np.where(np.repeat(bad, 2), np.nan, V), 2, bad)
+    assert np.max(np.abs(out - V)) < 1e-12
+    # every node bad: nothing is known, all zero
+    allbad = np.ones(nn, bool)
+    assert np.all(orc.mesh_fill(xy, el, V, 2, allbad) == 0.0)
+    # a single good corner node and one sweep: only its neighbours get its value
+    one = np.ones(nn, bool)
+    one[0] = False
+    out = orc.mesh_fill(xy, el, V, 2, one, sweeps=1).reshape(-1, 2)
+    nb = {int(v) for e in el if 0 in e for v in e if v >= 0} - {0}
+    for i in range(nn):
+        if i == 0 or i in nb:
+            assert np.allclose(out[i], V.reshape(-1, 2)[0])
+        else:
+            assert np.all(out[i] == 0.0)
+
+
+def test_quadtree_hole_mesh_is_conforming(speckle):
+    # every element edge holds at most one other node, at its midpoint, recorded as the
+    # element's midside node; the Dirichlet nodes are the ones inside the hole; windows
+    # are twice the smallest incident cell
+    xy, el, d, ws = speckle.quadtree_hole_mesh(384, 384, 192.5, 192.5, 60.0, margin=48,
+                                               band_near=24, band_mid=60)
+    P = {tuple(p): i for i, p in enumerate(xy)}
+    for e in el:
+        for k, (a, b) in enumerate(((1, 2), (2, 3), (3, 0), (0, 1))):
+            A, B = xy[e[a]], xy[e[b]]
+            L = int(round(np.abs(B - A).max()))
+            inner = [(t, P[tuple(A + (B - A) * t / L)]) for t in range(1, L)
+                     if tuple(A + (B - A) * t / L) in P]
+            assert len(inner) <= 1
+            if inner:
+                assert inner[0][0] * 2 == L and inner[0][1] == e[4 + k]
+            else:
+                assert e[4 + k] == -1
+    r = np.hypot(xy[:, 0] - 192.5, xy[:, 1] - 192.5)
+    assert np.array_equal(d, np.flatnonzero(r < 60.0))
+    assert set(np.unique(ws)) <= {16, 32, 64} and np.any(el[:, 4:] >= 0)
