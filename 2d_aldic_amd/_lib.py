@@ -296,11 +296,12 @@ class Context:
         F = np.empty(4 * n)
         it6 = np.empty(n, np.int32)
         it2 = np.empty(n, np.int32)
-        t = np.empty(4)
+        t = np.empty(6)
         self._ck(self.L.aldic_qframe_get(self.h, _p(U), _p(F), _p(it6), _p(it2), _p(t)),
                  "aldic_qframe_get")
         return dict(U=U, F=F, iters6=it6, iters2=it2, frame_ms=t[0], local_ms=t[1],
-                    global_ms=t[2], pcg_iters=int(t[3]))
+                    global_ms=t[2], pcg_iters=int(t[3]), pcg_budget=int(t[4]),
+                    pcg_reruns=int(t[5]))
 
     def nodal_strain(self, U, swap_fix=False):
         F = np.empty(4 * self.n_nodes)

@@ -112,6 +112,11 @@ struct QFrameState {
     std::vector<int> boff, bn, bppl;  // 6-DOF launches: list offset, length and pixel class
     int pcg_iters = 0;                // PCG iterations of the last frame's global steps
     int cur_F2 = 0;                   // which of F2a / F2b holds the last F
+    int pcg_budget = 64;              // PCG iterations enqueued per global step unpolled
+    int pcg_reruns = 0;               // frames re-run with polled PCG
+    int pcg_most = 0;                 // most PCG iterations of one global step, last frame
+    DBuf pstat;                       // {converged, iterations, residual} per global step
+    double *hstat = nullptr;          // pinned host copy
     double t_frame = 0, t_local = 0, t_global = 0;  // ms of the last frame
     hipEvent_t ev[4] = {};
 };
@@ -245,8 +250,11 @@ hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const
 hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st);
 hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, const double *F,
                    const double *udual, const double *vdual, double *b, hipStream_t st);
+// stats_dev (nullable): the device copy of {converged, iterations, residual} of the last
+// iteration kernel (no host wait).
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
-                   DBuf *work, double *stats_host, hipStream_t st, int poll);
+                   DBuf *work, double *stats_host, hipStream_t st, int poll,
+                   double *stats_dev = nullptr);
 // Solves the assembled system: fast diagonalisation (direct, FP64 GEMMs) on structured
 // grids, Jacobi-PCG otherwise.  stats_host (nullable) = {iterations (0 = direct),
 // relative residual estimate}.

@@ -282,8 +282,10 @@ void aldic_destroy(aldic_ctx *ctx)
     QFrameState &QF = ctx->qf;
     for (DBuf *bb : {&QF.ws, &QF.U0, &QF.U6, &QF.F6, &QF.H21, &QF.it6, &QF.U1, &QF.F1, &QF.U2,
                      &QF.F2a, &QF.F2b, &QF.udual, &QF.vdual, &QF.it2, &QF.bad, &QF.part,
-                     &QF.known, &QF.tri, &QF.w, &QF.Fgp, &QF.lists, &QF.rl, &QF.b, &QF.cc})
+                     &QF.known, &QF.tri, &QF.w, &QF.Fgp, &QF.lists, &QF.rl, &QF.b, &QF.cc,
+                     &QF.pstat})
         bb->release();
+    if (QF.hstat) (void)hipHostFree(QF.hstat);
     for (auto &e : QF.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : F.ev) (void)hipEventDestroy(e);
@@ -1381,6 +1383,10 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
     CK(Q.known.ensure(sizeof(int) * 2 * (size_t)S));
     CK(Q.rl.ensure(sizeof(int) * ((size_t)S + 1)));
     CK(Q.part.ensure(sizeof(double) * 3 * 256));
+    CK(Q.pstat.ensure(sizeof(double) * 3 * (size_t)(opts->n_admm + 1)));
+    if (Q.hstat) (void)hipHostFree(Q.hstat);
+    Q.hstat = nullptr;
+    CK(hipHostMalloc((void **)&Q.hstat, sizeof(double) * 3 * (size_t)(opts->n_admm + 1)));
     for (auto &e : Q.ev)
         if (!e) CK(hipEventCreate(&e));
     CK(aldic::quad_assemble(q, opts->beta, opts->mu, st));
@@ -1391,7 +1397,9 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
     return ALDIC_OK;
 }
 
-int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
+// One quadtree frame.  polled = false: every PCG gets pcg_budget iteration kernels and
+// no host wait, its status goes to pstat; true: PCG polls every 8 iterations.
+static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
 {
     if (!ctx) return ALDIC_ERR_ARG;
     QFrameState &Q = ctx->qf;
@@ -1454,14 +1462,15 @@ int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
     int cur = 0;
     const double *F1p = F1;
     static thread_local DBuf work[8];
-    int pcg_total = 0;
+    int nglob = 0;
     float t_glob = 0.f;
     auto global_step = [&]() -> int {
         CK(hipEventRecord(Q.ev[2], st));
         CK(aldic::quad_rhs(q, o.beta, o.mu, U1, F1p, Q.b.as<double>(), st));
-        double stt[2] = {0, 0};
-        CK(aldic::fem_pcg(q.m, Q.b.as<double>(), U2, o.cg_rtol, o.cg_maxit, work, stt, st, 8));
-        pcg_total += (int)stt[0];
+        double *rec = Q.pstat.as<double>() + 3 * nglob++;
+        CK(aldic::fem_pcg(q.m, Q.b.as<double>(), U2, o.cg_rtol,
+                          polled ? o.cg_maxit : std::min(o.cg_maxit, Q.pcg_budget), work, nullptr,
+                          st, polled ? 8 : 0, rec));
         CK(aldic::quad_finish(q, U1, U2, st));
         CK(aldic::quad_gp_strain(q, U2, Q.Fgp.as<double>(), nullptr, st));
         CK(aldic::quad_gp_gather(S, Q.tri.as<int>(), Q.w.as<double>(), Q.Fgp.as<double>(),
@@ -1471,10 +1480,12 @@ int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
         CK(launch_sub(F2buf[cur], F1p, Q.udual.as<double>(), 4 * (long)S, st));
         CK(launch_sub(U2, U1, Q.vdual.as<double>(), 2 * (long)S, st));
         CK(hipEventRecord(Q.ev[3], st));
-        CK(hipEventSynchronize(Q.ev[3]));
-        float ms = 0.f;
-        CK(hipEventElapsedTime(&ms, Q.ev[2], Q.ev[3]));
-        t_glob += ms;
+        if (polled) {
+            CK(hipEventSynchronize(Q.ev[3]));
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, Q.ev[2], Q.ev[3]));
+            t_glob += ms;
+        }
         return ALDIC_OK;
     };
     int r = global_step();
@@ -1518,14 +1529,42 @@ int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
         if ((r = global_step())) return r;
     }
     Q.cur_F2 = cur;
+    // one wait per frame: the PCG records
+    CK(hipMemcpyAsync(Q.hstat, Q.pstat.p, 3 * sizeof(double) * nglob, hipMemcpyDeviceToHost, st));
+    CK(hipEventSynchronize(Q.ev[3]));
+    CK(hipStreamSynchronize(st));
     float ms_frame = 0.f, ms_local = 0.f;
     CK(hipEventElapsedTime(&ms_frame, Q.ev[0], Q.ev[3]));
     CK(hipEventElapsedTime(&ms_local, Q.ev[0], Q.ev[1]));
     Q.t_frame = ms_frame;
     Q.t_local = ms_local;
-    Q.t_global = t_glob;
-    Q.pcg_iters = pcg_total;
-    return ALDIC_OK;
+    Q.t_global = polled ? t_glob : NAN;
+    int total = 0, most = 0;
+    bool ok = true;
+    for (int k = 0; k < nglob; ++k) {
+        ok = ok && Q.hstat[3 * k] != 0.0;
+        total += (int)Q.hstat[3 * k + 1];
+        most = std::max(most, (int)Q.hstat[3 * k + 1]);
+    }
+    Q.pcg_iters = total;
+    Q.pcg_most = most;
+    return ok ? ALDIC_OK : -1000;  // a PCG ran out of its budget
+}
+
+int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    QFrameState &Q = ctx->qf;
+    int r = qframe_pass(ctx, U0, false);
+    if (r == -1000) {
+        // a global step needed more than the budget: run the frame again with polled PCGs
+        // and size the budget for the next frames from its iteration counts
+        ++Q.pcg_reruns;
+        r = qframe_pass(ctx, U0, true);
+        if (r == -1000) return fail(ctx, ALDIC_ERR_NUMERIC, "qframe_run: PCG did not converge");
+    }
+    if (r == ALDIC_OK) Q.pcg_budget = std::min(Q.o.cg_maxit, std::max(16, Q.pcg_most + 8));
+    return r;
 }
 
 int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *iters2,
@@ -1546,6 +1585,8 @@ int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *ite
         times[1] = Q.t_local;
         times[2] = Q.t_global;
         times[3] = Q.pcg_iters;
+        times[4] = Q.pcg_budget;
+        times[5] = Q.pcg_reruns;
     }
     return ALDIC_OK;
 }

@@ -579,7 +579,7 @@ int pcg_blocks(int nn)
 // poll == 0: enqueue exactly maxit iterations (kernels after convergence return at
 // once), no host synchronisation -- usable inside stream capture.
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
-                   DBuf *work, double *stats_host, hipStream_t st, int poll)
+                   DBuf *work, double *stats_host, hipStream_t st, int poll, double *stats_dev)
 {
     const int nn = m.nn;
     const int nb = pcg_blocks(nn);
@@ -619,6 +619,10 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
         if (done != 0.0) break;
     }
+    if (stats_dev &&
+        (e = hipMemcpyAsync(stats_dev, P.stat, 3 * sizeof(double), hipMemcpyDeviceToDevice, st)) !=
+            hipSuccess)
+        return e;
     if (stats_host) {
         // status of the last spmv launch: {done, iterations, relative residual}
         double hs[3];

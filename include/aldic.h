@@ -35,6 +35,7 @@ extern "C" {
 #define ALDIC_ERR_ARG -1   /* invalid argument (sizes, NULL, unsupported window) */
 #define ALDIC_ERR_HIP -2   /* HIP runtime error */
 #define ALDIC_ERR_STATE -3 /* call order (e.g. no images or mesh set) */
+#define ALDIC_ERR_NUMERIC -4 /* an iterative solve did not converge within its limit */
 
 enum aldic_dtype { ALDIC_U8 = 0, ALDIC_U16 = 1, ALDIC_F32 = 2, ALDIC_F64 = 3 };
 
@@ -247,8 +248,11 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
 int aldic_qframe_run(aldic_ctx *ctx, const double *U0);
 /* Results of the last frame, any may be NULL: U (2 nn) and F (4 nn) after the last
  * global step, iters6 of LocalICGN, iters2 of the last Subpb1, times = {frame ms,
- * LocalICGN + bad-point step ms, global steps ms (Subpb2 + strain + duals, all steps),
- * PCG iterations (all steps)}. */
+ * LocalICGN + bad-point step ms, global steps ms (Subpb2 + strain + duals, all steps;
+ * NaN unless the frame ran with polled PCGs), PCG iterations (all steps), PCG iteration
+ * kernels enqueued per global step without a host wait (sized from the previous frame),
+ * frames re-run with polled PCGs because a step needed more}.  The frame waits on the
+ * host once, at its end; a PCG that did not converge in its budget re-runs the frame. */
 int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *iters2,
                      double *times);
 

@@ -755,4 +755,10 @@ def test_qframe_plate_with_hole_matches_oracle(gpu_ctx, speckle, orc):
     err = np.hypot(r["U"][0::2][far] - ut, r["U"][1::2][far] - vt)
     assert np.sqrt(np.mean(err ** 2)) < 0.05, np.sqrt(np.mean(err ** 2))
     assert r["frame_ms"] > 0 and r["pcg_iters"] > 0
+    # a second frame runs its PCGs on the budget the first one set, with no host wait,
+    # and gives the same result
+    gpu_ctx.qframe_run(U0)
+    r2 = gpu_ctx.qframe_get()
+    assert r2["pcg_budget"] >= 16
+    assert np.max(np.abs(r2["U"] - r["U"])) <= 1e-9 * np.abs(r["U"]).max()
     gpu_ctx.set_images(p.f, p.g)
