@@ -251,10 +251,10 @@ hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStr
 hipError_t fem_rhs(const MeshDev &m, double beta, double mu, const double *U, const double *F,
                    const double *udual, const double *vdual, double *b, hipStream_t st);
 // stats_dev (nullable): the device copy of {converged, iterations, residual} of the last
-// iteration kernel (no host wait).
+// iteration kernel (no host wait).  warm: x holds the initial guess on entry (else 0).
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
                    DBuf *work, double *stats_host, hipStream_t st, int poll,
-                   double *stats_dev = nullptr);
+                   double *stats_dev = nullptr, bool warm = false);
 // Solves the assembled system: fast diagonalisation (direct, FP64 GEMMs) on structured
 // grids, Jacobi-PCG otherwise.  stats_host (nullable) = {iterations (0 = direct),
 // relative residual estimate}.
@@ -272,6 +272,8 @@ hipError_t quad_assemble(QuadDev &q, double beta, double mu, hipStream_t st);
 hipError_t quad_rhs(QuadDev &q, double beta, double mu, const double *U, const double *F,
                     double *b, hipStream_t st);
 hipError_t quad_finish(QuadDev &q, const double *U, double *x, hipStream_t st);
+// x = U at the free nodes, 0 at the fixed ones (a warm start for the quadtree PCG)
+hipError_t quad_start(const QuadDev &q, const double *U, double *x, hipStream_t st);
 // Gauss-point strains of the quadtree mesh (midside nodes dropped): Fgp 4E x [u_x u_y v_x
 // v_y], gpxy (nullable) 4E x 2; and node values by given linear-interpolation weights.
 hipError_t quad_gp_strain(const QuadDev &q, const double *U, double *Fgp, double *gpxy,

@@ -1468,9 +1468,11 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
         CK(hipEventRecord(Q.ev[2], st));
         CK(aldic::quad_rhs(q, o.beta, o.mu, U1, F1p, Q.b.as<double>(), st));
         double *rec = Q.pstat.as<double>() + 3 * nglob++;
+        // warm start from Subpb1's U (the global step moves it little)
+        CK(aldic::quad_start(q, U1, U2, st));
         CK(aldic::fem_pcg(q.m, Q.b.as<double>(), U2, o.cg_rtol,
                           polled ? o.cg_maxit : std::min(o.cg_maxit, Q.pcg_budget), work, nullptr,
-                          st, polled ? 8 : 0, rec));
+                          st, polled ? 8 : 0, rec, true));
         CK(aldic::quad_finish(q, U1, U2, st));
         CK(aldic::quad_gp_strain(q, U2, Q.Fgp.as<double>(), nullptr, st));
         CK(aldic::quad_gp_gather(S, Q.tri.as<int>(), Q.w.as<double>(), Q.Fgp.as<double>(),

@@ -214,6 +214,46 @@ __global__ __launch_bounds__(256) void k_pcg_init(int nn, const double2 *b, cons
     if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
 }
 
+// Warm start: x holds x0; r = b - A x0, z = D^-1 r, p = z; |b|^2 stays the reference of
+// the relative-residual stop.
+__global__ __launch_bounds__(256) void k_pcg_init_warm(int nn, const int *rowptr, const int *col,
+                                                       const double *aval, const double2 *b,
+                                                       const double *diag, const double2 *x,
+                                                       double2 *r, double2 *z, double2 *p,
+                                                       PcgPart P)
+{
+    __shared__ double red[8];
+    double rz = 0.0, bb = 0.0, rr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        double au = 0.0, av = 0.0;
+        for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+            const double2 xc = x[col[q]];
+            au += aval[q] * xc.x;
+            av += aval[q] * xc.y;
+        }
+        const double2 bi = b[i];
+        const double2 ri = make_double2(bi.x - au, bi.y - av);
+        const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
+        const double2 zi = make_double2(ri.x * dinv, ri.y * dinv);
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        rz += ri.x * zi.x + ri.y * zi.y;
+        rr += ri.x * ri.x + ri.y * ri.y;
+        bb += bi.x * bi.x + bi.y * bi.y;
+    }
+    block_reduce2(rz, rr, red);
+    __syncthreads();
+    double dummy = 0.0;
+    block_reduce2(bb, dummy, red);
+    if (threadIdx.x == 0) {
+        P.rz0[blockIdx.x] = rz;
+        P.rr0[blockIdx.x] = rr;
+        P.bb[blockIdx.x] = bb;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
+}
+
 __global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, const int *col,
                                                   const double *aval, const double2 *z,
                                                   const double2 *pold, double2 *pnew,
@@ -225,8 +265,8 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, con
     const double *rzk = (k & 1) ? P.rz1 : P.rz0, *rzm = (k & 1) ? P.rz0 : P.rz1;
     const double *rrk = (k & 1) ? P.rr1 : P.rr0;
     double s[4];
-    const double *ps[4] = {rzk, rzm, k == 0 ? P.bb : rrk, P.bb};
-    const int cnt[4] = {P.nrz, P.nrz, k == 0 ? P.nb : P.nrr, P.nb};
+    const double *ps[4] = {rzk, rzm, rrk, P.bb};  // both inits write rr0 (= b.b when cold)
+    const int cnt[4] = {P.nrz, P.nrz, P.nrr, P.nb};
     part_sums<4>(ps, cnt, s);
     const double rr = s[2], bb = s[3];
     // stop at the tolerance, on b == 0, on a NaN residual, or on a breakdown
@@ -579,7 +619,8 @@ int pcg_blocks(int nn)
 // poll == 0: enqueue exactly maxit iterations (kernels after convergence return at
 // once), no host synchronisation -- usable inside stream capture.
 hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, int maxit,
-                   DBuf *work, double *stats_host, hipStream_t st, int poll, double *stats_dev)
+                   DBuf *work, double *stats_host, hipStream_t st, int poll, double *stats_dev,
+                   bool warm)
 {
     const int nn = m.nn;
     const int nb = pcg_blocks(nn);
@@ -595,8 +636,14 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
     P.rtol2 = rtol * rtol;
     double2 *r = work[0].as<double2>(), *z = work[1].as<double2>(), *pA = work[2].as<double2>(),
             *pB = work[3].as<double2>(), *Ap = work[4].as<double2>();
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, (const double2 *)b,
-                       m.diag.as<double>(), (double2 *)x, r, z, pA, P);
+    if (warm)
+        hipLaunchKernelGGL(k_pcg_init_warm, dim3(nb), dim3(kPcgThreads), 0, st, nn,
+                           m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(),
+                           (const double2 *)b, m.diag.as<double>(), (const double2 *)x, r, z, pA,
+                           P);
+    else
+        hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn,
+                           (const double2 *)b, m.diag.as<double>(), (double2 *)x, r, z, pA, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {

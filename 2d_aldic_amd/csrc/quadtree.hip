@@ -324,6 +324,12 @@ __global__ __launch_bounds__(256) void k_fill_rest(int nn, const int *known, dou
     if (F) F[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
+__global__ __launch_bounds__(256) void k_start(int nn, const int *code, const double2 *U, double2 *x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) x[i] = code[i] == 0 ? U[i] : make_double2(0.0, 0.0);
+}
+
 __global__ __launch_bounds__(256) void k_copy_dir(int nn, const int *code, const double4 *src,
                                                   double4 *x)
 {
@@ -397,6 +403,14 @@ hipError_t launch_meshfill(const MeshDev &m, const int *bad, double *U, double *
     }
     hipLaunchKernelGGL(k_fill_rest, dim3(nb), dim3(256), 0, st, nn, (const int *)k0,
                        reinterpret_cast<double2 *>(U), reinterpret_cast<double4 *>(F));
+    return hipGetLastError();
+}
+
+hipError_t quad_start(const QuadDev &q, const double *U, double *x, hipStream_t st)
+{
+    const int nn = q.m.nn;
+    hipLaunchKernelGGL(k_start, dim3((nn + 255) / 256), dim3(256), 0, st, nn, q.code.as<int>(),
+                       reinterpret_cast<const double2 *>(U), reinterpret_cast<double2 *>(x));
     return hipGetLastError();
 }
 
