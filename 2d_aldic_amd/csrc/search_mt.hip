@@ -89,7 +89,7 @@ bool mt_geom(int ws, int sx, int sy, int dig, MTGeom &G)
     G.off_T = 0;
     G.off_W = r16(dig * G.tplane);
     G.off_P = r16(G.off_W + dig * G.wplane);
-    G.off_N = G.off_P + 32 * 32 * 8;
+    G.off_N = G.off_P;  // (no P tile in LDS: lags are evaluated in the MFMA layout)
     G.total = r16(G.off_N + 16 * 8);
     return G.total <= 64 * 1024;
 }
@@ -268,16 +268,16 @@ __device__ __forceinline__ void tile_gemm(const uint8_t *Tl, const uint8_t *Wl, 
     }
 }
 
-// num and den2 (NCC = num / sqrt(den2)) of lag (lx, ly); P its correlation sum.
+// num and den2 (NCC = num / sqrt(den2)) of lag (lx, ly); P its correlation sum, sw / sw2
+// the window box sums of the lag's full footprint (box-sum planes at (ly, lx)).
 template <class T>
-__device__ __forceinline__ void mt_terms(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
-                                         const T *g, int nx, int ny, int Z, int lx, int ly,
-                                         long long P, double &num, double &den2)
+__device__ __forceinline__ void mt_terms_sw(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
+                                            const T *g, int nx, int ny, int Z, int lx, int ly,
+                                            long long P, long long sw, long long sw2, double &num,
+                                            double &den2)
 {
     const bool top = ly == 0, bot = ly == G.nry - 1, lft = lx == 0, rgt = lx == G.nrx - 1;
     const int ye = s.y0 + ly, xe = s.x0 + lx;
-    const size_t o = (size_t)(ye + 1) * bp.pitch + (xe + 1);
-    long long sw = bp.bs1[o], sw2 = bp.bs2[o];
     const int yr = top ? s.y0 : s.y0 + G.Dy + 1, xr = lft ? s.x0 : s.x0 + G.Dx + 1;
     if (top || bot) {
         const size_t orow = (size_t)(yr + 1) * bp.pitch + (xe + 1);
@@ -307,15 +307,59 @@ __device__ __forceinline__ void mt_terms(const MTStats &s, const MTGeom &G, cons
     den2 = (double)s.stn * (var > 0.0 ? var : 0.0);
 }
 
+template <class T>
+__device__ __forceinline__ void mt_terms(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
+                                         const T *g, int nx, int ny, int Z, int lx, int ly,
+                                         long long P, double &num, double &den2)
+{
+    const size_t o = (size_t)(s.y0 + ly + 1) * bp.pitch + (s.x0 + lx + 1);
+    mt_terms_sw(s, G, bp, g, nx, ny, Z, lx, ly, P, (long long)bp.bs1[o], bp.bs2[o], num, den2);
+}
+
+// Division-free argmax over one tile's lags straight from the accumulators: lane l's
+// register r is lag (ox + 8 (r / 4) + 4 (l / 32) + r % 4, oy + l % 32).  The lane's 16
+// box sums are loaded before any is used so their latencies overlap.
+template <int DIG, class T>
+__device__ __forceinline__ void tile_argmax(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
+                                            const T *g, int nx, int ny, int Z, int ox, int oy,
+                                            int lq, int h, const v16i &acc0, const v16i &acc1,
+                                            const v16i &acc2, double &bk, double &bd, int &bi)
+{
+    const int ly = oy + lq;
+    const int lyc = ly < G.nry ? ly : G.nry - 1;
+    const size_t rowo = (size_t)(s.y0 + lyc + 1) * bp.pitch + (s.x0 + 1);
+    int sw1[16];
+    long long sw2[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int lx = min(ox + 8 * (r >> 2) + 4 * h + (r & 3), G.nrx - 1);
+        sw1[r] = bp.bs1[rowo + lx];
+        sw2[r] = bp.bs2[rowo + lx];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int lx = ox + 8 * (r >> 2) + 4 * h + (r & 3);
+        if (lx >= G.nrx || ly >= G.nry) continue;
+        const long long P = DIG == 1 ? (long long)acc0[r]
+                                     : 65536LL * acc0[r] + 256LL * acc1[r] + acc2[r];
+        double num, den2;
+        mt_terms_sw(s, G, bp, g, nx, ny, Z, lx, ly, P, (long long)sw1[r], sw2[r], num, den2);
+        double key = num * fabs(num);
+        if (!(den2 > 0.0)) { key = 0.0; den2 = 1.0; }
+        const int p = ly * G.nrx + lx;
+        const double l = key * bd, rr = bk * den2;
+        if (l > rr || (l == rr && p < bi)) { bk = key; bd = den2; bi = p; }
+    }
+}
+
 template <int DIG, int NK, class T>
-__global__ __launch_bounds__(64) void k_search_mt(SearchArgs a, MTGeom G, BoxPlanes bp,
+__global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, BoxPlanes bp,
                                                   const T *fimg, const T *gimg)
 {
     extern __shared__ __align__(16) uint8_t lds8[];
     const int lane = threadIdx.x;
     uint8_t *Tl = lds8 + G.off_T;
     uint8_t *Wl = lds8 + G.off_W;
-    long long *Pt = reinterpret_cast<long long *>(lds8 + G.off_P);
     double *N9 = reinterpret_cast<double *>(lds8 + G.off_N);
     for (int k = lane; k < G.total / 4; k += 64) reinterpret_cast<unsigned *>(lds8)[k] = 0u;
     wave_sync_lds();
@@ -351,6 +395,7 @@ __global__ __launch_bounds__(64) void k_search_mt(SearchArgs a, MTGeom G, BoxPla
         // template T(j, i) at plane[j TS + kPL + i] and its integer statistics
         long long st = 0, st2 = 0;
         int r0 = 0, rl = 0, c0 = 0, cl = 0, k00 = 0, k0l = 0, kl0 = 0, kll = 0;
+#pragma unroll 2
         for (int it = lane; it < tw * G.tgrp; it += 64) {
             const int j = it / G.tgrp, g4 = it - j * G.tgrp;
             unsigned p0, p1;
@@ -376,6 +421,7 @@ __global__ __launch_bounds__(64) void k_search_mt(SearchArgs a, MTGeom G, BoxPla
         }
         // window rows 1 .. Dy, columns 1 .. Dx (the ring and the columns past Dx stay 0):
         // window (r, c) = image (x0 + c, y0 + r)
+#pragma unroll 4
         for (int it = lane; it < G.Dy * G.wgrp; it += 64) {
             const int r = 1 + it / G.wgrp, g4 = it - (r - 1) * G.wgrp;
             unsigned p0, p1;
@@ -419,25 +465,8 @@ __global__ __launch_bounds__(64) void k_search_mt(SearchArgs a, MTGeom G, BoxPla
             for (int tx = 0; tx < G.ntx; ++tx) {
                 const int ox = 32 * tx, oy = 32 * ty;
                 tile_gemm<DIG, NK>(Tl, Wl, G, ox, oy, lq, h, b0, s3, acc0, acc1, acc2);
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const long long P = DIG == 1 ? (long long)acc0[r]
-                                                 : 65536LL * acc0[r] + 256LL * acc1[r] + acc2[r];
-                    Pt[lq * 32 + 8 * (r >> 2) + 4 * h + (r & 3)] = P;
-                }
-                wave_sync_lds();
-                for (int pl = lane; pl < 1024; pl += 64) {
-                    const int lx = ox + (pl & 31), ly = oy + (pl >> 5);
-                    if (lx >= G.nrx || ly >= G.nry) continue;
-                    double num, den2;
-                    mt_terms(s, G, bp, gimg, nx, ny, Z, lx, ly, Pt[pl], num, den2);
-                    double key = num * fabs(num);
-                    if (!(den2 > 0.0)) { key = 0.0; den2 = 1.0; }
-                    const int p = ly * G.nrx + lx;
-                    const double l = key * bd, rr = bk * den2;
-                    if (l > rr || (l == rr && p < bi)) { bk = key; bd = den2; bi = p; }
-                }
-                wave_sync_lds();
+                tile_argmax<DIG>(s, G, bp, gimg, nx, ny, Z, ox, oy, lq, h, acc0, acc1, acc2, bk,
+                                 bd, bi);
             }
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) {
@@ -449,31 +478,28 @@ __global__ __launch_bounds__(64) void k_search_mt(SearchArgs a, MTGeom G, BoxPla
         // the tile around the peak again, for the exact NCC of its 3 x 3 neighbourhood
         const int bx = bi % G.nrx, by = bi / G.nrx;
         const int ox = (max(bx - 1, 0) / 16) * 16, oy = max(by - 1, 0);
+        if (lane < 9) N9[lane] = 0.0;  // neighbours off the map stay 0
         tile_gemm<DIG, NK>(Tl, Wl, G, ox, oy, lq, h, b0, s3, acc0, acc1, acc2);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const long long P = DIG == 1 ? (long long)acc0[r]
-                                         : 65536LL * acc0[r] + 256LL * acc1[r] + acc2[r];
-            Pt[lq * 32 + 8 * (r >> 2) + 4 * h + (r & 3)] = P;
-        }
         wave_sync_lds();
-        double best = 0.0;
         {
-            const int dxk = lane % 3 - 1, dyk = lane / 3 - 1;
-            const int px = bx + dxk, py = by + dyk;
-            double c = 0.0;
-            if (lane < 9 && px >= 0 && px < G.nrx && py >= 0 && py < G.nry) {
+            // u9[(dc + 1) 3 + (dr + 1)]: dc = y-lag offset, dr = x-lag offset; the lanes
+            // holding the peak's neighbourhood write it
+            const int ly = oy + lq;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int lx = ox + 8 * (r >> 2) + 4 * h + (r & 3);
+                const int dr = lx - bx, dc = ly - by;
+                if (dr < -1 || dr > 1 || dc < -1 || dc > 1 || lx >= G.nrx || ly >= G.nry) continue;
+                const long long P = DIG == 1 ? (long long)acc0[r]
+                                             : 65536LL * acc0[r] + 256LL * acc1[r] + acc2[r];
                 double num, den2;
-                mt_terms(s, G, bp, gimg, nx, ny, Z, px, py, Pt[(py - oy) * 32 + (px - ox)], num,
-                         den2);
+                mt_terms(s, G, bp, gimg, nx, ny, Z, lx, ly, P, num, den2);
                 const double den = sqrt(den2);
-                c = den > 0.0 ? num / den : 0.0;
+                N9[(dc + 1) * 3 + (dr + 1)] = den > 0.0 ? num / den : 0.0;
             }
-            best = __shfl(c, 4, 64);
-            // u9[(dc + 1) 3 + (dr + 1)]: dc = y-lag offset, dr = x-lag offset
-            if (lane < 9) N9[(dyk + 1) * 3 + (dxk + 1)] = c;
         }
         wave_sync_lds();
+        const double best = N9[4];
         if (lane == 0) {
             const int r = bx + 1, c = by + 1;  // 1-based row = x-lag, column = y-lag
             double xp = c, yp = r, mf = best;
