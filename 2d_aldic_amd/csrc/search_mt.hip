@@ -48,7 +48,8 @@ struct MTGeom {
 
 struct BoxPlanes {
     const int *bs1, *rs1, *cs1;
-    const long long *bs2, *rs2, *cs2;
+    const long long *rs2;
+    const double *bs2, *cs2;  // exact integers (< 2^53), kept as FP64 for the lag arithmetic
     int pitch;  // nx + 2: element (ye, xe) at (ye + 1) pitch + xe + 1, ye in [-1, ny], xe in [-1, nx]
 };
 
@@ -125,8 +126,7 @@ __global__ __launch_bounds__(256) void k_box_rows(const T *g, int nx, int ny, in
 template <class T>
 __global__ __launch_bounds__(256) void k_box_cols(const T *g, int nx, int ny, int tw, int Z,
                                                   const int *rs1, const long long *rs2, int *cs1,
-                                                  long long *cs2, int *bs1, long long *bs2,
-                                                  int pitch)
+                                                  double *cs2, int *bs1, double *bs2, int pitch)
 {
     const int xe = (int)(blockIdx.x * 256 + threadIdx.x) - 1;
     if (xe > nx) return;
@@ -148,9 +148,9 @@ __global__ __launch_bounds__(256) void k_box_cols(const T *g, int nx, int ny, in
     for (int ye = y0; ye < y1; ++ye) {
         const size_t o = (size_t)(ye + 1) * pitch + xe + 1;
         cs1[o] = c1;
-        cs2[o] = c2;
+        cs2[o] = (double)c2;
         bs1[o] = b1;
-        bs2[o] = b2;
+        bs2[o] = (double)b2;
         // slide down one row: drop ye, take ye + tw
         const int yi = ye + tw;
         const int wo = (xin && ye >= 0 && ye < ny) ? (int)g[(size_t)ye * nx + xe] - Z : 0;
@@ -171,6 +171,7 @@ __global__ __launch_bounds__(256) void k_box_cols(const T *g, int nx, int ny, in
 
 struct MTStats {
     long long st, st2, stn, n;
+    double nd, std, stnd;  // n, st, stn as FP64 (exact)
     int r0, rl, c0, cl, k00, k0l, kl0, kll;  // |.| <= 2^16 * 66
     double d;
     int x0, y0;  // 0-based image position of window element (0, 0) (the zero ring's corner)
@@ -270,29 +271,32 @@ __device__ __forceinline__ void tile_gemm(const uint8_t *Tl, const uint8_t *Wl, 
 
 // num and den2 (NCC = num / sqrt(den2)) of lag (lx, ly); P its correlation sum, sw / sw2
 // the window box sums of the lag's full footprint (box-sum planes at (ly, lx)).
-template <class T>
+template <bool WIDE, class T>
 __device__ __forceinline__ void mt_terms_sw(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
                                             const T *g, int nx, int ny, int Z, int lx, int ly,
-                                            long long P, long long sw, long long sw2, double &num,
+                                            double P, double sw, double sw2, double &num,
                                             double &den2)
 {
+    // P, SW, SW2 and the statistics are integers below 2^53; so are n P, ST SW, n SW2 and
+    // SW^2 unless WIDE (16-bit pixels and n^2 32896^2 >= 2^53, templates above 53 x 53),
+    // so FP64 products of them are exact; WIDE forms those in int64
     const bool top = ly == 0, bot = ly == G.nry - 1, lft = lx == 0, rgt = lx == G.nrx - 1;
     const int ye = s.y0 + ly, xe = s.x0 + lx;
     const int yr = top ? s.y0 : s.y0 + G.Dy + 1, xr = lft ? s.x0 : s.x0 + G.Dx + 1;
     if (top || bot) {
         const size_t orow = (size_t)(yr + 1) * bp.pitch + (xe + 1);
-        sw -= bp.rs1[orow];
-        sw2 -= bp.rs2[orow];
+        sw -= (double)bp.rs1[orow];
+        sw2 -= (double)bp.rs2[orow];
     }
     if (lft || rgt) {
         const size_t ocol = (size_t)(ye + 1) * bp.pitch + (xr + 1);
-        sw -= bp.cs1[ocol];
+        sw -= (double)bp.cs1[ocol];
         sw2 -= bp.cs2[ocol];
     }
     const int ro = (top || bot) ? 1 : 0, co = (lft || rgt) ? 1 : 0;
     if (ro && co) {  // the corner pixel was taken off twice
-        const long long cv =
-            (yr >= 0 && yr < ny && xr >= 0 && xr < nx) ? (long long)g[(size_t)yr * nx + xr] - Z : 0;
+        const double cv =
+            (yr >= 0 && yr < ny && xr >= 0 && xr < nx) ? (double)((int)g[(size_t)yr * nx + xr] - Z) : 0.0;
         sw += cv;
         sw2 += cv * cv;
     }
@@ -301,25 +305,34 @@ __device__ __forceinline__ void mt_terms_sw(const MTStats &s, const MTGeom &G, c
     const long long kc = (ro && co) ? (top ? (lft ? s.k00 : s.k0l) : (lft ? s.kl0 : s.kll)) : 0;
     const long long stin = s.st - rs - cs + kc;
     const long long nin = (long long)(G.tw - ro) * (G.tw - co), nout = s.n - nin;
-    num = (double)(s.n * P - s.st * sw) + s.d * (double)(s.n * stin - nin * s.st);
-    const double var = (double)(s.n * sw2 - sw * sw) + 2.0 * s.d * (double)sw * (double)nout +
-                       s.d * s.d * (double)nin * (double)nout;
-    den2 = (double)s.stn * (var > 0.0 ? var : 0.0);
+    double t1, t2;
+    if constexpr (WIDE) {
+        const long long Pi = (long long)P, swi = (long long)sw, sw2i = (long long)sw2;
+        t1 = (double)(s.n * Pi - s.st * swi);
+        t2 = (double)(s.n * sw2i - swi * swi);
+    } else {
+        const double a1 = s.nd * P, a2 = s.std * sw, b1 = s.nd * sw2, b2 = sw * sw;
+        t1 = a1 - a2;
+        t2 = b1 - b2;
+    }
+    num = t1 + s.d * (double)(s.n * stin - nin * s.st);
+    const double var = t2 + 2.0 * s.d * sw * (double)nout + s.d * s.d * (double)nin * (double)nout;
+    den2 = s.stnd * (var > 0.0 ? var : 0.0);
 }
 
-template <class T>
+template <bool WIDE, class T>
 __device__ __forceinline__ void mt_terms(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
                                          const T *g, int nx, int ny, int Z, int lx, int ly,
-                                         long long P, double &num, double &den2)
+                                         double P, double &num, double &den2)
 {
     const size_t o = (size_t)(s.y0 + ly + 1) * bp.pitch + (s.x0 + lx + 1);
-    mt_terms_sw(s, G, bp, g, nx, ny, Z, lx, ly, P, (long long)bp.bs1[o], bp.bs2[o], num, den2);
+    mt_terms_sw<WIDE>(s, G, bp, g, nx, ny, Z, lx, ly, P, (double)bp.bs1[o], bp.bs2[o], num, den2);
 }
 
 // Division-free argmax over one tile's lags straight from the accumulators: lane l's
 // register r is lag (ox + 8 (r / 4) + 4 (l / 32) + r % 4, oy + l % 32).  The lane's 16
 // box sums are loaded before any is used so their latencies overlap.
-template <int DIG, class T>
+template <int DIG, bool WIDE, class T>
 __device__ __forceinline__ void tile_argmax(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
                                             const T *g, int nx, int ny, int Z, int ox, int oy,
                                             int lq, int h, const v16i &acc0, const v16i &acc1,
@@ -329,7 +342,7 @@ __device__ __forceinline__ void tile_argmax(const MTStats &s, const MTGeom &G, c
     const int lyc = ly < G.nry ? ly : G.nry - 1;
     const size_t rowo = (size_t)(s.y0 + lyc + 1) * bp.pitch + (s.x0 + 1);
     int sw1[16];
-    long long sw2[16];
+    double sw2[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int lx = min(ox + 8 * (r >> 2) + 4 * h + (r & 3), G.nrx - 1);
@@ -340,10 +353,11 @@ __device__ __forceinline__ void tile_argmax(const MTStats &s, const MTGeom &G, c
     for (int r = 0; r < 16; ++r) {
         const int lx = ox + 8 * (r >> 2) + 4 * h + (r & 3);
         if (lx >= G.nrx || ly >= G.nry) continue;
-        const long long P = DIG == 1 ? (long long)acc0[r]
-                                     : 65536LL * acc0[r] + 256LL * acc1[r] + acc2[r];
+        const double P = DIG == 1 ? (double)acc0[r]
+                                  : fma(65536.0, (double)acc0[r],
+                                        fma(256.0, (double)acc1[r], (double)acc2[r]));
         double num, den2;
-        mt_terms_sw(s, G, bp, g, nx, ny, Z, lx, ly, P, (long long)sw1[r], sw2[r], num, den2);
+        mt_terms_sw<WIDE>(s, G, bp, g, nx, ny, Z, lx, ly, P, (double)sw1[r], sw2[r], num, den2);
         double key = num * fabs(num);
         if (!(den2 > 0.0)) { key = 0.0; den2 = 1.0; }
         const int p = ly * G.nrx + lx;
@@ -352,7 +366,7 @@ __device__ __forceinline__ void tile_argmax(const MTStats &s, const MTGeom &G, c
     }
 }
 
-template <int DIG, int NK, class T>
+template <int DIG, int NK, bool WIDE, class T>
 __global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, BoxPlanes bp,
                                                   const T *fimg, const T *gimg)
 {
@@ -442,6 +456,9 @@ __global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, Box
         s.kl0 = (int)wave_sum_ll(kl0);
         s.kll = (int)wave_sum_ll(kll);
         s.stn = s.n * s.st2 - s.st * s.st;
+        s.nd = (double)s.n;
+        s.std = (double)s.st;
+        s.stnd = (double)s.stn;
         wave_sync_lds();
 
         const double xc = (ii + ii + a.ws) / 2.0, yc = (jj + jj + a.ws) / 2.0;
@@ -465,7 +482,7 @@ __global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, Box
             for (int tx = 0; tx < G.ntx; ++tx) {
                 const int ox = 32 * tx, oy = 32 * ty;
                 tile_gemm<DIG, NK>(Tl, Wl, G, ox, oy, lq, h, b0, s3, acc0, acc1, acc2);
-                tile_argmax<DIG>(s, G, bp, gimg, nx, ny, Z, ox, oy, lq, h, acc0, acc1, acc2, bk,
+                tile_argmax<DIG, WIDE>(s, G, bp, gimg, nx, ny, Z, ox, oy, lq, h, acc0, acc1, acc2, bk,
                                  bd, bi);
             }
 #pragma unroll
@@ -490,10 +507,11 @@ __global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, Box
                 const int lx = ox + 8 * (r >> 2) + 4 * h + (r & 3);
                 const int dr = lx - bx, dc = ly - by;
                 if (dr < -1 || dr > 1 || dc < -1 || dc > 1 || lx >= G.nrx || ly >= G.nry) continue;
-                const long long P = DIG == 1 ? (long long)acc0[r]
-                                             : 65536LL * acc0[r] + 256LL * acc1[r] + acc2[r];
+                const double P = DIG == 1 ? (double)acc0[r]
+                                          : fma(65536.0, (double)acc0[r],
+                                                fma(256.0, (double)acc1[r], (double)acc2[r]));
                 double num, den2;
-                mt_terms(s, G, bp, gimg, nx, ny, Z, lx, ly, P, num, den2);
+                mt_terms<WIDE>(s, G, bp, gimg, nx, ny, Z, lx, ly, P, num, den2);
                 const double den = sqrt(den2);
                 N9[(dc + 1) * 3 + (dr + 1)] = den > 0.0 ? num / den : 0.0;
             }
@@ -527,7 +545,8 @@ hipError_t box_planes(const T *g, int nx, int ny, int tw, int Z, DBuf &buf, BoxP
     hipError_t e;
     if ((e = buf.ensure(np * (3 * sizeof(int) + 3 * sizeof(long long)))) != hipSuccess) return e;
     long long *l = buf.as<long long>();
-    long long *rs2 = l, *cs2 = l + np, *bs2 = l + 2 * np;
+    long long *rs2 = l;
+    double *cs2 = reinterpret_cast<double *>(l + np), *bs2 = reinterpret_cast<double *>(l + 2 * np);
     int *i = reinterpret_cast<int *>(l + 3 * np);
     int *rs1 = i, *cs1 = i + np, *bs1 = i + 2 * np;
     hipLaunchKernelGGL(k_box_rows<T>, dim3((pitch + 255) / 256, ny + 2), dim3(256), 0, st, g, nx,
@@ -545,11 +564,11 @@ hipError_t box_planes(const T *g, int nx, int ny, int tw, int Z, DBuf &buf, BoxP
     return hipGetLastError();
 }
 
-template <int DIG, int NK, class T>
+template <int DIG, int NK, bool WIDE, class T>
 hipError_t launch_mt(const SearchArgs &a, const MTGeom &G, const BoxPlanes &bp, const T *f,
                      const T *g, int S, hipStream_t st)
 {
-    auto kern = k_search_mt<DIG, NK, T>;
+    auto kern = k_search_mt<DIG, NK, WIDE, T>;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -566,10 +585,15 @@ template <int DIG, class T>
 hipError_t launch_dig(const SearchArgs &a, const MTGeom &G, const BoxPlanes &bp, const T *f,
                       const T *g, int S, hipStream_t st)
 {
+    // 16-bit templates from 54 x 54 up: n P and n SW2 may pass 2^53, formed in int64
+    const double n = (double)G.tw * G.tw;
+    const bool wide = DIG == 2 && n * n * 32896.0 * 32896.0 >= 9007199254740992.0;
     switch (G.nk) {
-    case 1: return launch_mt<DIG, 1>(a, G, bp, f, g, S, st);
-    case 2: return launch_mt<DIG, 2>(a, G, bp, f, g, S, st);
-    default: return launch_mt<DIG, 3>(a, G, bp, f, g, S, st);
+    case 1: return launch_mt<DIG, 1, false>(a, G, bp, f, g, S, st);
+    case 2: return launch_mt<DIG, 2, false>(a, G, bp, f, g, S, st);
+    default:
+        return wide ? launch_mt<DIG, 3, true>(a, G, bp, f, g, S, st)
+                    : launch_mt<DIG, 3, false>(a, G, bp, f, g, S, st);
     }
 }
 

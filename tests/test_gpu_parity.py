@@ -132,12 +132,13 @@ def test_search_8bit_cases(gpu_ctx, speckle, orc, case):
 
 
 @pytest.mark.parametrize("case", ["u12_ws20_s8", "u12_ws32_s24", "u16_full_range_ws16_s12",
-                                  "u12_flat_and_image_edges"])
+                                  "u12_flat_and_image_edges", "u16_full_range_ws56_s4"])
 def test_search_16bit_cases(gpu_ctx, speckle, orc, case):
     """16-bit images: two int8 digit planes per image, four int8 GEMMs per lag tile, the
     window box sums from int64 image planes.  12-bit speckle (the 4096^2 workload's
     format) at radius 8 and 24 (51 x 51 lags, 2 x 2 tiles), full 16-bit range, and
-    constant templates with subsets against the image border."""
+    constant templates with subsets against the image border; a 57 x 57 template of full-range
+    16-bit pixels, whose n P and n SW2 pass 2^53 (formed in int64 there)."""
     _flush(f"search 16-bit case {case}")
     n, ws, sx, sy, step, u0, v0, bits = 208, 20, 8, 8, 12, 5.3, -6.6, 12
     if case == "u12_ws32_s24":
@@ -146,6 +147,8 @@ def test_search_16bit_cases(gpu_ctx, speckle, orc, case):
         ws, sx, sy, u0, v0, bits = 16, 12, 12, -10.6, 3.1, 16
     elif case == "u12_flat_and_image_edges":
         ws, sx, sy, step, u0, v0 = 24, 6, 6, 8, 2.2, 1.4
+    elif case == "u16_full_range_ws56_s4":
+        n, ws, sx, sy, step, u0, v0, bits = 224, 56, 4, 4, 16, 2.6, -1.3, 16
     p = speckle.small_pair(n=n, u0=u0, v0=v0, ux=0.002, vy=-0.001, bits=bits, seed=47)
     fr, gr = p.f.copy(), p.g.copy()
     assert fr.dtype == np.uint16
