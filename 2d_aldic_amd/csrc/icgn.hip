@@ -253,6 +253,63 @@ __device__ __forceinline__ void icgn_pixel(const uint2 e, const char *const (&ro
     }
 }
 
+// The same split in stages for an explicit software pipeline: icgn_addr warps the slot's
+// offset and returns the byte offset of its first tap row (and the fractional parts),
+// icgn_taps loads the 16 taps, icgn_accum interpolates and accumulates.
+__device__ __forceinline__ unsigned icgn_addr(const uint2 e, int nx, float p0, float p1, float p2,
+                                              float p3, float p4, float p5, float &dx, float &dy,
+                                              float &tx, float &ty)
+{
+    const __half2 h = *reinterpret_cast<const __half2 *>(&e.x);
+    dx = __low2float(h);
+    dy = __high2float(h);
+    const float ux = p0 * dx + p2 * dy + p4;
+    const float uy = p1 * dx + p3 * dy + p5;
+    tx = __builtin_amdgcn_fractf(ux);
+    ty = __builtin_amdgcn_fractf(uy);
+    return 4u * (unsigned)((int)e.y + (int)floorf(ux) + __mul24((int)floorf(uy), nx));
+}
+
+__device__ __forceinline__ void icgn_taps(const char *const (&rows)[4], unsigned ob, float (&v)[16])
+{
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const float *row = reinterpret_cast<const float *>(rows[b] + ob);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[4 * b + i] = row[i];
+    }
+}
+
+template <bool SIX, int NS>
+__device__ __forceinline__ void icgn_accum(const float (&v)[16], float tx, float ty, float dx,
+                                           float dy, float fn, float fx, float fy, bool in,
+                                           float (&acc)[NS])
+{
+    const float tx2 = tx * tx, tx3 = tx2 * tx, ty2 = ty * ty, ty3 = ty2 * ty;
+    const float wx0 = -tx + 2.f * tx2 - tx3, wx1 = 2.f - 5.f * tx2 + 3.f * tx3;
+    const float wx2 = tx + 4.f * tx2 - 3.f * tx3, wx3 = tx3 - tx2;
+    const float wy0 = -ty + 2.f * ty2 - ty3, wy1 = 2.f - 5.f * ty2 + 3.f * ty3;
+    const float wy2 = ty + 4.f * ty2 - 3.f * ty3, wy3 = ty3 - ty2;
+    float r[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        r[b] = wx0 * v[4 * b] + wx1 * v[4 * b + 1] + wx2 * v[4 * b + 2] + wx3 * v[4 * b + 3];
+    float gv = 0.25f * (wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3]);
+    gv = in ? gv : 0.f;
+    const float t1 = gv * fx, t2 = gv * fy;
+    acc[0] += gv;
+    acc[1] += gv * gv;
+    if constexpr (SIX) {
+        acc[2] += t1 * dx; acc[3] += t2 * dx;
+        acc[4] += t1 * dy; acc[5] += t2 * dy;
+        acc[6] += t1; acc[7] += t2;
+        acc[8] += gv * fn;
+    } else {
+        acc[2] += t1; acc[3] += t2;
+        acc[4] += gv * fn;
+    }
+}
+
 // Shared body for both modes.  SIX = true: 6-DOF funICGN; false: 2-DOF funICGN_Subpb1.
 template <int PPL, int WPS, bool SIX>
 __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
@@ -531,15 +588,26 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
             const char *gb = reinterpret_cast<const char *>(a.g) - 4 * (1 + (size_t)nx);
             const char *const rows[4] = {gb, gb + 4 * (size_t)nx, gb + 8 * (size_t)nx,
                                          gb + 12 * (size_t)nx};
-            if (SIX && n > (PPL - 1) * T) {
+            if (n > (PPL - 1) * T) {
                 // every slot holds a pixel on some lane (a 33 x 33 window in 18 slots): one
-                // branch-free run over the slots, so loads of later pixels can be issued
-                // ahead; only the last slot's empty lanes are masked out.  (The 2-DOF
-                // variant compiles this run into serialised loads and keeps the branches.)
+                // branch-free run over the slots, software-pipelined by hand: the taps of
+                // slot k + 1 are in flight while slot k is interpolated; only the last
+                // slot's empty lanes are masked out
+                float v[2][16], sdx[2], sdy[2], stx[2], sty[2];
+                unsigned ob = icgn_addr(tab[tid], nx, p0, p1, p2, p3, p4, p5, sdx[0], sdy[0],
+                                        stx[0], sty[0]);
+                icgn_taps(rows, ob, v[0]);
 #pragma unroll
-                for (int k = 0; k < PPL; ++k)
-                    icgn_pixel<SIX, NS>(tab[k * T + tid], rows, nx, p0, p1, p2, p3, p4, p5, fv[k],
-                                        fxv[k], fyv[k], k < PPL - 1 || k * T < lim, acc);
+                for (int k = 0; k < PPL; ++k) {
+                    const int c = k & 1, m = c ^ 1;
+                    if (k + 1 < PPL) {
+                        ob = icgn_addr(tab[(k + 1) * T + tid], nx, p0, p1, p2, p3, p4, p5, sdx[m],
+                                       sdy[m], stx[m], sty[m]);
+                        icgn_taps(rows, ob, v[m]);
+                    }
+                    icgn_accum<SIX, NS>(v[c], stx[c], sty[c], sdx[c], sdy[c], fv[k], fxv[k],
+                                        fyv[k], k < PPL - 1 || k * T < lim, acc);
+                }
             } else {
 #pragma unroll
                 for (int k = 0; k < PPL; ++k)
