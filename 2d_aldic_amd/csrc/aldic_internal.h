@@ -213,7 +213,9 @@ hipError_t launch_icgn6(const IcgnArgs &a, int ppl, hipStream_t st);
 hipError_t launch_icgn2(const IcgnArgs &a, int ppl, hipStream_t st);
 int icgn_ppl_for(int ws);   // pixels per lane class that holds a (ws+1)^2 window, 0 if none
 // list = indices j (ascending) with it[j] == code, *count = their number; one workgroup
-hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count, hipStream_t st);
+// list: int[S], count: int[1], scratch: int[(S + 4095) / 4096]
+hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count, int *scratch,
+                          hipStream_t st);
 
 struct SearchArgs {
     const float *f, *g;

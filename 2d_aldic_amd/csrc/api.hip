@@ -981,7 +981,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(F.bad.ensure(sizeof(int) * S));
     CK(F.part.ensure(3 * 256 * sizeof(double)));
     CK(F.cc.ensure(8 * sizeof(double) * S));
-    CK(F.rl.ensure(sizeof(int) * (S + 1)));
+    CK(F.rl.ensure(sizeof(int) * ((size_t)S + 1 + (S + 4095) / 4096)));  // list, count, run counts
     // gridfits per frame: Subpb1's refit per ADMM step, and with smoothing two per field
     // smoothed (F1; F2 of every step; U1, U2 of every ADMM step)
     F.gf_cap = 4 + 8 * (o->n_admm + 1);
@@ -1124,7 +1124,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     IcgnArgs ar = a;
     if (relaunch) {
         int *rl = F.rl.as<int>();
-        CK(launch_list_eq(F.it6.as<int>(), S, -2, rl, rl + S, st));
+        CK(launch_list_eq(F.it6.as<int>(), S, -2, rl, rl + S, rl + S + 1, st));
         ar.list = rl;
         ar.nlist = S;
         ar.nlist_dev = rl + S;
@@ -1381,7 +1381,7 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
     CK(Q.it2.ensure(sizeof(int) * (size_t)S));
     CK(Q.bad.ensure(sizeof(int) * (size_t)S));
     CK(Q.known.ensure(sizeof(int) * 2 * (size_t)S));
-    CK(Q.rl.ensure(sizeof(int) * ((size_t)S + 1)));
+    CK(Q.rl.ensure(sizeof(int) * ((size_t)S + 1 + (S + 4095) / 4096)));  // list, count, run counts
     CK(Q.part.ensure(sizeof(double) * 3 * 256));
     CK(Q.pstat.ensure(sizeof(double) * 3 * (size_t)(opts->n_admm + 1)));
     if (Q.hstat) (void)hipHostFree(Q.hstat);
@@ -1446,7 +1446,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     {
         IcgnArgs ar = a;
         int *rl = Q.rl.as<int>();
-        CK(launch_list_eq(Q.it6.as<int>(), S, -2, rl, rl + S, st));
+        CK(launch_list_eq(Q.it6.as<int>(), S, -2, rl, rl + S, rl + S + 1, st));
         ar.list = rl;
         ar.nlist = S;
         ar.nlist_dev = rl + S;
@@ -1517,7 +1517,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
         {
             IcgnArgs ar2 = a2;
             int *rl = Q.rl.as<int>();
-            CK(launch_list_eq(Q.it2.as<int>(), S, -2, rl, rl + S, st));
+            CK(launch_list_eq(Q.it2.as<int>(), S, -2, rl, rl + S, rl + S + 1, st));
             ar2.list = rl;
             ar2.nlist = S;
             ar2.nlist_dev = rl + S;
@@ -1565,7 +1565,7 @@ int aldic_qframe_run(aldic_ctx *ctx, const double *U0)
         r = qframe_pass(ctx, U0, true);
         if (r == -1000) return fail(ctx, ALDIC_ERR_NUMERIC, "qframe_run: PCG did not converge");
     }
-    if (r == ALDIC_OK) Q.pcg_budget = std::min(Q.o.cg_maxit, std::max(16, Q.pcg_most + 8));
+    if (r == ALDIC_OK) Q.pcg_budget = std::min(Q.o.cg_maxit, std::max(12, Q.pcg_most + 4));
     return r;
 }
 

@@ -694,55 +694,73 @@ __global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
     }
 }
 
-// Indices j with it[j] == code, in increasing order, and their count (one workgroup):
-// each wave owns a contiguous run, counts it with ballots, then writes at its offset.
-__global__ __launch_bounds__(1024) void k_list_eq(const int *it, int S, int code, int *list,
-                                                  int *count)
+// Indices j with it[j] == code, in increasing order, and their count.  Two passes over
+// runs of 4096 statuses per workgroup: k_list_cnt counts each run with ballots;
+// k_list_put sums the counts of the runs before its own (every workgroup reads the
+// same array) and writes its matches in order.
+constexpr int kListRun = 4096;
+
+__global__ __launch_bounds__(256) void k_list_cnt(const int *it, int S, int code, int *cnt)
 {
-    __shared__ int wc[16];
+    __shared__ int wc[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int per = ((S + 15) / 16 + 511) & ~511;
-    const int b0 = min(S, w * per), b1 = min(S, b0 + per);
+    const int b0 = blockIdx.x * kListRun, b1 = min(S, b0 + kListRun);
     int c = 0;
-    for (int base = b0; base < b1; base += 512) {  // 8 loads in flight per lane
-        int v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int i = base + 64 * k + lane;
-            v[k] = i < b1 ? it[i] : code + 1;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c += __popcll(__ballot(v[k] == code));
-    }
+    for (int i = b0 + w * 64 + lane; i - lane < b1; i += 256)
+        c += __popcll(__ballot(i < b1 && it[i] == code));
     if (lane == 0) wc[w] = c;
     __syncthreads();
-    int o = 0;
-    for (int k = 0; k < w; ++k) o += wc[k];
-    for (int base = b0; base < b1; base += 512) {
-        int v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int i = base + 64 * k + lane;
-            v[k] = i < b1 ? it[i] : code + 1;
+    if (threadIdx.x == 0) cnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(256) void k_list_put(const int *it, int S, int code, const int *cnt,
+                                                  int nrun, int *list, int *count)
+{
+    __shared__ int wc[4];
+    __shared__ int base;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x < 64) {
+        int sb = 0, st = 0;
+        for (int k = threadIdx.x; k < nrun; k += 64) {
+            const int v = cnt[k];
+            if (k < (int)blockIdx.x) sb += v;
+            st += v;
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const bool m = v[k] == code;
-            const unsigned long long bal = __ballot(m);
-            if (m) list[o + __popcll(bal & ((1ull << lane) - 1ull))] = base + 64 * k + lane;
-            o += __popcll(bal);
+        for (int m = 32; m >= 1; m >>= 1) {
+            sb += __shfl_xor(sb, m, 64);
+            st += __shfl_xor(st, m, 64);
+        }
+        if (threadIdx.x == 0) {
+            base = sb;
+            if (blockIdx.x == 0) *count = st;
         }
     }
-    if (threadIdx.x == 0) {
-        int tot = 0;
-        for (int k = 0; k < 16; ++k) tot += wc[k];
-        *count = tot;
+    // each wave takes a contiguous quarter of the run
+    const int b0 = blockIdx.x * kListRun, b1 = min(S, b0 + kListRun);
+    const int q = kListRun / 4, w0 = min(b1, b0 + w * q), w1 = min(b1, w0 + q);
+    int c = 0;
+    for (int i = w0 + lane; i - lane < w1; i += 64) c += __popcll(__ballot(i < w1 && it[i] == code));
+    if (lane == 0) wc[w] = c;
+    __syncthreads();
+    int o = base;
+    for (int k = 0; k < w; ++k) o += wc[k];
+    for (int i = w0 + lane; i - lane < w1; i += 64) {
+        const bool m = i < w1 && it[i] == code;
+        const unsigned long long bal = __ballot(m);
+        if (m) list[o + __popcll(bal & ((1ull << lane) - 1ull))] = i;
+        o += __popcll(bal);
     }
 }
 
-hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count, hipStream_t st)
+// scratch: int[(S + 4095) / 4096] of run counts
+hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count, int *scratch,
+                          hipStream_t st)
 {
-    hipLaunchKernelGGL(k_list_eq, dim3(1), dim3(1024), 0, st, it, S, code, list, count);
+    const int nrun = (S + kListRun - 1) / kListRun;
+    if (nrun <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_list_cnt, dim3(nrun), dim3(256), 0, st, it, S, code, scratch);
+    hipLaunchKernelGGL(k_list_put, dim3(nrun), dim3(256), 0, st, it, S, code,
+                       (const int *)scratch, nrun, list, count);
     return hipGetLastError();
 }
 
