@@ -52,6 +52,45 @@ def copy_calibration(nbytes):
     log(f"[bench] copy calibration: {nbytes} bytes device-to-device")
 
 
+def measure_copy_bw(nbytes=1 << 30, reps=5):
+    """Device-to-device copy bandwidth (read + write bytes / time) over `reps` copies of
+    nbytes, timed with HIP events: the measured HBM ceiling beside the nominal 8 TB/s."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        return None
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)
+    hip.hipDeviceSynchronize()
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float(0.0)
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return round(2.0 * nbytes * reps / (ms.value / 1e3) / 1e9, 1) if ms.value > 0 else None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,7 +201,8 @@ def main():
                 frac=round(icgn_bytes / t_i / 1e9 / HBM_PEAK_GBS, 5),
                 traffic=measured("k_icgn_6dof"), algorithmic_bytes=icgn_bytes,
                 avg_ms=round(icgn6_ms, 5),
-                note="VALU- and latency-bound (FP32 Keys gathers and sums, 66 VALU per pixel pass)")
+                note="VALU- and latency-bound (FP32 Keys gathers and sums, 66 VALU per pixel pass)",
+                copy_gbs_measured=measure_copy_bw())
     # The integer search beside it: int8 matrix cores, algorithmic ops = the NCC
     # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
@@ -203,7 +243,7 @@ def main():
         itc = o["iters6"]
         convc = int(np.sum((itc >= 1) & (itc <= 100)))
         cpu = dict(value=round(convc / o["times"]["frame"], 2), unit="converged subsets/s",
-                   cores=threads, kind="port",
+                   cores=threads, kind="port", cpu_model=cpu_model(),
                    sample=f"one full frame of the same workload ({S} subsets), FP64 oracle, "
                           f"{o['times']['frame']:.2f} s",
                    stage_s={k: round(v, 4) for k, v in o["times"].items()})
