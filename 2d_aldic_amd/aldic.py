@@ -143,8 +143,9 @@ def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winste
 
 def IntegerSearch(ImgRef, ImgDef, file_name, DICpara):
     """IntegerSearch.m:1-167 without the prompts: the search radius comes from
-    DICpara['SizeOfFFTSearchRegion']; masked nodes become NaN (:68-76).  The
-    interactive outlier removal of :113-114 is not part of this call."""
+    DICpara['SizeOfFFTSearchRegion']; masked nodes become NaN (:68-76); the median test
+    of funRemoveOutliers(u, v, cc, 0.5, 100) runs (:113-114) but not the interactive
+    cleanup that follows it."""
     s = DICpara["SizeOfFFTSearchRegion"]
     roi = DICpara["gridxyROIRange"]
     x0, y0, u, v, cc = funIntegerSearch(ImgRef, ImgDef, s, roi["gridx"], roi["gridy"],
@@ -156,6 +157,7 @@ def IntegerSearch(ImgRef, ImgDef, file_name, DICpara):
         hole = np.asarray(mask)[xi, yi] == 0
         u = np.where(hole, np.nan, u)
         v = np.where(hole, np.nan, v)
+    u, v, cc, _, _, _ = funRemoveOutliers(u, v, cc, 0.5, 100)
     DICpara = dict(DICpara)
     DICpara["InitFFTSearchMethod"] = 1
     return DICpara, x0, y0, u, v, cc
@@ -217,6 +219,108 @@ def inpaint_nans_springs(A):
     flat = flat.copy()
     flat[unk] = sol
     return flat.reshape(n, m)
+
+
+def inpaint_nans_plate(A):
+    """inpaint_nans.m method 3: the del^4 (biharmonic) operator on every NaN node and the
+    known nodes its 13-point template reaches, reduced near the border (5-point Laplacian
+    on the second ring, second differences along the first row/column, nothing at the
+    corners); the known values are moved to the right-hand side and the NaNs solved in
+    least squares."""
+    A = np.array(A, dtype=np.float64)
+    nan = np.isnan(A)
+    if not nan.any():
+        return A
+    n, m = A.shape
+    talks = [(-2, 0), (-1, -1), (-1, 0), (-1, 1), (0, -2), (0, -1), (0, 1), (0, 2), (1, -1),
+             (1, 0), (1, 1), (2, 0)]
+    nodes = set(zip(*np.nonzero(nan)))
+    for (i, j) in list(nodes):
+        for di, dj in talks:
+            if 0 <= i + di < n and 0 <= j + dj < m:
+                nodes.add((i + di, j + dj))
+    d4 = [((0, -2), 1), ((-1, -1), 2), ((0, -1), -8), ((1, -1), 2), ((-2, 0), 1), ((-1, 0), -8),
+          ((0, 0), 20), ((1, 0), -8), ((2, 0), 1), ((-1, 1), 2), ((0, 1), -8), ((1, 1), 2),
+          ((0, 2), 1)]
+    lap = [((0, -1), 1), ((-1, 0), 1), ((0, 0), -4), ((1, 0), 1), ((0, 1), 1)]
+    rows = []
+    for (i, j) in sorted(nodes):  # 1-based r = i + 1, c = j + 1
+        r, c = i + 1, j + 1
+        if 3 <= r <= n - 2 and 3 <= c <= m - 2:
+            st = d4
+        elif ((r in (2, n - 1)) and 2 <= c <= m - 1) or ((c in (2, m - 1)) and 2 <= r <= n - 1):
+            st = lap
+        elif r in (1, n) and 2 <= c <= m - 1:
+            st = [((0, -1), 1), ((0, 0), -2), ((0, 1), 1)]
+        elif c in (1, m) and 2 <= r <= n - 1:
+            st = [((-1, 0), 1), ((0, 0), -2), ((1, 0), 1)]
+        else:
+            continue
+        rows.append([((i + di, j + dj), w) for (di, dj), w in st])
+    unk = sorted(zip(*np.nonzero(nan)))
+    col = {p: k for k, p in enumerate(unk)}
+    M = np.zeros((len(rows), len(unk)))
+    rhs = np.zeros(len(rows))
+    for k, st in enumerate(rows):
+        for p, w in st:
+            if p in col:
+                M[k, col[p]] += w
+            else:
+                rhs[k] -= w * A[p]
+    keep = np.any(M != 0.0, axis=1)
+    sol = np.linalg.lstsq(M[keep], rhs[keep], rcond=None)[0]
+    for p, k in col.items():
+        A[p] = sol[k]
+    return A
+
+
+def _medfilt2_3(V):
+    """medFilt2(V, [2 2], 'symmetric', 2) as funRemoveOutliers.m:381-418 calls it: padding
+    by one mirrored row and column after the array, the 2 x 2 neighbourhood without its
+    second member, i.e. the median of V(i, j), V(i+1, j), V(i+1, j+1), taken in single
+    precision."""
+    P = np.pad(np.asarray(V, np.float64), ((0, 1), (0, 1)), mode="symmetric")
+    st = np.sort(np.stack([P[:-1, :-1], P[1:, :-1], P[1:, 1:]]).astype(np.float32), axis=0)
+    return st[1].astype(np.float64)
+
+
+def funMedRemoveOutliers(u, epsilon=0.1):
+    """funRemoveOutliers.m:371-379: normalised median fluctuation of u (its NaNs in-painted
+    with method 3 first)."""
+    u = inpaint_nans_plate(u)
+    med = _medfilt2_3(u)
+    fluct = u - med
+    res = _medfilt2_3(np.abs(fluct))
+    return med, np.abs(fluct / (res + epsilon))
+
+
+def funRemoveOutliers(u, v, cc, qDICOrNot, Thr0, *varargin):
+    """funRemoveOutliers.m:1-140 without its prompts: the normalised median test of u and v
+    (epsilon 0.1, funRemoveOutliers.m:30-35) at threshold Thr0, the flagged nodes and the
+    NaNs in-painted by springs (method 4), then the original NaNs put back (:56).  The
+    q-factor removal (qDICOrNot == 1) and the interactive bound / click cleanup that
+    follow when qDICOrNot != 0 are not provided; Thr0 must be given."""
+    if qDICOrNot == 1:
+        raise NotImplementedError("q-factor based removal (bimodal fit) is not provided")
+    if not Thr0:
+        raise NotImplementedError("the threshold prompt is interactive; pass Thr0")
+    u = np.array(u, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    hole = np.isnan(u)
+    _, nfu = funMedRemoveOutliers(u)
+    _, nfv = funMedRemoveOutliers(v)
+    mag = np.sqrt(nfu ** 2 + nfv ** 2)
+    flagged = mag > Thr0
+    u2 = u.copy()
+    v2 = v.copy()
+    u2[flagged] = np.nan
+    v2[flagged] = np.nan
+    u2 = inpaint_nans_springs(u2)
+    v2 = inpaint_nans_springs(v2)
+    u2[hole] = np.nan
+    v2[hole] = np.nan
+    lst = np.flatnonzero(flagged.ravel(order="F")) + 1  # MATLAB linear indices
+    return u2, v2, cc, [], [], lst
 
 
 def Init(u, v, Phi, x, y, index=0):
@@ -419,5 +523,6 @@ def funGlobal_NodalStrainAvg(coordinatesFEM, elementsFEM, U, GaussPtOrder=2, swa
 
 __all__ = ["AldicError", "get_context", "funNormalizeImg", "funImgGradient", "ba_interp2",
            "funIntegerSearch", "IntegerSearch", "MeshSetUp", "Init", "inpaint_nans_springs",
+           "inpaint_nans_plate", "funRemoveOutliers", "funMedRemoveOutliers",
            "LocalICGN", "funICGN", "Subpb1", "funICGN_Subpb1", "Subpb2",
            "funGlobal_NodalStrainAvg"]

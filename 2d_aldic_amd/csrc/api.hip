@@ -1087,7 +1087,8 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     sa.U0 = F.U0.as<double>();
     search_raw8(ctx, sa);
     CK(launch_search(sa, st));
-    // search NaNs (constant templates) are filled like Init's inpaint_nans
+    // the search's NaNs (constant templates) go to LocalICGN as they are: their subsets
+    // end non-finite (status 101) and the bad-point refill covers them
     const int e_search = mark();
     span(1, e_prep, e_search);
 
