@@ -4,10 +4,11 @@
 //   2-DOF: funICGN_Subpb1.m:1-362 (per subset of Subpb1.m:119-152)
 //
 // Layout: WPS wavefronts work on one subset (WPS = 1 for windows up to 33 x 33,
-// 4 for larger ones); each lane keeps PPL pixels of the (w+1)^2 window in registers:
-// the normalised reference value f_n, the gradients fx, fy and the offsets dx, dy.
-// Pixel p of the window (x fastest) sits in slot k = p / (64 * WPS) of thread
-// p % (64 * WPS).  The Hessian, its inverse (FP64), mean/norm of f and the constant
+// 4 for larger ones); each lane owns PPL pixel slots of the (w+1)^2 window: pixel p
+// (x fastest) sits in slot k = p / (64 * WPS) of thread p % (64 * WPS).  A per-thread
+// LDS table holds each slot's offsets (dx, dy) and image index; every pass re-reads
+// f, fx, fy at the slot's pixel (L1/L2 hits), which keeps the kernel at 128 VGPRs and
+// four waves per SIMD to cover the gathers.  The Hessian, its inverse (FP64), mean/norm of f and the constant
 // sums sum J_k and sum J_k f_n are formed once per subset.  Each iteration then makes
 // ONE pass over the pixels: warp, Keys-cubic gather of g (16 loads through L1/L2),
 // and the nine sums  S_g, S_gg, S_{g J_k} (k = 1..6), S_{g f_n}, from which
@@ -507,17 +508,16 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     }  // !cached
     const double bottomf = sqrt(ssf);
     uint2 *tab = s_tab + sub_in_block * (PPL * T);
+    // the passes re-read f, fx, fy at each slot's pixel (L1/L2 hits) instead of holding
+    // 3 PPL registers: fewer registers, more waves in flight to cover the gathers
+    const float mfn = (float)meanf;
+    const float ibn = (float)(1.0 / bottomf);
     {
-        const float mf = (float)meanf;
-        const float ib = (float)(1.0 / bottomf);
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) fv[k] = (fv[k] - mf) * ib;  // f_n (0 in empty slots)
         const float half = 0.5f * (float)w;
         int i = i0, r = r0;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
             const bool in = r < w1;
-            if (!in) fv[k] = 0.f;
             const int ii = in ? i : 0, rr = in ? r : 0;  // empty slots: an in-window pixel
             const __half2 h = __floats2half2_rn((float)ii - half, (float)rr - half);
             uint2 e;
@@ -593,27 +593,37 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
                 // branch-free run over the slots, software-pipelined by hand: the taps of
                 // slot k + 1 are in flight while slot k is interpolated; only the last
                 // slot's empty lanes are masked out
-                float v[2][16], sdx[2], sdy[2], stx[2], sty[2];
-                unsigned ob = icgn_addr(tab[tid], nx, p0, p1, p2, p3, p4, p5, sdx[0], sdy[0],
-                                        stx[0], sty[0]);
-                icgn_taps(rows, ob, v[0]);
+                float v[2][16], sdx[2], sdy[2], stx[2], sty[2], sf[2], sgx[2], sgy[2];
+                {
+                    const uint2 e = tab[tid];
+                    const unsigned ob = icgn_addr(e, nx, p0, p1, p2, p3, p4, p5, sdx[0], sdy[0],
+                                                  stx[0], sty[0]);
+                    icgn_taps(rows, ob, v[0]);
+                    sf[0] = a.f[e.y]; sgx[0] = a.gx[e.y]; sgy[0] = a.gy[e.y];
+                }
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
                     const int c = k & 1, m = c ^ 1;
                     if (k + 1 < PPL) {
-                        ob = icgn_addr(tab[(k + 1) * T + tid], nx, p0, p1, p2, p3, p4, p5, sdx[m],
-                                       sdy[m], stx[m], sty[m]);
+                        const uint2 e = tab[(k + 1) * T + tid];
+                        const unsigned ob = icgn_addr(e, nx, p0, p1, p2, p3, p4, p5, sdx[m],
+                                                      sdy[m], stx[m], sty[m]);
                         icgn_taps(rows, ob, v[m]);
+                        sf[m] = a.f[e.y]; sgx[m] = a.gx[e.y]; sgy[m] = a.gy[e.y];
                     }
-                    icgn_accum<SIX, NS>(v[c], stx[c], sty[c], sdx[c], sdy[c], fv[k], fxv[k],
-                                        fyv[k], k < PPL - 1 || k * T < lim, acc);
+                    icgn_accum<SIX, NS>(v[c], stx[c], sty[c], sdx[c], sdy[c],
+                                        (sf[c] - mfn) * ibn, sgx[c], sgy[c],
+                                        k < PPL - 1 || k * T < lim, acc);
                 }
             } else {
 #pragma unroll
                 for (int k = 0; k < PPL; ++k)
-                    if (k * T < lim)
-                        icgn_pixel<SIX, NS>(tab[k * T + tid], rows, nx, p0, p1, p2, p3, p4, p5,
-                                            fv[k], fxv[k], fyv[k], true, acc);
+                    if (k * T < lim) {
+                        const uint2 e = tab[k * T + tid];
+                        icgn_pixel<SIX, NS>(e, rows, nx, p0, p1, p2, p3, p4, p5,
+                                            (a.f[e.y] - mfn) * ibn, a.gx[e.y], a.gy[e.y], true,
+                                            acc);
+                    }
             }
         }
         block_sumN<WPS, NS>(acc, s_red, wid);
@@ -674,7 +684,7 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
 // Subset slot = blockIdx.x * SPB + wave / WPS.  LOOP: the grid is capped and steps over
 // a list whose length is only known on the device.
 template <int PPL, int WPS, bool SIX, bool LOOP>
-__global__ __launch_bounds__(256, 3) void k_icgn(IcgnArgs a)
+__global__ __launch_bounds__(256, 4) void k_icgn(IcgnArgs a)
 {
     constexpr int SPB = 4 / WPS;
     const int nlist = a.nlist_dev ? *a.nlist_dev : a.nlist;
