@@ -871,26 +871,51 @@ __global__ __launch_bounds__(256) void k_mark(const int *it, int S, const double
     }
 }
 
-// k_axisfill: thread per node; bad nodes read only good nodes, so one pass suffices.
-__global__ __launch_bounds__(256) void k_axisfill(int Nx, int Ny, const int *bad, double *U,
-                                                  double *F)
+// k_markfill: the threshold from k_itstats' partials (same order in every block), then
+// per node the bad flag and, for a bad node, the axis refill: along each of the four grid
+// directions the nearest good node (its flag recomputed from its status), weighted by
+// inverse distance.  U (and F) are refilled in place: bad nodes are written, good nodes
+// only read, so one pass suffices.
+__device__ __forceinline__ bool is_bad(int v, int rule, double C, double thr)
 {
+    return v < 0 || v > 99 || (rule && C > 0 && (double)v > thr);
+}
+
+__global__ __launch_bounds__(256) void k_markfill(const int *it, int Nx, int Ny, const double *part,
+                                                  int np, int rule, double *U, double *F, int *bad)
+{
+    __shared__ double tot[3];
+    if (threadIdx.x < 64) {
+        double a[3] = {0, 0, 0};
+        for (int k = threadIdx.x; k < np; k += 64)
+            for (int q = 0; q < 3; ++q) a[q] += part[3 * k + q];
+        for (int q = 0; q < 3; ++q) a[q] = wave_sum(a[q]);
+        if (threadIdx.x == 0) { tot[0] = a[0]; tot[1] = a[1]; tot[2] = a[2]; }
+    }
+    __syncthreads();
+    const double C = tot[0], Sm = tot[1], S2 = tot[2];
+    const double mean = C > 0 ? Sm / C : 0.0;
+    const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
+    const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nx * Ny || !bad[i]) return;
+    if (i >= Nx * Ny) return;
+    const bool b = is_bad(it[i], rule, C, thr);
+    bad[i] = b ? 1 : 0;
+    if (!b) return;
     const int x = i % Nx, y = i / Nx;
     int src[4];
     double w[4];
     int q = x - 1;
-    while (q >= 0 && bad[y * Nx + q]) --q;
+    while (q >= 0 && is_bad(it[y * Nx + q], rule, C, thr)) --q;
     src[0] = q >= 0 ? y * Nx + q : -1; w[0] = q >= 0 ? 1.0 / (x - q) : 0.0;
     q = x + 1;
-    while (q < Nx && bad[y * Nx + q]) ++q;
+    while (q < Nx && is_bad(it[y * Nx + q], rule, C, thr)) ++q;
     src[1] = q < Nx ? y * Nx + q : -1; w[1] = q < Nx ? 1.0 / (q - x) : 0.0;
     q = y - 1;
-    while (q >= 0 && bad[q * Nx + x]) --q;
+    while (q >= 0 && is_bad(it[q * Nx + x], rule, C, thr)) --q;
     src[2] = q >= 0 ? q * Nx + x : -1; w[2] = q >= 0 ? 1.0 / (y - q) : 0.0;
     q = y + 1;
-    while (q < Ny && bad[q * Nx + x]) ++q;
+    while (q < Ny && is_bad(it[q * Nx + x], rule, C, thr)) ++q;
     src[3] = q < Ny ? q * Nx + x : -1; w[3] = q < Ny ? 1.0 / (q - y) : 0.0;
     double ws = 0.0;
 #pragma unroll
@@ -912,18 +937,19 @@ __global__ __launch_bounds__(256) void k_axisfill(int Nx, int Ny, const int *bad
         }
 }
 
+// In place (U, F hold the values to refill); Usrc / Fsrc must be null.
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
                           double *part, hipStream_t st)
 {
     if (S <= 0) return hipSuccess;
+    if (Usrc || Fsrc) return hipErrorInvalidValue;
     int nb = (S + 255) / 256;
     if (nb > 256) nb = 256;
     const int nbf = (S + 255) / 256;
     hipLaunchKernelGGL(k_itstats, dim3(nb), dim3(256), 0, st, iters, S, part);
-    hipLaunchKernelGGL(k_mark, dim3(nbf), dim3(256), 0, st, iters, S, (const double *)part, nb,
-                       use_mean_rule, Usrc, Fsrc, U, F, bad);
-    hipLaunchKernelGGL(k_axisfill, dim3(nbf), dim3(256), 0, st, Nx, Ny, (const int *)bad, U, F);
+    hipLaunchKernelGGL(k_markfill, dim3(nbf), dim3(256), 0, st, iters, Nx, Ny,
+                       (const double *)part, nb, use_mean_rule, U, F, bad);
     return hipGetLastError();
 }
 
