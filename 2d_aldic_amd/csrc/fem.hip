@@ -126,24 +126,27 @@ __global__ __launch_bounds__(256) void k_rhs(int nn, const int *rowptr, const in
                         beta * (g.x * (f.y - w.y) + g.y * (f.w - w.w)) + mu * sv);
 }
 
-// ---- Jacobi-PCG ---------------------------------------------------------------------
-// Every reduction is left as per-block partials; the NEXT kernel reduces them in its
-// prologue (each block, same fixed order, so every block computes bitwise-identical
-// scalars).  Kernel boundaries make the partials visible across the per-XCD L2s, so
-// no fence or atomic is needed and the solve is deterministic.
-//   init      : x = 0, r = b, z = D^-1 r, p = z; partials r.z -> rz[0], r.r -> rr[0], b.b
-//   spmv(k)   : beta_k = rz[k]/rz[k-1]; stop when |r| <= rtol |b| (the stop is sticky:
-//               block 0 records it and every later kernel returns at once); p = z + beta
-//               p (on the fly for the neighbours), Ap, partials p.Ap
-//   update(k) : alpha = rz[k]/p.Ap; x += alpha p; r -= alpha Ap; z = D^-1 r;
-//               partials r.z -> rz[k+1], r.r -> rr[k+1]
-// Block 0 of each spmv records {done, iterations} for the host's convergence poll.
-struct PcgPart {
-    double *pap, *rz0, *rz1, *rr0, *rr1, *bb;
-    int nb;        // partials of pap and bb (node-parallel kernels)
-    int nrz, nrr;  // partials of rz and rr (equal to nb on the Jacobi path)
+// ---- Jacobi-PCG, one reduction and one kernel per iteration -------------------------
+// Chronopoulos-Gear form: with u = D^-1 r and w = A u,
+//   gamma = r.u, delta = w.u;  beta = gamma / gamma_prev,
+//   alpha = gamma / (delta - beta gamma / alpha_prev);
+//   p = u + beta p, s = w + beta s (= A p), x += alpha p, r -= alpha s.
+// Iteration i reads parity i & 1 and writes the other.  Each row forms its neighbours'
+// next residual r_j - alpha (w_j + beta s_j) from the previous kernel's vectors, so the
+// product w = A u of the next u needs no second grid-wide step; the per-block partials
+// of gamma, delta and r.r are reduced by the next kernel in every block in the same
+// order (identical scalars, deterministic).  A sticky stop flag (block 0 of the deciding
+// kernel) turns later kernels into no-ops.
+struct CgPart {
+    double *G[2], *D[2], *RR[2], *BB;  // per-block partials
+    double *hist;                      // {alpha, gamma} per parity
+    double *stat;                      // [0] done, [1] iterations, [2] relative residual
+    int nb;
     double rtol2;
-    double *stat;  // [0] done, [1] iterations, [2] relative residual
+};
+
+struct CgVec {
+    double2 *r[2], *w[2], *s[2], *p[2];
 };
 
 constexpr int kPcgThreads = 256;
@@ -157,189 +160,151 @@ __device__ __forceinline__ double part_sum(const double *p, int nb)
     return wave_sum(s);
 }
 
-template <int N>
-__device__ __forceinline__ void part_sums(const double *const (&ps)[N], const int (&cnt)[N],
-                                          double (&out)[N])
+__device__ __forceinline__ void block_reduce3(double a, double b, double c, double *pa, double *pb,
+                                              double *pc)
 {
-    __shared__ double sh[4];
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for (int base = 0; base < N; base += 4) {
-        if (base + w < N) {
-            const double v = part_sum(ps[base + w], cnt[base + w]);
-            if ((threadIdx.x & 63) == 0) sh[w] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (base + q < N) out[base + q] = sh[q];
-        __syncthreads();
-    }
-}
-
-__device__ __forceinline__ void block_reduce2(double &a, double &b, double *red)
-{
+    __shared__ double red[3][4];
     a = wave_sum(a);
     b = wave_sum(b);
-    const int wid = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[2 * wid] = a; red[2 * wid + 1] = b; }
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = a; red[1][w] = b; red[2][w] = c; }
     __syncthreads();
-    a = red[0] + red[2] + red[4] + red[6];
-    b = red[1] + red[3] + red[5] + red[7];
+    if (threadIdx.x == 0) {
+        pa[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        pb[blockIdx.x] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        pc[blockIdx.x] = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+    }
 }
 
-__global__ __launch_bounds__(256) void k_pcg_init(int nn, const double2 *b, const double *diag,
-                                                  double2 *x, double2 *r, double2 *z,
-                                                  double2 *p, PcgPart P)
+__device__ __forceinline__ double2 csr_row(int i, const int *rowptr, const int *col,
+                                           const double *aval, const double2 *v)
 {
-    __shared__ double red[8];
-    double rz = 0.0, bb = 0.0;
+    double au = 0.0, av = 0.0;
+    for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        const double2 vc = v[col[q]];
+        au += aval[q] * vc.x;
+        av += aval[q] * vc.y;
+    }
+    return make_double2(au, av);
+}
+
+// r0 = b - A x0 (x0 = 0 unless warm), s = p = 0; partial b.b
+__global__ __launch_bounds__(kPcgThreads) void k_cg_init(int nn, const int *rowptr, const int *col,
+                                                         const double *aval, const double2 *b,
+                                                         double2 *x, int warm, CgVec v, CgPart P)
+{
+    double bb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         const double2 bi = b[i];
-        const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
-        const double2 zi = make_double2(bi.x * dinv, bi.y * dinv);
-        x[i] = make_double2(0.0, 0.0);
-        r[i] = bi;
-        z[i] = zi;
-        p[i] = zi;
-        rz += bi.x * zi.x + bi.y * zi.y;
-        bb += bi.x * bi.x + bi.y * bi.y;
-    }
-    block_reduce2(rz, bb, red);
-    if (threadIdx.x == 0) {
-        P.rz0[blockIdx.x] = rz;
-        P.rr0[blockIdx.x] = bb;
-        P.bb[blockIdx.x] = bb;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
-}
-
-// Warm start: x holds x0; r = b - A x0, z = D^-1 r, p = z; |b|^2 stays the reference of
-// the relative-residual stop.
-__global__ __launch_bounds__(256) void k_pcg_init_warm(int nn, const int *rowptr, const int *col,
-                                                       const double *aval, const double2 *b,
-                                                       const double *diag, const double2 *x,
-                                                       double2 *r, double2 *z, double2 *p,
-                                                       PcgPart P)
-{
-    __shared__ double red[8];
-    double rz = 0.0, bb = 0.0, rr = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        double au = 0.0, av = 0.0;
-        for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-            const double2 xc = x[col[q]];
-            au += aval[q] * xc.x;
-            av += aval[q] * xc.y;
+        double2 ri = bi;
+        if (warm) {
+            const double2 ax = csr_row(i, rowptr, col, aval, x);
+            ri = make_double2(bi.x - ax.x, bi.y - ax.y);
+        } else {
+            x[i] = make_double2(0.0, 0.0);
         }
-        const double2 bi = b[i];
-        const double2 ri = make_double2(bi.x - au, bi.y - av);
-        const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
-        const double2 zi = make_double2(ri.x * dinv, ri.y * dinv);
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        rz += ri.x * zi.x + ri.y * zi.y;
-        rr += ri.x * ri.x + ri.y * ri.y;
+        v.r[0][i] = ri;
+        v.s[0][i] = make_double2(0.0, 0.0);
+        v.p[0][i] = make_double2(0.0, 0.0);
         bb += bi.x * bi.x + bi.y * bi.y;
     }
-    block_reduce2(rz, rr, red);
-    __syncthreads();
-    double dummy = 0.0;
-    block_reduce2(bb, dummy, red);
-    if (threadIdx.x == 0) {
-        P.rz0[blockIdx.x] = rz;
-        P.rr0[blockIdx.x] = rr;
-        P.bb[blockIdx.x] = bb;
-    }
+    block_reduce3(bb, 0.0, 0.0, P.BB, P.D[1], P.RR[1]);
     if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
 }
 
-__global__ __launch_bounds__(256) void k_pcg_spmv(int nn, const int *rowptr, const int *col,
-                                                  const double *aval, const double2 *z,
-                                                  const double2 *pold, double2 *pnew,
-                                                  double2 *Ap, PcgPart P, int k)
+// w0 = A D^-1 r0; partials gamma0, delta0, r0.r0
+__global__ __launch_bounds__(kPcgThreads) void k_cg_w0(int nn, const int *rowptr, const int *col,
+                                                       const double *aval, const double *diag,
+                                                       CgVec v, CgPart P)
 {
-    __shared__ double red[8];
-    // sticky stop: once a step has finished the solve, later partials are stale
-    if (P.stat[0] != 0.0) return;
-    const double *rzk = (k & 1) ? P.rz1 : P.rz0, *rzm = (k & 1) ? P.rz0 : P.rz1;
-    const double *rrk = (k & 1) ? P.rr1 : P.rr0;
-    double s[4];
-    const double *ps[4] = {rzk, rzm, rrk, P.bb};  // both inits write rr0 (= b.b when cold)
-    const int cnt[4] = {P.nrz, P.nrz, P.nrr, P.nb};
-    part_sums<4>(ps, cnt, s);
-    const double rr = s[2], bb = s[3];
-    // stop at the tolerance, on b == 0, on a NaN residual, or on a breakdown
-    // (r.z of the previous step zero, or a non-finite beta)
-    bool done = !(rr > P.rtol2 * bb);
-    if (k > 0 && !(s[1] != 0.0 && isfinite(s[0] / s[1]))) done = true;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        P.stat[0] = done ? 1.0 : 0.0;
-        P.stat[1] = (double)k;
-        P.stat[2] = bb > 0 ? sqrt(rr / bb) : 0.0;
-    }
-    if (done) return;
-    const bool first = k == 0;
-    const double beta = first ? 0.0 : s[0] / s[1];
-    double pap = 0.0, dummy = 0.0;
+    double g = 0.0, dl = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
         double au = 0.0, av = 0.0;
         for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
             const int c = col[q];
-            const double2 pc = pold[c];
-            const double a = aval[q];
-            if (first) {
-                au += a * pc.x;
-                av += a * pc.y;
-            } else {
-                const double2 zc = z[c];
-                au += a * (zc.x + beta * pc.x);
-                av += a * (zc.y + beta * pc.y);
-            }
+            const double dc = diag[c] != 0.0 ? 1.0 / diag[c] : 1.0;
+            const double2 rc = v.r[0][c];
+            au += aval[q] * (rc.x * dc);
+            av += aval[q] * (rc.y * dc);
         }
-        const double2 zi = z[i], pi = pold[i];
-        const double2 pn = first ? pi : make_double2(zi.x + beta * pi.x, zi.y + beta * pi.y);
-        pnew[i] = pn;
-        Ap[i] = make_double2(au, av);
-        pap += pn.x * au + pn.y * av;
-    }
-    block_reduce2(pap, dummy, red);
-    if (threadIdx.x == 0) P.pap[blockIdx.x] = pap;
-}
-
-__global__ __launch_bounds__(256) void k_pcg_update(int nn, const double *diag, const double2 *p,
-                                                    const double2 *Ap, double2 *x, double2 *r,
-                                                    double2 *z, PcgPart P, int k)
-{
-    __shared__ double red[8];
-    const double *rzk = (k & 1) ? P.rz1 : P.rz0;
-    double *rzn = (k & 1) ? P.rz0 : P.rz1, *rrn = (k & 1) ? P.rr0 : P.rr1;
-    if (P.stat[0] != 0.0) return;            // the spmv of this step finished the solve
-    double s[2];
-    const double *ps[2] = {rzk, P.pap};
-    const int cnt[2] = {P.nrz, P.nb};
-    part_sums<2>(ps, cnt, s);
-    const double alpha = s[0] / s[1];
-    if (!(s[1] > 0.0) || !isfinite(alpha)) {  // breakdown: p.Ap <= 0 -> report r = 0, stop
-        if (threadIdx.x == 0) { rzn[blockIdx.x] = 0.0; rrn[blockIdx.x] = 0.0; }
-        return;
-    }
-    double rz = 0.0, rr = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const double2 pi = p[i], api = Ap[i];
-        double2 xi = x[i], ri = r[i];
-        xi.x += alpha * pi.x; xi.y += alpha * pi.y;
-        ri.x -= alpha * api.x; ri.y -= alpha * api.y;
-        x[i] = xi;
-        r[i] = ri;
-        const double dinv = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
-        const double2 zi = make_double2(ri.x * dinv, ri.y * dinv);
-        z[i] = zi;
-        rz += ri.x * zi.x + ri.y * zi.y;
+        const double di = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
+        const double2 ri = v.r[0][i];
+        v.w[0][i] = make_double2(au, av);
+        g += ri.x * ri.x * di + ri.y * ri.y * di;
+        dl += au * ri.x * di + av * ri.y * di;
         rr += ri.x * ri.x + ri.y * ri.y;
     }
-    block_reduce2(rz, rr, red);
-    if (threadIdx.x == 0) { rzn[blockIdx.x] = rz; rrn[blockIdx.x] = rr; }
+    block_reduce3(g, dl, rr, P.G[0], P.D[0], P.RR[0]);
+}
+
+__global__ __launch_bounds__(kPcgThreads) void k_cg_it(int nn, const int *rowptr, const int *col,
+                                                       const double *aval, const double *diag,
+                                                       double2 *x, CgVec v, CgPart P, int i)
+{
+    if (P.stat[0] != 0.0) return;  // sticky stop (uniform: every thread leaves)
+    const int cur = i & 1, nxt = cur ^ 1;
+    __shared__ double sh[4];
+    {
+        const int w = threadIdx.x >> 6;
+        const double *ps[4] = {P.G[cur], P.D[cur], P.RR[cur], P.BB};
+        const double val = part_sum(ps[w], P.nb);
+        if ((threadIdx.x & 63) == 0) sh[w] = val;
+        __syncthreads();
+    }
+    const double gam = sh[0], del = sh[1], rr = sh[2], bb = sh[3];
+    double alpha, beta;
+    if (i == 0) {
+        beta = 0.0;
+        alpha = gam / del;
+    } else {
+        const double *h = P.hist + 2 * nxt;  // {alpha_{i-1}, gamma_{i-1}}
+        beta = gam / h[1];
+        alpha = gam / (del - beta * gam / h[0]);
+    }
+    // stop at the tolerance, on b == 0, on a NaN residual, or on a breakdown (the
+    // system is symmetric positive definite: a non-positive or non-finite step)
+    const bool done = !(rr > P.rtol2 * bb) || !(alpha > 0.0) || !isfinite(alpha) ||
+                      !isfinite(beta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.stat[0] = done ? 1.0 : 0.0;
+        P.stat[1] = (double)i;
+        P.stat[2] = bb > 0 ? sqrt(rr / bb) : 0.0;
+        P.hist[2 * cur] = alpha;
+        P.hist[2 * cur + 1] = gam;
+    }
+    if (done) return;
+    const double2 *rc = v.r[cur], *wc = v.w[cur], *sc = v.s[cur], *pc = v.p[cur];
+    double g = 0.0, dl = 0.0, rn2 = 0.0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += gridDim.x * blockDim.x) {
+        double2 wa = make_double2(0.0, 0.0), rnew = wa, snew = wa, unew = wa;
+        for (int k = rowptr[q]; k < rowptr[q + 1]; ++k) {
+            const int j = col[k];
+            const double2 wj = wc[j], sj = sc[j], rj = rc[j];
+            const double dj = diag[j] != 0.0 ? 1.0 / diag[j] : 1.0;
+            const double2 snj = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
+            const double2 rnj = make_double2(rj.x - alpha * snj.x, rj.y - alpha * snj.y);
+            const double2 unj = make_double2(rnj.x * dj, rnj.y * dj);
+            wa.x += aval[k] * unj.x;
+            wa.y += aval[k] * unj.y;
+            if (j == q) { rnew = rnj; snew = snj; unew = unj; }
+        }
+        const double dq = diag[q] != 0.0 ? 1.0 / diag[q] : 1.0;
+        const double2 rq = rc[q], pq = pc[q];
+        const double2 pn = make_double2(rq.x * dq + beta * pq.x, rq.y * dq + beta * pq.y);
+        double2 xq = x[q];
+        xq.x += alpha * pn.x;
+        xq.y += alpha * pn.y;
+        x[q] = xq;
+        v.p[nxt][q] = pn;
+        v.s[nxt][q] = snew;
+        v.r[nxt][q] = rnew;
+        v.w[nxt][q] = wa;
+        g += rnew.x * unew.x + rnew.y * unew.y;
+        dl += wa.x * unew.x + wa.y * unew.y;
+        rn2 += rnew.x * rnew.x + rnew.y * rnew.y;
+    }
+    block_reduce3(g, dl, rn2, P.G[nxt], P.D[nxt], P.RR[nxt]);
 }
 
 // ---- fast diagonalisation for structured grids ------------------------------------
@@ -626,37 +591,33 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
     const int nb = pcg_blocks(nn);
     const size_t vb = sizeof(double2) * (size_t)nn;
     hipError_t e;
-    for (int k = 0; k < 5; ++k)
-        if ((e = work[k].ensure(vb)) != hipSuccess) return e;
-    if ((e = work[5].ensure(sizeof(double) * (6 * (size_t)nb + 8))) != hipSuccess) return e;
+    // work[0]: r, w, s, p in two parities; work[5]: partials, history, status
+    if ((e = work[0].ensure(8 * vb)) != hipSuccess) return e;
+    if ((e = work[5].ensure(sizeof(double) * (7 * (size_t)nb + 8))) != hipSuccess) return e;
     double *pp = work[5].as<double>();
-    PcgPart P;
-    P.pap = pp; P.rz0 = pp + nb; P.rz1 = pp + 2 * nb; P.rr0 = pp + 3 * nb; P.rr1 = pp + 4 * nb;
-    P.bb = pp + 5 * nb; P.stat = pp + 6 * nb; P.nb = nb; P.nrz = nb; P.nrr = nb;
-    P.rtol2 = rtol * rtol;
-    double2 *r = work[0].as<double2>(), *z = work[1].as<double2>(), *pA = work[2].as<double2>(),
-            *pB = work[3].as<double2>(), *Ap = work[4].as<double2>();
-    if (warm)
-        hipLaunchKernelGGL(k_pcg_init_warm, dim3(nb), dim3(kPcgThreads), 0, st, nn,
-                           m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(),
-                           (const double2 *)b, m.diag.as<double>(), (const double2 *)x, r, z, pA,
-                           P);
-    else
-        hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn,
-                           (const double2 *)b, m.diag.as<double>(), (double2 *)x, r, z, pA, P);
+    CgPart P;
+    P.G[0] = pp; P.G[1] = pp + nb; P.D[0] = pp + 2 * nb; P.D[1] = pp + 3 * nb;
+    P.RR[0] = pp + 4 * nb; P.RR[1] = pp + 5 * nb; P.BB = pp + 6 * nb;
+    P.hist = pp + 7 * nb; P.stat = P.hist + 4; P.nb = nb; P.rtol2 = rtol * rtol;
+    double2 *vv = work[0].as<double2>();
+    CgVec v;
+    for (int k = 0; k < 2; ++k) {
+        v.r[k] = vv + (0 + k) * (size_t)nn;
+        v.w[k] = vv + (2 + k) * (size_t)nn;
+        v.s[k] = vv + (4 + k) * (size_t)nn;
+        v.p[k] = vv + (6 + k) * (size_t)nn;
+    }
+    const int *rp = m.rowptr.as<int>(), *cl = m.col.as<int>();
+    const double *av = m.aval.as<double>(), *dg = m.diag.as<double>();
+    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kPcgThreads), 0, st, nn, rp, cl, av,
+                       (const double2 *)b, (double2 *)x, warm ? 1 : 0, v, P);
+    hipLaunchKernelGGL(k_cg_w0, dim3(nb), dim3(kPcgThreads), 0, st, nn, rp, cl, av, dg, v, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
-        for (int q = 0; q < chunk && it + q < maxit; ++q) {
-            const int k = it + q;
-            double2 *pold = (k & 1) ? pB : pA;
-            double2 *pnew = (k & 1) ? pA : pB;
-            hipLaunchKernelGGL(k_pcg_spmv, dim3(nb), dim3(kPcgThreads), 0, st, nn,
-                               m.rowptr.as<int>(), m.col.as<int>(), m.aval.as<double>(), z, pold,
-                               pnew, Ap, P, k);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kPcgThreads), 0, st, nn,
-                               m.diag.as<double>(), pnew, Ap, (double2 *)x, r, z, P, k);
-        }
+        for (int q = 0; q < chunk && it + q < maxit; ++q)
+            hipLaunchKernelGGL(k_cg_it, dim3(nb), dim3(kPcgThreads), 0, st, nn, rp, cl, av, dg,
+                               (double2 *)x, v, P, it + q);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
         double done = 0.0;
@@ -671,7 +632,7 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
             hipSuccess)
         return e;
     if (stats_host) {
-        // status of the last spmv launch: {done, iterations, relative residual}
+        // status of the last iteration kernel: {done, iterations, relative residual}
         double hs[3];
         if ((e = hipMemcpyAsync(hs, P.stat, sizeof hs, hipMemcpyDeviceToHost, st)) != hipSuccess)
             return e;

@@ -179,7 +179,9 @@ template <int NP>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
 {
     const int cur = i & 1, nxt = cur ^ 1;
-    const double stop = c.stat[0];  // sticky: later partials are stale
+    // sticky stop: once set, later partials are stale and the kernel does nothing (the
+    // flag is uniform, so every thread leaves before the barrier)
+    if (c.stat[0] != 0.0) return;
     __shared__ double sh[4];
     {
         const int w = threadIdx.x >> 6;
@@ -188,7 +190,6 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
         if ((threadIdx.x & 63) == 0) sh[w] = v;
         __syncthreads();
     }
-    if (stop != 0.0) return;
     const double gam = sh[0], del = sh[1], rr = sh[2], bb = sh[3];
     double alpha, beta;
     if (i == 0) {
