@@ -6,6 +6,23 @@ The product (the HIP library and its Python mirror) never imports this module.
 
 Array conventions follow aldic_oracle.c: images (ny, nx) C-ordered (x fastest),
 1-based pixel coordinates, interleaved nodal vectors.
+
+Pinning of the functions restated here (tests/test_oracle_pinning.py):
+  gridfit_system/gridfit  springs energy against an explicit edge loop, |Areg|_1 closed
+                          forms, dense QR least squares, constants, coordinate scaling
+  subpb2_quadtree         equals the Q4 step without hanging nodes; patch test with
+                          hanging and Dirichlet nodes
+  gauss_strain_quadtree   bilinear fields give the exact gradient at the Gauss points,
+                          whose positions and order are checked in closed form
+  gather_weights          weight rows that select single points and linear combinations
+  badpoint                hand-counted thresholds of rules 0, 1 and 2
+  mesh_fill, axis_fill    symmetric stencils reproduce linear fields; reach per sweep;
+                          the all-bad case
+  gauss_kernel, nanconv_edge, smooth_nodal, beta_choice
+                          fspecial / scipy.ndimage equivalents, brute force, the
+                          parabola vertex
+  frame                   composed of the above; the bad-node refill (axis_fill) stands
+                          in for scatteredInterpolant 'natural': parity unpinned there
 """
 from __future__ import annotations
 

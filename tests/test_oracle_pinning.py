@@ -684,3 +684,27 @@ def test_quadtree_hole_mesh_is_conforming(speckle):
     r = np.hypot(xy[:, 0] - 192.5, xy[:, 1] - 192.5)
     assert np.array_equal(d, np.flatnonzero(r < 60.0))
     assert set(np.unique(ws)) <= {16, 32, 64} and np.any(el[:, 4:] >= 0)
+
+
+def test_badpoint_rules_by_hand(orc):
+    # statuses: two failures (-1, 101), a masked 102, the rest 3 .. 12 and one slow 40
+    it = np.array([-1, 101, 102, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 40])
+    good = it[(it >= 0) & (it <= 99)]
+    m, sd = good.mean(), good.std(ddof=1)
+    thr = m + 0.15 * sd                       # = 11.42 for these values
+    assert 11 < thr < 12
+    b0 = orc.badpoint(it, 0)
+    assert list(np.flatnonzero(b0)) == [0, 1, 2]
+    b1 = orc.badpoint(it, 1)
+    assert list(np.flatnonzero(b1)) == [0, 1, 2, 12, 13]   # 12 and 40 exceed 11.42
+    b2 = orc.badpoint(it, 2)
+    assert list(np.flatnonzero(b2)) == [0, 1, 2, 13]       # max(11.42, 20) = 20
+
+
+def test_gather_weights_selects_and_combines(orc):
+    Fgp = np.arange(20, dtype=float).reshape(5, 4)
+    tri = np.array([[0, 1, 2], [4, 4, 4], [1, 3, 2]])
+    w = np.array([[1.0, 0.0, 0.0], [0.2, 0.3, 0.5], [0.25, 0.25, 0.5]])
+    out = orc.gather_weights(Fgp, tri, w).reshape(3, 4)
+    assert np.array_equal(out[0], Fgp[0]) and np.allclose(out[1], Fgp[4])
+    assert np.allclose(out[2], 0.25 * Fgp[1] + 0.25 * Fgp[3] + 0.5 * Fgp[2])
