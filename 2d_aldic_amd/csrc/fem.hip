@@ -242,25 +242,27 @@ __global__ __launch_bounds__(kPcgThreads) void k_cg_it(int nn, const int *rowptr
                                                        const double *aval, const double *diag,
                                                        double2 *x, CgVec v, CgPart P, int i)
 {
-    if (P.stat[0] != 0.0) return;  // sticky stop (uniform: every thread leaves)
     const int cur = i & 1, nxt = cur ^ 1;
+    // stop flag, partials and history are loaded together.  Wave w sums partial array w
+    // (G, D, RR of this parity, then BB); the arrays are consecutive runs of nb words from
+    // G[0] (G0 G1 D0 D1 RR0 RR1 BB), so the offset is arithmetic, not a pointer select.
+    const double stop = P.stat[0];
+    const double h_alpha = P.hist[2 * nxt], h_gamma = P.hist[2 * nxt + 1];  // i - 1
+    const int w = threadIdx.x >> 6;
+    const double *ps = P.G[0] + (size_t)(w < 3 ? 2 * w + cur : 6) * P.nb;
+    const double val = part_sum(ps, P.nb);
+    if (stop != 0.0) return;  // sticky stop (uniform: every thread leaves)
     __shared__ double sh[4];
-    {
-        const int w = threadIdx.x >> 6;
-        const double *ps[4] = {P.G[cur], P.D[cur], P.RR[cur], P.BB};
-        const double val = part_sum(ps[w], P.nb);
-        if ((threadIdx.x & 63) == 0) sh[w] = val;
-        __syncthreads();
-    }
+    if ((threadIdx.x & 63) == 0) sh[w] = val;
+    __syncthreads();
     const double gam = sh[0], del = sh[1], rr = sh[2], bb = sh[3];
     double alpha, beta;
     if (i == 0) {
         beta = 0.0;
         alpha = gam / del;
     } else {
-        const double *h = P.hist + 2 * nxt;  // {alpha_{i-1}, gamma_{i-1}}
-        beta = gam / h[1];
-        alpha = gam / (del - beta * gam / h[0]);
+        beta = gam / h_gamma;
+        alpha = gam / (del - beta * gam / h_alpha);
     }
     // stop at the tolerance, on b == 0, on a NaN residual, or on a breakdown (the
     // system is symmetric positive definite: a non-positive or non-finite step)
@@ -407,86 +409,137 @@ static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DB
 // in LDS: workgroup (row strip of kFdRows rows, component) computes
 //   out[m, :] = ((L B)[m, :] R) (./ (mu + cK (lamy[m] + lamx[n])) when SCALE)
 // with L(m, k) = Lsrc[k * gy + m] and R(j, n) = Rsrc[j * gx + n].  B and out are the
-// interleaved gy x 2gx nodal arrays.  Every load of B and R is issued up front into
-// registers (one wave per SIMD, so each lane can hold 2 x 64 doubles): one memory
-// round trip per kernel.  B, then R, goes to LDS; the strip of L and the intermediate
-// row block are kept transposed (row index fastest); each FMA step reads one B or R
-// word and one broadcast strip word.
-constexpr int kFdRows = 2;
-constexpr int kFdMax = 128;           // gx, gy limit of the fused path
-constexpr int kFdPer = kFdMax * kFdMax / 256;  // words of a component per thread
+// interleaved gy x 2gx nodal arrays.  1024 threads: every load of B and R is issued up
+// front into registers (at most kFdPer words of each per thread), B then R goes to LDS.
+// In each product thread (column c, slice q) accumulates all kFdRows rows over the
+// q-th eighth of the inner index, so every B or R word is read from LDS once and the
+// strip words are broadcasts; the eight slices are summed in a fixed order through LDS.
+constexpr int kFdRows = 8;
+constexpr int kFdMax = 128;            // gx, gy limit of the fused path
+constexpr int kFdThreads = 1024;
+constexpr int kFdPer = kFdMax * kFdMax / kFdThreads;  // words of a component per thread
+constexpr int kFdSlices = kFdThreads / kFdMax;        // 8
+static_assert(kFdSlices == kFdRows, "the slice index doubles as the row in the sums");
+
+// sum of the slices' partials of row r, column c (fixed order)
+__device__ __forceinline__ double fd_slice_sum(const double *part, int gx, int r, int c)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < kFdSlices; ++q) v += part[(q * kFdRows + r) * gx + c];
+    return v;
+}
 
 template <bool SCALE>
-__global__ __launch_bounds__(256, 1) void k_fd_pass(int gy, int gx, const double *Lsrc,
-                                                    const double *B, const double *Rsrc,
-                                                    double *out, const double *lamy,
-                                                    const double *lamx, double mu, double cK)
+__global__ __launch_bounds__(kFdThreads) void k_fd_pass(int gy, int gx, const double *Lsrc,
+                                                        const double *B, const double *Rsrc,
+                                                        double *out, const double *lamy,
+                                                        const double *lamx, double mu, double cK)
 {
     extern __shared__ __align__(16) double sm[];
-    const int big = ((gy > gx ? gy : gx) * gx + 1) & ~1;  // keeps Lt, Tt 16-byte aligned
-    double *Ms = sm;                        // B (gy x gx), later R (gx x gx)
+    const int mx = gy > gx ? gy : gx;
+    const int big = (((mx > kFdSlices * kFdRows ? mx : kFdSlices * kFdRows) * gx) + 1) & ~1;
+    double *Ms = sm;                        // B (gy x gx), R (gx x gx), or the partials
     double *Lt = Ms + big;                  // Lt[k][r] = L(m0 + r, k), k < gy
     double *Tt = Lt + gy * kFdRows;         // Tt[j][r] = (L B)(m0 + r, j), j < gx
     const int comp = blockIdx.y, m0 = blockIdx.x * kFdRows;
     const int t = threadIdx.x;
     const int w2 = 2 * gx;
-    // every load issued before the first use: L strip word, B (component comp, walked as
-    // e = t + 256 i over the gy x gx array without divisions), R (contiguous)
+    const int nb = gy * gx, nr = gx * gx;
     double vl = 0.0;
     if (t < gy * kFdRows) {
         const int k = t / kFdRows, r = t % kFdRows;
         if (m0 + r < gy) vl = Lsrc[(long)k * gy + m0 + r];
     }
-    const int nb = gy * gx, nr = gx * gx;
-    const int dk = 256 / gx, dc = 256 % gx;
+    // B walked as e = t + 1024 i over the gy x gx array (row index clamped for the tail,
+    // whose words are not stored), R contiguous (index clamped likewise)
     double vb[kFdPer], vr[kFdPer];
     {
+        const int dk = kFdThreads / gx, dc = kFdThreads % gx;
         int k = t / gx, c = t % gx;
 #pragma unroll
         for (int i = 0; i < kFdPer; ++i) {
-            vb[i] = (t + 256 * i < nb) ? B[(long)k * w2 + 2 * c + comp] : 0.0;
+            vb[i] = B[(long)min(k, gy - 1) * w2 + 2 * c + comp];
             k += dk;
             c += dc;
             if (c >= gx) { c -= gx; ++k; }
         }
     }
 #pragma unroll
-    for (int i = 0; i < kFdPer; ++i) vr[i] = (t + 256 * i < nr) ? Rsrc[t + 256 * i] : 0.0;
+    for (int i = 0; i < kFdPer; ++i) vr[i] = Rsrc[min(t + kFdThreads * i, nr - 1)];
     if (t < gy * kFdRows) Lt[t] = vl;
 #pragma unroll
     for (int i = 0; i < kFdPer; ++i)
-        if (t + 256 * i < nb) Ms[t + 256 * i] = vb[i];
+        if (t + kFdThreads * i < nb) Ms[t + kFdThreads * i] = vb[i];
     __syncthreads();
-    // thread: column c, row h of the strip (the row is the same in a whole wave, so its
-    // strip words are LDS broadcasts)
-    const int c = t & 127, h = t >> 7;
-    if (c < gx) {
-        double acc = 0.0;
-#pragma unroll 8
-        for (int k = 0; k < gy; ++k) acc += Lt[k * kFdRows + h] * Ms[k * gx + c];
-        Tt[c * kFdRows + h] = acc;
+    const int c = t & (kFdMax - 1), q = t >> 7;
+    double acc[kFdRows];
+    // Tt = L_strip B: slice q of k
+    {
+        const int ks = (gy + kFdSlices - 1) / kFdSlices, k0 = q * ks, k1 = min(gy, k0 + ks);
+#pragma unroll
+        for (int r = 0; r < kFdRows; ++r) acc[r] = 0.0;
+        if (c < gx) {
+            for (int k = k0; k < k1; ++k) {
+                const double b = Ms[k * gx + c];
+                const double2 *l = reinterpret_cast<const double2 *>(Lt + k * kFdRows);
+#pragma unroll
+                for (int r2 = 0; r2 < kFdRows / 2; ++r2) {
+                    const double2 lv = l[r2];
+                    acc[2 * r2] += lv.x * b;
+                    acc[2 * r2 + 1] += lv.y * b;
+                }
+            }
+        }
     }
+    __syncthreads();
+    if (c < gx) {
+#pragma unroll
+        for (int r = 0; r < kFdRows; ++r) Ms[(q * kFdRows + r) * gx + c] = acc[r];
+    }
+    __syncthreads();
+    if (c < gx) Tt[c * kFdRows + q] = fd_slice_sum(Ms, gx, q, c);  // q doubles as the row
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kFdPer; ++i)
-        if (t + 256 * i < nr) Ms[t + 256 * i] = vr[i];
+        if (t + kFdThreads * i < nr) Ms[t + kFdThreads * i] = vr[i];
+    __syncthreads();
+    // out strip = Tt R: slice q of j
+    {
+        const int js = (gx + kFdSlices - 1) / kFdSlices, j0 = q * js, j1 = min(gx, j0 + js);
+#pragma unroll
+        for (int r = 0; r < kFdRows; ++r) acc[r] = 0.0;
+        if (c < gx) {
+            for (int j = j0; j < j1; ++j) {
+                const double rv = Ms[j * gx + c];
+                const double2 *tt = reinterpret_cast<const double2 *>(Tt + j * kFdRows);
+#pragma unroll
+                for (int r2 = 0; r2 < kFdRows / 2; ++r2) {
+                    const double2 tv = tt[r2];
+                    acc[2 * r2] += tv.x * rv;
+                    acc[2 * r2 + 1] += tv.y * rv;
+                }
+            }
+        }
+    }
     __syncthreads();
     if (c < gx) {
-        double acc = 0.0;
-#pragma unroll 8
-        for (int j = 0; j < gx; ++j) acc += Tt[j * kFdRows + h] * Ms[j * gx + c];
-        const int m = m0 + h;
-        if (m < gy) {
-            if (SCALE) acc /= mu + cK * (lamy[m] + lamx[c]);
-            out[(long)m * w2 + 2 * c + comp] = acc;
-        }
+#pragma unroll
+        for (int r = 0; r < kFdRows; ++r) Ms[(q * kFdRows + r) * gx + c] = acc[r];
+    }
+    __syncthreads();
+    const int m = m0 + q;
+    if (c < gx && m < gy) {
+        double v = fd_slice_sum(Ms, gx, q, c);
+        if (SCALE) v /= mu + cK * (lamy[m] + lamx[c]);
+        out[(long)m * w2 + 2 * c + comp] = v;
     }
 }
 
 static size_t fd_pass_lds(int gy, int gx)
 {
-    return sizeof(double) * ((((size_t)(gy > gx ? gy : gx) * gx + 1) & ~(size_t)1) +
-                             (size_t)(gy + gx) * kFdRows);
+    const size_t mx = std::max(std::max(gy, gx), kFdSlices * kFdRows);
+    return sizeof(double) * (((mx * gx + 1) & ~(size_t)1) + (size_t)(gy + gx) * kFdRows);
 }
 
 static hipError_t fem_fd_solve(const MeshDev &m, const double *b, double *x, DBuf *work,
@@ -511,10 +564,10 @@ static hipError_t fem_fd_solve(const MeshDev &m, const double *b, double *x, DBu
         }
         const dim3 grid((gy + kFdRows - 1) / kFdRows, 2);
         // T2 = (Vy^T B Vx) ./ D:  L(m, k) = Vy[k][m],  R(j, n) = Vx[j][n]
-        hipLaunchKernelGGL(k_fd_pass<true>, grid, dim3(256), lds, st, gy, gx, Vy, b, Vx, T2, ly,
+        hipLaunchKernelGGL(k_fd_pass<true>, grid, dim3(kFdThreads), lds, st, gy, gx, Vy, b, Vx, T2, ly,
                            lx, m.mu, m.beta + m.alpha * m.mu);
         // X = Vy T2 Vx^T:  L(m, k) = Vy[m][k] = VyT[k][m],  R(j, n) = Vx[n][j] = VxT[j][n]
-        hipLaunchKernelGGL(k_fd_pass<false>, grid, dim3(256), lds, st, gy, gx, VyT,
+        hipLaunchKernelGGL(k_fd_pass<false>, grid, dim3(kFdThreads), lds, st, gy, gx, VyT,
                            (const double *)T2, VxT, x, nullptr, nullptr, 0.0, 0.0);
         return hipGetLastError();
     }

@@ -27,14 +27,7 @@ namespace {
 
 constexpr int kGfThreads = 256;
 constexpr int kGfMaxBlocks = 512;
-
-// Sum of one partial array, identical in every workgroup (fixed order).
-__device__ __forceinline__ double gf_part(const double *p, int nb)
-{
-    double s = 0.0;
-    for (int k = threadIdx.x & 63; k < nb; k += 64) s += p[k];
-    return wave_sum(s);
-}
+constexpr int kGfPreBlocks = 4096;  // 1,048,576 nodes
 
 // block sums of three values, written by thread 0 to o0[b], o1[b], o2[b]
 __device__ __forceinline__ void gf_block3(double a, double b, double c, double *o0, double *o1,
@@ -174,31 +167,147 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg1(GfCg c)
     gf_block3(g, dl, rr, c.G[0], c.D[0], c.RR[0]);
 }
 
-// iteration i (reads parity i & 1, writes (i + 1) & 1)
+// Operands of one node's iteration: the 9 stencil values scaled by M^-1 at the neighbour
+// (0 outside the grid, where the neighbour index is clamped into it), the neighbours'
+// r_i, w_i, s_{i-1}, and the node's p_{i-1}, x_i.  None depends on the iteration's
+// scalars, so they are all loaded before the global reduction.
 template <int NP>
+struct GfOps {
+    double av[9], dq;
+    double2 r[9][NP], w[9][NP], s[9][NP], p[NP], x[NP];
+};
+
+template <int NP>
+__device__ __forceinline__ void gf_load(const GfCg &c, int cur, int q, GfOps<NP> &o)
+{
+    const int gx = c.gx, gy = c.gy;
+    const int ix = q % gx, iy = q / gx;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int jx = min(max(ix + k % 3 - 1, 0), gx - 1);
+        const int jy = min(max(iy + k / 3 - 1, 0), gy - 1);
+        const int j = jy * gx + jx;
+        o.av[k] = c.a9[k * c.n + q] * c.dinv[j];
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            o.r[k][m] = c.r[cur][j * NP + m];
+            o.w[k][m] = c.w[cur][j * NP + m];
+            o.s[k][m] = c.s[cur][j * NP + m];
+        }
+    }
+    o.dq = c.dinv[q];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        o.p[m] = c.p[q * NP + m];
+        o.x[m] = c.x[q * NP + m];
+    }
+}
+
+__device__ __forceinline__ void gf_pin(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void gf_pin(double2 v) { gf_pin(v.x); gf_pin(v.y); }
+
+// Holds every operand in registers at this point, so the loads are issued (and waited
+// for) before the reduction instead of being sunk past it.
+template <int NP>
+__device__ __forceinline__ void gf_pin(const GfOps<NP> &o)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        gf_pin(o.av[k]);
+#pragma unroll
+        for (int m = 0; m < NP; ++m) { gf_pin(o.r[k][m]); gf_pin(o.w[k][m]); gf_pin(o.s[k][m]); }
+    }
+    gf_pin(o.dq);
+#pragma unroll
+    for (int m = 0; m < NP; ++m) { gf_pin(o.p[m]); gf_pin(o.x[m]); }
+}
+
+// One node of iteration i: r_{i+1} = r_i - alpha (w_i + beta s_{i-1}) at the node and its
+// neighbours, w_{i+1} = N M^-1 r_{i+1}, p_i, x_{i+1}; adds the node's gamma, delta, r.r.
+template <int NP>
+__device__ __forceinline__ void gf_node(const GfCg &c, int nxt, int q, const GfOps<NP> &o,
+                                        double alpha, double beta, double &g, double &dl,
+                                        double &rn)
+{
+    double2 wa[NP], rnew[NP], snew[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        wa[m] = make_double2(0.0, 0.0);
+        rnew[m] = wa[m];
+        snew[m] = wa[m];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int m = 0; m < NP; ++m) {
+            const double2 rj = o.r[k][m], wj = o.w[k][m], sj = o.s[k][m];
+            const double2 sn = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
+            const double2 rnj = make_double2(rj.x - alpha * sn.x, rj.y - alpha * sn.y);
+            wa[m].x += o.av[k] * rnj.x;
+            wa[m].y += o.av[k] * rnj.y;
+            if (k == 4) { rnew[m] = rnj; snew[m] = sn; }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        const int e = q * NP + m;
+        const double2 ri = o.r[4][m];
+        // p_i = u_i + beta p_{i-1}, x_{i+1} = x_i + alpha p_i
+        double2 pi = o.p[m];
+        pi.x = ri.x * o.dq + beta * pi.x;
+        pi.y = ri.y * o.dq + beta * pi.y;
+        c.p[e] = pi;
+        double2 xi = o.x[m];
+        xi.x += alpha * pi.x;
+        xi.y += alpha * pi.y;
+        c.x[e] = xi;
+        c.r[nxt][e] = rnew[m];
+        c.s[nxt][e] = snew[m];
+        c.w[nxt][e] = wa[m];
+        const double uu = rnew[m].x * o.dq, uv = rnew[m].y * o.dq;
+        g += rnew[m].x * uu + rnew[m].y * uv;
+        dl += wa[m].x * uu + wa[m].y * uv;
+        rn += rnew[m].x * rnew[m].x + rnew[m].y * rnew[m].y;
+    }
+}
+
+// iteration i (reads parity i & 1, writes (i + 1) & 1).  With one value pair per node
+// and one node per thread (the launch covers the grid), every load of the iteration --
+// stop flag, partials, history and the node's operands -- is issued at once: one memory
+// round trip per kernel.  Otherwise the node loads follow the reduction.
+template <int NP, bool PRE>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
 {
     const int cur = i & 1, nxt = cur ^ 1;
+    const double stop = c.stat[0];
+    // wave wv sums partial array wv (G, D, RR of this parity, then BB); the arrays are
+    // consecutive runs of nb words from G[0] (G0 G1 D0 D1 RR0 RR1 BB), so the offset is
+    // arithmetic, not a pointer select
+    const int wv = threadIdx.x >> 6;
+    const double *ps = c.G[0] + (size_t)(wv < 3 ? 2 * wv + cur : 6) * c.nb;
+    const int lane = threadIdx.x & 63;
+    double part = lane < c.nb ? ps[lane] : 0.0;
+    const double h_alpha = c.hist[2 * nxt], h_gamma = c.hist[2 * nxt + 1];  // i - 1
+    const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
+    GfOps<NP> o;
+    if (PRE) gf_load(c, cur, min(q0, c.n - 1), o);
+    for (int k = lane + 64; k < c.nb; k += 64) part += ps[k];
+    if (PRE) gf_pin(o);
     // sticky stop: once set, later partials are stale and the kernel does nothing (the
     // flag is uniform, so every thread leaves before the barrier)
-    if (c.stat[0] != 0.0) return;
+    if (stop != 0.0) return;
     __shared__ double sh[4];
-    {
-        const int w = threadIdx.x >> 6;
-        const double *ps[4] = {c.G[cur], c.D[cur], c.RR[cur], c.BB};
-        const double v = gf_part(ps[w], c.nb);
-        if ((threadIdx.x & 63) == 0) sh[w] = v;
-        __syncthreads();
-    }
+    part = wave_sum(part);
+    if ((threadIdx.x & 63) == 0) sh[wv] = part;
+    __syncthreads();
     const double gam = sh[0], del = sh[1], rr = sh[2], bb = sh[3];
     double alpha, beta;
     if (i == 0) {
         beta = 0.0;
         alpha = gam / del;
     } else {
-        const double *h = c.hist + 2 * nxt;  // {alpha_{i-1}, gamma_{i-1}}
-        beta = gam / h[1];
-        alpha = gam / (del - beta * gam / h[0]);
+        beta = gam / h_gamma;
+        alpha = gam / (del - beta * gam / h_alpha);
     }
     // stop at the tolerance, on b == 0, or on a breakdown (non-finite or non-positive
     // step: the system is symmetric positive definite)
@@ -212,54 +321,13 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
         c.hist[2 * cur + 1] = gam;
     }
     if (done) return;
-    const int gx = c.gx, gy = c.gy;
     double g = 0.0, dl = 0.0, rn = 0.0;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < c.n; q += gridDim.x * blockDim.x) {
-        const int ix = q % gx, iy = q / gx;
-        double2 wa[NP], rnew[NP], snew[NP];
-#pragma unroll
-        for (int m = 0; m < NP; ++m) {
-            wa[m] = make_double2(0.0, 0.0);
-            rnew[m] = wa[m];
-            snew[m] = wa[m];
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            if (!gf_in(ix, iy, k, gx, gy)) continue;
-            const int j = q + (k / 3 - 1) * gx + (k % 3 - 1);
-            const double av = c.a9[k * c.n + q] * c.dinv[j];
-#pragma unroll
-            for (int m = 0; m < NP; ++m) {
-                const double2 rj = c.r[cur][j * NP + m], wj = c.w[cur][j * NP + m],
-                              sj = c.s[cur][j * NP + m];
-                const double2 sn = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
-                const double2 rnj = make_double2(rj.x - alpha * sn.x, rj.y - alpha * sn.y);
-                wa[m].x += av * rnj.x;
-                wa[m].y += av * rnj.y;
-                if (k == 4) { rnew[m] = rnj; snew[m] = sn; }
-            }
-        }
-        const double dinv = c.dinv[q];
-#pragma unroll
-        for (int m = 0; m < NP; ++m) {
-            const int e = q * NP + m;
-            const double2 ri = c.r[cur][e];
-            // p_i = u_i + beta p_{i-1}, x_{i+1} = x_i + alpha p_i
-            double2 pi = c.p[e];
-            pi.x = ri.x * dinv + beta * pi.x;
-            pi.y = ri.y * dinv + beta * pi.y;
-            c.p[e] = pi;
-            double2 xi = c.x[e];
-            xi.x += alpha * pi.x;
-            xi.y += alpha * pi.y;
-            c.x[e] = xi;
-            c.r[nxt][e] = rnew[m];
-            c.s[nxt][e] = snew[m];
-            c.w[nxt][e] = wa[m];
-            const double uu = rnew[m].x * dinv, uv = rnew[m].y * dinv;
-            g += rnew[m].x * uu + rnew[m].y * uv;
-            dl += wa[m].x * uu + wa[m].y * uv;
-            rn += rnew[m].x * rnew[m].x + rnew[m].y * rnew[m].y;
+    if (PRE) {
+        if (q0 < c.n) gf_node(c, nxt, q0, o, alpha, beta, g, dl, rn);
+    } else {
+        for (int q = q0; q < c.n; q += gridDim.x * blockDim.x) {
+            gf_load(c, cur, q, o);
+            gf_node(c, nxt, q, o, alpha, beta, g, dl, rn);
         }
     }
     gf_block3(g, dl, rn, c.G[nxt], c.D[nxt], c.RR[nxt]);
@@ -327,7 +395,11 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
                            DBuf *work, double *stats_dev, hipStream_t st, int poll)
 {
     const int n = g.gx * g.gy;
-    const int nb = std::min(kGfMaxBlocks, (n + kGfThreads - 1) / kGfThreads);
+    // one node per thread when the grid allows it (the operand prefetch needs it, and
+    // the partial arrays stay short enough to sum in every workgroup)
+    const int nfull = (n + kGfThreads - 1) / kGfThreads;
+    const bool pre = NP == 1 && nfull <= kGfPreBlocks;
+    const int nb = pre ? nfull : std::min(kGfMaxBlocks, nfull);
     hipError_t e;
     const size_t vb = sizeof(double2) * (size_t)n * NP;
     if ((e = work[0].ensure(7 * vb)) != hipSuccess) return e;
@@ -357,8 +429,14 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
     // poll <= 0: maxit iteration kernels and no host wait (the caller reads stat)
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
-        for (int q = 0; q < chunk && it + q < maxit; ++q)
-            hipLaunchKernelGGL(k_gf_cgit<NP>, dim3(nb), dim3(kGfThreads), 0, st, c, it + q);
+        for (int q = 0; q < chunk && it + q < maxit; ++q) {
+            if (pre)
+                hipLaunchKernelGGL((k_gf_cgit<NP, NP == 1>), dim3(nb), dim3(kGfThreads), 0, st, c,
+                                   it + q);
+            else
+                hipLaunchKernelGGL((k_gf_cgit<NP, false>), dim3(nb), dim3(kGfThreads), 0, st, c,
+                                   it + q);
+        }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
         double done = 0.0;
