@@ -405,147 +405,102 @@ static hipError_t gemm(const GemmArgs &g, int batch, bool scale, hipStream_t st)
 static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DBuf *work,
                                 hipStream_t st);
 
-// Fused pass of the fast-diagonalisation solve for grids whose gy x gx component fits
-// in LDS: workgroup (row strip of kFdRows rows, component) computes
+// Fused pass of the fast-diagonalisation solve for gx, gy <= 128: workgroup (row strip
+// of 16 rows, component) computes
 //   out[m, :] = ((L B)[m, :] R) (./ (mu + cK (lamy[m] + lamx[n])) when SCALE)
 // with L(m, k) = Lsrc[k * gy + m] and R(j, n) = Rsrc[j * gx + n].  B and out are the
-// interleaved gy x 2gx nodal arrays.  1024 threads: every load of B and R is issued up
-// front into registers (at most kFdPer words of each per thread), B then R goes to LDS.
-// In each product thread (column c, slice q) accumulates all kFdRows rows over the
-// q-th eighth of the inner index, so every B or R word is read from LDS once and the
-// strip words are broadcasts; the eight slices are summed in a fixed order through LDS.
-constexpr int kFdRows = 8;
-constexpr int kFdMax = 128;            // gx, gy limit of the fused path
+// interleaved gy x 2gx nodal arrays.  Both products run on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: A 16 x 4 with lane l holding (l % 16, l / 16), B 4 x 16 with
+// lane l holding (l / 16, l % 16), D 16 x 16 with lane l, element v holding
+// (4 (l / 16) + v, l % 16)).  16 waves = 8 column tiles of 16 x 2 halves of the inner
+// index; every wave loads its 16 A, B and R fragments from global memory up front (one
+// round trip), indices clamped in range and the words past gy or gx zeroed.  The strip
+// (L B) goes through LDS once; the halves are summed in a fixed order.
+constexpr int kFdRows = 16;
+constexpr int kFdMax = 128;                 // gx, gy limit of the fused path
 constexpr int kFdThreads = 1024;
-constexpr int kFdPer = kFdMax * kFdMax / kFdThreads;  // words of a component per thread
-constexpr int kFdSlices = kFdThreads / kFdMax;        // 8
-static_assert(kFdSlices == kFdRows, "the slice index doubles as the row in the sums");
+constexpr int kFdTiles = kFdMax / 16;       // column tiles
+constexpr int kFdSteps = kFdMax / 4 / 2;    // inner steps of 4 per half
 
-// sum of the slices' partials of row r, column c (fixed order)
-__device__ __forceinline__ double fd_slice_sum(const double *part, int gx, int r, int c)
-{
-    double v = 0.0;
-#pragma unroll
-    for (int q = 0; q < kFdSlices; ++q) v += part[(q * kFdRows + r) * gx + c];
-    return v;
-}
+typedef double fd_d4 __attribute__((ext_vector_type(4)));
 
 template <bool SCALE>
-__global__ __launch_bounds__(kFdThreads) void k_fd_pass(int gy, int gx, const double *Lsrc,
-                                                        const double *B, const double *Rsrc,
-                                                        double *out, const double *lamy,
-                                                        const double *lamx, double mu, double cK)
+__global__ __launch_bounds__(kFdThreads) void k_fd_pass(int gy, int gx,
+                                                        const double *__restrict__ Lsrc,
+                                                        const double *__restrict__ B,
+                                                        const double *__restrict__ Rsrc,
+                                                        double *__restrict__ out,
+                                                        const double *__restrict__ lamy,
+                                                        const double *__restrict__ lamx,
+                                                        double mu, double cK)
 {
-    extern __shared__ __align__(16) double sm[];
-    const int mx = gy > gx ? gy : gx;
-    const int big = (((mx > kFdSlices * kFdRows ? mx : kFdSlices * kFdRows) * gx) + 1) & ~1;
-    double *Ms = sm;                        // B (gy x gx), R (gx x gx), or the partials
-    double *Lt = Ms + big;                  // Lt[k][r] = L(m0 + r, k), k < gy
-    double *Tt = Lt + gy * kFdRows;         // Tt[j][r] = (L B)(m0 + r, j), j < gx
+    __shared__ double Tt[kFdMax * kFdRows];        // Tt[j * 16 + i] = (L B)(m0 + i, j)
+    __shared__ double Pp[kFdTiles][64 * 4];        // second half's tiles
     const int comp = blockIdx.y, m0 = blockIdx.x * kFdRows;
-    const int t = threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ct = wv % kFdTiles, h = wv / kFdTiles;
+    const int li = lane & 15, lk = lane >> 4;
     const int w2 = 2 * gx;
-    const int nb = gy * gx, nr = gx * gx;
-    double vl = 0.0;
-    if (t < gy * kFdRows) {
-        const int k = t / kFdRows, r = t % kFdRows;
-        if (m0 + r < gy) vl = Lsrc[(long)k * gy + m0 + r];
+    const int kh = ((gy + 3) / 4 + 1) / 2, jh = ((gx + 3) / 4 + 1) / 2;  // steps per half
+    const int rowc = min(m0 + li, gy - 1);    // A row (rows past gy are not stored)
+    const int col = ct * 16 + li, colc = min(col, gx - 1);
+    double fa[kFdSteps], fb[kFdSteps], fr[kFdSteps];
+#pragma unroll
+    for (int s = 0; s < kFdSteps; ++s) {
+        const int k = (h * kh + s) * 4 + lk, kc = min(k, gy - 1);
+        const bool kin = s < kh && k < gy;
+        const double av = Lsrc[(long)kc * gy + rowc];
+        const double bv = B[(long)kc * w2 + 2 * colc + comp];
+        fa[s] = kin ? av : 0.0;
+        fb[s] = kin && col < gx ? bv : 0.0;
+        const int j = (h * jh + s) * 4 + lk, jc = min(j, gx - 1);
+        const double rv = Rsrc[(long)jc * gx + colc];
+        fr[s] = s < jh && j < gx && col < gx ? rv : 0.0;
     }
-    // B walked as e = t + 1024 i over the gy x gx array (row index clamped for the tail,
-    // whose words are not stored), R contiguous (index clamped likewise)
-    double vb[kFdPer], vr[kFdPer];
-    {
-        const int dk = kFdThreads / gx, dc = kFdThreads % gx;
-        int k = t / gx, c = t % gx;
+    // Tt = L_strip B
+    fd_d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int i = 0; i < kFdPer; ++i) {
-            vb[i] = B[(long)min(k, gy - 1) * w2 + 2 * c + comp];
-            k += dk;
-            c += dc;
-            if (c >= gx) { c -= gx; ++k; }
-        }
+    for (int s = 0; s < kFdSteps; ++s)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s], fb[s], acc, 0, 0, 0);
+    if (h == 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Pp[ct][lane * 4 + v] = acc[v];
     }
-#pragma unroll
-    for (int i = 0; i < kFdPer; ++i) vr[i] = Rsrc[min(t + kFdThreads * i, nr - 1)];
-    if (t < gy * kFdRows) Lt[t] = vl;
-#pragma unroll
-    for (int i = 0; i < kFdPer; ++i)
-        if (t + kFdThreads * i < nb) Ms[t + kFdThreads * i] = vb[i];
     __syncthreads();
-    const int c = t & (kFdMax - 1), q = t >> 7;
-    double acc[kFdRows];
-    // Tt = L_strip B: slice q of k
-    {
-        const int ks = (gy + kFdSlices - 1) / kFdSlices, k0 = q * ks, k1 = min(gy, k0 + ks);
+    if (h == 0) {
 #pragma unroll
-        for (int r = 0; r < kFdRows; ++r) acc[r] = 0.0;
-        if (c < gx) {
-            for (int k = k0; k < k1; ++k) {
-                const double b = Ms[k * gx + c];
-                const double2 *l = reinterpret_cast<const double2 *>(Lt + k * kFdRows);
+        for (int v = 0; v < 4; ++v) Tt[col * kFdRows + 4 * lk + v] = acc[v] + Pp[ct][lane * 4 + v];
+    }
+    __syncthreads();
+    // out strip = Tt R
+    acc = fd_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int r2 = 0; r2 < kFdRows / 2; ++r2) {
-                    const double2 lv = l[r2];
-                    acc[2 * r2] += lv.x * b;
-                    acc[2 * r2 + 1] += lv.y * b;
-                }
+    for (int s = 0; s < kFdSteps; ++s) {
+        const double ta = Tt[((h * jh + s) * 4 + lk) * kFdRows + li];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, fr[s], acc, 0, 0, 0);
+    }
+    if (h == 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Pp[ct][lane * 4 + v] = acc[v];
+    }
+    __syncthreads();
+    if (h == 0 && col < gx) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = m0 + 4 * lk + v;
+            if (m < gy) {
+                double o = acc[v] + Pp[ct][lane * 4 + v];
+                if (SCALE) o /= mu + cK * (lamy[m] + lamx[col]);
+                out[(long)m * w2 + 2 * col + comp] = o;
             }
         }
     }
-    __syncthreads();
-    if (c < gx) {
-#pragma unroll
-        for (int r = 0; r < kFdRows; ++r) Ms[(q * kFdRows + r) * gx + c] = acc[r];
-    }
-    __syncthreads();
-    if (c < gx) Tt[c * kFdRows + q] = fd_slice_sum(Ms, gx, q, c);  // q doubles as the row
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kFdPer; ++i)
-        if (t + kFdThreads * i < nr) Ms[t + kFdThreads * i] = vr[i];
-    __syncthreads();
-    // out strip = Tt R: slice q of j
-    {
-        const int js = (gx + kFdSlices - 1) / kFdSlices, j0 = q * js, j1 = min(gx, j0 + js);
-#pragma unroll
-        for (int r = 0; r < kFdRows; ++r) acc[r] = 0.0;
-        if (c < gx) {
-            for (int j = j0; j < j1; ++j) {
-                const double rv = Ms[j * gx + c];
-                const double2 *tt = reinterpret_cast<const double2 *>(Tt + j * kFdRows);
-#pragma unroll
-                for (int r2 = 0; r2 < kFdRows / 2; ++r2) {
-                    const double2 tv = tt[r2];
-                    acc[2 * r2] += tv.x * rv;
-                    acc[2 * r2 + 1] += tv.y * rv;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (c < gx) {
-#pragma unroll
-        for (int r = 0; r < kFdRows; ++r) Ms[(q * kFdRows + r) * gx + c] = acc[r];
-    }
-    __syncthreads();
-    const int m = m0 + q;
-    if (c < gx && m < gy) {
-        double v = fd_slice_sum(Ms, gx, q, c);
-        if (SCALE) v /= mu + cK * (lamy[m] + lamx[c]);
-        out[(long)m * w2 + 2 * c + comp] = v;
-    }
-}
-
-static size_t fd_pass_lds(int gy, int gx)
-{
-    const size_t mx = std::max(std::max(gy, gx), kFdSlices * kFdRows);
-    return sizeof(double) * (((mx * gx + 1) & ~(size_t)1) + (size_t)(gy + gx) * kFdRows);
 }
 
 static hipError_t fem_fd_solve(const MeshDev &m, const double *b, double *x, DBuf *work,
                                hipStream_t st)
 {
-    if (m.gx <= kFdMax && m.gy <= kFdMax && fd_pass_lds(m.gy, m.gx) <= 160 * 1024 - 1024) {
+    if (m.gx <= kFdMax && m.gy <= kFdMax) {
         const int gx = m.gx, gy = m.gy;
         hipError_t e;
         if ((e = work[0].ensure(sizeof(double) * 2 * (size_t)gx * gy)) != hipSuccess) return e;
@@ -553,21 +508,12 @@ static hipError_t fem_fd_solve(const MeshDev &m, const double *b, double *x, DBu
         const double *Vx = m.fdV.as<double>(), *Vy = Vx + (size_t)gx * gx;
         const double *VxT = Vy + (size_t)gy * gy, *VyT = VxT + (size_t)gx * gx;
         const double *lx = m.fdlam.as<double>(), *ly = lx + gx;
-        const size_t lds = fd_pass_lds(gy, gx);
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute((const void *)k_fd_pass<true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
-            (void)hipFuncSetAttribute((const void *)k_fd_pass<false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
-            attr = true;
-        }
         const dim3 grid((gy + kFdRows - 1) / kFdRows, 2);
         // T2 = (Vy^T B Vx) ./ D:  L(m, k) = Vy[k][m],  R(j, n) = Vx[j][n]
-        hipLaunchKernelGGL(k_fd_pass<true>, grid, dim3(kFdThreads), lds, st, gy, gx, Vy, b, Vx, T2, ly,
+        hipLaunchKernelGGL(k_fd_pass<true>, grid, dim3(kFdThreads), 0, st, gy, gx, Vy, b, Vx, T2, ly,
                            lx, m.mu, m.beta + m.alpha * m.mu);
         // X = Vy T2 Vx^T:  L(m, k) = Vy[m][k] = VyT[k][m],  R(j, n) = Vx[n][j] = VxT[j][n]
-        hipLaunchKernelGGL(k_fd_pass<false>, grid, dim3(kFdThreads), lds, st, gy, gx, VyT,
+        hipLaunchKernelGGL(k_fd_pass<false>, grid, dim3(kFdThreads), 0, st, gy, gx, VyT,
                            (const double *)T2, VxT, x, nullptr, nullptr, 0.0, 0.0);
         return hipGetLastError();
     }

@@ -320,6 +320,10 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     constexpr int NCS = 2 + 6 + 6 + (SIX ? 21 : 0);
     __shared__ double s_hinv[SPB][36];
     __shared__ double s_cst[SPB][12];
+    // per subset: the warp P[0..5], then bottomf, bottomf^2 and the 2-DOF system t00,
+    // t01, t11.  The Gauss-Newton loop re-reads them after each pass, so no FP64 value
+    // is held in registers across the pass (which would spill to scratch).
+    __shared__ double s_k[SPB][11];
     __shared__ double s_cs[SPB][NCS];
     __shared__ float s_red[16 * WPS];
     __shared__ double s_redd[WPS];
@@ -544,6 +548,11 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
         t11 = h11 + 1e-3 * dmax;
     }
 
+    double *sk = s_k[sub_in_block];  // every lane writes the same values, reads its own
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sk[q] = P[q];
+    sk[6] = bottomf; sk[7] = bf2; sk[8] = t00; sk[9] = t01; sk[10] = t11;
+
     // ---- Gauss-Newton loop (funICGN.m:107-294 / funICGN_Subpb1.m:94-337) ----------
     const float Xa = (float)xa, Ya = (float)ya;  // window pixel (i, r) is X = Xa + i, Y = Ya + r
     const float fhalf = 0.5f * (float)w;
@@ -552,6 +561,9 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     int it = 0;
     while (it <= kMaxIter && nrmNew > tol && nrmAbs > tol) {
         ++it;
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < 6; ++q) P[q] = sk[q];
         bool finite = true;
 #pragma unroll
         for (int q = 0; q < 6; ++q) finite = finite && isfinite(P[q]);
@@ -627,6 +639,10 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
             }
         }
         block_sumN<WPS, NS>(acc, s_red, wid);
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < 6; ++q) P[q] = sk[q];
+        const double bottomf = sk[6], bf2 = sk[7], t00 = sk[8], t01 = sk[9], t11 = sk[10];
         const double Sg = acc[0], Sgg = acc[1];
         const double gbar = Sg / n;
         const double vg = Sgg - Sg * gbar;
@@ -657,7 +673,7 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
         if (SIX) {
             if (singular) {  // MATLAB's backslash on a singular H gives non-finite steps
 #pragma unroll
-                for (int q = 0; q < 6; ++q) P[q] = NAN;
+                for (int q = 0; q < 6; ++q) sk[q] = P[q] = NAN;
                 continue;
             }
 #pragma unroll
@@ -670,11 +686,13 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
         } else {
             const double det = t00 * t11 - t01 * t01;
             D[0] = D[1] = D[2] = D[3] = 0.0;
-            if (det == 0.0) { P[4] = P[5] = NAN; continue; }
+            if (det == 0.0) { sk[4] = P[4] = NAN; sk[5] = P[5] = NAN; continue; }
             D[4] = -(t11 * b[0] - t01 * b[1]) / det;
             D[5] = -(-t01 * b[0] + t00 * b[1]) / det;
         }
         if (!ic_compose(P, D)) break;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sk[q] = P[q];
     }
     const bool conv = (nrmNew < tol) || (nrmAbs < tol);
     if (!conv || isnan(nrmNew)) it = 101;
