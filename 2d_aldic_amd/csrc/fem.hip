@@ -151,6 +151,7 @@ struct CgVec {
 
 constexpr int kPcgThreads = 256;
 constexpr int kPcgMaxBlocks = 512;
+constexpr int kCgChunk = 8;  // row entries whose loads are issued together
 
 // Sum of one partial array, identical in every block (fixed order).
 __device__ __forceinline__ double part_sum(const double *p, int nb)
@@ -280,16 +281,42 @@ __global__ __launch_bounds__(kPcgThreads) void k_cg_it(int nn, const int *rowptr
     double g = 0.0, dl = 0.0, rn2 = 0.0;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += gridDim.x * blockDim.x) {
         double2 wa = make_double2(0.0, 0.0), rnew = wa, snew = wa, unew = wa;
-        for (int k = rowptr[q]; k < rowptr[q + 1]; ++k) {
-            const int j = col[k];
-            const double2 wj = wc[j], sj = sc[j], rj = rc[j];
-            const double dj = diag[j] != 0.0 ? 1.0 / diag[j] : 1.0;
-            const double2 snj = make_double2(wj.x + beta * sj.x, wj.y + beta * sj.y);
-            const double2 rnj = make_double2(rj.x - alpha * snj.x, rj.y - alpha * snj.y);
-            const double2 unj = make_double2(rnj.x * dj, rnj.y * dj);
-            wa.x += aval[k] * unj.x;
-            wa.y += aval[k] * unj.y;
-            if (j == q) { rnew = rnj; snew = snj; unew = unj; }
+        // the row in chunks of kCgChunk entries: the chunk's column indices and values
+        // are loaded together, then all its neighbours' operands, then the arithmetic
+        // in entry order (entries past the row's end point at the row's own node with
+        // value 0 and are skipped)
+        const int rs = rowptr[q], re = rowptr[q + 1];
+        for (int k0 = rs; k0 < re; k0 += kCgChunk) {
+            int cj[kCgChunk];
+            double av[kCgChunk];
+#pragma unroll
+            for (int u = 0; u < kCgChunk; ++u) {
+                const bool in = k0 + u < re;
+                const int kc = min(k0 + u, re - 1);  // in the row: loads stay in bounds
+                cj[u] = in ? col[kc] : q;
+                av[u] = in ? aval[kc] : 0.0;
+            }
+            double2 wj[kCgChunk], sj[kCgChunk], rj[kCgChunk];
+            double dd[kCgChunk];
+#pragma unroll
+            for (int u = 0; u < kCgChunk; ++u) {
+                wj[u] = wc[cj[u]];
+                sj[u] = sc[cj[u]];
+                rj[u] = rc[cj[u]];
+                dd[u] = diag[cj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < kCgChunk; ++u) {
+                if (k0 + u >= re) break;
+                const int j = cj[u];
+                const double dj = dd[u] != 0.0 ? 1.0 / dd[u] : 1.0;
+                const double2 snj = make_double2(wj[u].x + beta * sj[u].x, wj[u].y + beta * sj[u].y);
+                const double2 rnj = make_double2(rj[u].x - alpha * snj.x, rj[u].y - alpha * snj.y);
+                const double2 unj = make_double2(rnj.x * dj, rnj.y * dj);
+                wa.x += av[u] * unj.x;
+                wa.y += av[u] * unj.y;
+                if (j == q) { rnew = rnj; snew = snj; unew = unj; }
+            }
         }
         const double dq = diag[q] != 0.0 ? 1.0 / diag[q] : 1.0;
         const double2 rq = rc[q], pq = pc[q];
