@@ -141,6 +141,7 @@ struct CgPart {
     double *G[2], *D[2], *RR[2], *BB;  // per-block partials
     double *hist;                      // {alpha, gamma} per parity
     double *stat;                      // [0] done, [1] iterations, [2] relative residual
+    double *rec;                       // optional copy of stat, written alongside it
     int nb;
     double rtol2;
 };
@@ -211,7 +212,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_cg_init(int nn, const int *rowp
         bb += bi.x * bi.x + bi.y * bi.y;
     }
     block_reduce3(bb, 0.0, 0.0, P.BB, P.D[1], P.RR[1]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.stat[0] = 0.0; P.stat[1] = 0.0; P.stat[2] = 1.0;
+        if (P.rec) { P.rec[0] = 0.0; P.rec[1] = 0.0; P.rec[2] = 1.0; }
+    }
 }
 
 // w0 = A D^-1 r0; partials gamma0, delta0, r0.r0
@@ -273,6 +277,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_cg_it(int nn, const int *rowptr
         P.stat[0] = done ? 1.0 : 0.0;
         P.stat[1] = (double)i;
         P.stat[2] = bb > 0 ? sqrt(rr / bb) : 0.0;
+        if (P.rec) { P.rec[0] = P.stat[0]; P.rec[1] = P.stat[1]; P.rec[2] = P.stat[2]; }
         P.hist[2 * cur] = alpha;
         P.hist[2 * cur + 1] = gam;
     }
@@ -625,6 +630,7 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
     P.G[0] = pp; P.G[1] = pp + nb; P.D[0] = pp + 2 * nb; P.D[1] = pp + 3 * nb;
     P.RR[0] = pp + 4 * nb; P.RR[1] = pp + 5 * nb; P.BB = pp + 6 * nb;
     P.hist = pp + 7 * nb; P.stat = P.hist + 4; P.nb = nb; P.rtol2 = rtol * rtol;
+    P.rec = stats_dev;
     double2 *vv = work[0].as<double2>();
     CgVec v;
     for (int k = 0; k < 2; ++k) {
@@ -653,10 +659,7 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
         if (done != 0.0) break;
     }
-    if (stats_dev &&
-        (e = hipMemcpyAsync(stats_dev, P.stat, 3 * sizeof(double), hipMemcpyDeviceToDevice, st)) !=
-            hipSuccess)
-        return e;
+    // stats_dev, when given, is written by the kernels alongside P.stat
     if (stats_host) {
         // status of the last iteration kernel: {done, iterations, relative residual}
         double hs[3];

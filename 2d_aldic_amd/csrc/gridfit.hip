@@ -65,6 +65,7 @@ struct GfCg {
     double *G[2], *D[2], *RR[2], *BB;  // per-workgroup partials (gamma, delta, r.r, b.b)
     double *hist;              // [parity][2] = {alpha_i, gamma_i}, written by workgroup 0
     double *stat;              // {done, iterations, relative residual}
+    double *rec;               // optional copy of stat, written alongside it
     int nb;
     double rtol2;
 };
@@ -127,7 +128,10 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg0(GfCg c, const double *w, 
         }
     }
     gf_block3(bb, 0.0, 0.0, c.BB, nullptr, nullptr);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { c.stat[0] = 0.0; c.stat[1] = 0.0; c.stat[2] = 1.0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.stat[0] = 0.0; c.stat[1] = 0.0; c.stat[2] = 1.0;
+        if (c.rec) { c.rec[0] = 0.0; c.rec[1] = 0.0; c.rec[2] = 1.0; }
+    }
 }
 
 // K1: w_0 = N u_0 with u_0 = M^-1 r_0; partials gamma_0, delta_0, r_0.r_0
@@ -317,6 +321,7 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
         c.stat[0] = done ? 1.0 : 0.0;
         c.stat[1] = (double)i;
         c.stat[2] = bb > 0 ? sqrt(rr / bb) : 0.0;
+        if (c.rec) { c.rec[0] = c.stat[0]; c.rec[1] = c.stat[1]; c.rec[2] = c.stat[2]; }
         c.hist[2 * cur] = alpha;
         c.hist[2 * cur + 1] = gam;
     }
@@ -420,6 +425,7 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
     c.RR[0] = ps + 4 * nb; c.RR[1] = ps + 5 * nb; c.BB = ps + 6 * nb;
     c.hist = ps + 7 * nb;
     c.stat = ps + 7 * nb + 4;
+    c.rec = stats_dev;
     c.nb = nb;
     c.rtol2 = rtol * rtol;
     hipLaunchKernelGGL(k_gf_cg0<NP>, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
@@ -446,12 +452,8 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
         if (done != 0.0) break;
     }
-    if (stats_dev) {
-        // {done, iterations, relative residual} from the last iteration kernel's record
-        if ((e = hipMemcpyAsync(stats_dev, c.stat, 3 * sizeof(double),
-                                hipMemcpyDeviceToDevice, st)) != hipSuccess)
-            return e;
-    }
+    // stats_dev ({done, iterations, relative residual} of the last iteration kernel that
+    // ran) is written by the kernels themselves
     return hipSuccess;
 }
 
