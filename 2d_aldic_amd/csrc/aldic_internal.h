@@ -179,8 +179,9 @@ constexpr int kStatParts = 1024;
 // Largest image: FP32 planes are addressed with 32-bit byte offsets in the kernels,
 // and row offsets (4 nx bytes) with 24-bit multiplies, so nx < 2^22.
 constexpr size_t kMaxPixels = (size_t)1 << 29;  // per-block partials written by launch_stats
-hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
-                        hipStream_t st);
+// both images' partials in one launch: acc = 2 kStatParts of f, then 2 kStatParts of g
+hipError_t launch_stats(const void *rawf, const void *rawg, int dtype, int nx, int ny,
+                        const int *roi, double *acc, hipStream_t st);
 hipError_t launch_prep(const void *rawf, const void *rawg, int dtype, int nx, int ny,
                        const uint8_t *mask, const double *acc, double n_roi, double *ms,
                        float *f, float *g, float *gx, float *gy, hipStream_t st);

@@ -101,8 +101,7 @@ hipError_t run_prep(aldic_ctx *c)
         (e = c->stats.ensure((4 * kStatParts + 4) * sizeof(double))))
         return e;
     double *acc = c->stats.as<double>();  // partials of f, then of g
-    if ((e = launch_stats(c->rawf.p, c->raw_dtype, c->nx, c->ny, c->roi, acc, c->stream))) return e;
-    if ((e = launch_stats(c->rawg.p, c->raw_dtype, c->nx, c->ny, c->roi, acc + 2 * kStatParts,
+    if ((e = launch_stats(c->rawf.p, c->rawg.p, c->raw_dtype, c->nx, c->ny, c->roi, acc,
                           c->stream)))
         return e;
     const double nroi = (double)(c->roi[1] - c->roi[0] + 1) * (c->roi[3] - c->roi[2] + 1);

@@ -15,14 +15,17 @@ template <class T> struct Acc { using type = double; };
 template <> struct Acc<uint8_t> { using type = unsigned long long; };
 template <> struct Acc<uint16_t> { using type = unsigned long long; };
 
-// Per-block partial sums over the ROI: part[2b] = sum, part[2b+1] = sum of squares.
+// Per-block partial sums over the ROI of image blockIdx.y (raw0, then raw1):
+// part[2b] = sum, part[2b+1] = sum of squares, image 1's partials after image 0's.
 // Integer pixels accumulate in 64-bit integers (exact); block b takes rows y1 + b,
 // y1 + b + gridDim.x, ...
 template <class T>
-__global__ __launch_bounds__(256) void k_stats(const T *raw, int nx, int x1, int x2, int y1,
-                                               int y2, double *part)
+__global__ __launch_bounds__(256) void k_stats(const T *raw0, const T *raw1, int nx, int x1,
+                                               int x2, int y1, int y2, double *part)
 {
     using A = typename Acc<T>::type;
+    const T *raw = blockIdx.y ? raw1 : raw0;
+    part += 2 * (size_t)gridDim.x * blockIdx.y;
     A s = 0, s2 = 0;
     for (int y = y1 - 1 + blockIdx.x; y < y2; y += gridDim.x) {
         const T *row = raw + (size_t)y * nx;
@@ -128,21 +131,22 @@ __global__ __launch_bounds__(256) void k_gradient(const float *f, int nx, int ny
 }
 
 template <class T>
-static void stats_t(const void *raw, int nx, const int *roi, double *part, hipStream_t st)
+static void stats_t(const void *rawf, const void *rawg, int nx, const int *roi, double *part,
+                    hipStream_t st)
 {
-    hipLaunchKernelGGL(k_stats<T>, dim3(kStatParts), dim3(256), 0, st, (const T *)raw, nx,
-                       roi[0], roi[1], roi[2], roi[3], part);
+    hipLaunchKernelGGL(k_stats<T>, dim3(kStatParts, 2), dim3(256), 0, st, (const T *)rawf,
+                       (const T *)rawg, nx, roi[0], roi[1], roi[2], roi[3], part);
 }
 
-hipError_t launch_stats(const void *raw, int dtype, int nx, int ny, const int *roi, double *acc,
-                        hipStream_t st)
+hipError_t launch_stats(const void *rawf, const void *rawg, int dtype, int nx, int ny,
+                        const int *roi, double *acc, hipStream_t st)
 {
     (void)ny;
     switch (dtype) {
-    case ALDIC_U8: stats_t<uint8_t>(raw, nx, roi, acc, st); break;
-    case ALDIC_U16: stats_t<uint16_t>(raw, nx, roi, acc, st); break;
-    case ALDIC_F32: stats_t<float>(raw, nx, roi, acc, st); break;
-    default: stats_t<double>(raw, nx, roi, acc, st); break;
+    case ALDIC_U8: stats_t<uint8_t>(rawf, rawg, nx, roi, acc, st); break;
+    case ALDIC_U16: stats_t<uint16_t>(rawf, rawg, nx, roi, acc, st); break;
+    case ALDIC_F32: stats_t<float>(rawf, rawg, nx, roi, acc, st); break;
+    default: stats_t<double>(rawf, rawg, nx, roi, acc, st); break;
     }
     return hipGetLastError();
 }

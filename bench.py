@@ -30,6 +30,7 @@ HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+VALU_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 
 
 def log(msg):
@@ -203,6 +204,10 @@ def main():
                 avg_ms=round(icgn6_ms, 5),
                 note="VALU- and latency-bound (FP32 Keys gathers and sums, 66 VALU per pixel pass)",
                 copy_gbs_measured=measure_copy_bw())
+    # VALU issue share: SQ_INSTS_VALU per launch (rocprofv3 counter run) x 4 cycles per
+    # wave64 VALU instruction / (1024 SIMDs x 2.4 GHz x the live average duration)
+    vi = traffic.get("k_icgn_6dof", {}).get("valu_instr_per_launch")
+    roof["valu_frac"] = round(vi * 4 / (1024 * VALU_CLOCK_HZ * t_i), 4) if vi else None
     # The integer search beside it: int8 matrix cores, algorithmic ops = the NCC
     # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
