@@ -155,10 +155,18 @@ constexpr int kPcgMaxBlocks = 512;
 constexpr int kCgChunk = 8;  // row entries whose loads are issued together
 
 // Sum of one partial array, identical in every block (fixed order).
+// nb <= kPcgMaxBlocks = 8 x 64: eight independent (clamped) loads per lane, summed in
+// the order k = lane, lane + 64, ...
 __device__ __forceinline__ double part_sum(const double *p, int nb)
 {
+    const int lane = threadIdx.x & 63;
+    double v[kPcgMaxBlocks / 64];
+#pragma unroll
+    for (int u = 0; u < kPcgMaxBlocks / 64; ++u) v[u] = p[min(lane + 64 * u, nb - 1)];
     double s = 0.0;
-    for (int k = threadIdx.x & 63; k < nb; k += 64) s += p[k];
+#pragma unroll
+    for (int u = 0; u < kPcgMaxBlocks / 64; ++u)
+        if (lane + 64 * u < nb) s += v[u];
     return wave_sum(s);
 }
 

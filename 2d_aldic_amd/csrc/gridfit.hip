@@ -27,7 +27,6 @@ namespace {
 
 constexpr int kGfThreads = 256;
 constexpr int kGfMaxBlocks = 512;
-constexpr int kGfPreBlocks = 4096;  // 1,048,576 nodes
 
 // block sums of three values, written by thread 0 to o0[b], o1[b], o2[b]
 __device__ __forceinline__ void gf_block3(double a, double b, double c, double *o0, double *o1,
@@ -289,13 +288,20 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
     // arithmetic, not a pointer select
     const int wv = threadIdx.x >> 6;
     const double *ps = c.G[0] + (size_t)(wv < 3 ? 2 * wv + cur : 6) * c.nb;
+    // nb <= kGfMaxBlocks = 8 x 64: eight independent (clamped) loads per lane, summed in
+    // the order k = lane, lane + 64, ...
     const int lane = threadIdx.x & 63;
-    double part = lane < c.nb ? ps[lane] : 0.0;
+    double pv[kGfMaxBlocks / 64];
+#pragma unroll
+    for (int u = 0; u < kGfMaxBlocks / 64; ++u) pv[u] = ps[min(lane + 64 * u, c.nb - 1)];
     const double h_alpha = c.hist[2 * nxt], h_gamma = c.hist[2 * nxt + 1];  // i - 1
     const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
     GfOps<NP> o;
     if (PRE) gf_load(c, cur, min(q0, c.n - 1), o);
-    for (int k = lane + 64; k < c.nb; k += 64) part += ps[k];
+    double part = 0.0;
+#pragma unroll
+    for (int u = 0; u < kGfMaxBlocks / 64; ++u)
+        if (lane + 64 * u < c.nb) part += pv[u];
     if (PRE) gf_pin(o);
     // sticky stop: once set, later partials are stale and the kernel does nothing (the
     // flag is uniform, so every thread leaves before the barrier)
@@ -400,11 +406,11 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
                            DBuf *work, double *stats_dev, hipStream_t st, int poll)
 {
     const int n = g.gx * g.gy;
-    // one node per thread when the grid allows it (the operand prefetch needs it, and
-    // the partial arrays stay short enough to sum in every workgroup)
+    // one node per thread when the grid allows it (the operand prefetch needs it); at
+    // most kGfMaxBlocks partials per array
     const int nfull = (n + kGfThreads - 1) / kGfThreads;
-    const bool pre = NP == 1 && nfull <= kGfPreBlocks;
-    const int nb = pre ? nfull : std::min(kGfMaxBlocks, nfull);
+    const bool pre = NP == 1 && nfull <= kGfMaxBlocks;
+    const int nb = std::min(kGfMaxBlocks, nfull);
     hipError_t e;
     const size_t vb = sizeof(double2) * (size_t)n * NP;
     if ((e = work[0].ensure(7 * vb)) != hipSuccess) return e;
