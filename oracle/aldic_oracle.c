@@ -1,9 +1,9 @@
 /*
  * aldic_oracle.c -- FP64 CPU restatement of the AL-DIC correlation path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: the tests, the smoke check
- * and the CPU-baseline leg of the benchmark load it to compare the HIP library
- * against.  Nothing in the product links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: the tests, the smoke check
+ * and the CPU-baseline leg of the benchmark load it as the reference the HIP library
+ * is compared against.  Nothing in the product links or calls it.
  *
  * Each function restates one reference function (MATLAB, jyang526843/2D_ALDIC) and
  * cites the file:line it follows.  Conventions shared with the reference:

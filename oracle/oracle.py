@@ -1,7 +1,8 @@
 """ctypes front end of the FP64 CPU oracle (liborc.so, built from aldic_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by
-the CPU-baseline leg of ``bench.py``, where it is the checker and the CPU baseline.
+the CPU-baseline leg of ``bench.py``, where it is the reference the HIP library is
+compared against and the CPU baseline.
 The product (the HIP library and its Python mirror) never imports this module.
 
 Array conventions follow aldic_oracle.c: images (ny, nx) C-ordered (x fastest),
