@@ -516,16 +516,21 @@ def test_frame_matches_oracle_frame_config1(gpu_ctx, cfg1, orc):
     assert t["frame"] > 0
 
 
-@pytest.mark.parametrize("case", ["uniform_122x100", "uniform_140x131", "nonuniform"])
+@pytest.mark.parametrize("case", ["uniform_122x100", "uniform_140x131", "nonuniform",
+                                  "uniform_1100x120"])
 def test_gridfit_matches_oracle(gpu_ctx, orc, case):
     # Subpb1's gridfit (FP64 Jacobi-PCG on the normal equations) against the oracle's
-    # sparse direct least squares
+    # sparse direct least squares.  Up to 131,072 nodes the iteration kernels take one
+    # node per thread with every operand loaded before the reduction; 1100 x 120
+    # (132,000 nodes) takes the grid-stride form.
     _flush(f"gridfit case {case}")
     rng = np.random.default_rng(11)
     if case == "uniform_122x100":
         xn, yn = 17.0 + 16 * np.arange(122), 17.0 + 16 * np.arange(100)
     elif case == "uniform_140x131":
         xn, yn = 9.0 + 8 * np.arange(140), 9.0 + 8 * np.arange(131)
+    elif case == "uniform_1100x120":
+        xn, yn = 9.0 + 8 * np.arange(1100), 9.0 + 8 * np.arange(120)
     else:
         xn = np.cumsum(rng.uniform(5, 15, 23))
         yn = np.cumsum(rng.uniform(5, 15, 19))
