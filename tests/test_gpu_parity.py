@@ -348,10 +348,11 @@ def _grid_mesh(nx, ny, h, x0=20.0, y0=30.0):
     return coords, np.array(el, np.int32)
 
 
-@pytest.mark.parametrize("shape", [(23, 17), (141, 9)])
+@pytest.mark.parametrize("shape", [(23, 17), (141, 9), (5, 3), (128, 40)])
 def test_subpb2_and_strain_match_oracle(gpu_ctx, orc, shape):
     # (23, 17): fused two-pass solve; (141, 9): more than 128 nodes per row, the
-    # four-GEMM solve
+    # four-GEMM solve; (5, 3): the fused solve with mostly zero-padded matrix-core
+    # fragments; (128, 40): the fused solve at its width limit
     _flush(f"subpb2 grid {shape}")
     coords, el = _grid_mesh(shape[0], shape[1], 10.0)
     n = len(coords)
