@@ -155,17 +155,18 @@ constexpr int kPcgMaxBlocks = 512;
 constexpr int kCgChunk = 8;  // row entries whose loads are issued together
 
 // Sum of one partial array, identical in every block (fixed order).
-// nb <= kPcgMaxBlocks = 8 x 64: eight independent (clamped) loads per lane, summed in
-// the order k = lane, lane + 64, ...
+// nb <= 64 NPL: NPL independent (clamped) loads per lane, summed in the order
+// k = lane, lane + 64, ...
+template <int NPL>
 __device__ __forceinline__ double part_sum(const double *p, int nb)
 {
     const int lane = threadIdx.x & 63;
-    double v[kPcgMaxBlocks / 64];
+    double v[NPL];
 #pragma unroll
-    for (int u = 0; u < kPcgMaxBlocks / 64; ++u) v[u] = p[min(lane + 64 * u, nb - 1)];
+    for (int u = 0; u < NPL; ++u) v[u] = p[min(lane + 64 * u, nb - 1)];
     double s = 0.0;
 #pragma unroll
-    for (int u = 0; u < kPcgMaxBlocks / 64; ++u)
+    for (int u = 0; u < NPL; ++u)
         if (lane + 64 * u < nb) s += v[u];
     return wave_sum(s);
 }
@@ -251,6 +252,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_cg_w0(int nn, const int *rowptr
     block_reduce3(g, dl, rr, P.G[0], P.D[0], P.RR[0]);
 }
 
+template <int NPL>
 __global__ __launch_bounds__(kPcgThreads) void k_cg_it(int nn, const int *rowptr, const int *col,
                                                        const double *aval, const double *diag,
                                                        double2 *x, CgVec v, CgPart P, int i)
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_cg_it(int nn, const int *rowptr
     const double h_alpha = P.hist[2 * nxt], h_gamma = P.hist[2 * nxt + 1];  // i - 1
     const int w = threadIdx.x >> 6;
     const double *ps = P.G[0] + (size_t)(w < 3 ? 2 * w + cur : 6) * P.nb;
-    const double val = part_sum(ps, P.nb);
+    const double val = part_sum<NPL>(ps, P.nb);
     if (stop != 0.0) return;  // sticky stop (uniform: every thread leaves)
     __shared__ double sh[4];
     if ((threadIdx.x & 63) == 0) sh[w] = val;
@@ -656,8 +658,12 @@ hipError_t fem_pcg(const MeshDev &m, const double *b, double *x, double rtol, in
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
         for (int q = 0; q < chunk && it + q < maxit; ++q)
-            hipLaunchKernelGGL(k_cg_it, dim3(nb), dim3(kPcgThreads), 0, st, nn, rp, cl, av, dg,
-                               (double2 *)x, v, P, it + q);
+            if (nb <= 64)
+                hipLaunchKernelGGL(k_cg_it<1>, dim3(nb), dim3(kPcgThreads), 0, st, nn, rp, cl,
+                                   av, dg, (double2 *)x, v, P, it + q);
+            else
+                hipLaunchKernelGGL(k_cg_it<kPcgMaxBlocks / 64>, dim3(nb), dim3(kPcgThreads), 0,
+                                   st, nn, rp, cl, av, dg, (double2 *)x, v, P, it + q);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
         double done = 0.0;

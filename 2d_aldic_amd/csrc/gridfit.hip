@@ -278,7 +278,7 @@ __device__ __forceinline__ void gf_node(const GfCg &c, int nxt, int q, const GfO
 // and one node per thread (the launch covers the grid), every load of the iteration --
 // stop flag, partials, history and the node's operands -- is issued at once: one memory
 // round trip per kernel.  Otherwise the node loads follow the reduction.
-template <int NP, bool PRE>
+template <int NP, bool PRE, int NPL>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
 {
     const int cur = i & 1, nxt = cur ^ 1;
@@ -288,19 +288,19 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
     // arithmetic, not a pointer select
     const int wv = threadIdx.x >> 6;
     const double *ps = c.G[0] + (size_t)(wv < 3 ? 2 * wv + cur : 6) * c.nb;
-    // nb <= kGfMaxBlocks = 8 x 64: eight independent (clamped) loads per lane, summed in
-    // the order k = lane, lane + 64, ...
+    // nb <= 64 NPL: NPL independent (clamped) loads per lane, summed in the order
+    // k = lane, lane + 64, ...
     const int lane = threadIdx.x & 63;
-    double pv[kGfMaxBlocks / 64];
+    double pv[NPL];
 #pragma unroll
-    for (int u = 0; u < kGfMaxBlocks / 64; ++u) pv[u] = ps[min(lane + 64 * u, c.nb - 1)];
+    for (int u = 0; u < NPL; ++u) pv[u] = ps[min(lane + 64 * u, c.nb - 1)];
     const double h_alpha = c.hist[2 * nxt], h_gamma = c.hist[2 * nxt + 1];  // i - 1
     const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
     GfOps<NP> o;
     if (PRE) gf_load(c, cur, min(q0, c.n - 1), o);
     double part = 0.0;
 #pragma unroll
-    for (int u = 0; u < kGfMaxBlocks / 64; ++u)
+    for (int u = 0; u < NPL; ++u)
         if (lane + 64 * u < c.nb) part += pv[u];
     if (PRE) gf_pin(o);
     // sticky stop: once set, later partials are stale and the kernel does nothing (the
@@ -442,12 +442,15 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
     const int chunk = poll > 0 ? poll : maxit;
     for (int it = 0; it < maxit; it += chunk) {
         for (int q = 0; q < chunk && it + q < maxit; ++q) {
-            if (pre)
-                hipLaunchKernelGGL((k_gf_cgit<NP, NP == 1>), dim3(nb), dim3(kGfThreads), 0, st, c,
-                                   it + q);
+            if (pre && nb <= 64)
+                hipLaunchKernelGGL((k_gf_cgit<NP, NP == 1, 1>), dim3(nb), dim3(kGfThreads), 0, st,
+                                   c, it + q);
+            else if (pre)
+                hipLaunchKernelGGL((k_gf_cgit<NP, NP == 1, kGfMaxBlocks / 64>), dim3(nb),
+                                   dim3(kGfThreads), 0, st, c, it + q);
             else
-                hipLaunchKernelGGL((k_gf_cgit<NP, false>), dim3(nb), dim3(kGfThreads), 0, st, c,
-                                   it + q);
+                hipLaunchKernelGGL((k_gf_cgit<NP, false, kGfMaxBlocks / 64>), dim3(nb),
+                                   dim3(kGfThreads), 0, st, c, it + q);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (poll <= 0) break;
