@@ -165,7 +165,7 @@ int icgn_dispatch(aldic_ctx *ctx, bool six, IcgnArgs a, const int *ws_host)
         if (w < 2 || cls[j] == 0)
             return fail(ctx, ALDIC_ERR_ARG, "icgn: subset size must be 2..64");
     }
-    const int classes[3] = {8, 18, 72};
+    const int classes[4] = {5, 8, 18, 72};
     std::vector<int> all(S);
     DBuf &lst = ctx->scratch[7];
     bool single = std::all_of(cls.begin(), cls.end(), [&](int v) { return v == cls[0]; });
@@ -176,7 +176,7 @@ int icgn_dispatch(aldic_ctx *ctx, bool six, IcgnArgs a, const int *ws_host)
         a.nlist = S;
         CK(six ? launch_icgn6(a, cls[0], ctx->stream) : launch_icgn2(a, cls[0], ctx->stream));
     } else {
-        for (int ci = 0; ci < 3; ++ci) {
+        for (int ci = 0; ci < 4; ++ci) {
             std::vector<int> l;
             for (int j = 0; j < S; ++j)
                 if (cls[j] == classes[ci]) l.push_back(j);
@@ -201,7 +201,8 @@ int icgn_dispatch(aldic_ctx *ctx, bool six, IcgnArgs a, const int *ws_host)
         int next = 0;
         for (int j : l) {
             const int cur = cls[j];
-            next = cur == 8 ? 18 : 72;
+            // round(sqrt(2) w) of a window of class 5 or 8 fits class 18
+            next = cur <= 8 ? 18 : 72;
             cls[j] = next;
         }
         if (round == 2 || next == 0) {
@@ -211,7 +212,7 @@ int icgn_dispatch(aldic_ctx *ctx, bool six, IcgnArgs a, const int *ws_host)
                               ctx->stream));
             break;
         }
-        for (int ci = 0; ci < 3; ++ci) {
+        for (int ci = 0; ci < 4; ++ci) {
             std::vector<int> l2;
             for (int j : l)
                 if (cls[j] == classes[ci]) l2.push_back(j);
@@ -1355,7 +1356,7 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
     }
     std::vector<int> lists;
     Q.boff.clear(); Q.bn.clear(); Q.bppl.clear();
-    for (int c : {8, 18, 72}) {
+    for (int c : {5, 8, 18, 72}) {
         const int off = (int)lists.size();
         for (int i = 0; i < S; ++i)
             if (cls[i] == c) lists.push_back(i);

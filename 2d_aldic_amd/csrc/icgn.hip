@@ -795,6 +795,7 @@ hipError_t launch_list_eq(const int *it, int S, int code, int *list, int *count,
 int icgn_ppl_for(int ws)
 {
     const int n = (ws + 1) * (ws + 1);
+    if (n <= 5 * 64) return 5;      // windows up to 17 x 17
     if (n <= 8 * 64) return 8;
     if (n <= 18 * 64) return 18;
     if (n <= 18 * 256) return 72;   // 4 waves x 18 pixels
@@ -808,7 +809,14 @@ static hipError_t launch_any(const IcgnArgs &a, int ppl, hipStream_t st)
     // a device-side length (a.nlist its upper bound) gets a capped, looping grid
     constexpr int cap = 512;
     const bool loop = a.nlist_dev != nullptr;
-    if (ppl == 8) {
+    if (ppl == 5) {
+        if (loop)
+            hipLaunchKernelGGL((k_icgn<5, 1, SIX, true>), dim3(std::min(cap, (a.nlist + 3) / 4)),
+                               dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_icgn<5, 1, SIX, false>), dim3((a.nlist + 3) / 4), dim3(256), 0,
+                               st, a);
+    } else if (ppl == 8) {
         if (loop)
             hipLaunchKernelGGL((k_icgn<8, 1, SIX, true>), dim3(std::min(cap, (a.nlist + 3) / 4)),
                                dim3(256), 0, st, a);
