@@ -225,6 +225,11 @@ def main():
                            traffic=measured("k_search_i8") if small else None,
                            algorithmic_ops=ops, algorithmic_bytes=search_bytes,
                            avg_ms=round(search_ms, 5))
+        # the kernel is bound by VALU issue, not the matrix cores: its VALU share as for
+        # IC-GN (counter run of the default workload only)
+        svi = traffic.get("k_search_i8", {}).get("valu_instr_per_launch") if small else None
+        search_roof["valu_frac"] = (round(svi * 4 / (1024 * VALU_CLOCK_HZ * t_s), 4)
+                                    if svi else None)
     else:
         search_bytes = npx * 8 + 32 * S  # the FP32 pair once + outputs
         search_roof = dict(kernel="k_search", bound="mfma",
