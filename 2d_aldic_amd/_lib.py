@@ -25,7 +25,7 @@ EXPORTS = (
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
     "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
-    "aldic_qframe_run", "aldic_qframe_get",
+    "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc",
 )
 
 
@@ -86,6 +86,7 @@ def load():
     L.aldic_qframe_setup.argtypes = [P, C.POINTER(QFrameOpts), vp, vp, vp]
     L.aldic_qframe_run.argtypes = [P, vp]
     L.aldic_qframe_get.argtypes = [P, vp, vp, vp, vp, vp]
+    L.aldic_prep_noalloc.argtypes = [P, vp, vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 

@@ -186,6 +186,8 @@ hipError_t launch_prep(const void *rawf, const void *rawg, int dtype, int nx, in
                        const uint8_t *mask, const double *acc, double n_roi, double *ms,
                        float *f, float *g, float *gx, float *gy, hipStream_t st);
 hipError_t launch_gradient(const float *f, int nx, int ny, float *gx, float *gy, hipStream_t st);
+// Looks up the device address of aldic_prep_noalloc's work array on `device` (current).
+hipError_t noalloc_init(int device);
 
 struct IcgnArgs {
     const float *f, *g, *gx, *gy;

@@ -240,6 +240,7 @@ int aldic_create(aldic_ctx **out, int device)
     if (!ctx) return ALDIC_ERR_ARG;
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = noalloc_init(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
     if (e != hipSuccess) {

@@ -54,6 +54,23 @@ const char *aldic_last_error(const aldic_ctx *ctx);
 int aldic_set_images(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
                      const uint8_t *mask, const int *roi);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * pre-pass of aldic_set_images (same kernels, same results) on caller-owned DEVICE
+ * buffers: raw pair f, g (nx*ny of dtype), mask (nx*ny bytes, NULL = all ones), roi as in
+ * aldic_set_images (host int[4], NULL = whole image) -> FP32 planes fo, go (normalised),
+ * gxo, gyo (gradients, 0 outside the valid band), nx*ny floats each.  Calls no hipMalloc /
+ * hipFree: its work memory (the per-block ROI partial sums and the four statistics,
+ * 32,800 bytes) is a fixed device array of the library, one per GPU, looked up by
+ * aldic_create; images are limited to 2^29 pixels and 2^22 columns as everywhere (larger
+ * ones are rejected with ALDIC_ERR_ARG).  Enqueued on `stream` (a hipStream_t; NULL = the
+ * context's stream) and not waited for: synchronise that stream before reading the
+ * outputs.  Every call on one GPU shares that GPU's work array, so calls that overlap on
+ * one GPU (from several contexts or streams) race. */
+int aldic_prep_noalloc(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
+                       const uint8_t *mask, const int *roi, float *fo, float *go, float *gxo,
+                       float *gyo, void *stream);
+
 /* The same with the pair already normalised by the caller (the reference's
  * fNormalized/gNormalized, FP64).  Gradients are computed on the device unless
  * aldic_set_gradients() supplies them afterwards. */
