@@ -25,7 +25,7 @@ EXPORTS = (
     "aldic_frame_run", "aldic_frame_get", "aldic_frame_times", "aldic_gridfit",
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
     "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
-    "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc",
+    "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc", "aldic_natural_fill",
 )
 
 
@@ -87,6 +87,7 @@ def load():
     L.aldic_qframe_run.argtypes = [P, vp]
     L.aldic_qframe_get.argtypes = [P, vp, vp, vp, vp, vp]
     L.aldic_prep_noalloc.argtypes = [P, vp, vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp]
+    L.aldic_natural_fill.argtypes = [P, ip, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -309,6 +310,19 @@ class Context:
         self._ck(self.L.aldic_nodal_strain(self.h, _p(_c(U, np.float64)), int(bool(swap_fix)),
                                            _p(F)), "aldic_nodal_strain")
         return F
+
+    def natural_fill(self, coords, bad, U, F=None):
+        """scatteredInterpolant(good nodes, 'natural') of U (2 per node) and F (4 per node)
+        at the bad nodes (LocalICGN.m:141-161) -> refilled copies (U, F)."""
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        n = coords.shape[0]
+        b = _c(np.asarray(bad, bool).astype(np.int32), np.int32)
+        U = _c(U, np.float64).copy()
+        F = None if F is None else _c(F, np.float64).copy()
+        assert U.size == 2 * n and (F is None or F.size == 4 * n)
+        self._ck(self.L.aldic_natural_fill(self.h, n, _p(coords), _p(b), _p(U), _p(F)),
+                 "aldic_natural_fill")
+        return U, F
 
     def bicubic(self, X, Y):
         X = _c(X, np.float64).ravel()

@@ -337,66 +337,16 @@ def Init(u, v, Phi, x, y, index=0):
 
 # ---- local steps -------------------------------------------------------------------------
 
-def _node_grid(coordinatesFEM):
-    xs = np.unique(coordinatesFEM[:, 0])
-    ys = np.unique(coordinatesFEM[:, 1])
-    ix = np.searchsorted(xs, coordinatesFEM[:, 0])
-    iy = np.searchsorted(ys, coordinatesFEM[:, 1])
-    return xs.size, ys.size, ix, iy
-
-
-def axis_fill(V, nc, bad, Nx, Ny):
-    """Fill bad nodes of an Nx x Ny node grid (x fastest, nc interleaved components):
-    each takes the inverse-distance average of the nearest good node along the four grid
-    directions (left, right, down, up, summed in that order); 0 when none exists."""
-    G = np.asarray(V, dtype=np.float64).reshape(Ny, Nx, nc)
-    B = np.asarray(bad, dtype=bool).reshape(Ny, Nx)
-    yy, xx = np.mgrid[0:Ny, 0:Nx]
-    A = np.where(~B, xx, -1)
-    L = np.concatenate([np.full((Ny, 1), -1), np.maximum.accumulate(A, axis=1)[:, :-1]], axis=1)
-    A = np.where(~B, xx, Nx)
-    R = np.concatenate([np.minimum.accumulate(A[:, ::-1], axis=1)[:, ::-1][:, 1:],
-                        np.full((Ny, 1), Nx)], axis=1)
-    A = np.where(~B, yy, -1)
-    D = np.concatenate([np.full((1, Nx), -1), np.maximum.accumulate(A, axis=0)[:-1, :]], axis=0)
-    A = np.where(~B, yy, Ny)
-    Up = np.concatenate([np.minimum.accumulate(A[::-1, :], axis=0)[::-1, :][1:, :],
-                         np.full((1, Nx), Ny)], axis=0)
-    srcs = []
-    for ok, idx_y, idx_x, dist in ((L >= 0, yy, np.clip(L, 0, Nx - 1), xx - L),
-                                   (R < Nx, yy, np.clip(R, 0, Nx - 1), R - xx),
-                                   (D >= 0, np.clip(D, 0, Ny - 1), xx, yy - D),
-                                   (Up < Ny, np.clip(Up, 0, Ny - 1), xx, Up - yy)):
-        w = np.where(ok, 1.0 / np.maximum(dist, 1), 0.0)
-        srcs.append((ok, w, G[idx_y, idx_x]))
-    ws = np.zeros((Ny, Nx))
-    s = np.zeros((Ny, Nx, nc))
-    for ok, w, v in srcs:
-        ws = ws + w
-        s = s + np.where(ok[:, :, None], w[:, :, None] * v, 0.0)
-    fillv = np.where(ws[:, :, None] > 0, s / np.where(ws > 0, ws, 1.0)[:, :, None], 0.0)
-    out = np.where(B[:, :, None], fillv, G)
-    return out.reshape(-1)
-
-
 def _refill(coordinatesFEM, U, F=None):
-    """Refill NaN nodes on the node grid (axis_fill; the reference uses scatteredInterpolant
-    'natural', LocalICGN.m:144-161; equal for isolated holes: the 4-neighbour mean)."""
-    Nx, Ny, ix, iy = _node_grid(coordinatesFEM)
-    if Nx * Ny != coordinatesFEM.shape[0]:
-        raise NotImplementedError("refill needs a full rectangular node grid")
-    order = iy * Nx + ix
-    bad = np.zeros(Nx * Ny, bool)
-    bad[order] = np.isnan(U[0::2]) | np.isnan(U[1::2])
-    def fill(vec, nc):
-        g = np.empty(Nx * Ny * nc)
-        g.reshape(-1, nc)[order] = vec.reshape(-1, nc)
-        out = axis_fill(g, nc, bad, Nx, Ny).reshape(-1, nc)
-        return out[order].ravel()
-    U = fill(U, 2)
+    """LocalICGN.m:141-161: scatteredInterpolant(good nodes, 'natural') of U (and F) at
+    the NaN nodes, evaluated on the device (aldic_natural_fill)."""
+    coords = np.asarray(coordinatesFEM, dtype=np.float64)
+    bad = np.isnan(U[0::2]) | np.isnan(U[1::2])
     if F is not None:
-        F = fill(F, 4)
-    return U, F
+        bad = bad | np.isnan(F.reshape(-1, 4)).any(axis=1)
+    if not bad.any():
+        return U, F
+    return get_context().natural_fill(coords, bad, U, F)
 
 
 def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",
@@ -444,7 +394,8 @@ def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, bet
     With ALSolveStep == 1 the reference passes a zero Hessian (:69-77).  Failed nodes
     (status < 0 or > 99) become NaN and u, v are then refitted at every node by gridfit
     with the springs regulariser over the node grid (:165-199, device CG); with
-    smoothing='off' the failed nodes are refilled by axis_fill instead."""
+    smoothing='off' the failed nodes are refilled by the natural-neighbour interpolant
+    instead (as LocalICGN does)."""
     if smoothing not in ("gridfit", "off"):
         raise ValueError("smoothing must be 'gridfit' or 'off'")
     ctx = _ensure_images(ImgRef, ImgDef, Df)

@@ -47,6 +47,24 @@ struct DBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Node set of a natural-neighbour refill: buckets of side h (CSR by bucket, x fastest)
+// and rows of equal y (x ascending), plus the refill's work memory.
+struct NatGrid {
+    int nn = 0, BX = 0, BY = 0, R = 0;
+    double ox = 0, oy = 0, h = 1;
+    DBuf bstart, bnode, rstart, rnode, work;
+};
+hipError_t natgrid_setup(NatGrid &g, const double *coords_host, int nn, hipStream_t st);
+// LocalICGN.m:141-161 / Subpb1Quadtree.m:188-199 refill in place: bad[i] != 0 nodes of U
+// (2 per node) and F (4 per node, nullable) from the good ones (scatteredInterpolant
+// 'natural', linear rule on and outside the good nodes' hull).  No host wait.
+// err (nullable, device): refills whose cell could not be formed are added to *err.
+hipError_t natural_fill(const NatGrid &g, const double *coords_dev, const int *bad, double *U,
+                        double *F, hipStream_t st, int *err = nullptr);
+hipError_t natural_err_clear(int *err, hipStream_t st);
+// Refills of the last natural_fill on g whose Sibson cell could not be formed (waits).
+int natural_errors(const NatGrid &g, hipStream_t st);
+
 // Q4 mesh in node-row (CSR-like) form: for node i, the elements around it and the
 // node's local index in each; plus the scalar node-to-node sparsity pattern (the
 // u- and v-blocks of the global matrix share it).
@@ -77,6 +95,7 @@ struct MeshDev {
 struct QuadDev {
     MeshDev m;
     DBuf elems, n2e, code, cval;
+    NatGrid nat;  // its nodes for the bad-node refill
 };
 
 // gridfit ('springs') of nodal data on a rectilinear node grid, data on the nodes
@@ -116,6 +135,8 @@ struct QFrameState {
     int pcg_reruns = 0;               // frames re-run with polled PCG
     int pcg_most = 0;                 // most PCG iterations of one global step, last frame
     DBuf pstat;                       // {converged, iterations, residual} per global step
+    DBuf nat_err;                     // refills whose cell could not be formed (device int)
+    int *nat_err_h = nullptr;         // pinned copy
     double *hstat = nullptr;          // pinned host copy
     double t_frame = 0, t_local = 0, t_global = 0;  // ms of the last frame
     hipEvent_t ev[4] = {};
@@ -129,6 +150,9 @@ struct FrameState {
         bad, tmp, part, cc, rl, F2b, gfstat,
         Us, Fs, U1s, U1old, U2old, errs;  // smoothing temporaries, stop-rule copies, norms
     GridfitDev gf;  // Subpb1's gridfit over the node grid
+    NatGrid nat;    // the node grid for LocalICGN's bad-node refill
+    DBuf nat_err;   // refills of the frame whose cell could not be formed (device int)
+    int *nat_err_h = nullptr;  // pinned copy, read after the frame's wait
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
     double *gfh = nullptr;  // pinned: {converged, iterations, residual} per frame gridfit
     double *hostd = nullptr;  // pinned: norms read back by the beta sweep and stop rule
@@ -169,6 +193,7 @@ struct aldic_ctx {
     aldic::FrameState fr;
     aldic::QFrameState qf;
     aldic::GridfitDev gf;            // aldic_gridfit
+    aldic::NatGrid nat;              // aldic_natural_fill
     hipEvent_t ev[10] = {};
 };
 

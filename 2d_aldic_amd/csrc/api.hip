@@ -273,6 +273,12 @@ void aldic_destroy(aldic_ctx *ctx)
         g->dinv.release();
     }
     F.gfstat.release();
+    F.nat_err.release();
+    ctx->qf.nat_err.release();
+    if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
+    if (ctx->qf.nat_err_h) (void)hipHostFree(ctx->qf.nat_err_h);
+    for (aldic::NatGrid *g : {&F.nat, &ctx->nat, &ctx->quad.nat})
+        for (DBuf *bb : {&g->bstart, &g->bnode, &g->rstart, &g->rnode, &g->work}) bb->release();
     if (F.gfh) (void)hipHostFree(F.gfh);
     if (F.hostd) (void)hipHostFree(F.hostd);
     for (DBuf *bb : {&F.Us, &F.Fs, &F.U1s, &F.U1old, &F.U2old, &F.errs}) bb->release();
@@ -773,6 +779,7 @@ int aldic_set_quadtree(aldic_ctx *ctx, const double *coords, int nn, const int *
     CK(q.cval.ensure(sizeof(double) * col.size()));
     CK(m.diag.ensure(sizeof(double) * nn));
     CK(m.gvec.ensure(sizeof(double) * 2 * nn));
+    CK(natgrid_setup(q.nat, coords, nn, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     m.beta = m.mu = m.alpha = -1;
     ctx->have_quad = true;
@@ -896,6 +903,35 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
     return ALDIC_OK;
 }
 
+int aldic_natural_fill(aldic_ctx *ctx, int nn, const double *coords, const int *bad, double *U,
+                       double *F)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (nn < 0 || (nn > 0 && (!coords || !bad || !U)))
+        return fail(ctx, ALDIC_ERR_ARG, "natural_fill: bad arguments");
+    if (nn == 0) return ALDIC_OK;
+    CK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DBuf *sc = ctx->scratch;
+    CK(natgrid_setup(ctx->nat, coords, nn, st));
+    CK(up(sc[0], coords, 2 * (size_t)nn, st));
+    CK(up(sc[1], bad, (size_t)nn, st));
+    CK(up(sc[2], U, 2 * (size_t)nn, st));
+    if (F) CK(up(sc[3], F, 4 * (size_t)nn, st));
+    CK(natural_fill(ctx->nat, sc[0].as<double>(), sc[1].as<int>(), sc[2].as<double>(),
+                    F ? sc[3].as<double>() : nullptr, st));
+    CK(down(U, sc[2].p, 2 * (size_t)nn, st));
+    if (F) CK(down(F, sc[3].p, 4 * (size_t)nn, st));
+    CK(hipStreamSynchronize(st));
+    const int nerr = natural_errors(ctx->nat, st);
+    if (nerr != 0) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "natural_fill: %d Sibson cells could not be formed", nerr);
+        return fail(ctx, ALDIC_ERR_NUMERIC, buf);
+    }
+    return ALDIC_OK;
+}
+
 int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, double *Z)
 {
     if (!ctx) return ALDIC_ERR_ARG;
@@ -970,6 +1006,9 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     int r = aldic_set_mesh(ctx, F.hcoords.data(), S, F.helems.data(), (int)F.helems.size() / 4);
     if (r) return r;
     CK(up(F.coords, F.hcoords.data(), F.hcoords.size(), ctx->stream));
+    CK(natgrid_setup(F.nat, F.hcoords.data(), S, ctx->stream));
+    CK(F.nat_err.ensure(sizeof(int)));
+    if (!F.nat_err_h) CK(hipHostMalloc((void **)&F.nat_err_h, sizeof(int), hipHostMallocDefault));
     const size_t sd = sizeof(double) * S;
     DBuf *v2[] = {&F.U0, &F.U6, &F.Ucur, &F.U1, &F.U2, &F.vdual};
     for (DBuf *b : v2) CK(b->ensure(2 * sd));
@@ -1056,6 +1095,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         spans.push_back(stage); spans.push_back(e0); spans.push_back(e1);
     };
     const int e_start = mark();
+    CK(natural_err_clear(F.nat_err.as<int>(), st));
     CK(run_prep(ctx));
     ctx->have_images = true;
     ctx->images_from_raw = true;
@@ -1134,9 +1174,12 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     }
     const int e_icgn6 = mark();
     span(2, e_search, e_icgn6);
-    // LocalICGN post-processing: bad-point rule and refill (U1, F1 hold copies of U6, F6)
-    CK(launch_badfill(F.it6.as<int>(), S, F.Nx, F.Ny, 1, nullptr, nullptr, F.U1.as<double>(),
+    // LocalICGN post-processing (U1, F1 hold copies of U6, F6): the bad-point rule
+    // (LocalICGN.m:119-139) and the natural-neighbour refill of U and F (:141-161)
+    CK(launch_badmark(F.it6.as<int>(), S, 1, nullptr, nullptr, F.U1.as<double>(),
                       F.F1.as<double>(), F.bad.as<int>(), F.part.as<double>(), st));
+    CK(natural_fill(F.nat, F.coords.as<double>(), F.bad.as<int>(), F.U1.as<double>(),
+                    F.F1.as<double>(), st, F.nat_err.as<int>()));
     const int e_post6 = smark();
     span(3, e_icgn6, e_post6);
 
@@ -1253,11 +1296,19 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
             ar2.nlist_dev = ar.nlist_dev;
             CK(launch_icgn2(ar2, ppl_big, st));
         }
-        CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, U1, nullptr,
-                          F.bad.as<int>(), F.part.as<double>(), st));
-        // Subpb1.m:185-199: gridfit of u and v over the good nodes, written to every node
-        // (the refilled values above are only its starting guess)
-        if (!o.no_subpb1_fit) CK(gfit(U1, 1, F.bad.as<int>()));
+        if (!o.no_subpb1_fit) {
+            // Subpb1.m:165-199: bad nodes (status < 0 or > 99) carry no datum; gridfit of u
+            // and v over the good nodes, written to every node.  The axis fill of the bad
+            // nodes is only the CG's starting guess.
+            CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, U1, nullptr,
+                              F.bad.as<int>(), F.part.as<double>(), st));
+            CK(gfit(U1, 1, F.bad.as<int>()));
+        } else {
+            CK(launch_badmark(F.it2.as<int>(), S, 0, nullptr, nullptr, U1, nullptr,
+                              F.bad.as<int>(), F.part.as<double>(), st));
+            CK(natural_fill(F.nat, F.coords.as<double>(), F.bad.as<int>(), U1, nullptr, st,
+                            F.nat_err.as<int>()));
+        }
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC.m:364)
         cur ^= 1;
         if (stop) CK(hipMemcpyAsync(F.U1s.p, U1, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
@@ -1283,6 +1334,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     F.gf_count = ngf;
     if (ngf > 0)
         CK(hipMemcpyAsync(F.gfh, F.gfstat.p, 3 * sizeof(double) * ngf, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(F.nat_err_h, F.nat_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     const int e_end = mark();
     span(7, e_start, e_end);
     if (e_end < 0) return fail(ctx, ALDIC_ERR_HIP, "frame_run: event pool exhausted");
@@ -1294,6 +1346,8 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         acc[spans[k]] += t;
     }
     for (int k = 0; k < 8; ++k) F.ms[k] = acc[k];
+    if (*F.nat_err_h != 0)
+        return fail(ctx, ALDIC_ERR_NUMERIC, "frame_run: a natural-neighbour cell could not be formed");
     return ALDIC_OK;
 }
 
@@ -1389,6 +1443,8 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
     if (Q.hstat) (void)hipHostFree(Q.hstat);
     Q.hstat = nullptr;
     CK(hipHostMalloc((void **)&Q.hstat, sizeof(double) * 3 * (size_t)(opts->n_admm + 1)));
+    CK(Q.nat_err.ensure(sizeof(int)));
+    if (!Q.nat_err_h) CK(hipHostMalloc((void **)&Q.nat_err_h, sizeof(int), hipHostMallocDefault));
     for (auto &e : Q.ev)
         if (!e) CK(hipEventCreate(&e));
     CK(aldic::quad_assemble(q, opts->beta, opts->mu, st));
@@ -1416,6 +1472,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     const aldic_qframe_opts &o = Q.o;
     CK(up(Q.U0, U0, 2 * (size_t)S, st));
     CK(hipEventRecord(Q.ev[0], st));
+    CK(natural_err_clear(Q.nat_err.as<int>(), st));
     // LocalICGN: one launch per window class over its node list
     IcgnArgs a{};
     a.f = ctx->f.as<float>();
@@ -1457,7 +1514,8 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     double *U1 = Q.U1.as<double>(), *F1 = Q.F1.as<double>(), *U2 = Q.U2.as<double>();
     CK(launch_badmark(Q.it6.as<int>(), S, 1, nullptr, nullptr, U1, F1, Q.bad.as<int>(),
                       Q.part.as<double>(), st));
-    CK(launch_meshfill(q.m, Q.bad.as<int>(), U1, F1, Q.known.as<int>(), 8, st));
+    CK(natural_fill(q.nat, q.m.coords.as<double>(), Q.bad.as<int>(), U1, F1, st,
+                    Q.nat_err.as<int>()));
     CK(hipEventRecord(Q.ev[1], st));
 
     double *F2buf[2] = {Q.F2a.as<double>(), Q.F2b.as<double>()};
@@ -1527,7 +1585,8 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
         }
         CK(launch_badmark(Q.it2.as<int>(), S, 2, nullptr, nullptr, U1, nullptr, Q.bad.as<int>(),
                           Q.part.as<double>(), st));
-        CK(launch_meshfill(q.m, Q.bad.as<int>(), U1, nullptr, Q.known.as<int>(), 8, st));
+        CK(natural_fill(q.nat, q.m.coords.as<double>(), Q.bad.as<int>(), U1, nullptr, st,
+                        Q.nat_err.as<int>()));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC_Quadtree.m:258)
         cur ^= 1;
         if ((r = global_step())) return r;
@@ -1535,6 +1594,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     Q.cur_F2 = cur;
     // one wait per frame: the PCG records
     CK(hipMemcpyAsync(Q.hstat, Q.pstat.p, 3 * sizeof(double) * nglob, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(Q.nat_err_h, Q.nat_err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     CK(hipEventSynchronize(Q.ev[3]));
     CK(hipStreamSynchronize(st));
     float ms_frame = 0.f, ms_local = 0.f;
@@ -1552,6 +1612,8 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     }
     Q.pcg_iters = total;
     Q.pcg_most = most;
+    if (*Q.nat_err_h != 0)
+        return fail(ctx, ALDIC_ERR_NUMERIC, "qframe_run: a natural-neighbour cell could not be formed");
     return ok ? ALDIC_OK : -1000;  // a PCG ran out of its budget
 }
 

@@ -1,0 +1,703 @@
+// Bad-node refill of LocalICGN and Subpb1Quadtree: scatteredInterpolant(good xy, good
+// values, 'natural') evaluated at every node (LocalICGN.m:141-161,
+// Subpb1Quadtree.m:188-199; MATLAB's default extrapolation for 'natural' is 'linear').
+//
+//   Good nodes keep their values.  A bad node q strictly inside the convex hull of the
+//   good nodes P takes Sibson's natural-neighbour interpolant: the weight of p is the
+//   area the Voronoi cell V(q) of q (in the diagram of P + {q}) takes from p's cell, over
+//   area(V(q)).  V(q) = a large square clipped by the half-planes |x - q| <= |x - p|,
+//   p taken from rings of buckets around q until the next ring is at least twice the
+//   cell's radius away; the part of V(q) nearest p_i = V(q) clipped by
+//   |x - p_i| <= |x - p_j| over the other natural neighbours p_j.  Polygons are cyclic
+//   lists of lines (each entry: the line's tag and the vertex where it meets the next
+//   line), every vertex the meet of two lines given by the sites themselves.
+//   A bad node on or outside the hull (the rule chosen for MATLAB's 'linear'
+//   extrapolation, which is documented only as "based on boundary gradients"): b = the
+//   nearest hull point; inside a hull edge, the plane through the good nodes a, c next
+//   to b on that edge and the edge's Delaunay third vertex r (least inward bulge, ties to
+//   the lowest node index); at a hull vertex v, the plane through v and its neighbours
+//   along the two hull edges.  Fewer than three non-collinear good nodes: the nearest
+//   good node (ties to the lowest index); none: NaN.
+//
+// Kernels (one refill = 5 launches, no host wait):
+//   k_nat_rows      per row of equal y: first / last good node, good count
+//   k_nat_hull      one workgroup: good count, strict convex hull of the row extremes
+//                   (monotone chain), counters cleared
+//   k_nat_classify  per bad node: inside the hull -> inside list, else -> hull list
+//   k_nat_sibson    one thread per inside node, its two polygons in LDS
+//   k_nat_hullfill  one workgroup per hull-list node, scans of all nodes
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "aldic_internal.h"
+#include "common.h"
+
+namespace aldic {
+
+namespace {
+
+constexpr int kNatMax = 24;        // lines per polygon
+constexpr int kNatThreads = 64;    // k_nat_sibson workgroup
+constexpr int kRegion = 0x40000000;  // tag bit: bisector of p_i and p_(tag & ~kRegion)
+
+struct NatArgs {
+    const double2 *xy;
+    const int *bad;
+    double2 *U;      // 2 values per node, in place
+    double4 *F;      // 4 values per node, in place (nullable)
+    int nn;
+    // buckets
+    int BX, BY;
+    double ox, oy, h;
+    const int *bstart, *bnode;
+    // rows (equal y, x ascending)
+    int R;
+    const int *rstart, *rnode;
+    // work
+    int *rows;       // 3 R: first, last, count
+    int *chain;      // 6 R + 4: row extremes, then the chain stack
+    int *hull;       // 2 R + 2 (counter-clockwise, from the lowest-x, lowest-y vertex)
+    int *ctl;        // {hull size, good count, n inside, n hull, errors}
+    int *lin, *lhull;  // nn each
+    int *err;        // nullable: refills whose cell could not be formed are also counted here
+};
+
+__device__ __forceinline__ void nat_error(const NatArgs &a)
+{
+    atomicAdd(&a.ctl[4], 1);
+    if (a.err) atomicAdd(a.err, 1);
+}
+
+__device__ __forceinline__ double cross3(double2 a, double2 b, double2 c)
+{
+    return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+}
+
+__global__ __launch_bounds__(256) void k_nat_rows(NatArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.R) return;
+    int first = -1, last = -1, cnt = 0;
+    for (int k = a.rstart[r]; k < a.rstart[r + 1]; ++k) {
+        const int i = a.rnode[k];
+        if (a.bad[i]) continue;
+        if (first < 0) first = i;
+        last = i;
+        ++cnt;
+    }
+    a.rows[3 * r] = first;
+    a.rows[3 * r + 1] = last;
+    a.rows[3 * r + 2] = cnt;
+}
+
+// Strict hull (collinear points dropped) of the row extremes, which are sorted by (y, x):
+// the monotone chain in the axis-swapped frame (cross products change sign), reversed to
+// counter-clockwise and rotated to start at the lowest-x, lowest-y vertex.
+__global__ __launch_bounds__(256) void k_nat_hull(NatArgs a)
+{
+    __shared__ int red[4];
+    int c = 0;
+    for (int r = threadIdx.x; r < a.R; r += blockDim.x) c += a.rows[3 * r + 2];
+    c = (int)wave_sum((double)c);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int ng = red[0] + red[1] + red[2] + red[3];
+    int *P = a.chain;            // candidates
+    int np = 0;
+    for (int r = 0; r < a.R; ++r) {
+        const int f = a.rows[3 * r], l = a.rows[3 * r + 1];
+        if (f < 0) continue;
+        P[np++] = f;
+        if (l != f) P[np++] = l;
+    }
+    int *H = a.chain + np;       // chain stack (at most 2 np + 1 entries)
+    int k = 0;
+    for (int s = 0; s < np; ++s) {
+        while (k >= 2 && cross3(a.xy[H[k - 2]], a.xy[H[k - 1]], a.xy[P[s]]) >= 0.0) --k;
+        H[k++] = P[s];
+    }
+    for (int s = np - 2, t = k + 1; s >= 0; --s) {
+        while (k >= t && cross3(a.xy[H[k - 2]], a.xy[H[k - 1]], a.xy[P[s]]) >= 0.0) --k;
+        H[k++] = P[s];
+    }
+    int h = np >= 3 ? k - 1 : 0;
+    if (h < 3) h = 0;
+    // reverse (counter-clockwise) and start at the lowest (x, y)
+    int s0 = 0;
+    for (int s = 1; s < h; ++s) {
+        const double2 p = a.xy[H[s]], b = a.xy[H[s0]];
+        if (p.x < b.x || (p.x == b.x && p.y < b.y)) s0 = s;
+    }
+    for (int s = 0; s < h; ++s) a.hull[s] = H[((s0 - s) % h + h) % h];
+    a.ctl[0] = h;
+    a.ctl[1] = ng;
+    a.ctl[2] = 0;
+    a.ctl[3] = 0;
+    a.ctl[4] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_nat_classify(NatArgs a)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.nn || !a.bad[q]) return;
+    const int h = a.ctl[0];
+    bool inside = h >= 3 && a.ctl[1] >= 3;
+    const double2 Q = a.xy[q];
+    for (int e = 0; e < h && inside; ++e) {
+        const double2 A = a.xy[a.hull[e]], C = a.xy[a.hull[e + 1 < h ? e + 1 : 0]];
+        const double ex = C.x - A.x, ey = C.y - A.y, px = Q.x - A.x, py = Q.y - A.y;
+        const double cr = ex * py - ey * px;
+        if (!(cr > 1e-12 * sqrt(ex * ex + ey * ey) * sqrt(px * px + py * py))) inside = false;
+    }
+    if (inside) a.lin[atomicAdd(&a.ctl[2], 1)] = q;
+    else a.lhull[atomicAdd(&a.ctl[3], 1)] = q;
+}
+
+// ---- per-thread polygons in LDS ------------------------------------------------------
+struct Poly {
+    int *tag;        // [kNatMax][kNatThreads], this thread's column
+    double *vx, *vy;
+};
+
+struct Line {
+    double nx, ny, c;
+};
+
+// Line of a tag, coordinates relative to q; pi = p_i - q for region tags.
+__device__ __forceinline__ Line nat_line(const NatArgs &a, double2 Q, double B, int tag,
+                                         double2 pi)
+{
+    Line L;
+    if (tag < 0) {
+        const int k = -1 - tag;
+        L.nx = k == 0 ? 1.0 : (k == 2 ? -1.0 : 0.0);
+        L.ny = k == 1 ? 1.0 : (k == 3 ? -1.0 : 0.0);
+        L.c = B;
+    } else if (tag & kRegion) {
+        const double2 P = a.xy[tag & ~kRegion];
+        const double px = P.x - Q.x, py = P.y - Q.y;
+        L.nx = px - pi.x;
+        L.ny = py - pi.y;
+        L.c = 0.5 * ((px * px + py * py) - (pi.x * pi.x + pi.y * pi.y));
+    } else {
+        const double2 P = a.xy[tag];
+        const double px = P.x - Q.x, py = P.y - Q.y;
+        L.nx = px;
+        L.ny = py;
+        L.c = 0.5 * (px * px + py * py);
+    }
+    return L;
+}
+
+__device__ __forceinline__ void nat_meet(const Line &l1, const Line &l2, double &x, double &y)
+{
+    const double det = l1.nx * l2.ny - l1.ny * l2.nx;
+    x = (l1.c * l2.ny - l2.c * l1.ny) / det;
+    y = (l1.nx * l2.c - l2.nx * l1.c) / det;
+}
+
+#define PT(P, k) (P).tag[(k) * kNatThreads]
+#define PX(P, k) (P).vx[(k) * kNatThreads]
+#define PY(P, k) (P).vy[(k) * kNatThreads]
+
+// Clip P (m entries) with line M (tag mt): keep M.n . x <= M.c.  Returns the new size,
+// 0 when nothing is left, -1 on overflow.
+__device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly P, int m,
+                        const Line &M, int mt)
+{
+    unsigned mask = 0u;
+    for (int k = 0; k < m; ++k) {
+        const double u = M.nx * PX(P, k), v = M.ny * PY(P, k);
+        const double d = u + v - M.c;
+        if (d > 1e-12 * (fabs(u) + fabs(v) + fabs(M.c))) mask |= 1u << k;
+    }
+    if (!mask) return m;
+    const int len = __popc(mask);
+    if (len == m) return 0;
+    const unsigned full = m == 32 ? 0xffffffffu : ((1u << m) - 1u);
+    const unsigned prev = ((mask << 1) | (mask >> (m - 1))) & full;  // bit k: out[k - 1]
+    const int aa = __ffs(mask & ~prev) - 1;                          // first of the run
+    int run = 0;
+    while ((mask >> ((aa + run) % m)) & 1u) ++run;
+    if (run != len) return -1;  // not one run: the polygon lost its convexity numerically
+    // delete entries aa+1 .. aa+len-1 (cyclic), insert M after aa
+    int ins;  // slot of M
+    int nm;
+    if (len >= 2) {
+        ins = aa + 1 < m ? aa + 1 : 0;
+        PT(P, ins) = mt;
+        const int d = len - 2;
+        nm = m - d;
+        if (d > 0) {
+            const int s = ins + 1 < m ? ins + 1 : 0;
+            if (s + d <= m) {
+                for (int k = s; k + d < m; ++k) {
+                    PT(P, k) = PT(P, k + d); PX(P, k) = PX(P, k + d); PY(P, k) = PY(P, k + d);
+                }
+            } else {
+                const int sh = s + d - m;  // entries [sh, s) are kept, moved to 0
+                for (int k = 0; k < s - sh; ++k) {
+                    PT(P, k) = PT(P, k + sh); PX(P, k) = PX(P, k + sh); PY(P, k) = PY(P, k + sh);
+                }
+            }
+        }
+    } else {
+        if (m + 1 > kNatMax) return -1;
+        for (int k = m; k > aa + 1; --k) {
+            PT(P, k) = PT(P, k - 1); PX(P, k) = PX(P, k - 1); PY(P, k) = PY(P, k - 1);
+        }
+        ins = aa + 1;
+        PT(P, ins) = mt;
+        nm = m + 1;
+    }
+    // locate M again (the deletion may have moved it) and refresh the two new vertices
+    int km = 0;
+    while (km < nm && PT(P, km) != mt) ++km;
+    if (km == nm) return -1;
+    const int kp = km > 0 ? km - 1 : nm - 1, kn = km + 1 < nm ? km + 1 : 0;
+    const Line Lp = nat_line(a, Q, B, PT(P, kp), pi), Ln = nat_line(a, Q, B, PT(P, kn), pi);
+    double x, y;
+    nat_meet(Lp, M, x, y);
+    PX(P, kp) = x; PY(P, kp) = y;
+    nat_meet(M, Ln, x, y);
+    PX(P, km) = x; PY(P, km) = y;
+    return nm;
+}
+
+__device__ double nat_radius(Poly P, int m)
+{
+    double r2 = 0.0;
+    for (int k = 0; k < m; ++k) r2 = fmax(r2, PX(P, k) * PX(P, k) + PY(P, k) * PY(P, k));
+    return sqrt(r2);
+}
+
+__device__ double nat_area(Poly P, int m)
+{
+    double s = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const int k0 = k > 0 ? k - 1 : m - 1;
+        s += PX(P, k0) * PY(P, k) - PX(P, k) * PY(P, k0);
+    }
+    return 0.5 * s;
+}
+
+__global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
+{
+    __shared__ int s_tag[2][kNatMax * kNatThreads];
+    __shared__ double s_vx[2][kNatMax * kNatThreads], s_vy[2][kNatMax * kNatThreads];
+    const int t = threadIdx.x;
+    Poly V{&s_tag[0][t], &s_vx[0][t], &s_vy[0][t]};
+    Poly R{&s_tag[1][t], &s_vx[1][t], &s_vy[1][t]};
+    const int n_in = a.ctl[2];
+    for (int li = blockIdx.x * kNatThreads + t; li < n_in; li += gridDim.x * kNatThreads) {
+        const int q = a.lin[li];
+        const double2 Q = a.xy[q];
+        const double B = 1e6 * (a.h * (a.BX + a.BY) + 1.0);
+        const double2 zero = make_double2(0.0, 0.0);
+        // initial square: lines x <= B, y <= B, -x <= B, -y <= B (counter-clockwise)
+        int m = 4;
+        for (int k = 0; k < 4; ++k) PT(V, k) = -1 - k;
+        for (int k = 0; k < 4; ++k) {
+            double x, y;
+            nat_meet(nat_line(a, Q, B, -1 - k, zero), nat_line(a, Q, B, -1 - ((k + 1) & 3), zero),
+                     x, y);
+            PX(V, k) = x; PY(V, k) = y;
+        }
+        double rho = nat_radius(V, m);
+        int bx = (int)floor((Q.x - a.ox) / a.h), by = (int)floor((Q.y - a.oy) / a.h);
+        bx = bx < 0 ? 0 : (bx >= a.BX ? a.BX - 1 : bx);
+        by = by < 0 ? 0 : (by >= a.BY ? a.BY - 1 : by);
+        const int rmax = max(max(bx, a.BX - 1 - bx), max(by, a.BY - 1 - by));
+        bool fail = false;
+        // the good nodes of bucket (i, j) cut V(q)
+        auto bucket = [&](int i, int j) {
+            if (fail || i < 0 || i >= a.BX || j < 0 || j >= a.BY) return;
+            const int b = j * a.BX + i;
+            for (int s = a.bstart[b]; s < a.bstart[b + 1]; ++s) {
+                const int p = a.bnode[s];
+                if (p == q || a.bad[p]) continue;
+                const double2 P = a.xy[p];
+                const double px = P.x - Q.x, py = P.y - Q.y;
+                if (px * px + py * py >= 4.0 * rho * rho) continue;  // cannot reach the cell
+                const Line M{px, py, 0.5 * (px * px + py * py)};
+                m = nat_clip(a, Q, B, zero, V, m, M, p);
+                if (m <= 0) { fail = true; return; }
+                rho = nat_radius(V, m);
+            }
+        };
+        for (int r = 0; r <= rmax && !fail; ++r) {
+            // rings 0 .. r-1 done: every node left is more than (r - 1) h from q
+            if (r > 0 && (double)(r - 1) * a.h >= 2.0 * rho) break;
+            if (r == 0) {
+                bucket(bx, by);
+                continue;
+            }
+            for (int i = bx - r; i <= bx + r; ++i) { bucket(i, by - r); bucket(i, by + r); }
+            for (int j = by - r + 1; j <= by + r - 1; ++j) { bucket(bx - r, j); bucket(bx + r, j); }
+        }
+        for (int k = 0; k < m && !fail; ++k)
+            if (PT(V, k) < 0) fail = true;
+        double2 u = make_double2(NAN, NAN);
+        double4 f = make_double4(NAN, NAN, NAN, NAN);
+        if (!fail) {
+            double tot = 0.0;
+            double2 su = make_double2(0.0, 0.0);
+            double4 sf = make_double4(0.0, 0.0, 0.0, 0.0);
+            for (int k = 0; k < m && !fail; ++k) {
+                const int ti = PT(V, k);
+                const double2 Pi = a.xy[ti];
+                const double2 pi = make_double2(Pi.x - Q.x, Pi.y - Q.y);
+                for (int l = 0; l < m; ++l) {
+                    PT(R, l) = PT(V, l); PX(R, l) = PX(V, l); PY(R, l) = PY(V, l);
+                }
+                int mr = m;
+                for (int l = 0; l < m && mr > 0; ++l) {
+                    if (l == k) continue;
+                    const int tj = PT(V, l);
+                    const double2 Pj = a.xy[tj];
+                    const double pjx = Pj.x - Q.x, pjy = Pj.y - Q.y;
+                    const Line M{pjx - pi.x, pjy - pi.y,
+                                 0.5 * ((pjx * pjx + pjy * pjy) - (pi.x * pi.x + pi.y * pi.y))};
+                    mr = nat_clip(a, Q, B, pi, R, mr, M, tj | kRegion);
+                    if (mr < 0) { fail = true; break; }
+                }
+                if (fail) break;
+                const double w = mr > 0 ? nat_area(R, mr) : 0.0;
+                tot += w;
+                const double2 uv = a.U[ti];
+                su.x += w * uv.x; su.y += w * uv.y;
+                if (a.F) {
+                    const double4 fv = a.F[ti];
+                    sf.x += w * fv.x; sf.y += w * fv.y; sf.z += w * fv.z; sf.w += w * fv.w;
+                }
+            }
+            if (!fail) {
+                const double it = 1.0 / tot;
+                u = make_double2(su.x * it, su.y * it);
+                f = make_double4(sf.x * it, sf.y * it, sf.z * it, sf.w * it);
+            }
+        }
+        if (fail) nat_error(a);
+        a.U[q] = u;
+        if (a.F) a.F[q] = f;
+    }
+}
+
+#undef PT
+#undef PX
+#undef PY
+
+// ---- hull-list nodes: one workgroup each --------------------------------------------
+struct KeyI {
+    double k;
+    int i;
+};
+
+// Block minimum of (k, i) lexicographically (every thread gets the result).
+__device__ KeyI block_min(KeyI v, KeyI *sh)
+{
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double k2 = __shfl_xor(v.k, o, kWave);
+        const int i2 = __shfl_xor(v.i, o, kWave);
+        if (k2 < v.k || (k2 == v.k && i2 < v.i)) { v.k = k2; v.i = i2; }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    KeyI r = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+        if (sh[w].k < r.k || (sh[w].k == r.k && sh[w].i < r.i)) r = sh[w];
+    return r;
+}
+
+__device__ __forceinline__ bool on_seg(double2 A, double2 C, double2 P, double &s)
+{
+    const double ex = C.x - A.x, ey = C.y - A.y, px = P.x - A.x, py = P.y - A.y;
+    const double L2 = ex * ex + ey * ey;
+    const double cr = ex * py - ey * px;
+    if (fabs(cr) > 1e-12 * sqrt(L2) * sqrt(px * px + py * py)) return false;
+    s = (px * ex + py * ey) / L2;
+    return s >= 0.0 && s <= 1.0;
+}
+
+__device__ void plane_at(const NatArgs &a, int ia, int ib, int ic, double2 Q, double2 &u,
+                         double4 &f)
+{
+    const double2 A = a.xy[ia], Bp = a.xy[ib], Cp = a.xy[ic];
+    const double bx = Bp.x - A.x, by = Bp.y - A.y, cx = Cp.x - A.x, cy = Cp.y - A.y;
+    const double qx = Q.x - A.x, qy = Q.y - A.y;
+    const double den = bx * cy - by * cx;
+    const double al = (qx * cy - qy * cx) / den, be = (bx * qy - by * qx) / den;
+    const double2 ua = a.U[ia], ub = a.U[ib], uc = a.U[ic];
+    u = make_double2(ua.x + al * (ub.x - ua.x) + be * (uc.x - ua.x),
+                     ua.y + al * (ub.y - ua.y) + be * (uc.y - ua.y));
+    if (a.F) {
+        const double4 fa = a.F[ia], fb = a.F[ib], fc = a.F[ic];
+        f = make_double4(fa.x + al * (fb.x - fa.x) + be * (fc.x - fa.x),
+                         fa.y + al * (fb.y - fa.y) + be * (fc.y - fa.y),
+                         fa.z + al * (fb.z - fa.z) + be * (fc.z - fa.z),
+                         fa.w + al * (fb.w - fa.w) + be * (fc.w - fa.w));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nat_hullfill(NatArgs a)
+{
+    __shared__ KeyI sh[4];
+    __shared__ int s_case, s_e, s_v;
+    __shared__ double s_t;
+    const int nh = a.ctl[3], h = a.ctl[0], ng = a.ctl[1];
+    for (int li = blockIdx.x; li < nh; li += gridDim.x) {
+        const int q = a.lhull[li];
+        const double2 Q = a.xy[q];
+        double2 u = make_double2(NAN, NAN);
+        double4 f = make_double4(NAN, NAN, NAN, NAN);
+        if (ng == 0) {
+            // no good node
+        } else if (h < 3 || ng < 3) {
+            KeyI best{INFINITY, 0x7fffffff};
+            for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
+                if (a.bad[p]) continue;
+                const double2 P = a.xy[p];
+                const double d = (P.x - Q.x) * (P.x - Q.x) + (P.y - Q.y) * (P.y - Q.y);
+                if (d < best.k || (d == best.k && p < best.i)) { best.k = d; best.i = p; }
+            }
+            best = block_min(best, sh);
+            u = a.U[best.i];
+            if (a.F) f = a.F[best.i];
+        } else {
+            if (threadIdx.x == 0) {
+                int be = 0;
+                double bt = 0.0, bd = INFINITY;
+                for (int e = 0; e < h; ++e) {
+                    const double2 A = a.xy[a.hull[e]], C = a.xy[a.hull[e + 1 < h ? e + 1 : 0]];
+                    const double ex = C.x - A.x, ey = C.y - A.y;
+                    double tt = ((Q.x - A.x) * ex + (Q.y - A.y) * ey) / (ex * ex + ey * ey);
+                    tt = tt < 0.0 ? 0.0 : (tt > 1.0 ? 1.0 : tt);
+                    const double dx = A.x + tt * ex - Q.x, dy = A.y + tt * ey - Q.y;
+                    const double d = dx * dx + dy * dy;
+                    if (d < bd) { bd = d; be = e; bt = tt; }
+                }
+                s_e = be;
+                s_t = bt;
+                s_case = (bt > 0.0 && bt < 1.0) ? 0 : 1;
+                s_v = bt <= 0.0 ? be : (be + 1 < h ? be + 1 : 0);
+            }
+            __syncthreads();
+            const int cs = s_case;
+            if (cs == 0) {
+                const int A = a.hull[s_e], C = a.hull[s_e + 1 < h ? s_e + 1 : 0];
+                const double bt = s_t;
+                const double2 XA = a.xy[A], XC = a.xy[C];
+                // a: largest s <= bt; c: smallest s > bt (ties: lowest index)
+                KeyI ma{0.0, A}, mc{1.0, C};
+                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
+                    double s;
+                    if (a.bad[p] || !on_seg(XA, XC, a.xy[p], s)) continue;
+                    if (s <= bt && (-s < -ma.k || (s == ma.k && p < ma.i))) { ma.k = s; ma.i = p; }
+                    if (s > bt && (s < mc.k || (s == mc.k && p < mc.i))) { mc.k = s; mc.i = p; }
+                }
+                KeyI na{-ma.k, ma.i};
+                na = block_min(na, sh);
+                mc = block_min(mc, sh);
+                const int ia = na.i, ic = mc.i;
+                const double2 Pa = a.xy[ia], Pc = a.xy[ic];
+                const double ex = Pc.x - Pa.x, ey = Pc.y - Pa.y;
+                const double L = sqrt(ex * ex + ey * ey);
+                const double mx = Pa.x + 0.5 * ex, my = Pa.y + 0.5 * ey, nxv = -ey / L, nyv = ex / L;
+                KeyI bs{INFINITY, 0x7fffffff};
+                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
+                    if (a.bad[p]) continue;
+                    const double2 P = a.xy[p];
+                    const double px = P.x - Pa.x, py = P.y - Pa.y;
+                    const double cr = ex * py - ey * px;
+                    if (!(cr > 1e-12 * L * sqrt(px * px + py * py))) continue;
+                    const double dx = P.x - mx, dy = P.y - my;
+                    const double s = (dx * dx + dy * dy - 0.25 * L * L) / (2.0 * (dx * nxv + dy * nyv));
+                    if (s < bs.k) { bs.k = s; bs.i = p; }
+                }
+                bs = block_min(bs, sh);
+                const double lim = bs.k + 1e-12 * (fabs(bs.k) + L);
+                KeyI br{0.0, 0x7fffffff};
+                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
+                    if (a.bad[p]) continue;
+                    const double2 P = a.xy[p];
+                    const double px = P.x - Pa.x, py = P.y - Pa.y;
+                    const double cr = ex * py - ey * px;
+                    if (!(cr > 1e-12 * L * sqrt(px * px + py * py))) continue;
+                    const double dx = P.x - mx, dy = P.y - my;
+                    const double s = (dx * dx + dy * dy - 0.25 * L * L) / (2.0 * (dx * nxv + dy * nyv));
+                    if (s <= lim && p < br.i) br.i = p;
+                }
+                br = block_min(br, sh);
+                if (br.i < a.nn) plane_at(a, ia, ic, br.i, Q, u, f);
+                else if (threadIdx.x == 0) nat_error(a);
+            } else {
+                const int vi = s_v;
+                const int v = a.hull[vi], Pv = a.hull[vi > 0 ? vi - 1 : h - 1],
+                          Nv = a.hull[vi + 1 < h ? vi + 1 : 0];
+                const double2 XP = a.xy[Pv], XV = a.xy[v], XN = a.xy[Nv];
+                KeyI k1{-0.0, Pv}, k2{1.0, Nv};  // u1: largest s < 1 on P->v; u2: smallest s > 0 on v->N
+                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
+                    double s;
+                    if (a.bad[p] || p == v) continue;
+                    const double2 X = a.xy[p];
+                    if (on_seg(XP, XV, X, s) && s < 1.0 && (-s < k1.k || (-s == k1.k && p < k1.i))) {
+                        k1.k = -s; k1.i = p;
+                    }
+                    if (on_seg(XV, XN, X, s) && s > 0.0 && (s < k2.k || (s == k2.k && p < k2.i))) {
+                        k2.k = s; k2.i = p;
+                    }
+                }
+                k1 = block_min(k1, sh);
+                k2 = block_min(k2, sh);
+                plane_at(a, v, k2.i, k1.i, Q, u, f);
+            }
+        }
+        if (threadIdx.x == 0) {
+            a.U[q] = u;
+            if (a.F) a.F[q] = f;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_nat_clear(int *err) { *err = 0; }
+
+}  // namespace
+
+static hipError_t natgrid_setup_impl(NatGrid &g, const double *coords, int nn, hipStream_t st)
+{
+    g.nn = nn;
+    if (nn <= 0) return hipSuccess;
+    double xmin = coords[0], xmax = coords[0], ymin = coords[1], ymax = coords[1];
+    for (int i = 1; i < nn; ++i) {
+        xmin = std::min(xmin, coords[2 * i]); xmax = std::max(xmax, coords[2 * i]);
+        ymin = std::min(ymin, coords[2 * i + 1]); ymax = std::max(ymax, coords[2 * i + 1]);
+    }
+    if (!std::isfinite(xmin) || !std::isfinite(xmax) || !std::isfinite(ymin) || !std::isfinite(ymax))
+        return hipErrorInvalidValue;
+    // bucket side ~ the mean node spacing (one node per bucket on a uniform grid), and at
+    // least extent / (2 sqrt(nn) + 1), so a thin or collinear set still gets at most
+    // (2 sqrt(nn) + 3)^2 buckets
+    const double ext = std::max(xmax - xmin, ymax - ymin);
+    const double rn = std::sqrt((double)nn);
+    double h = std::sqrt((xmax - xmin) * (ymax - ymin) / nn);
+    h = std::max(h, ext / (2.0 * rn + 1.0));
+    // a uniform lattice: its spacing (node i+1 to the right of node i)
+    if (nn >= 2 && coords[3] == coords[1] && coords[2] > coords[0]) {
+        const double sp = coords[2] - coords[0];
+        if (std::fabs(sp - h) < 0.5 * h) h = sp;
+    }
+    if (!(h > 0)) h = 1.0;
+    g.h = h;
+    g.ox = xmin - 0.5 * h;
+    g.oy = ymin - 0.5 * h;
+    g.BX = (int)std::floor((xmax - g.ox) / h) + 1;
+    g.BY = (int)std::floor((ymax - g.oy) / h) + 1;
+    const size_t nb = (size_t)g.BX * g.BY;
+    std::vector<int> cnt(nb + 1, 0), bnode(nn), rnode(nn);
+    std::vector<int> bk(nn);
+    for (int i = 0; i < nn; ++i) {
+        int bx = (int)std::floor((coords[2 * i] - g.ox) / h);
+        int by = (int)std::floor((coords[2 * i + 1] - g.oy) / h);
+        bx = std::min(std::max(bx, 0), g.BX - 1);
+        by = std::min(std::max(by, 0), g.BY - 1);
+        bk[i] = by * g.BX + bx;
+        cnt[bk[i] + 1]++;
+    }
+    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < nn; ++i) bnode[pos[bk[i]]++] = i;
+    // rows: equal y, x ascending
+    for (int i = 0; i < nn; ++i) rnode[i] = i;
+    std::sort(rnode.begin(), rnode.end(), [&](int p, int q) {
+        if (coords[2 * p + 1] != coords[2 * q + 1]) return coords[2 * p + 1] < coords[2 * q + 1];
+        if (coords[2 * p] != coords[2 * q]) return coords[2 * p] < coords[2 * q];
+        return p < q;
+    });
+    std::vector<int> rstart;
+    for (int k = 0; k < nn; ++k)
+        if (k == 0 || coords[2 * rnode[k] + 1] != coords[2 * rnode[k - 1] + 1]) rstart.push_back(k);
+    rstart.push_back(nn);
+    g.R = (int)rstart.size() - 1;
+    hipError_t e;
+    if ((e = g.bstart.ensure(sizeof(int) * (nb + 1))) || (e = g.bnode.ensure(sizeof(int) * nn)) ||
+        (e = g.rstart.ensure(sizeof(int) * (g.R + 1))) || (e = g.rnode.ensure(sizeof(int) * nn)) ||
+        (e = g.work.ensure(sizeof(int) * (3 * (size_t)g.R + 6 * (size_t)g.R + 4 +
+                                          2 * (size_t)g.R + 2 + 8 + 2 * (size_t)nn))))
+        return e;
+    if ((e = hipMemcpyAsync(g.bstart.p, cnt.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(g.bnode.p, bnode.data(), sizeof(int) * nn, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(g.rstart.p, rstart.data(), sizeof(int) * (g.R + 1), hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(g.rnode.p, rnode.data(), sizeof(int) * nn, hipMemcpyHostToDevice, st)))
+        return e;
+    return hipStreamSynchronize(st);  // the host vectors go out of scope
+}
+
+hipError_t natgrid_setup(NatGrid &g, const double *coords, int nn, hipStream_t st)
+{
+    try {
+        return natgrid_setup_impl(g, coords, nn, st);
+    } catch (...) {  // host allocation: no exception crosses the C ABI
+        return hipErrorOutOfMemory;
+    }
+}
+
+hipError_t natural_err_clear(int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_nat_clear, dim3(1), dim3(1), 0, st, err);
+    return hipGetLastError();
+}
+
+hipError_t natural_fill(const NatGrid &g, const double *coords, const int *bad, double *U,
+                        double *F, hipStream_t st, int *err)
+{
+    if (g.nn <= 0) return hipSuccess;
+    NatArgs a{};
+    a.xy = reinterpret_cast<const double2 *>(coords);
+    a.bad = bad;
+    a.U = reinterpret_cast<double2 *>(U);
+    a.F = reinterpret_cast<double4 *>(F);
+    a.nn = g.nn;
+    a.BX = g.BX;
+    a.BY = g.BY;
+    a.ox = g.ox;
+    a.oy = g.oy;
+    a.h = g.h;
+    a.bstart = g.bstart.as<int>();
+    a.bnode = g.bnode.as<int>();
+    a.R = g.R;
+    a.rstart = g.rstart.as<int>();
+    a.rnode = g.rnode.as<int>();
+    int *w = g.work.as<int>();
+    a.rows = w;
+    a.chain = a.rows + 3 * (size_t)g.R;
+    a.hull = a.chain + 6 * (size_t)g.R + 4;
+    a.ctl = a.hull + 2 * (size_t)g.R + 2;
+    a.lin = a.ctl + 8;
+    a.lhull = a.lin + g.nn;
+    a.err = err;
+    hipLaunchKernelGGL(k_nat_rows, dim3((g.R + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_nat_hull, dim3(1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_nat_classify, dim3((g.nn + 255) / 256), dim3(256), 0, st, a);
+    const int nbs = std::min((g.nn + kNatThreads - 1) / kNatThreads, 4096);
+    hipLaunchKernelGGL(k_nat_sibson, dim3(nbs), dim3(kNatThreads), 0, st, a);
+    const int nbh = std::min(g.nn, 1024);
+    hipLaunchKernelGGL(k_nat_hullfill, dim3(nbh), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+int natural_errors(const NatGrid &g, hipStream_t st)
+{
+    int ctl[5] = {0, 0, 0, 0, 0};
+    const int *w = g.work.as<int>();
+    if (!w) return 0;
+    const int *c = w + 3 * (size_t)g.R + 6 * (size_t)g.R + 4 + 2 * (size_t)g.R + 2;
+    if (hipMemcpyAsync(ctl, c, sizeof ctl, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    return ctl[4];
+}
+
+}  // namespace aldic

@@ -3,8 +3,9 @@
 Same names and argument meaning as the reference functions; the compute runs in the HIP
 library (include/aldic.h: aldic_set_quadtree, aldic_subpb2_quadtree,
 aldic_nodal_strain_quadtree, aldic_icgn2).  This module holds the host geometry the
-reference leaves to MATLAB's scatteredInterpolant: the linear interpolation weights of a
-Delaunay triangulation (built once per mesh, like the mesh itself).
+reference leaves to MATLAB's scatteredInterpolant 'linear' (the nodal strain's Gauss-point
+interpolation weights of a Delaunay triangulation, built once per mesh, like the mesh
+itself); its 'natural' refill of bad nodes runs in the library (aldic_natural_fill).
 
 Mesh convention: DICmesh['coordinatesFEM'] (nn, 2); DICmesh['elementsFEM'] (E, 8),
 1-based, 0 where an edge has no midside node (GenerateQuadtreeMesh.m); optional
@@ -56,17 +57,20 @@ def linear_weights(points, queries):
 
 
 def scattered_fill(coords, V, nc, bad):
-    """Refill the bad nodes of a scattered node set by linear interpolation of the good
-    ones (the reference uses scatteredInterpolant 'natural', Subpb1Quadtree.m:194-199;
-    equal for a linear field)."""
+    """scatteredInterpolant(good nodes, 'natural') at the bad nodes (Subpb1Quadtree.m:
+    188-199) on the device (aldic_natural_fill); nc = 2 (U) or 4 (F) values per node."""
     coords = np.asarray(coords, np.float64)
     bad = np.asarray(bad, bool)
-    V = np.asarray(V, np.float64).reshape(-1, nc).copy()
-    if bad.any() and (~bad).sum() >= 3:
-        good = np.flatnonzero(~bad)
-        tri, w = linear_weights(coords[good], coords[bad])
-        V[bad] = np.einsum("nk,nkc->nc", w, V[good][tri])
-    return V.ravel()
+    V = np.asarray(V, np.float64).ravel()
+    if not bad.any():
+        return V.copy()
+    ctx = get_context()
+    if nc == 2:
+        return ctx.natural_fill(coords, bad, V)[0]
+    if nc == 4:
+        U0 = np.zeros(2 * coords.shape[0])
+        return ctx.natural_fill(coords, bad, U0, V)[1]
+    raise ValueError("nc must be 2 or 4")
 
 
 def _ensure_quadtree(DICmesh):

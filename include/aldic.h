@@ -166,6 +166,19 @@ int aldic_nodal_strain_quadtree(aldic_ctx *ctx, const double *U, const int *tri,
  * swap_fix = 1 writes [u_x v_x u_y v_y]. */
 int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F);
 
+/* LocalICGN.m:141-161, Subpb1Quadtree.m:188-199: scatteredInterpolant(good nodes, values,
+ * 'natural') evaluated at every node.  nn nodes (nn x 2 coordinates), bad[i] != 0 marks a
+ * node with no datum; U (2 nn) and F (4 nn, nullable) are refilled in place at the bad
+ * nodes (good nodes keep their values).  Inside the good nodes' convex hull: Sibson's
+ * natural-neighbour interpolant (Voronoi area stolen by the node); on or outside it (where
+ * MATLAB extrapolates 'linear'ly from "boundary gradients", which is not documented
+ * further): the plane through the good nodes next to the nearest hull point and the
+ * Delaunay third vertex of that hull edge (at a hull vertex: the vertex and its two hull
+ * neighbours); fewer than three non-collinear good nodes: the nearest good node; none:
+ * NaN.  ALDIC_ERR_NUMERIC if a Voronoi cell could not be formed (its node gets NaN). */
+int aldic_natural_fill(aldic_ctx *ctx, int nn, const double *coords, const int *bad, double *U,
+                       double *F);
+
 /* ba_interp2.cpp:318-335 (funICGN.m:125): Keys-cubic samples of the deformed image
  * at (X[i], Y[i]) (X along dim 1), clamp-to-edge. */
 int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, double *Z);

@@ -17,13 +17,13 @@ Pinning of the functions restated here (tests/test_oracle_pinning.py):
                           whose positions and order are checked in closed form
   gather_weights          weight rows that select single points and linear combinations
   badpoint                hand-counted thresholds of rules 0, 1 and 2
-  mesh_fill, axis_fill    symmetric stencils reproduce linear fields; reach per sweep;
-                          the all-bad case
+  sibson, natural_fill    1/4 weights of an isolated lattice hole, weights >= 0 summing to
+                          1 with linear precision, brute-force area sampling, the hull
+                          rule's closed forms (tests/test_oracle_natural.py)
   gauss_kernel, nanconv_edge, smooth_nodal, beta_choice
                           fspecial / scipy.ndimage equivalents, brute force, the
                           parabola vertex
-  frame                   composed of the above; the bad-node refill (axis_fill) stands
-                          in for scatteredInterpolant 'natural': parity unpinned there
+  frame                   composed of the above
 """
 from __future__ import annotations
 
@@ -75,6 +75,10 @@ def lib():
         L.orc_pcg.argtypes = [I, i32p, i32p, f64p, f64p, f64p, D, I]
         L.orc_pcg.restype = I
         L.orc_nodal_strain.argtypes = [f64p, I, i32p, I, f64p, I, f64p]
+        L.orc_sibson.argtypes = [I, f64p, D, D, i32p, f64p, I]
+        L.orc_sibson.restype = I
+        L.orc_natural_fill.argtypes = [I, f64p, i32p, I, f64p, i32p]
+        L.orc_natural_fill.restype = I
         for fn in ("orc_normalize", "orc_gradient", "orc_bicubic", "orc_ic_compose", "orc_icgn6",
                    "orc_icgn2", "orc_search_grid", "orc_ncc_full", "orc_findpeak", "orc_search",
                    "orc_subpb2_assemble", "orc_nodal_strain"):
@@ -552,69 +556,35 @@ def badpoint(iters, mean_rule):
     return bad
 
 
-def mesh_fill(coords, elems8, V, nc, bad, sweeps=8):
-    """Refill of bad nodes on a mesh (the reference interpolates them with
-    scatteredInterpolant 'natural'; this rule is the library's): in each sweep every node
-    not yet known takes the inverse-distance average of its known neighbours (nodes
-    sharing an element), the known set growing after the sweep; after `sweeps` sweeps
-    the rest get 0."""
-    coords = np.asarray(coords, np.float64).reshape(-1, 2)
-    el = np.asarray(elems8, np.int64).reshape(-1, 8)
+def sibson(pxy, q, maxn=512):
+    """natural_oracle.c orc_sibson: Sibson natural-neighbour coordinates of point q with
+    respect to the points pxy (n, 2), q strictly inside their convex hull -> (indices,
+    weights) of the natural neighbours; raises if the cell cannot be formed."""
+    pxy = _f64(pxy).reshape(-1, 2)
+    idx = np.zeros(maxn, np.int32)
+    w = np.zeros(maxn)
+    n = lib().orc_sibson(pxy.shape[0], pxy, float(q[0]), float(q[1]), idx, w, maxn)
+    if n < 0:
+        raise ValueError(f"orc_sibson failed ({n})")
+    return idx[:n].copy(), w[:n].copy()
+
+
+def natural_fill(coords, V, nc, bad, return_kind=False):
+    """LocalICGN.m:141-161 / Subpb1Quadtree.m:188-199: scatteredInterpolant(good nodes,
+    'natural') of nc interleaved values per node evaluated at every node (natural_oracle.c:
+    Sibson inside the good nodes' hull, the chosen linear rule on and outside it).  Good
+    nodes keep their values.  kind per node: 0 good, 1 Sibson, 2 hull edge, 3 hull vertex,
+    4 nearest good node (degenerate hull), 5 no good node."""
+    coords = _f64(coords).reshape(-1, 2)
     nn = coords.shape[0]
-    nbr = [set() for _ in range(nn)]
-    for e in el:
-        nd = [int(v) for v in e if v >= 0]
-        for a in nd:
-            nbr[a].update(nd)
-    V = np.asarray(V, np.float64).reshape(nn, nc).copy()
-    known = ~np.asarray(bad, bool)
-    for _ in range(sweeps):
-        new = known.copy()
-        out = V.copy()
-        for i in np.flatnonzero(~known):
-            js = [j for j in sorted(nbr[i]) if j != i and known[j]]
-            if not js:
-                continue
-            w = np.array([1.0 / np.hypot(*(coords[j] - coords[i])) for j in js])
-            out[i] = (w[:, None] * V[js]).sum(0) / w.sum()
-            new[i] = True
-        V, known = out, new
-    V[~known] = 0.0
-    return V.ravel()
-
-
-def axis_fill(V, nc, bad, Nx, Ny):
-    """Fill bad nodes of an Nx x Ny node grid (x fastest, nc interleaved components):
-    each takes the inverse-distance average of the nearest good node along the four grid
-    directions (left, right, down, up, summed in that order); 0 when none exists."""
-    G = np.asarray(V, dtype=np.float64).reshape(Ny, Nx, nc)
-    B = np.asarray(bad, dtype=bool).reshape(Ny, Nx)
-    yy, xx = np.mgrid[0:Ny, 0:Nx]
-    A = np.where(~B, xx, -1)
-    L = np.concatenate([np.full((Ny, 1), -1), np.maximum.accumulate(A, axis=1)[:, :-1]], axis=1)
-    A = np.where(~B, xx, Nx)
-    R = np.concatenate([np.minimum.accumulate(A[:, ::-1], axis=1)[:, ::-1][:, 1:],
-                        np.full((Ny, 1), Nx)], axis=1)
-    A = np.where(~B, yy, -1)
-    D = np.concatenate([np.full((1, Nx), -1), np.maximum.accumulate(A, axis=0)[:-1, :]], axis=0)
-    A = np.where(~B, yy, Ny)
-    Up = np.concatenate([np.minimum.accumulate(A[::-1, :], axis=0)[::-1, :][1:, :],
-                         np.full((1, Nx), Ny)], axis=0)
-    srcs = []
-    for ok, idx_y, idx_x, dist in ((L >= 0, yy, np.clip(L, 0, Nx - 1), xx - L),
-                                   (R < Nx, yy, np.clip(R, 0, Nx - 1), R - xx),
-                                   (D >= 0, np.clip(D, 0, Ny - 1), xx, yy - D),
-                                   (Up < Ny, np.clip(Up, 0, Ny - 1), xx, Up - yy)):
-        w = np.where(ok, 1.0 / np.maximum(dist, 1), 0.0)
-        srcs.append((ok, w, G[idx_y, idx_x]))
-    ws = np.zeros((Ny, Nx))
-    s = np.zeros((Ny, Nx, nc))
-    for ok, w, v in srcs:
-        ws = ws + w
-        s = s + np.where(ok[:, :, None], w[:, :, None] * v, 0.0)
-    fillv = np.where(ws[:, :, None] > 0, s / np.where(ws > 0, ws, 1.0)[:, :, None], 0.0)
-    out = np.where(B[:, :, None], fillv, G)
-    return out.reshape(-1)
+    out = _f64(V).reshape(-1).copy()
+    assert out.size == nc * nn
+    kind = np.zeros(nn, np.int32)
+    rc = lib().orc_natural_fill(nn, coords, _i32(np.asarray(bad, bool).astype(np.int32)), nc,
+                                out, kind)
+    if rc != 0:
+        raise ValueError("orc_natural_fill: a Sibson cell could not be formed")
+    return (out, kind) if return_kind else out
 
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
@@ -622,9 +592,9 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
     refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1: 2-DOF IC-GN, bad
     points, gridfit of every node; Subpb2, strain, duals).  The LocalICGN refill of bad
-    nodes (the reference's scatteredInterpolant 'natural', LocalICGN.m:144-161) is
-    axis_fill; parity unpinned.  subpb1_fit=False replaces Subpb1's gridfit with the
-    same axis_fill of its bad nodes.  smooth=True adds main_ALDIC.m's smoothing:
+    nodes is the reference's scatteredInterpolant 'natural' (LocalICGN.m:144-161,
+    natural_fill).  subpb1_fit=False replaces Subpb1's gridfit with the same
+    natural_fill of its bad nodes.  smooth=True adds main_ALDIC.m's smoothing:
     funSmoothStrain of F1 after LocalICGN (:233) and of F2 after every global step
     (:320, :407), funSmoothDisp of U1 after Subpb1 and of U2 after Subpb2 in the ADMM
     steps (:381, :400).  beta_sweep=True picks beta as main_ALDIC.m:237,274-301 does
@@ -655,8 +625,8 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     U1 = U6.copy(); F1 = F6.copy()
     U1.reshape(-1, 2)[bad] = np.nan
     F1.reshape(-1, 4)[bad] = np.nan
-    U1 = axis_fill(U1, 2, bad, Nx, Ny)
-    F1 = axis_fill(F1, 4, bad, Nx, Ny)
+    U1 = natural_fill(xy, U1, 2, bad)
+    F1 = natural_fill(xy, F1, 4, bad)
     T["post6"] = time.perf_counter() - t
     el = []
     for l in range(Ny - 1):
@@ -706,7 +676,7 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
         if subpb1_fit:
             U1 = subpb1_smooth(U1, xy)              # Subpb1.m:185-199
         else:
-            U1 = axis_fill(U1, 2, bad, Nx, Ny)
+            U1 = natural_fill(xy, U1, 2, bad)
         F1 = F2
         u1_saved = U1.copy()
         if smooth:

@@ -390,28 +390,6 @@ def test_q4_mass_is_kronecker_of_1d_mass(orc):
     assert np.allclose(M, np.kron(M1(gy), M1(gx)), atol=1e-12)
 
 
-def test_axis_fill_rules(orc, pkg):
-    # isolated bad node -> mean of its 4 neighbours; distances weight longer gaps less;
-    # good nodes untouched; the host glue's copy computes the same values
-    Nx, Ny = 7, 5
-    rng = np.random.default_rng(2)
-    V = rng.normal(size=(Nx * Ny * 2))
-    bad = np.zeros(Nx * Ny, bool)
-    bad[2 * Nx + 3] = True                         # isolated
-    bad[[1 * Nx + 1, 1 * Nx + 2]] = True           # a pair in row 1
-    out = orc.axis_fill(V, 2, bad, Nx, Ny)
-    G = V.reshape(Ny, Nx, 2)
-    exp = (G[2, 2] + G[2, 4] + G[1, 3] + G[3, 3]) / 4
-    assert np.allclose(out.reshape(Ny, Nx, 2)[2, 3], exp, atol=1e-15)
-    # node (1,1): left (1,0) d=1, right (1,3) d=2, down (0,1) d=1, up (2,1) d=1
-    w = np.array([1, 0.5, 1, 1])
-    v = np.stack([G[1, 0], G[1, 3], G[0, 1], G[2, 1]])
-    assert np.allclose(out.reshape(Ny, Nx, 2)[1, 1], (w[:, None] * v).sum(0) / w.sum())
-    keep = ~bad
-    assert np.array_equal(out.reshape(-1, 2)[keep], V.reshape(-1, 2)[keep])
-    assert np.array_equal(pkg.aldic.axis_fill(V, 2, bad, Nx, Ny), out)
-
-
 # ---- gridfit (Subpb1's smoothing) ---------------------------------------------------
 
 def _grid_nodes(nx, ny, hx, hy, x0=10.0, y0=20.0):
@@ -625,42 +603,6 @@ def test_linear_weights_reproduce_linear_fields(pkg):
     assert inside.sum() > 100 and (~inside).sum() > 50
     gq = np.einsum("nk,nk->n", w, g[tri])
     assert np.max(np.abs(gq[inside] - ref[inside])) < 1e-9
-    # scattered_fill: bad nodes of a linear field come back exactly
-    bad = rng.random(300) < 0.2
-    V = np.column_stack([f, 2 * f]).ravel()
-    Vb = V.copy()
-    Vb[np.repeat(bad, 2)] = np.nan
-    assert np.allclose(qt.scattered_fill(P, Vb, 2, bad), V, atol=1e-10)
-
-
-def test_mesh_fill_rule(orc):
-    # an interior bad node of a uniform Q4 grid sees 8 neighbours placed symmetrically,
-    # so the inverse-distance average reproduces a linear field exactly; a bad block
-    # fills from the outside in, one ring per sweep; what no sweep reaches gets 0
-    from meshes import quadtree_mesh
-    xy, el = quadtree_mesh(nc=6, refine=())
-    nn = len(xy)
-    f = 0.5 + 0.01 * xy[:, 0] - 0.03 * xy[:, 1]
-    V = np.column_stack([f, -f]).ravel()
-    inner = np.flatnonzero((xy[:, 0] > xy[:, 0].min()) & (xy[:, 0] < xy[:, 0].max()) &
-                           (xy[:, 1] > xy[:, 1].min()) & (xy[:, 1] < xy[:, 1].max()))
-    bad = np.zeros(nn, bool)
-    bad[inner[len(inner) // 2]] = True
-    out = orc.mesh_fill(xy, el, np.where(np.repeat(bad, 2), np.nan, V), 2, bad)
-    assert np.max(np.abs(out - V)) < 1e-12
-    # every node bad: nothing is known, all zero
-    allbad = np.ones(nn, bool)
-    assert np.all(orc.mesh_fill(xy, el, V, 2, allbad) == 0.0)
-    # a single good corner node and one sweep: only its neighbours get its value
-    one = np.ones(nn, bool)
-    one[0] = False
-    out = orc.mesh_fill(xy, el, V, 2, one, sweeps=1).reshape(-1, 2)
-    nb = {int(v) for e in el if 0 in e for v in e if v >= 0} - {0}
-    for i in range(nn):
-        if i == 0 or i in nb:
-            assert np.allclose(out[i], V.reshape(-1, 2)[0])
-        else:
-            assert np.all(out[i] == 0.0)
 
 
 def test_quadtree_hole_mesh_is_conforming(speckle):
