@@ -393,9 +393,10 @@ class Context:
         return dict(U=U, F=F, U6=U6, F6=F6, iters6=i6, iters2=i2, U0=u0)
 
     def frame_times(self):
-        ms = np.zeros(8)
-        self._ck(self.L.aldic_frame_times(self.h, _p(ms), 8), "aldic_frame_times")
-        keys = ("prep", "search", "icgn6", "post6", "subpb2", "icgn2", "strain_dual", "frame")
+        ms = np.zeros(10)
+        self._ck(self.L.aldic_frame_times(self.h, _p(ms), 10), "aldic_frame_times")
+        keys = ("prep", "search", "icgn6", "post6", "subpb2", "icgn2", "strain_dual", "frame",
+                "search_kernel", "icgn6_kernel")
         return dict(zip(keys, ms.tolist()))
 
     def frame_scalars(self):

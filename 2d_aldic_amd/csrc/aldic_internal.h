@@ -166,7 +166,8 @@ struct FrameState {
     std::vector<double> hcoords;
     std::vector<int> helems;
     std::vector<hipEvent_t> ev;  // stage-boundary events of frame_run
-    float ms[8] = {0};
+    hipEvent_t ev_skern = nullptr, ev_ikern = nullptr;  // search kernel start, 6-DOF kernel end
+    float ms[10] = {0};
 };
 
 }  // namespace aldic
@@ -258,6 +259,7 @@ struct SearchArgs {
     const uint16_t *f16 = nullptr, *g16 = nullptr;
     const double *ms = nullptr;
     DBuf *planes = nullptr;
+    hipEvent_t ev_kernel = nullptr;  // recorded just before the main search kernel (nullable)
 };
 // Runs an exact int8 matrix-core search when raw pixels are given: k_search_i8 for 8-bit
 // images with 2s+3 <= 32 lags per axis and ws + 2s + 3 <= 64, k_search_mt for 16-bit

@@ -267,6 +267,8 @@ void aldic_destroy(aldic_ctx *ctx)
                   &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
                   &F.part, &F.cc, &F.rl, &F.F2b};
     for (DBuf *b : fb) b->release();
+    if (F.ev_skern) (void)hipEventDestroy(F.ev_skern);
+    if (F.ev_ikern) (void)hipEventDestroy(F.ev_ikern);
     for (aldic::GridfitDev *g : {&F.gf, &ctx->gf}) {
         g->w.release();
         g->a9.release();
@@ -975,6 +977,8 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(hipSetDevice(ctx->device));
     FrameState &F = ctx->fr;
     F.o = *o;
+    if (!F.ev_skern) CK(hipEventCreate(&F.ev_skern));
+    if (!F.ev_ikern) CK(hipEventCreate(&F.ev_ikern));
     for (size_t nev = 8 + 5 * (size_t)(o->n_admm + 1); F.ev.size() < nev;) {
         hipEvent_t e;
         CK(hipEventCreate(&e));
@@ -1126,6 +1130,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     sa.qf = nullptr;
     sa.status = reinterpret_cast<int *>(s4 + 4 * (size_t)S);
     sa.U0 = F.U0.as<double>();
+    sa.ev_kernel = F.ev_skern;
     search_raw8(ctx, sa);
     CK(launch_search(sa, st));
     // the search's NaNs (constant templates) go to LocalICGN as they are: their subsets
@@ -1159,6 +1164,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     a.Fcopy = F.F1.as<double>();
     const int ppl = icgn_ppl_for(o.ws);
     CK(launch_icgn6(a, ppl, st));
+    CK(hipEventRecord(F.ev_ikern, st));
     // windows the mask rule enlarged beyond the class (status -2, funICGN.m:56-60) run
     // again one class up, over a list built on the device
     const int ppl_big = icgn_ppl_for((int)std::floor(std::sqrt(2.0) * o.ws + 0.5));
@@ -1346,6 +1352,11 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
         acc[spans[k]] += t;
     }
     for (int k = 0; k < 8; ++k) F.ms[k] = acc[k];
+    // the search kernel alone (after its box-sum planes, if any) and the main 6-DOF kernel
+    // alone (before the re-launch of enlarged windows)
+    F.ms[8] = F.ms[9] = 0.f;
+    if (e_search >= 0) CK(hipEventElapsedTime(&F.ms[8], F.ev_skern, ev[e_search]));
+    if (e_search >= 0) CK(hipEventElapsedTime(&F.ms[9], ev[e_search], F.ev_ikern));
     if (*F.nat_err_h != 0)
         return fail(ctx, ALDIC_ERR_NUMERIC, "frame_run: a natural-neighbour cell could not be formed");
     return ALDIC_OK;
@@ -1686,7 +1697,7 @@ int aldic_frame_get(aldic_ctx *ctx, double *U, double *Fo, double *U6, double *F
 
 int aldic_frame_times(aldic_ctx *ctx, double *ms, int n)
 {
-    if (!ctx || !ms || n < 0 || n > 8) return ALDIC_ERR_ARG;
+    if (!ctx || !ms || n < 0 || n > 10) return ALDIC_ERR_ARG;
     for (int k = 0; k < n; ++k) ms[k] = ctx->fr.ms[k];
     return ALDIC_OK;
 }

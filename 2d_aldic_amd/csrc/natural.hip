@@ -20,12 +20,13 @@
 //   good node (ties to the lowest index); none: NaN.
 //
 // Kernels (one refill = 5 launches, no host wait):
-//   k_nat_rows      per row of equal y: first / last good node, good count
-//   k_nat_hull      one workgroup: good count, strict convex hull of the row extremes
+//   k_nat_rows      one wave per row of equal y: first / last good node, good count
+//   k_nat_hull      one workgroup: good count, the row extremes pruned by their
+//                   eight-direction extreme polygon, strict convex hull of the rest
 //                   (monotone chain), counters cleared
 //   k_nat_classify  per bad node: inside the hull -> inside list, else -> hull list
 //   k_nat_sibson    one thread per inside node, its two polygons in LDS
-//   k_nat_hullfill  one workgroup per hull-list node, scans of all nodes
+//   k_nat_hullfill  one thread per hull-list node, scans over rings of buckets
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -74,63 +75,150 @@ __device__ __forceinline__ double cross3(double2 a, double2 b, double2 c)
     return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
 }
 
+// One wave per row: first and last good node (x ascending) and the good count.
 __global__ __launch_bounds__(256) void k_nat_rows(NatArgs a)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r >= a.R) return;
-    int first = -1, last = -1, cnt = 0;
-    for (int k = a.rstart[r]; k < a.rstart[r + 1]; ++k) {
-        const int i = a.rnode[k];
-        if (a.bad[i]) continue;
-        if (first < 0) first = i;
-        last = i;
-        ++cnt;
+    const int k0 = a.rstart[r], k1 = a.rstart[r + 1];
+    int first = 0x7fffffff, last = -1, cnt = 0;
+    for (int k = k0 + lane; k < k1; k += 64)
+        if (!a.bad[a.rnode[k]]) {
+            first = min(first, k);
+            last = max(last, k);
+            ++cnt;
+        }
+    for (int m = 32; m >= 1; m >>= 1) {
+        first = min(first, __shfl_xor(first, m, kWave));
+        last = max(last, __shfl_xor(last, m, kWave));
+        cnt += __shfl_xor(cnt, m, kWave);
     }
-    a.rows[3 * r] = first;
-    a.rows[3 * r + 1] = last;
-    a.rows[3 * r + 2] = cnt;
+    if (lane == 0) {
+        a.rows[3 * r] = cnt ? a.rnode[first] : -1;
+        a.rows[3 * r + 1] = cnt ? a.rnode[last] : -1;
+        a.rows[3 * r + 2] = cnt;
+    }
 }
 
-// Strict hull (collinear points dropped) of the row extremes, which are sorted by (y, x):
-// the monotone chain in the axis-swapped frame (cross products change sign), reversed to
-// counter-clockwise and rotated to start at the lowest-x, lowest-y vertex.
+// Strict hull (collinear points dropped) of the good nodes, from the row extremes (a
+// node between the first and last good node of its row is not a strict vertex).  The
+// extremes in the eight directions of 45-degree steps form a convex polygon of good
+// nodes; candidates inside it or on its boundary are not strict vertices either, so only
+// its vertices and the candidates strictly outside it enter the monotone chain, which
+// runs in the axis-swapped frame (the candidates are sorted by (y, x); cross products
+// change sign), is reversed to counter-clockwise and rotated to start at the lowest-x,
+// lowest-y vertex.
 __global__ __launch_bounds__(256) void k_nat_hull(NatArgs a)
 {
     __shared__ int red[4];
+    __shared__ double sv[8][4];
+    __shared__ int si[8][4];
+    __shared__ int oct[9], noct, cvx;
+    __shared__ int pre[257];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     int c = 0;
-    for (int r = threadIdx.x; r < a.R; r += blockDim.x) c += a.rows[3 * r + 2];
-    c = (int)wave_sum((double)c);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const int ng = red[0] + red[1] + red[2] + red[3];
-    int *P = a.chain;            // candidates
-    int np = 0;
-    for (int r = 0; r < a.R; ++r) {
+    for (int r = tid; r < a.R; r += blockDim.x) c += a.rows[3 * r + 2];
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, kWave);
+    if (lane == 0) red[w] = c;
+    const int nc = 2 * a.R;  // candidate slot t: row t / 2, first (even) or last (odd)
+    auto cand = [&](int t) -> int {
+        const int r = t >> 1;
         const int f = a.rows[3 * r], l = a.rows[3 * r + 1];
-        if (f < 0) continue;
-        P[np++] = f;
-        if (l != f) P[np++] = l;
+        return (t & 1) ? (l != f ? l : -1) : f;
+    };
+    // extremes in the directions (1,0) (1,1) (0,1) (-1,1) (-1,0) (-1,-1) (0,-1) (1,-1)
+    const double dx[8] = {1, 1, 0, -1, -1, -1, 0, 1}, dy[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+    double bv[8];
+    int bi[8];
+    for (int k = 0; k < 8; ++k) { bv[k] = -INFINITY; bi[k] = 0x7fffffff; }
+    for (int t = tid; t < nc; t += blockDim.x) {
+        const int p = cand(t);
+        if (p < 0) continue;
+        const double2 P = a.xy[p];
+        for (int k = 0; k < 8; ++k) {
+            const double v = dx[k] * P.x + dy[k] * P.y;
+            if (v > bv[k] || (v == bv[k] && p < bi[k])) { bv[k] = v; bi[k] = p; }
+        }
     }
-    int *H = a.chain + np;       // chain stack (at most 2 np + 1 entries)
+    for (int k = 0; k < 8; ++k) {
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double v2 = __shfl_xor(bv[k], m, kWave);
+            const int i2 = __shfl_xor(bi[k], m, kWave);
+            if (v2 > bv[k] || (v2 == bv[k] && i2 < bi[k])) { bv[k] = v2; bi[k] = i2; }
+        }
+        if (lane == 0) { sv[k][w] = bv[k]; si[k][w] = bi[k]; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int n = 0;
+        for (int k = 0; k < 8; ++k) {
+            double v = sv[k][0];
+            int i = si[k][0];
+            for (int q = 1; q < 4; ++q)
+                if (sv[k][q] > v || (sv[k][q] == v && si[k][q] < i)) { v = sv[k][q]; i = si[k][q]; }
+            if (i == 0x7fffffff) continue;
+            if (n == 0 || oct[n - 1] != i) oct[n++] = i;
+        }
+        while (n > 1 && oct[n - 1] == oct[0]) --n;
+        int convex = n >= 3;
+        for (int e = 0; e < n && convex; ++e)
+            if (!(cross3(a.xy[oct[e]], a.xy[oct[(e + 1) % n]], a.xy[oct[(e + 2) % n]]) > 0.0))
+                convex = 0;
+        noct = n;
+        cvx = convex;
+    }
+    __syncthreads();
+    // survivors: the octagon's vertices and candidates strictly outside it (all, when it
+    // is not a proper convex polygon); counted per thread over a contiguous chunk
+    const int per = (nc + 255) / 256, t0 = tid * per, t1 = min(nc, t0 + per);
+    auto survives = [&](int p) -> bool {
+        if (!cvx) return true;
+        for (int e = 0; e < noct; ++e)
+            if (oct[e] == p) return true;
+        const double2 P = a.xy[p];
+        for (int e = 0; e < noct; ++e)
+            if (cross3(a.xy[oct[e]], a.xy[oct[e + 1 < noct ? e + 1 : 0]], P) < 0.0) return true;
+        return false;
+    };
+    int mine = 0;
+    for (int t = t0; t < t1; ++t) {
+        const int p = cand(t);
+        if (p >= 0 && survives(p)) ++mine;
+    }
+    pre[tid + 1] = mine;
+    __syncthreads();
+    if (tid == 0) {
+        pre[0] = 0;
+        for (int k = 1; k <= 256; ++k) pre[k] += pre[k - 1];
+    }
+    __syncthreads();
+    int *P = a.chain;
+    for (int t = t0, o = pre[tid]; t < t1; ++t) {
+        const int p = cand(t);
+        if (p >= 0 && survives(p)) P[o++] = p;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    const int ng = red[0] + red[1] + red[2] + red[3];
+    const int np = pre[256];
+    int *H = a.chain + np;       // chain stack (at most 2 np entries)
     int k = 0;
-    for (int s = 0; s < np; ++s) {
-        while (k >= 2 && cross3(a.xy[H[k - 2]], a.xy[H[k - 1]], a.xy[P[s]]) >= 0.0) --k;
-        H[k++] = P[s];
+    for (int t = 0; t < np; ++t) {
+        while (k >= 2 && cross3(a.xy[H[k - 2]], a.xy[H[k - 1]], a.xy[P[t]]) >= 0.0) --k;
+        H[k++] = P[t];
     }
-    for (int s = np - 2, t = k + 1; s >= 0; --s) {
-        while (k >= t && cross3(a.xy[H[k - 2]], a.xy[H[k - 1]], a.xy[P[s]]) >= 0.0) --k;
-        H[k++] = P[s];
+    for (int t = np - 2, lo = k + 1; t >= 0; --t) {
+        while (k >= lo && cross3(a.xy[H[k - 2]], a.xy[H[k - 1]], a.xy[P[t]]) >= 0.0) --k;
+        H[k++] = P[t];
     }
     int h = np >= 3 ? k - 1 : 0;
     if (h < 3) h = 0;
-    // reverse (counter-clockwise) and start at the lowest (x, y)
     int s0 = 0;
-    for (int s = 1; s < h; ++s) {
-        const double2 p = a.xy[H[s]], b = a.xy[H[s0]];
-        if (p.x < b.x || (p.x == b.x && p.y < b.y)) s0 = s;
+    for (int t = 1; t < h; ++t) {
+        const double2 p = a.xy[H[t]], b = a.xy[H[s0]];
+        if (p.x < b.x || (p.x == b.x && p.y < b.y)) s0 = t;
     }
-    for (int s = 0; s < h; ++s) a.hull[s] = H[((s0 - s) % h + h) % h];
+    for (int t = 0; t < h; ++t) a.hull[t] = H[((s0 - t) % h + h) % h];
     a.ctl[0] = h;
     a.ctl[1] = ng;
     a.ctl[2] = 0;
@@ -389,29 +477,7 @@ __global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
 #undef PX
 #undef PY
 
-// ---- hull-list nodes: one workgroup each --------------------------------------------
-struct KeyI {
-    double k;
-    int i;
-};
-
-// Block minimum of (k, i) lexicographically (every thread gets the result).
-__device__ KeyI block_min(KeyI v, KeyI *sh)
-{
-    for (int o = 32; o >= 1; o >>= 1) {
-        const double k2 = __shfl_xor(v.k, o, kWave);
-        const int i2 = __shfl_xor(v.i, o, kWave);
-        if (k2 < v.k || (k2 == v.k && i2 < v.i)) { v.k = k2; v.i = i2; }
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    KeyI r = sh[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-        if (sh[w].k < r.k || (sh[w].k == r.k && sh[w].i < r.i)) r = sh[w];
-    return r;
-}
-
+// ---- hull-list nodes: one thread each, scans over rings of buckets -----------------
 __device__ __forceinline__ bool on_seg(double2 A, double2 C, double2 P, double &s)
 {
     const double ex = C.x - A.x, ey = C.y - A.y, px = P.x - A.x, py = P.y - A.y;
@@ -420,6 +486,35 @@ __device__ __forceinline__ bool on_seg(double2 A, double2 C, double2 P, double &
     if (fabs(cr) > 1e-12 * sqrt(L2) * sqrt(px * px + py * py)) return false;
     s = (px * ex + py * ey) / L2;
     return s >= 0.0 && s <= 1.0;
+}
+
+// Visits the good nodes of the buckets in rings around point X, ring by ring; before ring
+// r (r >= 1) stop(D) is asked with D = (r - 1) h: every node not yet visited is more than
+// D from X.
+template <class Visit, class Stop>
+__device__ void ring_scan(const NatArgs &a, double2 X, Visit visit, Stop stop)
+{
+    int bx = (int)floor((X.x - a.ox) / a.h), by = (int)floor((X.y - a.oy) / a.h);
+    bx = bx < 0 ? 0 : (bx >= a.BX ? a.BX - 1 : bx);
+    by = by < 0 ? 0 : (by >= a.BY ? a.BY - 1 : by);
+    const int rmax = max(max(bx, a.BX - 1 - bx), max(by, a.BY - 1 - by));
+    auto bucket = [&](int i, int j) {
+        if (i < 0 || i >= a.BX || j < 0 || j >= a.BY) return;
+        const int b = j * a.BX + i;
+        for (int t = a.bstart[b]; t < a.bstart[b + 1]; ++t) {
+            const int p = a.bnode[t];
+            if (!a.bad[p]) visit(p);
+        }
+    };
+    for (int r = 0; r <= rmax; ++r) {
+        if (r > 0 && stop((double)(r - 1) * a.h)) break;
+        if (r == 0) {
+            bucket(bx, by);
+            continue;
+        }
+        for (int i = bx - r; i <= bx + r; ++i) { bucket(i, by - r); bucket(i, by + r); }
+        for (int j = by - r + 1; j <= by + r - 1; ++j) { bucket(bx - r, j); bucket(bx + r, j); }
+    }
 }
 
 __device__ void plane_at(const NatArgs &a, int ia, int ib, int ic, double2 Q, double2 &u,
@@ -442,124 +537,131 @@ __device__ void plane_at(const NatArgs &a, int ia, int ib, int ic, double2 Q, do
     }
 }
 
-__global__ __launch_bounds__(256) void k_nat_hullfill(NatArgs a)
+// Circle through a and c (edge vector e, length L, midpoint m, inner unit normal n) and
+// node p strictly on the inner side: its centre is m + s n.
+__device__ __forceinline__ bool bulge(double2 Pa, double ex, double ey, double L, double2 M,
+                                      double nxv, double nyv, double2 P, double &s)
 {
-    __shared__ KeyI sh[4];
-    __shared__ int s_case, s_e, s_v;
-    __shared__ double s_t;
+    const double px = P.x - Pa.x, py = P.y - Pa.y;
+    const double cr = ex * py - ey * px;
+    if (!(cr > 1e-12 * L * sqrt(px * px + py * py))) return false;
+    const double dx = P.x - M.x, dy = P.y - M.y;
+    s = (dx * dx + dy * dy - 0.25 * L * L) / (2.0 * (dx * nxv + dy * nyv));
+    return true;
+}
+
+__global__ __launch_bounds__(64) void k_nat_hullfill(NatArgs a)
+{
     const int nh = a.ctl[3], h = a.ctl[0], ng = a.ctl[1];
-    for (int li = blockIdx.x; li < nh; li += gridDim.x) {
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nh; li += gridDim.x * blockDim.x) {
         const int q = a.lhull[li];
         const double2 Q = a.xy[q];
         double2 u = make_double2(NAN, NAN);
         double4 f = make_double4(NAN, NAN, NAN, NAN);
         if (ng == 0) {
-            // no good node
+            // no good node: NaN
         } else if (h < 3 || ng < 3) {
-            KeyI best{INFINITY, 0x7fffffff};
-            for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
-                if (a.bad[p]) continue;
-                const double2 P = a.xy[p];
-                const double d = (P.x - Q.x) * (P.x - Q.x) + (P.y - Q.y) * (P.y - Q.y);
-                if (d < best.k || (d == best.k && p < best.i)) { best.k = d; best.i = p; }
-            }
-            best = block_min(best, sh);
-            u = a.U[best.i];
-            if (a.F) f = a.F[best.i];
+            // nearest good node, ties to the lowest index
+            double bd = INFINITY;
+            int bi = 0x7fffffff;
+            ring_scan(a, Q,
+                      [&](int p) {
+                          const double2 P = a.xy[p];
+                          const double d = (P.x - Q.x) * (P.x - Q.x) + (P.y - Q.y) * (P.y - Q.y);
+                          if (d < bd || (d == bd && p < bi)) { bd = d; bi = p; }
+                      },
+                      [&](double D) { return bi != 0x7fffffff && D * D >= bd; });
+            u = a.U[bi];
+            if (a.F) f = a.F[bi];
         } else {
-            if (threadIdx.x == 0) {
-                int be = 0;
-                double bt = 0.0, bd = INFINITY;
-                for (int e = 0; e < h; ++e) {
-                    const double2 A = a.xy[a.hull[e]], C = a.xy[a.hull[e + 1 < h ? e + 1 : 0]];
-                    const double ex = C.x - A.x, ey = C.y - A.y;
-                    double tt = ((Q.x - A.x) * ex + (Q.y - A.y) * ey) / (ex * ex + ey * ey);
-                    tt = tt < 0.0 ? 0.0 : (tt > 1.0 ? 1.0 : tt);
-                    const double dx = A.x + tt * ex - Q.x, dy = A.y + tt * ey - Q.y;
-                    const double d = dx * dx + dy * dy;
-                    if (d < bd) { bd = d; be = e; bt = tt; }
-                }
-                s_e = be;
-                s_t = bt;
-                s_case = (bt > 0.0 && bt < 1.0) ? 0 : 1;
-                s_v = bt <= 0.0 ? be : (be + 1 < h ? be + 1 : 0);
+            // the hull point nearest q
+            int be = 0;
+            double bt = 0.0, bd = INFINITY;
+            for (int e = 0; e < h; ++e) {
+                const double2 A = a.xy[a.hull[e]], C = a.xy[a.hull[e + 1 < h ? e + 1 : 0]];
+                const double ex = C.x - A.x, ey = C.y - A.y;
+                double tt = ((Q.x - A.x) * ex + (Q.y - A.y) * ey) / (ex * ex + ey * ey);
+                tt = tt < 0.0 ? 0.0 : (tt > 1.0 ? 1.0 : tt);
+                const double dx = A.x + tt * ex - Q.x, dy = A.y + tt * ey - Q.y;
+                const double d = dx * dx + dy * dy;
+                if (d < bd) { bd = d; be = e; bt = tt; }
             }
-            __syncthreads();
-            const int cs = s_case;
-            if (cs == 0) {
-                const int A = a.hull[s_e], C = a.hull[s_e + 1 < h ? s_e + 1 : 0];
-                const double bt = s_t;
+            if (bt > 0.0 && bt < 1.0) {
+                // inside the hull edge A-C: good nodes a (largest s <= bt) and c (smallest
+                // s > bt) on it next to the nearest point
+                const int A = a.hull[be], C = a.hull[be + 1 < h ? be + 1 : 0];
                 const double2 XA = a.xy[A], XC = a.xy[C];
-                // a: largest s <= bt; c: smallest s > bt (ties: lowest index)
-                KeyI ma{0.0, A}, mc{1.0, C};
-                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
-                    double s;
-                    if (a.bad[p] || !on_seg(XA, XC, a.xy[p], s)) continue;
-                    if (s <= bt && (-s < -ma.k || (s == ma.k && p < ma.i))) { ma.k = s; ma.i = p; }
-                    if (s > bt && (s < mc.k || (s == mc.k && p < mc.i))) { mc.k = s; mc.i = p; }
-                }
-                KeyI na{-ma.k, ma.i};
-                na = block_min(na, sh);
-                mc = block_min(mc, sh);
-                const int ia = na.i, ic = mc.i;
+                const double LAC = sqrt((XC.x - XA.x) * (XC.x - XA.x) + (XC.y - XA.y) * (XC.y - XA.y));
+                const double2 Bp = make_double2(XA.x + bt * (XC.x - XA.x), XA.y + bt * (XC.y - XA.y));
+                double sa = 0.0, sc = 1.0;
+                int ia = A, ic = C;
+                ring_scan(a, Bp,
+                          [&](int p) {
+                              double sp;
+                              if (!on_seg(XA, XC, a.xy[p], sp)) return;
+                              if (sp <= bt && (sp > sa || (sp == sa && p < ia))) { sa = sp; ia = p; }
+                              if (sp > bt && (sp < sc || (sp == sc && p < ic))) { sc = sp; ic = p; }
+                          },
+                          [&](double D) { return D >= (bt - sa) * LAC && D >= (sc - bt) * LAC; });
+                // Delaunay third vertex of a-c: least bulge s over the inner side, then the
+                // lowest index within the tolerance of it
                 const double2 Pa = a.xy[ia], Pc = a.xy[ic];
                 const double ex = Pc.x - Pa.x, ey = Pc.y - Pa.y;
                 const double L = sqrt(ex * ex + ey * ey);
-                const double mx = Pa.x + 0.5 * ex, my = Pa.y + 0.5 * ey, nxv = -ey / L, nyv = ex / L;
-                KeyI bs{INFINITY, 0x7fffffff};
-                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
-                    if (a.bad[p]) continue;
-                    const double2 P = a.xy[p];
-                    const double px = P.x - Pa.x, py = P.y - Pa.y;
-                    const double cr = ex * py - ey * px;
-                    if (!(cr > 1e-12 * L * sqrt(px * px + py * py))) continue;
-                    const double dx = P.x - mx, dy = P.y - my;
-                    const double s = (dx * dx + dy * dy - 0.25 * L * L) / (2.0 * (dx * nxv + dy * nyv));
-                    if (s < bs.k) { bs.k = s; bs.i = p; }
-                }
-                bs = block_min(bs, sh);
-                const double lim = bs.k + 1e-12 * (fabs(bs.k) + L);
-                KeyI br{0.0, 0x7fffffff};
-                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
-                    if (a.bad[p]) continue;
-                    const double2 P = a.xy[p];
-                    const double px = P.x - Pa.x, py = P.y - Pa.y;
-                    const double cr = ex * py - ey * px;
-                    if (!(cr > 1e-12 * L * sqrt(px * px + py * py))) continue;
-                    const double dx = P.x - mx, dy = P.y - my;
-                    const double s = (dx * dx + dy * dy - 0.25 * L * L) / (2.0 * (dx * nxv + dy * nyv));
-                    if (s <= lim && p < br.i) br.i = p;
-                }
-                br = block_min(br, sh);
-                if (br.i < a.nn) plane_at(a, ia, ic, br.i, Q, u, f);
-                else if (threadIdx.x == 0) nat_error(a);
+                const double2 M = make_double2(Pa.x + 0.5 * ex, Pa.y + 0.5 * ey);
+                const double nxv = -ey / L, nyv = ex / L;
+                double smin = INFINITY;
+                // nodes farther than D from m bulge more than (D^2 - L^2/4) / (2 D)
+                auto beyond = [&](double D, double lim) {
+                    return D > 0.5 * L && (D * D - 0.25 * L * L) / (2.0 * D) > lim;
+                };
+                ring_scan(a, M,
+                          [&](int p) {
+                              double sp;
+                              if (bulge(Pa, ex, ey, L, M, nxv, nyv, a.xy[p], sp) && sp < smin) smin = sp;
+                          },
+                          [&](double D) {
+                              return smin < INFINITY && beyond(D, smin + 1e-12 * (fabs(smin) + L));
+                          });
+                const double lim = smin + 1e-12 * (fabs(smin) + L);
+                int ir = 0x7fffffff;
+                ring_scan(a, M,
+                          [&](int p) {
+                              double sp;
+                              if (p < ir && bulge(Pa, ex, ey, L, M, nxv, nyv, a.xy[p], sp) && sp <= lim)
+                                  ir = p;
+                          },
+                          [&](double D) { return beyond(D, lim); });
+                if (ir != 0x7fffffff) plane_at(a, ia, ic, ir, Q, u, f);
+                else nat_error(a);
             } else {
-                const int vi = s_v;
+                // at the hull vertex v: the good nodes next to v along its two hull edges
+                const int vi = bt <= 0.0 ? be : (be + 1 < h ? be + 1 : 0);
                 const int v = a.hull[vi], Pv = a.hull[vi > 0 ? vi - 1 : h - 1],
                           Nv = a.hull[vi + 1 < h ? vi + 1 : 0];
                 const double2 XP = a.xy[Pv], XV = a.xy[v], XN = a.xy[Nv];
-                KeyI k1{-0.0, Pv}, k2{1.0, Nv};  // u1: largest s < 1 on P->v; u2: smallest s > 0 on v->N
-                for (int p = threadIdx.x; p < a.nn; p += blockDim.x) {
-                    double s;
-                    if (a.bad[p] || p == v) continue;
-                    const double2 X = a.xy[p];
-                    if (on_seg(XP, XV, X, s) && s < 1.0 && (-s < k1.k || (-s == k1.k && p < k1.i))) {
-                        k1.k = -s; k1.i = p;
-                    }
-                    if (on_seg(XV, XN, X, s) && s > 0.0 && (s < k2.k || (s == k2.k && p < k2.i))) {
-                        k2.k = s; k2.i = p;
-                    }
-                }
-                k1 = block_min(k1, sh);
-                k2 = block_min(k2, sh);
-                plane_at(a, v, k2.i, k1.i, Q, u, f);
+                const double L1 = sqrt((XV.x - XP.x) * (XV.x - XP.x) + (XV.y - XP.y) * (XV.y - XP.y));
+                const double L2 = sqrt((XN.x - XV.x) * (XN.x - XV.x) + (XN.y - XV.y) * (XN.y - XV.y));
+                double s1 = 0.0, s2 = 1.0;
+                int i1 = Pv, i2 = Nv;
+                ring_scan(a, XV,
+                          [&](int p) {
+                              if (p == v) return;
+                              double sp;
+                              const double2 X = a.xy[p];
+                              if (on_seg(XP, XV, X, sp) && sp < 1.0 && (sp > s1 || (sp == s1 && p < i1))) {
+                                  s1 = sp; i1 = p;
+                              }
+                              if (on_seg(XV, XN, X, sp) && sp > 0.0 && (sp < s2 || (sp == s2 && p < i2))) {
+                                  s2 = sp; i2 = p;
+                              }
+                          },
+                          [&](double D) { return D >= (1.0 - s1) * L1 && D >= s2 * L2; });
+                plane_at(a, v, i2, i1, Q, u, f);
             }
         }
-        if (threadIdx.x == 0) {
-            a.U[q] = u;
-            if (a.F) a.F[q] = f;
-        }
-        __syncthreads();
+        a.U[q] = u;
+        if (a.F) a.F[q] = f;
     }
 }
 
@@ -679,13 +781,13 @@ hipError_t natural_fill(const NatGrid &g, const double *coords, const int *bad, 
     a.lin = a.ctl + 8;
     a.lhull = a.lin + g.nn;
     a.err = err;
-    hipLaunchKernelGGL(k_nat_rows, dim3((g.R + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_nat_rows, dim3((g.R + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nat_hull, dim3(1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nat_classify, dim3((g.nn + 255) / 256), dim3(256), 0, st, a);
     const int nbs = std::min((g.nn + kNatThreads - 1) / kNatThreads, 4096);
     hipLaunchKernelGGL(k_nat_sibson, dim3(nbs), dim3(kNatThreads), 0, st, a);
-    const int nbh = std::min(g.nn, 1024);
-    hipLaunchKernelGGL(k_nat_hullfill, dim3(nbh), dim3(256), 0, st, a);
+    const int nbh = std::min((g.nn + 63) / 64, 1024);
+    hipLaunchKernelGGL(k_nat_hullfill, dim3(nbh), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

@@ -368,11 +368,16 @@ hipError_t launch_search(const SearchArgs &a, hipStream_t st)
 {
     const int S = a.Nx * a.Ny;
     if (S <= 0) return hipSuccess;
+    if (a.ms && a.planes && !(a.f8 && a.g8 && search_i8_supported(a.ws, a.sx, a.sy)) &&
+        ((a.f8 && a.g8 && search_mt_supported(a.ws, a.sx, a.sy, 8)) ||
+         (a.f16 && a.g16 && search_mt_supported(a.ws, a.sx, a.sy, 16))))
+        return launch_search_mt(a, st);  // records ev_kernel after its box-sum planes
+    if (a.ev_kernel) {
+        const hipError_t e = hipEventRecord(a.ev_kernel, st);
+        if (e != hipSuccess) return e;
+    }
     if (a.f8 && a.g8 && a.ms && search_i8_supported(a.ws, a.sx, a.sy))
         return launch_search_i8(a, st);
-    if (a.ms && a.planes && ((a.f8 && a.g8 && search_mt_supported(a.ws, a.sx, a.sy, 8)) ||
-                             (a.f16 && a.g16 && search_mt_supported(a.ws, a.sx, a.sy, 16))))
-        return launch_search_mt(a, st);
     bool ok;
     const SearchGeom G = search_geom(a.ws, a.sx, a.sy, ok);
     if (!ok) return hipErrorInvalidValue;

@@ -617,10 +617,12 @@ hipError_t launch_search_mt(const SearchArgs &a, hipStream_t st)
     if (dig == 2) {
         if ((e = box_planes(a.g16, a.nx, a.ny, G.tw, 32896, *a.planes, bp, st)) != hipSuccess)
             return e;
+        if (a.ev_kernel && (e = hipEventRecord(a.ev_kernel, st)) != hipSuccess) return e;
         return launch_dig<2>(a, G, bp, a.f16, a.g16, S, st);
     }
     if (!a.f8 || !a.g8) return hipErrorInvalidValue;
     if ((e = box_planes(a.g8, a.nx, a.ny, G.tw, 128, *a.planes, bp, st)) != hipSuccess) return e;
+    if (a.ev_kernel && (e = hipEventRecord(a.ev_kernel, st)) != hipSuccess) return e;
     return launch_dig<1>(a, G, bp, a.f8, a.g8, S, st);
 }
 

@@ -85,6 +85,40 @@ def _render(dots: DotField, nx, ny, Xmap=None, Ymap=None, fwd=None, chunk=8192):
     return img.reshape(ny, nx)
 
 
+_PAR = {}
+
+
+def _render_group(k):
+    a = _PAR
+    lo, hi = a["bounds"][k], a["bounds"][k + 1]
+    d = a["dots"]
+    sub = DotField(d.cx[lo:hi], d.cy[lo:hi], d.amp[lo:hi], d.sigma[lo:hi])
+    return _render(sub, a["nx"], a["ny"], a["Xmap"], a["Ymap"], a["fwd"])
+
+
+def render_parallel(dots: DotField, nx, ny, Xmap=None, Ymap=None, fwd=None, groups=16,
+                    workers=None):
+    """_render over `groups` fixed, contiguous groups of dots in forked worker processes;
+    the partial images are summed in group order, so the result does not depend on the
+    number of workers."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import os
+    n = dots.cx.size
+    bounds = [n * k // groups for k in range(groups + 1)]
+    _PAR.update(dots=dots, nx=nx, ny=ny, Xmap=Xmap, Ymap=Ymap, fwd=fwd, bounds=bounds)
+    workers = workers or max(1, min(groups, len(os.sched_getaffinity(0)), 16))
+    try:
+        with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
+            parts = list(ex.map(_render_group, range(groups)))
+    finally:
+        _PAR.clear()
+    img = np.zeros((ny, nx))
+    for part in parts:
+        img += part
+    return img
+
+
 def inverse_map(ufun, nx, ny, iters=60, tol=1e-12):
     """Solve X + u(X) = x for every pixel x by fixed-point iteration (u Lipschitz < 1)."""
     y, x = np.mgrid[1 : ny + 1, 1 : nx + 1].astype(np.float64)
@@ -113,7 +147,10 @@ def make_pair(nx, ny, n_dots, sigma, ufun, noise, bits=8, amp=(0.4, 1.0), seed=0
     """hole = (cx, cy, a): a black disc of radius a in the reference, carried into the
     deformed image through the inverse map."""
     dots = draw_dots(n_dots, nx, ny, sigma, amp, seed)
-    fr = _render(dots, nx, ny)
+    # large workloads render in fixed dot groups on several processes (same sum order
+    # whatever the worker count); small ones in one pass
+    render = render_parallel if n_dots >= 300_000 else _render
+    fr = render(dots, nx, ny)
     full = float((1 << bits) - 1)
     scale = full / fr.max()
     Xm, Ym = inverse_map(ufun, nx, ny)
@@ -122,7 +159,7 @@ def make_pair(nx, ny, n_dots, sigma, ufun, noise, bits=8, amp=(0.4, 1.0), seed=0
         u, v = ufun(cx, cy)
         return cx + u, cy + v
 
-    gd = _render(dots, nx, ny, Xm, Ym, fwd)
+    gd = render(dots, nx, ny, Xm, Ym, fwd)
     rng = np.random.default_rng(seed + 1)
     dt = np.uint8 if bits <= 8 else np.uint16
     f = np.clip(np.rint(fr * scale), 0, full).astype(dt)

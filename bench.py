@@ -1,16 +1,21 @@
 """Benchmark: one AL-DIC frame on one MI355X.
 
-Workload (one "step" = one frame): 2048x2048 uint8 Gaussian-dot speckle (150k dots,
-sigma 2.5 px), sinusoidal field u = 2 sin(2 pi X/256), v = 0.5 cos(2 pi Y/512), noise
-sigma 2, subsets 32 px on a 16 px grid (14,884 subsets), integer search radius 8:
-normalisation + gradients, integer search, 6-DOF IC-GN with the bad-point rule,
-step-1 global step, and 3 ADMM iterations (2-DOF IC-GN, Q4 FEM global step, nodal
-strain, dual update).  The raw pair is resident in HBM before the timed region.
+Default workload (one "step" = one frame): 4096x4096 12-bit speckle stored as uint16
+(600k Gaussian dots, sigma U[1.5, 3] px), 0.5 % tension plus a Gaussian shear band (width
+200 px, peak shear 3 %), noise sigma 2; subsets 32 px on an 8 px grid (247,009 subsets),
+integer search radius 24: normalisation + gradients, integer search, 6-DOF IC-GN with the
+bad-point rule and natural-neighbour refill, step-1 global step, and 3 ADMM iterations
+(2-DOF IC-GN + gridfit refit, Q4 FEM global step, nodal strain, dual update).  The raw
+pair is resident in HBM before the timed region.  --workload 2 runs the 2048^2 uint8
+ws32/step16 frame (14,884 subsets), --workload 4 the quadtree plate with a hole.
 
 value = converged subsets (6-DOF IC-GN status 1..100) per second of whole frame.
-The CPU baseline is the FP64 restatement of the same frame on the host cores.
+accuracy_vs_fp64 and cpu_baseline come from the FP64 oracle on a seeded 2 % sample of
+the subsets (search + 6-DOF IC-GN + three 2-DOF IC-GN passes per subset, OpenMP; the
+full-frame local-step time is that sample scaled by S / n_sample; the global steps are
+not in the CPU figure); workload 2 runs the oracle's whole frame instead.
 
-  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-stages]
+  python bench.py [--steps K] [--warmup W] [--workload 2|3|4] [--no-cpu] [--no-stages]
 """
 from __future__ import annotations
 
@@ -95,16 +100,20 @@ def cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the untimed per-stage breakdown frames")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", type=int, default=2, choices=(2, 3, 4),
-                    help="2: 2048^2 uint8 ws32/step16 s8 (default); 3: 4096^2 12-bit "
-                         "ws32/step8 s24, ~247k subsets (generation takes ~1.5 min); 4: "
-                         "2048^2 plate with a hole on a quadtree mesh, windows 16-64 px")
+    ap.add_argument("--workload", type=int, default=3, choices=(2, 3, 4),
+                    help="3: 4096^2 12-bit ws32/step8 s24, 247,009 subsets (default); 2: "
+                         "2048^2 uint8 ws32/step16 s8; 4: 2048^2 plate with a hole on a "
+                         "quadtree mesh, windows 16-64 px")
+    ap.add_argument("--sample", type=float, default=0.02,
+                    help="fraction of the subsets the FP64 oracle checks and times (workload 3)")
+    ap.add_argument("--no-smooth", action="store_true",
+                    help="skip the smoothing-on frames (funSmoothDisp / funSmoothStrain)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -119,9 +128,6 @@ def main():
         return run_quadtree(args, pkg)
     t = time.perf_counter()
     if args.workload == 3:
-        if not args.no_cpu:
-            log("[bench] workload 3: the FP64 CPU frame would take hours; CPU baseline skipped")
-            args.no_cpu = True
         pair = sp.config3()
         ws, step, srad, side = 32, 8, 24, 4096
         wname = "config3: 4096x4096 12-bit speckle in uint16, 0.5% tension + shear band, ws32/step8"
@@ -139,133 +145,122 @@ def main():
     if args.copy_calib:
         copy_calibration(256 << 20)
     ctx.frame_stage(pair.f, pair.g)
-    # timed frames carry events only around the search and the whole frame; the
-    # per-stage breakdown comes from a second, untimed set of frames with an event at
-    # every stage boundary
+    # timed frames carry events only around the search, the 6-DOF IC-GN, their two main
+    # kernels and the whole frame; the per-stage breakdown comes from a second, untimed
+    # set of frames with an event at every stage boundary
     S = ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
                         cg_maxit=2000, cg_rtol=1e-10, stage_times=False)
     log(f"[bench] frame set up: {S} subsets")
     for _ in range(args.warmup):
         ctx.frame_run()
     log("[bench] warmup done")
-    search_ms = icgn6_ms = 0.0
+    acc_t = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.frame_run()
-        t = ctx.frame_times()
-        search_ms += t["search"]
-        icgn6_ms += t["icgn6"]
+        for k, v in ctx.frame_times().items():
+            acc_t[k] = acc_t.get(k, 0.0) + v
     wall = time.perf_counter() - t0
-    search_ms /= args.steps
-    icgn6_ms /= args.steps
+    kt = {k: v / args.steps for k, v in acc_t.items()}
     ms_per_step = wall * 1e3 / args.steps
     out = ctx.frame_get()
     counters = ctx.frame_counters()
+    log(f"[bench] {ms_per_step:.3f} ms/frame (search kernel {kt['search_kernel']:.3f} ms, "
+        f"6-DOF kernel {kt['icgn6_kernel']:.3f} ms)")
     stage = {}
     if not args.no_stages:
         ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
                         cg_maxit=2000, cg_rtol=1e-10, stage_times=True)
-        for _ in range(args.warmup + args.steps):
+        nst = max(3, min(args.steps, 10))
+        for _ in range(nst):
             ctx.frame_run()
             for k, v in ctx.frame_times().items():
                 stage[k] = stage.get(k, 0.0) + v
-        stage = {k: v / (args.warmup + args.steps) for k, v in stage.items()}
+        stage = {k: v / nst for k, v in stage.items()}
+        log(f"[bench] stages {stage}")
+    smooth_ms = None
+    if not args.no_smooth:
+        # main_ALDIC.m's smoothing passes on (funSmoothStrain of F1 and every F2,
+        # funSmoothDisp of U1 and U2 in the ADMM steps: main_ALDIC.m:233,320,381,400,407)
+        ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
+                        cg_maxit=2000, cg_rtol=1e-10, stage_times=False, smooth=True)
+        ctx.frame_run()
+        nsm = max(3, min(args.steps, 10))
+        t0 = time.perf_counter()
+        for _ in range(nsm):
+            ctx.frame_run()
+        smooth_ms = (time.perf_counter() - t0) * 1e3 / nsm
+        log(f"[bench] smoothing on: {smooth_ms:.3f} ms/frame")
     it6 = out["iters6"]
     conv = int(np.sum((it6 >= 1) & (it6 <= 100)))
     value = conv / (ms_per_step / 1e3)
-    log(f"[bench] {ms_per_step:.3f} ms/frame, stages {stage}")
 
-    # Roofline of the dominant kernel, the 6-DOF IC-GN (one k_icgn launch per frame plus
-    # the device-listed relaunch of enlarged windows, event-timed on the library's
-    # stream over the timed frames).  Bandwidth class: achieved = compulsory HBM bytes
-    # per launch / time, with 16 N_px (f, fx, fy, g as FP32) + 124 S (2 words in; 6 + 21
-    # + 2 words out, 4 B each) per launch.  "traffic" is the measured HBM bytes per
-    # launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a 256 MiB
-    # device copy reads on the same counters), kept in profiles/hbm_traffic.json.
     npx = side * side
     traffic = {}
     try:
         with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh)
+            traffic = json.load(fh).get(f"workload{args.workload}", {})
     except (OSError, ValueError):
         pass
 
-    def measured(key):
-        return traffic.get(key, {}).get("hbm_bytes_per_launch")
+    def measured(kernel, key="hbm_bytes_per_launch"):
+        return traffic.get(kernel, {}).get(key)
 
-    icgn_bytes = 16 * npx + 124 * S
-    t_i = icgn6_ms / 1e3
-    if args.workload != 2:
-        traffic = {}  # the measured table is for the default workload
-    roof = dict(kernel="k_icgn (6-DOF IC-GN, ws 32)", bound="hbm",
-                achieved=round(icgn_bytes / t_i / 1e9, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(icgn_bytes / t_i / 1e9 / HBM_PEAK_GBS, 5),
-                traffic=measured("k_icgn_6dof"), algorithmic_bytes=icgn_bytes,
-                avg_ms=round(icgn6_ms, 5),
-                note="VALU- and latency-bound (FP32 Keys gathers and sums, 66 VALU per pixel pass)",
-                copy_gbs_measured=measure_copy_bw())
-    # VALU issue share: SQ_INSTS_VALU per launch (rocprofv3 counter run) x 4 cycles per
-    # wave64 VALU instruction / (1024 SIMDs x 2.4 GHz x the live average duration)
-    vi = traffic.get("k_icgn_6dof", {}).get("valu_instr_per_launch")
-    roof["valu_frac"] = round(vi * 4 / (1024 * VALU_CLOCK_HZ * t_i), 4) if vi else None
-    # The integer search beside it: int8 matrix cores, algorithmic ops = the NCC
-    # numerator's multiply-adds, 2 (2s+3)^2 (w+1)^2 per subset
+    # The integer search: int8 matrix cores, algorithmic ops = the NCC numerator's
+    # multiply-adds x 2, 2 (2s+3)^2 (w+1)^2 per subset; bytes = raw pair once + outputs
+    # (+ the window box-sum planes of the multi-tile kernel).  Timed live: the kernel's
+    # own events on the library's stream, averaged over the timed frames.
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
-    t_s = search_ms / 1e3
-    if pair.f.dtype in (np.uint8, np.uint16):
-        small = pair.f.dtype == np.uint8 and srad <= 14
-        bpp = pair.f.dtype.itemsize
-        # raw pair once + outputs (+ the window box-sum planes for the multi-tile kernel)
-        search_bytes = npx * 2 * bpp + 32 * S + (0 if small else npx * 36)
-        search_roof = dict(kernel="k_search_i8" if small else "k_search_mt", bound="mfma",
-                           engine="int8 MFMA (32x32x32)" + ("" if bpp == 1 else
-                                                            ", two digit planes (4 GEMMs)"),
-                           achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS,
-                           unit="TOP/s", frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
-                           traffic=measured("k_search_i8") if small else None,
-                           algorithmic_ops=ops, algorithmic_bytes=search_bytes,
-                           avg_ms=round(search_ms, 5))
-        # the kernel is bound by VALU issue, not the matrix cores: its VALU share as for
-        # IC-GN (counter run of the default workload only)
-        svi = traffic.get("k_search_i8", {}).get("valu_instr_per_launch") if small else None
-        search_roof["valu_frac"] = (round(svi * 4 / (1024 * VALU_CLOCK_HZ * t_s), 4)
-                                    if svi else None)
+    t_s = kt["search_kernel"] / 1e3
+    small = pair.f.dtype == np.uint8 and srad <= 14
+    skern = "k_search_i8" if small else "k_search_mt"
+    bpp = pair.f.dtype.itemsize
+    search_bytes = npx * 2 * bpp + 32 * S + (0 if small else npx * 36)
+    vs = measured(skern, "valu_instr_per_launch")
+    search_roof = dict(kernel=skern, bound="mfma",
+                       engine="int8 MFMA (32x32x32)" + ("" if bpp == 1 else
+                                                        ", two digit planes (4 GEMMs)"),
+                       achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS,
+                       unit="TOP/s", frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
+                       traffic=measured(skern), algorithmic_ops=ops,
+                       algorithmic_bytes=search_bytes, avg_ms=round(kt["search_kernel"], 5),
+                       valu_frac=(round(vs * 4 / (1024 * VALU_CLOCK_HZ * t_s), 4) if vs else None),
+                       mfma_per_launch=measured(skern, "mfma_instr_per_launch"))
+    # The 6-DOF IC-GN (one k_icgn launch per frame): bandwidth class, compulsory bytes
+    # 16 N_px (f, fx, fy, g as FP32) + 124 S (2 words in; 6 + 21 + 2 words out) per launch
+    icgn_bytes = 16 * npx + 124 * S
+    t_i = kt["icgn6_kernel"] / 1e3
+    vi = measured("k_icgn_6dof", "valu_instr_per_launch")
+    icgn_roof = dict(kernel="k_icgn (6-DOF IC-GN, ws 32)", bound="hbm",
+                     achieved=round(icgn_bytes / t_i / 1e9, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=round(icgn_bytes / t_i / 1e9 / HBM_PEAK_GBS, 5),
+                     traffic=measured("k_icgn_6dof"), algorithmic_bytes=icgn_bytes,
+                     avg_ms=round(kt["icgn6_kernel"], 5),
+                     valu_frac=(round(vi * 4 / (1024 * VALU_CLOCK_HZ * t_i), 4) if vi else None),
+                     note="VALU-issue-bound (FP32 Keys gathers and sums per pixel pass)")
+    # the dominant kernel of the frame is the roofline line
+    if kt["search_kernel"] >= kt["icgn6_kernel"]:
+        roof, extra_roof, extra_key = search_roof, icgn_roof, "icgn_roofline"
     else:
-        search_bytes = npx * 8 + 32 * S  # the FP32 pair once + outputs
-        search_roof = dict(kernel="k_search", bound="mfma",
-                           engine="VALU FP32 FMA (packed FP32 off)",
-                           achieved=round(ops / t_s / 1e12, 3), peak=FP32_PEAK_TFLOPS,
-                           unit="TFLOP/s", frac=round(ops / t_s / 1e12 / FP32_PEAK_TFLOPS, 5),
-                           traffic=None, algorithmic_ops=ops, algorithmic_bytes=search_bytes,
-                           avg_ms=round(search_ms, 5))
+        roof, extra_roof, extra_key = icgn_roof, search_roof, "search_roofline"
+    roof["copy_gbs_measured"] = measure_copy_bw()
 
     cpu = None
     acc = {}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+            len(os.sched_getaffinity(0))
         os.environ["OMP_NUM_THREADS"] = str(threads)
         import oracle
         oracle.build()
-        log(f"[bench] CPU baseline: FP64 oracle frame on {threads} threads")
-        o = oracle.frame(pair.f, pair.g, ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta,
-                         n_admm=n_admm)
-        itc = o["iters6"]
-        convc = int(np.sum((itc >= 1) & (itc <= 100)))
-        cpu = dict(value=round(convc / o["times"]["frame"], 2), unit="converged subsets/s",
-                   cores=threads, kind="port", cpu_model=cpu_model(),
-                   sample=f"one full frame of the same workload ({S} subsets), FP64 oracle, "
-                          f"{o['times']['frame']:.2f} s",
-                   stage_s={k: round(v, 4) for k, v in o["times"].items()})
-        same = (it6 == itc) & (it6 >= 1) & (it6 <= 100)
-        d6 = (out["U6"] - o["U6"]).reshape(-1, 2)[same]
-        dF = (out["F6"] - o["F6"]).reshape(-1, 4)[same]
-        acc = dict(icgn6_status_agree=round(float(np.mean(it6 == itc)), 5),
-                   icgn6_rms_px=float(np.sqrt(np.mean(d6 ** 2))),
-                   icgn6_max_px=float(np.abs(d6).max()),
-                   icgn6_max_F=float(np.abs(dF).max()),
-                   frame_rms_px=float(np.sqrt(np.nanmean((out["U"] - o["U"]) ** 2))),
-                   search_max_px=float(np.nanmax(np.abs(out["U0"] - o["U0"]))))
+        if args.workload == 3:
+            cpu, acc = cpu_sample(oracle, pair, out, S, ws, step, srad, grid, mu, args.sample,
+                                  threads)
+        else:
+            cpu, acc = cpu_full_frame(oracle, pair, out, S, ws, step, srad, grid, mu, beta,
+                                      n_admm, threads)
+        log(f"[bench] CPU baseline {cpu}")
         log(f"[bench] accuracy vs FP64: {acc}")
 
     line = {
@@ -284,16 +279,105 @@ def main():
         "config": {"workload": f"{wname} ({S} subsets), search {srad}, 3 ADMM iterations, "
                                "FEM Q4",
                    "subsets": S, "converged": conv, "frame_ms": round(ms_per_step, 4),
+                   "frame_ms_smoothing_on": None if smooth_ms is None else round(smooth_ms, 4),
+                   "kernel_ms": {k: round(kt[k], 4) for k in ("search_kernel", "icgn6_kernel")},
                    "stage_ms": {k: round(v, 4) for k, v in stage.items()},
                    "stage_ms_from": "separate untimed frames with an event at every stage boundary",
                    "gridfit": counters,
-                   "icgn6_subsets_per_s": round(conv / (icgn6_ms / 1e3), 1)},
+                   "icgn6_subsets_per_s": round(conv / t_i, 1)},
         "roofline": roof,
-        "search_roofline": search_roof,
+        extra_key: extra_roof,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": acc or None,
     }
     print(json.dumps(line), flush=True)
+
+
+def cpu_full_frame(oracle, pair, out, S, ws, step, srad, grid, mu, beta, n_admm, threads):
+    """The oracle's whole frame (workload 2): CPU baseline and accuracy of the device frame."""
+    log(f"[bench] CPU baseline: FP64 oracle frame on {threads} threads")
+    o = oracle.frame(pair.f, pair.g, ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta,
+                     n_admm=n_admm)
+    it6 = out["iters6"]
+    itc = o["iters6"]
+    convc = int(np.sum((itc >= 1) & (itc <= 100)))
+    cpu = dict(value=round(convc / o["times"]["frame"], 2), unit="converged subsets/s",
+               cores=threads, kind="port", cpu_model=cpu_model(),
+               sample=f"one full frame of the same workload ({S} subsets), FP64 oracle, "
+                      f"{o['times']['frame']:.2f} s",
+               stage_s={k: round(v, 4) for k, v in o["times"].items()})
+    same = (it6 == itc) & (it6 >= 1) & (it6 <= 100)
+    d6 = (out["U6"] - o["U6"]).reshape(-1, 2)[same]
+    dF = (out["F6"] - o["F6"]).reshape(-1, 4)[same]
+    acc = dict(icgn6_status_agree=round(float(np.mean(it6 == itc)), 5),
+               icgn6_rms_px=float(np.sqrt(np.mean(d6 ** 2))),
+               icgn6_max_px=float(np.abs(d6).max()),
+               icgn6_max_F=float(np.abs(dF).max()),
+               frame_rms_px=float(np.sqrt(np.nanmean((out["U"] - o["U"]) ** 2))),
+               search_max_px=float(np.nanmax(np.abs(out["U0"] - o["U0"]))))
+    acc["gate_0p01px"] = bool(acc["icgn6_rms_px"] <= 0.01)
+    return cpu, acc
+
+
+def cpu_sample(oracle, pair, out, S, ws, step, srad, grid, mu, frac, threads):
+    """Workload 3: the FP64 oracle on a seeded sample of the subsets.  Accuracy: the
+    device's search peaks and 6-DOF IC-GN results on the sample against the oracle's.
+    CPU time: normalisation + gradients of the whole pair once, then search + 6-DOF
+    IC-GN + three 2-DOF IC-GN passes on the sample (median of 3), scaled by S / n."""
+    n = max(1, int(round(frac * S)))
+    q = np.sort(np.random.default_rng(2026).choice(S, n, replace=False)).astype(np.int32)
+    log(f"[bench] FP64 oracle on a seeded {n}-subset sample ({100 * n / S:.1f} %), "
+        f"{threads} threads")
+    t = time.perf_counter()
+    f, _ = oracle.normalize(pair.f.astype(np.float64))
+    g, _ = oracle.normalize(pair.g.astype(np.float64))
+    gx, gy = oracle.gradient(f)
+    t_prep = time.perf_counter() - t
+    s2 = [srad, srad]
+    times = []
+    for rep in range(3):
+        t = time.perf_counter()
+        r = oracle.search_sample(f, g, grid, ws, step, s2, q)
+        t1 = time.perf_counter()
+        xy = np.stack([r["x"], r["y"]], 1)
+        U0 = np.stack([r["u"], r["v"]], 1).ravel()
+        U6, F6, H, it6o, _ = oracle.icgn6(f, g, gx, gy, xy, U0, ws)
+        t2 = time.perf_counter()
+        z4, z2 = np.zeros(4 * n), np.zeros(2 * n)
+        for _ in range(3):
+            oracle.icgn2(f, g, gx, gy, xy, ws, H, mu, z4, z2, U6, F6)
+        t3 = time.perf_counter()
+        times.append((t3 - t, t1 - t, t2 - t1, t3 - t2))
+    times.sort()
+    tot, ts, t6, t2 = times[1]
+    conv_o = int(np.sum((it6o >= 1) & (it6o <= 100)))
+    frame_s = t_prep + tot * S / n
+    cpu = dict(value=round(conv_o * S / n / frame_s, 2), unit="converged subsets/s",
+               cores=threads, kind="port", cpu_model=cpu_model(),
+               sample=f"seeded {n}-subset sample ({100 * n / S:.1f} %) of the same frame: "
+                      f"search + 6-DOF + 3 x 2-DOF IC-GN in {tot:.2f} s (median of 3), scaled "
+                      f"by S/n to {frame_s:.1f} s per frame with the whole-image "
+                      f"normalisation ({t_prep:.2f} s); global steps not included",
+               stage_s=dict(prep=round(t_prep, 3), search=round(ts * S / n, 2),
+                            icgn6=round(t6 * S / n, 2), icgn2_x3=round(t2 * S / n, 2),
+                            frame_local=round(frame_s, 2)))
+    Ud0 = out["U0"].reshape(-1, 2)[q]
+    Ud6 = out["U6"].reshape(-1, 2)[q]
+    Fd6 = out["F6"].reshape(-1, 4)[q]
+    itd = out["iters6"][q]
+    fin = np.isfinite(U0.reshape(-1, 2)).all(1)
+    same = (itd == it6o) & (itd >= 1) & (itd <= 100)
+    d6 = Ud6[same] - U6.reshape(-1, 2)[same]
+    dF = Fd6[same] - F6.reshape(-1, 4)[same]
+    acc = dict(sample=int(n),
+               search_peaks_identical=bool(np.array_equal(Ud0[fin], U0.reshape(-1, 2)[fin])),
+               search_max_px=float(np.abs(Ud0[fin] - U0.reshape(-1, 2)[fin]).max(initial=0.0)),
+               icgn6_status_agree=round(float(np.mean(itd == it6o)), 5),
+               icgn6_rms_px=float(np.sqrt(np.mean(d6 ** 2))),
+               icgn6_max_px=float(np.abs(d6).max(initial=0.0)),
+               icgn6_max_F=float(np.abs(dF).max(initial=0.0)))
+    acc["gate_0p01px"] = bool(acc["icgn6_rms_px"] <= 0.01)
+    return cpu, acc
 
 
 def run_quadtree(args, pkg):

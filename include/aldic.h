@@ -238,8 +238,11 @@ int aldic_frame_run(aldic_ctx *ctx);
 int aldic_frame_get(aldic_ctx *ctx, double *U, double *F, double *U6, double *F6, int *iters6,
                     int *iters2, double *u0);
 /* Stage times of the last frame in ms: {prep, search, icgn6, post6, subpb2 total,
- * icgn2 total, strain+dual total, frame}; with stage_times 0 only search, icgn6 and
- * frame are measured (the others read 0). n <= 8. */
+ * icgn2 total, strain+dual total, frame, search kernel, 6-DOF IC-GN kernel}; with
+ * stage_times 0 only search, icgn6, frame and the two kernels are measured (the others
+ * read 0).  The search kernel's time excludes the window box-sum planes built before it;
+ * the 6-DOF kernel's excludes the re-launch of windows enlarged by the mask rule.
+ * n <= 10. */
 int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
 /* Counters of the frame driver (no counterpart in the reference): {iteration kernels
  * each Subpb1 gridfit of the next frame runs before its convergence is checked,

@@ -500,7 +500,34 @@ void orc_search_grid(int nx, int ny, const int *grid, int w, int step, const int
  * max(sum D^2 - (sum D)^2 / n, 0)) over the template footprint, n = (w+1)^2;
  * 0 where the window variance is zero.  Returns -1 for a constant template (the
  * reference's normxcorr2 raises, :153-157). */
-static int ncc_full(const double *T, int tw, const double *D, int Dx, int Dy, double *C)
+/* NCC at one output position (cx, cy) of the full map (template corner at D offset
+ * (cx - tw + 1, cy - tw + 1)); mt, st = template mean and sum of squared deviations. */
+static double ncc_lag(const double *T, int tw, double mt, double st, const double *D, int Dx,
+                      int Dy, int cx, int cy)
+{
+    int n = tw * tw;
+    int ox = cx - (tw - 1), oy = cy - (tw - 1); /* template corner in D (0-based) */
+    double num = 0.0, sd = 0.0, sd2 = 0.0;
+    for (int j = 0; j < tw; ++j) {
+        int dy = oy + j;
+        if (dy < 0 || dy >= Dy) continue;
+        for (int i = 0; i < tw; ++i) {
+            int dx = ox + i;
+            if (dx < 0 || dx >= Dx) continue;
+            double d = D[(size_t)dy * Dx + dx];
+            num += (T[j * tw + i] - mt) * d;
+            sd += d;
+            sd2 += d * d;
+        }
+    }
+    double va = sd2 - sd * sd / n;
+    if (va < 0) va = 0;
+    double den = sqrt(st * va);
+    return den > 0 ? num / den : 0.0;
+}
+
+/* Template mean and sum of squared deviations; -1 for a constant template. */
+static int tstats(const double *T, int tw, double *mt_out, double *st_out)
 {
     int n = tw * tw;
     double mt = 0.0;
@@ -510,29 +537,18 @@ static int ncc_full(const double *T, int tw, const double *D, int Dx, int Dy, do
     for (int i = 0; i < n; ++i) st += (T[i] - mt) * (T[i] - mt);
     int allsame = 1;
     for (int i = 1; i < n; ++i) if (T[i] != T[0]) { allsame = 0; break; }
-    if (allsame) return -1;
+    *mt_out = mt;
+    *st_out = st;
+    return allsame ? -1 : 0;
+}
+
+static int ncc_full(const double *T, int tw, const double *D, int Dx, int Dy, double *C)
+{
+    double mt, st;
+    if (tstats(T, tw, &mt, &st)) return -1;
     int Cx = tw - 1 + Dx, Cy = tw - 1 + Dy;
     for (int cy = 0; cy < Cy; ++cy)
-        for (int cx = 0; cx < Cx; ++cx) {
-            int ox = cx - (tw - 1), oy = cy - (tw - 1); /* template corner in D (0-based) */
-            double num = 0.0, sd = 0.0, sd2 = 0.0;
-            for (int j = 0; j < tw; ++j) {
-                int dy = oy + j;
-                if (dy < 0 || dy >= Dy) continue;
-                for (int i = 0; i < tw; ++i) {
-                    int dx = ox + i;
-                    if (dx < 0 || dx >= Dx) continue;
-                    double d = D[(size_t)dy * Dx + dx];
-                    num += (T[j * tw + i] - mt) * d;
-                    sd += d;
-                    sd2 += d * d;
-                }
-            }
-            double va = sd2 - sd * sd / n;
-            if (va < 0) va = 0;
-            double den = sqrt(st * va);
-            C[(size_t)cy * Cx + cx] = den > 0 ? num / den : 0.0;
-        }
+        for (int cx = 0; cx < Cx; ++cx) C[(size_t)cy * Cx + cx] = ncc_lag(T, tw, mt, st, D, Dx, Dy, cx, cy);
     return 0;
 }
 
@@ -658,6 +674,50 @@ void orc_search(const double *f, const double *g, int nx, int ny, int x1, int y1
         }
         free(T); free(D); free(C); free(crop);
     }
+}
+
+/* orc_search restricted to a list of subsets (entries q = l*Nx + k of the whole-field
+ * grid) and to the cropped lags (funIntegerSearch.m:134), without q-factors: the same
+ * per-lag NCC as the full map, for sampled accuracy checks and CPU timing on large
+ * frames.  Outputs are per list entry. */
+void orc_search_sample(const double *f, const double *g, int nx, int ny, int x1, int y1, int Nx,
+                       int w, int step, const int *s, int nq, const int *qlist, double *x,
+                       double *y, double *u, double *v, double *ccmax, int *status)
+{
+    int tw = w + 1, Dx = tw + 2 * s[0], Dy = tw + 2 * s[1];
+    int nr = 2 * s[0] + 3, nc = 2 * s[1] + 3;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int t = 0; t < nq; ++t) {
+        int q = qlist[t];
+        int k = q % Nx, l = q / Nx;
+        int ii = x1 + k * step, jj = y1 + l * step;
+        double *T = (double *)malloc(sizeof(double) * tw * tw);
+        double *D = (double *)malloc(sizeof(double) * Dx * Dy);
+        double *crop = (double *)malloc(sizeof(double) * nr * nc);
+        for (int j = 0; j < tw; ++j)
+            for (int i = 0; i < tw; ++i) T[j * tw + i] = f[IDX(ii + i, jj + j, nx)];
+        for (int j = 0; j < Dy; ++j)
+            for (int i = 0; i < Dx; ++i) D[(size_t)j * Dx + i] = g[IDX(ii - s[0] + i, jj - s[1] + j, nx)];
+        x[t] = (ii + ii + w) / 2.0;
+        y[t] = (jj + jj + w) / 2.0;
+        double mt, st;
+        if (tstats(T, tw, &mt, &st)) {
+            u[t] = v[t] = ccmax[t] = NAN;
+            status[t] = -1;
+        } else {
+            for (int c = 0; c < nc; ++c)
+                for (int r = 0; r < nr; ++r)
+                    crop[c * nr + r] = ncc_lag(T, tw, mt, st, D, Dx, Dy, w - 1 + r, w - 1 + c);
+            double xp, yp, mf;
+            findpeak(crop, nr, nc, &xp, &yp, &mf);
+            u[t] = yp - (s[0] + 2);
+            v[t] = xp - (s[1] + 2);
+            ccmax[t] = mf;
+            status[t] = 0;
+        }
+        free(T); free(D); free(crop);
+    }
+    (void)ny;
 }
 
 /* ------------------------------------------------------------------------------

@@ -70,6 +70,8 @@ def lib():
         L.orc_findpeak.argtypes = [f64p, I, I, f64p]
         L.orc_search.argtypes = [f64p, f64p, I, I, I, I, I, I, I, I, i32p, f64p, f64p, f64p,
                                  f64p, f64p, f64p, f64p, i32p]
+        L.orc_search_sample.argtypes = [f64p, f64p, I, I, I, I, I, I, I, i32p, I, i32p, f64p,
+                                        f64p, f64p, f64p, f64p, i32p]
         L.orc_subpb2_assemble.argtypes = [f64p, I, i32p, I, D, D, D, f64p, f64p, f64p, f64p,
                                           i32p, i32p, f64p, f64p]
         L.orc_pcg.argtypes = [I, i32p, i32p, f64p, f64p, f64p, D, I]
@@ -81,6 +83,7 @@ def lib():
         L.orc_natural_fill.restype = I
         for fn in ("orc_normalize", "orc_gradient", "orc_bicubic", "orc_ic_compose", "orc_icgn6",
                    "orc_icgn2", "orc_search_grid", "orc_ncc_full", "orc_findpeak", "orc_search",
+                   "orc_search_sample",
                    "orc_subpb2_assemble", "orc_nodal_strain"):
             getattr(L, fn).restype = V
         _LIB = L
@@ -206,6 +209,24 @@ def search(f, g, grid, w, step, s):
     x, y, u, v, cm, pce, ppe = [o.reshape(Ny, Nx) for o in outs]
     return dict(x=x, y=y, u=u, v=v, ccmax=cm, pce=pce, ppe=ppe, status=st.reshape(Ny, Nx),
                 x1=x1, y1=y1, Nx=Nx, Ny=Ny)
+
+
+def search_sample(f, g, grid, w, step, s, qlist):
+    """search() on the listed subsets only (entries l*Nx + k of the whole-field grid),
+    cropped lags only, no q-factors -> per-entry x, y, u, v, ccmax, status."""
+    f = _f64(f)
+    g = _f64(g)
+    ny, nx = f.shape
+    s = _i32(s if np.ndim(s) else [s, s])
+    x1, y1, Nx, Ny = search_grid(nx, ny, grid, w, step, s)
+    q = _i32(qlist)
+    assert q.size == 0 or (q.min() >= 0 and q.max() < Nx * Ny)
+    n = q.size
+    outs = [np.empty(n) for _ in range(5)]
+    st = np.empty(n, np.int32)
+    lib().orc_search_sample(f, g, nx, ny, x1, y1, Nx, w, step, s, n, q, *outs, st)
+    x, y, u, v, cm = outs
+    return dict(x=x, y=y, u=u, v=v, ccmax=cm, status=st)
 
 
 def subpb2_assemble(coords, elems, beta, mu, U, F, udual, vdual, alpha=0.0):
