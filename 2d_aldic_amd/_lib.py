@@ -26,6 +26,7 @@ EXPORTS = (
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
     "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
     "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc", "aldic_natural_fill",
+    "aldic_frame_run_seeded",
 )
 
 
@@ -88,6 +89,7 @@ def load():
     L.aldic_qframe_get.argtypes = [P, vp, vp, vp, vp, vp]
     L.aldic_prep_noalloc.argtypes = [P, vp, vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp]
     L.aldic_natural_fill.argtypes = [P, ip, vp, vp, vp, vp]
+    L.aldic_frame_run_seeded.argtypes = [P, vp]
     _lib = L
     return L
 
@@ -382,6 +384,13 @@ class Context:
 
     def frame_run(self):
         self._ck(self.L.aldic_frame_run(self.h), "aldic_frame_run")
+
+    def frame_run_seeded(self, U0):
+        """The frame from the initial guess U0 (2S) instead of the integer search (a later
+        frame of a sequence, main_ALDIC.m:134-137)."""
+        U0 = _c(U0, np.float64)
+        assert U0.size == 2 * self.frame_nodes
+        self._ck(self.L.aldic_frame_run_seeded(self.h, _p(U0)), "aldic_frame_run_seeded")
 
     def frame_get(self):
         S = self.frame_nodes

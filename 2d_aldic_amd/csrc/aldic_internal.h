@@ -151,6 +151,7 @@ struct FrameState {
         Us, Fs, U1s, U1old, U2old, errs;  // smoothing temporaries, stop-rule copies, norms
     GridfitDev gf;  // Subpb1's gridfit over the node grid
     NatGrid nat;    // the node grid for LocalICGN's bad-node refill
+    DBuf seed;      // aldic_frame_run_seeded: the caller's initial guess (2S)
     DBuf nat_err;   // refills of the frame whose cell could not be formed (device int)
     int *nat_err_h = nullptr;  // pinned copy, read after the frame's wait
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F

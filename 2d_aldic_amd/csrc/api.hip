@@ -276,6 +276,7 @@ void aldic_destroy(aldic_ctx *ctx)
     }
     F.gfstat.release();
     F.nat_err.release();
+    F.seed.release();
     ctx->qf.nat_err.release();
     if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
     if (ctx->qf.nat_err_h) (void)hipHostFree(ctx->qf.nat_err_h);
@@ -1075,7 +1076,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
 // One pass of the frame.  gf_poll false: each Subpb1 gridfit runs a fixed budget of
 // iteration kernels with no host wait and records whether it converged in F.gfh; true:
 // each polls the host for convergence.
-static int frame_pass(aldic_ctx *ctx, bool gf_poll)
+static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
 {
     FrameState &F = ctx->fr;
     hipStream_t st = ctx->stream;
@@ -1132,7 +1133,13 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     sa.U0 = F.U0.as<double>();
     sa.ev_kernel = F.ev_skern;
     search_raw8(ctx, sa);
-    CK(launch_search(sa, st));
+    if (!seeded) {
+        CK(launch_search(sa, st));
+    } else {
+        // main_ALDIC.m:136-137: the previous frame's U is the initial guess (no search)
+        CK(hipEventRecord(F.ev_skern, st));
+        CK(hipMemcpyAsync(F.U0.p, F.seed.p, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
+    }
     // the search's NaNs (constant templates) go to LocalICGN as they are: their subsets
     // end non-finite (status 101) and the bad-point refill covers them
     const int e_search = mark();
@@ -1362,16 +1369,13 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll)
     return ALDIC_OK;
 }
 
-int aldic_frame_run(aldic_ctx *ctx)
+static int frame_run_impl(aldic_ctx *ctx, bool seeded)
 {
-    if (!ctx) return ALDIC_ERR_ARG;
     FrameState &F = ctx->fr;
-    if (!F.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_run: frame_setup first");
-    CK(hipSetDevice(ctx->device));
     // The frame runs without a host round trip; should a gridfit have needed more than
     // its iteration budget, the frame runs again with polled gridfits (it depends only
     // on the staged images and options, so the second pass gives the polled result).
-    int r = frame_pass(ctx, false);
+    int r = frame_pass(ctx, false, seeded);
     if (r) return r;
     if (F.gf_count == 0) return ALDIC_OK;
     bool ok = true;
@@ -1382,13 +1386,32 @@ int aldic_frame_run(aldic_ctx *ctx)
     }
     if (!ok) {
         ++F.gf_reruns;
-        if ((r = frame_pass(ctx, true)) != ALDIC_OK) return r;
+        if ((r = frame_pass(ctx, true, seeded)) != ALDIC_OK) return r;
         for (int k = 0; k < F.gf_count; ++k) most = std::max(most, (int)F.gfh[3 * k + 1]);
     }
     // the next frame's budget: the most iterations seen here and a margin
     F.gf_most = most;
     F.gf_budget = std::min(64, std::max(6, most + 3));
     return ALDIC_OK;
+}
+
+int aldic_frame_run(aldic_ctx *ctx)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->fr.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_run: frame_setup first");
+    CK(hipSetDevice(ctx->device));
+    return frame_run_impl(ctx, false);
+}
+
+int aldic_frame_run_seeded(aldic_ctx *ctx, const double *U0)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    FrameState &F = ctx->fr;
+    if (!F.ready) return fail(ctx, ALDIC_ERR_STATE, "frame_run_seeded: frame_setup first");
+    if (!U0) return fail(ctx, ALDIC_ERR_ARG, "frame_run_seeded: U0");
+    CK(hipSetDevice(ctx->device));
+    CK(up(F.seed, U0, 2 * (size_t)F.S, ctx->stream));
+    return frame_run_impl(ctx, true);
 }
 
 int aldic_frame_scalars(aldic_ctx *ctx, double *out, int n)

@@ -233,6 +233,10 @@ int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int 
                       const uint8_t *mask, const int *roi);
 int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *opts, int *n_nodes);
 int aldic_frame_run(aldic_ctx *ctx);
+/* main_ALDIC.m:134-137 (a later frame of a sequence with DICpara.NewFFTSearch = 0, images
+ * 3..6): the same frame with the integer search replaced by the caller's initial guess
+ * U0 (2S, interleaved; e.g. the previous frame's U, ResultDisp{ImgSeqNum-2}.U). */
+int aldic_frame_run_seeded(aldic_ctx *ctx, const double *U0);
 /* Results of the last frame: U, F (after the last global step), U6/F6/iters6 of the
  * 6-DOF IC-GN before post-processing, iters of the last 2-DOF step; any may be NULL. */
 int aldic_frame_get(aldic_ctx *ctx, double *U, double *F, double *U6, double *F6, int *iters6,

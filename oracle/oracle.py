@@ -609,7 +609,8 @@ def natural_fill(coords, V, nc, bad, return_kind=False):
 
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
-          cg_rtol=1e-10, subpb1_fit=True, smooth=False, beta_sweep=False, admm_tol=0.0):
+          cg_rtol=1e-10, subpb1_fit=True, smooth=False, beta_sweep=False, admm_tol=0.0,
+          U0=None):
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
     refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1: 2-DOF IC-GN, bad
     points, gridfit of every node; Subpb2, strain, duals).  The LocalICGN refill of bad
@@ -621,7 +622,9 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     steps (:381, :400).  beta_sweep=True picks beta as main_ALDIC.m:237,274-301 does
     (FE branch) from the step-1 global step at the six candidates.  admm_tol > 0 stops
     the ADMM steps once the RMS change of U2 or of Subpb1's U (before smoothing) between
-    steps falls below it (main_ALDIC.m:409-447).  Returns the stages' outputs and
+    steps falls below it (main_ALDIC.m:409-447).  U0 (2S) replaces the integer search
+    with the given initial guess (a later frame of a sequence, main_ALDIC.m:134-137).
+    Returns the stages' outputs and
     times, the beta used and the number of ADMM steps run."""
     import time
     import scipy.sparse as sp
@@ -632,12 +635,21 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     gx, gy = gradient(f)
     T["prep"] = time.perf_counter() - t0
     t = time.perf_counter()
-    r = search(f, g, grid, ws, step, [s, s] if np.isscalar(s) else s)
+    if U0 is None:
+        r = search(f, g, grid, ws, step, [s, s] if np.isscalar(s) else s)
+        Nx, Ny = r["Nx"], r["Ny"]
+        xy = np.stack([r["x"].ravel(), r["y"].ravel()], 1)
+        U0 = np.stack([r["u"].ravel(), r["v"].ravel()], 1).ravel()
+    else:
+        # main_ALDIC.m:134-137: a later frame starts from the previous frame's U
+        ny_, nx_ = f.shape
+        x1, y1, Nx, Ny = search_grid(nx_, ny_, grid, ws, step, [s, s] if np.isscalar(s) else s)
+        kk, ll = np.meshgrid(np.arange(Nx), np.arange(Ny))
+        xy = np.stack([(x1 + kk * step + 0.5 * ws).ravel(), (y1 + ll * step + 0.5 * ws).ravel()],
+                      1).astype(np.float64)
+        U0 = np.asarray(U0, np.float64).ravel().copy()
     T["search"] = time.perf_counter() - t
-    Nx, Ny = r["Nx"], r["Ny"]
     S = Nx * Ny
-    xy = np.stack([r["x"].ravel(), r["y"].ravel()], 1)
-    U0 = np.stack([r["u"].ravel(), r["v"].ravel()], 1).ravel()
     t = time.perf_counter()
     U6, F6, H, it6, z = icgn6(f, g, gx, gy, xy, U0, ws, tol)
     T["icgn6"] = time.perf_counter() - t
