@@ -189,41 +189,27 @@ int icgn_dispatch(aldic_ctx *ctx, bool six, IcgnArgs a, const int *ws_host)
             CK(hipStreamSynchronize(ctx->stream));  // list buffer is reused
         }
     }
-    // windows enlarged by the mask rule beyond their class: relaunch one class up
-    for (int round = 0; round < 3; ++round) {
-        std::vector<int> it(S);
-        CK(down(it.data(), iters, S, ctx->stream));
+    // windows the mask rule enlarged to round(sqrt(2) w) beyond their class (status -2,
+    // funICGN.m:57-65): one relaunch in the class that holds the enlarged window; when no
+    // class does (enlarged window above 67 x 67 px) the subset keeps status -2
+    std::vector<int> it(S);
+    CK(down(it.data(), iters, S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int ci = 0; ci < 4; ++ci) {
+        std::vector<int> l2;
+        for (int j = 0; j < S; ++j) {
+            if (it[j] != -2) continue;
+            const int w = ws_host ? ws_host[j] : a.ws_all;
+            const int big = icgn_ppl_for((int)std::floor(std::sqrt(2.0) * w + 0.5));
+            if (big == classes[ci] && big != cls[j]) l2.push_back(j);
+        }
+        if (l2.empty()) continue;
+        CK(up(lst, l2.data(), l2.size(), ctx->stream));
+        a.list = lst.as<int>();
+        a.nlist = (int)l2.size();
+        CK(six ? launch_icgn6(a, classes[ci], ctx->stream)
+               : launch_icgn2(a, classes[ci], ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
-        std::vector<int> l;
-        for (int j = 0; j < S; ++j)
-            if (it[j] == -2) l.push_back(j);
-        if (l.empty()) break;
-        int next = 0;
-        for (int j : l) {
-            const int cur = cls[j];
-            // round(sqrt(2) w) of a window of class 5 or 8 fits class 18
-            next = cur <= 8 ? 18 : 72;
-            cls[j] = next;
-        }
-        if (round == 2 || next == 0) {
-            // beyond the largest class: reported as an indexing error
-            for (int j : l) it[j] = -1;
-            CK(hipMemcpyAsync(iters, it.data(), sizeof(int) * S, hipMemcpyHostToDevice,
-                              ctx->stream));
-            break;
-        }
-        for (int ci = 0; ci < 4; ++ci) {
-            std::vector<int> l2;
-            for (int j : l)
-                if (cls[j] == classes[ci]) l2.push_back(j);
-            if (l2.empty()) continue;
-            CK(up(lst, l2.data(), l2.size(), ctx->stream));
-            a.list = lst.as<int>();
-            a.nlist = (int)l2.size();
-            CK(six ? launch_icgn6(a, classes[ci], ctx->stream)
-                   : launch_icgn2(a, classes[ci], ctx->stream));
-            CK(hipStreamSynchronize(ctx->stream));
-        }
     }
     return ALDIC_OK;
 }
@@ -1255,7 +1241,11 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
         const int g0 = smark();
         CK(fem_rhs(ctx->mesh, beta, o.mu, U1, F1p, zero_duals ? nullptr : F.udual.as<double>(),
                    zero_duals ? nullptr : F.vdual.as<double>(), b, st));
-        CK(fem_solve(ctx->mesh, b, U2, o.cg_rtol, o.cg_maxit, work, nullptr, st, 8));
+        double gst[2] = {0.0, 0.0};
+        CK(fem_solve(ctx->mesh, b, U2, o.cg_rtol, o.cg_maxit, work,
+                     ctx->mesh.structured ? nullptr : gst, st, 8));
+        if (!ctx->mesh.structured && !(std::isfinite(gst[1]) && gst[1] <= o.cg_rtol))
+            return fail(ctx, ALDIC_ERR_NUMERIC, "frame_run: global-step PCG did not converge");
         const int g1 = smark();
         span(4, g0, g1);
         // nodal strain and the dual update of the FEM branch (main_ALDIC.m:340,440):
@@ -1640,7 +1630,10 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     int total = 0, most = 0;
     bool ok = true;
     for (int k = 0; k < nglob; ++k) {
-        ok = ok && Q.hstat[3 * k] != 0.0;
+        // converged: the stop flag AND a finite relative residual within the tolerance (the
+        // flag is also set on a breakdown or a NaN residual)
+        const double rel = Q.hstat[3 * k + 2];
+        ok = ok && Q.hstat[3 * k] != 0.0 && std::isfinite(rel) && rel <= Q.o.cg_rtol;
         total += (int)Q.hstat[3 * k + 1];
         most = std::max(most, (int)Q.hstat[3 * k + 1]);
     }

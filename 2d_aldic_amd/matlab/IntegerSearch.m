@@ -1,0 +1,18 @@
+function [DICpara,x0,y0,u,v,cc]= IntegerSearch(ImgRef,ImgDef,file_name,DICpara)
+% Library-backed replacement of IntegerSearch.m:1-167 (called main_ALDIC.m:90): the
+% whole-field FFT-free integer search of funIntegerSearch.m:82-200 (normxcorr2 cropped
+% to the (2s+3)^2 lags, findpeak.m fit) on the GPU.  Non-interactive: the search radius
+% is DICpara.SizeOfFFTSearchRegion (scalar or [sx sy]), the grid DICpara.gridxyROIRange,
+% winsize and winstepsize as in DICpara.  The interactive outlier clean-up of the
+% reference is not part of this call.  ImgRef / ImgDef are fNormalized / gNormalized.
+s = DICpara.SizeOfFFTSearchRegion; if numel(s) == 1, s = [s s]; end
+grid = [DICpara.gridxyROIRange.gridx(1) DICpara.gridxyROIRange.gridx(2) ...
+        DICpara.gridxyROIRange.gridy(1) DICpara.gridxyROIRange.gridy(2)];
+aldic_mex('set_normalized', double(ImgRef), double(ImgDef));
+[nx, ny] = size(ImgRef);
+[x0, y0, u, v, ccmax, qf, ~] = aldic_mex('integer_search', nx, ny, grid, ...
+    DICpara.winsize, DICpara.winstepsize, s, 1);
+cc = struct('max', ccmax, 'qfactors', qf, 'A', []);
+DICpara.SizeOfFFTSearchRegion = s;
+if nargin >= 3 && ~isempty(file_name), DICpara.ImgSize = [nx ny]; end
+end

@@ -15,8 +15,11 @@
  *   - Per-subset status (the reference's ConvItPerEle): 1..100 = converged after
  *     that many residual evaluations, 101 = not converged / left the image / NaN,
  *     102 = more than half of the subset masked, -1 = indexing error (window or
- *     its gradient outside the image).  Failed subsets return NaN like the
- *     reference's try/catch.
+ *     its gradient outside the image), -2 = the mask rule (funICGN.m:57-65) enlarged
+ *     the window to round(sqrt(2) ws) > 66 px, beyond the largest subset this library
+ *     holds (67 x 67 px), so it was not processed (the reference would process it;
+ *     reachable only for ws >= 48 with 10-50 % masked gradients).  Failed subsets
+ *     return NaN like the reference's try/catch; the bad-point rules treat -2 as failed.
  *   - Buffers are caller-owned host memory; the library owns device memory.  Calls
  *     are synchronous on return.  One context per host thread.
  *   - Return value: 0 on success, a negative ALDIC_ERR_* code otherwise
@@ -124,8 +127,11 @@ int aldic_set_mesh(aldic_ctx *ctx, const double *coords, int nn, const int *elem
 
 /* Subpb2.m:1-197 (called main_ALDIC.m:287,304,393): assembles
  * (beta K + mu M + alpha mu K) Uhat = beta G (F - udual) + mu M (U - vdual) with
- * 3x3 Gauss points and solves it with FP64 Jacobi-PCG to relative residual rtol
- * (maxit iterations at most).  stats (nullable) = {iterations, relative residual}. */
+ * 3x3 Gauss points and solves it: directly by fast diagonalisation (FP64 GEMMs with
+ * the 1-D generalised eigenbases) on a structured grid (nodes on a uniform lattice, x
+ * fastest, the elements of MeshSetUp), else with FP64 Jacobi-PCG to relative residual
+ * rtol (maxit iterations at most).  stats (nullable) = {iterations (0 = direct),
+ * relative residual}. */
 int aldic_subpb2(aldic_ctx *ctx, double beta, double mu, double alpha, const double *U,
                  const double *F, const double *udual, const double *vdual, double rtol,
                  int maxit, double *Uhat, double *stats);
@@ -200,7 +206,8 @@ int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const do
  * nodal strain + dual update at fixed beta, and n_admm iterations of
  * (2-DOF IC-GN with Subpb1's bad-point rule and gridfit refit of every node
  * (Subpb1.m:165-199) -> Subpb2 -> nodal strain -> dual update)
- * (main_ALDIC.m:357-451).  Smoothing (funSmoothDisp/funSmoothStrain) is off.
+ * (main_ALDIC.m:357-451).  Smoothing (funSmoothDisp/funSmoothStrain) runs only with
+ * the `smooth` option below.
  * A gridfit runs a budget of CG iterations (the most the previous frame needed plus
  * a margin) without a host wait; if one needs more, the frame runs a second time with
  * gridfits that poll for convergence, so the result never depends on the budget. */
