@@ -610,7 +610,7 @@ def natural_fill(coords, V, nc, bad, return_kind=False):
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
           cg_rtol=1e-10, subpb1_fit=True, smooth=False, beta_sweep=False, admm_tol=0.0,
-          U0=None):
+          U0=None, mask=None):
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
     refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1: 2-DOF IC-GN, bad
     points, gridfit of every node; Subpb2, strain, duals).  The LocalICGN refill of bad
@@ -623,7 +623,8 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     (FE branch) from the step-1 global step at the six candidates.  admm_tol > 0 stops
     the ADMM steps once the RMS change of U2 or of Subpb1's U (before smoothing) between
     steps falls below it (main_ALDIC.m:409-447).  U0 (2S) replaces the integer search
-    with the given initial guess (a later frame of a sequence, main_ALDIC.m:134-137).
+    with the given initial guess (a later frame of a sequence, main_ALDIC.m:134-137);
+    mask (ny x nx, 0/1) multiplies the reference gradients (funImgGradient.m:77-78).
     Returns the stages' outputs and
     times, the beta used and the number of ADMM steps run."""
     import time
@@ -632,7 +633,7 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     t0 = time.perf_counter()
     f, _ = normalize(np.asarray(fraw, np.float64))
     g, _ = normalize(np.asarray(graw, np.float64))
-    gx, gy = gradient(f)
+    gx, gy = gradient(f, mask)
     T["prep"] = time.perf_counter() - t0
     t = time.perf_counter()
     if U0 is None:

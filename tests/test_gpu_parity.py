@@ -35,6 +35,16 @@ def _frame_parity(out, o, tol_u=1e-4, tol_f=1e-5):
     return dU, dF
 
 
+def _assert_tie_free(orc, imgs, o, ws, tol, rel=2e-4):
+    """The oracle's LocalICGN counts do not change when the stop tolerance moves by
+    +-rel: no node stops with its update norm that close to the tolerance, so FP32
+    rounding on the device cannot flip a count."""
+    f, g, gx, gy = imgs
+    for t in (tol * (1 - rel), tol * (1 + rel)):
+        it = orc.icgn6(f, g, gx, gy, o["xy"], o["U0"], ws, tol=t)[3]
+        assert np.array_equal(it, o["iters6"]), np.flatnonzero(it != o["iters6"])
+
+
 @pytest.fixture
 def c1(gpu_ctx, cfg1, cfg1_norm):
     _flush("config1: set_images")
@@ -327,6 +337,85 @@ def test_icgn6_far_initial_guess_and_large_stretch(gpu_ctx, speckle, orc, case):
     Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gx, gy, xy, U0, 32)
     ok = _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.95)
     assert ok.mean() > 0.5
+
+
+@pytest.mark.parametrize("ws", [10, 12, 14])
+def test_small_windows_class5(gpu_ctx, cfg1, cfg1_norm, orc, ws):
+    """Windows of 11^2 to 15^2 px: the 5-pixel-slot class in its non-pipelined form, 6-DOF
+    and 2-DOF against the oracle."""
+    gpu_ctx.set_images(cfg1.f, cfg1.g)
+    f, g, gx, gy = cfg1_norm
+    o = orc.search(f, g, [1, 256, 1, 256], ws, 10, [8, 8])
+    xy = np.stack([o["x"].ravel(), o["y"].ravel()], 1)
+    S = len(xy)
+    U0 = np.stack([o["u"].ravel(), o["v"].ravel()], 1).ravel()
+    U, F, H, it, z = gpu_ctx.icgn6(xy, U0, ws)
+    Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gx, gy, xy, U0, ws)
+    _cmp_icgn(U, F, it, Uo, Fo, ito, frac=0.95)
+    Uold = np.where(np.isnan(Uo), 0, Uo)
+    Fold = np.where(np.isnan(Fo), 0, Fo)
+    vdual = np.random.default_rng(ws).normal(scale=0.01, size=2 * S)
+    U2, it2 = gpu_ctx.icgn2(xy, ws, Ho, 1e-3, vdual, Uold, Fold)
+    U2o, it2o = orc.icgn2(f, g, gx, gy, xy, ws, Ho, 1e-3, np.zeros(4 * S), vdual, Uold, Fold)
+    same = it2 == it2o
+    assert same.mean() >= 0.95
+    ok = same & (it2 >= 1) & (it2 <= 100)
+    assert np.abs(U2.reshape(-1, 2)[ok] - U2o.reshape(-1, 2)[ok]).max() <= 1e-3
+
+
+@pytest.mark.parametrize("ws", [12, 14])
+def test_masked_small_window_enlarged(gpu_ctx, cfg1, cfg1_norm, orc, ws):
+    """A class-5 subset with 10-50 % exact-zero gradients grows to round(sqrt(2) ws)
+    (funICGN.m:57-65). For ws 14 that is 20 px: the class-5 launch returns status -2 and
+    the relaunch in the 21^2-slot class finishes it. For ws 12 it is 17 px, whose half
+    width 8.5 puts the window corner off the pixel grid: the reference's indexing error,
+    status -1. Standalone (icgn6, icgn2) and inside the frame."""
+    f, g, gx, gy = cfg1_norm
+    mask = np.ones((256, 256), np.uint8)
+    mask[142:147, 140:158] = 0   # about 31-33 % of the window centred at (150, 150)
+    gpu_ctx.set_images(cfg1.f, cfg1.g, mask=mask)
+    try:
+        gxm, gym = orc.gradient(f, mask)
+        xy = np.array([[150.0, 150.0], [60.0, 200.0], [120.0, 90.0]])
+        U0 = np.tile([2.4, -1.6], 3)
+        U, F, H, it, z = gpu_ctx.icgn6(xy, U0, ws)
+        Uo, Fo, Ho, ito, zo = orc.icgn6(f, g, gxm, gym, xy, U0, ws)
+        assert list(it) == list(ito)
+        assert it[0] == (-1 if ws == 12 else it[0]) and np.all((it[1:] >= 1) & (it[1:] <= 100))
+        ok = (it >= 1) & (it <= 100)
+        assert np.abs(U.reshape(-1, 2)[ok] - Uo.reshape(-1, 2)[ok]).max() <= 1e-3
+        if ws == 14:
+            assert 1 <= it[0] <= 100
+        Uold = np.where(np.isnan(Uo), 0, Uo)
+        Fold = np.where(np.isnan(Fo), 0, Fo)
+        Hold = np.where(np.isnan(Ho), 0, Ho)
+        U2, it2 = gpu_ctx.icgn2(xy, ws, Hold, 1e-3, np.zeros(6), Uold, Fold)
+        U2o, it2o = orc.icgn2(f, g, gxm, gym, xy, ws, Hold, 1e-3, np.zeros(12), np.zeros(6),
+                              Uold, Fold)
+        assert list(it2) == list(it2o)
+        ok2 = (it2 >= 1) & (it2 <= 100)
+        assert np.abs(U2.reshape(-1, 2)[ok2] - U2o.reshape(-1, 2)[ok2]).max() <= 1e-3
+        # The frame's stop tolerance is 2e-2: at 1e-2 one node of this 21 x 21 grid stops
+        # with its update norm within 1e-4 (relative) of the tolerance, where the device's
+        # FP32 images may take the other branch. The premise is asserted below.
+        tol = 2e-2
+        o = orc.frame(cfg1.f, cfg1.g, ws, 10, 8, [1, 256, 1, 256], tol=tol, mu=1e-3,
+                      beta=0.1, n_admm=3, mask=mask)
+        _assert_tie_free(orc, (f, g, gxm, gym), o, ws, tol)
+        xyi = o["xy"].astype(int) - 1
+        h = ws // 2
+        frac = np.array([1 - mask[y - h:y + h + 1, x - h:x + h + 1].mean() for x, y in xyi])
+        assert np.any((frac > 0.1) & (frac < 0.5))   # some frame windows are enlarged
+        _flush(f"frame ws {ws} with a masked patch")
+        gpu_ctx.frame_stage(cfg1.f, cfg1.g, mask=mask)
+        gpu_ctx.frame_setup(ws, 10, 8, [1, 256, 1, 256], tol=tol, mu=1e-3, beta=0.1,
+                            n_admm=3)
+        gpu_ctx.frame_run()
+        out = gpu_ctx.frame_get()
+        assert not np.any(out["iters6"] == -2)
+        _frame_parity(out, o)
+    finally:
+        gpu_ctx.set_images(cfg1.f, cfg1.g)
 
 
 def test_icgn2_matches_oracle(gpu_ctx, cfg1, cfg1_norm, orc):
