@@ -111,6 +111,17 @@ hipError_t run_prep(aldic_ctx *c)
                        c->gx.as<float>(), c->gy.as<float>(), c->stream);
 }
 
+// Whether the search that will run fits its limits: the int8 searches on staged 8- or
+// 16-bit pixels have their own geometry limits, the FP32 search needs its window in LDS.
+bool search_fits(const aldic_ctx *c, bool raw, int ws, int sx, int sy)
+{
+    if (raw && c->raw_dtype == ALDIC_U8 &&
+        (search_i8_supported(ws, sx, sy) || search_mt_supported(ws, sx, sy, 8)))
+        return true;
+    if (raw && c->raw_dtype == ALDIC_U16 && search_mt_supported(ws, sx, sy, 16)) return true;
+    return search_lds_bytes(ws, sx, sy) <= 160 * 1024 - 256;
+}
+
 // 8- or 16-bit raw images behind f, g: the search runs on them exactly (int8 matrix cores)
 void search_raw8(aldic_ctx *c, SearchArgs &a)
 {
@@ -404,7 +415,7 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
         return fail(ctx, ALDIC_ERR_ARG, "integer_search: bad grid arguments");
     const int S = gr[2] * gr[3];
     if (S <= 0) return ALDIC_OK;
-    if (search_lds_bytes(ws, s[0], s[1]) > 160 * 1024 - 256)
+    if (!search_fits(ctx, ctx->images_from_raw, ws, s[0], s[1]))
         return fail(ctx, ALDIC_ERR_ARG, "integer_search: window too large for LDS");
     CK(hipSetDevice(ctx->device));
     DBuf *sc = ctx->scratch;
@@ -959,7 +970,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     if (o->n_admm < 0 || o->cg_maxit < 1 || !(o->cg_rtol > 0))
         return fail(ctx, ALDIC_ERR_ARG, "frame_setup: bad options");
     if (icgn_ppl_for(o->ws) == 0) return fail(ctx, ALDIC_ERR_ARG, "frame_setup: ws must be 2..64");
-    if (search_lds_bytes(o->ws, o->s[0], o->s[1]) > 160 * 1024 - 256)
+    if (!search_fits(ctx, true, o->ws, o->s[0], o->s[1]))
         return fail(ctx, ALDIC_ERR_ARG, "frame_setup: search window too large for LDS");
     CK(hipSetDevice(ctx->device));
     FrameState &F = ctx->fr;

@@ -27,6 +27,7 @@
 
 #include "aldic_internal.h"
 #include "common.h"
+#include "search_box.h"
 #include "search_peak.h"
 
 namespace aldic {
@@ -44,13 +45,6 @@ struct MTGeom {
     int tplane, wplane;  // bytes per digit plane
     int tgrp, wgrp;      // destination dwords (4 pixels) per template / window row
     int off_T, off_W, off_P, off_N, total;
-};
-
-struct BoxPlanes {
-    const int *bs1, *rs1, *cs1;
-    const long long *rs2;
-    const double *bs2, *cs2;  // exact integers (< 2^53), kept as FP64 for the lag arithmetic
-    int pitch;  // nx + 2: element (ye, xe) at (ye + 1) pitch + xe + 1, ye in [-1, ny], xe in [-1, nx]
 };
 
 int r16(int v) { return (v + 15) & ~15; }
@@ -97,11 +91,15 @@ bool mt_geom(int ws, int sx, int sy, int dig, MTGeom &G)
 
 // ---- box planes of the (centred) window image --------------------------------------
 
+// Z centres the pixels (the digit planes' offset); bits (nullable) selects zsmall instead
+// when every pixel of both images is below 4096 (the 12-bit search's raw planes).
 template <class T>
 __global__ __launch_bounds__(256) void k_box_rows(const T *g, int nx, int ny, int tw, int Z,
-                                                  int *rs1, long long *rs2, int pitch)
+                                                  const unsigned *bits, int zsmall, int *rs1,
+                                                  long long *rs2, int pitch)
 {
     __shared__ int v[256 + 128];
+    if (bits && *bits < 4096u) Z = zsmall;
     const int ye = (int)blockIdx.y - 1;
     const int xb = (int)blockIdx.x * 256 - 1;
     for (int k = threadIdx.x; k < 256 + tw - 1; k += 256) {
@@ -125,9 +123,11 @@ __global__ __launch_bounds__(256) void k_box_rows(const T *g, int nx, int ny, in
 
 template <class T>
 __global__ __launch_bounds__(256) void k_box_cols(const T *g, int nx, int ny, int tw, int Z,
+                                                  const unsigned *bits, int zsmall,
                                                   const int *rs1, const long long *rs2, int *cs1,
                                                   double *cs2, int *bs1, double *bs2, int pitch)
 {
+    if (bits && *bits < 4096u) Z = zsmall;
     const int xe = (int)(blockIdx.x * 256 + threadIdx.x) - 1;
     if (xe > nx) return;
     const int y0 = (int)blockIdx.y * 64 - 1, y1 = min(y0 + 64, ny + 1);
@@ -167,15 +167,29 @@ __global__ __launch_bounds__(256) void k_box_cols(const T *g, int nx, int ny, in
     }
 }
 
-// ---- the search kernel --------------------------------------------------------------
+// OR of every pixel of both 16-bit images into *bits (cleared by k_bits_clear first): below
+// 4096 means 12-bit data.
+__global__ void k_bits_clear(unsigned *bits) { *bits = 0u; }
 
-struct MTStats {
-    long long st, st2, stn, n;
-    double nd, std, stnd;  // n, st, stn as FP64 (exact)
-    int r0, rl, c0, cl, k00, k0l, kl0, kll;  // |.| <= 2^16 * 66
-    double d;
-    int x0, y0;  // 0-based image position of window element (0, 0) (the zero ring's corner)
-};
+__global__ __launch_bounds__(256) void k_bits_or(const uint16_t *f, const uint16_t *g, size_t n,
+                                                 unsigned *bits)
+{
+    unsigned acc = 0;
+    const size_t n8 = n / 8, stride = (size_t)gridDim.x * blockDim.x;
+    const uint4 *f4 = reinterpret_cast<const uint4 *>(f), *g4 = reinterpret_cast<const uint4 *>(g);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += stride) {
+        const uint4 a = f4[i], b = g4[i];
+        acc |= a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w;
+    }
+    acc = (acc & 0xffffu) | (acc >> 16);  // each dword held two pixels
+    if (blockIdx.x == 0)
+        for (size_t i = 8 * n8 + threadIdx.x; i < n; i += blockDim.x) acc |= (unsigned)f[i] | g[i];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc |= __shfl_xor(acc, m, kWave);
+    if ((threadIdx.x & 63) == 0 && acc) atomicOr(bits, acc);
+}
+
+// ---- the search kernel --------------------------------------------------------------
 
 // Pixel values of a row segment as centred digit planes: 4 pixels from pixel index pix
 // (pixels with k outside [k0, k1) become 0).  DIG 1: one byte plane (v - 128); DIG 2:
@@ -269,66 +283,6 @@ __device__ __forceinline__ void tile_gemm(const uint8_t *Tl, const uint8_t *Wl, 
     }
 }
 
-// num and den2 (NCC = num / sqrt(den2)) of lag (lx, ly); P its correlation sum, sw / sw2
-// the window box sums of the lag's full footprint (box-sum planes at (ly, lx)).
-template <bool WIDE, class T>
-__device__ __forceinline__ void mt_terms_sw(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
-                                            const T *g, int nx, int ny, int Z, int lx, int ly,
-                                            double P, double sw, double sw2, double &num,
-                                            double &den2)
-{
-    // P, SW, SW2 and the statistics are integers below 2^53; so are n P, ST SW, n SW2 and
-    // SW^2 unless WIDE (16-bit pixels and n^2 32896^2 >= 2^53, templates above 53 x 53),
-    // so FP64 products of them are exact; WIDE forms those in int64
-    const bool top = ly == 0, bot = ly == G.nry - 1, lft = lx == 0, rgt = lx == G.nrx - 1;
-    const int ye = s.y0 + ly, xe = s.x0 + lx;
-    const int yr = top ? s.y0 : s.y0 + G.Dy + 1, xr = lft ? s.x0 : s.x0 + G.Dx + 1;
-    if (top || bot) {
-        const size_t orow = (size_t)(yr + 1) * bp.pitch + (xe + 1);
-        sw -= (double)bp.rs1[orow];
-        sw2 -= (double)bp.rs2[orow];
-    }
-    if (lft || rgt) {
-        const size_t ocol = (size_t)(ye + 1) * bp.pitch + (xr + 1);
-        sw -= (double)bp.cs1[ocol];
-        sw2 -= bp.cs2[ocol];
-    }
-    const int ro = (top || bot) ? 1 : 0, co = (lft || rgt) ? 1 : 0;
-    if (ro && co) {  // the corner pixel was taken off twice
-        const double cv =
-            (yr >= 0 && yr < ny && xr >= 0 && xr < nx) ? (double)((int)g[(size_t)yr * nx + xr] - Z) : 0.0;
-        sw += cv;
-        sw2 += cv * cv;
-    }
-    const long long rs = top ? s.r0 : (bot ? s.rl : 0);
-    const long long cs = lft ? s.c0 : (rgt ? s.cl : 0);
-    const long long kc = (ro && co) ? (top ? (lft ? s.k00 : s.k0l) : (lft ? s.kl0 : s.kll)) : 0;
-    const long long stin = s.st - rs - cs + kc;
-    const long long nin = (long long)(G.tw - ro) * (G.tw - co), nout = s.n - nin;
-    double t1, t2;
-    if constexpr (WIDE) {
-        const long long Pi = (long long)P, swi = (long long)sw, sw2i = (long long)sw2;
-        t1 = (double)(s.n * Pi - s.st * swi);
-        t2 = (double)(s.n * sw2i - swi * swi);
-    } else {
-        const double a1 = s.nd * P, a2 = s.std * sw, b1 = s.nd * sw2, b2 = sw * sw;
-        t1 = a1 - a2;
-        t2 = b1 - b2;
-    }
-    num = t1 + s.d * (double)(s.n * stin - nin * s.st);
-    const double var = t2 + 2.0 * s.d * sw * (double)nout + s.d * s.d * (double)nin * (double)nout;
-    den2 = s.stnd * (var > 0.0 ? var : 0.0);
-}
-
-template <bool WIDE, class T>
-__device__ __forceinline__ void mt_terms(const MTStats &s, const MTGeom &G, const BoxPlanes &bp,
-                                         const T *g, int nx, int ny, int Z, int lx, int ly,
-                                         double P, double &num, double &den2)
-{
-    const size_t o = (size_t)(s.y0 + ly + 1) * bp.pitch + (s.x0 + lx + 1);
-    mt_terms_sw<WIDE>(s, G, bp, g, nx, ny, Z, lx, ly, P, (double)bp.bs1[o], bp.bs2[o], num, den2);
-}
-
 // Division-free argmax over one tile's lags straight from the accumulators: lane l's
 // register r is lag (ox + 8 (r / 4) + 4 (l / 32) + r % 4, oy + l % 32).  The lane's 16
 // box sums are loaded before any is used so their latencies overlap.
@@ -366,10 +320,13 @@ __device__ __forceinline__ void tile_argmax(const MTStats &s, const MTGeom &G, c
     }
 }
 
+// skip (nullable): the 12-bit search handles the frame when *skip < 4096.
 template <int DIG, int NK, bool WIDE, class T>
 __global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, BoxPlanes bp,
-                                                  const T *fimg, const T *gimg)
+                                                  const T *fimg, const T *gimg,
+                                                  const unsigned *skip)
 {
+    if (skip && *skip < 4096u) return;
     extern __shared__ __align__(16) uint8_t lds8[];
     const int lane = threadIdx.x;
     uint8_t *Tl = lds8 + G.off_T;
@@ -536,24 +493,28 @@ __global__ __launch_bounds__(64, 2) void k_search_mt(SearchArgs a, MTGeom G, Box
     }
 }
 
+// Work buffer layout: rs2, cs2, bs2 (8-byte), rs1, cs1, bs1 (4-byte), 256 spare bytes (a
+// lane may read a few elements past the last row), then the bits word.
+size_t planes_bytes(size_t np) { return np * (3 * sizeof(int) + 3 * sizeof(long long)) + 256; }
+
 template <class T>
-hipError_t box_planes(const T *g, int nx, int ny, int tw, int Z, DBuf &buf, BoxPlanes &bp,
-                      hipStream_t st)
+hipError_t box_planes(const T *g, int nx, int ny, int tw, int Z, const unsigned *bits, int zsmall,
+                      DBuf &buf, BoxPlanes &bp, hipStream_t st)
 {
     const int pitch = nx + 2;
     const size_t np = (size_t)pitch * (ny + 2);
     hipError_t e;
-    if ((e = buf.ensure(np * (3 * sizeof(int) + 3 * sizeof(long long)))) != hipSuccess) return e;
+    if ((e = buf.ensure(planes_bytes(np) + 256)) != hipSuccess) return e;
     long long *l = buf.as<long long>();
     long long *rs2 = l;
     double *cs2 = reinterpret_cast<double *>(l + np), *bs2 = reinterpret_cast<double *>(l + 2 * np);
     int *i = reinterpret_cast<int *>(l + 3 * np);
     int *rs1 = i, *cs1 = i + np, *bs1 = i + 2 * np;
     hipLaunchKernelGGL(k_box_rows<T>, dim3((pitch + 255) / 256, ny + 2), dim3(256), 0, st, g, nx,
-                       ny, tw, Z, rs1, rs2, pitch);
+                       ny, tw, Z, bits, zsmall, rs1, rs2, pitch);
     hipLaunchKernelGGL(k_box_cols<T>, dim3((pitch + 255) / 256, (ny + 2 + 63) / 64), dim3(256), 0,
-                       st, g, nx, ny, tw, Z, (const int *)rs1, (const long long *)rs2, cs1, cs2,
-                       bs1, bs2, pitch);
+                       st, g, nx, ny, tw, Z, bits, zsmall, (const int *)rs1,
+                       (const long long *)rs2, cs1, cs2, bs1, bs2, pitch);
     bp.bs1 = bs1;
     bp.rs1 = rs1;
     bp.cs1 = cs1;
@@ -566,7 +527,7 @@ hipError_t box_planes(const T *g, int nx, int ny, int tw, int Z, DBuf &buf, BoxP
 
 template <int DIG, int NK, bool WIDE, class T>
 hipError_t launch_mt(const SearchArgs &a, const MTGeom &G, const BoxPlanes &bp, const T *f,
-                     const T *g, int S, hipStream_t st)
+                     const T *g, int S, const unsigned *skip, hipStream_t st)
 {
     auto kern = k_search_mt<DIG, NK, WIDE, T>;
     static bool attr_set = false;
@@ -577,23 +538,23 @@ hipError_t launch_mt(const SearchArgs &a, const MTGeom &G, const BoxPlanes &bp, 
     }
     const int per_cu = std::max(1, std::min(16, (160 * 1024) / G.total));
     const int nwg = std::min(S, 256 * per_cu);
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), (size_t)G.total, st, a, G, bp, f, g);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(64), (size_t)G.total, st, a, G, bp, f, g, skip);
     return hipGetLastError();
 }
 
 template <int DIG, class T>
 hipError_t launch_dig(const SearchArgs &a, const MTGeom &G, const BoxPlanes &bp, const T *f,
-                      const T *g, int S, hipStream_t st)
+                      const T *g, int S, const unsigned *skip, hipStream_t st)
 {
     // 16-bit templates from 54 x 54 up: n P and n SW2 may pass 2^53, formed in int64
     const double n = (double)G.tw * G.tw;
     const bool wide = DIG == 2 && n * n * 32896.0 * 32896.0 >= 9007199254740992.0;
     switch (G.nk) {
-    case 1: return launch_mt<DIG, 1, false>(a, G, bp, f, g, S, st);
-    case 2: return launch_mt<DIG, 2, false>(a, G, bp, f, g, S, st);
+    case 1: return launch_mt<DIG, 1, false>(a, G, bp, f, g, S, skip, st);
+    case 2: return launch_mt<DIG, 2, false>(a, G, bp, f, g, S, skip, st);
     default:
-        return wide ? launch_mt<DIG, 3, true>(a, G, bp, f, g, S, st)
-                    : launch_mt<DIG, 3, false>(a, G, bp, f, g, S, st);
+        return wide ? launch_mt<DIG, 3, true>(a, G, bp, f, g, S, skip, st)
+                    : launch_mt<DIG, 3, false>(a, G, bp, f, g, S, skip, st);
     }
 }
 
@@ -615,15 +576,29 @@ hipError_t launch_search_mt(const SearchArgs &a, hipStream_t st)
     BoxPlanes bp;
     hipError_t e;
     if (dig == 2) {
-        if ((e = box_planes(a.g16, a.nx, a.ny, G.tw, 32896, *a.planes, bp, st)) != hipSuccess)
+        // 12-bit data (every pixel of both images below 4096, decided on the device) runs
+        // the three-GEMM search on raw digits (box planes uncentred); otherwise the
+        // four-GEMM search on centred 16-bit digits
+        const bool k3 = search_k3_supported(a.ws, a.sx, a.sy, a.step);
+        const size_t np = (size_t)(a.nx + 2) * (a.ny + 2);
+        if ((e = a.planes->ensure(planes_bytes(np) + 256)) != hipSuccess) return e;
+        unsigned *bits = reinterpret_cast<unsigned *>(a.planes->as<uint8_t>() + planes_bytes(np));
+        hipLaunchKernelGGL(k_bits_clear, dim3(1), dim3(1), 0, st, bits);
+        hipLaunchKernelGGL(k_bits_or, dim3(1024), dim3(256), 0, st, a.f16, a.g16,
+                           (size_t)a.nx * a.ny, bits);
+        if ((e = box_planes(a.g16, a.nx, a.ny, G.tw, 32896, bits, k3 ? 0 : 32896, *a.planes, bp,
+                            st)) != hipSuccess)
             return e;
         if (a.ev_kernel && (e = hipEventRecord(a.ev_kernel, st)) != hipSuccess) return e;
-        return launch_dig<2>(a, G, bp, a.f16, a.g16, S, st);
+        if (k3 && (e = launch_search_k3(a, bp, bits, st)) != hipSuccess) return e;
+        return launch_dig<2>(a, G, bp, a.f16, a.g16, S, k3 ? bits : nullptr, st);
     }
     if (!a.f8 || !a.g8) return hipErrorInvalidValue;
-    if ((e = box_planes(a.g8, a.nx, a.ny, G.tw, 128, *a.planes, bp, st)) != hipSuccess) return e;
+    if ((e = box_planes(a.g8, a.nx, a.ny, G.tw, 128, nullptr, 128, *a.planes, bp, st)) !=
+        hipSuccess)
+        return e;
     if (a.ev_kernel && (e = hipEventRecord(a.ev_kernel, st)) != hipSuccess) return e;
-    return launch_dig<1>(a, G, bp, a.f8, a.g8, S, st);
+    return launch_dig<1>(a, G, bp, a.f8, a.g8, S, nullptr, st);
 }
 
 }  // namespace aldic

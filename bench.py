@@ -213,13 +213,16 @@ def main():
     ops = 2.0 * (2 * srad + 3) ** 2 * (ws + 1) ** 2 * S
     t_s = kt["search_kernel"] / 1e3
     small = pair.f.dtype == np.uint8 and srad <= 14
-    skern = "k_search_i8" if small else "k_search_mt"
     bpp = pair.f.dtype.itemsize
+    b12 = bpp == 2 and int(pair.f.max()) < 4096 and int(pair.g.max()) < 4096
+    skern = "k_search_i8" if small else ("k_search_k3" if b12 else "k_search_mt")
+    engine = {"k_search_i8": "int8 MFMA (32x32x32), one 8-bit plane",
+              "k_search_k3": "int8 MFMA (32x32x32), base-64 digit planes, 3 GEMMs (Karatsuba)",
+              "k_search_mt": "int8 MFMA (32x32x32)" + ("" if bpp == 1 else
+                                                        ", two digit planes (4 GEMMs)")}[skern]
     search_bytes = npx * 2 * bpp + 32 * S + (0 if small else npx * 36)
     vs = measured(skern, "valu_instr_per_launch")
-    search_roof = dict(kernel=skern, bound="mfma",
-                       engine="int8 MFMA (32x32x32)" + ("" if bpp == 1 else
-                                                        ", two digit planes (4 GEMMs)"),
+    search_roof = dict(kernel=skern, bound="mfma", engine=engine,
                        achieved=round(ops / t_s / 1e12, 3), peak=INT8_PEAK_TOPS,
                        unit="TOP/s", frac=round(ops / t_s / 1e12 / INT8_PEAK_TOPS, 5),
                        traffic=measured(skern), algorithmic_ops=ops,

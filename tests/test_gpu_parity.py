@@ -156,17 +156,40 @@ def test_search_8bit_cases(gpu_ctx, speckle, orc, case):
 
 
 @pytest.mark.parametrize("case", ["u12_ws20_s8", "u12_ws32_s24", "u16_full_range_ws16_s12",
-                                  "u12_flat_and_image_edges", "u16_full_range_ws56_s4"])
+                                  "u12_flat_and_image_edges", "u16_full_range_ws56_s4",
+                                  "u12_ws32_s24_step8", "u12_ws16_s6_step40", "u12_ws56_s4",
+                                  "u12_ws20_s40_three_tiles", "u12_peaks_on_first_lag_row",
+                                  "u12_peaks_on_last_lag_row", "u12_ws12_s3x10"])
 def test_search_16bit_cases(gpu_ctx, speckle, orc, case):
-    """16-bit images: two int8 digit planes per image, four int8 GEMMs per lag tile, the
-    window box sums from int64 image planes.  12-bit speckle (the 4096^2 workload's
-    format) at radius 8 and 24 (51 x 51 lags, 2 x 2 tiles), full 16-bit range, and
-    constant templates with subsets against the image border; a 57 x 57 template of full-range
-    16-bit pixels, whose n P and n SW2 pass 2^53 (formed in int64 there)."""
+    """16-bit images.  Full 16-bit range: two centred int8 digit planes per image, four
+    int8 GEMMs per lag tile, window box sums from int64 image planes (a 57 x 57 template
+    of full-range pixels, whose n P and n SW2 pass 2^53, formed in int64 there).  12-bit
+    speckle (the 4096^2 workload's format): base-64 digits, three GEMMs per tile, four
+    subsets stacked in y sharing one window per workgroup: radius 8 and 24 (51 x 51 lags,
+    2 x 2 tiles) at step 8 (the 4096^2 workload's geometry), 12 and 16; windows that do not
+    overlap (step 40 > 31 window rows); three K steps per template row (57 x 57); three
+    x tiles (83 lags); unequal radii; integer peaks on the first and last lag rows, whose
+    footprints reach the zero ring rows that the stacked windows share with their
+    neighbours; constant templates with subsets against the image border; grids whose row
+    count is not a multiple of four."""
     _flush(f"search 16-bit case {case}")
     n, ws, sx, sy, step, u0, v0, bits = 208, 20, 8, 8, 12, 5.3, -6.6, 12
     if case == "u12_ws32_s24":
         n, ws, sx, sy, step, u0, v0 = 240, 32, 24, 24, 16, 17.4, -9.2
+    elif case == "u12_ws32_s24_step8":
+        n, ws, sx, sy, step, u0, v0 = 232, 32, 24, 24, 8, -15.3, 19.6
+    elif case == "u12_ws16_s6_step40":
+        n, ws, sx, sy, step, u0, v0 = 256, 16, 6, 6, 40, 2.4, -3.7
+    elif case == "u12_ws56_s4":
+        n, ws, sx, sy, step, u0, v0 = 224, 56, 4, 4, 16, 2.6, -1.3
+    elif case == "u12_ws20_s40_three_tiles":
+        n, ws, sx, sy, step, u0, v0 = 240, 20, 40, 40, 14, -27.2, 31.6
+    elif case == "u12_peaks_on_first_lag_row":
+        n, ws, sx, sy, step, u0, v0 = 208, 20, 8, 8, 10, 3.1, -8.7
+    elif case == "u12_peaks_on_last_lag_row":
+        n, ws, sx, sy, step, u0, v0 = 208, 20, 8, 8, 10, -2.2, 8.8
+    elif case == "u12_ws12_s3x10":
+        n, ws, sx, sy, step, u0, v0 = 200, 12, 3, 10, 9, 1.8, 7.4
     elif case == "u16_full_range_ws16_s12":
         ws, sx, sy, u0, v0, bits = 16, 12, 12, -10.6, 3.1, 16
     elif case == "u12_flat_and_image_edges":
@@ -189,6 +212,11 @@ def test_search_16bit_cases(gpu_ctx, speckle, orc, case):
         assert r["x1"] - ws // 2 <= sx + 2 and r["y1"] - ws // 2 <= sy + 2
     _search_exact(r, o)
     assert np.mean(r["status"] == 0) > 0.5
+    if case.startswith("u12_peaks_on"):
+        # integer y peaks at lag -s - 1 or s + 1 (the first or last lag row, whose
+        # footprint takes the top or bottom zero ring row of the window)
+        edge = -(sy + 1) if v0 < 0 else sy + 1
+        assert np.mean(np.round(o["v"][o["status"] == 0]) == edge) > 0.2
 
 
 def _qf_norm(q):
