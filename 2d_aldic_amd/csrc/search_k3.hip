@@ -85,26 +85,38 @@ bool k3_geom(int ws, int sx, int sy, int step, K3Geom &G)
         G.MW = mw;
         G.SH = G.DH + (mw - 1) * step;
         G.wplane = (G.SH + 1) * G.WS;
-        G.off_T = r16(2 * G.wplane);
+        G.off_T = r16(3 * G.wplane);  // window digit planes h, l and s = h + l
         G.total = G.off_T + mw * G.per_wave;
         if (G.total <= 80 * 1024) return true;  // two workgroups per CU
     }
     return false;
 }
 
-// Four 16-bit pixels from pixel index pix (pixels outside [k0, k1) -> 0) as base-64
-// digit bytes: h4 = v >> 6, l4 = v & 63 (pixels below 4096).
-__device__ __forceinline__ void load4_d64(const uint16_t *img, long long pix, int k0, int k1,
-                                          unsigned &h4, unsigned &l4)
+// Four 16-bit pixels from pixel index pix: the three dwords around them and the byte
+// shift (raw4_load), then (raw4_digits) the pixels outside [k0, k1) -> 0 and base-64
+// digit bytes h4 = v >> 6, l4 = v & 63 (pixels below 4096).  Split so that a thread can
+// have several rows' loads in flight before it converts any.
+struct Raw4 {
+    unsigned d0, d1, d2, s;
+};
+
+__device__ __forceinline__ Raw4 raw4_load(const uint16_t *img, long long pix)
 {
     const long long sa = 2 * pix, lo = sa & ~3LL;
     const uint8_t *b = reinterpret_cast<const uint8_t *>(img);
-    const unsigned d0 = *reinterpret_cast<const unsigned *>(b + (lo < 0 ? 0 : lo));
-    const unsigned d1 = *reinterpret_cast<const unsigned *>(b + lo + 4);
-    const unsigned d2 = *reinterpret_cast<const unsigned *>(b + lo + 8);
-    const unsigned s = (unsigned)(sa & 3);
-    unsigned w0 = __builtin_amdgcn_alignbyte(d1, d0, s);  // pixels 0 | 1
-    unsigned w1 = __builtin_amdgcn_alignbyte(d2, d1, s);  // pixels 2 | 3
+    Raw4 r;
+    r.d0 = *reinterpret_cast<const unsigned *>(b + (lo < 0 ? 0 : lo));
+    r.d1 = *reinterpret_cast<const unsigned *>(b + lo + 4);
+    r.d2 = *reinterpret_cast<const unsigned *>(b + lo + 8);
+    r.s = (unsigned)(sa & 3);
+    return r;
+}
+
+__device__ __forceinline__ void raw4_digits(const Raw4 &r, int k0, int k1, unsigned &h4,
+                                            unsigned &l4)
+{
+    unsigned w0 = __builtin_amdgcn_alignbyte(r.d1, r.d0, r.s);  // pixels 0 | 1
+    unsigned w1 = __builtin_amdgcn_alignbyte(r.d2, r.d1, r.s);  // pixels 2 | 3
     w0 &= ((k0 <= 0 && k1 >= 1) ? 0xffffu : 0u) | ((k0 <= 1 && k1 >= 2) ? 0xffff0000u : 0u);
     w1 &= ((k0 <= 2 && k1 >= 3) ? 0xffffu : 0u) | ((k0 <= 3 && k1 >= 4) ? 0xffff0000u : 0u);
     const unsigned l0 = w0 & 0x003f003fu, l1 = w1 & 0x003f003fu;
@@ -112,6 +124,8 @@ __device__ __forceinline__ void load4_d64(const uint16_t *img, long long pix, in
     l4 = __builtin_amdgcn_perm(l1, l0, 0x06040200u);
     h4 = __builtin_amdgcn_perm(h1, h0, 0x06040200u);
 }
+
+constexpr int kLB = 1;  // rows whose loads a thread has in flight at once
 
 __device__ __forceinline__ unsigned udot4(unsigned a, unsigned b, unsigned c)
 {
@@ -143,11 +157,11 @@ __device__ __forceinline__ void a_operand(const uint8_t *tr, int s3, v4i (&A)[NK
 }
 
 // One template row j of a pass: NT x tiles (ox, ox + 32), NK K steps, three digit GEMMs.
-// bh / bl: this lane's window row in the two planes (column ox + 16 h).
+// b: this lane's window row in the h plane (column ox + 16 h); the l and s planes follow
+// at wp and 2 wp bytes.
 template <int NK, int NT>
 __device__ __forceinline__ void pass_row(const uint8_t *th, const uint8_t *tl, int s3,
-                                         const uint8_t *bh, const uint8_t *bl,
-                                         v16i (&acc)[3][NT])
+                                         const uint8_t *b, int wp, v16i (&acc)[3][NT])
 {
     v4i Ah[NK], Al[NK], As[NK];
     a_operand<NK>(th, s3, Ah);
@@ -158,9 +172,9 @@ __device__ __forceinline__ void pass_row(const uint8_t *th, const uint8_t *tl, i
     v4i Bh[NC], Bl[NC], Bs[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        Bh[c] = *reinterpret_cast<const v4i *>(bh + 32 * c);
-        Bl[c] = *reinterpret_cast<const v4i *>(bl + 32 * c);
-        Bs[c] = Bh[c] + Bl[c];
+        Bh[c] = *reinterpret_cast<const v4i *>(b + 32 * c);
+        Bl[c] = *reinterpret_cast<const v4i *>(b + wp + 32 * c);
+        Bs[c] = *reinterpret_cast<const v4i *>(b + 2 * wp + 32 * c);
     }
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt)
@@ -193,11 +207,69 @@ __device__ __forceinline__ int edge_slot(const K3Geom &G, int lx, int ly)
     return -1;
 }
 
+// P of accumulator slot r of tile tt: 4096 P_hh + 64 (P_ss - P_hh - P_ll) + P_ll.
+template <int NT>
+__device__ __forceinline__ double slot_p(const v16i (&acc)[3][NT], int tt, int r)
+{
+    const int q = 64 * acc[2][tt][r] - 63 * acc[1][tt][r];  // |q| < 2^31
+    return fma(4032.0, (double)acc[0][tt][r], (double)q);
+}
+
+// The argmax over one pass's interior lags, straight from the accumulators.  Lane
+// (lq, h) holds lag (ox + 32 tt + 8 rg + 4 h + e, ly) in register 4 rg + e of tile tt;
+// its slots come in increasing lag index, so a tie keeps the earlier lag.  The window
+// box sums are fetched one 4-lag group ahead.  EDGE: the pass holds lag row 0 or nry - 1,
+// whose lanes store every P to the edge list.
+template <int NT, bool EDGE>
+__device__ __forceinline__ void pass_argmax(const K3Geom &G, const v16i (&acc)[3][NT], int ox,
+                                            int ly, int h, bool rowin, size_t rowo,
+                                            const MTStats &s, const BoxPlanes &bp, double *E,
+                                            Best &b)
+{
+    const bool rowedge = ly == 0 || ly == G.nry - 1;
+    const int ebase = ly == 0 ? 0 : G.nrx;
+    const int lo = 1 - 4 * h, hi = G.nrx - 2 - 4 * h;  // valid lxb + e for this lane
+    constexpr int NG = 4 * NT;
+    v4i_u w1 = *reinterpret_cast<const v4i_u *>(bp.bs1 + rowo + ox + 4 * h);
+    v4d_u w2 = *reinterpret_cast<const v4d_u *>(bp.bs2 + rowo + ox + 4 * h);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int tt = g >> 2, rg = g & 3;
+        const int lxb = ox + 32 * tt + 8 * rg;
+        if (lxb >= G.nrx) break;
+        v4i_u n1 = w1;
+        v4d_u n2 = w2;
+        if (g + 1 < NG && lxb + 8 < G.nrx) {
+            n1 = *reinterpret_cast<const v4i_u *>(bp.bs1 + rowo + lxb + 8 + 4 * h);
+            n2 = *reinterpret_cast<const v4d_u *>(bp.bs2 + rowo + lxb + 8 + 4 * h);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = 4 * rg + e;
+            const double P = slot_p<NT>(acc, tt, r);
+            if (EDGE && rowedge && lxb + e + 4 * h < G.nrx) E[ebase + lxb + e + 4 * h] = P;
+            const double sw = (double)w1[e];
+            const double num = fma(s.nd, P, -(s.std * sw));
+            const double var = fma(s.nd, w2[e], -(sw * sw));
+            const double key = num * fabs(num), den = fmax(var, 0.5);
+            const double l = key * b.d, rr = b.k * den;
+            if (rowin && lxb + e >= lo && lxb + e <= hi && l > rr) {
+                b.k = key;
+                b.d = den;
+                b.i = ly * G.nrx + lxb + e + 4 * h;
+            }
+        }
+        w1 = n1;
+        w2 = n2;
+        __builtin_amdgcn_sched_barrier(0);  // one group of box-sum loads in flight at a time
+    }
+}
+
 template <int NK, int NT>
 __device__ __forceinline__ void run_pass(const K3Geom &G, const uint8_t *Th, const uint8_t *Tl,
-                                         const uint8_t *Wh, const uint8_t *Wl, int mrow, int ox,
-                                         int oy, int lq, int h, const MTStats &s,
-                                         const BoxPlanes &bp, double *E, Best &b)
+                                         const uint8_t *Wh, int mrow, int ox, int oy, int lq,
+                                         int h, const MTStats &s, const BoxPlanes &bp, double *E,
+                                         Best &b)
 {
     v16i acc[3][NT];
 #pragma unroll
@@ -209,52 +281,38 @@ __device__ __forceinline__ void run_pass(const K3Geom &G, const uint8_t *Th, con
     const int ly = oy + lq, lyc = min(ly, G.nry - 1);
     const int o = kPL + 16 * h - lq, b0 = o & ~3, s3 = o & 3;
     const uint8_t *th = Th + b0, *tl = Tl + b0;
-    const int boff = ox + 16 * h;
-    const uint8_t *bh = Wh + (mrow + lyc) * G.WS + boff, *bl = Wl + (mrow + lyc) * G.WS + boff;
-    const uint8_t *zh = Wh + G.SH * G.WS + boff, *zl = Wl + G.SH * G.WS + boff;
+    const int wp = G.wplane;
+    const uint8_t *bw = Wh + (mrow + lyc) * G.WS + ox + 16 * h;
+    const uint8_t *zw = Wh + G.SH * G.WS + ox + 16 * h;
     const int tw = G.tw;
     // j = 0 and j = tw - 1 reach the window's zero ring for lag rows 0 and nry - 1
-    pass_row<NK, NT>(th, tl, s3, lyc == 0 ? zh : bh, lyc == 0 ? zl : bl, acc);
+    pass_row<NK, NT>(th, tl, s3, lyc == 0 ? zw : bw, wp, acc);
     for (int j = 1; j < tw - 1; ++j)
-        pass_row<NK, NT>(th + j * G.TS, tl + j * G.TS, s3, bh + j * G.WS, bl + j * G.WS, acc);
+        pass_row<NK, NT>(th + j * G.TS, tl + j * G.TS, s3, bw + j * G.WS, wp, acc);
     {
         const int j = tw - 1;
-        const bool z = lyc == G.nry - 1;
-        pass_row<NK, NT>(th + j * G.TS, tl + j * G.TS, s3, z ? zh : bh + j * G.WS,
-                         z ? zl : bl + j * G.WS, acc);
+        pass_row<NK, NT>(th + j * G.TS, tl + j * G.TS, s3,
+                         lyc == G.nry - 1 ? zw : bw + j * G.WS, wp, acc);
     }
-    // argmax over the interior lags; edge lags' P to E
-    const bool rowin = ly >= 1 && ly <= G.nry - 2, rowedge = ly == 0 || ly == G.nry - 1;
+    const bool rowin = ly >= 1 && ly <= G.nry - 2;
     const size_t rowo = (size_t)(s.y0 + lyc + 1) * bp.pitch + (s.x0 + 1);
+    if (oy == 0 || oy + 31 >= G.nry - 1)
+        pass_argmax<NT, true>(G, acc, ox, ly, h, rowin, rowo, s, bp, E, b);
+    else
+        pass_argmax<NT, false>(G, acc, ox, ly, h, rowin, rowo, s, bp, E, b);
+    // the interior rows' edge columns: lag 0 (register 0 of tile 0, h = 0) and lag
+    // nrx - 1 (register r1 of tile t1, h = h1)
+    if (ox == 0 && rowin && h == 0) E[2 * G.nrx + ly - 1] = slot_p<NT>(acc, 0, 0);
+    const int t1 = ((G.nrx - 1) >> 5) - (ox >> 5);
+    if (t1 >= 0 && t1 < NT) {
+        const int rem = (G.nrx - 1) & 31, r1 = 4 * (rem >> 3) + (rem & 3), h1 = (rem >> 2) & 1;
 #pragma unroll
-    for (int tt = 0; tt < NT; ++tt)
+        for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-        for (int rg = 0; rg < 4; ++rg) {
-            const int lxb = ox + 32 * tt + 8 * rg;
-            if (lxb >= G.nrx) break;
-            const int lx0 = lxb + 4 * h;
-            const v4i_u w1 = *reinterpret_cast<const v4i_u *>(bp.bs1 + rowo + lx0);
-            const v4d_u w2 = *reinterpret_cast<const v4d_u *>(bp.bs2 + rowo + lx0);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = 4 * rg + e, lx = lx0 + e;
-                const int q = 64 * acc[2][tt][r] - 63 * acc[1][tt][r];  // |q| < 2^31
-                const double P = fma(4032.0, (double)acc[0][tt][r], (double)q);
-                if ((rowedge || lx == 0 || lx == G.nrx - 1) && lx < G.nrx && ly < G.nry)
-                    E[edge_slot(G, lx, ly)] = P;
-                const double sw = (double)w1[e];
-                const double num = fma(s.nd, P, -(s.std * sw));
-                const double var = fma(s.nd, w2[e], -(sw * sw));
-                const double key = num * fabs(num), den = fmax(var, 0.5);
-                const double l = key * b.d, rr = b.k * den;
-                if (rowin && lx >= 1 && lx <= G.nrx - 2 && l > rr) {
-                    b.k = key;
-                    b.d = den;
-                    b.i = ly * G.nrx + lx;
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);  // keep the box-sum loads group by group
-        }
+            for (int r = 0; r < 16; ++r)
+                if (tt == t1 && r == r1 && rowin && h == h1)
+                    E[2 * G.nrx + G.nry - 2 + ly - 1] = slot_p<NT>(acc, tt, r);
+    }
 }
 
 template <int NK>
@@ -267,7 +325,7 @@ __global__ __launch_bounds__(256, 2) void k_search_k3(SearchArgs a, K3Geom G, Bo
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     for (int k = tid; k < G.total / 4; k += blockDim.x) reinterpret_cast<unsigned *>(lds8)[k] = 0u;
     __syncthreads();
-    uint8_t *Wh = lds8, *Wl = lds8 + G.wplane;
+    uint8_t *Wh = lds8, *Wl = lds8 + G.wplane, *Ws = lds8 + 2 * G.wplane;
     uint8_t *Tb = lds8 + G.off_T + wv * G.per_wave;
     uint8_t *Th = Tb, *Tl = Tb + G.tplane;
     double *E = reinterpret_cast<double *>(Tb + G.off_E);
@@ -276,6 +334,13 @@ __global__ __launch_bounds__(256, 2) void k_search_k3(SearchArgs a, K3Geom G, Bo
 
     const int tw = G.tw, nx = a.nx, ny = a.ny;
     const int lq = lane & 31, h = lane >> 5;
+    // load layouts without per-item division: thread -> (row, 4-pixel group)
+    const int wg4 = tid % G.wgrp, wr0 = tid / G.wgrp, wrs = (int)blockDim.x / G.wgrp;
+    const bool wact = wr0 < wrs;
+    const int wk0 = wg4 == 0 ? 1 : 0, wk1 = min(4, G.Dx + 1 - 4 * wg4);
+    const int tg4 = lane % G.tgrp, tj0 = lane / G.tgrp, tjs = 64 / G.tgrp;
+    const bool tact = tj0 < tjs;
+    const int tk1 = min(4, tw - 4 * tg4);
     MTStats s;
     s.n = (long long)tw * tw;
     s.nd = (double)s.n;
@@ -306,34 +371,57 @@ __global__ __launch_bounds__(256, 2) void k_search_k3(SearchArgs a, K3Geom G, Bo
         // shared window rows 1 .. SHv - 2 (shared row R = image row y00 + R), columns
         // 1 .. Dx; the ring columns and the zero row stay 0
         const int SHv = G.DH + (mv - 1) * a.step;
-        for (int it = tid; it < (SHv - 2) * G.wgrp; it += blockDim.x) {
-            const int R = 1 + it / G.wgrp, g4 = it - (R - 1) * G.wgrp;
-            unsigned h4, l4;
-            load4_d64(gimg, (long long)(y00 + R) * nx + s.x0 + 4 * g4, g4 == 0 ? 1 : 0,
-                      min(4, G.Dx + 1 - 4 * g4), h4, l4);
-            *reinterpret_cast<unsigned *>(Wh + R * G.WS + 4 * g4) = h4;
-            *reinterpret_cast<unsigned *>(Wl + R * G.WS + 4 * g4) = l4;
+        if (wact) {
+            const long long dpix = (long long)wrs * nx;
+            long long pix = (long long)(y00 + 1 + wr0) * nx + s.x0 + 4 * wg4;
+            for (int R = 1 + wr0; R < SHv - 1; R += kLB * wrs, pix += kLB * dpix) {
+                Raw4 raw[kLB];
+#pragma unroll
+                for (int k = 0; k < kLB; ++k)
+                    if (R + k * wrs < SHv - 1) raw[k] = raw4_load(gimg, pix + k * dpix);
+#pragma unroll
+                for (int k = 0; k < kLB; ++k)
+                    if (R + k * wrs < SHv - 1) {
+                        unsigned h4, l4;
+                        raw4_digits(raw[k], wk0, wk1, h4, l4);
+                        const int o = (R + k * wrs) * G.WS + 4 * wg4;
+                        *reinterpret_cast<unsigned *>(Wh + o) = h4;
+                        *reinterpret_cast<unsigned *>(Wl + o) = l4;
+                        *reinterpret_cast<unsigned *>(Ws + o) = h4 + l4;  // digit sums: no carries
+                    }
+            }
         }
         const bool active = wv < mv;
         const int jj = jj0 + wv * a.step;
         s.y0 = jj - G.sy - 2;
         int sh = 0, sl = 0, shh = 0, shl = 0, sll = 0, r0 = 0, rl = 0;
-        if (active) {
-            for (int it = lane; it < tw * G.tgrp; it += 64) {
-                const int j = it / G.tgrp, g4 = it - j * G.tgrp;
-                unsigned h4, l4;
-                load4_d64(fimg, (long long)(jj - 1 + j) * nx + (ii - 1) + 4 * g4, 0,
-                          min(4, tw - 4 * g4), h4, l4);
-                *reinterpret_cast<unsigned *>(Th + kPL + j * G.TS + 4 * g4) = h4;
-                *reinterpret_cast<unsigned *>(Tl + kPL + j * G.TS + 4 * g4) = l4;
-                const int gh = (int)udot4(h4, 0x01010101u, 0u), gl = (int)udot4(l4, 0x01010101u, 0u);
-                sh += gh;
-                sl += gl;
-                shh = (int)udot4(h4, h4, (unsigned)shh);
-                shl = (int)udot4(h4, l4, (unsigned)shl);
-                sll = (int)udot4(l4, l4, (unsigned)sll);
-                if (j == 0) r0 += 64 * gh + gl;
-                if (j == tw - 1) rl += 64 * gh + gl;
+        if (active && tact) {
+            const long long dpix = (long long)tjs * nx;
+            long long pix = (long long)(jj - 1 + tj0) * nx + (ii - 1) + 4 * tg4;
+            for (int j = tj0; j < tw; j += kLB * tjs, pix += kLB * dpix) {
+                Raw4 raw[kLB];
+#pragma unroll
+                for (int k = 0; k < kLB; ++k)
+                    if (j + k * tjs < tw) raw[k] = raw4_load(fimg, pix + k * dpix);
+#pragma unroll
+                for (int k = 0; k < kLB; ++k) {
+                    const int jr = j + k * tjs;
+                    if (jr < tw) {
+                        unsigned h4, l4;
+                        raw4_digits(raw[k], 0, tk1, h4, l4);
+                        *reinterpret_cast<unsigned *>(Th + kPL + jr * G.TS + 4 * tg4) = h4;
+                        *reinterpret_cast<unsigned *>(Tl + kPL + jr * G.TS + 4 * tg4) = l4;
+                        const int gh = (int)udot4(h4, 0x01010101u, 0u);
+                        const int gl = (int)udot4(l4, 0x01010101u, 0u);
+                        sh += gh;
+                        sl += gl;
+                        shh = (int)udot4(h4, h4, (unsigned)shh);
+                        shl = (int)udot4(h4, l4, (unsigned)shl);
+                        sll = (int)udot4(l4, l4, (unsigned)sll);
+                        if (jr == 0) r0 += 64 * gh + gl;
+                        if (jr == tw - 1) rl += 64 * gh + gl;
+                    }
+                }
             }
         }
         __syncthreads();  // window and templates staged
@@ -380,11 +468,10 @@ __global__ __launch_bounds__(256, 2) void k_search_k3(SearchArgs a, K3Geom G, Bo
                 for (int ty = 0; ty < G.nty; ++ty) {
                     int tx = 0;
                     for (; tx + kNT <= G.ntx; tx += kNT)
-                        run_pass<NK, kNT>(G, Th, Tl, Wh, Wl, mrow, 32 * tx, 32 * ty, lq, h, s, bp,
-                                          E, b);
+                        run_pass<NK, kNT>(G, Th, Tl, Wh, mrow, 32 * tx, 32 * ty, lq, h, s, bp, E,
+                                          b);
                     if (tx < G.ntx)
-                        run_pass<NK, 1>(G, Th, Tl, Wh, Wl, mrow, 32 * tx, 32 * ty, lq, h, s, bp, E,
-                                        b);
+                        run_pass<NK, 1>(G, Th, Tl, Wh, mrow, 32 * tx, 32 * ty, lq, h, s, bp, E, b);
                 }
                 wave_sync_lds();  // edge P in E
                 // edge lags: the closed-form correction for the zero padding
@@ -420,19 +507,22 @@ __global__ __launch_bounds__(256, 2) void k_search_k3(SearchArgs a, K3Geom G, Bo
                             const int wb = (mrow + ly + j) * G.WS + lx, wa = wb & ~3, ws3 = wb & 3;
                             const unsigned *wh = reinterpret_cast<const unsigned *>(Wh + wa);
                             const unsigned *wl = reinterpret_cast<const unsigned *>(Wl + wa);
+                            const unsigned *wsp = reinterpret_cast<const unsigned *>(Ws + wa);
                             const unsigned *tH = reinterpret_cast<const unsigned *>(Th + kPL + j * G.TS);
                             const unsigned *tL = reinterpret_cast<const unsigned *>(Tl + kPL + j * G.TS);
-                            unsigned hp = wh[0], lp = wl[0];
+                            unsigned hp = wh[0], lp = wl[0], sp = wsp[0];
                             for (int g4 = 0; g4 < G.tgrp; ++g4) {
-                                const unsigned hn = wh[g4 + 1], ln = wl[g4 + 1];
-                                const unsigned w_h = __builtin_amdgcn_alignbyte(hn, hp, ws3);
-                                const unsigned w_l = __builtin_amdgcn_alignbyte(ln, lp, ws3);
+                                const unsigned hn = wh[g4 + 1], ln = wl[g4 + 1], sn = wsp[g4 + 1];
                                 const unsigned t_h = tH[g4], t_l = tL[g4];
-                                ph = (int)udot4(t_h, w_h, (unsigned)ph);
-                                pl = (int)udot4(t_l, w_l, (unsigned)pl);
-                                ps = (int)udot4(t_h + t_l, w_h + w_l, (unsigned)ps);
+                                ph = (int)udot4(t_h, __builtin_amdgcn_alignbyte(hn, hp, ws3),
+                                                (unsigned)ph);
+                                pl = (int)udot4(t_l, __builtin_amdgcn_alignbyte(ln, lp, ws3),
+                                                (unsigned)pl);
+                                ps = (int)udot4(t_h + t_l, __builtin_amdgcn_alignbyte(sn, sp, ws3),
+                                                (unsigned)ps);
                                 hp = hn;
                                 lp = ln;
+                                sp = sn;
                             }
                         }
                     }
