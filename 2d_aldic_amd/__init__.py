@@ -7,3 +7,4 @@ The directory name starts with a digit, so import it with
 from ._lib import AldicError, Context, load as load_library, LIB_PATH  # noqa: F401
 from .aldic import *  # noqa: F401,F403
 from . import speckle  # noqa: F401
+from .sequence import run_sequence, ComputeStrain  # noqa: F401

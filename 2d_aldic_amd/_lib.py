@@ -26,7 +26,7 @@ EXPORTS = (
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
     "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
     "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc", "aldic_natural_fill",
-    "aldic_frame_run_seeded",
+    "aldic_frame_run_seeded", "aldic_compute_strain",
 )
 
 
@@ -90,6 +90,7 @@ def load():
     L.aldic_prep_noalloc.argtypes = [P, vp, vp, ip, ip, ip, vp, vp, vp, vp, vp, vp, vp]
     L.aldic_natural_fill.argtypes = [P, ip, vp, vp, vp, vp]
     L.aldic_frame_run_seeded.argtypes = [P, vp]
+    L.aldic_compute_strain.argtypes = [P, ip, ip, dp, vp, vp, ip, ip, ip, vp, vp]
     _lib = L
     return L
 
@@ -313,6 +314,20 @@ class Context:
                                            _p(F)), "aldic_nodal_strain")
         return F
 
+    def compute_strain(self, M, N, h, U=None, F=None, method=1, rad=1, strain_type=0):
+        """ComputeStrain.m on an M x N node grid -> (FStrainFinite (4 M N), FStrainWorld
+        (the trimmed nodes, F21 and F12 negated), r).  Method 3 uses the mesh of
+        set_mesh."""
+        r = 0 if method == 0 else (rad if method == 2 else 1)
+        nw = max(M - 2 * r, 0) * max(N - 2 * r, 0)
+        Ff = np.empty(4 * M * N)
+        Fw = np.empty(4 * nw) if nw else None
+        self._ck(self.L.aldic_compute_strain(
+            self.h, int(M), int(N), float(h),
+            _p(None if U is None else _c(U, np.float64)), _p(None if F is None else _c(F, np.float64)),
+            int(method), int(rad), int(strain_type), _p(Ff), _p(Fw)), "aldic_compute_strain")
+        return Ff, (Fw if Fw is not None else np.empty(0)), r
+
     def natural_fill(self, coords, bad, U, F=None):
         """scatteredInterpolant(good nodes, 'natural') of U (2 per node) and F (4 per node)
         at the bad nodes (LocalICGN.m:141-161) -> refilled copies (U, F)."""
@@ -417,7 +432,10 @@ class Context:
 
     def frame_counters(self):
         """{gridfit iteration budget of the next frame, frames re-run with polled
-        gridfits, most gridfit iterations in the last frame}."""
-        v = np.zeros(3, np.int32)
-        self._ck(self.L.aldic_frame_counters(self.h, _p(v), 3), "aldic_frame_counters")
-        return dict(zip(("gf_budget", "gf_reruns", "gf_most"), v.tolist()))
+        gridfits, most gridfit iterations in the last frame, bad-node count of each ADMM
+        step's Subpb1 (-1: step not run)}."""
+        v = np.zeros(11, np.int32)
+        self._ck(self.L.aldic_frame_counters(self.h, _p(v), 11), "aldic_frame_counters")
+        d = dict(zip(("gf_budget", "gf_reruns", "gf_most"), v[:3].tolist()))
+        d["subpb1_bad"] = v[3:].tolist()
+        return d

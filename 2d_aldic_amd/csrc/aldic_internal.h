@@ -142,6 +142,10 @@ struct QFrameState {
     hipEvent_t ev[4] = {};
 };
 
+// ADMM steps whose Subpb1 bad-node counts a frame records (ALSub1BadPtNum holds six
+// entries, main_ALDIC.m:174; the first is LocalICGN's).
+constexpr int kMaxSteps = 8;
+
 struct FrameState {
     bool ready = false;
     aldic_frame_opts o{};
@@ -153,6 +157,7 @@ struct FrameState {
     NatGrid nat;    // the node grid for LocalICGN's bad-node refill
     DBuf seed;      // aldic_frame_run_seeded: the caller's initial guess (2S)
     DBuf nat_err;   // refills of the frame whose cell could not be formed (device int)
+    DBuf nbad;      // bad-node count of each ADMM step's Subpb1 (device int[kMaxSteps], -1: not run)
     int *nat_err_h = nullptr;  // pinned copy, read after the frame's wait
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
     double *gfh = nullptr;  // pinned: {converged, iterations, residual} per frame gridfit
@@ -276,6 +281,15 @@ hipError_t launch_search_mt(const SearchArgs &a, hipStream_t st);
 size_t qfactor_lds_bytes(int ws, int sx, int sy);
 hipError_t launch_qfactors(const SearchArgs &a, hipStream_t st);
 
+// Strain post-processing on a uniform M x N node grid (strain.hip): F from U by finite
+// differences (method 1) or finite differences plus plane fits of half width R inside
+// the R ring (method 2); then the strain type in place on all nodes and the trimmed,
+// world-oriented copy W of the nodes at least r from the border.
+hipError_t launch_strain_grad(int M, int N, double h, int method, int R, const double *U,
+                              double *F, hipStream_t st);
+hipError_t launch_strain_finish(int M, int N, int type, int r, double *F, double *W,
+                                hipStream_t st);
+
 hipError_t launch_bicubic(const float *g, int nx, int ny, const double *X, const double *Y,
                           long n, double *Z, hipStream_t st);
 
@@ -323,6 +337,9 @@ hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_
                           double *part, hipStream_t st);
 // The bad-point rule alone (rule 0: status < 0 or > 99; 1: also > mean + 0.15 std of the
 // good ones; 2: also > max(mean + 0.15 std, 20)); bad U (and F) become NaN.
+// Count of statuses < 0 or > 99 into *out; p[0 .. n) = v (device ints).
+hipError_t launch_count_bad(const int *iters, int S, int *out, hipStream_t st);
+hipError_t launch_iset(int *p, int n, int v, hipStream_t st);
 hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
                           const double *Fsrc, double *U, double *F, int *bad, double *part,
                           hipStream_t st);

@@ -865,6 +865,46 @@ int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
     return ALDIC_OK;
 }
 
+int aldic_compute_strain(aldic_ctx *ctx, int M, int N, double h, const double *U,
+                         const double *F, int method, int rad, int strain_type, double *Ffinite,
+                         double *Fworld)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (M < 1 || N < 1 || !(h > 0) || method < 0 || method > 3 || strain_type < 0 ||
+        strain_type > 2 || !Ffinite)
+        return fail(ctx, ALDIC_ERR_ARG, "compute_strain: bad arguments");
+    if ((method == 0 && !F) || (method != 0 && !U))
+        return fail(ctx, ALDIC_ERR_ARG, "compute_strain: NULL input");
+    // the trimming ring: 0 for the solved F, 1 for finite differences and the FE strains,
+    // the plane-fit half width for plane fits (ComputeStrain.m:17,32,45,82)
+    const int r = method == 0 ? 0 : (method == 2 ? rad : 1);
+    if (method == 2 && (rad < 1 || 2 * rad + 1 > M || 2 * rad + 1 > N))
+        return fail(ctx, ALDIC_ERR_ARG, "compute_strain: plane-fit half width must be 1..(min(M,N)-1)/2");
+    const size_t nn = (size_t)M * N;
+    if (method == 3 && (!ctx->have_mesh || ctx->mesh.nn != (int)nn))
+        return fail(ctx, ALDIC_ERR_STATE, "compute_strain: method 3 needs the M x N mesh (aldic_set_mesh)");
+    CK(hipSetDevice(ctx->device));
+    DBuf *sc = ctx->scratch;
+    CK(sc[1].ensure(sizeof(double) * 4 * nn));
+    double *Fd = sc[1].as<double>();
+    if (method == 0) {
+        CK(up(sc[1], F, 4 * nn, ctx->stream));
+    } else {
+        CK(up(sc[0], U, 2 * nn, ctx->stream));
+        if (method == 3)
+            CK(fem_strain(ctx->mesh, sc[0].as<double>(), 0, Fd, ctx->stream));
+        else
+            CK(launch_strain_grad(M, N, h, method, rad, sc[0].as<double>(), Fd, ctx->stream));
+    }
+    const long nw = (M - 2L * r) > 0 && (N - 2L * r) > 0 ? (M - 2L * r) * (N - 2L * r) : 0;
+    CK(sc[2].ensure(sizeof(double) * 4 * (nw ? nw : 1)));
+    CK(launch_strain_finish(M, N, strain_type, r, Fd, sc[2].as<double>(), ctx->stream));
+    CK(down(Ffinite, Fd, 4 * nn, ctx->stream));
+    if (Fworld && nw) CK(down(Fworld, sc[2].p, 4 * (size_t)nw, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
 int aldic_gridfit(aldic_ctx *ctx, int gx, int gy, const double *xnodes, const double *ynodes,
                   const double *z, double rtol, int maxit, double *out, double *stats)
 {
@@ -1010,6 +1050,7 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     CK(up(F.coords, F.hcoords.data(), F.hcoords.size(), ctx->stream));
     CK(natgrid_setup(F.nat, F.hcoords.data(), S, ctx->stream));
     CK(F.nat_err.ensure(sizeof(int)));
+    CK(F.nbad.ensure(sizeof(int) * kMaxSteps));
     if (!F.nat_err_h) CK(hipHostMalloc((void **)&F.nat_err_h, sizeof(int), hipHostMallocDefault));
     const size_t sd = sizeof(double) * S;
     DBuf *v2[] = {&F.U0, &F.U6, &F.Ucur, &F.U1, &F.U2, &F.vdual};
@@ -1098,6 +1139,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
     };
     const int e_start = mark();
     CK(natural_err_clear(F.nat_err.as<int>(), st));
+    CK(launch_iset(F.nbad.as<int>(), kMaxSteps, -1, st));
     CK(run_prep(ctx));
     ctx->have_images = true;
     ctx->images_from_raw = true;
@@ -1310,6 +1352,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
             ar2.nlist_dev = ar.nlist_dev;
             CK(launch_icgn2(ar2, ppl_big, st));
         }
+        if (step < kMaxSteps) CK(launch_count_bad(F.it2.as<int>(), S, F.nbad.as<int>() + step, st));
         if (!o.no_subpb1_fit) {
             // Subpb1.m:165-199: bad nodes (status < 0 or > 99) carry no datum; gridfit of u
             // and v over the good nodes, written to every node.  The axis fill of the bad
@@ -1697,8 +1740,14 @@ int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *ite
 
 int aldic_frame_counters(aldic_ctx *ctx, int *out, int n)
 {
-    if (!ctx || !out || n < 0 || n > 3) return ALDIC_ERR_ARG;
-    const int v[3] = {ctx->fr.gf_budget, ctx->fr.gf_reruns, ctx->fr.gf_most};
+    if (!ctx || !out || n < 0 || n > 3 + kMaxSteps) return ALDIC_ERR_ARG;
+    int v[3 + kMaxSteps] = {ctx->fr.gf_budget, ctx->fr.gf_reruns, ctx->fr.gf_most};
+    for (int k = 0; k < kMaxSteps; ++k) v[3 + k] = -1;
+    if (n > 3 && ctx->fr.ready && ctx->fr.nbad.p) {
+        CK(hipSetDevice(ctx->device));
+        CK(hipStreamSynchronize(ctx->stream));
+        CK(hipMemcpy(v + 3, ctx->fr.nbad.p, sizeof(int) * kMaxSteps, hipMemcpyDeviceToHost));
+    }
     for (int k = 0; k < n; ++k) out[k] = v[k];
     return ALDIC_OK;
 }

@@ -957,6 +957,36 @@ hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_
     return hipGetLastError();
 }
 
+// Subpb1.m:166-169: the number of statuses < 0 or > 99 (one workgroup, deterministic).
+__global__ __launch_bounds__(256) void k_count_bad(const int *it, int S, int *out)
+{
+    __shared__ int red[4];
+    int c = 0;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) c += (it[i] < 0 || it[i] > 99) ? 1 : 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, kWave);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_iset(int *p, int n, int v)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+hipError_t launch_count_bad(const int *iters, int S, int *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_count_bad, dim3(1), dim3(256), 0, st, iters, S, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_iset(int *p, int n, int v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_iset, dim3(1), dim3(64), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
                           const double *Fsrc, double *U, double *F, int *bad, double *part,
                           hipStream_t st)

@@ -265,6 +265,22 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         check(aldic_nodal_strain(g_ctx, dbl(prhs[1], "U"), (int)mxGetScalar(prhs[2]),
                                  mxGetDoubles(plhs[0])),
               "nodal_strain");
+    } else if (cmd == "compute_strain") {
+        // [Ffinite, Fworld] = aldic_mex('compute_strain', M, N, winstepsize, ULocal, FLocal,
+        //                               MethodToComputeStrain, Rad, StrainType)
+        need(nrhs, 9, cmd.c_str());
+        const int M = (int)mxGetScalar(prhs[1]), N = (int)mxGetScalar(prhs[2]);
+        const int method = (int)mxGetScalar(prhs[6]), rad = (int)mxGetScalar(prhs[7]);
+        const int r = method == 0 ? 0 : (method == 2 ? rad : 1);
+        const int Mi = M - 2 * r > 0 ? M - 2 * r : 0, Ni = N - 2 * r > 0 ? N - 2 * r : 0;
+        plhs[0] = dmat(4 * (size_t)M * N, 1);
+        plhs[1] = dmat(4 * (size_t)Mi * Ni, 1);
+        const double *U = numel(prhs[4]) ? dbl(prhs[4], "ULocal") : nullptr;
+        const double *F = numel(prhs[5]) ? dbl(prhs[5], "FLocal") : nullptr;
+        check(aldic_compute_strain(g_ctx, M, N, mxGetScalar(prhs[3]), U, F, method, rad,
+                                   (int)mxGetScalar(prhs[8]), mxGetDoubles(plhs[0]),
+                                   (Mi > 0 && Ni > 0) ? mxGetDoubles(plhs[1]) : nullptr),
+              "compute_strain");
     } else if (cmd == "set_quadtree") {  // aldic_mex('set_quadtree', coords, elementsFEM (E x 8), dirichlet)
         need(nrhs, 4, cmd.c_str());
         const std::vector<double> xy = coords2(prhs[1]);
@@ -400,11 +416,11 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     } else if (cmd == "frame_times") {  // ms = aldic_mex('frame_times')
         plhs[0] = dmat(1, 10);
         check(aldic_frame_times(g_ctx, mxGetDoubles(plhs[0]), 10), "frame_times");
-    } else if (cmd == "frame_counters") {  // c = aldic_mex('frame_counters')
-        int c[3];
-        check(aldic_frame_counters(g_ctx, c, 3), "frame_counters");
-        plhs[0] = dmat(1, 3);
-        for (int k = 0; k < 3; ++k) mxGetDoubles(plhs[0])[k] = c[k];
+    } else if (cmd == "frame_counters") {  // c = aldic_mex('frame_counters') (1 x 11)
+        int c[11];
+        check(aldic_frame_counters(g_ctx, c, 11), "frame_counters");
+        plhs[0] = dmat(1, 11);
+        for (int k = 0; k < 11; ++k) mxGetDoubles(plhs[0])[k] = c[k];
     } else if (cmd == "frame_scalars") {  // v = aldic_mex('frame_scalars')
         plhs[0] = dmat(1, 4);
         check(aldic_frame_scalars(g_ctx, mxGetDoubles(plhs[0]), 4), "frame_scalars");

@@ -172,6 +172,23 @@ int aldic_nodal_strain_quadtree(aldic_ctx *ctx, const double *U, const int *tri,
  * swap_fix = 1 writes [u_x v_x u_y v_y]. */
 int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F);
 
+/* ComputeStrain.m:11-139 (main_ALDIC.m:611 runs it per frame) on a uniform M x N node
+ * grid, node n = i + M j (x fastest, MeshSetUp's order), U interleaved (2 M N), F per
+ * node (F11, F21, F12, F22).  method 0: the solved F (ComputeStrain.m:12-24);
+ * 1: funDerivativeOp.m's finite differences of U with step h (central inside,
+ * one-sided on the border; :27-42); 2: those, then PlaneFit2 plane fits over
+ * (2 rad + 1)^2 nodes for the nodes at least rad from the border (:45-76; the
+ * reference asks for rad interactively); 3: funGlobal_NodalStrainAvg of U on the mesh
+ * set by aldic_set_mesh, which must be this grid (:79-94, its slot order kept).
+ * strain_type 0 infinitesimal, 1 Eulerian, 2 Green-Lagrangian (:98-136; 3, "code by
+ * yourself", is rejected).  Ffinite (4 M N) receives FStrainFinite; Fworld (nullable,
+ * 4 (M - 2r)(N - 2r)) the trimmed FStrainWorld (nodes at least r from the border, r = 0,
+ * 1, rad, 1 for methods 0..3, x fastest, F21 and F12 negated).  Replaces the
+ * ComputeStrain.m script (no reference function signature). */
+int aldic_compute_strain(aldic_ctx *ctx, int M, int N, double h, const double *U,
+                         const double *F, int method, int rad, int strain_type, double *Ffinite,
+                         double *Fworld);
+
 /* LocalICGN.m:141-161, Subpb1Quadtree.m:188-199: scatteredInterpolant(good nodes, values,
  * 'natural') evaluated at every node.  nn nodes (nn x 2 coordinates), bad[i] != 0 marks a
  * node with no datum; U (2 nn) and F (4 nn, nullable) are refilled in place at the bad
@@ -258,7 +275,10 @@ int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
 /* Counters of the frame driver (no counterpart in the reference): {iteration kernels
  * each Subpb1 gridfit of the next frame runs before its convergence is checked,
  * frames run a second time because a gridfit needed more, largest gridfit iteration
- * count of the last frame}. n <= 3. */
+ * count of the last frame}, then for ADMM steps 1..8 the bad-node count of that step's
+ * Subpb1 (statuses < 0 or > 99, Subpb1.m:166-169; -1 for a step not run), which with
+ * LocalICGN's count (from iters6, LocalICGN.m:117-122) is ALSub1BadPtNum
+ * (main_ALDIC.m:180,365,456). n <= 11. */
 int aldic_frame_counters(aldic_ctx *ctx, int *out, int n);
 /* Scalars of the last frame (no counterpart in the reference's outputs; main_ALDIC.m
  * prints them): {beta used, ADMM steps run, RMS change of U2 in the last step, RMS change

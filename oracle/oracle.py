@@ -726,3 +726,126 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     T["frame"] = time.perf_counter() - t0
     return dict(U=U2, F=F2, U6=U6, F6=F6, iters6=it6, iters2=it2, U0=U0, xy=xy, Nx=Nx, Ny=Ny,
                 times=T, beta=beta, admm_steps=steps, update=(upd, upd2))
+
+
+# ----------------------------------------------------------------------------------
+# f4  ComputeStrain.m:11-139 on a uniform M x N node grid (node n = i + M j, x fastest)
+# ----------------------------------------------------------------------------------
+def derivative_op(M, N, h):
+    """funDerivativeOp.m:28-67: the sparse 4MN x 2MN operator F = D U, built entry by
+    entry as the reference does (border rows use the node itself as the missing
+    neighbour with weights -2 / 2 instead of -1 / 1; duplicate entries summed and exact
+    zeros dropped, as MATLAB's sparse() does)."""
+    import scipy.sparse as sp
+    I, J, V = [], [], []
+    for t in range(M * N):
+        x, y = t % M + 1, t // M + 1                       # ndgrid(1:M, 1:N) order
+        bo = [1, 1, 1, 1]                                  # {back, front, right, left}
+        if y == 1: bo[1] = 0
+        if y == N: bo[0] = 0
+        if x == 1: bo[3] = 0
+        if x == M: bo[2] = 0
+        idx = x + M * (y - 1)
+        nb = [idx + bo[0] * M, idx - bo[1] * M, idx + bo[2], idx - bo[3]]
+        rows = [4 * idx - 3, 4 * idx - 2, 4 * idx - 1, 4 * idx]
+        c1 = [2 * nb[3] - 1, 2 * nb[3], 2 * nb[1] - 1, 2 * nb[1]]
+        c2 = [2 * nb[2] - 1, 2 * nb[2], 2 * nb[0] - 1, 2 * nb[0]]
+        v1 = [-1.0, -1.0, -1.0, -1.0]
+        v2 = [1.0, 1.0, 1.0, 1.0]
+        if bo[2] * bo[3] == 0:
+            v1[0] = v1[1] = -2.0
+            v2[0] = v2[1] = 2.0
+        if bo[0] * bo[1] == 0:
+            v1[2] = v1[3] = -2.0
+            v2[2] = v2[3] = 2.0
+        I += rows + rows
+        J += c1 + c2
+        V += v1 + v2
+    A = sp.coo_matrix((V, (np.array(I) - 1, np.array(J) - 1)), shape=(4 * M * N, 2 * M * N)).tocsr()
+    A.sum_duplicates()
+    A.eliminate_zeros()
+    return (1.0 / (2 * h)) * A
+
+
+def plane_fit2(U, hx, hy, R):
+    """plotFiles/PlaneFit2.m: per node (ii, jj) at least R from the border, the least
+    squares of [1, ti hx, tj hy] over U(ii + ti, jj + tj), solved from its normal
+    equations as the reference does ((A'A) \\ (A'b)).  U is M x N (x along rows).
+    Returns (UY, UX, UNew) with the border copied from U."""
+    U = np.asarray(U, np.float64)
+    M, N = U.shape
+    UY, UX, UNew = U.copy(), U.copy(), U.copy()
+    L = np.ones(((2 * R + 1) ** 2, 3))
+    for tj in range(-R, R + 1):
+        for ti in range(-R, R + 1):
+            L[(2 * R + 1) * (tj + R) + ti + R] = [1.0, ti * hx, tj * hy]
+    AtA = L.T @ L
+    for ii in range(R, M - R):
+        for jj in range(R, N - R):
+            b = np.empty((2 * R + 1) ** 2)
+            for tj in range(-R, R + 1):
+                for ti in range(-R, R + 1):
+                    b[(2 * R + 1) * (tj + R) + ti + R] = U[ii + ti, jj + tj]
+            v = np.linalg.solve(AtA, L.T @ b)
+            UNew[ii, jj], UX[ii, jj], UY[ii, jj] = v
+    return UY, UX, UNew
+
+
+def strain_type(Fs, t):
+    """ComputeStrain.m:98-136: infinitesimal (0), Eulerian (1), Green-Lagrangian (2)."""
+    F = np.array(Fs, np.float64)
+    ux, vx, uy, vy = (np.asarray(Fs, np.float64)[k::4] for k in range(4))
+    if t == 1:
+        F[0::4] = 1.0 / (1.0 - ux) - 1.0
+        F[3::4] = 1.0 / (1.0 - vy) - 1.0
+        F[2::4] = uy / (1.0 - vy)
+        F[1::4] = vx / (1.0 - ux)
+    elif t == 2:
+        F[0::4] = 0.5 * (ux * 2 - ux ** 2 - vx ** 2)
+        F[3::4] = 0.5 * (vy * 2 - uy ** 2 - vy ** 2)
+        F[2::4] = 0.5 * (uy + vx - ux * uy - vx * vy)
+        F[1::4] = 0.5 * (vx + uy - uy * ux - vy * vx)
+    elif t != 0:
+        raise ValueError("strain type 0, 1 or 2")
+    return F
+
+
+def trim_index(M, N, r):
+    """ComputeStrain.m:18-24: temp3, the 0-based F entries of the nodes at least r from
+    the border, reshape(1:MN, M, N)(r+1:M-r, r+1:N-r) in column-major order."""
+    temp = np.arange(M * N).reshape(N, M).T               # reshape(temp, M, N), 0-based
+    t2 = temp[r:M - r, r:N - r].flatten(order="F")
+    return (4 * t2[:, None] + np.arange(4)[None, :]).ravel()
+
+
+def compute_strain(M, N, h, U=None, F=None, method=1, rad=1, stype=0, coords=None, elems=None):
+    """ComputeStrain.m:11-139: FStrain by the method, the strain type, then the trimmed
+    FStraintemp and FStrainWorld (F21 and F12 negated).  Method 3 uses nodal_strain
+    (funGlobal_NodalStrainAvg) on the given mesh.  Returns a dict."""
+    if method == 0:
+        Fs, r = np.array(F, np.float64), 0
+    elif method == 1:
+        Fs, r = derivative_op(M, N, h) @ np.asarray(U, np.float64), 1
+    elif method == 2:
+        U = np.asarray(U, np.float64)
+        Fs, r = derivative_op(M, N, h) @ U, rad
+        uy, ux, _ = plane_fit2(U[0::2].reshape(N, M).T, h, h, rad)
+        vy, vx, _ = plane_fit2(U[1::2].reshape(N, M).T, h, h, rad)
+        n = (M - 2 * rad) * (N - 2 * rad)
+        ft = np.zeros(4 * n)
+        ft[0::4] = ux[rad:M - rad, rad:N - rad].flatten(order="F")
+        ft[1::4] = vx[rad:M - rad, rad:N - rad].flatten(order="F")
+        ft[2::4] = uy[rad:M - rad, rad:N - rad].flatten(order="F")
+        ft[3::4] = vy[rad:M - rad, rad:N - rad].flatten(order="F")
+        Fs[trim_index(M, N, rad)] = ft
+    elif method == 3:
+        Fs, r = nodal_strain(coords, elems, U), 1
+    else:
+        raise ValueError("method 0..3")
+    Ff = strain_type(Fs, stype)
+    t3 = trim_index(M, N, r)
+    Ft = Ff[t3]
+    Fw = Ft.copy()
+    Fw[1::4] = -Fw[1::4]
+    Fw[2::4] = -Fw[2::4]
+    return dict(FStrain=Fs, FStrainFinite=Ff, FStraintemp=Ft, FStrainWorld=Fw, Rad=r, temp3=t3)
