@@ -26,7 +26,8 @@ EXPORTS = (
     "aldic_frame_counters", "aldic_frame_scalars", "aldic_set_quadtree",
     "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
     "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc", "aldic_natural_fill",
-    "aldic_frame_run_seeded", "aldic_compute_strain",
+    "aldic_frame_run_seeded", "aldic_compute_strain", "aldic_inpaint_nans",
+    "aldic_remove_outliers",
 )
 
 
@@ -35,7 +36,7 @@ class FrameOpts(C.Structure):
                 ("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
                 ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
                 ("no_subpb1_fit", C.c_int), ("stage_times", C.c_int), ("smooth", C.c_int),
-                ("beta_sweep", C.c_int), ("admm_tol", C.c_double)]
+                ("beta_sweep", C.c_int), ("admm_tol", C.c_double), ("outlier_thr", C.c_double)]
 
 
 class QFrameOpts(C.Structure):
@@ -91,6 +92,8 @@ def load():
     L.aldic_natural_fill.argtypes = [P, ip, vp, vp, vp, vp]
     L.aldic_frame_run_seeded.argtypes = [P, vp]
     L.aldic_compute_strain.argtypes = [P, ip, ip, dp, vp, vp, ip, ip, ip, vp, vp]
+    L.aldic_inpaint_nans.argtypes = [P, vp, ip, ip, ip]
+    L.aldic_remove_outliers.argtypes = [P, vp, vp, ip, ip, dp, vp, vp]
     _lib = L
     return L
 
@@ -314,6 +317,28 @@ class Context:
                                            _p(F)), "aldic_nodal_strain")
         return F
 
+    def inpaint_nans(self, A, method):
+        """inpaint_nans(A, method) for method 3 (plate) or 4 (springs) on a 2-D array."""
+        A = np.asarray(A, dtype=np.float64)
+        rows, cols = A.shape
+        f = np.asfortranarray(A).copy(order="F")
+        self._ck(self.L.aldic_inpaint_nans(self.h, f.ctypes.data, rows, cols, int(method)),
+                 "aldic_inpaint_nans")
+        return np.ascontiguousarray(f)
+
+    def remove_outliers(self, u, v, thr):
+        """funRemoveOutliers(u, v, [], 0.5, thr) without prompts -> (u, v, flagged)."""
+        u = np.asarray(u, dtype=np.float64)
+        rows, cols = u.shape
+        uf = np.asfortranarray(u).copy(order="F")
+        vf = np.asfortranarray(np.asarray(v, dtype=np.float64)).copy(order="F")
+        fl = np.zeros((rows, cols), np.int32, order="F")
+        nf = C.c_int(0)
+        self._ck(self.L.aldic_remove_outliers(self.h, uf.ctypes.data, vf.ctypes.data, rows, cols,
+                                              float(thr), fl.ctypes.data, C.byref(nf)),
+                 "aldic_remove_outliers")
+        return np.ascontiguousarray(uf), np.ascontiguousarray(vf), fl.astype(bool)
+
     def compute_strain(self, M, N, h, U=None, F=None, method=1, rad=1, strain_type=0):
         """ComputeStrain.m on an M x N node grid -> (FStrainFinite (4 M N), FStrainWorld
         (the trimmed nodes, F21 and F12 negated), r).  Method 3 uses the mesh of
@@ -374,7 +399,7 @@ class Context:
 
     def frame_setup(self, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
                     cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True, stage_times=True,
-                    smooth=False, beta_sweep=False, admm_tol=0.0):
+                    smooth=False, beta_sweep=False, admm_tol=0.0, outlier_thr=100.0):
         o = FrameOpts()
         o.ws, o.step = int(ws), int(step)
         o.s[0], o.s[1] = (int(s), int(s)) if np.isscalar(s) else (int(s[0]), int(s[1]))
@@ -391,6 +416,7 @@ class Context:
         o.smooth = 1 if smooth else 0
         o.beta_sweep = 1 if beta_sweep else 0
         o.admm_tol = float(admm_tol)
+        o.outlier_thr = float(outlier_thr)
         n = C.c_int(0)
         self._ck(self.L.aldic_frame_setup(self.h, C.byref(o), C.byref(n)), "aldic_frame_setup")
         self.frame_nodes = n.value

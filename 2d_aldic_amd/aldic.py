@@ -186,140 +186,31 @@ def MeshSetUp(x, y, DICpara):
 
 
 def inpaint_nans_springs(A):
-    """inpaint_nans.m method 4: springs between each NaN and its 4-neighbours, solved in
-    least squares (each NaN = mean of its neighbours, known values fixed)."""
-    import scipy.sparse as sp
-    import scipy.sparse.linalg as spl
-    A = np.array(A, dtype=np.float64)
-    nan = np.isnan(A)
-    if not nan.any():
-        return A
-    n, m = A.shape
-    idx = np.arange(n * m).reshape(n, m)
-    rows, cols, vals = [], [], []
-    k = 0
-    pairs = set()
-    for (i, j) in zip(*np.nonzero(nan)):
-        for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
-            a, b = i + di, j + dj
-            if 0 <= a < n and 0 <= b < m:
-                pairs.add(tuple(sorted((idx[i, j], idx[a, b]))))
-    for (p, q) in sorted(pairs):
-        rows += [k, k]
-        cols += [p, q]
-        vals += [1.0, -1.0]
-        k += 1
-    S = sp.csr_matrix((vals, (rows, cols)), shape=(k, n * m))
-    flat = A.ravel()
-    known = ~nan.ravel()
-    rhs = -S[:, known] @ flat[known]
-    unk = np.nonzero(~known)[0]
-    Su = S[:, unk]
-    sol = spl.spsolve((Su.T @ Su).tocsc(), Su.T @ rhs)
-    flat = flat.copy()
-    flat[unk] = sol
-    return flat.reshape(n, m)
+    """inpaint_nans(A, 4) on the device (aldic_inpaint_nans): springs between each NaN and
+    its horizontal and vertical neighbours, least squares with the known values fixed."""
+    return get_context().inpaint_nans(A, 4)
 
 
 def inpaint_nans_plate(A):
-    """inpaint_nans.m method 3: the del^4 (biharmonic) operator on every NaN node and the
-    known nodes its 13-point template reaches, reduced near the border (5-point Laplacian
-    on the second ring, second differences along the first row/column, nothing at the
-    corners); the known values are moved to the right-hand side and the NaNs solved in
-    least squares."""
-    A = np.array(A, dtype=np.float64)
-    nan = np.isnan(A)
-    if not nan.any():
-        return A
-    n, m = A.shape
-    talks = [(-2, 0), (-1, -1), (-1, 0), (-1, 1), (0, -2), (0, -1), (0, 1), (0, 2), (1, -1),
-             (1, 0), (1, 1), (2, 0)]
-    nodes = set(zip(*np.nonzero(nan)))
-    for (i, j) in list(nodes):
-        for di, dj in talks:
-            if 0 <= i + di < n and 0 <= j + dj < m:
-                nodes.add((i + di, j + dj))
-    d4 = [((0, -2), 1), ((-1, -1), 2), ((0, -1), -8), ((1, -1), 2), ((-2, 0), 1), ((-1, 0), -8),
-          ((0, 0), 20), ((1, 0), -8), ((2, 0), 1), ((-1, 1), 2), ((0, 1), -8), ((1, 1), 2),
-          ((0, 2), 1)]
-    lap = [((0, -1), 1), ((-1, 0), 1), ((0, 0), -4), ((1, 0), 1), ((0, 1), 1)]
-    rows = []
-    for (i, j) in sorted(nodes):  # 1-based r = i + 1, c = j + 1
-        r, c = i + 1, j + 1
-        if 3 <= r <= n - 2 and 3 <= c <= m - 2:
-            st = d4
-        elif ((r in (2, n - 1)) and 2 <= c <= m - 1) or ((c in (2, m - 1)) and 2 <= r <= n - 1):
-            st = lap
-        elif r in (1, n) and 2 <= c <= m - 1:
-            st = [((0, -1), 1), ((0, 0), -2), ((0, 1), 1)]
-        elif c in (1, m) and 2 <= r <= n - 1:
-            st = [((-1, 0), 1), ((0, 0), -2), ((1, 0), 1)]
-        else:
-            continue
-        rows.append([((i + di, j + dj), w) for (di, dj), w in st])
-    unk = sorted(zip(*np.nonzero(nan)))
-    col = {p: k for k, p in enumerate(unk)}
-    M = np.zeros((len(rows), len(unk)))
-    rhs = np.zeros(len(rows))
-    for k, st in enumerate(rows):
-        for p, w in st:
-            if p in col:
-                M[k, col[p]] += w
-            else:
-                rhs[k] -= w * A[p]
-    keep = np.any(M != 0.0, axis=1)
-    sol = np.linalg.lstsq(M[keep], rhs[keep], rcond=None)[0]
-    for p, k in col.items():
-        A[p] = sol[k]
-    return A
-
-
-def _medfilt2_3(V):
-    """medFilt2(V, [2 2], 'symmetric', 2) as funRemoveOutliers.m:381-418 calls it: padding
-    by one mirrored row and column after the array, the 2 x 2 neighbourhood without its
-    second member, i.e. the median of V(i, j), V(i+1, j), V(i+1, j+1), taken in single
-    precision."""
-    P = np.pad(np.asarray(V, np.float64), ((0, 1), (0, 1)), mode="symmetric")
-    st = np.sort(np.stack([P[:-1, :-1], P[1:, :-1], P[1:, 1:]]).astype(np.float32), axis=0)
-    return st[1].astype(np.float64)
-
-
-def funMedRemoveOutliers(u, epsilon=0.1):
-    """funRemoveOutliers.m:371-379: normalised median fluctuation of u (its NaNs in-painted
-    with method 3 first)."""
-    u = inpaint_nans_plate(u)
-    med = _medfilt2_3(u)
-    fluct = u - med
-    res = _medfilt2_3(np.abs(fluct))
-    return med, np.abs(fluct / (res + epsilon))
+    """inpaint_nans(A, 3) on the device (aldic_inpaint_nans): the del^4 plate template,
+    reduced near the border, least squares with the known values fixed."""
+    return get_context().inpaint_nans(A, 3)
 
 
 def funRemoveOutliers(u, v, cc, qDICOrNot, Thr0, *varargin):
-    """funRemoveOutliers.m:1-140 without its prompts: the normalised median test of u and v
-    (epsilon 0.1, funRemoveOutliers.m:30-35) at threshold Thr0, the flagged nodes and the
-    NaNs in-painted by springs (method 4), then the original NaNs put back (:56).  The
-    q-factor removal (qDICOrNot == 1) and the interactive bound / click cleanup that
-    follow when qDICOrNot != 0 are not provided; Thr0 must be given."""
+    """funRemoveOutliers.m:1-86 without its prompts, on the device
+    (aldic_remove_outliers): the single-precision normalised median test of u and v at
+    threshold Thr0, the flagged elements and the NaNs in-painted by springs, the NaNs put
+    back.  Returns (u, v, cc, BadptRow, BadptCol, RemoveOutliersList) with the list as
+    MATLAB's 1-based column-major indices.  The q-factor removal (qDICOrNot == 1) and the
+    interactive bound / click cleanup that follow when qDICOrNot != 0 are not provided;
+    Thr0 must be given."""
     if qDICOrNot == 1:
         raise NotImplementedError("q-factor based removal (bimodal fit) is not provided")
     if not Thr0:
         raise NotImplementedError("the threshold prompt is interactive; pass Thr0")
-    u = np.array(u, dtype=np.float64)
-    v = np.array(v, dtype=np.float64)
-    hole = np.isnan(u)
-    _, nfu = funMedRemoveOutliers(u)
-    _, nfv = funMedRemoveOutliers(v)
-    mag = np.sqrt(nfu ** 2 + nfv ** 2)
-    flagged = mag > Thr0
-    u2 = u.copy()
-    v2 = v.copy()
-    u2[flagged] = np.nan
-    v2[flagged] = np.nan
-    u2 = inpaint_nans_springs(u2)
-    v2 = inpaint_nans_springs(v2)
-    u2[hole] = np.nan
-    v2[hole] = np.nan
-    lst = np.flatnonzero(flagged.ravel(order="F")) + 1  # MATLAB linear indices
+    u2, v2, flagged = get_context().remove_outliers(u, v, Thr0)
+    lst = np.flatnonzero(flagged.ravel(order="F")) + 1
     return u2, v2, cc, [], [], lst
 
 
@@ -474,6 +365,6 @@ def funGlobal_NodalStrainAvg(coordinatesFEM, elementsFEM, U, GaussPtOrder=2, swa
 
 __all__ = ["AldicError", "get_context", "funNormalizeImg", "funImgGradient", "ba_interp2",
            "funIntegerSearch", "IntegerSearch", "MeshSetUp", "Init", "inpaint_nans_springs",
-           "inpaint_nans_plate", "funRemoveOutliers", "funMedRemoveOutliers",
+           "inpaint_nans_plate", "funRemoveOutliers",
            "LocalICGN", "funICGN", "Subpb1", "funICGN_Subpb1", "Subpb2",
            "funGlobal_NodalStrainAvg"]

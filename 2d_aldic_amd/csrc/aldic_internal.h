@@ -158,6 +158,7 @@ struct FrameState {
     DBuf seed;      // aldic_frame_run_seeded: the caller's initial guess (2S)
     DBuf nat_err;   // refills of the frame whose cell could not be formed (device int)
     DBuf nbad;      // bad-node count of each ADMM step's Subpb1 (device int[kMaxSteps], -1: not run)
+    int n_outliers = 0;  // nodes the last searched frame's outlier test flagged
     int *nat_err_h = nullptr;  // pinned copy, read after the frame's wait
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
     double *gfh = nullptr;  // pinned: {converged, iterations, residual} per frame gridfit
@@ -176,6 +177,8 @@ struct FrameState {
     float ms[10] = {0};
 };
 
+struct OutlierWork;  // outlier.hip
+
 }  // namespace aldic
 
 struct aldic_ctx {
@@ -190,6 +193,7 @@ struct aldic_ctx {
     int roi[4] = {0, 0, 0, 0};
     bool have_mask = false;
     bool images_from_raw = false;    // f, g normalised on the device from rawf, rawg
+    aldic::OutlierWork *outl = nullptr;  // work buffers of the outlier test and in-painting
     aldic::DBuf stats;               // ROI partial sums, then {mean_f, 1/std_f, mean_g, 1/std_g}
     aldic::DBuf boxp;                // window box-sum planes of the exact search
     aldic::DBuf scratch[8];          // per-call work buffers
@@ -280,6 +284,18 @@ hipError_t launch_search_mt(const SearchArgs &a, hipStream_t st);
 // subsets), from the full NCC map; LDS need per workgroup (limit 150 KiB).
 size_t qfactor_lds_bytes(int ws, int sx, int sy);
 hipError_t launch_qfactors(const SearchArgs &a, hipStream_t st);
+
+// Outlier removal and NaN in-painting on strided device views (outlier.hip): element
+// (i, j) at p[i si + j sj].  Return 0, -1 (HIP error) or an ALDIC error code with msg.
+OutlierWork *outlier_work_new();
+void outlier_work_free(OutlierWork *w);
+int inpaint_view(OutlierWork &w, double *p, int rows, int cols, long si, long sj, int method,
+                 hipStream_t st, const char *&msg);
+int remove_outliers_views(OutlierWork &w, double *up, double *vp, int rows, int cols, long si,
+                          long sj, float thr, bool keep_nan, int *flagged, long fsi, long fsj,
+                          int *n_flagged, hipStream_t st, const char *&msg);
+hipError_t launch_mask_nodes(double *U, const double *coords, int S, const uint8_t *mask, int nx,
+                             int ny, hipStream_t st);
 
 // Strain post-processing on a uniform M x N node grid (strain.hip): F from U by finite
 // differences (method 1) or finite differences plus plane fits of half width R inside

@@ -273,6 +273,9 @@ void aldic_destroy(aldic_ctx *ctx)
     }
     F.gfstat.release();
     F.nat_err.release();
+    F.nbad.release();
+    outlier_work_free(ctx->outl);
+    ctx->outl = nullptr;
     F.seed.release();
     ctx->qf.nat_err.release();
     if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
@@ -865,6 +868,64 @@ int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
     return ALDIC_OK;
 }
 
+// Runs an outlier/in-painting step on the context's work buffers and maps its status.
+static int outlier_status(aldic_ctx *ctx, int r, const char *msg, const char *what)
+{
+    if (r == 0) return ALDIC_OK;
+    if (r == -1) return fail(ctx, ALDIC_ERR_HIP, std::string(what) + ": HIP error");
+    return fail(ctx, r, msg ? msg : what);
+}
+
+int aldic_inpaint_nans(aldic_ctx *ctx, double *A, int rows, int cols, int method)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!A || rows < 1 || cols < 1 || (method != 3 && method != 4))
+        return fail(ctx, ALDIC_ERR_ARG, "inpaint_nans: bad arguments (method 3 or 4)");
+    CK(hipSetDevice(ctx->device));
+    if (!ctx->outl) ctx->outl = outlier_work_new();
+    const size_t N = (size_t)rows * cols;
+    DBuf &d = ctx->scratch[0];
+    CK(up(d, A, N, ctx->stream));
+    const char *msg = nullptr;
+    const int r = inpaint_view(*ctx->outl, d.as<double>(), rows, cols, 1, rows, method,
+                               ctx->stream, msg);
+    if (r) return outlier_status(ctx, r, msg, "inpaint_nans");
+    CK(down(A, d.p, N, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_remove_outliers(aldic_ctx *ctx, double *u, double *v, int rows, int cols, double thr,
+                          int *flagged, int *n_flagged)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!u || !v || rows < 1 || cols < 1 || !(thr > 0))
+        return fail(ctx, ALDIC_ERR_ARG, "remove_outliers: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    if (!ctx->outl) ctx->outl = outlier_work_new();
+    const size_t N = (size_t)rows * cols;
+    DBuf *sc = ctx->scratch;
+    CK(sc[0].ensure(sizeof(double) * 2 * N));
+    CK(hipMemcpyAsync(sc[0].p, u, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(sc[0].as<double>() + N, v, sizeof(double) * N, hipMemcpyHostToDevice,
+                      ctx->stream));
+    if (flagged) CK(sc[1].ensure(sizeof(int) * N));
+    const char *msg = nullptr;
+    int nf = 0;
+    const int r = remove_outliers_views(*ctx->outl, sc[0].as<double>(), sc[0].as<double>() + N,
+                                        rows, cols, 1, rows, (float)thr, true,
+                                        flagged ? sc[1].as<int>() : nullptr, 1, rows, &nf,
+                                        ctx->stream, msg);
+    if (r) return outlier_status(ctx, r, msg, "remove_outliers");
+    CK(hipMemcpyAsync(u, sc[0].p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipMemcpyAsync(v, sc[0].as<double>() + N, sizeof(double) * N, hipMemcpyDeviceToHost,
+                      ctx->stream));
+    if (flagged) CK(down(flagged, sc[1].p, N, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (n_flagged) *n_flagged = nf;
+    return ALDIC_OK;
+}
+
 int aldic_compute_strain(aldic_ctx *ctx, int M, int N, double h, const double *U,
                          const double *F, int method, int rad, int strain_type, double *Ffinite,
                          double *Fworld)
@@ -1179,8 +1240,21 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
         CK(hipEventRecord(F.ev_skern, st));
         CK(hipMemcpyAsync(F.U0.p, F.seed.p, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
     }
-    // the search's NaNs (constant templates) go to LocalICGN as they are: their subsets
-    // end non-finite (status 101) and the bad-point refill covers them
+    if (!seeded && o.outlier_thr > 0) {
+        // IntegerSearch.m:68-76 (mask holes NaN), :114 (funRemoveOutliers at Thr0) and
+        // Init.m:13: flagged and NaN nodes in-painted by springs in one solve
+        if (ctx->have_mask)
+            CK(launch_mask_nodes(F.U0.as<double>(), F.coords.as<double>(), S,
+                                 ctx->mask.as<uint8_t>(), ctx->nx, ctx->ny, st));
+        if (!ctx->outl) ctx->outl = outlier_work_new();
+        const char *msg = nullptr;
+        const int r0 = remove_outliers_views(*ctx->outl, F.U0.as<double>(), F.U0.as<double>() + 1,
+                                             F.Ny, F.Nx, 2L * F.Nx, 2L, (float)o.outlier_thr,
+                                             false, nullptr, 0, 0, &F.n_outliers, st, msg);
+        if (r0) return outlier_status(ctx, r0, msg, "frame: search outlier test");
+    }
+    // (with outlier_thr 0 the search's NaNs go to LocalICGN as they are: their subsets end
+    // non-finite, status 101, and the bad-point refill covers them)
     const int e_search = mark();
     span(1, e_prep, e_search);
 

@@ -265,6 +265,25 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         check(aldic_nodal_strain(g_ctx, dbl(prhs[1], "U"), (int)mxGetScalar(prhs[2]),
                                  mxGetDoubles(plhs[0])),
               "nodal_strain");
+    } else if (cmd == "inpaint_nans") {  // B = aldic_mex('inpaint_nans', A, method)
+        need(nrhs, 3, cmd.c_str());
+        plhs[0] = mxDuplicateArray(prhs[1]);
+        check(aldic_inpaint_nans(g_ctx, mxGetDoubles(plhs[0]), (int)mxGetM(prhs[1]),
+                                 (int)mxGetN(prhs[1]), (int)mxGetScalar(prhs[2])),
+              "inpaint_nans");
+    } else if (cmd == "remove_outliers") {
+        // [u, v, flagged] = aldic_mex('remove_outliers', u, v, Thr0)
+        need(nrhs, 4, cmd.c_str());
+        const int rows = (int)mxGetM(prhs[1]), cols = (int)mxGetN(prhs[1]);
+        plhs[0] = mxDuplicateArray(prhs[1]);
+        plhs[1] = mxDuplicateArray(prhs[2]);
+        std::vector<int> fl((size_t)rows * cols);
+        int nf = 0;
+        check(aldic_remove_outliers(g_ctx, mxGetDoubles(plhs[0]), mxGetDoubles(plhs[1]), rows, cols,
+                                    mxGetScalar(prhs[3]), fl.data(), &nf),
+              "remove_outliers");
+        plhs[2] = dmat(rows, cols);
+        for (size_t k = 0; k < fl.size(); ++k) mxGetDoubles(plhs[2])[k] = fl[k];
     } else if (cmd == "compute_strain") {
         // [Ffinite, Fworld] = aldic_mex('compute_strain', M, N, winstepsize, ULocal, FLocal,
         //                               MethodToComputeStrain, Rad, StrainType)
@@ -378,7 +397,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
               "frame_stage");
     } else if (cmd == "frame_setup") {
         // S = aldic_mex('frame_setup', ws, step, s, grid, tol, mu, beta, n_admm, cg_maxit,
-        //               cg_rtol, smooth, beta_sweep, admm_tol)
+        //               cg_rtol, smooth, beta_sweep, admm_tol [, outlier_thr (100; 0: off)])
         need(nrhs, 14, cmd.c_str());
         aldic_frame_opts o{};
         o.ws = (int)mxGetScalar(prhs[1]);
@@ -396,6 +415,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         o.smooth = (int)mxGetScalar(prhs[11]);
         o.beta_sweep = (int)mxGetScalar(prhs[12]);
         o.admm_tol = mxGetScalar(prhs[13]);
+        o.outlier_thr = nrhs > 14 ? mxGetScalar(prhs[14]) : 100.0;
         int S = 0;
         check(aldic_frame_setup(g_ctx, &o, &S), "frame_setup");
         plhs[0] = mxCreateDoubleScalar(S);

@@ -185,6 +185,25 @@ int aldic_nodal_strain(aldic_ctx *ctx, const double *U, int swap_fix, double *F)
  * 4 (M - 2r)(N - 2r)) the trimmed FStrainWorld (nodes at least r from the border, r = 0,
  * 1, rad, 1 for methods 0..3, x fastest, F21 and F12 negated).  Replaces the
  * ComputeStrain.m script (no reference function signature). */
+/* inpaint_nans.m method 3 (del^4 plate) or 4 (springs) of a rows x cols array in
+ * MATLAB's column-major layout, in place (as Init.m:13 and funRemoveOutliers.m:58,375
+ * call it): the NaNs get the least-squares solution of the method's stencil rows with
+ * the known values fixed, solved on the device by a banded Cholesky of the normal
+ * equations.  ALDIC_ERR_NUMERIC if every element is NaN or the band would exceed
+ * 512 MiB. */
+int aldic_inpaint_nans(aldic_ctx *ctx, double *A, int rows, int cols, int method);
+
+/* funRemoveOutliers(u, v, cc, qDICOrNot != 1, Thr0) (funRemoveOutliers.m:45-86) without
+ * its prompts, on two rows x cols column-major arrays in place: the single-precision
+ * normalised median test (funMedRemoveOutliers, medFilt2 [2 2] 'symmetric' skipping the
+ * second neighbour, NaNs pre-filled by method 3), elements with fluctuation magnitude
+ * above thr set NaN and in-painted by springs, the original NaNs put back.  flagged
+ * (nullable, rows x cols column-major) receives RemoveOutliersList as 0/1; n_flagged
+ * (nullable) its count.  The q-factor removal (qDICOrNot == 1) and the interactive bound
+ * and click removal are not provided. */
+int aldic_remove_outliers(aldic_ctx *ctx, double *u, double *v, int rows, int cols, double thr,
+                          int *flagged, int *n_flagged);
+
 int aldic_compute_strain(aldic_ctx *ctx, int M, int N, double h, const double *U,
                          const double *F, int method, int rad, int strain_type, double *Ffinite,
                          double *Fworld);
@@ -251,6 +270,13 @@ typedef struct {
     double admm_tol;   /* > 0: stop the ADMM steps once the RMS change of U2 or of Subpb1's U
                         * (before smoothing) between steps is below it (main_ALDIC.m:409-447,
                         * tol2 = 1e-4); n_admm stays the most steps (the reference: 4) */
+    double outlier_thr; /* > 0: after the integer search, nodes whose centre pixel is masked
+                        * become NaN (IntegerSearch.m:68-76), the normalised median test of
+                        * funRemoveOutliers at Thr0 = outlier_thr (the reference passes 100,
+                        * IntegerSearch.m:114) flags outliers, and the flagged and NaN nodes
+                        * are in-painted by springs (inpaint_nans method 4; the same solve as
+                        * funRemoveOutliers' followed by Init.m:13's) before LocalICGN;
+                        * 0: the search's field goes to LocalICGN as it is */
 } aldic_frame_opts;
 
 int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,

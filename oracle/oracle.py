@@ -610,7 +610,7 @@ def natural_fill(coords, V, nc, bad, return_kind=False):
 
 def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
           cg_rtol=1e-10, subpb1_fit=True, smooth=False, beta_sweep=False, admm_tol=0.0,
-          U0=None, mask=None):
+          U0=None, mask=None, outlier_thr=100.0):
     """FP64 restatement of one frame: prep, search, LocalICGN (+ bad-point rule and
     refill), step-1 Subpb2 + strain + duals, then n_admm x (Subpb1: 2-DOF IC-GN, bad
     points, gridfit of every node; Subpb2, strain, duals).  The LocalICGN refill of bad
@@ -625,6 +625,9 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
     steps falls below it (main_ALDIC.m:409-447).  U0 (2S) replaces the integer search
     with the given initial guess (a later frame of a sequence, main_ALDIC.m:134-137);
     mask (ny x nx, 0/1) multiplies the reference gradients (funImgGradient.m:77-78).
+    After the search (not with U0), outlier_thr > 0 runs IntegerSearch.m:68-76 (nodes on
+    masked pixels NaN), :114 (funRemoveOutliers at Thr0 = outlier_thr, remove_outliers)
+    and Init.m:13 (the remaining NaNs in-painted by springs), in those steps.
     Returns the stages' outputs and
     times, the beta used and the number of ADMM steps run."""
     import time
@@ -641,6 +644,18 @@ def frame(fraw, graw, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
         Nx, Ny = r["Nx"], r["Ny"]
         xy = np.stack([r["x"].ravel(), r["y"].ravel()], 1)
         U0 = np.stack([r["u"].ravel(), r["v"].ravel()], 1).ravel()
+        if outlier_thr > 0:
+            u, v = r["u"].astype(np.float64).copy(), r["v"].astype(np.float64).copy()
+            if mask is not None:
+                xi, yi = r["x"], r["y"]
+                integral = np.all(xi == np.floor(xi)) and np.all(yi == np.floor(yi))
+                if integral:
+                    hole = np.asarray(mask)[yi.astype(int) - 1, xi.astype(int) - 1] == 0
+                    u[hole] = np.nan
+                    v[hole] = np.nan
+            u, v, _ = remove_outliers(u, v, outlier_thr)
+            u, v = inpaint_springs(u), inpaint_springs(v)
+            U0 = np.stack([u.ravel(), v.ravel()], 1).ravel()
     else:
         # main_ALDIC.m:134-137: a later frame starts from the previous frame's U
         ny_, nx_ = f.shape
@@ -849,3 +864,137 @@ def compute_strain(M, N, h, U=None, F=None, method=1, rad=1, stype=0, coords=Non
     Fw[1::4] = -Fw[1::4]
     Fw[2::4] = -Fw[2::4]
     return dict(FStrain=Fs, FStrainFinite=Ff, FStraintemp=Ft, FStrainWorld=Fw, Rad=r, temp3=t3)
+
+
+# ----------------------------------------------------------------------------------
+# f2  funRemoveOutliers.m:45-86,371-445 (non-interactive median test) and
+#     inpaint_nans.m methods 3 and 4 (Init.m:13)
+# ----------------------------------------------------------------------------------
+def inpaint_springs(A):
+    """inpaint_nans.m method 4: a spring between each NaN element and each of its in-array
+    horizontal and vertical neighbours (each unique pair once), the known values moved to
+    the right-hand side, the NaNs solved in least squares."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    A = np.array(A, dtype=np.float64)
+    nan = np.isnan(A)
+    if not nan.any():
+        return A
+    n, m = A.shape
+    idx = np.arange(n * m).reshape(n, m)
+    pairs = set()
+    for (i, j) in zip(*np.nonzero(nan)):
+        for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+            a, b = i + di, j + dj
+            if 0 <= a < n and 0 <= b < m:
+                pairs.add(tuple(sorted((idx[i, j], idx[a, b]))))
+    rows, cols, vals = [], [], []
+    for k, (p, q) in enumerate(sorted(pairs)):
+        rows += [k, k]
+        cols += [p, q]
+        vals += [1.0, -1.0]
+    S = sp.csr_matrix((vals, (rows, cols)), shape=(len(pairs), n * m))
+    flat = A.ravel()
+    known = ~nan.ravel()
+    rhs = -S[:, known] @ flat[known]
+    unk = np.nonzero(~known)[0]
+    Su = S[:, unk]
+    sol = spl.spsolve((Su.T @ Su).tocsc(), Su.T @ rhs)
+    flat = flat.copy()
+    flat[unk] = sol
+    return flat.reshape(n, m)
+
+
+def inpaint_plate(A):
+    """inpaint_nans.m method 3: the del^4 template on every element the NaNs' 13-point
+    template reaches, reduced near the border (the 5-point Laplacian on the second ring,
+    second differences along the first row and column, nothing at the corners), the
+    known values moved to the right-hand side and the NaNs solved in least squares."""
+    A = np.array(A, dtype=np.float64)
+    nan = np.isnan(A)
+    if not nan.any():
+        return A
+    n, m = A.shape
+    talks = [(-2, 0), (-1, -1), (-1, 0), (-1, 1), (0, -2), (0, -1), (0, 1), (0, 2), (1, -1),
+             (1, 0), (1, 1), (2, 0)]
+    nodes = set(zip(*np.nonzero(nan)))
+    for (i, j) in list(nodes):
+        for di, dj in talks:
+            if 0 <= i + di < n and 0 <= j + dj < m:
+                nodes.add((i + di, j + dj))
+    d4 = [((0, -2), 1), ((-1, -1), 2), ((0, -1), -8), ((1, -1), 2), ((-2, 0), 1), ((-1, 0), -8),
+          ((0, 0), 20), ((1, 0), -8), ((2, 0), 1), ((-1, 1), 2), ((0, 1), -8), ((1, 1), 2),
+          ((0, 2), 1)]
+    lap = [((0, -1), 1), ((-1, 0), 1), ((0, 0), -4), ((1, 0), 1), ((0, 1), 1)]
+    rows = []
+    for (i, j) in sorted(nodes):
+        r, c = i + 1, j + 1
+        if 3 <= r <= n - 2 and 3 <= c <= m - 2:
+            st = d4
+        elif ((r in (2, n - 1)) and 2 <= c <= m - 1) or ((c in (2, m - 1)) and 2 <= r <= n - 1):
+            st = lap
+        elif r in (1, n) and 2 <= c <= m - 1:
+            st = [((0, -1), 1), ((0, 0), -2), ((0, 1), 1)]
+        elif c in (1, m) and 2 <= r <= n - 1:
+            st = [((-1, 0), 1), ((0, 0), -2), ((1, 0), 1)]
+        else:
+            continue
+        rows.append([((i + di, j + dj), w) for (di, dj), w in st])
+    unk = sorted(zip(*np.nonzero(nan)))
+    col = {p: k for k, p in enumerate(unk)}
+    M = np.zeros((len(rows), len(unk)))
+    rhs = np.zeros(len(rows))
+    for k, st in enumerate(rows):
+        for p, w in st:
+            if p in col:
+                M[k, col[p]] += w
+            else:
+                rhs[k] -= w * A[p]
+    keep = np.any(M != 0.0, axis=1)
+    sol = np.linalg.lstsq(M[keep], rhs[keep], rcond=None)[0]
+    for p, k in col.items():
+        A[p] = sol[k]
+    return A
+
+
+def medfilt3(V):
+    """medFilt2(V, [2 2], 'symmetric', 2) as funRemoveOutliers.m:373-376 calls it: one
+    mirrored row and column after the array, the 2 x 2 neighbourhood without its second
+    member, i.e. the median of V(i, j), V(i+1, j), V(i+1, j+1), in single precision
+    (the neighbour stack is single, :404)."""
+    P = np.pad(np.asarray(V), ((0, 1), (0, 1)), mode="symmetric").astype(np.float32)
+    st = np.sort(np.stack([P[:-1, :-1], P[1:, :-1], P[1:, 1:]]), axis=0)
+    return st[1]
+
+
+def med_fluct(u, epsilon=0.1):
+    """funRemoveOutliers.m:371-379 (funMedRemoveOutliers): NaNs in-painted by method 3,
+    then the normalised fluctuation |fluct ./ (medianRes + epsilon)|.  The medians are
+    single, so (MATLAB's mixed arithmetic) fluct and everything after it are single."""
+    u = inpaint_plate(u)
+    med = medfilt3(u)
+    fluct = u.astype(np.float32) - med
+    res = medfilt3(np.abs(fluct))
+    nf = np.abs(fluct / (res + np.float32(epsilon)))
+    return med, nf
+
+
+def remove_outliers(u, v, thr=100.0):
+    """funRemoveOutliers(u, v, cc, 0.5, Thr0) without its prompts (:45-86): the median
+    test of u and v, elements with sqrt(nf_u^2 + nf_v^2) > Thr0 set NaN and in-painted
+    by springs together with the NaNs, then the NaNs put back.  Returns (u, v, flagged)."""
+    u = np.array(u, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    hole = np.isnan(u)
+    _, nfu = med_fluct(u)
+    _, nfv = med_fluct(v)
+    mag = np.sqrt(nfu * nfu + nfv * nfv)                   # single, :50-51
+    flagged = mag > np.float32(thr)
+    u2, v2 = u.copy(), v.copy()
+    u2[flagged] = np.nan
+    v2[flagged] = np.nan
+    u2 = inpaint_springs(u2)
+    v2 = inpaint_springs(v2)
+    u2[hole] = np.nan
+    v2[hole] = np.nan
+    return u2, v2, flagged
