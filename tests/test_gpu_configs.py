@@ -169,3 +169,63 @@ def test_quadtree_sequence_seeded(gpu_ctx, speckle, orc):
         assert np.sqrt(np.mean(err ** 2)) < 0.05, (k, np.sqrt(np.mean(err ** 2)))
         prev = r["U"].copy()
     gpu_ctx.set_images(p.f, p.g)
+
+
+def test_reference_quadtree_flow_plate_with_hole(gpu_ctx, speckle, orc):
+    """main_ALDIC_Quadtree.m:23-33 and :64-118 on a plate with a round hole: the device
+    integer search with the nodes on masked pixels emptied and the median outlier test,
+    Init's in-painting, the uniform mesh refined at the hole by GenerateQuadtreeMesh (4 px
+    elements at the rim, hole-edge Dirichlet nodes, unused nodes inside the hole kept) with
+    U0 carried to the new nodes, then the device quadtree frame with the reference's
+    uniform 32 px windows against the oracle's restatement from the same U0."""
+    from test_gpu_parity import _qframe_oracle
+    qt = importlib.import_module("2d_aldic_amd.quadtree")
+    al = importlib.import_module("2d_aldic_amd.aldic")
+    n, a = 384, 60.0
+    c = (n + 1) / 2.0
+    ufun = speckle.kirsch_field(c, c, a, 0.01)
+    p = speckle.make_pair(n, n, int(0.05 * n * n), 2.0, ufun, 1.0, 8, seed=7, hole=(c, c, a))
+    yy, xx = np.mgrid[1:n + 1, 1:n + 1]
+    mask = ((xx - c) ** 2 + (yy - c) ** 2 >= a * a).astype(np.uint8)
+    ws, step, s = 32, 16, 8
+    _flush("reference quadtree flow: search")
+    gpu_ctx.set_images(p.f, p.g, mask=mask)
+    r = gpu_ctx.integer_search([1, n, 1, n], ws, step, [s, s])
+    x, y, u, v = r["x"], r["y"], r["u"].copy(), r["v"].copy()
+    hole = mask[y.astype(int) - 1, x.astype(int) - 1] == 0
+    u[hole] = np.nan
+    v[hole] = np.nan
+    u, v, _ = gpu_ctx.remove_outliers(u, v, 100)
+    u, v = gpu_ctx.inpaint_nans(u, 4), gpu_ctx.inpaint_nans(v, 4)
+    mesh = al.MeshSetUp(x, y, dict(ImgSize=(n, n)))
+    mesh["elementMinSize"] = 2
+    U0u = np.column_stack([u.ravel(), v.ravel()]).ravel()
+    out, U0 = qt.GenerateQuadtreeMesh(mesh, mask.T, dict(winsize=ws, winstepsize=step), U0u)
+    xy = out["coordinatesFEM"]
+    el = (out["elementsFEM"] - 1).astype(np.int32)
+    dr = (out["dirichlet"] - 1).astype(np.int32)
+    _flush(f"reference quadtree flow: {len(xy)} nodes, {len(el)} elements, {len(dr)} Dirichlet")
+    gpu_ctx.set_quadtree(xy, el, dr)
+    _, _, gxy = gpu_ctx.nodal_strain_quadtree(U0)
+    tri, w = qt.linear_weights(gxy, xy)
+    mu = 1e-3
+    beta = 1e-3 * step ** 2 * mu
+    wsn = np.full(len(xy), ws, np.int32)
+    gpu_ctx.qframe_setup(wsn, tri, w, mu=mu, beta=beta, n_admm=3, cg_rtol=1e-12)
+    gpu_ctx.qframe_run(U0)
+    rq = gpu_ctx.qframe_get()
+    f, _ = orc.normalize(p.f.astype(np.float64))
+    g, _ = orc.normalize(p.g.astype(np.float64))
+    gx, gy = orc.gradient(f, mask)
+    o = _qframe_oracle(orc, f, g, gx, gy, xy, el, dr, wsn, tri, w, U0, beta, mu, 3)
+    assert np.mean(rq["iters6"] == o["iters6"]) >= 0.97
+    dU = np.abs(rq["U"] - o["U"])
+    _flush(f"reference quadtree flow: rms dU {np.sqrt(np.mean(dU ** 2)):.2e} max {dU.max():.2e}")
+    assert np.sqrt(np.mean(dU ** 2)) <= 2e-3, (np.sqrt(np.mean(dU ** 2)), dU.max())
+    used = np.zeros(len(xy), bool)
+    used[el[el >= 0]] = True
+    far = used & (np.hypot(xy[:, 0] - c, xy[:, 1] - c) > a + 16)
+    ut, vt = ufun(xy[far, 0], xy[far, 1])
+    err = np.hypot(rq["U"][0::2][far] - ut, rq["U"][1::2][far] - vt)
+    assert np.sqrt(np.mean(err ** 2)) < 0.05, np.sqrt(np.mean(err ** 2))
+    gpu_ctx.set_images(p.f, p.g)
