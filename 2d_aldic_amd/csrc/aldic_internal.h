@@ -52,7 +52,7 @@ struct DBuf {
 struct NatGrid {
     int nn = 0, BX = 0, BY = 0, R = 0;
     double ox = 0, oy = 0, h = 1;
-    DBuf bstart, bnode, rstart, rnode, work;
+    DBuf bstart, bnode, rstart, rnode, work, big;  // big: second-pass Sibson polygons
 };
 hipError_t natgrid_setup(NatGrid &g, const double *coords_host, int nn, hipStream_t st);
 // LocalICGN.m:141-161 / Subpb1Quadtree.m:188-199 refill in place: bad[i] != 0 nodes of U

@@ -19,13 +19,15 @@
 //   along the two hull edges.  Fewer than three non-collinear good nodes: the nearest
 //   good node (ties to the lowest index); none: NaN.
 //
-// Kernels (one refill = 5 launches, no host wait):
+// Kernels (one refill = 6 launches, no host wait):
 //   k_nat_rows      one wave per row of equal y: first / last good node, good count
 //   k_nat_hull      one workgroup: good count, the row extremes pruned by their
 //                   eight-direction extreme polygon, strict convex hull of the rest
 //                   (monotone chain), counters cleared
 //   k_nat_classify  per bad node: inside the hull -> inside list, else -> hull list
-//   k_nat_sibson    one thread per inside node, its two polygons in LDS
+//   k_nat_sibson    one thread per inside node, its two polygons in LDS (24 lines);
+//                   a cell with more natural neighbours goes to a list for
+//   k_nat_sibson_big the same with polygons of up to 512 lines in device memory
 //   k_nat_hullfill  one thread per hull-list node, scans over rings of buckets
 #include <algorithm>
 #include <cmath>
@@ -38,8 +40,10 @@ namespace aldic {
 
 namespace {
 
-constexpr int kNatMax = 24;        // lines per polygon
+constexpr int kNatMax = 24;        // lines per polygon in LDS (first pass)
 constexpr int kNatThreads = 64;    // k_nat_sibson workgroup
+constexpr int kNatBig = 512;       // lines per polygon in the second pass (device memory)
+constexpr int kNatBigThreads = 1024;  // second-pass threads (polygon slots)
 constexpr int kRegion = 0x40000000;  // tag bit: bisector of p_i and p_(tag & ~kRegion)
 
 struct NatArgs {
@@ -59,9 +63,13 @@ struct NatArgs {
     int *rows;       // 3 R: first, last, count
     int *chain;      // 6 R + 4: row extremes, then the chain stack
     int *hull;       // 2 R + 2 (counter-clockwise, from the lowest-x, lowest-y vertex)
-    int *ctl;        // {hull size, good count, n inside, n hull, errors}
-    int *lin, *lhull;  // nn each
+    int *ctl;        // {hull size, good count, n inside, n hull, errors, n big}
+    int *lin, *lhull, *lbig;  // nn each
     int *err;        // nullable: refills whose cell could not be formed are also counted here
+    // second pass: kNatBigThreads polygon pairs of kNatBig lines, entry k of slot g at
+    // [k * kNatBigThreads + g]
+    int *btag;
+    double *bvx, *bvy;
 };
 
 __device__ __forceinline__ void nat_error(const NatArgs &a)
@@ -224,6 +232,7 @@ __global__ __launch_bounds__(256) void k_nat_hull(NatArgs a)
     a.ctl[2] = 0;
     a.ctl[3] = 0;
     a.ctl[4] = 0;
+    a.ctl[5] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_nat_classify(NatArgs a)
@@ -245,8 +254,9 @@ __global__ __launch_bounds__(256) void k_nat_classify(NatArgs a)
 
 // ---- per-thread polygons in LDS ------------------------------------------------------
 struct Poly {
-    int *tag;        // [kNatMax][kNatThreads], this thread's column
+    int *tag;        // entry k at [k * st]: a column of [lines][threads]
     double *vx, *vy;
+    int st;
 };
 
 struct Line {
@@ -286,30 +296,36 @@ __device__ __forceinline__ void nat_meet(const Line &l1, const Line &l2, double 
     y = (l1.nx * l2.c - l2.nx * l1.c) / det;
 }
 
-#define PT(P, k) (P).tag[(k) * kNatThreads]
-#define PX(P, k) (P).vx[(k) * kNatThreads]
-#define PY(P, k) (P).vy[(k) * kNatThreads]
+#define PT(P, k) (P).tag[(k) * (P).st]
+#define PX(P, k) (P).vx[(k) * (P).st]
+#define PY(P, k) (P).vy[(k) * (P).st]
 
-// Clip P (m entries) with line M (tag mt): keep M.n . x <= M.c.  Returns the new size,
-// 0 when nothing is left, -1 on overflow.
+// Clip P (m entries, room for maxm) with line M (tag mt): keep M.n . x <= M.c.  Returns
+// the new size, 0 when nothing is left, -1 when the vertices outside M do not form one
+// run (the polygon lost its convexity numerically), -2 when maxm entries are too few.
 __device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly P, int m,
-                        const Line &M, int mt)
+                        int maxm, const Line &M, int mt)
 {
-    unsigned mask = 0u;
-    for (int k = 0; k < m; ++k) {
+    auto outside = [&](int k) {
         const double u = M.nx * PX(P, k), v = M.ny * PY(P, k);
         const double d = u + v - M.c;
-        if (d > 1e-12 * (fabs(u) + fabs(v) + fabs(M.c))) mask |= 1u << k;
+        return d > 1e-12 * (fabs(u) + fabs(v) + fabs(M.c));
+    };
+    int len = 0, aa = -1;
+    bool prev = outside(m - 1);
+    for (int k = 0; k < m; ++k) {
+        const bool o = outside(k);
+        if (o) {
+            ++len;
+            if (!prev && aa < 0) aa = k;  // first vertex of the outside run
+        }
+        prev = o;
     }
-    if (!mask) return m;
-    const int len = __popc(mask);
+    if (len == 0) return m;
     if (len == m) return 0;
-    const unsigned full = m == 32 ? 0xffffffffu : ((1u << m) - 1u);
-    const unsigned prev = ((mask << 1) | (mask >> (m - 1))) & full;  // bit k: out[k - 1]
-    const int aa = __ffs(mask & ~prev) - 1;                          // first of the run
     int run = 0;
-    while ((mask >> ((aa + run) % m)) & 1u) ++run;
-    if (run != len) return -1;  // not one run: the polygon lost its convexity numerically
+    while (run < m && outside((aa + run) % m)) ++run;
+    if (run != len) return -1;
     // delete entries aa+1 .. aa+len-1 (cyclic), insert M after aa
     int ins;  // slot of M
     int nm;
@@ -332,7 +348,7 @@ __device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly 
             }
         }
     } else {
-        if (m + 1 > kNatMax) return -1;
+        if (m + 1 > maxm) return -2;
         for (int k = m; k > aa + 1; --k) {
             PT(P, k) = PT(P, k - 1); PX(P, k) = PX(P, k - 1); PY(P, k) = PY(P, k - 1);
         }
@@ -371,106 +387,134 @@ __device__ double nat_area(Poly P, int m)
     return 0.5 * s;
 }
 
+// Sibson weights of one inside node q with polygons V (the cell of q) and R (the part
+// of it taken from one natural neighbour), maxm lines each.  Returns 0 (values written),
+// -1 (the cell could not be formed) or -2 (maxm lines are too few; nothing written).
+__device__ int nat_sibson_node(const NatArgs &a, int q, Poly V, Poly R, int maxm)
+{
+    const double2 Q = a.xy[q];
+    const double B = 1e6 * (a.h * (a.BX + a.BY) + 1.0);
+    const double2 zero = make_double2(0.0, 0.0);
+    // initial square: lines x <= B, y <= B, -x <= B, -y <= B (counter-clockwise)
+    int m = 4;
+    for (int k = 0; k < 4; ++k) PT(V, k) = -1 - k;
+    for (int k = 0; k < 4; ++k) {
+        double x, y;
+        nat_meet(nat_line(a, Q, B, -1 - k, zero), nat_line(a, Q, B, -1 - ((k + 1) & 3), zero),
+                 x, y);
+        PX(V, k) = x; PY(V, k) = y;
+    }
+    double rho = nat_radius(V, m);
+    int bx = (int)floor((Q.x - a.ox) / a.h), by = (int)floor((Q.y - a.oy) / a.h);
+    bx = bx < 0 ? 0 : (bx >= a.BX ? a.BX - 1 : bx);
+    by = by < 0 ? 0 : (by >= a.BY ? a.BY - 1 : by);
+    const int rmax = max(max(bx, a.BX - 1 - bx), max(by, a.BY - 1 - by));
+    int fail = 0;
+    // the good nodes of bucket (i, j) cut V(q)
+    auto bucket = [&](int i, int j) {
+        if (fail || i < 0 || i >= a.BX || j < 0 || j >= a.BY) return;
+        const int b = j * a.BX + i;
+        for (int s = a.bstart[b]; s < a.bstart[b + 1]; ++s) {
+            const int p = a.bnode[s];
+            if (p == q || a.bad[p]) continue;
+            const double2 P = a.xy[p];
+            const double px = P.x - Q.x, py = P.y - Q.y;
+            if (px * px + py * py >= 4.0 * rho * rho) continue;  // cannot reach the cell
+            const Line M{px, py, 0.5 * (px * px + py * py)};
+            m = nat_clip(a, Q, B, zero, V, m, maxm, M, p);
+            if (m <= 0) { fail = m == -2 ? -2 : -1; return; }
+            rho = nat_radius(V, m);
+        }
+    };
+    for (int r = 0; r <= rmax && !fail; ++r) {
+        // rings 0 .. r-1 done: every node left is more than (r - 1) h from q
+        if (r > 0 && (double)(r - 1) * a.h >= 2.0 * rho) break;
+        if (r == 0) {
+            bucket(bx, by);
+            continue;
+        }
+        for (int i = bx - r; i <= bx + r; ++i) { bucket(i, by - r); bucket(i, by + r); }
+        for (int j = by - r + 1; j <= by + r - 1; ++j) { bucket(bx - r, j); bucket(bx + r, j); }
+    }
+    if (fail == -2) return -2;
+    for (int k = 0; k < m && !fail; ++k)
+        if (PT(V, k) < 0) fail = -1;
+    double2 u = make_double2(NAN, NAN);
+    double4 f = make_double4(NAN, NAN, NAN, NAN);
+    if (!fail) {
+        double tot = 0.0;
+        double2 su = make_double2(0.0, 0.0);
+        double4 sf = make_double4(0.0, 0.0, 0.0, 0.0);
+        for (int k = 0; k < m && !fail; ++k) {
+            const int ti = PT(V, k);
+            const double2 Pi = a.xy[ti];
+            const double2 pi = make_double2(Pi.x - Q.x, Pi.y - Q.y);
+            for (int l = 0; l < m; ++l) {
+                PT(R, l) = PT(V, l); PX(R, l) = PX(V, l); PY(R, l) = PY(V, l);
+            }
+            int mr = m;
+            for (int l = 0; l < m && mr > 0; ++l) {
+                if (l == k) continue;
+                const int tj = PT(V, l);
+                const double2 Pj = a.xy[tj];
+                const double pjx = Pj.x - Q.x, pjy = Pj.y - Q.y;
+                const Line M{pjx - pi.x, pjy - pi.y,
+                             0.5 * ((pjx * pjx + pjy * pjy) - (pi.x * pi.x + pi.y * pi.y))};
+                mr = nat_clip(a, Q, B, pi, R, mr, maxm, M, tj | kRegion);
+                if (mr < 0) { fail = mr; break; }
+            }
+            if (fail) break;
+            const double w = mr > 0 ? nat_area(R, mr) : 0.0;
+            tot += w;
+            const double2 uv = a.U[ti];
+            su.x += w * uv.x; su.y += w * uv.y;
+            if (a.F) {
+                const double4 fv = a.F[ti];
+                sf.x += w * fv.x; sf.y += w * fv.y; sf.z += w * fv.z; sf.w += w * fv.w;
+            }
+        }
+        if (fail == -2) return -2;
+        if (!fail) {
+            const double it = 1.0 / tot;
+            u = make_double2(su.x * it, su.y * it);
+            f = make_double4(sf.x * it, sf.y * it, sf.z * it, sf.w * it);
+        }
+    }
+    a.U[q] = u;
+    if (a.F) a.F[q] = f;
+    return fail;
+}
+
+// First pass: one thread per inside node, polygons of kNatMax lines in LDS; a node whose
+// cell needs more lines goes to the second pass's list.
 __global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
 {
     __shared__ int s_tag[2][kNatMax * kNatThreads];
     __shared__ double s_vx[2][kNatMax * kNatThreads], s_vy[2][kNatMax * kNatThreads];
     const int t = threadIdx.x;
-    Poly V{&s_tag[0][t], &s_vx[0][t], &s_vy[0][t]};
-    Poly R{&s_tag[1][t], &s_vx[1][t], &s_vy[1][t]};
+    const Poly V{&s_tag[0][t], &s_vx[0][t], &s_vy[0][t], kNatThreads};
+    const Poly R{&s_tag[1][t], &s_vx[1][t], &s_vy[1][t], kNatThreads};
     const int n_in = a.ctl[2];
     for (int li = blockIdx.x * kNatThreads + t; li < n_in; li += gridDim.x * kNatThreads) {
         const int q = a.lin[li];
-        const double2 Q = a.xy[q];
-        const double B = 1e6 * (a.h * (a.BX + a.BY) + 1.0);
-        const double2 zero = make_double2(0.0, 0.0);
-        // initial square: lines x <= B, y <= B, -x <= B, -y <= B (counter-clockwise)
-        int m = 4;
-        for (int k = 0; k < 4; ++k) PT(V, k) = -1 - k;
-        for (int k = 0; k < 4; ++k) {
-            double x, y;
-            nat_meet(nat_line(a, Q, B, -1 - k, zero), nat_line(a, Q, B, -1 - ((k + 1) & 3), zero),
-                     x, y);
-            PX(V, k) = x; PY(V, k) = y;
-        }
-        double rho = nat_radius(V, m);
-        int bx = (int)floor((Q.x - a.ox) / a.h), by = (int)floor((Q.y - a.oy) / a.h);
-        bx = bx < 0 ? 0 : (bx >= a.BX ? a.BX - 1 : bx);
-        by = by < 0 ? 0 : (by >= a.BY ? a.BY - 1 : by);
-        const int rmax = max(max(bx, a.BX - 1 - bx), max(by, a.BY - 1 - by));
-        bool fail = false;
-        // the good nodes of bucket (i, j) cut V(q)
-        auto bucket = [&](int i, int j) {
-            if (fail || i < 0 || i >= a.BX || j < 0 || j >= a.BY) return;
-            const int b = j * a.BX + i;
-            for (int s = a.bstart[b]; s < a.bstart[b + 1]; ++s) {
-                const int p = a.bnode[s];
-                if (p == q || a.bad[p]) continue;
-                const double2 P = a.xy[p];
-                const double px = P.x - Q.x, py = P.y - Q.y;
-                if (px * px + py * py >= 4.0 * rho * rho) continue;  // cannot reach the cell
-                const Line M{px, py, 0.5 * (px * px + py * py)};
-                m = nat_clip(a, Q, B, zero, V, m, M, p);
-                if (m <= 0) { fail = true; return; }
-                rho = nat_radius(V, m);
-            }
-        };
-        for (int r = 0; r <= rmax && !fail; ++r) {
-            // rings 0 .. r-1 done: every node left is more than (r - 1) h from q
-            if (r > 0 && (double)(r - 1) * a.h >= 2.0 * rho) break;
-            if (r == 0) {
-                bucket(bx, by);
-                continue;
-            }
-            for (int i = bx - r; i <= bx + r; ++i) { bucket(i, by - r); bucket(i, by + r); }
-            for (int j = by - r + 1; j <= by + r - 1; ++j) { bucket(bx - r, j); bucket(bx + r, j); }
-        }
-        for (int k = 0; k < m && !fail; ++k)
-            if (PT(V, k) < 0) fail = true;
-        double2 u = make_double2(NAN, NAN);
-        double4 f = make_double4(NAN, NAN, NAN, NAN);
-        if (!fail) {
-            double tot = 0.0;
-            double2 su = make_double2(0.0, 0.0);
-            double4 sf = make_double4(0.0, 0.0, 0.0, 0.0);
-            for (int k = 0; k < m && !fail; ++k) {
-                const int ti = PT(V, k);
-                const double2 Pi = a.xy[ti];
-                const double2 pi = make_double2(Pi.x - Q.x, Pi.y - Q.y);
-                for (int l = 0; l < m; ++l) {
-                    PT(R, l) = PT(V, l); PX(R, l) = PX(V, l); PY(R, l) = PY(V, l);
-                }
-                int mr = m;
-                for (int l = 0; l < m && mr > 0; ++l) {
-                    if (l == k) continue;
-                    const int tj = PT(V, l);
-                    const double2 Pj = a.xy[tj];
-                    const double pjx = Pj.x - Q.x, pjy = Pj.y - Q.y;
-                    const Line M{pjx - pi.x, pjy - pi.y,
-                                 0.5 * ((pjx * pjx + pjy * pjy) - (pi.x * pi.x + pi.y * pi.y))};
-                    mr = nat_clip(a, Q, B, pi, R, mr, M, tj | kRegion);
-                    if (mr < 0) { fail = true; break; }
-                }
-                if (fail) break;
-                const double w = mr > 0 ? nat_area(R, mr) : 0.0;
-                tot += w;
-                const double2 uv = a.U[ti];
-                su.x += w * uv.x; su.y += w * uv.y;
-                if (a.F) {
-                    const double4 fv = a.F[ti];
-                    sf.x += w * fv.x; sf.y += w * fv.y; sf.z += w * fv.z; sf.w += w * fv.w;
-                }
-            }
-            if (!fail) {
-                const double it = 1.0 / tot;
-                u = make_double2(su.x * it, su.y * it);
-                f = make_double4(sf.x * it, sf.y * it, sf.z * it, sf.w * it);
-            }
-        }
-        if (fail) nat_error(a);
-        a.U[q] = u;
-        if (a.F) a.F[q] = f;
+        const int r = nat_sibson_node(a, q, V, R, kNatMax);
+        if (r == -2) a.lbig[atomicAdd(&a.ctl[5], 1)] = q;
+        else if (r < 0) nat_error(a);
     }
+}
+
+// Second pass over the nodes with more than kNatMax natural neighbours (cells that reach
+// across a large cluster of bad nodes): kNatBigThreads threads, polygons of kNatBig lines
+// in device memory.
+__global__ __launch_bounds__(64) void k_nat_sibson_big(NatArgs a)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;  // < kNatBigThreads
+    const size_t half = (size_t)kNatBig * kNatBigThreads;
+    const Poly V{a.btag + g, a.bvx + g, a.bvy + g, kNatBigThreads};
+    const Poly R{a.btag + half + g, a.bvx + half + g, a.bvy + half + g, kNatBigThreads};
+    const int nb = a.ctl[5];
+    for (int li = g; li < nb; li += kNatBigThreads)
+        if (nat_sibson_node(a, a.lbig[li], V, R, kNatBig) != 0) nat_error(a);
 }
 
 #undef PT
@@ -728,7 +772,8 @@ static hipError_t natgrid_setup_impl(NatGrid &g, const double *coords, int nn, h
     if ((e = g.bstart.ensure(sizeof(int) * (nb + 1))) || (e = g.bnode.ensure(sizeof(int) * nn)) ||
         (e = g.rstart.ensure(sizeof(int) * (g.R + 1))) || (e = g.rnode.ensure(sizeof(int) * nn)) ||
         (e = g.work.ensure(sizeof(int) * (3 * (size_t)g.R + 6 * (size_t)g.R + 4 +
-                                          2 * (size_t)g.R + 2 + 8 + 2 * (size_t)nn))))
+                                          2 * (size_t)g.R + 2 + 8 + 3 * (size_t)nn))) ||
+        (e = g.big.ensure((sizeof(int) + 2 * sizeof(double)) * 2 * (size_t)kNatBig * kNatBigThreads)))
         return e;
     if ((e = hipMemcpyAsync(g.bstart.p, cnt.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(g.bnode.p, bnode.data(), sizeof(int) * nn, hipMemcpyHostToDevice, st)) ||
@@ -780,12 +825,18 @@ hipError_t natural_fill(const NatGrid &g, const double *coords, const int *bad, 
     a.ctl = a.hull + 2 * (size_t)g.R + 2;
     a.lin = a.ctl + 8;
     a.lhull = a.lin + g.nn;
+    a.lbig = a.lhull + g.nn;
     a.err = err;
+    const size_t nbig = 2 * (size_t)kNatBig * kNatBigThreads;
+    a.bvx = g.big.as<double>();
+    a.bvy = a.bvx + nbig;
+    a.btag = reinterpret_cast<int *>(a.bvy + nbig);
     hipLaunchKernelGGL(k_nat_rows, dim3((g.R + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nat_hull, dim3(1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nat_classify, dim3((g.nn + 255) / 256), dim3(256), 0, st, a);
     const int nbs = std::min((g.nn + kNatThreads - 1) / kNatThreads, 4096);
     hipLaunchKernelGGL(k_nat_sibson, dim3(nbs), dim3(kNatThreads), 0, st, a);
+    hipLaunchKernelGGL(k_nat_sibson_big, dim3(kNatBigThreads / 64), dim3(64), 0, st, a);
     const int nbh = std::min((g.nn + 63) / 64, 1024);
     hipLaunchKernelGGL(k_nat_hullfill, dim3(nbh), dim3(64), 0, st, a);
     return hipGetLastError();

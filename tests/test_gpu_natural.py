@@ -94,3 +94,26 @@ def test_degenerate_sets(gpu_ctx, orc):
     bad = np.ones(6, bool)
     Ud, Fd = gpu_ctx.natural_fill(xy, bad, np.zeros(12), np.zeros(24))
     assert np.all(np.isnan(Ud)) and np.all(np.isnan(Fd))
+
+
+def test_large_bad_clusters_need_the_second_pass(gpu_ctx, orc):
+    """A round hole of bad nodes in a lattice, ringed by a fine band of good nodes (a
+    quadtree rim): the cells of the hole's nodes reach across it; 308 of them have more
+    than 24 natural neighbours (up to 39), beyond the first pass's 24-line polygons."""
+    rng = np.random.default_rng(5)
+    yy, xx = np.mgrid[0:41, 0:41]
+    xy = np.stack([10.0 + 8 * xx.ravel(), 10.0 + 8 * yy.ravel()], 1).astype(np.float64)
+    c = np.array([170.0, 170.0])
+    t = np.arange(120) * (2 * np.pi / 120)
+    rr = 92.0 + rng.uniform(-1.5, 1.5, 120)
+    rim = c + rr[:, None] * np.stack([np.cos(t), np.sin(t)], 1)
+    xy = np.vstack([xy, rim])
+    d = np.hypot(*(xy - c).T)
+    bad = d < 88.0
+    bad |= rng.random(len(xy)) < 0.03
+    U = rng.normal(size=2 * len(xy))
+    F = rng.normal(scale=0.01, size=4 * len(xy))
+    _flush(f"hole cluster: {bad.sum()} bad of {len(xy)}")
+    assert bad.sum() > 400
+    kind = _check(gpu_ctx, orc, xy, bad, U, F)
+    assert (kind == 1).sum() > 400
