@@ -348,17 +348,17 @@ hipError_t quad_gp_gather(int nn, const int *tri, const double *w, const double 
 hipError_t launch_sub(const double *a, const double *b, double *out, long n, hipStream_t st);
 // out[0] = sum (a - b)^2 (device scalar)
 hipError_t launch_sqdiff(const double *a, const double *b, long n, double *out, hipStream_t st);
+// nbad (nullable, device): the number of statuses < 0 or > 99 is written there.
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
-                          double *part, hipStream_t st);
+                          double *part, hipStream_t st, int *nbad = nullptr);
 // The bad-point rule alone (rule 0: status < 0 or > 99; 1: also > mean + 0.15 std of the
-// good ones; 2: also > max(mean + 0.15 std, 20)); bad U (and F) become NaN.
-// Count of statuses < 0 or > 99 into *out; p[0 .. n) = v (device ints).
-hipError_t launch_count_bad(const int *iters, int S, int *out, hipStream_t st);
-hipError_t launch_iset(int *p, int n, int v, hipStream_t st);
+// good ones; 2: also > max(mean + 0.15 std, 20)); bad U (and F) become NaN; nbad as above.
 hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
                           const double *Fsrc, double *U, double *F, int *bad, double *part,
-                          hipStream_t st);
+                          hipStream_t st, int *nbad = nullptr);
+// p[0 .. n) = v (device ints).
+hipError_t launch_iset(int *p, int n, int v, hipStream_t st);
 // Refill of the bad nodes of an unstructured mesh from the node adjacency (rowptr/col):
 // sweeps over the nodes still unknown, each taking the inverse-distance average of its
 // known neighbours; nodes left unknown after `sweeps` get 0.  known: int[nn] scratch.

@@ -1426,17 +1426,18 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
             ar2.nlist_dev = ar.nlist_dev;
             CK(launch_icgn2(ar2, ppl_big, st));
         }
-        if (step < kMaxSteps) CK(launch_count_bad(F.it2.as<int>(), S, F.nbad.as<int>() + step, st));
+        // the step's count of statuses < 0 or > 99 (Subpb1.m:166-169) comes with the rule
+        int *nbad_step = step < kMaxSteps ? F.nbad.as<int>() + step : nullptr;
         if (!o.no_subpb1_fit) {
             // Subpb1.m:165-199: bad nodes (status < 0 or > 99) carry no datum; gridfit of u
             // and v over the good nodes, written to every node.  The axis fill of the bad
             // nodes is only the CG's starting guess.
             CK(launch_badfill(F.it2.as<int>(), S, F.Nx, F.Ny, 0, nullptr, nullptr, U1, nullptr,
-                              F.bad.as<int>(), F.part.as<double>(), st));
+                              F.bad.as<int>(), F.part.as<double>(), st, nbad_step));
             CK(gfit(U1, 1, F.bad.as<int>()));
         } else {
             CK(launch_badmark(F.it2.as<int>(), S, 0, nullptr, nullptr, U1, nullptr,
-                              F.bad.as<int>(), F.part.as<double>(), st));
+                              F.bad.as<int>(), F.part.as<double>(), st, nbad_step));
             CK(natural_fill(F.nat, F.coords.as<double>(), F.bad.as<int>(), U1, nullptr, st,
                             F.nat_err.as<int>()));
         }

@@ -842,7 +842,8 @@ __global__ __launch_bounds__(256) void k_itstats(const int *it, int S, double *p
 // mean_rule 2 is the quadtree Subpb1's max(mean + 0.15 std, 20).
 __global__ __launch_bounds__(256) void k_mark(const int *it, int S, const double *part, int np,
                                               int mean_rule, const double *Usrc,
-                                              const double *Fsrc, double *U, double *F, int *bad)
+                                              const double *Fsrc, double *U, double *F, int *bad,
+                                              int *nbad)
 {
     __shared__ double tot[3];
     if (threadIdx.x < 64) {
@@ -858,6 +859,8 @@ __global__ __launch_bounds__(256) void k_mark(const int *it, int S, const double
     const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
     double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
     if (mean_rule == 2) thr = thr > 20.0 ? thr : 20.0;  // Subpb1Quadtree.m:180
+    // statuses < 0 or > 99 = all but the counted good ones (Subpb1.m:166-169)
+    if (nbad && blockIdx.x == 0 && threadIdx.x == 0) *nbad = S - (int)C;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x) {
         const int v = it[i];
         const bool b = v < 0 || v > 99 || (mean_rule && C > 0 && (double)v > thr);
@@ -886,7 +889,8 @@ __device__ __forceinline__ bool is_bad(int v, int rule, double C, double thr)
 }
 
 __global__ __launch_bounds__(256) void k_markfill(const int *it, int Nx, int Ny, const double *part,
-                                                  int np, int rule, double *U, double *F, int *bad)
+                                                  int np, int rule, double *U, double *F, int *bad,
+                                                  int *nbad)
 {
     __shared__ double tot[3];
     if (threadIdx.x < 64) {
@@ -901,6 +905,7 @@ __global__ __launch_bounds__(256) void k_markfill(const int *it, int Nx, int Ny,
     const double mean = C > 0 ? Sm / C : 0.0;
     const double var = C > 1 ? (S2 - Sm * mean) / (C - 1) : 0.0;
     const double thr = mean + 0.15 * sqrt(var > 0 ? var : 0.0);
+    if (nbad && blockIdx.x == 0 && threadIdx.x == 0) *nbad = Nx * Ny - (int)C;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nx * Ny) return;
     const bool b = is_bad(it[i], rule, C, thr);
@@ -944,7 +949,7 @@ __global__ __launch_bounds__(256) void k_markfill(const int *it, int Nx, int Ny,
 // In place (U, F hold the values to refill); Usrc / Fsrc must be null.
 hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_rule,
                           const double *Usrc, const double *Fsrc, double *U, double *F, int *bad,
-                          double *part, hipStream_t st)
+                          double *part, hipStream_t st, int *nbad)
 {
     if (S <= 0) return hipSuccess;
     if (Usrc || Fsrc) return hipErrorInvalidValue;
@@ -953,32 +958,13 @@ hipError_t launch_badfill(const int *iters, int S, int Nx, int Ny, int use_mean_
     const int nbf = (S + 255) / 256;
     hipLaunchKernelGGL(k_itstats, dim3(nb), dim3(256), 0, st, iters, S, part);
     hipLaunchKernelGGL(k_markfill, dim3(nbf), dim3(256), 0, st, iters, Nx, Ny,
-                       (const double *)part, nb, use_mean_rule, U, F, bad);
+                       (const double *)part, nb, use_mean_rule, U, F, bad, nbad);
     return hipGetLastError();
-}
-
-// Subpb1.m:166-169: the number of statuses < 0 or > 99 (one workgroup, deterministic).
-__global__ __launch_bounds__(256) void k_count_bad(const int *it, int S, int *out)
-{
-    __shared__ int red[4];
-    int c = 0;
-    for (int i = threadIdx.x; i < S; i += blockDim.x) c += (it[i] < 0 || it[i] > 99) ? 1 : 0;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, kWave);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
 }
 
 __global__ void k_iset(int *p, int n, int v)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
-}
-
-hipError_t launch_count_bad(const int *iters, int S, int *out, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_count_bad, dim3(1), dim3(256), 0, st, iters, S, out);
-    return hipGetLastError();
 }
 
 hipError_t launch_iset(int *p, int n, int v, hipStream_t st)
@@ -989,7 +975,7 @@ hipError_t launch_iset(int *p, int n, int v, hipStream_t st)
 
 hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
                           const double *Fsrc, double *U, double *F, int *bad, double *part,
-                          hipStream_t st)
+                          hipStream_t st, int *nbad)
 {
     if (S <= 0) return hipSuccess;
     int nb = (S + 255) / 256;
@@ -997,7 +983,7 @@ hipError_t launch_badmark(const int *iters, int S, int rule, const double *Usrc,
     const int nbf = (S + 255) / 256;
     hipLaunchKernelGGL(k_itstats, dim3(nb), dim3(256), 0, st, iters, S, part);
     hipLaunchKernelGGL(k_mark, dim3(nbf), dim3(256), 0, st, iters, S, (const double *)part, nb,
-                       rule, Usrc, Fsrc, U, F, bad);
+                       rule, Usrc, Fsrc, U, F, bad, nbad);
     return hipGetLastError();
 }
 

@@ -11,9 +11,9 @@
 // four waves per SIMD to cover the gathers.  The Hessian, its inverse (FP64), mean/norm of f and the constant
 // sums sum J_k and sum J_k f_n are formed once per subset.  Each iteration then makes
 // ONE pass over the pixels: warp, Keys-cubic gather of g (16 loads through L1/L2),
-// and the nine sums  S_g, S_gg, S_{g J_k} (k = 1..6), S_{g f_n}, from which
+// and the nine sums  S_g, S_gg, S_{g J_k} (k = 1..6), S_{g f}, from which
 //   b_k  = bottomf * (S_{J_k f_n} - (S_{g J_k} - gbar S_{J_k}) / bottomg)
-//   zncc = S_{g f_n} / bottomg
+//   zncc = (S_{g f} - fbar S_g) / (bottomf bottomg)
 // follow exactly (the identity sum J_k r with r = f_n - (g - gbar)/bottomg).  The
 // convergence test, the 6x6 solve and the inverse-compositional update run in FP64
 // on every lane with bitwise-identical inputs (butterfly all-reduce), so the loop
@@ -92,50 +92,16 @@ __device__ __forceinline__ void block_sumN(float (&v)[N], float *red, int wid)
     }
 }
 
-// Inverse of a symmetric positive definite 6x6 (row-major, FP64) by Cholesky.
-// Returns false when a pivot is not positive.
-__device__ __forceinline__ bool spd_inverse6(const double *H, double *Hi)
-{
-    double L[21];  // packed lower, row-major
-#define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
-    for (int r = 0; r < 6; ++r) {
-        for (int c = 0; c <= r; ++c) {
-            double s = H[r * 6 + c];
-            for (int k = 0; k < c; ++k) s -= LI(r, k) * LI(c, k);
-            if (r == c) {
-                if (!(s > 0.0)) return false;
-                LI(r, r) = sqrt(s);
-            } else {
-                LI(r, c) = s / LI(c, c);
-            }
-        }
-    }
-    // columns of H^-1: solve L L^T x = e_j
-    for (int j = 0; j < 6; ++j) {
-        double y[6];
-        for (int r = 0; r < 6; ++r) {
-            double s = (r == j) ? 1.0 : 0.0;
-            for (int k = 0; k < r; ++k) s -= LI(r, k) * y[k];
-            y[r] = s / LI(r, r);
-        }
-        for (int r = 5; r >= 0; --r) {
-            double s = y[r];
-            for (int k = r + 1; k < 6; ++k) s -= LI(k, r) * Hi[k * 6 + j];
-            Hi[r * 6 + j] = s / LI(r, r);
-        }
-    }
-#undef LI
-    return true;
-}
-
-// Same from the packed lower triangle h21 (MATLAB column-major order, as in H21),
-// writing H^-1 (row-major) to hi.  Every lane of the caller runs it with identical
-// inputs; the factor stays in registers, H and H^-1 live in LDS.
+// Inverse of a symmetric positive definite 6x6 by Cholesky, from the packed lower
+// triangle h21 (MATLAB column-major order, as in H21), writing H^-1 (row-major) to hi.
+// Returns false when a pivot is not positive.  Every lane of the caller runs it with
+// identical inputs; the factor stays in registers, H and H^-1 live in LDS.  The six
+// pivots are inverted once and every later division is a product.
 __device__ __forceinline__ bool spd_inverse6_lds(const double *h21, double *hi)
 {
     // column-major packed index of (r, c), r >= c
     auto hp = [&](int r, int c) { return h21[c * 6 - c * (c - 1) / 2 + (r - c)]; };
-    double L[21];
+    double L[21], id[6];
 #define LI(r, c) L[(r) * ((r) + 1) / 2 + (c)]
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
@@ -147,8 +113,9 @@ __device__ __forceinline__ bool spd_inverse6_lds(const double *h21, double *hi)
             if (r == c) {
                 if (!(s > 0.0)) return false;
                 LI(r, r) = sqrt(s);
+                id[r] = 1.0 / LI(r, r);
             } else {
-                LI(r, c) = s / LI(c, c);
+                LI(r, c) = s * id[c];
             }
         }
     }
@@ -160,7 +127,7 @@ __device__ __forceinline__ bool spd_inverse6_lds(const double *h21, double *hi)
             double s = (r == j) ? 1.0 : 0.0;
 #pragma unroll
             for (int k = 0; k < r; ++k) s -= LI(r, k) * y[k];
-            y[r] = s / LI(r, r);
+            y[r] = s * id[r];
         }
         double xcol[6];
 #pragma unroll
@@ -168,7 +135,7 @@ __device__ __forceinline__ bool spd_inverse6_lds(const double *h21, double *hi)
             double s = y[r];
 #pragma unroll
             for (int k = r + 1; k < 6; ++k) s -= LI(k, r) * xcol[k];
-            xcol[r] = s / LI(r, r);
+            xcol[r] = s * id[r];
         }
 #pragma unroll
         for (int r = 0; r < 6; ++r) hi[r * 6 + j] = xcol[r];
@@ -182,12 +149,13 @@ __device__ __forceinline__ bool ic_compose(double P[6], const double D[6])
 {
     const double det = (1.0 + D[0]) * (1.0 + D[3]) - D[1] * D[2];
     if (det == 0.0) return false;
-    const double t1 = (-D[0] - D[0] * D[3] + D[1] * D[2]) / det;
-    const double t2 = -D[1] / det;
-    const double t3 = -D[2] / det;
-    const double t4 = (-D[3] - D[0] * D[3] + D[1] * D[2]) / det;
-    const double t5 = (-D[4] - D[3] * D[4] + D[2] * D[5]) / det;
-    const double t6 = (-D[5] - D[0] * D[5] + D[1] * D[4]) / det;
+    const double id = 1.0 / det;  // one division, six products
+    const double t1 = (-D[0] - D[0] * D[3] + D[1] * D[2]) * id;
+    const double t2 = -D[1] * id;
+    const double t3 = -D[2] * id;
+    const double t4 = (-D[3] - D[0] * D[3] + D[1] * D[2]) * id;
+    const double t5 = (-D[4] - D[3] * D[4] + D[2] * D[5]) * id;
+    const double t6 = (-D[5] - D[0] * D[5] + D[1] * D[4]) * id;
     const double a11 = 1.0 + P[0], a12 = P[2], a13 = P[4];
     const double a21 = P[1], a22 = 1.0 + P[3], a23 = P[5];
     P[0] = a11 * (1.0 + t1) + a12 * t2 - 1.0;
@@ -199,76 +167,57 @@ __device__ __forceinline__ bool ic_compose(double P[6], const double D[6])
     return true;
 }
 
-// Keys-cubic sample of g at window pixel `off` (0-based image index of the pixel at
-// the integer position X, Y) displaced by (ax, ay); the sample is known to lie in
-// [3, nx-2] x [3, ny-2] (1-based), so every tap is inside the image.  rows[b] is the
-// uniform byte address of g(-1 + b*nx - nx) -- the tap rows' bases -- so that every
-// tap row is one load at the same 32-bit per-lane byte offset.  The weights are kept
-// doubled (2 w, exact in binary) and the result scaled by 1/4 at the end.
-__device__ __forceinline__ float sample_g(const char *const (&rows)[4], int nx, int off, float ax,
-                                          float ay)
+// Keys-cubic weights (a = -1/2, ba_interp2.cpp:153-162) of the fraction t, doubled
+// (2 w, integer coefficients): w3 = t^3 - t^2, w0 = t^2 - t - w3, w1 = 2 - 2 t^2 + 3 w3,
+// w2 = t + t^2 - 3 w3 (8 VALU).  A sample from doubled weights on both axes is 4 g.
+__device__ __forceinline__ void keys_w2(float t, float (&w)[4])
 {
-    const float tx = __builtin_amdgcn_fractf(ax), ty = __builtin_amdgcn_fractf(ay);
-    const int ix = (int)floorf(ax), iy = (int)floorf(ay);
-    const float tx2 = tx * tx, tx3 = tx2 * tx, ty2 = ty * ty, ty3 = ty2 * ty;
-    const float wx0 = -tx + 2.f * tx2 - tx3, wx1 = 2.f - 5.f * tx2 + 3.f * tx3;
-    const float wx2 = tx + 4.f * tx2 - 3.f * tx3, wx3 = tx3 - tx2;
-    const float wy0 = -ty + 2.f * ty2 - ty3, wy1 = 2.f - 5.f * ty2 + 3.f * ty3;
-    const float wy2 = ty + 4.f * ty2 - 3.f * ty3, wy3 = ty3 - ty2;
-    // images hold < 2^29 pixels: the byte offset fits in 32 bits
-    const unsigned ob = 4u * (unsigned)(off + ix + __mul24(iy, nx));
-    float r[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const float *row = reinterpret_cast<const float *>(rows[b] + ob);
-        r[b] = wx0 * row[0] + wx1 * row[1] + wx2 * row[2] + wx3 * row[3];
-    }
-    return 0.25f * (wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3]);
+    const float t2 = t * t;
+    const float w3 = fmaf(t2, t, -t2);
+    w[0] = (t2 - t) - w3;
+    w[1] = fmaf(3.f, w3, fmaf(-2.f, t2, 2.f));
+    w[2] = fmaf(-3.f, w3, t + t2);
+    w[3] = w3;
 }
 
-// One pixel of an IC-GN pass: warp its window offset, sample g, accumulate the sums
-// (S_g, S_gg, then S_{g J_k} and S_{g f_n}).  `in` = false counts nothing.
-template <bool SIX, int NS>
-__device__ __forceinline__ void icgn_pixel(const uint2 e, const char *const (&rows)[4], int nx,
-                                           float p0, float p1, float p2, float p3, float p4,
-                                           float p5, float fn, float fx, float fy, bool in,
-                                           float (&acc)[NS])
+// floor(x) as an integer in one instruction (x finite and well inside the int range)
+__device__ __forceinline__ int floor_i(float x)
 {
-    const __half2 h = *reinterpret_cast<const __half2 *>(&e.x);
-    const float dx = __low2float(h), dy = __high2float(h);
-    const float ux = p0 * dx + p2 * dy + p4;
-    const float uy = p1 * dx + p3 * dy + p5;
-    float gv = sample_g(rows, nx, (int)e.y, ux, uy);
-    gv = in ? gv : 0.f;
-    const float t1 = gv * fx, t2 = gv * fy;
-    acc[0] += gv;
-    acc[1] += gv * gv;
-    if constexpr (SIX) {
-        acc[2] += t1 * dx; acc[3] += t2 * dx;
-        acc[4] += t1 * dy; acc[5] += t2 * dy;
-        acc[6] += t1; acc[7] += t2;
-        acc[8] += gv * fn;
-    } else {
-        acc[2] += t1; acc[3] += t2;
-        acc[4] += gv * fn;
-    }
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
-// The same split in stages for an explicit software pipeline: icgn_addr warps the slot's
-// offset and returns the byte offset of its first tap row (and the fractional parts),
-// icgn_taps loads the 16 taps, icgn_accum interpolates and accumulates.
-__device__ __forceinline__ unsigned icgn_addr(const uint2 e, int nx, float p0, float p1, float p2,
+__device__ __forceinline__ float ld_byte(const float *p, unsigned off)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + off);
+}
+
+// One slot of an IC-GN pass, in stages for an explicit software pipeline.  The slot's
+// table entry e holds its offsets (dx, dy) from the window centre as two halves and the
+// byte offset of its pixel.  icgn_addr warps the offsets, u = p0 dx + p2 dy + p4,
+// v = p1 dx + p3 dy + p5, and returns the byte offset of the sample's first tap row
+// relative to rows[] (rows[b] = g - 4 (1 + nx) + 4 b nx: every tap row is one load at the
+// same per-lane offset; the sample is known to lie in [3, nx-2] x [3, ny-2], 1-based, so
+// every tap is inside the image) with the fractional parts; icgn_taps loads the 16 taps;
+// icgn_accum interpolates and accumulates.
+__device__ __forceinline__ unsigned icgn_addr(const uint2 e, int nx4, float p0, float p1, float p2,
                                               float p3, float p4, float p5, float &dx, float &dy,
                                               float &tx, float &ty)
 {
     const __half2 h = *reinterpret_cast<const __half2 *>(&e.x);
     dx = __low2float(h);
     dy = __high2float(h);
-    const float ux = p0 * dx + p2 * dy + p4;
-    const float uy = p1 * dx + p3 * dy + p5;
+    const float ux = fmaf(p0, dx, fmaf(p2, dy, p4));
+    const float uy = fmaf(p1, dx, fmaf(p3, dy, p5));
     tx = __builtin_amdgcn_fractf(ux);
     ty = __builtin_amdgcn_fractf(uy);
-    return 4u * (unsigned)((int)e.y + (int)floorf(ux) + __mul24((int)floorf(uy), nx));
+    // images hold < 2^29 pixels and 4 nx < 2^23: byte offsets fit in 32 bits (arithmetic
+    // mod 2^32), the row term in a 24-bit multiply; two instructions
+    unsigned ob;
+    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ob) : "v"(floor_i(ux)), "v"(e.y));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(ob) : "v"(floor_i(uy)), "s"(nx4), "v"(ob));
+    return ob;
 }
 
 __device__ __forceinline__ void icgn_taps(const char *const (&rows)[4], unsigned ob, float (&v)[16])
@@ -281,21 +230,23 @@ __device__ __forceinline__ void icgn_taps(const char *const (&rows)[4], unsigned
     }
 }
 
+// Accumulates, for a sample g (taken as 4 g from the doubled weights; the sums are
+// scaled back by powers of two after the pass, exactly): S_g, S_gg, then S_{g J_k} with
+// J = (fx dx, fy dx, fx dy, fy dy, fx, fy) and S_{g f} (6-DOF), or S_{g fx}, S_{g fy},
+// S_{g f} (2-DOF).  `in` = false counts nothing.
 template <bool SIX, int NS>
 __device__ __forceinline__ void icgn_accum(const float (&v)[16], float tx, float ty, float dx,
-                                           float dy, float fn, float fx, float fy, bool in,
+                                           float dy, float f, float fx, float fy, bool in,
                                            float (&acc)[NS])
 {
-    const float tx2 = tx * tx, tx3 = tx2 * tx, ty2 = ty * ty, ty3 = ty2 * ty;
-    const float wx0 = -tx + 2.f * tx2 - tx3, wx1 = 2.f - 5.f * tx2 + 3.f * tx3;
-    const float wx2 = tx + 4.f * tx2 - 3.f * tx3, wx3 = tx3 - tx2;
-    const float wy0 = -ty + 2.f * ty2 - ty3, wy1 = 2.f - 5.f * ty2 + 3.f * ty3;
-    const float wy2 = ty + 4.f * ty2 - 3.f * ty3, wy3 = ty3 - ty2;
+    float wx[4], wy[4];
+    keys_w2(tx, wx);
+    keys_w2(ty, wy);
     float r[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
-        r[b] = wx0 * v[4 * b] + wx1 * v[4 * b + 1] + wx2 * v[4 * b + 2] + wx3 * v[4 * b + 3];
-    float gv = 0.25f * (wy0 * r[0] + wy1 * r[1] + wy2 * r[2] + wy3 * r[3]);
+        r[b] = wx[0] * v[4 * b] + wx[1] * v[4 * b + 1] + wx[2] * v[4 * b + 2] + wx[3] * v[4 * b + 3];
+    float gv = wy[0] * r[0] + wy[1] * r[1] + wy[2] * r[2] + wy[3] * r[3];
     gv = in ? gv : 0.f;
     const float t1 = gv * fx, t2 = gv * fy;
     acc[0] += gv;
@@ -304,11 +255,24 @@ __device__ __forceinline__ void icgn_accum(const float (&v)[16], float tx, float
         acc[2] += t1 * dx; acc[3] += t2 * dx;
         acc[4] += t1 * dy; acc[5] += t2 * dy;
         acc[6] += t1; acc[7] += t2;
-        acc[8] += gv * fn;
+        acc[8] += gv * f;
     } else {
         acc[2] += t1; acc[3] += t2;
-        acc[4] += gv * fn;
+        acc[4] += gv * f;
     }
+}
+
+// One slot without the pipeline.
+template <bool SIX, int NS>
+__device__ __forceinline__ void icgn_pixel(const uint2 e, const char *const (&rows)[4], int nx4,
+                                           float p0, float p1, float p2, float p3, float p4,
+                                           float p5, float f, float fx, float fy,
+                                           float (&acc)[NS])
+{
+    float dx, dy, tx, ty, v[16];
+    const unsigned ob = icgn_addr(e, nx4, p0, p1, p2, p3, p4, p5, dx, dy, tx, ty);
+    icgn_taps(rows, ob, v);
+    icgn_accum<SIX, NS>(v, tx, ty, dx, dy, f, fx, fy, true, acc);
 }
 
 // Shared body for both modes.  SIX = true: 6-DOF funICGN; false: 2-DOF funICGN_Subpb1.
@@ -320,15 +284,16 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     constexpr int NCS = 2 + 6 + 6 + (SIX ? 21 : 0);
     __shared__ double s_hinv[SPB][36];
     __shared__ double s_cst[SPB][12];
-    // per subset: the warp P[0..5], then bottomf, bottomf^2 and the 2-DOF system t00,
-    // t01, t11.  The Gauss-Newton loop re-reads them after each pass, so no FP64 value
-    // is held in registers across the pass (which would spill to scratch).
-    __shared__ double s_k[SPB][11];
+    // per subset: the warp P[0..5], then bottomf, bottomf^2, the 2-DOF system t00,
+    // t01, t11, mean f, 1 / bottomf and 1 / n.  The Gauss-Newton loop re-reads them
+    // after each pass, so no FP64 value is held in registers across the pass (which
+    // would spill to scratch).
+    __shared__ double s_k[SPB][14];
     __shared__ double s_cs[SPB][NCS];
     __shared__ float s_red[16 * WPS];
     __shared__ double s_redd[WPS];
     // per pixel slot of every thread: the offsets (dx, dy) from the window centre as
-    // two halves (exact: multiples of 1/2 below 1024) and the pixel's image index.
+    // two halves (exact: multiples of 1/2 below 1024) and the pixel's byte offset.
     // Each thread writes and reads only its own entries.
     __shared__ uint2 s_tab[SPB * PPL * T];
 
@@ -380,20 +345,42 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     };
 
     // ---- window, mask rule and per-subset constants (funICGN.m:34-94) -------------
-    // One unrolled pass loads the window into registers and accumulates, per lane in
-    // FP32 (<= PPL terms), the exact zeros of DfDx, sum f, sum f^2, sum J_k,
-    // sum J_k f and (6-DOF) the 21 Hessian entries; lanes are combined in FP64.
-    // Pixel p = tid + k*T sits at column i, row r (p = r*w1 + i); the walk over k
-    // advances (i, r) by T without divisions.
+    // One unrolled pass over the window's slots accumulates, per lane in FP32 (<= PPL
+    // terms), the exact zeros of DfDx, sum f, sum f^2, sum J_k, sum J_k f and (6-DOF)
+    // the 21 Hessian entries; lanes are combined in FP64.  Pixel p = tid + k*T sits at
+    // column i, row r (p = r*w1 + i); the table's walk over k advances (i, r) by T
+    // without divisions.
+    uint2 *tab = s_tab + sub_in_block * (PPL * T);
     int xa = 0, ya = 0;
+    int w1 = w + 1, n = w1 * w1;
+    // The slot table of the current window: offsets from the centre and the pixel's byte
+    // offset; an empty slot (past the window's last pixel) holds pixel (0, 0).  Every pass
+    // over the window reads it back (f, fx, fy are re-read at each slot's pixel, L1/L2
+    // hits, instead of holding 3 PPL registers: more waves in flight for the gathers).
+    auto build_table = [&]() {
+        const int di = T % w1, dr = T / w1;
+        const float half = 0.5f * (float)w;
+        int i = tid % w1, r = tid / w1;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            const bool in = r < w1;
+            const int ii = in ? i : 0, rr = in ? r : 0;
+            const __half2 h = __floats2half2_rn((float)ii - half, (float)rr - half);
+            uint2 e;
+            e.x = *reinterpret_cast<const unsigned *>(&h);
+            e.y = 4u * (unsigned)((ya - 1 + rr) * nx + (xa - 1 + ii));
+            tab[k * T + tid] = e;
+            i += di; r += dr;
+            if (i >= w1) { i -= w1; ++r; }
+        }
+        // read back from LDS by every later pass, not held in registers
+        asm volatile("" ::: "memory");
+    };
     constexpr int NH = SIX ? 21 : 0;
     constexpr int NC = 2 + 6 + 6 + NH;   // sum f, sum f^2, sum J, sum J f, H
-    float fv[PPL], fxv[PPL], fyv[PPL];
     double *cs = s_cs[sub_in_block];  // reduced sums, same value from every lane
     double *cst = s_cst[sub_in_block];  // [0..5] sum J_k, [6..11] sum J_k f_n
     double *hinv = s_hinv[sub_in_block];
-    int w1 = w + 1, n = w1 * w1;
-    int di = 0, dr = 0, i0 = 0, r0 = 0;
     double meanf = 0.0, ssf = 0.0;
     bool singular = false;
     bool cached = false;
@@ -409,19 +396,7 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
             w1 = w + 1;
             n = w1 * w1;
             if (n > PPL * T) { if (tid == 0) a.iters[j] = -2; return; }
-            di = T % w1; dr = T / w1; i0 = tid % w1; r0 = tid / w1;
-            int i = i0, r = r0;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                if (r < w1) {
-                    const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
-                    fv[k] = a.f[o]; fxv[k] = a.gx[o]; fyv[k] = a.gy[o];
-                } else {
-                    fv[k] = fxv[k] = fyv[k] = 0.f;
-                }
-                i += di; r += dr;
-                if (i >= w1) { i -= w1; ++r; }
-            }
+            build_table();
             meanf = cc[0];
             ssf = cc[1];
             cst[4] = cc[2]; cst[5] = cc[3]; cst[10] = cc[4]; cst[11] = cc[5];
@@ -438,21 +413,20 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
             }
             return;
         }
-        di = T % w1; dr = T / w1; i0 = tid % w1; r0 = tid / w1;
+        build_table();
         float acc[NC];
 #pragma unroll
         for (int q = 0; q < NC; ++q) acc[q] = 0.f;
         float zeros = 0.f;
-        const float half = 0.5f * (float)w;
-        int i = i0, r = r0;
+        const int lim = n - tid;  // slot k holds a pixel iff k * T < lim
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
-            if (r < w1) {
-                const size_t o = (size_t)(ya - 1 + r) * nx + (xa - 1 + i);
-                const float fval = a.f[o], fx = a.gx[o], fy = a.gy[o];
-                fv[k] = fval; fxv[k] = fx; fyv[k] = fy;
+            if (k * T < lim) {
+                const uint2 e = tab[k * T + tid];
+                const __half2 h = *reinterpret_cast<const __half2 *>(&e.x);
+                const float dx = __low2float(h), dy = __high2float(h);
+                const float fval = ld_byte(a.f, e.y), fx = ld_byte(a.gx, e.y), fy = ld_byte(a.gy, e.y);
                 zeros += fx == 0.f ? 1.f : 0.f;
-                const float dx = (float)i - half, dy = (float)r - half;
                 const float J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
                 acc[0] += fval;
                 acc[1] += fval * fval;
@@ -465,11 +439,7 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
 #pragma unroll
                         for (int rr = c; rr < 6; ++rr) acc[q++] += J[rr] * J[c];
                 }
-            } else {
-                fv[k] = fxv[k] = fyv[k] = 0.f;
             }
-            i += di; r += dr;
-            if (i >= w1) { i -= w1; ++r; }
         }
         if (pass == 0) {  // mask rule on the original window
             const float n0 = block_sum1<WPS>(zeros, s_red, wid);
@@ -511,28 +481,6 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     }
     }  // !cached
     const double bottomf = sqrt(ssf);
-    uint2 *tab = s_tab + sub_in_block * (PPL * T);
-    // the passes re-read f, fx, fy at each slot's pixel (L1/L2 hits) instead of holding
-    // 3 PPL registers: fewer registers, more waves in flight to cover the gathers
-    const float mfn = (float)meanf;
-    const float ibn = (float)(1.0 / bottomf);
-    {
-        const float half = 0.5f * (float)w;
-        int i = i0, r = r0;
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-            const bool in = r < w1;
-            const int ii = in ? i : 0, rr = in ? r : 0;  // empty slots: an in-window pixel
-            const __half2 h = __floats2half2_rn((float)ii - half, (float)rr - half);
-            uint2 e;
-            e.x = *reinterpret_cast<const unsigned *>(&h);
-            e.y = (unsigned)((ya - 1 + rr) * nx + (xa - 1 + ii));
-            tab[k * T + tid] = e;
-            i += di; r += dr;
-            if (i >= w1) { i -= w1; ++r; }
-        }
-    }
-
     // 2-DOF system matrix
     double t00 = 0, t01 = 0, t11 = 0;
     const double bf2 = ssf;  // bottomf^2
@@ -551,7 +499,8 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     double *sk = s_k[sub_in_block];  // every lane writes the same values, reads its own
 #pragma unroll
     for (int q = 0; q < 6; ++q) sk[q] = P[q];
-    sk[6] = bottomf; sk[7] = bf2; sk[8] = t00; sk[9] = t01; sk[10] = t11;
+    sk[6] = bottomf; sk[7] = bf2; sk[8] = t00; sk[9] = t01; sk[10] = t11; sk[11] = meanf;
+    sk[12] = 1.0 / bottomf; sk[13] = 1.0 / n;
 
     // ---- Gauss-Newton loop (funICGN.m:107-294 / funICGN_Subpb1.m:94-337) ----------
     const float Xa = (float)xa, Ya = (float)ya;  // window pixel (i, r) is X = Xa + i, Y = Ya + r
@@ -597,6 +546,7 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
             // would otherwise hold all of it in registers)
             asm volatile("" ::: "memory");
             const int lim = n - tid;  // slot k holds a pixel iff k * T < lim
+            const int nx4 = 4 * nx;
             const char *gb = reinterpret_cast<const char *>(a.g) - 4 * (1 + (size_t)nx);
             const char *const rows[4] = {gb, gb + 4 * (size_t)nx, gb + 8 * (size_t)nx,
                                          gb + 12 * (size_t)nx};
@@ -608,33 +558,33 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
                 float v[2][16], sdx[2], sdy[2], stx[2], sty[2], sf[2], sgx[2], sgy[2];
                 {
                     const uint2 e = tab[tid];
-                    const unsigned ob = icgn_addr(e, nx, p0, p1, p2, p3, p4, p5, sdx[0], sdy[0],
+                    const unsigned ob = icgn_addr(e, nx4, p0, p1, p2, p3, p4, p5, sdx[0], sdy[0],
                                                   stx[0], sty[0]);
                     icgn_taps(rows, ob, v[0]);
-                    sf[0] = a.f[e.y]; sgx[0] = a.gx[e.y]; sgy[0] = a.gy[e.y];
+                    sf[0] = ld_byte(a.f, e.y); sgx[0] = ld_byte(a.gx, e.y);
+                    sgy[0] = ld_byte(a.gy, e.y);
                 }
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
                     const int c = k & 1, m = c ^ 1;
                     if (k + 1 < PPL) {
                         const uint2 e = tab[(k + 1) * T + tid];
-                        const unsigned ob = icgn_addr(e, nx, p0, p1, p2, p3, p4, p5, sdx[m],
+                        const unsigned ob = icgn_addr(e, nx4, p0, p1, p2, p3, p4, p5, sdx[m],
                                                       sdy[m], stx[m], sty[m]);
                         icgn_taps(rows, ob, v[m]);
-                        sf[m] = a.f[e.y]; sgx[m] = a.gx[e.y]; sgy[m] = a.gy[e.y];
+                        sf[m] = ld_byte(a.f, e.y); sgx[m] = ld_byte(a.gx, e.y);
+                        sgy[m] = ld_byte(a.gy, e.y);
                     }
-                    icgn_accum<SIX, NS>(v[c], stx[c], sty[c], sdx[c], sdy[c],
-                                        (sf[c] - mfn) * ibn, sgx[c], sgy[c],
-                                        k < PPL - 1 || k * T < lim, acc);
+                    icgn_accum<SIX, NS>(v[c], stx[c], sty[c], sdx[c], sdy[c], sf[c], sgx[c],
+                                        sgy[c], k < PPL - 1 || k * T < lim, acc);
                 }
             } else {
 #pragma unroll
                 for (int k = 0; k < PPL; ++k)
                     if (k * T < lim) {
                         const uint2 e = tab[k * T + tid];
-                        icgn_pixel<SIX, NS>(e, rows, nx, p0, p1, p2, p3, p4, p5,
-                                            (a.f[e.y] - mfn) * ibn, a.gx[e.y], a.gy[e.y], true,
-                                            acc);
+                        icgn_pixel<SIX, NS>(e, rows, nx4, p0, p1, p2, p3, p4, p5, ld_byte(a.f, e.y),
+                                            ld_byte(a.gx, e.y), ld_byte(a.gy, e.y), acc);
                     }
             }
         }
@@ -643,25 +593,29 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
 #pragma unroll
         for (int q = 0; q < 6; ++q) P[q] = sk[q];
         const double bottomf = sk[6], bf2 = sk[7], t00 = sk[8], t01 = sk[9], t11 = sk[10];
-        const double Sg = acc[0], Sgg = acc[1];
-        const double gbar = Sg / n;
+        // the pass summed 4 g: scale back (exact), S_{g f_n} = (S_{g f} - mean f S_g) / bottomf
+        const double Sg = 0.25 * acc[0], Sgg = 0.0625 * acc[1];
+        const double gbar = Sg * sk[13];
         const double vg = Sgg - Sg * gbar;
         const double bottomg = sqrt(vg > 0 ? vg : 0.0);
+        const double ibg = 1.0 / bottomg;
+        const double meanf_k = sk[11];
         double b[6];
         if (SIX) {
 #pragma unroll
             for (int q = 0; q < 6; ++q)
-                b[q] = bottomf * (cst[6 + q] - ((double)acc[2 + q] - gbar * cst[q]) / bottomg);
-            zc = (float)((double)acc[8] / bottomg);
+                b[q] = bottomf * (cst[6 + q] - (0.25 * acc[2 + q] - gbar * cst[q]) * ibg);
+            zc = (float)((0.25 * acc[8] - meanf_k * Sg) * sk[12] * ibg);
             nrmAbs = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2] + b[3] * b[3] + b[4] * b[4] +
                           b[5] * b[5]);
         } else {
-            const double b5 = bottomf * (cst[10] - ((double)acc[2] - gbar * cst[4]) / bottomg);
-            const double b6 = bottomf * (cst[11] - ((double)acc[3] - gbar * cst[5]) / bottomg);
-            zc = (float)((double)acc[4] / bottomg);
+            const double b5 = bottomf * (cst[10] - (0.25 * acc[2] - gbar * cst[4]) * ibg);
+            const double b6 = bottomf * (cst[11] - (0.25 * acc[3] - gbar * cst[5]) * ibg);
+            zc = (float)((0.25 * acc[4] - meanf_k * Sg) * sk[12] * ibg);
             // tempb (funICGN_Subpb1.m:269)
-            b[0] = b5 * 2.0 / bf2 + a.mu * (P[4] - a.Uold[2 * j] - a.vdual[2 * j]);
-            b[1] = b6 * 2.0 / bf2 + a.mu * (P[5] - a.Uold[2 * j + 1] - a.vdual[2 * j + 1]);
+            const double two_bf2 = 2.0 / bf2;
+            b[0] = b5 * two_bf2 + a.mu * (P[4] - a.Uold[2 * j] - a.vdual[2 * j]);
+            b[1] = b6 * two_bf2 + a.mu * (P[5] - a.Uold[2 * j + 1] - a.vdual[2 * j + 1]);
             nrmAbs = sqrt(b[0] * b[0] + b[1] * b[1]);
         }
         nrmNew = nrmAbs;
@@ -687,8 +641,9 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
             const double det = t00 * t11 - t01 * t01;
             D[0] = D[1] = D[2] = D[3] = 0.0;
             if (det == 0.0) { sk[4] = P[4] = NAN; sk[5] = P[5] = NAN; continue; }
-            D[4] = -(t11 * b[0] - t01 * b[1]) / det;
-            D[5] = -(-t01 * b[0] + t00 * b[1]) / det;
+            const double id = 1.0 / det;
+            D[4] = -(t11 * b[0] - t01 * b[1]) * id;
+            D[5] = -(-t01 * b[0] + t00 * b[1]) * id;
         }
         if (!ic_compose(P, D)) break;
 #pragma unroll

@@ -28,7 +28,7 @@
 //   k_nat_sibson    one thread per inside node, its two polygons in LDS (24 lines);
 //                   a cell with more natural neighbours goes to a list for
 //   k_nat_sibson_big the same with polygons of up to 512 lines in device memory
-//   k_nat_hullfill  one thread per hull-list node, scans over rings of buckets
+//   k_nat_hullfill  one wave per hull-list node, scans over rings of buckets
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -235,28 +235,55 @@ __global__ __launch_bounds__(256) void k_nat_hull(NatArgs a)
     a.ctl[5] = 0;
 }
 
+// Inside = strictly left of every hull edge (relative tolerance 1e-12).  The hull is
+// convex and counter-clockwise from v0 = hull[0]: after the two edges at v0, a binary
+// search over the fan from v0 finds the wedge (v0, v_k, v_k+1) holding q, and only the
+// edges around it (k - 1, k, k + 1) can be the one q is not strictly left of.
 __global__ __launch_bounds__(256) void k_nat_classify(NatArgs a)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.nn || !a.bad[q]) return;
+    const bool mine = q < a.nn && a.bad[q];
     const int h = a.ctl[0];
-    bool inside = h >= 3 && a.ctl[1] >= 3;
-    const double2 Q = a.xy[q];
-    for (int e = 0; e < h && inside; ++e) {
+    bool inside = mine && h >= 3 && a.ctl[1] >= 3;
+    const double2 Q = mine ? a.xy[q] : make_double2(0.0, 0.0);
+    auto left = [&](int e) {
         const double2 A = a.xy[a.hull[e]], C = a.xy[a.hull[e + 1 < h ? e + 1 : 0]];
         const double ex = C.x - A.x, ey = C.y - A.y, px = Q.x - A.x, py = Q.y - A.y;
         const double cr = ex * py - ey * px;
-        if (!(cr > 1e-12 * sqrt(ex * ex + ey * ey) * sqrt(px * px + py * py))) inside = false;
+        return cr > 1e-12 * sqrt(ex * ex + ey * ey) * sqrt(px * px + py * py);
+    };
+    if (inside) inside = left(0) && left(h - 1);
+    if (inside) {
+        const double2 V0 = a.xy[a.hull[0]];
+        int lo = 1, hi = h - 1;  // cross(v0, v_lo, q) > 0 >= cross(v0, v_hi, q)
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cross3(V0, a.xy[a.hull[mid]], Q) > 0.0) lo = mid;
+            else hi = mid;
+        }
+        inside = left(lo) && left(lo - 1) && left(lo + 1);
     }
-    if (inside) a.lin[atomicAdd(&a.ctl[2], 1)] = q;
-    else a.lhull[atomicAdd(&a.ctl[3], 1)] = q;
+    // one atomic per wave and list (every lane reaches the ballots)
+    const unsigned long long m_in = __ballot(mine && inside), m_out = __ballot(mine && !inside);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int b_in = 0, b_out = 0;
+    if (lane == 0) {
+        if (m_in) b_in = atomicAdd(&a.ctl[2], __popcll(m_in));
+        if (m_out) b_out = atomicAdd(&a.ctl[3], __popcll(m_out));
+    }
+    b_in = __shfl(b_in, 0, kWave);
+    b_out = __shfl(b_out, 0, kWave);
+    if (!mine) return;
+    if (inside) a.lin[b_in + __popcll(m_in & below)] = q;
+    else a.lhull[b_out + __popcll(m_out & below)] = q;
 }
 
 // ---- per-thread polygons in LDS ------------------------------------------------------
-struct Poly {
-    int *tag;        // entry k at [k * st]: a column of [lines][threads]
+template <int ST>
+struct Poly {        // entry k at [k * ST]: a column of [lines][threads]
+    int *tag;
     double *vx, *vy;
-    int st;
 };
 
 struct Line {
@@ -296,36 +323,54 @@ __device__ __forceinline__ void nat_meet(const Line &l1, const Line &l2, double 
     y = (l1.nx * l2.c - l2.nx * l1.c) / det;
 }
 
-#define PT(P, k) (P).tag[(k) * (P).st]
-#define PX(P, k) (P).vx[(k) * (P).st]
-#define PY(P, k) (P).vy[(k) * (P).st]
+#define PT(P, k) (P).tag[(k) * ST]
+#define PX(P, k) (P).vx[(k) * ST]
+#define PY(P, k) (P).vy[(k) * ST]
 
-// Clip P (m entries, room for maxm) with line M (tag mt): keep M.n . x <= M.c.  Returns
+// Clip P (m entries, room for MAXM) with line M (tag mt): keep M.n . x <= M.c.  Returns
 // the new size, 0 when nothing is left, -1 when the vertices outside M do not form one
-// run (the polygon lost its convexity numerically), -2 when maxm entries are too few.
-__device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly P, int m,
-                        int maxm, const Line &M, int mt)
+// run (the polygon lost its convexity numerically), -2 when MAXM entries are too few.
+template <int ST, int MAXM>
+__device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly<ST> P, int m,
+                        const Line &M, int mt)
 {
     auto outside = [&](int k) {
         const double u = M.nx * PX(P, k), v = M.ny * PY(P, k);
         const double d = u + v - M.c;
         return d > 1e-12 * (fabs(u) + fabs(v) + fabs(M.c));
     };
-    int len = 0, aa = -1;
-    bool prev = outside(m - 1);
-    for (int k = 0; k < m; ++k) {
-        const bool o = outside(k);
-        if (o) {
-            ++len;
-            if (!prev && aa < 0) aa = k;  // first vertex of the outside run
+    int len, aa;
+    if constexpr (MAXM <= 32) {  // the outside set as a bit mask
+        unsigned mask = 0u;
+        for (int k = 0; k < m; ++k)
+            if (outside(k)) mask |= 1u << k;
+        if (!mask) return m;
+        len = __popc(mask);
+        if (len == m) return 0;
+        const unsigned full = m == 32 ? 0xffffffffu : ((1u << m) - 1u);
+        const unsigned prev = ((mask << 1) | (mask >> (m - 1))) & full;  // bit k: out[k - 1]
+        aa = __ffs(mask & ~prev) - 1;                                    // first of the run
+        int run = 0;
+        while ((mask >> ((aa + run) % m)) & 1u) ++run;
+        if (run != len) return -1;
+    } else {
+        len = 0;
+        aa = -1;
+        bool prev = outside(m - 1);
+        for (int k = 0; k < m; ++k) {
+            const bool o = outside(k);
+            if (o) {
+                ++len;
+                if (!prev && aa < 0) aa = k;  // first vertex of the outside run
+            }
+            prev = o;
         }
-        prev = o;
+        if (len == 0) return m;
+        if (len == m) return 0;
+        int run = 0;
+        while (run < m && outside((aa + run) % m)) ++run;
+        if (run != len) return -1;
     }
-    if (len == 0) return m;
-    if (len == m) return 0;
-    int run = 0;
-    while (run < m && outside((aa + run) % m)) ++run;
-    if (run != len) return -1;
     // delete entries aa+1 .. aa+len-1 (cyclic), insert M after aa
     int ins;  // slot of M
     int nm;
@@ -348,7 +393,7 @@ __device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly 
             }
         }
     } else {
-        if (m + 1 > maxm) return -2;
+        if (m + 1 > MAXM) return -2;
         for (int k = m; k > aa + 1; --k) {
             PT(P, k) = PT(P, k - 1); PX(P, k) = PX(P, k - 1); PY(P, k) = PY(P, k - 1);
         }
@@ -370,14 +415,16 @@ __device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly 
     return nm;
 }
 
-__device__ double nat_radius(Poly P, int m)
+template <int ST>
+__device__ double nat_radius(Poly<ST> P, int m)
 {
     double r2 = 0.0;
     for (int k = 0; k < m; ++k) r2 = fmax(r2, PX(P, k) * PX(P, k) + PY(P, k) * PY(P, k));
     return sqrt(r2);
 }
 
-__device__ double nat_area(Poly P, int m)
+template <int ST>
+__device__ double nat_area(Poly<ST> P, int m)
 {
     double s = 0.0;
     for (int k = 0; k < m; ++k) {
@@ -388,9 +435,10 @@ __device__ double nat_area(Poly P, int m)
 }
 
 // Sibson weights of one inside node q with polygons V (the cell of q) and R (the part
-// of it taken from one natural neighbour), maxm lines each.  Returns 0 (values written),
-// -1 (the cell could not be formed) or -2 (maxm lines are too few; nothing written).
-__device__ int nat_sibson_node(const NatArgs &a, int q, Poly V, Poly R, int maxm)
+// of it taken from one natural neighbour), MAXM lines each.  Returns 0 (values written),
+// -1 (the cell could not be formed) or -2 (MAXM lines are too few; nothing written).
+template <int ST, int MAXM>
+__device__ int nat_sibson_node(const NatArgs &a, int q, Poly<ST> V, Poly<ST> R)
 {
     const double2 Q = a.xy[q];
     const double B = 1e6 * (a.h * (a.BX + a.BY) + 1.0);
@@ -421,7 +469,7 @@ __device__ int nat_sibson_node(const NatArgs &a, int q, Poly V, Poly R, int maxm
             const double px = P.x - Q.x, py = P.y - Q.y;
             if (px * px + py * py >= 4.0 * rho * rho) continue;  // cannot reach the cell
             const Line M{px, py, 0.5 * (px * px + py * py)};
-            m = nat_clip(a, Q, B, zero, V, m, maxm, M, p);
+            m = nat_clip<ST, MAXM>(a, Q, B, zero, V, m, M, p);
             if (m <= 0) { fail = m == -2 ? -2 : -1; return; }
             rho = nat_radius(V, m);
         }
@@ -460,7 +508,7 @@ __device__ int nat_sibson_node(const NatArgs &a, int q, Poly V, Poly R, int maxm
                 const double pjx = Pj.x - Q.x, pjy = Pj.y - Q.y;
                 const Line M{pjx - pi.x, pjy - pi.y,
                              0.5 * ((pjx * pjx + pjy * pjy) - (pi.x * pi.x + pi.y * pi.y))};
-                mr = nat_clip(a, Q, B, pi, R, mr, maxm, M, tj | kRegion);
+                mr = nat_clip<ST, MAXM>(a, Q, B, pi, R, mr, M, tj | kRegion);
                 if (mr < 0) { fail = mr; break; }
             }
             if (fail) break;
@@ -492,12 +540,12 @@ __global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
     __shared__ int s_tag[2][kNatMax * kNatThreads];
     __shared__ double s_vx[2][kNatMax * kNatThreads], s_vy[2][kNatMax * kNatThreads];
     const int t = threadIdx.x;
-    const Poly V{&s_tag[0][t], &s_vx[0][t], &s_vy[0][t], kNatThreads};
-    const Poly R{&s_tag[1][t], &s_vx[1][t], &s_vy[1][t], kNatThreads};
+    const Poly<kNatThreads> V{&s_tag[0][t], &s_vx[0][t], &s_vy[0][t]};
+    const Poly<kNatThreads> R{&s_tag[1][t], &s_vx[1][t], &s_vy[1][t]};
     const int n_in = a.ctl[2];
     for (int li = blockIdx.x * kNatThreads + t; li < n_in; li += gridDim.x * kNatThreads) {
         const int q = a.lin[li];
-        const int r = nat_sibson_node(a, q, V, R, kNatMax);
+        const int r = nat_sibson_node<kNatThreads, kNatMax>(a, q, V, R);
         if (r == -2) a.lbig[atomicAdd(&a.ctl[5], 1)] = q;
         else if (r < 0) nat_error(a);
     }
@@ -510,11 +558,11 @@ __global__ __launch_bounds__(64) void k_nat_sibson_big(NatArgs a)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;  // < kNatBigThreads
     const size_t half = (size_t)kNatBig * kNatBigThreads;
-    const Poly V{a.btag + g, a.bvx + g, a.bvy + g, kNatBigThreads};
-    const Poly R{a.btag + half + g, a.bvx + half + g, a.bvy + half + g, kNatBigThreads};
+    const Poly<kNatBigThreads> V{a.btag + g, a.bvx + g, a.bvy + g};
+    const Poly<kNatBigThreads> R{a.btag + half + g, a.bvx + half + g, a.bvy + half + g};
     const int nb = a.ctl[5];
     for (int li = g; li < nb; li += kNatBigThreads)
-        if (nat_sibson_node(a, a.lbig[li], V, R, kNatBig) != 0) nat_error(a);
+        if (nat_sibson_node<kNatBigThreads, kNatBig>(a, a.lbig[li], V, R) != 0) nat_error(a);
 }
 
 #undef PT
@@ -594,10 +642,13 @@ __device__ __forceinline__ bool bulge(double2 Pa, double ex, double ey, double L
     return true;
 }
 
+// One wave per hull-list node: the lanes share the search for the nearest hull edge,
+// the rest runs on every lane with the same values and lane 0 writes.
 __global__ __launch_bounds__(64) void k_nat_hullfill(NatArgs a)
 {
     const int nh = a.ctl[3], h = a.ctl[0], ng = a.ctl[1];
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nh; li += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x;
+    for (int li = blockIdx.x; li < nh; li += gridDim.x) {
         const int q = a.lhull[li];
         const double2 Q = a.xy[q];
         double2 u = make_double2(NAN, NAN);
@@ -618,10 +669,10 @@ __global__ __launch_bounds__(64) void k_nat_hullfill(NatArgs a)
             u = a.U[bi];
             if (a.F) f = a.F[bi];
         } else {
-            // the hull point nearest q
-            int be = 0;
+            // the hull point nearest q: least distance, ties to the lowest edge
+            int be = 0x7fffffff;
             double bt = 0.0, bd = INFINITY;
-            for (int e = 0; e < h; ++e) {
+            for (int e = lane; e < h; e += 64) {
                 const double2 A = a.xy[a.hull[e]], C = a.xy[a.hull[e + 1 < h ? e + 1 : 0]];
                 const double ex = C.x - A.x, ey = C.y - A.y;
                 double tt = ((Q.x - A.x) * ex + (Q.y - A.y) * ey) / (ex * ex + ey * ey);
@@ -629,6 +680,12 @@ __global__ __launch_bounds__(64) void k_nat_hullfill(NatArgs a)
                 const double dx = A.x + tt * ex - Q.x, dy = A.y + tt * ey - Q.y;
                 const double d = dx * dx + dy * dy;
                 if (d < bd) { bd = d; be = e; bt = tt; }
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) {
+                const double od = __shfl_xor(bd, m, kWave), ot = __shfl_xor(bt, m, kWave);
+                const int oe = __shfl_xor(be, m, kWave);
+                if (od < bd || (od == bd && oe < be)) { bd = od; be = oe; bt = ot; }
             }
             if (bt > 0.0 && bt < 1.0) {
                 // inside the hull edge A-C: good nodes a (largest s <= bt) and c (smallest
@@ -677,7 +734,7 @@ __global__ __launch_bounds__(64) void k_nat_hullfill(NatArgs a)
                           },
                           [&](double D) { return beyond(D, lim); });
                 if (ir != 0x7fffffff) plane_at(a, ia, ic, ir, Q, u, f);
-                else nat_error(a);
+                else if (lane == 0) nat_error(a);
             } else {
                 // at the hull vertex v: the good nodes next to v along its two hull edges
                 const int vi = bt <= 0.0 ? be : (be + 1 < h ? be + 1 : 0);
@@ -704,8 +761,10 @@ __global__ __launch_bounds__(64) void k_nat_hullfill(NatArgs a)
                 plane_at(a, v, i2, i1, Q, u, f);
             }
         }
-        a.U[q] = u;
-        if (a.F) a.F[q] = f;
+        if (lane == 0) {
+            a.U[q] = u;
+            if (a.F) a.F[q] = f;
+        }
     }
 }
 
@@ -837,7 +896,7 @@ hipError_t natural_fill(const NatGrid &g, const double *coords, const int *bad, 
     const int nbs = std::min((g.nn + kNatThreads - 1) / kNatThreads, 4096);
     hipLaunchKernelGGL(k_nat_sibson, dim3(nbs), dim3(kNatThreads), 0, st, a);
     hipLaunchKernelGGL(k_nat_sibson_big, dim3(kNatBigThreads / 64), dim3(64), 0, st, a);
-    const int nbh = std::min((g.nn + 63) / 64, 1024);
+    const int nbh = std::min(g.nn, 2048);
     hipLaunchKernelGGL(k_nat_hullfill, dim3(nbh), dim3(64), 0, st, a);
     return hipGetLastError();
 }
