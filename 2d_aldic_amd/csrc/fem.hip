@@ -370,82 +370,115 @@ struct GemmArgs {
     double mu, cK;
 };
 
-constexpr int kGK = 16;
-
-// C tile of TILE x TILE per 256-thread block, each thread (TILE/16)^2 outputs; K in
-// chunks of 16 through LDS.  The tile shrinks for small problems so the grid still
-// spreads over the CUs (a 122 x 244 product is 128 blocks at TILE 16).
-template <int TILE, bool SCALE>
-__global__ __launch_bounds__(256) void k_gemm64(GemmArgs g)
-{
-    constexpr int R = TILE / 16;
-    __shared__ double As[kGK][TILE + 1];
-    __shared__ double Bs[kGK][TILE + 1];
-    const int t = threadIdx.x, tm = t / 16, tn = t % 16;
-    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
-    const long bz = blockIdx.z;
-    const double *A = g.A + bz * g.boA;
-    const double *B = g.B + bz * g.boB;
-    double *C = g.C + bz * g.boC;
-    double acc[R][R] = {};
-    for (int k0 = 0; k0 < g.K; k0 += kGK) {
-        for (int e = t; e < kGK * TILE; e += 256) {
-            const int mm = e / kGK, kk = e % kGK;  // A tile: TILE m x 16 k
-            const int gm = m0 + mm, gk = k0 + kk;
-            As[kk][mm] = (gm < g.M && gk < g.K) ? A[gm * g.sam + gk * g.sak] : 0.0;
-            const int kb = e / TILE, nb = e % TILE;  // B tile: 16 k x TILE n
-            const int gk2 = k0 + kb, gn = n0 + nb;
-            Bs[kb][nb] = (gk2 < g.K && gn < g.N) ? B[gk2 * g.sbk + gn * g.sbn] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kGK; ++kk) {
-            double a[R], b[R];
-#pragma unroll
-            for (int i = 0; i < R; ++i) a[i] = As[kk][tm + 16 * i];
-#pragma unroll
-            for (int j = 0; j < R; ++j) b[j] = Bs[kk][tn + 16 * j];
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int j = 0; j < R; ++j) acc[i][j] += a[i] * b[j];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        const int gm = m0 + tm + 16 * i;
-        if (gm >= g.M) continue;
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const int gn = n0 + tn + 16 * j;
-            if (gn >= g.N) continue;
-            double v = acc[i][j];
-            if (SCALE) v /= g.mu + g.cK * (g.lamy[gm] + g.lamx[gn]);
-            C[gm * g.scm + gn * g.scn] = v;
-        }
-    }
-}
-
-template <int TILE>
-static void gemm_launch(const GemmArgs &g, int batch, bool scale, hipStream_t st)
-{
-    dim3 grid((g.N + TILE - 1) / TILE, (g.M + TILE - 1) / TILE, batch);
-    if (scale) hipLaunchKernelGGL((k_gemm64<TILE, true>), grid, dim3(256), 0, st, g);
-    else hipLaunchKernelGGL((k_gemm64<TILE, false>), grid, dim3(256), 0, st, g);
-}
-
-static hipError_t gemm(const GemmArgs &g, int batch, bool scale, hipStream_t st)
-{
-    auto blocks = [&](int T) { return (long)((g.N + T - 1) / T) * ((g.M + T - 1) / T) * batch; };
-    if (blocks(64) >= 256) gemm_launch<64>(g, batch, scale, st);
-    else if (blocks(32) >= 256) gemm_launch<32>(g, batch, scale, st);
-    else gemm_launch<16>(g, batch, scale, st);
-    return hipGetLastError();
-}
-
 static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DBuf *work,
                                 hipStream_t st);
+
+// C = A B (then ./ (mu + cK (lamy[m] + lamx[n])) when SCALE) on the FP64 matrix cores, for
+// the four-pass solve: one wave per TM x 32 tile of C (TM / 16 x 2 tiles of
+// v_mfma_f64_16x16x4f64: A 16 x 4 with lane l holding (row l % 16, k l / 16), B 4 x 16 with
+// lane l holding (k l / 16, col l % 16), D 16 x 16 with lane l, element v holding
+// (row l / 16 + 4 v, col l % 16)).  Every lane loads its own fragments from global
+// memory (no LDS, no barrier; the matrices hold < 2^31 bytes, so offsets are 32-bit).
+// K runs in chunks of 16 steps of 4: all of a chunk's fragments are loaded into their own
+// registers and waited for, then one MFMA chain per output tile starts from zero and
+// runs the chunk back to back, and the chunk's result is added to an FP64 register sum
+// before the next chunk's loads: no fragment register is reloaded while an MFMA that
+// reads it may be in flight, and no MFMA accumulator is carried between chunks.  Addresses are clamped into the matrices; A's
+// word is multiplied by 0 past K (finite data), and rows or columns past M, N only feed
+// results that are not stored.  Tiles run along N fastest; the batch is blockIdx.y.
+typedef double gm_d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGemmChunk = 16;  // k steps of 4 per MFMA chain
+
+template <int TM, bool SCALE>
+__global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g)
+{
+    constexpr int MT = TM / 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tilesN = (g.N + 31) / 32, tilesM = (g.M + TM - 1) / TM;
+    const int tile = blockIdx.x * 4 + wv;
+    if (tile >= tilesM * tilesN) return;  // whole wave
+    const int m0 = (tile / tilesN) * TM, n0 = (tile % tilesN) * 32;
+    const int bz = blockIdx.y;
+    const double *A = g.A + (long)bz * g.boA;
+    const double *B = g.B + (long)bz * g.boB;
+    double *C = g.C + (long)bz * g.boC;
+    const int li = lane & 15, lk = lane >> 4;
+    const int sak = (int)g.sak, sbk = (int)g.sbk;
+    int ar[MT], bc[2];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) ar[i] = min(m0 + 16 * i + li, g.M - 1) * (int)g.sam;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bc[j] = min(n0 + 16 * j + li, g.N - 1) * (int)g.sbn;
+    double sum[MT][2][4];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sum[i][j][v] = 0.0;
+    const int ks = (g.K + 3) / 4;
+    for (int s0 = 0; s0 < ks; s0 += kGemmChunk) {
+        double a[kGemmChunk][MT], b[kGemmChunk][2];
+#pragma unroll
+        for (int u = 0; u < kGemmChunk; ++u) {
+            const int k = 4 * (s0 + u) + lk, kc = min(k, g.K - 1);
+            const double km = k < g.K ? 1.0 : 0.0;
+#pragma unroll
+            for (int i = 0; i < MT; ++i) a[u][i] = A[ar[i] + kc * sak] * km;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[u][j] = B[kc * sbk + bc[j]];
+        }
+        // every fragment of the chunk is loaded before its first MFMA (no instruction is
+        // scheduled across the barrier), and no load of the next chunk is issued before
+        // this chunk's results are summed
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                gm_d4 part = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int u = 0; u < kGemmChunk; ++u)
+                    part = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][i], b[u][j], part, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sum[i][j][v] += part[v];
+            }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int c = n0 + 16 * j + li;
+            if (c >= g.N) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = m0 + 16 * i + lk + 4 * v;
+                if (r >= g.M) continue;
+                double o = sum[i][j][v];
+                if (SCALE) o /= g.mu + g.cK * (g.lamy[r] + g.lamx[c]);
+                C[(long)r * g.scm + (long)c * g.scn] = o;
+            }
+        }
+}
+
+// Wave tiles of 32 rows when they already give one wave per SIMD (1024), else 16.
+static hipError_t gemm_mfma(const GemmArgs &g, int batch, bool scale, hipStream_t st)
+{
+    const long tn = (g.N + 31) / 32;
+    const bool big = ((g.M + 31) / 32) * tn * batch >= 1024;
+    const long tiles = (big ? (g.M + 31) / 32 : (g.M + 15) / 16) * tn;
+    const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)batch);
+    if (big) {
+        if (scale) hipLaunchKernelGGL((k_gemm_mfma<32, true>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_gemm_mfma<32, false>), grid, dim3(256), 0, st, g);
+    } else {
+        if (scale) hipLaunchKernelGGL((k_gemm_mfma<16, true>), grid, dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_gemm_mfma<16, false>), grid, dim3(256), 0, st, g);
+    }
+    return hipGetLastError();
+}
 
 // Fused pass of the fast-diagonalisation solve for gx, gy <= 128: workgroup (row strip
 // of 16 rows, component) computes
@@ -454,7 +487,7 @@ static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DB
 // interleaved gy x 2gx nodal arrays.  Both products run on the FP64 matrix cores
 // (v_mfma_f64_16x16x4f64: A 16 x 4 with lane l holding (l % 16, l / 16), B 4 x 16 with
 // lane l holding (l / 16, l % 16), D 16 x 16 with lane l, element v holding
-// (4 (l / 16) + v, l % 16)).  16 waves = 8 column tiles of 16 x 2 halves of the inner
+// (l / 16 + 4 v, l % 16)).  16 waves = 8 column tiles of 16 x 2 halves of the inner
 // index; every wave loads its 16 A, B and R fragments from global memory up front (one
 // round trip), indices clamped in range and the words past gy or gx zeroed.  The strip
 // (L B) goes through LDS once; the halves are summed in a fixed order.
@@ -511,7 +544,7 @@ __global__ __launch_bounds__(kFdThreads) void k_fd_pass(int gy, int gx,
     __syncthreads();
     if (h == 0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Tt[col * kFdRows + 4 * lk + v] = acc[v] + Pp[ct][lane * 4 + v];
+        for (int v = 0; v < 4; ++v) Tt[col * kFdRows + lk + 4 * v] = acc[v] + Pp[ct][lane * 4 + v];
     }
     __syncthreads();
     // out strip = Tt R
@@ -529,7 +562,7 @@ __global__ __launch_bounds__(kFdThreads) void k_fd_pass(int gy, int gx,
     if (h == 0 && col < gx) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int m = m0 + 4 * lk + v;
+            const int m = m0 + lk + 4 * v;
             if (m < gy) {
                 double o = acc[v] + Pp[ct][lane * 4 + v];
                 if (SCALE) o /= mu + cK * (lamy[m] + lamx[col]);
@@ -578,17 +611,17 @@ static hipError_t fem_fd_solve4(const MeshDev &m, const double *b, double *x, DB
     GemmArgs g{};
     // T1 = Vy^T B
     g = GemmArgs{gy, 2 * gx, gy, Vy, 1, gy, 0, b, w2, 1, 0, T1, w2, 1, 0, nullptr, nullptr, 0, 0};
-    if ((e = gemm(g, 1, false, st)) != hipSuccess) return e;
+    if ((e = gemm_mfma(g, 1, false, st)) != hipSuccess) return e;
     // T2 = (T1 Vx) ./ D, per component
     g = GemmArgs{gy, gx, gx, T1, w2, 2, 1, Vx, gx, 1, 0, T2, w2, 2, 1, ly, lx, m.mu,
                  m.beta + m.alpha * m.mu};
-    if ((e = gemm(g, 2, true, st)) != hipSuccess) return e;
+    if ((e = gemm_mfma(g, 2, true, st)) != hipSuccess) return e;
     // T1 = T2 Vx^T, per component
     g = GemmArgs{gy, gx, gx, T2, w2, 2, 1, Vx, 1, gx, 0, T1, w2, 2, 1, nullptr, nullptr, 0, 0};
-    if ((e = gemm(g, 2, false, st)) != hipSuccess) return e;
+    if ((e = gemm_mfma(g, 2, false, st)) != hipSuccess) return e;
     // X = Vy T1
     g = GemmArgs{gy, 2 * gx, gy, Vy, gy, 1, 0, T1, w2, 1, 0, x, w2, 1, 0, nullptr, nullptr, 0, 0};
-    return gemm(g, 1, false, st);
+    return gemm_mfma(g, 1, false, st);
 }
 
 hipError_t fem_assemble(MeshDev &m, double beta, double mu, double alpha, hipStream_t st)
