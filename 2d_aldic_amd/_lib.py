@@ -27,7 +27,9 @@ EXPORTS = (
     "aldic_subpb2_quadtree", "aldic_nodal_strain_quadtree", "aldic_qframe_setup",
     "aldic_qframe_run", "aldic_qframe_get", "aldic_prep_noalloc", "aldic_natural_fill",
     "aldic_frame_run_seeded", "aldic_compute_strain", "aldic_inpaint_nans",
-    "aldic_remove_outliers",
+    "aldic_remove_outliers", "aldic_icgn6_start", "aldic_cnn_nparams", "aldic_cnn_set_weights",
+    "aldic_cnn_get_weights", "aldic_cnn_predict", "aldic_cnn_add_samples",
+    "aldic_cnn_clear_samples", "aldic_cnn_train", "aldic_cnn_grad",
 )
 
 
@@ -94,6 +96,16 @@ def load():
     L.aldic_compute_strain.argtypes = [P, ip, ip, dp, vp, vp, ip, ip, ip, vp, vp]
     L.aldic_inpaint_nans.argtypes = [P, vp, ip, ip, ip]
     L.aldic_remove_outliers.argtypes = [P, vp, vp, ip, ip, dp, vp, vp]
+    L.aldic_icgn6_start.argtypes = [P, ip, vp, vp, vp, ip, dp, vp, vp, vp, vp, vp]
+    L.aldic_cnn_nparams.argtypes = []
+    L.aldic_cnn_nparams.restype = C.c_long
+    L.aldic_cnn_set_weights.argtypes = [P, vp, C.c_long]
+    L.aldic_cnn_get_weights.argtypes = [P, vp, C.c_long]
+    L.aldic_cnn_predict.argtypes = [P, ip, vp, vp]
+    L.aldic_cnn_add_samples.argtypes = [P, ip, vp, vp]
+    L.aldic_cnn_clear_samples.argtypes = [P]
+    L.aldic_cnn_train.argtypes = [P, ip, ip, C.c_float, C.c_ulonglong, vp]
+    L.aldic_cnn_grad.argtypes = [P, ip, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -209,6 +221,66 @@ class Context:
                                     int(ws) if wsa is None else 0, float(tol), _p(U), _p(F),
                                     _p(H), _p(it), _p(z)), "aldic_icgn6")
         return U, F, H, it, z
+
+    def icgn6_start(self, coords, P0, ws, tol=1e-2):
+        """6-DOF IC-GN started from P0 (S x 6: u_x, v_x, u_y, v_y, u, v)."""
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        S = coords.shape[0]
+        P0 = _c(P0, np.float64).reshape(S, 6)
+        wsa = None if np.isscalar(ws) else _c(ws, np.int32)
+        U = np.empty(2 * S)
+        F = np.empty(4 * S)
+        H = np.empty((S, 21))
+        it = np.empty(S, np.int32)
+        z = np.empty(S)
+        self._ck(self.L.aldic_icgn6_start(self.h, S, _p(coords), _p(P0), _p(wsa),
+                                          int(ws) if wsa is None else 0, float(tol), _p(U),
+                                          _p(F), _p(H), _p(it), _p(z)), "aldic_icgn6_start")
+        return U, F, H, it, z
+
+    # ---- learned initial guess ----------------------------------------------------
+    def cnn_nparams(self):
+        return int(self.L.aldic_cnn_nparams())
+
+    def cnn_set_weights(self, w):
+        w = _c(w, np.float32).ravel()
+        self._ck(self.L.aldic_cnn_set_weights(self.h, _p(w), w.size), "aldic_cnn_set_weights")
+
+    def cnn_get_weights(self):
+        w = np.empty(self.cnn_nparams(), np.float32)
+        self._ck(self.L.aldic_cnn_get_weights(self.h, _p(w), w.size), "aldic_cnn_get_weights")
+        return w
+
+    def cnn_predict(self, coords):
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        P = np.empty((coords.shape[0], 6))
+        self._ck(self.L.aldic_cnn_predict(self.h, coords.shape[0], _p(coords), _p(P)),
+                 "aldic_cnn_predict")
+        return P
+
+    def cnn_add_samples(self, coords, P):
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        P = _c(P, np.float64).reshape(coords.shape[0], 6)
+        self._ck(self.L.aldic_cnn_add_samples(self.h, coords.shape[0], _p(coords), _p(P)),
+                 "aldic_cnn_add_samples")
+
+    def cnn_clear_samples(self):
+        self._ck(self.L.aldic_cnn_clear_samples(self.h), "aldic_cnn_clear_samples")
+
+    def cnn_train(self, steps, batch, lr, seed=0):
+        loss = np.empty(max(int(steps), 1))
+        self._ck(self.L.aldic_cnn_train(self.h, int(steps), int(batch), float(lr), int(seed),
+                                        _p(loss)), "aldic_cnn_train")
+        return loss[:int(steps)]
+
+    def cnn_grad(self, rows):
+        rows = _c(rows, np.int32)
+        g = np.empty(self.cnn_nparams(), np.float32)
+        loss = np.empty(1)
+        out = np.empty((rows.size, 6))
+        self._ck(self.L.aldic_cnn_grad(self.h, rows.size, _p(rows), _p(g), _p(loss), _p(out)),
+                 "aldic_cnn_grad")
+        return g, float(loss[0]), out
 
     def icgn2(self, coords, ws, H21, mu, vdual, Uold, Fold, tol=1e-2):
         coords = _c(coords, np.float64).reshape(-1, 2)

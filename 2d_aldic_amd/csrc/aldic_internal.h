@@ -178,6 +178,7 @@ struct FrameState {
 };
 
 struct OutlierWork;  // outlier.hip
+struct CnnWork;      // cnn.hip
 
 }  // namespace aldic
 
@@ -194,9 +195,10 @@ struct aldic_ctx {
     bool have_mask = false;
     bool images_from_raw = false;    // f, g normalised on the device from rawf, rawg
     aldic::OutlierWork *outl = nullptr;  // work buffers of the outlier test and in-painting
+    aldic::CnnWork *cnn = nullptr;       // learned initial guess: weights, training set
     aldic::DBuf stats;               // ROI partial sums, then {mean_f, 1/std_f, mean_g, 1/std_g}
     aldic::DBuf boxp;                // window box-sum planes of the exact search
-    aldic::DBuf scratch[8];          // per-call work buffers
+    aldic::DBuf scratch[9];          // per-call work buffers (8: IC-GN start gradients)
     aldic::MeshDev mesh;
     bool have_mesh = false;
     aldic::QuadDev quad;             // aldic_set_quadtree / aldic_subpb2_quadtree
@@ -234,6 +236,7 @@ struct IcgnArgs {
     double tol;
     // 6-DOF
     const double *U0;       // 2S
+    const double *F0;       // 4S or nullptr: starting displacement gradient (else 0)
     double *U, *F, *H21, *zncc;
     int *iters;
     // 2-DOF
@@ -289,6 +292,29 @@ hipError_t launch_qfactors(const SearchArgs &a, hipStream_t st);
 // (i, j) at p[i si + j sj].  Return 0, -1 (HIP error) or an ALDIC error code with msg.
 OutlierWork *outlier_work_new();
 void outlier_work_free(OutlierWork *w);
+
+// Learned initial guess (cnn.hip): a fixed small network on 64 x 64 patch pairs.  All
+// pointers are device pointers unless named _host; every call enqueues on st.
+CnnWork *cnn_work_new();
+void cnn_work_free(CnnWork *c);
+long cnn_nparams();
+hipError_t cnn_set_weights(CnnWork &c, const float *w_host, hipStream_t st);  // waits
+hipError_t cnn_get_weights(CnnWork &c, float *w_host, hipStream_t st);        // waits
+// P (6 per node: u_x, v_x, u_y, v_y, u, v) at nodes xy (2 per node, 1-based) from f, g
+hipError_t cnn_forward(CnnWork &c, const float *f, const float *g, int nx, int ny,
+                       const double *xy, int S, double *P, hipStream_t st);
+// appends the nodes' patch pairs and targets P (6 per node) to the training set
+hipError_t cnn_add_samples(CnnWork &c, const float *f, const float *g, int nx, int ny,
+                           const double *xy, const double *P, int S, hipStream_t st);
+void cnn_clear_samples(CnnWork &c);
+long cnn_n_samples(const CnnWork &c);
+// forward + backward of training-set rows idx_host (B): loss to *loss_dev, gradient kept;
+// one Adam step when lr > 0
+hipError_t cnn_step(CnnWork &c, const int *idx_host, int B, float lr, double *loss_dev,
+                    hipStream_t st);
+const float *cnn_grad(const CnnWork &c);     // the last step's gradient (nparams)
+bool cnn_have_weights(const CnnWork &c);
+const float *cnn_outputs(const CnnWork &c);  // the last forward's scaled outputs (6 B)
 int inpaint_view(OutlierWork &w, double *p, int rows, int cols, long si, long sj, int method,
                  hipStream_t st, const char *&msg);
 int remove_outliers_views(OutlierWork &w, double *up, double *vp, int rows, int cols, long si,

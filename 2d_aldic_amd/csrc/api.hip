@@ -276,6 +276,8 @@ void aldic_destroy(aldic_ctx *ctx)
     F.nbad.release();
     outlier_work_free(ctx->outl);
     ctx->outl = nullptr;
+    cnn_work_free(ctx->cnn);
+    ctx->cnn = nullptr;
     F.seed.release();
     ctx->qf.nat_err.release();
     if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
@@ -461,9 +463,9 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
     return ALDIC_OK;
 }
 
-int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, const int *ws,
-                int ws_all, double tol, double *U, double *F, double *H21, int *iters,
-                double *zncc)
+static int icgn6_impl(aldic_ctx *ctx, int S, const double *coords, const double *U0,
+                      const double *F0, const int *ws, int ws_all, double tol, double *U, double *F,
+                      double *H21, int *iters, double *zncc)
 {
     if (!ctx) return ALDIC_ERR_ARG;
     if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "icgn6: no images");
@@ -474,6 +476,7 @@ int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, c
     DBuf *sc = ctx->scratch;
     CK(up(sc[0], coords, 2 * (size_t)S, ctx->stream));
     CK(up(sc[1], U0, 2 * (size_t)S, ctx->stream));
+    if (F0) CK(up(sc[8], F0, 4 * (size_t)S, ctx->stream));
     CK(sc[2].ensure(sizeof(double) * 2 * S));
     CK(sc[3].ensure(sizeof(double) * 4 * S));
     CK(sc[4].ensure(sizeof(double) * 21 * (size_t)S));
@@ -489,6 +492,7 @@ int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, c
     a.S = S;
     a.coords = sc[0].as<double>();
     a.U0 = sc[1].as<double>();
+    a.F0 = F0 ? sc[8].as<double>() : nullptr;
     a.ws = ws ? sc[6].as<int>() : nullptr;
     a.ws_all = ws_all;
     a.tol = tol;
@@ -506,6 +510,33 @@ int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, c
     if (zncc) CK(down(zncc, a.zncc, S, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     return ALDIC_OK;
+}
+
+int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, const int *ws,
+                int ws_all, double tol, double *U, double *F, double *H21, int *iters,
+                double *zncc)
+{
+    return icgn6_impl(ctx, S, coords, U0, nullptr, ws, ws_all, tol, U, F, H21, iters, zncc);
+}
+
+int aldic_icgn6_start(aldic_ctx *ctx, int S, const double *coords, const double *P0,
+                      const int *ws, int ws_all, double tol, double *U, double *F, double *H21,
+                      int *iters, double *zncc)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (S < 0 || (S > 0 && !P0)) return fail(ctx, ALDIC_ERR_ARG, "icgn6_start: bad arguments");
+    try {
+        std::vector<double> U0(2 * (size_t)S), F0(4 * (size_t)S);
+        for (int j = 0; j < S; ++j) {
+            for (int q = 0; q < 4; ++q) F0[4 * (size_t)j + q] = P0[6 * (size_t)j + q];
+            U0[2 * (size_t)j] = P0[6 * (size_t)j + 4];
+            U0[2 * (size_t)j + 1] = P0[6 * (size_t)j + 5];
+        }
+        return icgn6_impl(ctx, S, coords, U0.data(), F0.data(), ws, ws_all, tol, U, F, H21, iters,
+                          zncc);
+    } catch (...) {
+        return fail(ctx, ALDIC_ERR_HIP, "icgn6_start: host allocation failed");
+    }
 }
 
 int aldic_icgn2(aldic_ctx *ctx, int S, const double *coords, const int *ws, int ws_all,
@@ -1047,6 +1078,147 @@ int aldic_bicubic(aldic_ctx *ctx, const double *X, const double *Y, long n, doub
     CK(launch_bicubic(ctx->g.as<float>(), ctx->nx, ctx->ny, sc[0].as<double>(), sc[1].as<double>(),
                       n, sc[2].as<double>(), ctx->stream));
     CK(down(Z, sc[2].p, n, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+// ---- learned initial guess -------------------------------------------------------------
+long aldic_cnn_nparams(void) { return cnn_nparams(); }
+
+static int cnn_ready(aldic_ctx *ctx, const char *what, bool need_w)
+{
+    if (!ctx->cnn) ctx->cnn = cnn_work_new();
+    if (!ctx->cnn) return fail(ctx, ALDIC_ERR_HIP, what);
+    if (need_w && !cnn_have_weights(*ctx->cnn)) {
+        std::string m = std::string(what) + ": no network weights";
+        return fail(ctx, ALDIC_ERR_STATE, m.c_str());
+    }
+    return ALDIC_OK;
+}
+
+int aldic_cnn_set_weights(aldic_ctx *ctx, const float *w, long n)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!w || n != cnn_nparams()) return fail(ctx, ALDIC_ERR_ARG, "cnn_set_weights: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    if (int r = cnn_ready(ctx, "cnn_set_weights", false)) return r;
+    CK(cnn_set_weights(*ctx->cnn, w, ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_cnn_get_weights(aldic_ctx *ctx, float *w, long n)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!w || n != cnn_nparams()) return fail(ctx, ALDIC_ERR_ARG, "cnn_get_weights: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    if (int r = cnn_ready(ctx, "cnn_get_weights", true)) return r;
+    CK(cnn_get_weights(*ctx->cnn, w, ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_cnn_predict(aldic_ctx *ctx, int S, const double *coords, double *P)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "cnn_predict: no images");
+    if (S < 0 || (S > 0 && (!coords || !P))) return fail(ctx, ALDIC_ERR_ARG, "cnn_predict: bad arguments");
+    if (S == 0) return ALDIC_OK;
+    CK(hipSetDevice(ctx->device));
+    if (int r = cnn_ready(ctx, "cnn_predict", true)) return r;
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], coords, 2 * (size_t)S, ctx->stream));
+    CK(sc[1].ensure(sizeof(double) * 6 * (size_t)S));
+    CK(cnn_forward(*ctx->cnn, ctx->f.as<float>(), ctx->g.as<float>(), ctx->nx, ctx->ny,
+                   sc[0].as<double>(), S, sc[1].as<double>(), ctx->stream));
+    CK(down(P, sc[1].p, 6 * (size_t)S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_cnn_add_samples(aldic_ctx *ctx, int S, const double *coords, const double *P)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "cnn_add_samples: no images");
+    if (S < 0 || (S > 0 && (!coords || !P))) return fail(ctx, ALDIC_ERR_ARG, "cnn_add_samples: bad arguments");
+    if (S == 0) return ALDIC_OK;
+    CK(hipSetDevice(ctx->device));
+    if (int r = cnn_ready(ctx, "cnn_add_samples", false)) return r;
+    DBuf *sc = ctx->scratch;
+    CK(up(sc[0], coords, 2 * (size_t)S, ctx->stream));
+    CK(up(sc[1], P, 6 * (size_t)S, ctx->stream));
+    CK(cnn_add_samples(*ctx->cnn, ctx->f.as<float>(), ctx->g.as<float>(), ctx->nx, ctx->ny,
+                       sc[0].as<double>(), sc[1].as<double>(), S, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_cnn_clear_samples(aldic_ctx *ctx)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (ctx->cnn) cnn_clear_samples(*ctx->cnn);
+    return ALDIC_OK;
+}
+
+int aldic_cnn_train(aldic_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,
+                    double *loss)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (steps < 0 || batch <= 0 || batch > 65536 || !(lr > 0.f) || (steps > 0 && !loss))
+        return fail(ctx, ALDIC_ERR_ARG, "cnn_train: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    if (int r = cnn_ready(ctx, "cnn_train", true)) return r;
+    const long n = cnn_n_samples(*ctx->cnn);
+    if (n <= 0) return fail(ctx, ALDIC_ERR_STATE, "cnn_train: empty training set");
+    if (n >= (1L << 31)) return fail(ctx, ALDIC_ERR_ARG, "cnn_train: training set too large");
+    DBuf *sc = ctx->scratch;
+    CK(sc[2].ensure(sizeof(double) * (size_t)std::max(steps, 1)));
+    try {
+        // batches drawn with replacement by splitmix64 from the seed
+        std::vector<int> idx((size_t)batch);
+        unsigned long long st = seed;
+        for (int it = 0; it < steps; ++it) {
+            for (int b = 0; b < batch; ++b) {
+                st += 0x9e3779b97f4a7c15ull;
+                unsigned long long z = st;
+                z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+                z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+                z ^= z >> 31;
+                idx[(size_t)b] = (int)(z % (unsigned long long)n);
+            }
+            CK(cnn_step(*ctx->cnn, idx.data(), batch, lr, sc[2].as<double>() + it, ctx->stream));
+            // the index vector is reused: the copy above must finish before it changes
+            CK(hipStreamSynchronize(ctx->stream));
+        }
+    } catch (...) {
+        return fail(ctx, ALDIC_ERR_HIP, "cnn_train: host allocation failed");
+    }
+    if (steps > 0) CK(down(loss, sc[2].p, (size_t)steps, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_cnn_grad(aldic_ctx *ctx, int B, const int *rows, float *grad, double *loss,
+                   double *outputs)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (B <= 0 || !rows || !grad || !loss) return fail(ctx, ALDIC_ERR_ARG, "cnn_grad: bad arguments");
+    CK(hipSetDevice(ctx->device));
+    if (int r = cnn_ready(ctx, "cnn_grad", true)) return r;
+    const long n = cnn_n_samples(*ctx->cnn);
+    for (int b = 0; b < B; ++b)
+        if (rows[b] < 0 || rows[b] >= n) return fail(ctx, ALDIC_ERR_ARG, "cnn_grad: row outside the training set");
+    DBuf *sc = ctx->scratch;
+    CK(sc[2].ensure(sizeof(double)));
+    CK(cnn_step(*ctx->cnn, rows, B, 0.f, sc[2].as<double>(), ctx->stream));
+    CK(hipMemcpyAsync(grad, cnn_grad(*ctx->cnn), sizeof(float) * cnn_nparams(), hipMemcpyDeviceToHost,
+                      ctx->stream));
+    CK(down(loss, sc[2].p, 1, ctx->stream));
+    if (outputs) {
+        std::vector<float> y((size_t)B * 6);
+        CK(hipMemcpyAsync(y.data(), cnn_outputs(*ctx->cnn), sizeof(float) * y.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < y.size(); ++i) outputs[i] = y[i];
+    }
     CK(hipStreamSynchronize(ctx->stream));
     return ALDIC_OK;
 }

@@ -311,6 +311,8 @@ __device__ __forceinline__ void icgn_subset(const IcgnArgs &a, int slot)
     double P[6];
     if (SIX) {
         P[0] = P[1] = P[2] = P[3] = 0.0;
+        if (a.F0)
+            for (int q = 0; q < 4; ++q) P[q] = a.F0[4 * j + q];
         P[4] = a.U0[2 * j];
         P[5] = a.U0[2 * j + 1];
     } else {

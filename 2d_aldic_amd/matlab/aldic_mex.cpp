@@ -228,6 +228,66 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
                           mxGetDoubles(plhs[0]), mxGetDoubles(plhs[1]), mxGetDoubles(plhs[2]),
                           iptr(plhs[3]), mxGetDoubles(plhs[4])),
               "icgn6");
+    } else if (cmd == "icgn6_start") {
+        // [U, F, H21, iters, zncc] = aldic_mex('icgn6_start', P0 (6 x S), coords, ws, tol)
+        need(nrhs, 5, cmd.c_str());
+        const std::vector<double> xy = coords2(prhs[2]);
+        const int S = (int)(xy.size() / 2);
+        if (numel(prhs[1]) != 6 * (size_t)S) mexErrMsgIdAndTxt("aldic:args", "P0 must be 6 x S");
+        const bool per = numel(prhs[3]) == (size_t)S && S > 1;
+        const std::vector<int> wsv = per ? ints(prhs[3], 0) : std::vector<int>();
+        plhs[0] = dmat(2 * (size_t)S, 1);
+        plhs[1] = dmat(4 * (size_t)S, 1);
+        plhs[2] = dmat(21, S);
+        plhs[3] = imat(S, 1);
+        plhs[4] = dmat(S, 1);
+        check(aldic_icgn6_start(g_ctx, S, xy.data(), dbl(prhs[1], "P0"), per ? wsv.data() : nullptr,
+                                per ? 0 : (int)mxGetScalar(prhs[3]), mxGetScalar(prhs[4]),
+                                mxGetDoubles(plhs[0]), mxGetDoubles(plhs[1]), mxGetDoubles(plhs[2]),
+                                iptr(plhs[3]), mxGetDoubles(plhs[4])),
+              "icgn6_start");
+    } else if (cmd == "cnn_set_weights") {  // aldic_mex('cnn_set_weights', w) (single, nparams)
+        need(nrhs, 2, cmd.c_str());
+        if (mxGetClassID(prhs[1]) != mxSINGLE_CLASS || numel(prhs[1]) != (size_t)aldic_cnn_nparams())
+            mexErrMsgIdAndTxt("aldic:args", "weights must be single with aldic_cnn_nparams() elements");
+        check(aldic_cnn_set_weights(g_ctx, static_cast<const float *>(mxGetData(prhs[1])),
+                                    aldic_cnn_nparams()),
+              "cnn_set_weights");
+    } else if (cmd == "cnn_get_weights") {  // w = aldic_mex('cnn_get_weights')
+        const long n = aldic_cnn_nparams();
+        plhs[0] = mxCreateNumericMatrix((size_t)n, 1, mxSINGLE_CLASS, mxREAL);
+        check(aldic_cnn_get_weights(g_ctx, static_cast<float *>(mxGetData(plhs[0])), n), "cnn_get_weights");
+    } else if (cmd == "cnn_predict") {  // P = aldic_mex('cnn_predict', coords): 6 x S
+        need(nrhs, 2, cmd.c_str());
+        const std::vector<double> xy = coords2(prhs[1]);
+        const int S = (int)(xy.size() / 2);
+        plhs[0] = dmat(6, S);
+        check(aldic_cnn_predict(g_ctx, S, xy.data(), mxGetDoubles(plhs[0])), "cnn_predict");
+    } else if (cmd == "cnn_add_samples") {  // aldic_mex('cnn_add_samples', coords, P (6 x S))
+        need(nrhs, 3, cmd.c_str());
+        const std::vector<double> xy = coords2(prhs[1]);
+        const int S = (int)(xy.size() / 2);
+        if (numel(prhs[2]) != 6 * (size_t)S) mexErrMsgIdAndTxt("aldic:args", "P must be 6 x S");
+        check(aldic_cnn_add_samples(g_ctx, S, xy.data(), dbl(prhs[2], "P")), "cnn_add_samples");
+    } else if (cmd == "cnn_clear_samples") {  // aldic_mex('cnn_clear_samples')
+        check(aldic_cnn_clear_samples(g_ctx), "cnn_clear_samples");
+    } else if (cmd == "cnn_train") {  // loss = aldic_mex('cnn_train', steps, batch, lr, seed)
+        need(nrhs, 5, cmd.c_str());
+        const int steps = (int)mxGetScalar(prhs[1]);
+        plhs[0] = dmat(steps > 0 ? (size_t)steps : 0, 1);
+        check(aldic_cnn_train(g_ctx, steps, (int)mxGetScalar(prhs[2]), (float)mxGetScalar(prhs[3]),
+                              (unsigned long long)mxGetScalar(prhs[4]), mxGetDoubles(plhs[0])),
+              "cnn_train");
+    } else if (cmd == "cnn_grad") {  // [g, loss, out] = aldic_mex('cnn_grad', rows (0-based))
+        need(nrhs, 2, cmd.c_str());
+        const std::vector<int> rows = ints(prhs[1], 0);
+        const int B = (int)rows.size();
+        plhs[0] = mxCreateNumericMatrix((size_t)aldic_cnn_nparams(), 1, mxSINGLE_CLASS, mxREAL);
+        plhs[1] = dmat(1, 1);
+        plhs[2] = dmat(6, (size_t)B);
+        check(aldic_cnn_grad(g_ctx, B, rows.data(), static_cast<float *>(mxGetData(plhs[0])),
+                             mxGetDoubles(plhs[1]), mxGetDoubles(plhs[2])),
+              "cnn_grad");
     } else if (cmd == "icgn2") {
         // [U, iters] = aldic_mex('icgn2', coords, ws, H21 (21 x S), mu, vdual, Uold, Fold, tol)
         need(nrhs, 9, cmd.c_str());

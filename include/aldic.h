@@ -113,6 +113,14 @@ int aldic_icgn6(aldic_ctx *ctx, int S, const double *coords, const double *U0, c
                 int ws_all, double tol, double *U, double *F, double *H21, int *iters,
                 double *zncc);
 
+/* IC-GN 6-DOF as aldic_icgn6, started from a full warp P0 (6 per node: u_x, v_x, u_y, v_y,
+ * u, v) instead of (0, 0, 0, 0, U0): the caller-supplied start of the learned initial guess
+ * below.  No counterpart in the reference (funICGN.m:39 always starts from zero gradients);
+ * for callers that hold an affine start. */
+int aldic_icgn6_start(aldic_ctx *ctx, int S, const double *coords, const double *P0,
+                      const int *ws, int ws_all, double tol, double *U, double *F, double *H21,
+                      int *iters, double *zncc);
+
 /* Subpb1.m:119-152 -> funICGN_Subpb1.m:1-362: 2-DOF IC-GN with F frozen at Fold and
  * the penalty mu * (P(5:6) - Uold - vdual); H21 from LocalICGN.  udual is unused by
  * the reference and is not taken. */
@@ -345,6 +353,32 @@ int aldic_qframe_run(aldic_ctx *ctx, const double *U0);
  * host once, at its end; a PCG that did not converge in its budget re-runs the frame. */
 int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *iters2,
                      double *times);
+
+/* ---- learned initial guess (no counterpart in the reference) ------------------------
+ * A fixed small network (64 x 64 patch pairs of the normalised images centred on a node:
+ * 4 x 4 stride-4 patchify to 32 channels, two 3 x 3 stride-2 convolutions to 64, dense
+ * 1024 -> 128 -> 6, ReLU between) on the BF16 matrix cores, proposing each subset's six
+ * IC-GN parameters (u_x, v_x, u_y, v_y, u, v) in place of the integer search.  Weights
+ * (FP32, aldic_cnn_nparams() of them) are set by the caller or trained here on a
+ * device-held set of patch pairs with targets (mean squared error on (F / 0.05, U / 16),
+ * Adam).  The patches of nodes within 32 px of the border are clamped into the image. */
+long aldic_cnn_nparams(void);
+int aldic_cnn_set_weights(aldic_ctx *ctx, const float *w, long n);   /* resets Adam state */
+int aldic_cnn_get_weights(aldic_ctx *ctx, float *w, long n);
+/* P (6 per node) at nodes coords (1-based centres, S x 2) of the current images. */
+int aldic_cnn_predict(aldic_ctx *ctx, int S, const double *coords, double *P);
+/* Appends the nodes' patch pairs of the current images, with targets P (6 per node), to
+ * the training set; aldic_cnn_clear_samples empties it. */
+int aldic_cnn_add_samples(aldic_ctx *ctx, int S, const double *coords, const double *P);
+int aldic_cnn_clear_samples(aldic_ctx *ctx);
+/* steps Adam steps of batch rows drawn (with replacement, splitmix64 from seed) from the
+ * training set; loss[i] = the mean squared error of step i before its update. */
+int aldic_cnn_train(aldic_ctx *ctx, int steps, int batch, float lr, unsigned long long seed,
+                    double *loss);
+/* Loss, gradient (nparams) and scaled outputs (6 B, nullable) of training-set rows, no
+ * update (for gradient checks). */
+int aldic_cnn_grad(aldic_ctx *ctx, int B, const int *rows, float *grad, double *loss,
+                   double *outputs);
 
 #ifdef __cplusplus
 }
