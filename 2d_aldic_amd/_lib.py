@@ -29,7 +29,7 @@ EXPORTS = (
     "aldic_frame_run_seeded", "aldic_compute_strain", "aldic_inpaint_nans",
     "aldic_remove_outliers", "aldic_icgn6_start", "aldic_cnn_nparams", "aldic_cnn_set_weights",
     "aldic_cnn_get_weights", "aldic_cnn_predict", "aldic_cnn_add_samples",
-    "aldic_cnn_clear_samples", "aldic_cnn_train", "aldic_cnn_grad",
+    "aldic_cnn_clear_samples", "aldic_cnn_train", "aldic_cnn_grad", "aldic_local_icgn",
 )
 
 
@@ -106,6 +106,8 @@ def load():
     L.aldic_cnn_clear_samples.argtypes = [P]
     L.aldic_cnn_train.argtypes = [P, ip, ip, C.c_float, C.c_ulonglong, vp]
     L.aldic_cnn_grad.argtypes = [P, ip, vp, vp, vp, vp]
+    L.aldic_local_icgn.argtypes = [P, ip, vp, ip, ip, vp, C.c_double, ip, C.c_double,
+                                   vp, vp, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -281,6 +283,26 @@ class Context:
         self._ck(self.L.aldic_cnn_grad(self.h, rows.size, _p(rows), _p(g), _p(loss), _p(out)),
                  "aldic_cnn_grad")
         return g, float(loss[0]), out
+
+    def local_icgn(self, grid, ws, step, s, start="search", tol=1e-2, max_it=20, min_zncc=0.9):
+        """LocalICGN's start and 6-DOF IC-GN over the whole-field search grid, on the device:
+        start "search" (the integer search) or "learned" (the network's warp, the search as
+        fallback).  Returns dict(x, y, U, F, iters, zncc, fallback, n_fallback, times_ms)."""
+        grid = _c(grid, np.int32)
+        sv = _c(np.broadcast_to(np.asarray(s), (2,)), np.int32)
+        g4 = self.search_grid(grid, ws, step, sv)
+        S = g4[2] * g4[3]
+        x, y, zz = np.empty(S), np.empty(S), np.empty(S)
+        U, F = np.empty(2 * S), np.empty(4 * S)
+        it, fb = np.empty(S, np.int32), np.empty(S, np.int32)
+        nfb = np.zeros(1, np.int32)
+        ms = np.zeros(5, np.float32)
+        st = {"search": 0, "learned": 1}[start]
+        self._ck(self.L.aldic_local_icgn(self.h, st, _p(grid), int(ws), int(step), _p(sv), float(tol),
+                                         int(max_it), float(min_zncc), _p(x), _p(y), _p(U), _p(F),
+                                         _p(it), _p(zz), _p(fb), _p(nfb), _p(ms)), "aldic_local_icgn")
+        return dict(x=x, y=y, U=U, F=F, iters=it, zncc=zz, fallback=fb.astype(bool),
+                    n_fallback=int(nfb[0]), times_ms=ms.astype(float), Nx=g4[2], Ny=g4[3])
 
     def icgn2(self, coords, ws, H21, mu, vdual, Uold, Fold, tol=1e-2):
         coords = _c(coords, np.float64).reshape(-1, 2)

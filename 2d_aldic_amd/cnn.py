@@ -1,18 +1,21 @@
 """Learned initial guess for the 6-DOF IC-GN (a start in place of the integer search).
 
-A fixed small network on the BF16 matrix cores (csrc/cnn.hip; C ABI aldic_cnn_*) maps
-each subset's 64 x 64 reference and deformed patches of the normalised images, centred
-on the node, to the six IC-GN parameters (u_x, v_x, u_y, v_y, u, v).  IC-GN then starts
-from that warp (aldic_icgn6_start); the plain integer search is the fallback for every
-subset whose IC-GN did not converge within 20 iterations or ends with ZNCC below 0.9,
-and converged IC-GN stays the judge of every displacement.
+A siamese correlation network on the BF16 matrix cores (csrc/cnn.hip; C ABI aldic_cnn_*)
+maps each subset's 64 x 64 reference and deformed patches of the normalised images,
+centred on the node (plus a 4-pixel context ring), to the six IC-GN parameters (u_x, v_x,
+u_y, v_y, u, v): a shared two-layer encoder on 4 x 4 pixel cells, normalised features,
+cost volumes over displacements of up to 5 cells (20 px) for the four quadrants of the
+central 24 x 24 pixels, a soft-argmax of each and a dense 10 -> 6 head.  IC-GN then
+starts from that warp (aldic_icgn6_start); the plain integer search is the fallback for
+every subset whose IC-GN did not converge within 20 iterations or ends with ZNCC below
+0.9, and converged IC-GN stays the judge of every displacement.
 
-Training data come from the synthetic speckle warper (speckle.make_pair): image pairs
-with a random translation, displacement gradient and smooth sinusoidal perturbation
-per pair; each node is labelled by IC-GN started from the known field rounded to whole
-pixels (converged subsets only).  The trained weights ship as data/cnn_weights.npz,
-written by cnn_train.py.  No counterpart in the reference (its initial guess is the
-FFT search of IntegerSearch.m).
+Training data come from the synthetic speckle warper (speckle.make_pair): many small
+image pairs, each with its own random translation, displacement gradient and smooth
+sinusoidal perturbation; each node is labelled by IC-GN started from the known field
+rounded to whole pixels (converged subsets only).  The trained weights ship as
+data/cnn_weights.npz, written by cnn_train.py.  No counterpart in the reference (its
+initial guess is the FFT search of IntegerSearch.m).
 """
 from __future__ import annotations
 
@@ -23,23 +26,34 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "data", "cnn_weights.npz")
-MARGIN = 48          # nodes at least this far from the border (patch 32 + window 16)
+MARGIN = 48          # nodes at least this far from the border (patch 32 + ring 4 + window 12)
 MAX_IT = 20          # fallback when IC-GN from the learned start needs more
 MIN_ZNCC = 0.9       # or ends below this ZNCC
-
-
-# (outputs, inputs) of the five layers, as csrc/cnn.hip lays out the flat weight vector
-SHAPES = [(32, 32), (64, 288), (64, 576), (128, 1024), (6, 128)]
+MAX_DISP = 19.0      # labels kept within the cost volumes' reach (5 cells of 4 px)
+CH = 16              # encoder channels
+NPARAMS = CH * 16 + CH + CH * 9 * CH + CH + 1 + 6 * 10 + 6
+SCALE = np.array([0.05, 0.05, 0.05, 0.05, 16.0, 16.0])
 
 
 def init_weights(seed=0):
-    """He-normal weights, zero biases, in the device's flat order (FP32)."""
+    """Flat FP32 parameters in the device's order W1 b1 W2 b2 beta Wl bl: He-normal
+    encoder, zero biases, beta = 20, and a head that starts as the geometric estimate
+    (global soft-argmax -> u, v; quadrant differences over their 12 px spacing ->
+    gradients)."""
     rng = np.random.default_rng(seed)
-    parts = []
-    for n, k in SHAPES:
-        parts.append(rng.normal(scale=np.sqrt(2.0 / k), size=n * k))
-        parts.append(np.zeros(n))
-    return np.concatenate(parts).astype(np.float32)
+    W1 = rng.normal(scale=np.sqrt(2.0 / 16), size=(CH, 16))
+    W2 = rng.normal(scale=np.sqrt(1.0 / (9 * CH)), size=(CH, 9 * CH))
+    Wl = np.zeros((6, 10))
+    Wl[4, 0] = 1.0 / SCALE[4]
+    Wl[5, 1] = 1.0 / SCALE[5]
+    for comp, row_x, row_y in ((0, 0, 2), (1, 1, 3)):
+        for q in range(4):
+            qy, qx = divmod(q, 2)
+            Wl[row_x, 2 + 2 * q + comp] = (1.0 if qx else -1.0) / 24.0 / SCALE[row_x]
+            Wl[row_y, 2 + 2 * q + comp] = (1.0 if qy else -1.0) / 24.0 / SCALE[row_y]
+    w = np.concatenate([W1.ravel(), np.zeros(CH), W2.ravel(), np.zeros(CH), [20.0],
+                        Wl.ravel(), np.zeros(6)])
+    return w.astype(np.float32)
 
 
 def load_weights(ctx, path=WEIGHTS):
@@ -50,7 +64,7 @@ def load_weights(ctx, path=WEIGHTS):
     return w
 
 
-def random_field(rng, n, trans=20.0, grad=0.05, wiggle=3.0):
+def random_field(rng, n, trans=16.0, grad=0.05, wiggle=3.0):
     """u(X, Y), v(X, Y): translation U[-trans, trans], gradient U[-grad, grad] about the
     image centre, plus a sinusoid of amplitude U[0, wiggle] px and wavelength U[150, 400]
     px along a random direction (per component)."""
@@ -88,11 +102,12 @@ def _make(args):
     return p.f, p.g, p.params
 
 
-def training_pairs(n_pairs, n=768, seed=0, workers=8):
-    """n_pairs synthetic pairs (n x n) with their fields [(pair, ufun)], made in parallel
-    processes.  The field of each pair is the first draw of its own generator, so the
-    parent rebuilds it from the seed (closures do not cross process boundaries)."""
-    args = [(n, seed * 100003 + k) for k in range(n_pairs)]
+def training_pairs(n_pairs, n=192, seed=0, first=0, workers=8):
+    """Pairs first .. first + n_pairs - 1 of a seeded sequence of synthetic pairs (n x n)
+    with their fields [(pair, ufun)], made in parallel processes.  The field of each pair
+    is the first draw of its own generator, so the parent rebuilds it from the seed
+    (closures do not cross process boundaries)."""
+    args = [(n, seed * 100003 + k) for k in range(first, first + n_pairs)]
     if workers > 1:
         from multiprocessing import get_context
         with get_context("spawn").Pool(workers) as pool:
@@ -165,5 +180,5 @@ def local_icgn(ctx, xy, ws, tol=1e-2, grid=None, s=None, step=None):
     return U, F, it, z, info
 
 
-__all__ = ["SHAPES", "init_weights", "load_weights", "random_field", "training_pairs",
+__all__ = ["NPARAMS", "init_weights", "load_weights", "random_field", "training_pairs",
            "node_grid", "label", "local_icgn", "WEIGHTS", "MAX_IT", "MIN_ZNCC"]

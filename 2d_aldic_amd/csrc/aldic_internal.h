@@ -199,6 +199,7 @@ struct aldic_ctx {
     aldic::DBuf stats;               // ROI partial sums, then {mean_f, 1/std_f, mean_g, 1/std_g}
     aldic::DBuf boxp;                // window box-sum planes of the exact search
     aldic::DBuf scratch[9];          // per-call work buffers (8: IC-GN start gradients)
+    aldic::DBuf lic[8];              // aldic_local_icgn work buffers (kept between calls)
     aldic::MeshDev mesh;
     bool have_mesh = false;
     aldic::QuadDev quad;             // aldic_set_quadtree / aldic_subpb2_quadtree
@@ -273,11 +274,17 @@ struct SearchArgs {
     const double *ms = nullptr;
     DBuf *planes = nullptr;
     hipEvent_t ev_kernel = nullptr;  // recorded just before the main search kernel (nullable)
+    // FP32 search on a subset list only (launch_search_list): grid indices list[i] for
+    // i < *nlist_dev, at most Nx * Ny of them; outputs land at those indices
+    const int *list = nullptr, *nlist_dev = nullptr;
 };
 // Runs an exact int8 matrix-core search when raw pixels are given: k_search_i8 for 8-bit
 // images with 2s+3 <= 32 lags per axis and ws + 2s + 3 <= 64, k_search_mt for 16-bit
 // images or larger lag maps (ws <= 64); else the FP32 search on a.f, a.g.
 hipError_t launch_search(const SearchArgs &a, hipStream_t st);
+// The FP32 search (k_search on a.f, a.g) over the first n subsets a.list names (n at
+// most *a.nlist_dev).
+hipError_t launch_search_list(const SearchArgs &a, int n, hipStream_t st);
 size_t search_lds_bytes(int ws, int sx, int sy);
 bool search_i8_supported(int ws, int sx, int sy);
 hipError_t launch_search_i8(const SearchArgs &a, hipStream_t st);
@@ -293,16 +300,25 @@ hipError_t launch_qfactors(const SearchArgs &a, hipStream_t st);
 OutlierWork *outlier_work_new();
 void outlier_work_free(OutlierWork *w);
 
-// Learned initial guess (cnn.hip): a fixed small network on 64 x 64 patch pairs.  All
-// pointers are device pointers unless named _host; every call enqueues on st.
+// Learned initial guess (cnn.hip): a siamese correlation network on the 64 x 64 patch
+// pairs (plus a 4-pixel context ring) of the nodes.  All pointers are device pointers
+// unless named _host; every call enqueues on st.
 CnnWork *cnn_work_new();
 void cnn_work_free(CnnWork *c);
 long cnn_nparams();
 hipError_t cnn_set_weights(CnnWork &c, const float *w_host, hipStream_t st);  // waits
 hipError_t cnn_get_weights(CnnWork &c, float *w_host, hipStream_t st);        // waits
+// How cnn_forward runs on a node set (host arithmetic on the nodes, 2 per node, 1-based):
+// on one 4-pixel lattice of px0, py0 and nx1 x ny1 E1 cells, or per patch
+struct CnnPlan {
+    bool lattice = false;
+    int px0 = 0, py0 = 0;
+    long nx1 = 0, ny1 = 0;
+};
+CnnPlan cnn_plan(const double *xy_host, int S);
 // P (6 per node: u_x, v_x, u_y, v_y, u, v) at nodes xy (2 per node, 1-based) from f, g
 hipError_t cnn_forward(CnnWork &c, const float *f, const float *g, int nx, int ny,
-                       const double *xy, int S, double *P, hipStream_t st);
+                       const double *xy, const CnnPlan &plan, int S, double *P, hipStream_t st);
 // appends the nodes' patch pairs and targets P (6 per node) to the training set
 hipError_t cnn_add_samples(CnnWork &c, const float *f, const float *g, int nx, int ny,
                            const double *xy, const double *P, int S, hipStream_t st);
@@ -315,6 +331,12 @@ hipError_t cnn_step(CnnWork &c, const int *idx_host, int B, float lr, double *lo
 const float *cnn_grad(const CnnWork &c);     // the last step's gradient (nparams)
 bool cnn_have_weights(const CnnWork &c);
 const float *cnn_outputs(const CnnWork &c);  // the last forward's scaled outputs (6 B)
+// LocalICGN with the learned start (aldic_local_icgn): node centres of a search grid,
+// the network's P split into U0 and F0, and the fallback flags
+hipError_t launch_grid_coords(int x1, int y1, int Nx, int Ny, int step, int ws, double *xy, hipStream_t st);
+hipError_t launch_split_start(const double *P, int S, double *U0, double *F0, hipStream_t st);
+hipError_t launch_flag_fallback(const int *it, const double *z, int S, int max_it, double min_z, int *flag,
+                                hipStream_t st);
 int inpaint_view(OutlierWork &w, double *p, int rows, int cols, long si, long sj, int method,
                  hipStream_t st, const char *&msg);
 int remove_outliers_views(OutlierWork &w, double *up, double *vp, int rows, int cols, long si,

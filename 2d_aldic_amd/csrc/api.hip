@@ -259,6 +259,7 @@ void aldic_destroy(aldic_ctx *ctx)
                     &ctx->mesh.mval, &ctx->mesh.diag, &ctx->mesh.gvec, &ctx->mesh.fdV, &ctx->mesh.fdlam};
     for (DBuf *b : bufs) b->release();
     for (auto &b : ctx->scratch) b.release();
+    for (auto &b : ctx->lic) b.release();
     FrameState &F = ctx->fr;
     DBuf *fb[] = {&F.coords, &F.U0, &F.U6, &F.F6, &F.H21, &F.it6, &F.zncc, &F.Ucur, &F.Fcur,
                   &F.U1, &F.F1, &F.U2, &F.F2, &F.udual, &F.vdual, &F.it2, &F.bad, &F.tmp,
@@ -1128,7 +1129,7 @@ int aldic_cnn_predict(aldic_ctx *ctx, int S, const double *coords, double *P)
     CK(up(sc[0], coords, 2 * (size_t)S, ctx->stream));
     CK(sc[1].ensure(sizeof(double) * 6 * (size_t)S));
     CK(cnn_forward(*ctx->cnn, ctx->f.as<float>(), ctx->g.as<float>(), ctx->nx, ctx->ny,
-                   sc[0].as<double>(), S, sc[1].as<double>(), ctx->stream));
+                   sc[0].as<double>(), cnn_plan(coords, S), S, sc[1].as<double>(), ctx->stream));
     CK(down(P, sc[1].p, 6 * (size_t)S, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     return ALDIC_OK;
@@ -1220,6 +1221,179 @@ int aldic_cnn_grad(aldic_ctx *ctx, int B, const int *rows, float *grad, double *
         for (size_t i = 0; i < y.size(); ++i) outputs[i] = y[i];
     }
     CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+// ---- LocalICGN from the search or from the learned start -------------------------------
+// The start and 6-DOF IC-GN of LocalICGN over the whole-field search grid, resident on
+// the device: start 0 = the integer search (IntegerSearch.m:54 -> LocalICGN.m:91-117),
+// start 1 = the learned warp (aldic_cnn_predict on the grid's nodes) with the FP32
+// search and IC-GN from its peak redone on the subsets whose IC-GN from the warp did not
+// converge within max_it residual evaluations or ended below min_zncc.
+int aldic_local_icgn(aldic_ctx *ctx, int start, const int *grid, int ws, int step, const int *s, double tol,
+                     int max_it, double min_zncc, double *x, double *y, double *U, double *F, int *iters,
+                     double *zncc, int *fallback, int *n_fallback, float *times_ms)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "local_icgn: no images");
+    if ((start != 0 && start != 1) || !grid || !s || !U || !F || !iters || !(tol > 0) || max_it < 1 ||
+        s[0] < 0 || s[1] < 0)
+        return fail(ctx, ALDIC_ERR_ARG, "local_icgn: bad arguments");
+    int gr[4];
+    if (aldic_search_grid(ctx->nx, ctx->ny, grid, ws, step, s, gr))
+        return fail(ctx, ALDIC_ERR_ARG, "local_icgn: bad grid arguments");
+    const int S = gr[2] * gr[3];
+    if (n_fallback) *n_fallback = 0;
+    if (S <= 0) return ALDIC_OK;
+    const int ppl = icgn_ppl_for(ws);
+    if (ppl == 0 || ws < 2) return fail(ctx, ALDIC_ERR_ARG, "local_icgn: subset size must be 2..64");
+    if (!search_fits(ctx, start == 0 && ctx->images_from_raw, ws, s[0], s[1]))
+        return fail(ctx, ALDIC_ERR_ARG, "local_icgn: window too large for LDS");
+    CK(hipSetDevice(ctx->device));
+    if (start == 1)
+        if (int r = cnn_ready(ctx, "local_icgn", true)) return r;
+    hipStream_t st = ctx->stream;
+    DBuf *w = ctx->lic;
+    // 0: coords (2S), 1: U0 (2S) then the search's x, y, u, v, cc (5S), 2: P (6S), 3: F0 (4S),
+    // 4: U, F (6S), 5: H21 (21S), 6: zncc (S) + iters (S ints), 7: flags, list, count, scratch
+    CK(w[0].ensure(sizeof(double) * 2 * (size_t)S));
+    CK(w[1].ensure(sizeof(double) * 7 * (size_t)S + sizeof(int) * (size_t)S));
+    CK(w[2].ensure(sizeof(double) * 6 * (size_t)S));
+    CK(w[3].ensure(sizeof(double) * 4 * (size_t)S));
+    CK(w[4].ensure(sizeof(double) * 6 * (size_t)S));
+    CK(w[5].ensure(sizeof(double) * 21 * (size_t)S));
+    CK(w[6].ensure(sizeof(double) * (size_t)S + sizeof(int) * (size_t)S));
+    // flags (S), fallback list (S), its count (2), list-building runs, -2 list (S + 1)
+    const size_t nrun = ((size_t)S + 4095) / 4096;
+    CK(w[7].ensure(sizeof(int) * (3 * (size_t)S + 3 + nrun)));
+    double *xy = w[0].as<double>(), *U0 = w[1].as<double>(), *so = U0 + 2 * (size_t)S;
+    double *Ud = w[4].as<double>(), *Fd = Ud + 2 * (size_t)S, *zd = w[6].as<double>();
+    int *itd = reinterpret_cast<int *>(zd + S), *flag = w[7].as<int>(), *list = flag + S, *cnt = list + S;
+    int *runs = cnt + 2, *rl = runs + nrun;
+    // the node centres on the host and the network's plan, before the timed span
+    std::vector<double> xyh(2 * (size_t)S);
+    for (int i = 0; i < S; ++i) {
+        xyh[2 * i] = (2.0 * (gr[0] + (i % gr[2]) * step) + ws) / 2.0;
+        xyh[2 * i + 1] = (2.0 * (gr[1] + (i / gr[2]) * step) + ws) / 2.0;
+    }
+    const CnnPlan plan = start == 1 ? cnn_plan(xyh.data(), S) : CnnPlan{};
+    hipEvent_t *ev = ctx->ev;
+    CK(hipEventRecord(ev[0], st));
+    CK(launch_grid_coords(gr[0], gr[1], gr[2], gr[3], step, ws, xy, st));
+    SearchArgs sa{};
+    sa.f = ctx->f.as<float>();
+    sa.g = ctx->g.as<float>();
+    sa.nx = ctx->nx;
+    sa.ny = ctx->ny;
+    sa.x1 = gr[0];
+    sa.y1 = gr[1];
+    sa.Nx = gr[2];
+    sa.Ny = gr[3];
+    sa.ws = ws;
+    sa.step = step;
+    sa.sx = s[0];
+    sa.sy = s[1];
+    sa.x = so;
+    sa.y = so + S;
+    sa.u = so + 2 * (size_t)S;
+    sa.v = so + 3 * (size_t)S;
+    sa.ccmax = so + 4 * (size_t)S;
+    sa.status = reinterpret_cast<int *>(so + 5 * (size_t)S);
+    sa.U0 = U0;
+    const double *F0 = nullptr;
+    if (start == 0) {
+        search_raw8(ctx, sa);
+        CK(launch_search(sa, st));
+    } else {
+        CK(cnn_forward(*ctx->cnn, ctx->f.as<float>(), ctx->g.as<float>(), ctx->nx, ctx->ny, xy, plan, S,
+                       w[2].as<double>(), st));
+        CK(launch_split_start(w[2].as<double>(), S, U0, w[3].as<double>(), st));
+        F0 = w[3].as<double>();
+    }
+    CK(hipEventRecord(ev[1], st));
+    IcgnArgs a{};
+    a.f = ctx->f.as<float>();
+    a.g = ctx->g.as<float>();
+    a.gx = ctx->gx.as<float>();
+    a.gy = ctx->gy.as<float>();
+    a.nx = ctx->nx;
+    a.ny = ctx->ny;
+    a.S = S;
+    a.coords = xy;
+    a.ws_all = ws;
+    a.tol = tol;
+    a.U0 = U0;
+    a.F0 = F0;
+    a.U = Ud;
+    a.F = Fd;
+    a.H21 = w[5].as<double>();
+    a.zncc = zd;
+    a.iters = itd;
+    a.nlist = S;
+    CK(launch_icgn6(a, ppl, st));
+    // windows the mask rule enlarged beyond the class (status -2) run one class up
+    const int ppl_big = icgn_ppl_for((int)std::floor(std::sqrt(2.0) * ws + 0.5));
+    if (ppl_big != 0 && ppl_big != ppl) {
+        IcgnArgs ar = a;
+        CK(launch_list_eq(itd, S, -2, rl, rl + S, runs, st));
+        ar.list = rl;
+        ar.nlist_dev = rl + S;
+        CK(launch_icgn6(ar, ppl_big, st));
+    }
+    CK(hipEventRecord(ev[2], st));
+    int nfb = 0;
+    if (start == 1) {
+        // fallback: FP32 search on the flagged subsets, IC-GN from its peaks there
+        // (the list's length comes back to the host so that the search launches one
+        // workgroup per listed subset only)
+        CK(launch_flag_fallback(itd, zd, S, max_it, min_zncc, flag, st));
+        CK(launch_list_eq(flag, S, 1, list, cnt, runs, st));
+        CK(hipMemcpyAsync(&nfb, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        SearchArgs sl = sa;
+        sl.list = list;
+        sl.nlist_dev = cnt;
+        CK(launch_search_list(sl, nfb, st));
+        CK(hipEventRecord(ev[3], st));
+        if (nfb > 0) {
+            IcgnArgs af = a;
+            af.F0 = nullptr;
+            af.list = list;
+            af.nlist = nfb;
+            CK(launch_icgn6(af, ppl, st));
+            if (ppl_big != 0 && ppl_big != ppl) {
+                IcgnArgs ar = a;
+                ar.F0 = nullptr;
+                CK(launch_list_eq(itd, S, -2, rl, rl + S, runs, st));
+                ar.list = rl;
+                ar.nlist_dev = rl + S;
+                CK(launch_icgn6(ar, ppl_big, st));
+            }
+        }
+        CK(hipEventRecord(ev[4], st));
+    } else {
+        CK(hipEventRecord(ev[3], st));
+        CK(hipEventRecord(ev[4], st));
+    }
+    CK(hipStreamSynchronize(st));
+    if (n_fallback) *n_fallback = nfb;
+    if (times_ms) {
+        for (int k = 0; k < 4; ++k) CK(hipEventElapsedTime(&times_ms[k], ev[k], ev[k + 1]));
+        CK(hipEventElapsedTime(&times_ms[4], ev[0], ev[4]));
+    }
+    CK(down(U, Ud, 2 * (size_t)S, st));
+    CK(down(F, Fd, 4 * (size_t)S, st));
+    CK(down(iters, itd, S, st));
+    if (zncc) CK(down(zncc, zd, S, st));
+    if (fallback) {
+        if (start == 1) CK(down(fallback, flag, S, st));
+    }
+    CK(hipStreamSynchronize(st));
+    if (fallback && start == 0) std::fill(fallback, fallback + S, 0);
+    for (int i = 0; i < S; ++i) {
+        if (x) x[i] = xyh[2 * i];
+        if (y) y[i] = xyh[2 * i + 1];
+    }
     return ALDIC_OK;
 }
 

@@ -184,7 +184,11 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a, SearchGeom G)
     // so neighbouring windows share an L2
     const int S = a.Nx * a.Ny;
     const int b = blockIdx.x, xcd = b & 7, per = S >> 3, rm = S & 7;
-    const int q = xcd * per + min(xcd, rm) + (b >> 3);
+    int q = xcd * per + min(xcd, rm) + (b >> 3);
+    if (a.list) {  // subset list: workgroup b takes entry b (whole workgroups leave)
+        if (b >= *a.nlist_dev) return;
+        q = a.list[b];
+    }
     const int k = q % a.Nx, l = q / a.Nx;
     const int ii = a.x1 + k * a.step, jj = a.y1 + l * a.step;  // 1-based corner
     const int tw = G.tw, twp = G.twp, nx = a.nx;
@@ -386,6 +390,21 @@ hipError_t launch_search(const SearchArgs &a, hipStream_t st)
     case 5: return launch_x<5>(a, G, S, st);
     case 7: return launch_x<7>(a, G, S, st);
     default: return launch_x<9>(a, G, S, st);
+    }
+}
+
+hipError_t launch_search_list(const SearchArgs &a, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (!a.list || !a.nlist_dev || n > a.Nx * a.Ny) return hipErrorInvalidValue;
+    bool ok;
+    const SearchGeom G = search_geom(a.ws, a.sx, a.sy, ok);
+    if (!ok) return hipErrorInvalidValue;
+    switch (G.X) {
+    case 3: return launch_x<3>(a, G, n, st);
+    case 5: return launch_x<5>(a, G, n, st);
+    case 7: return launch_x<7>(a, G, n, st);
+    default: return launch_x<9>(a, G, n, st);
     }
 }
 

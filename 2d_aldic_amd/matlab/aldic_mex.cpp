@@ -288,6 +288,34 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         check(aldic_cnn_grad(g_ctx, B, rows.data(), static_cast<float *>(mxGetData(plhs[0])),
                              mxGetDoubles(plhs[1]), mxGetDoubles(plhs[2])),
               "cnn_grad");
+    } else if (cmd == "local_icgn") {
+        // [x, y, U, F, iters, zncc, fallback, ms] = aldic_mex('local_icgn', nx, ny, start, grid, ws,
+        //     step, s, tol, max_it, min_zncc)
+        need(nrhs, 11, cmd.c_str());
+        const std::vector<int> grid = ints(prhs[4], 0), s = ints(prhs[7], 0);
+        const int ws = (int)mxGetScalar(prhs[5]), step = (int)mxGetScalar(prhs[6]);
+        int g4[4];
+        if (aldic_search_grid((int)mxGetScalar(prhs[1]), (int)mxGetScalar(prhs[2]), grid.data(), ws,
+                              step, s.data(), g4))
+            mexErrMsgIdAndTxt("aldic:args", "local_icgn: bad grid");
+        const int S = g4[2] * g4[3];
+        plhs[0] = dmat(S, 1);
+        plhs[1] = dmat(S, 1);
+        plhs[2] = dmat(2 * (size_t)S, 1);
+        plhs[3] = dmat(4 * (size_t)S, 1);
+        plhs[4] = imat(S, 1);
+        plhs[5] = dmat(S, 1);
+        plhs[6] = imat(S, 1);
+        std::vector<float> ms(5);
+        int nfb = 0;
+        check(aldic_local_icgn(g_ctx, (int)mxGetScalar(prhs[3]), grid.data(), ws, step, s.data(),
+                               mxGetScalar(prhs[8]), (int)mxGetScalar(prhs[9]), mxGetScalar(prhs[10]),
+                               mxGetDoubles(plhs[0]), mxGetDoubles(plhs[1]), mxGetDoubles(plhs[2]),
+                               mxGetDoubles(plhs[3]), iptr(plhs[4]), mxGetDoubles(plhs[5]), iptr(plhs[6]),
+                               &nfb, ms.data()),
+              "local_icgn");
+        plhs[7] = dmat(1, 5);
+        for (int k = 0; k < 5; ++k) mxGetDoubles(plhs[7])[k] = ms[k];
     } else if (cmd == "icgn2") {
         // [U, iters] = aldic_mex('icgn2', coords, ws, H21 (21 x S), mu, vdual, Uold, Fold, tol)
         need(nrhs, 9, cmd.c_str());

@@ -114,6 +114,8 @@ def main():
                     help="fraction of the subsets the FP64 oracle checks and times (workload 3)")
     ap.add_argument("--no-smooth", action="store_true",
                     help="skip the smoothing-on frames (funSmoothDisp / funSmoothStrain)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip LocalICGN from the learned start against the search start")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -191,6 +193,10 @@ def main():
             ctx.frame_run()
         smooth_ms = (time.perf_counter() - t0) * 1e3 / nsm
         log(f"[bench] smoothing on: {smooth_ms:.3f} ms/frame")
+    learned = None
+    if args.workload == 3 and not args.no_learned:
+        learned = learned_start(ctx, pkg, pair, grid, ws, step, srad, args.steps)
+        log(f"[bench] learned start: {learned}")
     it6 = out["iters6"]
     conv = int(np.sum((it6 >= 1) & (it6 <= 100)))
     value = conv / (ms_per_step / 1e3)
@@ -287,13 +293,60 @@ def main():
                    "stage_ms": {k: round(v, 4) for k, v in stage.items()},
                    "stage_ms_from": "separate untimed frames with an event at every stage boundary",
                    "gridfit": counters,
-                   "icgn6_subsets_per_s": round(conv / t_i, 1)},
+                   "icgn6_subsets_per_s": round(conv / t_i, 1),
+                   "learned_start": learned},
         "roofline": roof,
         extra_key: extra_roof,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": acc or None,
     }
     print(json.dumps(line), flush=True)
+
+
+def learned_start(ctx, pkg, pair, grid, ws, step, srad, reps):
+    """LocalICGN's start + 6-DOF IC-GN over the whole grid (aldic_local_icgn), from the
+    integer search and from the learned warp (search fallback on the subsets that miss
+    20 iterations or ZNCC 0.9), on the same pair: device time from HIP events around the
+    stage (results' copies to the host excluded), converged subsets, iterations, fallback
+    count, and both forms' U against each other and against the analytic field."""
+    cnn = importlib.import_module("2d_aldic_amd.cnn")
+    ctx.set_images(pair.f, pair.g)
+    cnn.load_weights(ctx)
+    res, ms = {}, {}
+    for start in ("search", "learned"):
+        ctx.local_icgn(grid, ws, step, [srad, srad], start=start)          # warmup
+        acc = np.zeros(5)
+        n = max(3, min(reps, 10))
+        for _ in range(n):
+            r = ctx.local_icgn(grid, ws, step, [srad, srad], start=start)
+            acc += r["times_ms"]
+        res[start], ms[start] = r, acc / n
+    out = {}
+    xy = np.column_stack([res["search"]["x"], res["search"]["y"]])
+    ua, va = pair.ufun(xy[:, 0], xy[:, 1])
+    for start, r in res.items():
+        ok = (r["iters"] >= 1) & (r["iters"] <= 100)
+        U = r["U"].reshape(-1, 2)
+        err = np.hypot(U[ok, 0] - ua[ok], U[ok, 1] - va[ok])
+        out[start] = dict(stage_ms=round(float(ms[start][4]), 4),
+                          start_ms=round(float(ms[start][0]), 4),
+                          icgn_ms=round(float(ms[start][1]), 4),
+                          fallback_ms=round(float(ms[start][2] + ms[start][3]), 4),
+                          converged=int(ok.sum()),
+                          converged_subsets_per_s=round(float(ok.sum()) / (ms[start][4] / 1e3), 1),
+                          mean_iters=round(float(r["iters"][ok].mean()), 4),
+                          fallback=int(r["n_fallback"]),
+                          rms_vs_field_px=round(float(np.sqrt(np.mean(err ** 2))), 5))
+    both = ((res["search"]["iters"] >= 1) & (res["search"]["iters"] <= 100) &
+            (res["learned"]["iters"] >= 1) & (res["learned"]["iters"] <= 100))
+    d = (res["learned"]["U"] - res["search"]["U"]).reshape(-1, 2)[both]
+    out["learned_vs_search_rms_px"] = float(np.sqrt(np.mean(np.sum(d ** 2, 1))))
+    out["learned_vs_search_max_px"] = float(np.abs(d).max())
+    out["speedup"] = round(out["learned"]["converged_subsets_per_s"] /
+                           out["search"]["converged_subsets_per_s"], 3)
+    out["note"] = ("LocalICGN stage only (start + 6-DOF IC-GN over 247,009 subsets); the "
+                   "search start is the exact 12-bit integer search")
+    return out
 
 
 def cpu_full_frame(oracle, pair, out, S, ws, step, srad, grid, mu, beta, n_admm, threads):

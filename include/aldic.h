@@ -355,13 +355,17 @@ int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *ite
                      double *times);
 
 /* ---- learned initial guess (no counterpart in the reference) ------------------------
- * A fixed small network (64 x 64 patch pairs of the normalised images centred on a node:
- * 4 x 4 stride-4 patchify to 32 channels, two 3 x 3 stride-2 convolutions to 64, dense
- * 1024 -> 128 -> 6, ReLU between) on the BF16 matrix cores, proposing each subset's six
- * IC-GN parameters (u_x, v_x, u_y, v_y, u, v) in place of the integer search.  Weights
- * (FP32, aldic_cnn_nparams() of them) are set by the caller or trained here on a
- * device-held set of patch pairs with targets (mean squared error on (F / 0.05, U / 16),
- * Adam).  The patches of nodes within 32 px of the border are clamped into the image. */
+ * A siamese correlation network on the BF16 matrix cores proposes each subset's six IC-GN
+ * parameters (u_x, v_x, u_y, v_y, u, v) in place of the integer search: per node the
+ * 64 x 64 patches of the normalised f and g centred on it, plus a 4-pixel context ring
+ * (72 x 72); a shared encoder on 4 x 4 pixel cells (patchify to 16 channels + ReLU, 3 x 3
+ * 'valid' convolution), features normalised per cell, cost volumes of the four quadrants
+ * of the central 24 x 24 pixels over displacements of -5..5 cells, a soft-argmax of each
+ * (and of their mean) and a dense 10 -> 6 head.  Weights (FP32, aldic_cnn_nparams() of
+ * them) are set by the caller or trained here on a device-held set of patch pairs with
+ * targets (mean squared error on (F / 0.05, U / 16), Adam).  Pixels outside the image are
+ * clamped.  On node sets whose inputs share one 4-pixel lattice (grids with a step that
+ * is a multiple of 4) the encoder runs once per image. */
 long aldic_cnn_nparams(void);
 int aldic_cnn_set_weights(aldic_ctx *ctx, const float *w, long n);   /* resets Adam state */
 int aldic_cnn_get_weights(aldic_ctx *ctx, float *w, long n);
@@ -379,6 +383,18 @@ int aldic_cnn_train(aldic_ctx *ctx, int steps, int batch, float lr, unsigned lon
  * update (for gradient checks). */
 int aldic_cnn_grad(aldic_ctx *ctx, int B, const int *rows, float *grad, double *loss,
                    double *outputs);
+/* LocalICGN's start and 6-DOF IC-GN (LocalICGN.m:91-117, without the bad-point refill)
+ * over the whole-field search grid of aldic_integer_search (grid, ws, step, s as there),
+ * resident on the device.  start 0: from the integer search (IntegerSearch.m:54).  start
+ * 1: from the learned warp of every node, then, on the subsets whose IC-GN did not reach
+ * status 1..max_it or ended with ZNCC below min_zncc (fallback[i] = 1), the FP32 integer
+ * search and IC-GN from its peak.  Outputs per node (S = Nx * Ny, x fastest): x, y
+ * (nullable), U (2S), F (4S), iters, zncc and fallback (nullable); n_fallback (nullable);
+ * times_ms (5, nullable): start (search or network), IC-GN, fallback search, fallback
+ * IC-GN, total, from HIP events on the library's stream. */
+int aldic_local_icgn(aldic_ctx *ctx, int start, const int *grid, int ws, int step, const int *s, double tol,
+                     int max_it, double min_zncc, double *x, double *y, double *U, double *F, int *iters,
+                     double *zncc, int *fallback, int *n_fallback, float *times_ms);
 
 #ifdef __cplusplus
 }
