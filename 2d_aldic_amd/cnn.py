@@ -32,7 +32,7 @@ MIN_ZNCC = 0.9       # or ends below this ZNCC
 MAX_DISP = 19.0      # labels kept within the cost volumes' reach (5 cells of 4 px)
 CH = 16              # encoder channels
 NPARAMS = CH * 16 + CH + CH * 9 * CH + CH + 1 + 6 * 10 + 6
-SCALE = np.array([0.05, 0.05, 0.05, 0.05, 16.0, 16.0])
+SCALE = np.array([0.05, 0.05, 0.05, 0.05, 4.0, 4.0])
 
 
 def init_weights(seed=0):

@@ -1,7 +1,7 @@
 """Trains the learned initial guess (cnn.py) on the GPU and writes its weights.
 
     python 2d_aldic_amd/cnn_train.py [--out 2d_aldic_amd/data/cnn_weights.npz]
-        [--pairs 4000] [--size 192] [--step 12] [--steps 6000] [--batch 256] [--seed 0]
+        [--pairs 16000] [--size 192] [--step 12] [--steps 20000] [--batch 256] [--seed 0]
 
 Synthetic pairs (cnn.training_pairs), each with its own random field, nodes every `step`
 px at least 48 px inside, labels from IC-GN started at the known field, kept where IC-GN
@@ -30,10 +30,10 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(ROOT, "2d_aldic_amd", "data", "cnn_weights.npz"))
-    ap.add_argument("--pairs", type=int, default=4000)
+    ap.add_argument("--pairs", type=int, default=16000)
     ap.add_argument("--size", type=int, default=192)
     ap.add_argument("--step", type=int, default=12)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()

@@ -9,7 +9,7 @@
 //   C   quadrant cost volumes over the central 6 x 6 cells of f and displacements of -5..5
 //       cells: Q[q][d] = 1/9 sum_{p in q} <nf(p), ng(p + d)>, and their mean G
 //   S   soft-argmax of G and the four Q (learned sharpness beta): 10 displacements (px)
-//   L   dense 10 -> 6: the scaled outputs (F / 0.05, U / 16)
+//   L   dense 10 -> 6: the scaled outputs (F / 0.05, U / 4)
 //
 // E1 and E2 are GEMMs on the BF16 matrix cores (v_mfma_f32_16x16x16bf16_1k, FP32
 // accumulation) whose operands are gathered on the fly (implicit im2col) and rounded to
@@ -47,7 +47,7 @@ constexpr long kOffW1 = 0, kOffB1 = kOffW1 + kC * 16;
 constexpr long kOffW2 = kOffB1 + kC, kOffB2 = kOffW2 + kC * 9 * kC;
 constexpr long kOffBeta = kOffB2 + kC, kOffWl = kOffBeta + 1, kOffBl = kOffWl + kOut * kNF;
 constexpr long kNParams = kOffBl + kOut;
-__constant__ float c_scale[kOut] = {0.05f, 0.05f, 0.05f, 0.05f, 16.f, 16.f};
+__constant__ float c_scale[kOut] = {0.05f, 0.05f, 0.05f, 0.05f, 4.f, 4.f};
 
 // ---- operand loaders: element (row, col) of a GEMM operand -----------------------------
 struct LdDense {  // X[r * ld + c]

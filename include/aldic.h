@@ -363,7 +363,7 @@ int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *ite
  * of the central 24 x 24 pixels over displacements of -5..5 cells, a soft-argmax of each
  * (and of their mean) and a dense 10 -> 6 head.  Weights (FP32, aldic_cnn_nparams() of
  * them) are set by the caller or trained here on a device-held set of patch pairs with
- * targets (mean squared error on (F / 0.05, U / 16), Adam).  Pixels outside the image are
+ * targets (mean squared error on (F / 0.05, U / 4), Adam).  Pixels outside the image are
  * clamped.  On node sets whose inputs share one 4-pixel lattice (grids with a step that
  * is a multiple of 4) the encoder runs once per image. */
 long aldic_cnn_nparams(void);

@@ -11,7 +11,7 @@ IntegerSearch.m) is a siamese correlation network on 4 x 4 pixel cells:
       16), displacement d = (dx, dy) in -5..5 cells: Q[q, d] = 1/9 sum_{p in q} sum_ch
       nf[p] ng[p + d]; the global volume G = mean of the four
   S   soft-argmax of each of the five volumes: w = softmax(beta V), m = sum_d w (4 dx, 4 dy)
-  L   y = Wl m + bl (10 -> 6), the scaled outputs (u_x, v_x, u_y, v_y) / 0.05, (u, v) / 16
+  L   y = Wl m + bl (10 -> 6), the scaled outputs (u_x, v_x, u_y, v_y) / 0.05, (u, v) / 4
 
 Input per node: a 72 x 72 pair (f, g) whose central 64 x 64 is the node's patch (first pixel
 x0 - 32 for the 1-based node x0); the 4-pixel ring is the context of E2 at the patch's
@@ -33,7 +33,7 @@ C = 16
 R = 5
 ND = (2 * R + 1) ** 2
 SHAPES = [(C, 16), (C, 9 * C)]                 # (N, K) of the two GEMM layers
-SCALE = np.array([0.05, 0.05, 0.05, 0.05, 16.0, 16.0])
+SCALE = np.array([0.05, 0.05, 0.05, 0.05, 4.0, 4.0])
 EPS = 1e-3
 BETA0 = 20.0
 PATCH = 72
