@@ -19,15 +19,19 @@
 //   along the two hull edges.  Fewer than three non-collinear good nodes: the nearest
 //   good node (ties to the lowest index); none: NaN.
 //
-// Kernels (one refill = 6 launches, no host wait):
+// Kernels (one refill = 7 launches, no host wait):
 //   k_nat_rows      one wave per row of equal y: first / last good node, good count
 //   k_nat_hull      one workgroup: good count, the row extremes pruned by their
 //                   eight-direction extreme polygon, strict convex hull of the rest
 //                   (monotone chain), counters cleared
 //   k_nat_classify  per bad node: inside the hull -> inside list, else -> hull list
 //   k_nat_sibson    one thread per inside node, its two polygons in LDS (24 lines);
-//                   a cell with more natural neighbours goes to a list for
-//   k_nat_sibson_big the same with polygons of up to 512 lines in device memory
+//                   a cell with more natural neighbours, or still open past ring
+//                   kNatRingCap, goes to a list for
+//   k_nat_sibson_wave one wave per node: the cell built by wave-parallel clips in LDS
+//                   (512 lines), each natural neighbour's region on its own lane;
+//                   nodes beyond those limits go to
+//   k_nat_sibson_big one thread per node, polygons of up to 512 lines in device memory
 //   k_nat_hullfill  one wave per hull-list node, scans over rings of buckets
 #include <algorithm>
 #include <cmath>
@@ -45,6 +49,10 @@ constexpr int kNatThreads = 64;    // k_nat_sibson workgroup
 constexpr int kNatBig = 512;       // lines per polygon in the second pass (device memory)
 constexpr int kNatBigThreads = 1024;  // second-pass threads (polygon slots)
 constexpr int kRegion = 0x40000000;  // tag bit: bisector of p_i and p_(tag & ~kRegion)
+constexpr int kNatRingCap = 1;     // first pass: a cell still open past ring 1 -> big list
+constexpr int kNatWaveV = 512;     // k_nat_sibson_wave: lines of the cell (LDS, two buffers)
+constexpr int kNatWaveR = 24;      // lines of each lane's region polygon (LDS)
+constexpr int kNatWaveCand = 1024; // candidates gathered per chunk of 64 ring buckets
 
 struct NatArgs {
     const double2 *xy;
@@ -65,6 +73,7 @@ struct NatArgs {
     int *hull;       // 2 R + 2 (counter-clockwise, from the lowest-x, lowest-y vertex)
     int *ctl;        // {hull size, good count, n inside, n hull, errors, n big}
     int *lin, *lhull, *lbig;  // nn each
+    int *lbig2;      // nn: nodes beyond k_nat_sibson_wave's limits (ctl[6] of them)
     int *err;        // nullable: refills whose cell could not be formed are also counted here
     // second pass: kNatBigThreads polygon pairs of kNatBig lines, entry k of slot g at
     // [k * kNatBigThreads + g]
@@ -233,6 +242,7 @@ __global__ __launch_bounds__(256) void k_nat_hull(NatArgs a)
     a.ctl[3] = 0;
     a.ctl[4] = 0;
     a.ctl[5] = 0;
+    a.ctl[6] = 0;
 }
 
 // Inside = strictly left of every hull edge (relative tolerance 1e-12).  The hull is
@@ -290,9 +300,26 @@ struct Line {
     double nx, ny, c;
 };
 
+// Site lookups: coordinates of the site a tag names, relative to q.  SitesG reads the
+// node table (tags are node indices), SitesL an LDS copy of one cell's sites (tags are
+// local indices); both subtract q the same way.
+struct SitesG {
+    const double2 *xy;
+    double2 Q;
+    __device__ double2 rel(int t) const
+    {
+        const double2 P = xy[t];
+        return make_double2(P.x - Q.x, P.y - Q.y);
+    }
+};
+struct SitesL {
+    const double *sx, *sy;
+    __device__ double2 rel(int t) const { return make_double2(sx[t], sy[t]); }
+};
+
 // Line of a tag, coordinates relative to q; pi = p_i - q for region tags.
-__device__ __forceinline__ Line nat_line(const NatArgs &a, double2 Q, double B, int tag,
-                                         double2 pi)
+template <class S>
+__device__ __forceinline__ Line nat_line_s(const S &st, double B, int tag, double2 pi)
 {
     Line L;
     if (tag < 0) {
@@ -301,19 +328,25 @@ __device__ __forceinline__ Line nat_line(const NatArgs &a, double2 Q, double B, 
         L.ny = k == 1 ? 1.0 : (k == 3 ? -1.0 : 0.0);
         L.c = B;
     } else if (tag & kRegion) {
-        const double2 P = a.xy[tag & ~kRegion];
-        const double px = P.x - Q.x, py = P.y - Q.y;
+        const double2 P = st.rel(tag & ~kRegion);
+        const double px = P.x, py = P.y;
         L.nx = px - pi.x;
         L.ny = py - pi.y;
         L.c = 0.5 * ((px * px + py * py) - (pi.x * pi.x + pi.y * pi.y));
     } else {
-        const double2 P = a.xy[tag];
-        const double px = P.x - Q.x, py = P.y - Q.y;
+        const double2 P = st.rel(tag);
+        const double px = P.x, py = P.y;
         L.nx = px;
         L.ny = py;
         L.c = 0.5 * (px * px + py * py);
     }
     return L;
+}
+
+__device__ __forceinline__ Line nat_line(const NatArgs &a, double2 Q, double B, int tag,
+                                         double2 pi)
+{
+    return nat_line_s(SitesG{a.xy, Q}, B, tag, pi);
 }
 
 __device__ __forceinline__ void nat_meet(const Line &l1, const Line &l2, double &x, double &y)
@@ -330,9 +363,8 @@ __device__ __forceinline__ void nat_meet(const Line &l1, const Line &l2, double 
 // Clip P (m entries, room for MAXM) with line M (tag mt): keep M.n . x <= M.c.  Returns
 // the new size, 0 when nothing is left, -1 when the vertices outside M do not form one
 // run (the polygon lost its convexity numerically), -2 when MAXM entries are too few.
-template <int ST, int MAXM>
-__device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly<ST> P, int m,
-                        const Line &M, int mt)
+template <int ST, int MAXM, class S>
+__device__ int nat_clip_s(const S &st, double B, double2 pi, Poly<ST> P, int m, const Line &M, int mt)
 {
     auto outside = [&](int k) {
         const double u = M.nx * PX(P, k), v = M.ny * PY(P, k);
@@ -406,13 +438,20 @@ __device__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly<
     while (km < nm && PT(P, km) != mt) ++km;
     if (km == nm) return -1;
     const int kp = km > 0 ? km - 1 : nm - 1, kn = km + 1 < nm ? km + 1 : 0;
-    const Line Lp = nat_line(a, Q, B, PT(P, kp), pi), Ln = nat_line(a, Q, B, PT(P, kn), pi);
+    const Line Lp = nat_line_s(st, B, PT(P, kp), pi), Ln = nat_line_s(st, B, PT(P, kn), pi);
     double x, y;
     nat_meet(Lp, M, x, y);
     PX(P, kp) = x; PY(P, kp) = y;
     nat_meet(M, Ln, x, y);
     PX(P, km) = x; PY(P, km) = y;
     return nm;
+}
+
+template <int ST, int MAXM>
+__device__ __forceinline__ int nat_clip(const NatArgs &a, double2 Q, double B, double2 pi, Poly<ST> P,
+                                        int m, const Line &M, int mt)
+{
+    return nat_clip_s<ST, MAXM>(SitesG{a.xy, Q}, B, pi, P, m, M, mt);
 }
 
 template <int ST>
@@ -437,7 +476,7 @@ __device__ double nat_area(Poly<ST> P, int m)
 // Sibson weights of one inside node q with polygons V (the cell of q) and R (the part
 // of it taken from one natural neighbour), MAXM lines each.  Returns 0 (values written),
 // -1 (the cell could not be formed) or -2 (MAXM lines are too few; nothing written).
-template <int ST, int MAXM>
+template <int ST, int MAXM, int RING_CAP = 1 << 30>
 __device__ int nat_sibson_node(const NatArgs &a, int q, Poly<ST> V, Poly<ST> R)
 {
     const double2 Q = a.xy[q];
@@ -477,6 +516,10 @@ __device__ int nat_sibson_node(const NatArgs &a, int q, Poly<ST> V, Poly<ST> R)
     for (int r = 0; r <= rmax && !fail; ++r) {
         // rings 0 .. r-1 done: every node left is more than (r - 1) h from q
         if (r > 0 && (double)(r - 1) * a.h >= 2.0 * rho) break;
+        if (r > RING_CAP) {  // a cell still open this far out reaches across a bad cluster
+            for (int k = 0; k < m; ++k)
+                if (PT(V, k) < 0) return -2;
+        }
         if (r == 0) {
             bucket(bx, by);
             continue;
@@ -500,9 +543,12 @@ __device__ int nat_sibson_node(const NatArgs &a, int q, Poly<ST> V, Poly<ST> R)
             for (int l = 0; l < m; ++l) {
                 PT(R, l) = PT(V, l); PX(R, l) = PX(V, l); PY(R, l) = PY(V, l);
             }
+            // the other neighbours' bisectors outward from k along the cell (k + 1, k - 1,
+            // k + 2, ...): the adjacent ones cut the region down at once, so the rest test
+            // a few vertices each (the intersection does not depend on the order)
             int mr = m;
-            for (int l = 0; l < m && mr > 0; ++l) {
-                if (l == k) continue;
+            for (int d = 1; d < m && mr > 0; ++d) {
+                const int l = (k + ((d & 1) ? (d + 1) / 2 : m - d / 2)) % m;
                 const int tj = PT(V, l);
                 const double2 Pj = a.xy[tj];
                 const double pjx = Pj.x - Q.x, pjy = Pj.y - Q.y;
@@ -545,24 +591,297 @@ __global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
     const int n_in = a.ctl[2];
     for (int li = blockIdx.x * kNatThreads + t; li < n_in; li += gridDim.x * kNatThreads) {
         const int q = a.lin[li];
-        const int r = nat_sibson_node<kNatThreads, kNatMax>(a, q, V, R);
+        const int r = nat_sibson_node<kNatThreads, kNatMax, kNatRingCap>(a, q, V, R);
         if (r == -2) a.lbig[atomicAdd(&a.ctl[5], 1)] = q;
         else if (r < 0) nat_error(a);
     }
 }
 
-// Second pass over the nodes with more than kNatMax natural neighbours (cells that reach
-// across a large cluster of bad nodes): kNatBigThreads threads, polygons of kNatBig lines
-// in device memory.
+// ---- second pass: one wave per node whose cell has many natural neighbours ------------
+// The cell V(q) as two LDS buffers of cyclic line lists; a clip is done by the wave: every
+// lane tests its vertices against the line, the single run of outside vertices [aa, aa +
+// len) is located by ballots, and the survivors are copied to the other buffer rotated
+// so that the list reads (line aa, M, line aa + len, ..., line aa - 1); the two new
+// vertices are the meets of M with its neighbours.  Same tests and vertices as nat_clip.
+struct NatWaveLds {
+    int tag[2][kNatWaveV];
+    double vx[2][kNatWaveV], vy[2][kNatWaveV];
+    int cand[kNatWaveCand];
+    double cx[kNatWaveCand], cy[kNatWaveCand];   // candidates relative to q
+    double sx[kNatWaveV], sy[kNatWaveV];         // the cell's sites relative to q
+    int rtag[kNatWaveR * 64];
+    double rvx[kNatWaveR * 64], rvy[kNatWaveR * 64];
+};
+
+__device__ __forceinline__ bool nat_outside(double x, double y, const Line &M)
+{
+    const double u = M.nx * x, v = M.ny * y;
+    const double d = u + v - M.c;
+    return d > 1e-12 * (fabs(u) + fabs(v) + fabs(M.c));
+}
+
+__device__ __forceinline__ double wave_max_d(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// Returns the new size (m when M does not cut, 0 when nothing is left), -1 (the outside
+// vertices are not one run) or -2 (more than kNatWaveV lines); cur flips on a cut.
+__device__ int nat_wclip(const NatArgs &a, double2 Q, double B, NatWaveLds &L, int &cur, int m,
+                         const Line &M, int mt, int lane)
+{
+    int len = 0, nstart = 0, aa = -1;
+    for (int base = 0; base < m; base += kWave) {
+        const int k = base + lane;
+        bool o = false, op = false;
+        if (k < m) {
+            const int kp = k > 0 ? k - 1 : m - 1;
+            o = nat_outside(L.vx[cur][k], L.vy[cur][k], M);
+            op = nat_outside(L.vx[cur][kp], L.vy[cur][kp], M);
+        }
+        const unsigned long long bo = __ballot(o), bs = __ballot(o && !op);
+        len += __popcll(bo);
+        nstart += __popcll(bs);
+        if (bs && aa < 0) aa = base + __ffsll((long long)bs) - 1;
+    }
+    if (len == 0) return m;
+    if (len == m) return 0;
+    if (nstart != 1) return -1;
+    const int nm = m - len + 2;
+    if (nm > kNatWaveV) return -2;
+    const int nx = 1 - cur;
+    for (int j = lane; j < m - len; j += kWave) {
+        const int s = (aa + len + j) % m;
+        L.tag[nx][2 + j] = L.tag[cur][s];
+        L.vx[nx][2 + j] = L.vx[cur][s];
+        L.vy[nx][2 + j] = L.vy[cur][s];
+    }
+    if (lane == 0) {
+        const double2 zero = make_double2(0.0, 0.0);
+        const int ta = L.tag[cur][aa], tn = L.tag[cur][(aa + len) % m];
+        double x, y;
+        L.tag[nx][0] = ta;
+        L.tag[nx][1] = mt;
+        nat_meet(nat_line(a, Q, B, ta, zero), M, x, y);
+        L.vx[nx][0] = x; L.vy[nx][0] = y;
+        nat_meet(M, nat_line(a, Q, B, tn, zero), x, y);
+        L.vx[nx][1] = x; L.vy[nx][1] = y;
+    }
+    wave_sync_lds();
+    cur = nx;
+    return nm;
+}
+
+// Bucket t (< 8 r, or the one bucket of ring 0) of ring r around (bx, by), in the order
+// nat_sibson_node visits them.
+__device__ __forceinline__ void ring_bucket(int r, int t, int bx, int by, int &i, int &j)
+{
+    if (r == 0) { i = bx; j = by; return; }
+    const int n1 = 2 * (2 * r + 1);
+    if (t < n1) { i = bx - r + t / 2; j = (t & 1) ? by + r : by - r; return; }
+    t -= n1;
+    j = by - r + 1 + t / 2;
+    i = (t & 1) ? bx + r : bx - r;
+}
+
+// Sibson weights of inside node q by one wave: the cell as nat_sibson_node builds it
+// (same candidates, same order, wave-parallel clips), then lane k the region of natural
+// neighbour k: the square clipped by q's bisector with p_k and by p_k's bisectors with
+// p_(k+1) and p_(k-1) first, then by the others outward along the cell, so it stays a few
+// lines.  Returns 0, -1 (the cell could not be formed) or -2 (beyond the LDS limits).
+__device__ int nat_sibson_wave(const NatArgs &a, int q, NatWaveLds &L, int lane)
+{
+    const double2 Q = a.xy[q];
+    const double B = 1e6 * (a.h * (a.BX + a.BY) + 1.0);
+    const double2 zero = make_double2(0.0, 0.0);
+    int cur = 0, m = 4;
+    if (lane < 4) {
+        double x, y;
+        nat_meet(nat_line(a, Q, B, -1 - lane, zero), nat_line(a, Q, B, -1 - ((lane + 1) & 3), zero), x, y);
+        L.tag[0][lane] = -1 - lane;
+        L.vx[0][lane] = x;
+        L.vy[0][lane] = y;
+    }
+    wave_sync_lds();
+    // the cell's radius and bounding box (relative to q): a site p cuts the cell only if
+    // some vertex v has p . v > |p|^2 / 2, and max_v p . v is at most the box's support
+    double rho, bx0, bx1, by0, by1;
+    auto shape = [&]() {
+        double r2 = 0.0, x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+        for (int k = lane; k < m; k += kWave) {
+            const double x = L.vx[cur][k], y = L.vy[cur][k];
+            r2 = fmax(r2, x * x + y * y);
+            x0 = fmin(x0, x); x1 = fmax(x1, x); y0 = fmin(y0, y); y1 = fmax(y1, y);
+        }
+        rho = sqrt(wave_max_d(r2));
+        bx0 = -wave_max_d(-x0); bx1 = wave_max_d(x1);
+        by0 = -wave_max_d(-y0); by1 = wave_max_d(y1);
+    };
+    auto reaches = [&](double px, double py) {
+        const double d2 = px * px + py * py;
+        return d2 < 4.0 * rho * rho &&
+               fmax(px * bx0, px * bx1) + fmax(py * by0, py * by1) > 0.5 * d2;
+    };
+    shape();
+    int bx = (int)floor((Q.x - a.ox) / a.h), by = (int)floor((Q.y - a.oy) / a.h);
+    bx = bx < 0 ? 0 : (bx >= a.BX ? a.BX - 1 : bx);
+    by = by < 0 ? 0 : (by >= a.BY ? a.BY - 1 : by);
+    const int rmax = max(max(bx, a.BX - 1 - bx), max(by, a.BY - 1 - by));
+    for (int r = 0; r <= rmax; ++r) {
+        if (r > 0 && (double)(r - 1) * a.h >= 2.0 * rho) break;
+        const int nbk = r == 0 ? 1 : 8 * r;
+        for (int t0 = 0; t0 < nbk; t0 += kWave) {
+            // this lane's bucket: its good nodes within reach, appended in lane order
+            const int t = t0 + lane;
+            int s0 = 0, s1 = 0;
+            if (t < nbk) {
+                int i, j;
+                ring_bucket(r, t, bx, by, i, j);
+                if (i >= 0 && i < a.BX && j >= 0 && j < a.BY) {
+                    s0 = a.bstart[j * a.BX + i];
+                    s1 = a.bstart[j * a.BX + i + 1];
+                }
+            }
+            int c = 0;
+            for (int s = s0; s < s1; ++s) {
+                const int p = a.bnode[s];
+                if (p == q || a.bad[p]) continue;
+                const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
+                if (reaches(px, py)) ++c;
+            }
+            int off = c;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {   // inclusive scan over lanes
+                const int o = __shfl_up(off, d, kWave);
+                if (lane >= d) off += o;
+            }
+            const int total = __shfl(off, kWave - 1, kWave);
+            if (total > kNatWaveCand) return -2;
+            off -= c;
+            for (int s = s0; s < s1; ++s) {
+                const int p = a.bnode[s];
+                if (p == q || a.bad[p]) continue;
+                const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
+                if (reaches(px, py)) {
+                    L.cand[off] = p;
+                    L.cx[off] = px;
+                    L.cy[off] = py;
+                    ++off;
+                }
+            }
+            wave_sync_lds();
+            for (int ci = 0; ci < total; ++ci) {
+                const int p = L.cand[ci];
+                const double px = L.cx[ci], py = L.cy[ci];
+                if (px * px + py * py >= 4.0 * rho * rho) continue;   // the cell shrank since
+                if (!reaches(px, py)) continue;
+                const Line M{px, py, 0.5 * (px * px + py * py)};
+                const int m0 = m;
+                m = nat_wclip(a, Q, B, L, cur, m, M, p, lane);
+                if (m <= 0) return m == -2 ? -2 : -1;
+                if (m != m0 || L.tag[cur][1] == p) shape();
+            }
+            wave_sync_lds();
+        }
+    }
+    // a cell still touching the initial square could not be formed
+    bool open = false;
+    for (int k = lane; k < m; k += kWave) open = open || L.tag[cur][k] < 0;
+    if (__ballot(open)) return -1;
+    // the cell's sites in LDS (local tags 0 .. m-1)
+    for (int k = lane; k < m; k += kWave) {
+        const double2 P = a.xy[L.tag[cur][k]];
+        L.sx[k] = P.x - Q.x;
+        L.sy[k] = P.y - Q.y;
+    }
+    wave_sync_lds();
+    const SitesL sl{L.sx, L.sy};
+    // regions, lane k for natural neighbours k, k + 64, ...
+    const Poly<kWave> R{&L.rtag[lane], &L.rvx[lane], &L.rvy[lane]};
+    double tot = 0.0;
+    double2 su = make_double2(0.0, 0.0);
+    double4 sf = make_double4(0.0, 0.0, 0.0, 0.0);
+    int fail = 0;
+    for (int k = lane; k < m && !fail; k += kWave) {
+        const int tk = L.tag[cur][k];
+        const double2 pi = make_double2(L.sx[k], L.sy[k]);
+        int mr = 4;
+        for (int e = 0; e < 4; ++e) {
+            double x, y;
+            nat_meet(nat_line_s(sl, B, -1 - e, pi), nat_line_s(sl, B, -1 - ((e + 1) & 3), pi), x, y);
+            R.tag[e * kWave] = -1 - e;
+            R.vx[e * kWave] = x;
+            R.vy[e * kWave] = y;
+        }
+        // q's bisector with p_k, then outward: p_k's bisectors with p_(k+d), p_(k-d) and
+        // q's with them (local tags)
+        mr = nat_clip_s<kWave, kNatWaveR>(sl, B, pi, R, mr, nat_line_s(sl, B, k, pi), k);
+        for (int d = 1; d < m && mr > 0; ++d) {
+            const int l = (k + ((d & 1) ? (d + 1) / 2 : m - d / 2)) % m;
+            mr = nat_clip_s<kWave, kNatWaveR>(sl, B, pi, R, mr, nat_line_s(sl, B, l | kRegion, pi), l | kRegion);
+            if (mr > 0) mr = nat_clip_s<kWave, kNatWaveR>(sl, B, pi, R, mr, nat_line_s(sl, B, l, pi), l);
+        }
+        if (mr < 0) { fail = mr; break; }
+        const double w = mr > 0 ? nat_area(R, mr) : 0.0;
+        tot += w;
+        const double2 uv = a.U[tk];
+        su.x += w * uv.x; su.y += w * uv.y;
+        if (a.F) {
+            const double4 fv = a.F[tk];
+            sf.x += w * fv.x; sf.y += w * fv.y; sf.z += w * fv.z; sf.w += w * fv.w;
+        }
+    }
+    const unsigned long long bf = __ballot(fail != 0), b2 = __ballot(fail == -2);
+    if (b2) return -2;
+    if (bf) return -1;
+    tot = wave_sum(tot);
+    su.x = wave_sum(su.x); su.y = wave_sum(su.y);
+    if (a.F) { sf.x = wave_sum(sf.x); sf.y = wave_sum(sf.y); sf.z = wave_sum(sf.z); sf.w = wave_sum(sf.w); }
+    if (lane == 0) {
+        const double it = 1.0 / tot;
+        a.U[q] = make_double2(su.x * it, su.y * it);
+        if (a.F) a.F[q] = make_double4(sf.x * it, sf.y * it, sf.z * it, sf.w * it);
+    }
+    return 0;
+}
+
+// One wave per node of the big list; nodes beyond the LDS limits go to lbig2.
+__global__ __launch_bounds__(64) void k_nat_sibson_wave(NatArgs a)
+{
+    __shared__ NatWaveLds L;
+    const int lane = threadIdx.x;
+    const int nb = a.ctl[5];
+    for (int li = blockIdx.x; li < nb; li += gridDim.x) {
+        const int q = a.lbig[li];
+        const int r = nat_sibson_wave(a, q, L, lane);
+        if (lane == 0) {
+            if (r == -2) a.lbig2[atomicAdd(&a.ctl[6], 1)] = q;
+            else if (r != 0) nat_error(a);
+        }
+        __syncthreads();   // the next node reuses the LDS
+    }
+}
+
+// Third pass over the nodes beyond k_nat_sibson_wave's limits: kNatBigThreads threads,
+// polygons of kNatBig lines in device memory.
 __global__ __launch_bounds__(64) void k_nat_sibson_big(NatArgs a)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;  // < kNatBigThreads
     const size_t half = (size_t)kNatBig * kNatBigThreads;
     const Poly<kNatBigThreads> V{a.btag + g, a.bvx + g, a.bvy + g};
     const Poly<kNatBigThreads> R{a.btag + half + g, a.bvx + half + g, a.bvy + half + g};
-    const int nb = a.ctl[5];
+    const int nb = a.ctl[6];
     for (int li = g; li < nb; li += kNatBigThreads)
-        if (nat_sibson_node<kNatBigThreads, kNatBig>(a, a.lbig[li], V, R) != 0) nat_error(a);
+        if (nat_sibson_node<kNatBigThreads, kNatBig>(a, a.lbig2[li], V, R) != 0) nat_error(a);
 }
 
 #undef PT
@@ -831,7 +1150,7 @@ static hipError_t natgrid_setup_impl(NatGrid &g, const double *coords, int nn, h
     if ((e = g.bstart.ensure(sizeof(int) * (nb + 1))) || (e = g.bnode.ensure(sizeof(int) * nn)) ||
         (e = g.rstart.ensure(sizeof(int) * (g.R + 1))) || (e = g.rnode.ensure(sizeof(int) * nn)) ||
         (e = g.work.ensure(sizeof(int) * (3 * (size_t)g.R + 6 * (size_t)g.R + 4 +
-                                          2 * (size_t)g.R + 2 + 8 + 3 * (size_t)nn))) ||
+                                          2 * (size_t)g.R + 2 + 8 + 4 * (size_t)nn))) ||
         (e = g.big.ensure((sizeof(int) + 2 * sizeof(double)) * 2 * (size_t)kNatBig * kNatBigThreads)))
         return e;
     if ((e = hipMemcpyAsync(g.bstart.p, cnt.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st)) ||
@@ -885,6 +1204,7 @@ hipError_t natural_fill(const NatGrid &g, const double *coords, const int *bad, 
     a.lin = a.ctl + 8;
     a.lhull = a.lin + g.nn;
     a.lbig = a.lhull + g.nn;
+    a.lbig2 = a.lbig + g.nn;
     a.err = err;
     const size_t nbig = 2 * (size_t)kNatBig * kNatBigThreads;
     a.bvx = g.big.as<double>();
@@ -895,6 +1215,7 @@ hipError_t natural_fill(const NatGrid &g, const double *coords, const int *bad, 
     hipLaunchKernelGGL(k_nat_classify, dim3((g.nn + 255) / 256), dim3(256), 0, st, a);
     const int nbs = std::min((g.nn + kNatThreads - 1) / kNatThreads, 4096);
     hipLaunchKernelGGL(k_nat_sibson, dim3(nbs), dim3(kNatThreads), 0, st, a);
+    hipLaunchKernelGGL(k_nat_sibson_wave, dim3(std::min(g.nn, 2048)), dim3(64), 0, st, a);
     hipLaunchKernelGGL(k_nat_sibson_big, dim3(kNatBigThreads / 64), dim3(64), 0, st, a);
     const int nbh = std::min(g.nn, 2048);
     hipLaunchKernelGGL(k_nat_hullfill, dim3(nbh), dim3(64), 0, st, a);

@@ -176,6 +176,42 @@ def make_pair(nx, ny, n_dots, sigma, ufun, noise, bits=8, amp=(0.4, 1.0), seed=0
                                          noise=noise, bits=bits, seed=seed))
 
 
+def make_sequence(nx, ny, n_dots, sigma, ufuns, noise, bits=8, amp=(0.4, 1.0), seed=0,
+                  hole=None, progress=None):
+    """One reference image and a deformed image per field of ufuns (an incremental
+    sequence): the dots are drawn and the reference rendered once, every deformed frame
+    is rendered through its own inverse map (dot groups on several processes), with its
+    own noise (generator seed + 1 + k).  progress(k), when given, is called after each
+    deformed frame."""
+    dots = draw_dots(n_dots, nx, ny, sigma, amp, seed)
+    fr = render_parallel(dots, nx, ny)
+    full = float((1 << bits) - 1)
+    scale = full / fr.max()
+    dt = np.uint8 if bits <= 8 else np.uint16
+    f = np.clip(np.rint(fr * scale), 0, full).astype(dt)
+    y, x = np.mgrid[1 : ny + 1, 1 : nx + 1]
+    if hole is not None:
+        hx, hy, ha = hole
+        f[(x - hx) ** 2 + (y - hy) ** 2 < ha * ha] = 0
+    gs = []
+    for k, ufun in enumerate(ufuns):
+        Xm, Ym = inverse_map(ufun, nx, ny)
+
+        def fwd(cx, cy, ufun=ufun):
+            u, v = ufun(cx, cy)
+            return cx + u, cy + v
+
+        gn = render_parallel(dots, nx, ny, Xm, Ym, fwd) * scale
+        if noise > 0:
+            gn = gn + np.random.default_rng(seed + 1 + k).normal(0.0, noise, gn.shape)
+        if hole is not None:
+            gn[(Xm - hx) ** 2 + (Ym - hy) ** 2 < ha * ha] = 0.0
+        gs.append(np.clip(np.rint(gn), 0, full).astype(dt))
+        if progress:
+            progress(k)
+    return f, gs
+
+
 # --- the five workloads ------------------------------------------------------
 
 def config1(seed=0):
@@ -352,6 +388,26 @@ def config4(seed=0, n=2048, a=300.0, eps=0.01, band_near=96.0, band_mid=600.0, *
     p.params.update(name=f"config4-{n}-hole{int(a)}-quadtree", coords=coords, elems=elems,
                     dirichlet=dirichlet, ws=ws, mask=mask, hole=(c, c, a))
     return p
+
+
+def config4_sequence(seed=0, n=2048, a=300.0, frames=20, d_eps=0.0005, band_near=400.0,
+                     band_mid=1000.0, progress=None):
+    """The incremental form of workload 4: the plate with a hole at far-field strains
+    d_eps, 2 d_eps, ..., frames d_eps (Kirsch fields), one reference image, and the denser
+    quadtree mesh (cells 8-32 px, windows 16-64 px, ~29k nodes).  Returns (f, [g_k],
+    [ufun_k], params)."""
+    c = (n + 1) / 2.0
+    ufuns = [kirsch_field(c, c, a, d_eps * (k + 1)) for k in range(frames)]
+    n_dots = int(200_000 * (n / 2048.0) ** 2)
+    f, gs = make_sequence(n, n, n_dots, 2.0, ufuns, 1.0, 8, seed=seed, hole=(c, c, a),
+                          progress=progress)
+    coords, elems, dirichlet, ws = quadtree_hole_mesh(n, n, c, c, a, band_near=band_near,
+                                                      band_mid=band_mid)
+    y, x = np.mgrid[1 : n + 1, 1 : n + 1]
+    mask = ((x - c) ** 2 + (y - c) ** 2 >= a * a).astype(np.uint8)
+    return f, gs, ufuns, dict(name=f"config4-seq{frames}-{n}-hole{int(a)}-quadtree", coords=coords,
+                              elems=elems, dirichlet=dirichlet, ws=ws, mask=mask, hole=(c, c, a),
+                              d_eps=d_eps)
 
 
 def small_pair(n=96, n_dots=None, sigma=2.0, u0=1.3, v0=-0.7, ux=0.0, vy=0.0, uy=0.0, vx=0.0,
