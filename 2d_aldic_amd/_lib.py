@@ -30,6 +30,7 @@ EXPORTS = (
     "aldic_remove_outliers", "aldic_icgn6_start", "aldic_cnn_nparams", "aldic_cnn_set_weights",
     "aldic_cnn_get_weights", "aldic_cnn_predict", "aldic_cnn_add_samples",
     "aldic_cnn_clear_samples", "aldic_cnn_train", "aldic_cnn_grad", "aldic_local_icgn",
+    "aldic_integer_search_mg",
 )
 
 
@@ -70,6 +71,8 @@ def load():
     L.aldic_get_images.argtypes = [P, vp, vp, vp, vp]
     L.aldic_search_grid.argtypes = [ip, ip, vp, ip, ip, vp, vp]
     L.aldic_integer_search.argtypes = [P, vp, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.aldic_integer_search_mg.argtypes = [P, vp, ip, ip, C.c_long, vp, vp, vp, vp, vp, vp,
+                                          C.c_long, vp]
     L.aldic_icgn6.argtypes = [P, ip, vp, vp, vp, ip, dp, vp, vp, vp, vp, vp]
     L.aldic_icgn2.argtypes = [P, ip, vp, vp, ip, vp, dp, vp, vp, vp, dp, vp, vp]
     L.aldic_set_mesh.argtypes = [P, vp, ip, vp, ip]
@@ -207,6 +210,35 @@ class Context:
         x, y, u, v, cm = [o.reshape(Ny, Nx) for o in outs]
         return dict(x=x, y=y, u=u, v=v, ccmax=cm, qf=qf, status=st.reshape(Ny, Nx), x1=x1,
                     y1=y1, Nx=Nx, Ny=Ny)
+
+    def integer_search_mg(self, grid, ws, step):
+        """funIntegerSearchMg over grid = (x1, x2, y1, y2): dict(x, y, u, v, ccmax) on the
+        fitted (Ny, Nx) grid, the final regions' tempx, tempy, tempu, tempv, tempphi, and
+        levels, n_nan (regions the reference's catch would give NaN), fit_iters."""
+        grid = _c(grid, np.int32)
+        cap = ((int(grid[1]) - int(grid[0])) // int(step) + 1) * \
+              ((int(grid[3]) - int(grid[2])) // int(step) + 1)
+        cap = max(cap, 4)
+        ncap = 4 * (2 * (int(grid[1]) - int(grid[0])) // int(ws) + 2) * \
+                   (2 * (int(grid[3]) - int(grid[2])) // int(ws) + 2)
+        dims = np.zeros(6, np.int32)
+        for _ in range(2):
+            outs = [np.empty(cap) for _ in range(5)]
+            temp = np.empty(5 * ncap)
+            rc = self.L.aldic_integer_search_mg(self.h, _p(grid), int(ws), int(step), cap,
+                                                _p(dims), *[_p(o) for o in outs], temp.size,
+                                                _p(temp))
+            if rc != 0 and 5 * int(dims[3]) > temp.size and int(dims[3]) > 0:
+                ncap = int(dims[3])
+                continue
+            self._ck(rc, "aldic_integer_search_mg")
+            break
+        Nx, Ny, nr = int(dims[0]), int(dims[1]), int(dims[3])
+        x, y, u, v, cm = [o[:Nx * Ny].reshape(Ny, Nx) for o in outs]
+        t = temp[:5 * nr].reshape(5, nr)
+        return dict(x=x, y=y, u=u, v=v, ccmax=cm, tempx=t[0], tempy=t[1], tempu=t[2],
+                    tempv=t[3], tempphi=t[4], levels=int(dims[2]), n_nan=int(dims[4]),
+                    fit_iters=int(dims[5]), Nx=Nx, Ny=Ny)
 
     # -- IC-GN -----------------------------------------------------------------------
     def icgn6(self, coords, U0, ws, tol=1e-2):

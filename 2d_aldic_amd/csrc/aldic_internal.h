@@ -116,6 +116,14 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
 // wait, so the caller reads stats_dev to learn whether maxit sufficed.
 hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, double rtol, int maxit,
                          DBuf *work, double *stats_dev, hipStream_t st, int poll);
+// gridfit.m ('springs', 'triangle', smoothness 1) of np scattered points (px, py device,
+// strictly inside the grid; pd their value pairs) onto the grid g was set up for
+// (xnodes_dev, ynodes_dev: its nodes on the device): U (gx * gy pairs, node iy * gx + ix)
+// receives the fit.  work[0..2] are used; a9 is overwritten.
+hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const double *ynodes_dev,
+                             int np, const double *px, const double *py, const double2 *pd,
+                             double *U, double rtol, int maxit, DBuf *work, double *stats_dev,
+                             hipStream_t st);
 // nanconv(., fspecial('gaussian', 3, sigma), 'edge') of nc values per node on the gx x gy
 // node grid (out-of-place)
 hipError_t launch_blur3(const double *in, double *out, int gx, int gy, int nc, double sigma,
@@ -179,6 +187,7 @@ struct FrameState {
 
 struct OutlierWork;  // outlier.hip
 struct CnnWork;      // cnn.hip
+struct MgWork;       // search_mg.hip
 
 }  // namespace aldic
 
@@ -196,6 +205,7 @@ struct aldic_ctx {
     bool images_from_raw = false;    // f, g normalised on the device from rawf, rawg
     aldic::OutlierWork *outl = nullptr;  // work buffers of the outlier test and in-painting
     aldic::CnnWork *cnn = nullptr;       // learned initial guess: weights, training set
+    aldic::MgWork *mg = nullptr;         // multigrid integer search
     aldic::DBuf stats;               // ROI partial sums, then {mean_f, 1/std_f, mean_g, 1/std_g}
     aldic::DBuf boxp;                // window box-sum planes of the exact search
     aldic::DBuf scratch[9];          // per-call work buffers (8: IC-GN start gradients)
@@ -303,6 +313,18 @@ void outlier_work_free(OutlierWork *w);
 // Learned initial guess (cnn.hip): a siamese correlation network on the 64 x 64 patch
 // pairs (plus a 4-pixel context ring) of the nodes.  All pointers are device pointers
 // unless named _host; every call enqueues on st.
+// Multigrid integer search (funIntegerSearchMg.m:3-285): final-level region centres and
+// values, the fitted grid (xlist x ylist, x fastest) and the gridfit CG iterations.
+struct MgResult {
+    int levels = 0;
+    std::vector<double> tempx, tempy, tempu, tempv, tempphi, xlist, ylist, u, v, cc;
+    int fit_iters[2] = {0, 0};
+};
+MgWork *mg_work_new();
+void mg_work_free(MgWork *w);
+// 0 ok, 1 argument/data error (err says which), 2 HIP error
+int integer_search_mg(MgWork &W, const float *f, const float *g, int nx, int ny, const int *grid, int ws,
+                      int step, MgResult &res, std::string &err, hipStream_t st);
 CnnWork *cnn_work_new();
 void cnn_work_free(CnnWork *c);
 long cnn_nparams();

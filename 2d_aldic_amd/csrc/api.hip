@@ -279,6 +279,8 @@ void aldic_destroy(aldic_ctx *ctx)
     ctx->outl = nullptr;
     cnn_work_free(ctx->cnn);
     ctx->cnn = nullptr;
+    mg_work_free(ctx->mg);
+    ctx->mg = nullptr;
     F.seed.release();
     ctx->qf.nat_err.release();
     if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
@@ -461,6 +463,54 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
     if (qf) CK(down(qf, a.qf, 2 * (size_t)S, ctx->stream));
     CK(down(status, a.status, S, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_integer_search_mg(aldic_ctx *ctx, const int *grid, int ws, int step, long cap, int *dims, double *x,
+                            double *y, double *u, double *v, double *ccmax, long temp_cap, double *temp)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (!grid || !dims || ws < 1 || step < 1 || cap < 0 || (cap > 0 && (!x || !y || !u || !v || !ccmax)))
+        return fail(ctx, ALDIC_ERR_ARG, "integer_search_mg: bad arguments");
+    if (!ctx->have_images) return fail(ctx, ALDIC_ERR_STATE, "integer_search_mg: no images");
+    CK(hipSetDevice(ctx->device));
+    if (!ctx->mg) ctx->mg = mg_work_new();
+    if (!ctx->mg) return fail(ctx, ALDIC_ERR_HIP, "integer_search_mg: out of host memory");
+    MgResult r;
+    std::string msg;
+    const int rc = integer_search_mg(*ctx->mg, ctx->f.as<float>(), ctx->g.as<float>(), ctx->nx, ctx->ny, grid, ws,
+                                     step, r, msg, ctx->stream);
+    if (rc == 1) return fail(ctx, ALDIC_ERR_ARG, msg.c_str());
+    if (rc) return fail(ctx, ALDIC_ERR_HIP, msg.c_str());
+    const int Nx = (int)r.xlist.size(), Ny = (int)r.ylist.size(), nr = (int)r.tempx.size();
+    int nnan = 0;
+    for (int q = 0; q < nr; ++q) nnan += std::isnan(r.tempu[q]) ? 1 : 0;
+    dims[0] = Nx;
+    dims[1] = Ny;
+    dims[2] = r.levels;
+    dims[3] = nr;
+    dims[4] = nnan;
+    dims[5] = r.fit_iters[0];
+    if ((long)Nx * Ny > cap) return fail(ctx, ALDIC_ERR_ARG, "integer_search_mg: output capacity too small");
+    if (temp && 5L * nr > temp_cap)
+        return fail(ctx, ALDIC_ERR_ARG, "integer_search_mg: region output capacity too small");
+    for (int j = 0; j < Ny; ++j)
+        for (int i = 0; i < Nx; ++i) {
+            const long k = (long)j * Nx + i;
+            x[k] = r.xlist[i];
+            y[k] = r.ylist[j];
+            u[k] = r.u[k];
+            v[k] = r.v[k];
+            ccmax[k] = r.cc[k];
+        }
+    if (temp)
+        for (int q = 0; q < nr; ++q) {
+            temp[q] = r.tempx[q];
+            temp[nr + q] = r.tempy[q];
+            temp[2L * nr + q] = r.tempu[q];
+            temp[3L * nr + q] = r.tempv[q];
+            temp[4L * nr + q] = r.tempphi[q];
+        }
     return ALDIC_OK;
 }
 

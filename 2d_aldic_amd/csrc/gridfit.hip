@@ -344,6 +344,103 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cgit(GfCg c, int i)
     gf_block3(g, dl, rn, c.G[nxt], c.D[nxt], c.RR[nxt]);
 }
 
+// Scattered data (gridfit.m:460-511, 'triangle'): point p lies in cell (ix, iy) of the node
+// grid (histc: the last node counts in the last cell) at (tx, ty) in [0, 1]^2; it
+// interpolates from node (ix, iy) with weight 1 - max(tx, ty), (ix + 1, iy + 1) with
+// min(tx, ty), and (ix + 1, iy) (tx > ty) or (ix, iy + 1) with |tx - ty|.  Adds the point's
+// rows of A to A^T A (stencil slots), A^T d and the column sums of |A| (atomics: the
+// order of the FP64 sums varies between runs in the last bits).
+__device__ __forceinline__ int gf_cell(const double *nodes, int n, double v)
+{
+    int lo = 0, hi = n;  // count of nodes <= v
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (nodes[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return min(max(lo, 1), n - 1) - 1;
+}
+
+__global__ __launch_bounds__(256) void k_gf_scatter(int np, const double *px, const double *py,
+                                                    const double2 *pd, const double *xn, int gx,
+                                                    const double *yn, int gy, double *a9,
+                                                    double2 *b, double *colsum)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int ix = gf_cell(xn, gx, px[p]), iy = gf_cell(yn, gy, py[p]);
+    const double tx = fmin(fmax((px[p] - xn[ix]) / (xn[ix + 1] - xn[ix]), 0.0), 1.0);
+    const double ty = fmin(fmax((py[p] - yn[iy]) / (yn[iy + 1] - yn[iy]), 0.0), 1.0);
+    const double t1 = fmin(tx, ty), t2 = fmax(tx, ty);
+    const int n = gx * gy;
+    const int nd[3] = {iy * gx + ix, (iy + 1) * gx + ix + 1, tx > ty ? iy * gx + ix + 1 : (iy + 1) * gx + ix};
+    const int ox[3] = {0, 1, tx > ty ? 1 : 0}, oy[3] = {0, 1, tx > ty ? 0 : 1};
+    const double w[3] = {1.0 - t2, t1, t2 - t1};
+    const double2 d = pd[p];
+    for (int a = 0; a < 3; ++a) {
+        atomicAdd(&b[nd[a]].x, w[a] * d.x);
+        atomicAdd(&b[nd[a]].y, w[a] * d.y);
+        atomicAdd(&colsum[nd[a]], w[a]);
+        for (int c = 0; c < 3; ++c) {
+            const int k = (oy[c] - oy[a] + 1) * 3 + (ox[c] - ox[a] + 1);
+            atomicAdd(&a9[(size_t)k * n + nd[a]], w[a] * w[c]);
+        }
+    }
+}
+
+// lam^2 = (|A|_1 / |Areg|_1)^2 (smoothness 1, gridfit.m:651-654): out = reg2 * max colsum^2
+__global__ __launch_bounds__(256) void k_gf_lam2(const double *colsum, int n, double reg2, double *out)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, colsum[i]);
+    for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d, kWave));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        out[0] = reg2 * m * m;
+    }
+}
+
+// K0 of the scattered fit: N = A^T A (already in a9) + lam^2 L, r_0 = A^T d (x_0 = 0)
+__global__ __launch_bounds__(kGfThreads) void k_gf_cg0s(GfCg c, const double *w, const double *lam2p,
+                                                        const double2 *bvec)
+{
+    const int gx = c.gx, gy = c.gy;
+    const double lam2 = *lam2p;
+    const double *wx = w, *wy = wx + (gx - 1), *axs = wy + (gy - 1), *ays = axs + (gx - 1);
+    double bb = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
+        const int ix = i % gx, iy = i / gx;
+        double dsum = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            if (k == 4 || !gf_in(ix, iy, k, gx, gy)) continue;
+            const int dx = k % 3 - 1, dy = k / 3 - 1;
+            double wt;
+            if (dy == 0) wt = wx[min(ix, ix + dx)];
+            else if (dx == 0) wt = wy[min(iy, iy + dy)];
+            else wt = 1.0 / (axs[min(ix, ix + dx)] + ays[min(iy, iy + dy)]);
+            c.a9[k * c.n + i] -= lam2 * wt;
+            dsum += wt;
+        }
+        const double d = c.a9[4 * c.n + i] + lam2 * dsum;
+        c.a9[4 * c.n + i] = d;
+        c.dinv[i] = 1.0 / d;
+        const double2 b = bvec[i];
+        c.x[i] = make_double2(0.0, 0.0);
+        c.r[0][i] = b;
+        c.p[i] = make_double2(0.0, 0.0);
+        c.s[0][i] = make_double2(0.0, 0.0);
+        bb += b.x * b.x + b.y * b.y;
+    }
+    gf_block3(bb, 0.0, 0.0, c.BB, nullptr, nullptr);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.stat[0] = 0.0; c.stat[1] = 0.0; c.stat[2] = 1.0;
+        if (c.rec) { c.rec[0] = 0.0; c.rec[1] = 0.0; c.rec[2] = 1.0; }
+    }
+}
+
 }  // namespace
 
 hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const double *ynodes,
@@ -401,9 +498,16 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
     return hipStreamSynchronize(st);
 }
 
+// scat (nullable): the scattered fit's {lam^2 (device), A^T d (device)}, NP 1 only
+struct GfScat {
+    const double *lam2;
+    const double2 *b;
+};
+
 template <int NP>
 static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol, int maxit,
-                           DBuf *work, double *stats_dev, hipStream_t st, int poll)
+                           DBuf *work, double *stats_dev, hipStream_t st, int poll,
+                           const GfScat *scat = nullptr)
 {
     const int n = g.gx * g.gy;
     // one node per thread when the grid allows it (the operand prefetch needs it); at
@@ -434,8 +538,12 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
     c.rec = stats_dev;
     c.nb = nb;
     c.rtol2 = rtol * rtol;
-    hipLaunchKernelGGL(k_gf_cg0<NP>, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
-                       g.lam2, bad);
+    if (scat)
+        hipLaunchKernelGGL(k_gf_cg0s, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
+                           scat->lam2, scat->b);
+    else
+        hipLaunchKernelGGL(k_gf_cg0<NP>, dim3(nb), dim3(kGfThreads), 0, st, c,
+                           (const double *)g.w.p, g.lam2, bad);
     hipLaunchKernelGGL(k_gf_cg1<NP>, dim3(nb), dim3(kGfThreads), 0, st, c);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // poll <= 0: maxit iteration kernels and no host wait (the caller reads stat)
@@ -472,6 +580,29 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, doubl
     if (np == 1) return solve_np<1>(g, bad, U, rtol, maxit, work, stats_dev, st, poll);
     if (np == 2) return solve_np<2>(g, bad, U, rtol, maxit, work, stats_dev, st, poll);
     return hipErrorInvalidValue;
+}
+
+hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const double *ynodes_dev,
+                             int np, const double *px, const double *py, const double2 *pd,
+                             double *U, double rtol, int maxit, DBuf *work, double *stats_dev,
+                             hipStream_t st)
+{
+    const int n = g.gx * g.gy;
+    hipError_t e;
+    if (np <= 0) return hipErrorInvalidValue;
+    if ((e = work[2].ensure(sizeof(double2) * (size_t)n + sizeof(double) * ((size_t)n + 1))) != hipSuccess)
+        return e;
+    double2 *b = work[2].as<double2>();
+    double *colsum = reinterpret_cast<double *>(b + n), *lam2 = colsum + n;
+    if ((e = hipMemsetAsync(g.a9.p, 0, 9 * sizeof(double) * (size_t)n, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(work[2].p, 0, sizeof(double2) * (size_t)n + sizeof(double) * n, st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_gf_scatter, dim3((np + 255) / 256), dim3(256), 0, st, np, px, py, pd, xnodes_dev, g.gx,
+                       ynodes_dev, g.gy, g.a9.as<double>(), b, colsum);
+    hipLaunchKernelGGL(k_gf_lam2, dim3(1), dim3(256), 0, st, (const double *)colsum, n, g.lam2, lam2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const GfScat sc{lam2, b};
+    return solve_np<1>(g, nullptr, U, rtol, maxit, work, stats_dev, st, 32, &sc);
 }
 
 namespace {

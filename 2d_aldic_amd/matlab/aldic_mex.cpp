@@ -211,6 +211,27 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         }
         plhs[6] = imat(S, 1);
         std::memcpy(iptr(plhs[6]), st.data(), sizeof(int) * S);
+    } else if (cmd == "integer_search_mg") {
+        // [x, y, u, v, ccmax, dims] = aldic_mex('integer_search_mg', grid, ws, step)
+        need(nrhs, 4, cmd.c_str());
+        const std::vector<int> grid = ints(prhs[1], 0);
+        if (grid.size() != 4) mexErrMsgIdAndTxt("aldic:args", "integer_search_mg: grid = [x1 x2 y1 y2]");
+        const int ws = (int)mxGetScalar(prhs[2]), step = (int)mxGetScalar(prhs[3]);
+        if (ws < 1 || step < 1) mexErrMsgIdAndTxt("aldic:args", "integer_search_mg: bad ws / step");
+        const long cap = std::max(4L, (long)((grid[1] - grid[0]) / step + 1) * ((grid[3] - grid[2]) / step + 1));
+        std::vector<double> x(cap), y(cap), u(cap), v(cap), cm(cap);
+        int dims[6];
+        check(aldic_integer_search_mg(g_ctx, grid.data(), ws, step, cap, dims, x.data(), y.data(), u.data(),
+                                      v.data(), cm.data(), 0, nullptr),
+              "integer_search_mg");
+        const int Nx = dims[0], Ny = dims[1];
+        plhs[0] = grid_out(x, Nx, Ny);
+        plhs[1] = grid_out(y, Nx, Ny);
+        plhs[2] = grid_out(u, Nx, Ny);
+        plhs[3] = grid_out(v, Nx, Ny);
+        plhs[4] = grid_out(cm, Nx, Ny);
+        plhs[5] = imat(6, 1);
+        std::memcpy(iptr(plhs[5]), dims, sizeof(dims));
     } else if (cmd == "icgn6") {
         // [U, F, H21, iters, zncc] = aldic_mex('icgn6', U0, coords, ws, tol): ws scalar or S-vector
         need(nrhs, 5, cmd.c_str());

@@ -105,6 +105,20 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
                          double *x, double *y, double *u, double *v, double *ccmax, double *qf,
                          int *status);
 
+/* funIntegerSearchMg.m:3-285 (IntegerSearch.m:120, the multigrid search) on the staged
+ * images, grid = {x1, x2, y1, y2} (gridxROIRange, gridyROIRange), winsize ws, winstepsize
+ * step.  Level 1 correlates the whole ROI (normxcorr2 in full, FFT); each later level
+ * splits every region in two or four and correlates it with its g window (+-5 px) on the
+ * 13 x 14 lags findpeak reads; the last level's region centres are fitted onto the grid
+ * by gridfit ('springs').  dims (int[6]) = {Nx, Ny, levels, final regions, final regions
+ * with NaN (the reference's catch), CG iterations of the u/v fit}.  x, y, u, v, ccmax
+ * receive Nx*Ny values (x fastest; cap their capacity: ALDIC_ERR_ARG with dims set if
+ * smaller, so ((x2-x1)/step+1)*((y2-y1)/step+1) always suffices).  temp (nullable,
+ * temp_cap values) receives the final regions' tempx | tempy | tempu | tempv | tempPhi.
+ * cc.qfactors is not computed (only the q-factor outlier test reads it). */
+int aldic_integer_search_mg(aldic_ctx *ctx, const int *grid, int ws, int step, long cap, int *dims, double *x,
+                            double *y, double *u, double *v, double *ccmax, long temp_cap, double *temp);
+
 /* LocalICGN.m:91-107 -> funICGN.m:1-322: 6-DOF IC-GN of S subsets centred at coords
  * (S x 2) from U0 (2S), subset size ws[j] (NULL -> ws_all), stop tolerance tol.
  * Outputs U (2S), F (4S), H21 (S x 21), iters (S); zncc (S, nullable) is the
