@@ -387,19 +387,32 @@ __global__ __launch_bounds__(256) void k_gf_scatter(int np, const double *px, co
     }
 }
 
-// lam^2 = (|A|_1 / |Areg|_1)^2 (smoothness 1, gridfit.m:651-654): out = reg2 * max colsum^2
-__global__ __launch_bounds__(256) void k_gf_lam2(const double *colsum, int n, double reg2, double *out)
+// lam^2 = (|A|_1 / |Areg|_1)^2 (smoothness 1, gridfit.m:651-654): the largest column sum
+// per block, then out = reg2 * max^2 from one block
+__device__ __forceinline__ double gf_block_max(double m)
 {
     __shared__ double red[4];
-    double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, colsum[i]);
     for (int d = 32; d >= 1; d >>= 1) m = fmax(m, __shfl_xor(m, d, kWave));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-        out[0] = reg2 * m * m;
-    }
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+constexpr int kGfMaxParts = 256;
+
+__global__ __launch_bounds__(256) void k_gf_colmax(const double *colsum, int n, double *part)
+{
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        m = fmax(m, colsum[i]);
+    m = gf_block_max(m);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void k_gf_lam2(const double *part, int nb, double reg2, double *out)
+{
+    const double m = gf_block_max(threadIdx.x < nb ? part[threadIdx.x] : 0.0);
+    if (threadIdx.x == 0) out[0] = reg2 * m * m;
 }
 
 // K0 of the scattered fit: N = A^T A (already in a9) + lam^2 L, r_0 = A^T d (x_0 = 0)
@@ -590,16 +603,19 @@ hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const doub
     const int n = g.gx * g.gy;
     hipError_t e;
     if (np <= 0) return hipErrorInvalidValue;
-    if ((e = work[2].ensure(sizeof(double2) * (size_t)n + sizeof(double) * ((size_t)n + 1))) != hipSuccess)
+    if ((e = work[2].ensure(sizeof(double2) * (size_t)n + sizeof(double) * ((size_t)n + 1 + kGfMaxParts))) !=
+        hipSuccess)
         return e;
     double2 *b = work[2].as<double2>();
-    double *colsum = reinterpret_cast<double *>(b + n), *lam2 = colsum + n;
+    double *colsum = reinterpret_cast<double *>(b + n), *lam2 = colsum + n, *parts = lam2 + 1;
+    const int nbm = std::min(kGfMaxParts, (n + 255) / 256);
     if ((e = hipMemsetAsync(g.a9.p, 0, 9 * sizeof(double) * (size_t)n, st)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(work[2].p, 0, sizeof(double2) * (size_t)n + sizeof(double) * n, st)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_gf_scatter, dim3((np + 255) / 256), dim3(256), 0, st, np, px, py, pd, xnodes_dev, g.gx,
                        ynodes_dev, g.gy, g.a9.as<double>(), b, colsum);
-    hipLaunchKernelGGL(k_gf_lam2, dim3(1), dim3(256), 0, st, (const double *)colsum, n, g.lam2, lam2);
+    hipLaunchKernelGGL(k_gf_colmax, dim3(nbm), dim3(256), 0, st, (const double *)colsum, n, parts);
+    hipLaunchKernelGGL(k_gf_lam2, dim3(1), dim3(256), 0, st, (const double *)parts, nbm, g.lam2, lam2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const GfScat sc{lam2, b};
     return solve_np<1>(g, nullptr, U, rtol, maxit, work, stats_dev, st, 32, &sc);

@@ -69,11 +69,12 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(double2 *data, const double
     for (int i = threadIdx.x; i < N; i += kFftThreads) p[(long)i * estride] = s[i];
 }
 
-// The level-1 operands: T' = f - mean (ROI of m x n pixels from (x1, y1), 1-based) and D = g
-// on the same ROI, zero-padded into P x Q complex arrays ([y][x], x fastest).
+// The level-1 operands packed in one complex array, Z = T' + i D: T' = f - mean (ROI of m x
+// n pixels from (x1, y1), 1-based) and D = g on the same ROI, zero-padded to P x Q ([y][x],
+// x fastest).  Both are real, so one transform gives both spectra.
 __global__ void k_mg_operands(const float *__restrict__ f, const float *__restrict__ g, int nx, int x1,
                               int y1, int m, int n, const double *__restrict__ tmean, int P, int Q,
-                              double2 *__restrict__ ZT, double2 *__restrict__ ZD)
+                              double2 *__restrict__ Z)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)P * Q) return;
@@ -84,17 +85,26 @@ __global__ void k_mg_operands(const float *__restrict__ f, const float *__restri
         t = (double)f[o] - *tmean;
         d = (double)g[o];
     }
-    ZT[i] = make_double2(t, 0.0);
-    ZD[i] = make_double2(d, 0.0);
+    Z[i] = make_double2(t, d);
 }
 
-// ZT = conj(ZT) * ZD
-__global__ void k_mg_cmul(double2 *__restrict__ ZT, const double2 *__restrict__ ZD, long n)
+// From Z = FFT(T' + i D): FT(k) = (Z(k) + conj Z(-k)) / 2, FD(k) = (Z(k) - conj Z(-k)) / 2i;
+// Z(k) <- conj(FT(k)) FD(k), and Z(-k) <- its conjugate (the correlation is real).  The
+// thread of the smaller index of each pair does both.
+__global__ void k_mg_cross(double2 *__restrict__ Z, int P, int Q)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double2 a = ZT[i], b = ZD[i];
-    ZT[i] = make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+    if (i >= (long)P * Q) return;
+    const int kx = (int)(i % P), ky = (int)(i / P);
+    const long j = (long)((Q - ky) & (Q - 1)) * P + ((P - kx) & (P - 1));
+    if (j < i) return;
+    const double2 a = Z[i], b = Z[j];
+    // FT = ((a.x + b.x) + i (a.y - b.y)) / 2, FD = ((a.y + b.y) - i (a.x - b.x)) / 2
+    const double tr = 0.5 * (a.x + b.x), ti = 0.5 * (a.y - b.y);
+    const double dr = 0.5 * (a.y + b.y), di = -0.5 * (a.x - b.x);
+    const double rr = tr * dr + ti * di, ri = tr * di - ti * dr;
+    Z[i] = make_double2(rr, ri);
+    if (j != i) Z[j] = make_double2(rr, -ri);
 }
 
 // sum, sum of squares and largest |v - first pixel| of the ROI of f (blocks of partials,
@@ -135,33 +145,74 @@ __global__ void k_mg_tsum_final(const double *__restrict__ part, int nb, long cn
 }
 
 // Prefix sums S[y][x] = sum over x' < x, y' < y of D (and D^2) on the (m + 1) x (n + 1)
-// grid: one thread per row, then one per column.
-__global__ void k_mg_prefix_rows(const float *__restrict__ g, int nx, int x1, int y1, int m, int n,
-                                 double *__restrict__ S1, double *__restrict__ S2)
+// grid.  Rows: one wave per row, 64 pixels a step (coalesced), wave inclusive scan plus
+// the carry; row 0 is zero.
+__global__ __launch_bounds__(256) void k_mg_prefix_rows(const float *__restrict__ g, int nx, int x1, int y1,
+                                                        int m, int n, double *__restrict__ S1,
+                                                        double *__restrict__ S2)
 {
-    const int yy = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (yy > n) return;
     const long row = (long)yy * (m + 1);
-    S1[row] = 0.0;
-    S2[row] = 0.0;
-    double a = 0.0, b = 0.0;
-    for (int xx = 0; xx < m; ++xx) {
-        if (yy > 0) {
-            const double v = g[(long)(y1 - 1 + yy - 1) * nx + (x1 - 1 + xx)];
-            a += v;
-            b += v * v;
+    if (lane == 0) { S1[row] = 0.0; S2[row] = 0.0; }
+    double ca = 0.0, cb = 0.0;
+    for (int x0 = 0; x0 < m; x0 += 64) {
+        const int xx = x0 + lane;
+        double a = 0.0;
+        if (yy > 0 && xx < m) a = g[(long)(y1 - 1 + yy - 1) * nx + (x1 - 1 + xx)];
+        double b = a * a;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double ua = __shfl_up(a, d, kWave), ub = __shfl_up(b, d, kWave);
+            if (lane >= d) { a += ua; b += ub; }
         }
-        S1[row + xx + 1] = a;
-        S2[row + xx + 1] = b;
+        a += ca;
+        b += cb;
+        if (xx < m) { S1[row + xx + 1] = a; S2[row + xx + 1] = b; }
+        ca = __shfl(a, 63, kWave);
+        cb = __shfl(b, 63, kWave);
     }
 }
 
-__global__ void k_mg_prefix_cols(int m, int n, double *__restrict__ S1, double *__restrict__ S2)
+// Columns in chunks of kMgCh rows: chunk totals per column, their exclusive scan down the
+// chunks, then each chunk's inclusive scan plus its carry.
+constexpr int kMgCh = 64;
+
+__global__ void k_mg_col_chunks(int m, int n, const double *__restrict__ S1, const double *__restrict__ S2,
+                                double *__restrict__ c1, double *__restrict__ c2)
+{
+    const int xx = blockIdx.x * blockDim.x + threadIdx.x, ch = blockIdx.y;
+    if (xx > m) return;
+    double a = 0.0, b = 0.0;
+    for (int yy = ch * kMgCh; yy < min((ch + 1) * kMgCh, n + 1); ++yy) {
+        a += S1[(long)yy * (m + 1) + xx];
+        b += S2[(long)yy * (m + 1) + xx];
+    }
+    c1[(long)ch * (m + 1) + xx] = a;
+    c2[(long)ch * (m + 1) + xx] = b;
+}
+
+__global__ void k_mg_col_carry(int m, int nch, double *__restrict__ c1, double *__restrict__ c2)
 {
     const int xx = blockIdx.x * blockDim.x + threadIdx.x;
     if (xx > m) return;
     double a = 0.0, b = 0.0;
-    for (int yy = 0; yy <= n; ++yy) {
+    for (int ch = 0; ch < nch; ++ch) {
+        const long k = (long)ch * (m + 1) + xx;
+        const double ta = c1[k], tb = c2[k];
+        c1[k] = a;
+        c2[k] = b;
+        a += ta;
+        b += tb;
+    }
+}
+
+__global__ void k_mg_col_scan(int m, int n, double *__restrict__ S1, double *__restrict__ S2,
+                              const double *__restrict__ c1, const double *__restrict__ c2)
+{
+    const int xx = blockIdx.x * blockDim.x + threadIdx.x, ch = blockIdx.y;
+    if (xx > m) return;
+    double a = c1[(long)ch * (m + 1) + xx], b = c2[(long)ch * (m + 1) + xx];
+    for (int yy = ch * kMgCh; yy < min((ch + 1) * kMgCh, n + 1); ++yy) {
         const long i = (long)yy * (m + 1) + xx;
         a += S1[i];
         b += S2[i];
@@ -302,6 +353,64 @@ __global__ __launch_bounds__(256) void k_mg_tiles(const float *__restrict__ f, c
     }
 }
 
+// Coarse levels (many tiles per region): one wave per (region, lag) sums the region's tile
+// partials (lanes stride over the tiles, then the shuffle tree: a fixed order), lag kMgLags
+// being the template statistics; the result has the layout of one tile per region.
+__global__ __launch_bounds__(256) void k_mg_reduce(const int *__restrict__ toff, const double *__restrict__ part,
+                                                   const double *__restrict__ tstat, double *__restrict__ rpart,
+                                                   double *__restrict__ rstat)
+{
+    const int q = blockIdx.x, l = blockIdx.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (l > kMgLags) return;
+    const int t0 = toff[q], t1 = toff[q + 1];
+    if (l < kMgLags) {
+        double s = 0.0, sd = 0.0, sd2 = 0.0;
+        for (int t = t0 + lane; t < t1; t += 64) {
+            const double *o = part + ((long)t * kMgLags + l) * 3;
+            s += o[0];
+            sd += o[1];
+            sd2 += o[2];
+        }
+        s = wave_sum(s);
+        sd = wave_sum(sd);
+        sd2 = wave_sum(sd2);
+        if (lane == 0) {
+            double *o = rpart + ((long)q * kMgLags + l) * 3;
+            o[0] = s;
+            o[1] = sd;
+            o[2] = sd2;
+        }
+    } else {
+        double s = 0.0, s2 = 0.0, lo = INFINITY, hi = -INFINITY;
+        for (int t = t0 + lane; t < t1; t += 64) {
+            const double *o = tstat + (long)t * 4;
+            s += o[0];
+            s2 += o[1];
+            lo = fmin(lo, o[2]);
+            hi = fmax(hi, o[3]);
+        }
+        s = wave_sum(s);
+        s2 = wave_sum(s2);
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo = fmin(lo, __shfl_xor(lo, d, kWave));
+            hi = fmax(hi, __shfl_xor(hi, d, kWave));
+        }
+        if (lane == 0) {
+            double *o = rstat + (long)q * 4;
+            o[0] = s;
+            o[1] = s2;
+            o[2] = lo;
+            o[3] = hi;
+        }
+    }
+}
+
+__global__ void k_iota_i(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) p[i] = i;
+}
+
 // One workgroup per region: its tiles' partials summed in order, the 13 x 14 NCC map,
 // its first maximum and findpeak's fit.  out[q] = {du, dv, Phi} (displacement within the
 // window: peak - 7), NaN, NaN, 0 for a constant template.
@@ -370,6 +479,7 @@ struct MgWork {
     DBuf zt, zd, tw, s1, s2, map, part, red, peak, reg, tiles, toff, tpart, tstat, out;
     DBuf nodes, pts, fit, gstat, gw[3];
     GridfitDev gf;
+    std::vector<double> gx_last, gy_last;   // the node lists gf was set up for
     ~MgWork()
     {
         for (DBuf *b : {&zt, &zd, &tw, &s1, &s2, &map, &part, &red, &peak, &reg, &tiles, &toff, &tpart, &tstat,
@@ -423,7 +533,8 @@ hipError_t mg_level1(MgWork &W, const float *f, const float *g, int nx, int x1, 
     const int P = 1 << lp, Q = 1 << lq, L = P > Q ? P : Q;
     const long PQ = (long)P * Q, R = 2L * m - 1, C = 2L * n - 1;
     const int nbt = 512;
-    if ((e = W.zt.ensure(sizeof(double2) * PQ)) || (e = W.zd.ensure(sizeof(double2) * PQ)) ||
+    if ((e = W.zt.ensure(sizeof(double2) * PQ)) ||
+        (e = W.zd.ensure(2 * sizeof(double) * (size_t)(m + 1) * ((n + 1 + kMgCh - 1) / kMgCh))) ||
         (e = W.tw.ensure(sizeof(double2) * (L / 2 + 1))) ||
         (e = W.s1.ensure(sizeof(double) * (m + 1) * (size_t)(n + 1))) ||
         (e = W.s2.ensure(sizeof(double) * (m + 1) * (size_t)(n + 1))) || (e = W.map.ensure(sizeof(double) * R * C)) ||
@@ -436,16 +547,19 @@ hipError_t mg_level1(MgWork &W, const float *f, const float *g, int nx, int x1, 
     hipLaunchKernelGGL(k_mg_tsum_final, dim3(1), dim3(64), 0, st, (const double *)W.part.as<double>(), nbt,
                        (long)m * n, tst);
     hipLaunchKernelGGL(k_mg_operands, dim3(nblk(PQ)), dim3(256), 0, st, f, g, nx, x1, y1, m, n,
-                       (const double *)tst, P, Q, W.zt.as<double2>(), W.zd.as<double2>());
+                       (const double *)tst, P, Q, W.zt.as<double2>());
     if ((e = fft2(W.zt.as<double2>(), W.tw.as<double2>(), L, P, lp, Q, lq, false, st))) return e;
-    if ((e = fft2(W.zd.as<double2>(), W.tw.as<double2>(), L, P, lp, Q, lq, false, st))) return e;
-    hipLaunchKernelGGL(k_mg_cmul, dim3(nblk(PQ)), dim3(256), 0, st, W.zt.as<double2>(),
-                       (const double2 *)W.zd.as<double2>(), PQ);
+    hipLaunchKernelGGL(k_mg_cross, dim3(nblk(PQ)), dim3(256), 0, st, W.zt.as<double2>(), P, Q);
     if ((e = fft2(W.zt.as<double2>(), W.tw.as<double2>(), L, P, lp, Q, lq, true, st))) return e;
-    hipLaunchKernelGGL(k_mg_prefix_rows, dim3(nblk(n + 1)), dim3(256), 0, st, g, nx, x1, y1, m, n,
+    const int nch = (n + 1 + kMgCh - 1) / kMgCh;
+    double *c1 = W.zd.as<double>(), *c2 = c1 + (size_t)nch * (m + 1);
+    hipLaunchKernelGGL(k_mg_prefix_rows, dim3((n + 1 + 3) / 4), dim3(256), 0, st, g, nx, x1, y1, m, n,
                        W.s1.as<double>(), W.s2.as<double>());
-    hipLaunchKernelGGL(k_mg_prefix_cols, dim3(nblk(m + 1)), dim3(256), 0, st, m, n, W.s1.as<double>(),
-                       W.s2.as<double>());
+    hipLaunchKernelGGL(k_mg_col_chunks, dim3(nblk(m + 1), nch), dim3(256), 0, st, m, n,
+                       (const double *)W.s1.as<double>(), (const double *)W.s2.as<double>(), c1, c2);
+    hipLaunchKernelGGL(k_mg_col_carry, dim3(nblk(m + 1)), dim3(256), 0, st, m, nch, c1, c2);
+    hipLaunchKernelGGL(k_mg_col_scan, dim3(nblk(m + 1), nch), dim3(256), 0, st, m, n, W.s1.as<double>(),
+                       W.s2.as<double>(), (const double *)c1, (const double *)c2);
     hipLaunchKernelGGL(k_mg_ncc_full, dim3(nblk(R * C)), dim3(256), 0, st, (const double2 *)W.zt.as<double2>(), P,
                        Q, m, n, (const double *)W.s1.as<double>(), (const double *)W.s2.as<double>(),
                        (const double *)tst, W.map.as<double>());
@@ -479,10 +593,17 @@ hipError_t mg_regions(MgWork &W, const float *f, const float *g, int nx, const s
         toff[q + 1] = (int)(tiles.size() / 3);
     }
     const int nt = toff[nr];
+    int most = 0;
+    for (int q = 0; q < nr; ++q) most = std::max(most, toff[q + 1] - toff[q]);
+    // regions of many tiles: their partials reduced in parallel first (the finish kernel's
+    // loop over the tiles would be long and serial)
+    const bool coarse = most > 32;
+    const size_t nrp = coarse ? (size_t)nr : 0;
     if ((e = W.reg.ensure(sizeof(int) * 6 * (size_t)nr)) || (e = W.tiles.ensure(sizeof(int) * tiles.size())) ||
-        (e = W.toff.ensure(sizeof(int) * (nr + 1))) ||
-        (e = W.tpart.ensure(sizeof(double) * 3 * kMgLags * (size_t)nt)) ||
-        (e = W.tstat.ensure(sizeof(double) * 4 * (size_t)nt)) || (e = W.out.ensure(sizeof(double) * 3 * (size_t)nr)))
+        (e = W.toff.ensure(sizeof(int) * (2 * (size_t)nr + 2))) ||
+        (e = W.tpart.ensure(sizeof(double) * 3 * kMgLags * ((size_t)nt + nrp))) ||
+        (e = W.tstat.ensure(sizeof(double) * 4 * ((size_t)nt + nrp))) ||
+        (e = W.out.ensure(sizeof(double) * 3 * (size_t)nr)))
         return e;
     if ((e = hipMemcpyAsync(W.reg.p, reg.data(), sizeof(int) * 6 * (size_t)nr, hipMemcpyHostToDevice, st)) ||
         (e = hipMemcpyAsync(W.tiles.p, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, st)) ||
@@ -490,9 +611,20 @@ hipError_t mg_regions(MgWork &W, const float *f, const float *g, int nx, const s
         return e;
     hipLaunchKernelGGL(k_mg_tiles, dim3(nt), dim3(256), 0, st, f, g, nx, (const int *)W.reg.as<int>(),
                        (const int *)W.tiles.as<int>(), W.tpart.as<double>(), W.tstat.as<double>());
-    hipLaunchKernelGGL(k_mg_finish, dim3(nr), dim3(256), 0, st, (const int *)W.reg.as<int>(),
-                       (const int *)W.toff.as<int>(), (const double *)W.tpart.as<double>(),
-                       (const double *)W.tstat.as<double>(), W.out.as<double>());
+    const int *toff_d = W.toff.as<int>();
+    const double *fpart = W.tpart.as<double>(), *fstat = W.tstat.as<double>();
+    if (coarse) {
+        int *ident = W.toff.as<int>() + nr + 1;
+        double *rpart = W.tpart.as<double>() + 3 * (size_t)kMgLags * nt, *rstat = W.tstat.as<double>() + 4 * (size_t)nt;
+        hipLaunchKernelGGL(k_mg_reduce, dim3(nr, (kMgLags + 1 + 3) / 4), dim3(256), 0, st, toff_d, fpart, fstat,
+                           rpart, rstat);
+        hipLaunchKernelGGL(k_iota_i, dim3(nblk(nr + 1)), dim3(256), 0, st, ident, nr);
+        toff_d = ident;
+        fpart = rpart;
+        fstat = rstat;
+    }
+    hipLaunchKernelGGL(k_mg_finish, dim3(nr), dim3(256), 0, st, (const int *)W.reg.as<int>(), toff_d, fpart, fstat,
+                       W.out.as<double>());
     if ((e = hipGetLastError())) return e;
     if ((e = hipMemcpyAsync(out_host, W.out.p, sizeof(double) * 3 * (size_t)nr, hipMemcpyDeviceToHost, st))) return e;
     return hipStreamSynchronize(st);   // the host vectors go out of scope
@@ -718,8 +850,17 @@ int integer_search_mg(MgWork &W, const float *f, const float *g, int nx, int ny,
         return 1;
     }
     const int ng = Nx * Ny;
-    if ((e = gridfit_setup(W.gf, res.xlist.data(), Nx, res.ylist.data(), Ny, st)) ||
-        (e = W.nodes.ensure(sizeof(double) * (Nx + Ny))) || (e = W.fit.ensure(sizeof(double2) * 2 * (size_t)ng)) ||
+    // the grid's springs depend on the node lists only: set up again when they change
+    if (W.gx_last != res.xlist || W.gy_last != res.ylist) {
+        W.gx_last.clear();
+        if ((e = gridfit_setup(W.gf, res.xlist.data(), Nx, res.ylist.data(), Ny, st))) {
+            err = hipGetErrorString(e);
+            return 2;
+        }
+        W.gx_last = res.xlist;
+        W.gy_last = res.ylist;
+    }
+    if ((e = W.nodes.ensure(sizeof(double) * (Nx + Ny))) || (e = W.fit.ensure(sizeof(double2) * 2 * (size_t)ng)) ||
         (e = W.gstat.ensure(sizeof(double) * 6)) ||
         (e = W.pts.ensure(sizeof(double) * 4 * (pu.size() / 4 + pc.size() / 4))))
         { err = hipGetErrorString(e); return 2; }

@@ -120,6 +120,8 @@ def main():
                          "previous one's U) on the ~29k-node mesh")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip LocalICGN from the learned start against the search start")
+    ap.add_argument("--no-mg", action="store_true",
+                    help="skip the multigrid integer search line (funIntegerSearchMg)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -201,6 +203,10 @@ def main():
     if args.workload == 3 and not args.no_learned:
         learned = learned_start(ctx, pkg, pair, grid, ws, step, srad, args.steps)
         log(f"[bench] learned start: {learned}")
+    mgs = None
+    if not args.no_mg:
+        mgs = multigrid_search(ctx, pair, grid, ws, step, args.steps)
+        log(f"[bench] multigrid search: {mgs}")
     it6 = out["iters6"]
     conv = int(np.sum((it6 >= 1) & (it6 <= 100)))
     value = conv / (ms_per_step / 1e3)
@@ -298,7 +304,7 @@ def main():
                    "stage_ms_from": "separate untimed frames with an event at every stage boundary",
                    "gridfit": counters,
                    "icgn6_subsets_per_s": round(conv / t_i, 1),
-                   "learned_start": learned},
+                   "learned_start": learned, "multigrid_search": mgs},
         "roofline": roof,
         extra_key: extra_roof,
         "cpu_baseline": cpu,
@@ -351,6 +357,31 @@ def learned_start(ctx, pkg, pair, grid, ws, step, srad, reps):
     out["note"] = ("LocalICGN stage only (start + 6-DOF IC-GN over 247,009 subsets); the "
                    "search start is the exact 12-bit integer search")
     return out
+
+
+def multigrid_search(ctx, pair, grid, ws, step, reps):
+    """funIntegerSearchMg (IntegerSearch.m's method 0) over the workload's ROI on the
+    staged pair: wall time of the whole call (level-1 FFT correlation, every later level,
+    the host's region bookkeeping between levels and the final gridfit), levels, final
+    regions, and the fitted grid against the analytic field."""
+    ctx.set_images(pair.f, pair.g)
+    g4 = (grid[0], grid[1], grid[2], grid[3])
+    r = ctx.integer_search_mg(g4, ws, step)                 # warmup
+    n = max(3, min(reps, 10))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        r = ctx.integer_search_mg(g4, ws, step)
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    ua, va = pair.ufun(r["x"], r["y"])
+    err = np.hypot(r["u"] - ua, r["v"] - va)
+    ok = ~np.isnan(r["tempu"])
+    ta, tv = pair.ufun(r["tempx"][ok], r["tempy"][ok])
+    terr = np.hypot(r["tempu"][ok] - ta, r["tempv"][ok] - tv)
+    return dict(ms=round(ms, 3), levels=r["levels"], regions=int(r["tempx"].size),
+                regions_nan=r["n_nan"], grid=[r["Nx"], r["Ny"]], fit_cg_iters=r["fit_iters"],
+                region_rms_vs_field_px=round(float(np.sqrt(np.mean(terr ** 2))), 4),
+                grid_rms_vs_field_px=round(float(np.sqrt(np.mean(err ** 2))), 4),
+                note="wall time per call incl. host bookkeeping and per-level waits")
 
 
 def cpu_full_frame(oracle, pair, out, S, ws, step, srad, grid, mu, beta, n_admm, threads):
