@@ -117,11 +117,12 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
 hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, double rtol, int maxit,
                          DBuf *work, double *stats_dev, hipStream_t st, int poll);
 // gridfit.m ('springs', 'triangle', smoothness 1) of np scattered points (px, py device,
-// strictly inside the grid; pd their value pairs) onto the grid g was set up for
-// (xnodes_dev, ynodes_dev: its nodes on the device): U (gx * gy pairs, node iy * gx + ix)
-// receives the fit.  work[0..2] are used; a9 is overwritten.
+// strictly inside the grid; pd their npair (1 or 2) value pairs each) onto the grid g was
+// set up for (xnodes_dev, ynodes_dev: its nodes on the device): U (gx * gy nodes of npair
+// pairs, node iy * gx + ix) receives the fit of every pair (one CG on them together).
+// work[0..2] are used; a9 is overwritten.
 hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const double *ynodes_dev,
-                             int np, const double *px, const double *py, const double2 *pd,
+                             int np, const double *px, const double *py, const double2 *pd, int npair,
                              double *U, double rtol, int maxit, DBuf *work, double *stats_dev,
                              hipStream_t st);
 // nanconv(., fspecial('gaussian', 3, sigma), 'edge') of nc values per node on the gx x gy

@@ -360,6 +360,7 @@ __device__ __forceinline__ int gf_cell(const double *nodes, int n, double v)
     return min(max(lo, 1), n - 1) - 1;
 }
 
+template <int NP>
 __global__ __launch_bounds__(256) void k_gf_scatter(int np, const double *px, const double *py,
                                                     const double2 *pd, const double *xn, int gx,
                                                     const double *yn, int gy, double *a9,
@@ -375,10 +376,13 @@ __global__ __launch_bounds__(256) void k_gf_scatter(int np, const double *px, co
     const int nd[3] = {iy * gx + ix, (iy + 1) * gx + ix + 1, tx > ty ? iy * gx + ix + 1 : (iy + 1) * gx + ix};
     const int ox[3] = {0, 1, tx > ty ? 1 : 0}, oy[3] = {0, 1, tx > ty ? 0 : 1};
     const double w[3] = {1.0 - t2, t1, t2 - t1};
-    const double2 d = pd[p];
     for (int a = 0; a < 3; ++a) {
-        atomicAdd(&b[nd[a]].x, w[a] * d.x);
-        atomicAdd(&b[nd[a]].y, w[a] * d.y);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const double2 d = pd[(size_t)p * NP + q];
+            atomicAdd(&b[(size_t)nd[a] * NP + q].x, w[a] * d.x);
+            atomicAdd(&b[(size_t)nd[a] * NP + q].y, w[a] * d.y);
+        }
         atomicAdd(&colsum[nd[a]], w[a]);
         for (int c = 0; c < 3; ++c) {
             const int k = (oy[c] - oy[a] + 1) * 3 + (ox[c] - ox[a] + 1);
@@ -416,6 +420,7 @@ __global__ __launch_bounds__(256) void k_gf_lam2(const double *part, int nb, dou
 }
 
 // K0 of the scattered fit: N = A^T A (already in a9) + lam^2 L, r_0 = A^T d (x_0 = 0)
+template <int NP>
 __global__ __launch_bounds__(kGfThreads) void k_gf_cg0s(GfCg c, const double *w, const double *lam2p,
                                                         const double2 *bvec)
 {
@@ -440,12 +445,16 @@ __global__ __launch_bounds__(kGfThreads) void k_gf_cg0s(GfCg c, const double *w,
         const double d = c.a9[4 * c.n + i] + lam2 * dsum;
         c.a9[4 * c.n + i] = d;
         c.dinv[i] = 1.0 / d;
-        const double2 b = bvec[i];
-        c.x[i] = make_double2(0.0, 0.0);
-        c.r[0][i] = b;
-        c.p[i] = make_double2(0.0, 0.0);
-        c.s[0][i] = make_double2(0.0, 0.0);
-        bb += b.x * b.x + b.y * b.y;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const size_t e = (size_t)i * NP + q;
+            const double2 b = bvec[e];
+            c.x[e] = make_double2(0.0, 0.0);
+            c.r[0][e] = b;
+            c.p[e] = make_double2(0.0, 0.0);
+            c.s[0][e] = make_double2(0.0, 0.0);
+            bb += b.x * b.x + b.y * b.y;
+        }
     }
     gf_block3(bb, 0.0, 0.0, c.BB, nullptr, nullptr);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -511,7 +520,7 @@ hipError_t gridfit_setup(GridfitDev &g, const double *xnodes, int gx, const doub
     return hipStreamSynchronize(st);
 }
 
-// scat (nullable): the scattered fit's {lam^2 (device), A^T d (device)}, NP 1 only
+// scat (nullable): the scattered fit's {lam^2 (device), A^T d (device, NP pairs per node)}
 struct GfScat {
     const double *lam2;
     const double2 *b;
@@ -552,7 +561,7 @@ static hipError_t solve_np(GridfitDev &g, const int *bad, double *U, double rtol
     c.nb = nb;
     c.rtol2 = rtol * rtol;
     if (scat)
-        hipLaunchKernelGGL(k_gf_cg0s, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
+        hipLaunchKernelGGL(k_gf_cg0s<NP>, dim3(nb), dim3(kGfThreads), 0, st, c, (const double *)g.w.p,
                            scat->lam2, scat->b);
     else
         hipLaunchKernelGGL(k_gf_cg0<NP>, dim3(nb), dim3(kGfThreads), 0, st, c,
@@ -596,28 +605,33 @@ hipError_t gridfit_solve(GridfitDev &g, const int *bad, double *U, int np, doubl
 }
 
 hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const double *ynodes_dev,
-                             int np, const double *px, const double *py, const double2 *pd,
+                             int np, const double *px, const double *py, const double2 *pd, int npair,
                              double *U, double rtol, int maxit, DBuf *work, double *stats_dev,
                              hipStream_t st)
 {
     const int n = g.gx * g.gy;
     hipError_t e;
-    if (np <= 0) return hipErrorInvalidValue;
-    if ((e = work[2].ensure(sizeof(double2) * (size_t)n + sizeof(double) * ((size_t)n + 1 + kGfMaxParts))) !=
-        hipSuccess)
+    if (np <= 0 || npair < 1 || npair > 2) return hipErrorInvalidValue;
+    const size_t nb2 = (size_t)n * npair;
+    if ((e = work[2].ensure(sizeof(double2) * nb2 + sizeof(double) * ((size_t)n + 1 + kGfMaxParts))) != hipSuccess)
         return e;
     double2 *b = work[2].as<double2>();
-    double *colsum = reinterpret_cast<double *>(b + n), *lam2 = colsum + n, *parts = lam2 + 1;
+    double *colsum = reinterpret_cast<double *>(b + nb2), *lam2 = colsum + n, *parts = lam2 + 1;
     const int nbm = std::min(kGfMaxParts, (n + 255) / 256);
     if ((e = hipMemsetAsync(g.a9.p, 0, 9 * sizeof(double) * (size_t)n, st)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(work[2].p, 0, sizeof(double2) * (size_t)n + sizeof(double) * n, st)) != hipSuccess)
+    if ((e = hipMemsetAsync(work[2].p, 0, sizeof(double2) * nb2 + sizeof(double) * n, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_gf_scatter, dim3((np + 255) / 256), dim3(256), 0, st, np, px, py, pd, xnodes_dev, g.gx,
-                       ynodes_dev, g.gy, g.a9.as<double>(), b, colsum);
+    if (npair == 1)
+        hipLaunchKernelGGL(k_gf_scatter<1>, dim3((np + 255) / 256), dim3(256), 0, st, np, px, py, pd, xnodes_dev,
+                           g.gx, ynodes_dev, g.gy, g.a9.as<double>(), b, colsum);
+    else
+        hipLaunchKernelGGL(k_gf_scatter<2>, dim3((np + 255) / 256), dim3(256), 0, st, np, px, py, pd, xnodes_dev,
+                           g.gx, ynodes_dev, g.gy, g.a9.as<double>(), b, colsum);
     hipLaunchKernelGGL(k_gf_colmax, dim3(nbm), dim3(256), 0, st, (const double *)colsum, n, parts);
     hipLaunchKernelGGL(k_gf_lam2, dim3(1), dim3(256), 0, st, (const double *)parts, nbm, g.lam2, lam2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const GfScat sc{lam2, b};
+    if (npair == 2) return solve_np<2>(g, nullptr, U, rtol, maxit, work, stats_dev, st, 32, &sc);
     return solve_np<1>(g, nullptr, U, rtol, maxit, work, stats_dev, st, 32, &sc);
 }
 

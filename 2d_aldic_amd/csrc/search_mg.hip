@@ -38,6 +38,21 @@ __global__ void k_twiddles(double2 *tw, int N)
     tw[k] = make_double2(c, s);
 }
 
+template <bool INV>
+__device__ __forceinline__ double2 tw_at(const double2 *__restrict__ tw, int i)
+{
+    const double2 w = tw[i];
+    return INV ? make_double2(w.x, -w.y) : w;
+}
+
+// radix-2 butterfly: (a, b) <- (a + w b, a - w b)
+__device__ __forceinline__ void bfly(double2 &a, double2 &b, double2 w)
+{
+    const double2 wb = make_double2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
+    b = make_double2(a.x - wb.x, a.y - wb.y);
+    a = make_double2(a.x + wb.x, a.y + wb.y);
+}
+
 // One FFT of length N = 2^logN per workgroup, in LDS: element i of line `line` at
 // data[line * lstride + i * estride]; bit-reversed load, radix-2 decimation in time,
 // twiddles exp(-2 pi i k / N) = tw[k * tmul] of the table of length L = N * tmul
@@ -53,16 +68,36 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(double2 *data, const double
         s[j] = p[(long)i * estride];
     }
     __syncthreads();
-    for (int len = 2, step = N / 2; len <= N; len <<= 1, step >>= 1) {
-        const int half = len >> 1;
-        for (int t = threadIdx.x; t < N / 2; t += kFftThreads) {
-            const int k = t & (half - 1), i0 = (t / half) * len + k, i1 = i0 + half;
-            double2 w = tw[k * step * tmul];
-            if (INV) w.y = -w.y;
-            const double2 a = s[i0], b = s[i1];
-            const double2 wb = make_double2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
-            s[i0] = make_double2(a.x + wb.x, a.y + wb.y);
-            s[i1] = make_double2(a.x - wb.x, a.y - wb.y);
+    // stages of half-length h: two at a time in registers (lengths 2h and 4h: each thread
+    // owns elements i0, i0 + h, i0 + 2h, i0 + 3h), a single one when one is left
+    for (int h = 1; h < N;) {
+        if (2 * h < N) {
+            const int sa = N / (2 * h) * tmul, sb = N / (4 * h) * tmul;
+            for (int t = threadIdx.x; t < N / 4; t += kFftThreads) {
+                const int k = t & (h - 1), i0 = (t / h) * 4 * h + k;
+                double2 x0 = s[i0], x1 = s[i0 + h], x2 = s[i0 + 2 * h], x3 = s[i0 + 3 * h];
+                const double2 wa = tw_at<INV>(tw, k * sa), w0 = tw_at<INV>(tw, k * sb),
+                              w1 = tw_at<INV>(tw, (k + h) * sb);
+                bfly(x0, x1, wa);
+                bfly(x2, x3, wa);
+                bfly(x0, x2, w0);
+                bfly(x1, x3, w1);
+                s[i0] = x0;
+                s[i0 + h] = x1;
+                s[i0 + 2 * h] = x2;
+                s[i0 + 3 * h] = x3;
+            }
+            h *= 4;
+        } else {
+            const int sa = N / (2 * h) * tmul;
+            for (int t = threadIdx.x; t < N / 2; t += kFftThreads) {
+                const int k = t & (h - 1), i0 = (t / h) * 2 * h + k;
+                double2 x0 = s[i0], x1 = s[i0 + h];
+                bfly(x0, x1, tw_at<INV>(tw, k * sa));
+                s[i0] = x0;
+                s[i0 + h] = x1;
+            }
+            h *= 2;
         }
         __syncthreads();
     }
@@ -89,14 +124,15 @@ __global__ void k_mg_operands(const float *__restrict__ f, const float *__restri
 }
 
 // From Z = FFT(T' + i D): FT(k) = (Z(k) + conj Z(-k)) / 2, FD(k) = (Z(k) - conj Z(-k)) / 2i;
-// Z(k) <- conj(FT(k)) FD(k), and Z(-k) <- its conjugate (the correlation is real).  The
-// thread of the smaller index of each pair does both.
-__global__ void k_mg_cross(double2 *__restrict__ Z, int P, int Q)
+// Z(k) <- conj(FT(k)) FD(k), and Z(-k) <- its conjugate (the correlation is real).  Z is
+// A x B (B fastest, both powers of two); the thread of the smaller index of each pair
+// does both.
+__global__ void k_mg_cross(double2 *__restrict__ Z, int A, int B)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)P * Q) return;
-    const int kx = (int)(i % P), ky = (int)(i / P);
-    const long j = (long)((Q - ky) & (Q - 1)) * P + ((P - kx) & (P - 1));
+    if (i >= (long)A * B) return;
+    const int kb = (int)(i % B), ka = (int)(i / B);
+    const long j = (long)((A - ka) & (A - 1)) * B + ((B - kb) & (B - 1));
     if (j < i) return;
     const double2 a = Z[i], b = Z[j];
     // FT = ((a.x + b.x) + i (a.y - b.y)) / 2, FD = ((a.y + b.y) - i (a.x - b.x)) / 2
@@ -105,6 +141,23 @@ __global__ void k_mg_cross(double2 *__restrict__ Z, int P, int Q)
     const double rr = tr * dr + ti * di, ri = tr * di - ti * dr;
     Z[i] = make_double2(rr, ri);
     if (j != i) Z[j] = make_double2(rr, -ri);
+}
+
+// out[c * rows + r] = in[r * cols + c], 32 x 32 tiles through LDS (both sides coalesced)
+__global__ __launch_bounds__(256) void k_transpose(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                   int rows, int cols)
+{
+    __shared__ double2 t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) t[k][tx] = in[(long)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) out[(long)c * rows + r] = t[tx][k];
+    }
 }
 
 // sum, sum of squares and largest |v - first pixel| of the ROI of f (blocks of partials,
@@ -476,13 +529,13 @@ int nblk(long n, int t = 256) { return (int)((n + t - 1) / t); }
 }  // namespace
 
 struct MgWork {
-    DBuf zt, zd, tw, s1, s2, map, part, red, peak, reg, tiles, toff, tpart, tstat, out;
+    DBuf zt, zy, zd, tw, s1, s2, map, part, red, peak, reg, tiles, toff, tpart, tstat, out;
     DBuf nodes, pts, fit, gstat, gw[3];
     GridfitDev gf;
     std::vector<double> gx_last, gy_last;   // the node lists gf was set up for
     ~MgWork()
     {
-        for (DBuf *b : {&zt, &zd, &tw, &s1, &s2, &map, &part, &red, &peak, &reg, &tiles, &toff, &tpart, &tstat,
+        for (DBuf *b : {&zt, &zy, &zd, &tw, &s1, &s2, &map, &part, &red, &peak, &reg, &tiles, &toff, &tpart, &tstat,
                         &out, &nodes, &pts, &fit, &gstat, &gw[0], &gw[1], &gw[2], &gf.w, &gf.a9, &gf.dinv})
             b->release();
     }
@@ -493,8 +546,8 @@ void mg_work_free(MgWork *w) { delete w; }
 
 namespace {
 
-// rows (Q lines of length P, contiguous), then columns (P lines of length Q, stride P)
-hipError_t fft2(double2 *Z, const double2 *tw, int L, int P, int lp, int Q, int lq, bool inv, hipStream_t st)
+// nlines contiguous lines of length N = 2^logN from Z, transformed in place
+hipError_t fft_rows(double2 *Z, const double2 *tw, int L, int N, int logN, int nlines, bool inv, hipStream_t st)
 {
     static bool attr = false;
     if (!attr) {
@@ -506,18 +559,19 @@ hipError_t fft2(double2 *Z, const double2 *tw, int L, int P, int lp, int Q, int 
             return e;
         attr = true;
     }
-    if (inv) {
-        hipLaunchKernelGGL(k_fft<true>, dim3(Q), dim3(kFftThreads), sizeof(double2) * P, st, Z, tw, P, lp, L / P,
-                           (long)P, 1L);
-        hipLaunchKernelGGL(k_fft<true>, dim3(P), dim3(kFftThreads), sizeof(double2) * Q, st, Z, tw, Q, lq, L / Q, 1L,
-                           (long)P);
-    } else {
-        hipLaunchKernelGGL(k_fft<false>, dim3(Q), dim3(kFftThreads), sizeof(double2) * P, st, Z, tw, P, lp, L / P,
-                           (long)P, 1L);
-        hipLaunchKernelGGL(k_fft<false>, dim3(P), dim3(kFftThreads), sizeof(double2) * Q, st, Z, tw, Q, lq, L / Q,
-                           1L, (long)P);
-    }
+    if (nlines <= 0) return hipSuccess;
+    if (inv)
+        hipLaunchKernelGGL(k_fft<true>, dim3(nlines), dim3(kFftThreads), sizeof(double2) * N, st, Z, tw, N, logN,
+                           L / N, (long)N, 1L);
+    else
+        hipLaunchKernelGGL(k_fft<false>, dim3(nlines), dim3(kFftThreads), sizeof(double2) * N, st, Z, tw, N, logN,
+                           L / N, (long)N, 1L);
     return hipGetLastError();
+}
+
+void transpose(const double2 *in, double2 *out, int rows, int cols, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0, st, in, out, rows, cols);
 }
 
 // Level 1 on the ROI x1..x1+m-1, y1..y1+n-1 (1-based) of the device images: out4 (host) =
@@ -533,7 +587,7 @@ hipError_t mg_level1(MgWork &W, const float *f, const float *g, int nx, int x1, 
     const int P = 1 << lp, Q = 1 << lq, L = P > Q ? P : Q;
     const long PQ = (long)P * Q, R = 2L * m - 1, C = 2L * n - 1;
     const int nbt = 512;
-    if ((e = W.zt.ensure(sizeof(double2) * PQ)) ||
+    if ((e = W.zt.ensure(sizeof(double2) * PQ)) || (e = W.zy.ensure(sizeof(double2) * PQ)) ||
         (e = W.zd.ensure(2 * sizeof(double) * (size_t)(m + 1) * ((n + 1 + kMgCh - 1) / kMgCh))) ||
         (e = W.tw.ensure(sizeof(double2) * (L / 2 + 1))) ||
         (e = W.s1.ensure(sizeof(double) * (m + 1) * (size_t)(n + 1))) ||
@@ -548,9 +602,18 @@ hipError_t mg_level1(MgWork &W, const float *f, const float *g, int nx, int x1, 
                        (long)m * n, tst);
     hipLaunchKernelGGL(k_mg_operands, dim3(nblk(PQ)), dim3(256), 0, st, f, g, nx, x1, y1, m, n,
                        (const double *)tst, P, Q, W.zt.as<double2>());
-    if ((e = fft2(W.zt.as<double2>(), W.tw.as<double2>(), L, P, lp, Q, lq, false, st))) return e;
-    hipLaunchKernelGGL(k_mg_cross, dim3(nblk(PQ)), dim3(256), 0, st, W.zt.as<double2>(), P, Q);
-    if ((e = fft2(W.zt.as<double2>(), W.tw.as<double2>(), L, P, lp, Q, lq, true, st))) return e;
+    // 2-D transform as row transforms and transposes (every pass coalesced): Z is [y][x];
+    // rows y >= n are zero and stay zero under the first pass; the spectrum is formed and
+    // multiplied in the transposed layout [kx][ky]; the correlation returns to [y][x]
+    double2 *Z = W.zt.as<double2>(), *Y = W.zy.as<double2>();
+    const double2 *tw = W.tw.as<double2>();
+    if ((e = fft_rows(Z, tw, L, P, lp, n, false, st))) return e;
+    transpose(Z, Y, Q, P, st);
+    if ((e = fft_rows(Y, tw, L, Q, lq, P, false, st))) return e;
+    hipLaunchKernelGGL(k_mg_cross, dim3(nblk(PQ)), dim3(256), 0, st, Y, P, Q);
+    if ((e = fft_rows(Y, tw, L, Q, lq, P, true, st))) return e;
+    transpose(Y, Z, P, Q, st);
+    if ((e = fft_rows(Z, tw, L, P, lp, Q, true, st))) return e;
     const int nch = (n + 1 + kMgCh - 1) / kMgCh;
     double *c1 = W.zd.as<double>(), *c2 = c1 + (size_t)nch * (m + 1);
     hipLaunchKernelGGL(k_mg_prefix_rows, dim3((n + 1 + 3) / 4), dim3(256), 0, st, g, nx, x1, y1, m, n,
@@ -869,43 +932,61 @@ int integer_search_mg(MgWork &W, const float *f, const float *g, int nx, int ny,
         (e = hipMemcpyAsync(nodes + Nx, res.ylist.data(), sizeof(double) * Ny, hipMemcpyHostToDevice, st)))
         { err = hipGetErrorString(e); return 2; }
     // point arrays: x | y | (value pairs), per set
-    auto stage = [&](const std::vector<double> &p, double *dst) -> hipError_t {
-        const int np = (int)(p.size() / 4);
-        std::vector<double> soa(4 * (size_t)np);
+    // point arrays x | y | value pairs; u, v and Phi share one CG (two pairs per node) when
+    // every point has a displacement, else u, v and Phi are fitted apart
+    auto stage = [&](const std::vector<double> &p, int npair, double *dst) -> hipError_t {
+        const int w = 2 + 2 * npair, np = (int)(p.size() / w);
+        std::vector<double> soa(p.size());
         for (int i = 0; i < np; ++i) {
-            soa[i] = p[4 * i];
-            soa[np + i] = p[4 * i + 1];
-            soa[2 * np + 2 * i] = p[4 * i + 2];
-            soa[2 * np + 2 * i + 1] = p[4 * i + 3];
+            soa[i] = p[(size_t)w * i];
+            soa[np + i] = p[(size_t)w * i + 1];
+            for (int k = 0; k < 2 * npair; ++k) soa[2 * (size_t)np + 2 * npair * (size_t)i + k] = p[(size_t)w * i + 2 + k];
         }
         hipError_t r = hipMemcpyAsync(dst, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, st);
         if (r == hipSuccess) r = hipStreamSynchronize(st);
         return r;
     };
     const int npu = (int)(pu.size() / 4), npc = (int)(pc.size() / 4);
-    double *ppu = pts, *ppc = pts + 4 * (size_t)npu;
     double *fit = W.fit.as<double>();
-    if ((e = stage(pu, ppu)) || (e = stage(pc, ppc)) ||
-        (e = gridfit_scattered(W.gf, nodes, nodes + Nx, npu, ppu, ppu + npu,
-                               reinterpret_cast<const double2 *>(ppu + 2 * npu), fit, 1e-12, 4 * ng + 100, W.gw,
-                               W.gstat.as<double>(), st)) ||
-        (e = gridfit_scattered(W.gf, nodes, nodes + Nx, npc, ppc, ppc + npc,
-                               reinterpret_cast<const double2 *>(ppc + 2 * npc), fit + 2 * (size_t)ng, 1e-12,
-                               4 * ng + 100, W.gw, W.gstat.as<double>() + 3, st)))
-        { err = hipGetErrorString(e); return 2; }
+    int su, sc;   // node i: u = uv[su * i], v = uv[su * i + 1], Phi = cc[sc * i] (below)
+    if (npu == npc) {
+        std::vector<double> pj(6 * (size_t)npu);
+        for (int i = 0; i < npu; ++i) {
+            const double q[6] = {pu[4 * i], pu[4 * i + 1], pu[4 * i + 2], pu[4 * i + 3], pc[4 * i + 2], 0.0};
+            std::copy(q, q + 6, &pj[6 * (size_t)i]);
+        }
+        if ((e = stage(pj, 2, pts)) ||
+            (e = gridfit_scattered(W.gf, nodes, nodes + Nx, npu, pts, pts + npu,
+                                   reinterpret_cast<const double2 *>(pts + 2 * npu), 2, fit, 1e-12, 4 * ng + 100,
+                                   W.gw, W.gstat.as<double>(), st)))
+            { err = hipGetErrorString(e); return 2; }
+        su = sc = 4;
+    } else {
+        double *ppu = pts, *ppc = pts + 4 * (size_t)npu;
+        if ((e = stage(pu, 1, ppu)) || (e = stage(pc, 1, ppc)) ||
+            (e = gridfit_scattered(W.gf, nodes, nodes + Nx, npu, ppu, ppu + npu,
+                                   reinterpret_cast<const double2 *>(ppu + 2 * npu), 1, fit, 1e-12, 4 * ng + 100,
+                                   W.gw, W.gstat.as<double>(), st)) ||
+            (e = gridfit_scattered(W.gf, nodes, nodes + Nx, npc, ppc, ppc + npc,
+                                   reinterpret_cast<const double2 *>(ppc + 2 * npc), 1, fit + 2 * (size_t)ng,
+                                   1e-12, 4 * ng + 100, W.gw, W.gstat.as<double>() + 3, st)))
+            { err = hipGetErrorString(e); return 2; }
+        su = sc = 2;
+    }
     std::vector<double> hfit(4 * (size_t)ng);
-    double hst[6];
+    double hst[6] = {0, 0, 0, 0, 0, 0};
     if ((e = hipMemcpyAsync(hfit.data(), fit, sizeof(double) * hfit.size(), hipMemcpyDeviceToHost, st)) ||
-        (e = hipMemcpyAsync(hst, W.gstat.p, sizeof(hst), hipMemcpyDeviceToHost, st)) ||
+        (e = hipMemcpyAsync(hst, W.gstat.p, sizeof(double) * (npu == npc ? 3 : 6), hipMemcpyDeviceToHost, st)) ||
         (e = hipStreamSynchronize(st)))
         { err = hipGetErrorString(e); return 2; }
+    const double *uv = hfit.data(), *cc = npu == npc ? hfit.data() + 2 : hfit.data() + 2 * (size_t)ng;
     res.u.resize(ng);
     res.v.resize(ng);
     res.cc.resize(ng);
     for (int i = 0; i < ng; ++i) {
-        res.u[i] = hfit[2 * i];
-        res.v[i] = hfit[2 * i + 1];
-        res.cc[i] = hfit[2 * (size_t)ng + 2 * i];
+        res.u[i] = uv[(size_t)su * i];
+        res.v[i] = uv[(size_t)su * i + 1];
+        res.cc[i] = cc[(size_t)sc * i];
     }
     res.fit_iters[0] = (int)hst[1];
     res.fit_iters[1] = (int)hst[4];

@@ -74,6 +74,24 @@ def test_search_mg_matches_oracle_rectangular_large_shift(gpu_ctx, speckle):
     assert np.mean(np.abs(d["tempu"][ok] - 6.6) < 0.5) > 0.9
 
 
+def test_search_mg_flat_block_gives_nan_regions(gpu_ctx, speckle):
+    """A flat block in the reference: the regions inside it hit the reference's catch
+    (constant template -> NaN, Phi 0), their children stay NaN, gridfit drops them from
+    the u / v fit but keeps their Phi (u, v and Phi fitted apart)."""
+    import mg_oracle as mg
+    _flush("search_mg: 200 x 200 pair with a flat 60 x 60 block, ws 16, step 8")
+    p = speckle.small_pair(n=200, u0=1.6, v0=0.7, seed=8)
+    gpu_ctx.set_images(p.f, p.g)
+    f, g, _, _ = gpu_ctx.get_images()
+    f = f.copy()
+    f[20:80, 30:90] = f[20, 30]
+    gpu_ctx.set_normalized(f, g)
+    d = gpu_ctx.integer_search_mg((1, 200, 1, 200), 16, 8)
+    r = mg.search_mg(f, g, [1, 200], [1, 200], 16, 8)
+    assert d["n_nan"] > 0 and d["n_nan"] == int(np.isnan(r["tempu"]).sum())
+    _compare(d, r)
+
+
 def test_search_mg_rejects_constant_reference(gpu_ctx, pkg):
     _flush("search_mg: constant reference ROI")
     f = np.full((96, 96), 7.0)
