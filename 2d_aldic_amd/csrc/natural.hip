@@ -53,6 +53,8 @@ constexpr int kNatRingCap = 1;     // first pass: a cell still open past ring 1 
 constexpr int kNatWaveV = 512;     // k_nat_sibson_wave: lines of the cell (LDS, two buffers)
 constexpr int kNatWaveR = 24;      // lines of each lane's region polygon (LDS)
 constexpr int kNatWaveCand = 1024; // candidates gathered per chunk of 64 ring buckets
+constexpr int kNatPre = 8;         // ring buckets of at most this many nodes are read at once
+constexpr int kNatAllWave = 1024;  // up to this many inside bad nodes, all go to the wave pass
 
 struct NatArgs {
     const double2 *xy;
@@ -589,6 +591,7 @@ __global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
     const Poly<kNatThreads> V{&s_tag[0][t], &s_vx[0][t], &s_vy[0][t]};
     const Poly<kNatThreads> R{&s_tag[1][t], &s_vx[1][t], &s_vy[1][t]};
     const int n_in = a.ctl[2];
+    if (n_in <= kNatAllWave) return;   // few nodes: one wave each (lower latency per cell)
     for (int li = blockIdx.x * kNatThreads + t; li < n_in; li += gridDim.x * kNatThreads) {
         const int q = a.lin[li];
         const int r = nat_sibson_node<kNatThreads, kNatMax, kNatRingCap>(a, q, V, R);
@@ -751,12 +754,31 @@ __device__ int nat_sibson_wave(const NatArgs &a, int q, NatWaveLds &L, int lane)
                     s1 = a.bstart[j * a.BX + i + 1];
                 }
             }
+            // buckets of up to kNatPre nodes: every load issued at once, kept in registers
+            const bool pre = s1 - s0 <= kNatPre;
+            int pn[kNatPre];
+            double pxs[kNatPre], pys[kNatPre];
             int c = 0;
-            for (int s = s0; s < s1; ++s) {
-                const int p = a.bnode[s];
-                if (p == q || a.bad[p]) continue;
-                const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
-                if (reaches(px, py)) ++c;
+            if (pre) {
+#pragma unroll
+                for (int u = 0; u < kNatPre; ++u) pn[u] = s0 + u < s1 ? a.bnode[s0 + u] : -1;
+#pragma unroll
+                for (int u = 0; u < kNatPre; ++u) {
+                    const int p = pn[u] < 0 ? q : pn[u];
+                    const double2 P = a.xy[p];
+                    const int bd = a.bad[p];
+                    pxs[u] = P.x - Q.x;
+                    pys[u] = P.y - Q.y;
+                    if (p == q || bd || !reaches(pxs[u], pys[u])) pn[u] = -1;
+                    c += pn[u] >= 0;
+                }
+            } else {
+                for (int s = s0; s < s1; ++s) {
+                    const int p = a.bnode[s];
+                    if (p == q || a.bad[p]) continue;
+                    const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
+                    if (reaches(px, py)) ++c;
+                }
             }
             int off = c;
 #pragma unroll
@@ -767,15 +789,26 @@ __device__ int nat_sibson_wave(const NatArgs &a, int q, NatWaveLds &L, int lane)
             const int total = __shfl(off, kWave - 1, kWave);
             if (total > kNatWaveCand) return -2;
             off -= c;
-            for (int s = s0; s < s1; ++s) {
-                const int p = a.bnode[s];
-                if (p == q || a.bad[p]) continue;
-                const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
-                if (reaches(px, py)) {
-                    L.cand[off] = p;
-                    L.cx[off] = px;
-                    L.cy[off] = py;
-                    ++off;
+            if (pre) {
+#pragma unroll
+                for (int u = 0; u < kNatPre; ++u)
+                    if (pn[u] >= 0) {
+                        L.cand[off] = pn[u];
+                        L.cx[off] = pxs[u];
+                        L.cy[off] = pys[u];
+                        ++off;
+                    }
+            } else {
+                for (int s = s0; s < s1; ++s) {
+                    const int p = a.bnode[s];
+                    if (p == q || a.bad[p]) continue;
+                    const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
+                    if (reaches(px, py)) {
+                        L.cand[off] = p;
+                        L.cx[off] = px;
+                        L.cy[off] = py;
+                        ++off;
+                    }
                 }
             }
             wave_sync_lds();
@@ -859,9 +892,13 @@ __global__ __launch_bounds__(64) void k_nat_sibson_wave(NatArgs a)
 {
     __shared__ NatWaveLds L;
     const int lane = threadIdx.x;
-    const int nb = a.ctl[5];
+    // every inside node when they are few (the thread pass did not run), else the ones
+    // the thread pass left open
+    const bool all = a.ctl[2] <= kNatAllWave;
+    const int nb = all ? a.ctl[2] : a.ctl[5];
+    const int *list = all ? a.lin : a.lbig;
     for (int li = blockIdx.x; li < nb; li += gridDim.x) {
-        const int q = a.lbig[li];
+        const int q = list[li];
         const int r = nat_sibson_wave(a, q, L, lane);
         if (lane == 0) {
             if (r == -2) a.lbig2[atomicAdd(&a.ctl[6], 1)] = q;
