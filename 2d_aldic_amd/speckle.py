@@ -193,23 +193,56 @@ def make_sequence(nx, ny, n_dots, sigma, ufuns, noise, bits=8, amp=(0.4, 1.0), s
     if hole is not None:
         hx, hy, ha = hole
         f[(x - hx) ** 2 + (y - hy) ** 2 < ha * ha] = 0
-    gs = []
-    for k, ufun in enumerate(ufuns):
-        Xm, Ym = inverse_map(ufun, nx, ny)
-
-        def fwd(cx, cy, ufun=ufun):
-            u, v = ufun(cx, cy)
-            return cx + u, cy + v
-
-        gn = render_parallel(dots, nx, ny, Xm, Ym, fwd) * scale
-        if noise > 0:
-            gn = gn + np.random.default_rng(seed + 1 + k).normal(0.0, noise, gn.shape)
-        if hole is not None:
-            gn[(Xm - hx) ** 2 + (Ym - hy) ** 2 < ha * ha] = 0.0
-        gs.append(np.clip(np.rint(gn), 0, full).astype(dt))
-        if progress:
-            progress(k)
+    # one frame per forked worker; each renders its dots in the same 16 groups and sums
+    # them in the same order as render_parallel, so the frames are the same either way
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import os
+    n = dots.cx.size
+    _SEQ.update(dots=dots, nx=nx, ny=ny, ufuns=ufuns, noise=noise, seed=seed, hole=hole,
+                scale=scale, full=full, dt=dt, bounds=[n * k // 16 for k in range(17)])
+    workers = max(1, min(len(ufuns), len(os.sched_getaffinity(0)), 16))
+    gs = [None] * len(ufuns)
+    try:
+        with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
+            futs = {ex.submit(_seq_frame, k): k for k in range(len(ufuns))}
+            done = 0
+            for fu in cf.as_completed(futs):
+                gs[futs[fu]] = fu.result()
+                if progress:
+                    progress(done)
+                done += 1
+    finally:
+        _SEQ.clear()
     return f, gs
+
+
+_SEQ = {}
+
+
+def _seq_frame(k):
+    """Deformed frame k of make_sequence (run in a forked worker)."""
+    a = _SEQ
+    nx, ny, ufun = a["nx"], a["ny"], a["ufuns"][k]
+    Xm, Ym = inverse_map(ufun, nx, ny)
+
+    def fwd(cx, cy):
+        u, v = ufun(cx, cy)
+        return cx + u, cy + v
+
+    d, b = a["dots"], a["bounds"]
+    gn = np.zeros((ny, nx))
+    for j in range(len(b) - 1):
+        sub = DotField(d.cx[b[j]:b[j + 1]], d.cy[b[j]:b[j + 1]], d.amp[b[j]:b[j + 1]],
+                       d.sigma[b[j]:b[j + 1]])
+        gn += _render(sub, nx, ny, Xm, Ym, fwd)
+    gn = gn * a["scale"]
+    if a["noise"] > 0:
+        gn = gn + np.random.default_rng(a["seed"] + 1 + k).normal(0.0, a["noise"], gn.shape)
+    if a["hole"] is not None:
+        hx, hy, ha = a["hole"]
+        gn[(Xm - hx) ** 2 + (Ym - hy) ** 2 < ha * ha] = 0.0
+    return np.clip(np.rint(gn), 0, a["full"]).astype(a["dt"])
 
 
 # --- the five workloads ------------------------------------------------------
