@@ -50,11 +50,12 @@ constexpr int kNatBig = 512;       // lines per polygon in the second pass (devi
 constexpr int kNatBigThreads = 1024;  // second-pass threads (polygon slots)
 constexpr int kRegion = 0x40000000;  // tag bit: bisector of p_i and p_(tag & ~kRegion)
 constexpr int kNatRingCap = 1;     // first pass: a cell still open past ring 1 -> big list
-constexpr int kNatWaveV = 512;     // k_nat_sibson_wave: lines of the cell (LDS, two buffers)
+constexpr int kNatWaveV = 128;     // k_nat_sibson_wave: lines of the cell (LDS, two buffers)
 constexpr int kNatWaveR = 24;      // lines of each lane's region polygon (LDS)
-constexpr int kNatWaveCand = 1024; // candidates gathered per chunk of 64 ring buckets
-constexpr int kNatPre = 8;         // ring buckets of at most this many nodes are read at once
-constexpr int kNatAllWave = 1024;  // up to this many inside bad nodes, all go to the wave pass
+constexpr int kNatWaveCand = 1536; // candidates gathered per chunk of 64 ring buckets
+constexpr int kNatPre = 4;         // nodes per ring bucket read with their loads in flight
+constexpr int kNatLaneB = 4;       // ring buckets per lane in a chunk of the wave pass
+constexpr int kNatAllWave = 8192;  // up to this many inside bad nodes, all go to the wave pass
 
 struct NatArgs {
     const double2 *xy;
@@ -609,12 +610,20 @@ __global__ __launch_bounds__(kNatThreads) void k_nat_sibson(NatArgs a)
 struct NatWaveLds {
     int tag[2][kNatWaveV];
     double vx[2][kNatWaveV], vy[2][kNatWaveV];
-    int cand[kNatWaveCand];
-    double cx[kNatWaveCand], cy[kNatWaveCand];   // candidates relative to q
     double sx[kNatWaveV], sy[kNatWaveV];         // the cell's sites relative to q
-    int rtag[kNatWaveR * 64];
-    double rvx[kNatWaveR * 64], rvy[kNatWaveR * 64];
+    union {   // the candidates while the cell is built, then the lanes' region polygons
+        struct {
+            int cand[kNatWaveCand];
+            double cx[kNatWaveCand], cy[kNatWaveCand];   // candidates relative to q
+        };
+        struct {
+            int rtag[kNatWaveR * 64];
+            double rvx[kNatWaveR * 64], rvy[kNatWaveR * 64];
+        };
+    };
 };
+static_assert(kNatWaveCand >= kNatWaveR * 64, "region polygons share the candidates' LDS");
+static_assert(sizeof(NatWaveLds) <= 40 * 1024, "four wave-pass workgroups per CU");
 
 __device__ __forceinline__ bool nat_outside(double x, double y, const Line &M)
 {
@@ -686,6 +695,16 @@ __device__ int nat_wclip(const NatArgs &a, double2 Q, double B, NatWaveLds &L, i
 
 // Bucket t (< 8 r, or the one bucket of ring 0) of ring r around (bx, by), in the order
 // nat_sibson_node visits them.
+// ring of bucket T of the ring sequence (ring r >= 1 holds T = 1 + 4 r (r - 1) .. 4 r (r + 1))
+__device__ __forceinline__ int bucket_ring(long T)
+{
+    if (T <= 0) return 0;
+    int r = (int)((1.0 + sqrt((double)T)) * 0.5);
+    while (1 + 4L * r * (r + 1) <= T) ++r;
+    while (r > 1 && 1 + 4L * r * (r - 1) > T) --r;
+    return r;
+}
+
 __device__ __forceinline__ void ring_bucket(int r, int t, int bx, int by, int &i, int &j)
 {
     if (r == 0) { i = bx; j = by; return; }
@@ -696,8 +715,9 @@ __device__ __forceinline__ void ring_bucket(int r, int t, int bx, int by, int &i
     i = (t & 1) ? bx + r : bx - r;
 }
 
-// Sibson weights of inside node q by one wave: the cell as nat_sibson_node builds it
-// (same candidates, same order, wave-parallel clips), then lane k the region of natural
+// Sibson weights of inside node q by one wave: the cell nat_sibson_node builds (the same
+// bisectors; sites that cut nothing are dropped before the wave-parallel clips, and a
+// vertex is always the meet of its two lines, so the order does not change it), then lane k the region of natural
 // neighbour k: the square clipped by q's bisector with p_k and by p_k's bisectors with
 // p_(k+1) and p_(k-1) first, then by the others outward along the cell, so it stays a few
 // lines.  Returns 0, -1 (the cell could not be formed) or -2 (beyond the LDS limits).
@@ -715,116 +735,133 @@ __device__ int nat_sibson_wave(const NatArgs &a, int q, NatWaveLds &L, int lane)
         L.vy[0][lane] = y;
     }
     wave_sync_lds();
-    // the cell's radius and bounding box (relative to q): a site p cuts the cell only if
-    // some vertex v has p . v > |p|^2 / 2, and max_v p . v is at most the box's support
-    double rho, bx0, bx1, by0, by1;
+    // the cell's radius and its supports h(d) = max_v d . v (relative to q) in the eight
+    // directions +-x, +-y, (+-1, +-1): a site p cuts the cell only if h(p) > |p|^2 / 2, and
+    // h is sublinear, so with p = alpha a + beta b over the two of those directions around
+    // it (alpha, beta >= 0), h(p) <= alpha h(a) + beta h(b) -- an octagon around the cell,
+    // much tighter than its box for the long thin cells of nodes next to a hole
+    double rho, hx[2], hy[2], hd[4];   // hx[sign x], hy[sign y], hd[2 (y < 0) + (x < 0)]
     auto shape = [&]() {
-        double r2 = 0.0, x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+        double r2 = 0.0, e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) e[i] = -INFINITY;
         for (int k = lane; k < m; k += kWave) {
             const double x = L.vx[cur][k], y = L.vy[cur][k];
             r2 = fmax(r2, x * x + y * y);
-            x0 = fmin(x0, x); x1 = fmax(x1, x); y0 = fmin(y0, y); y1 = fmax(y1, y);
+            e[0] = fmax(e[0], x); e[1] = fmax(e[1], -x); e[2] = fmax(e[2], y); e[3] = fmax(e[3], -y);
+            e[4] = fmax(e[4], x + y); e[5] = fmax(e[5], y - x); e[6] = fmax(e[6], x - y);
+            e[7] = fmax(e[7], -x - y);
         }
         rho = sqrt(wave_max_d(r2));
-        bx0 = -wave_max_d(-x0); bx1 = wave_max_d(x1);
-        by0 = -wave_max_d(-y0); by1 = wave_max_d(y1);
+        hx[0] = wave_max_d(e[0]); hx[1] = wave_max_d(e[1]);
+        hy[0] = wave_max_d(e[2]); hy[1] = wave_max_d(e[3]);
+        hd[0] = wave_max_d(e[4]); hd[1] = wave_max_d(e[5]); hd[2] = wave_max_d(e[6]);
+        hd[3] = wave_max_d(e[7]);
     };
     auto reaches = [&](double px, double py) {
         const double d2 = px * px + py * py;
-        return d2 < 4.0 * rho * rho &&
-               fmax(px * bx0, px * bx1) + fmax(py * by0, py * by1) > 0.5 * d2;
+        if (!(d2 < 4.0 * rho * rho)) return false;
+        const int sx = px < 0.0, sy = py < 0.0;
+        const double ax = fabs(px), ay = fabs(py), sd = hd[2 * sy + sx];
+        const double h = ax >= ay ? (ax - ay) * hx[sx] + ay * sd : (ay - ax) * hy[sy] + ax * sd;
+        return h > 0.5 * d2;
     };
+    auto cuts = reaches;
     shape();
     int bx = (int)floor((Q.x - a.ox) / a.h), by = (int)floor((Q.y - a.oy) / a.h);
     bx = bx < 0 ? 0 : (bx >= a.BX ? a.BX - 1 : bx);
     by = by < 0 ? 0 : (by >= a.BY ? a.BY - 1 : by);
     const int rmax = max(max(bx, a.BX - 1 - bx), max(by, a.BY - 1 - by));
-    for (int r = 0; r <= rmax; ++r) {
-        if (r > 0 && (double)(r - 1) * a.h >= 2.0 * rho) break;
-        const int nbk = r == 0 ? 1 : 8 * r;
-        for (int t0 = 0; t0 < nbk; t0 += kWave) {
-            // this lane's bucket: its good nodes within reach, appended in lane order
-            const int t = t0 + lane;
-            int s0 = 0, s1 = 0;
-            if (t < nbk) {
-                int i, j;
-                ring_bucket(r, t, bx, by, i, j);
-                if (i >= 0 && i < a.BX && j >= 0 && j < a.BY) {
-                    s0 = a.bstart[j * a.BX + i];
-                    s1 = a.bstart[j * a.BX + i + 1];
+    // the ring buckets as one sequence (ring 0 is bucket 0, ring r >= 1 buckets
+    // 1 + 4 r (r - 1) ..), in chunks of 64 buckets, then of kNatLaneB x 64 (lane l takes
+    // buckets l, l + 64, ...): fewer round trips for the cells that reach far rings.  A
+    // chunk may run past the ring where the scan could stop; the extra sites cut nothing
+    // and the filter drops them.
+    const long nbt = 1 + 4L * rmax * (rmax + 1);
+    for (long T0 = 0, cs = kWave; T0 < nbt; T0 += cs, cs = kNatLaneB * kWave) {
+        const int r0 = bucket_ring(T0);
+        if (r0 > 0 && (double)(r0 - 1) * a.h >= 2.0 * rho) break;
+        const int nb_l = cs / kWave;
+        int s0[kNatLaneB], s1[kNatLaneB];
+#pragma unroll
+        for (int j = 0; j < kNatLaneB; ++j) {
+            s0[j] = s1[j] = 0;
+            const long T = T0 + lane + (long)kWave * j;
+            if (j < nb_l && T < nbt) {
+                const int r = bucket_ring(T);
+                int i, jj;
+                ring_bucket(r, (int)(r == 0 ? 0 : T - (1 + 4L * r * (r - 1))), bx, by, i, jj);
+                if (i >= 0 && i < a.BX && jj >= 0 && jj < a.BY) {
+                    s0[j] = a.bstart[jj * a.BX + i];
+                    s1[j] = a.bstart[jj * a.BX + i + 1];
                 }
             }
-            // buckets of up to kNatPre nodes: every load issued at once, kept in registers
-            const bool pre = s1 - s0 <= kNatPre;
-            int pn[kNatPre];
-            double pxs[kNatPre], pys[kNatPre];
-            int c = 0;
-            if (pre) {
-#pragma unroll
-                for (int u = 0; u < kNatPre; ++u) pn[u] = s0 + u < s1 ? a.bnode[s0 + u] : -1;
-#pragma unroll
-                for (int u = 0; u < kNatPre; ++u) {
-                    const int p = pn[u] < 0 ? q : pn[u];
-                    const double2 P = a.xy[p];
-                    const int bd = a.bad[p];
-                    pxs[u] = P.x - Q.x;
-                    pys[u] = P.y - Q.y;
-                    if (p == q || bd || !reaches(pxs[u], pys[u])) pn[u] = -1;
-                    c += pn[u] >= 0;
-                }
-            } else {
-                for (int s = s0; s < s1; ++s) {
-                    const int p = a.bnode[s];
-                    if (p == q || a.bad[p]) continue;
-                    const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
-                    if (reaches(px, py)) ++c;
-                }
-            }
-            int off = c;
-#pragma unroll
-            for (int d = 1; d < kWave; d <<= 1) {   // inclusive scan over lanes
-                const int o = __shfl_up(off, d, kWave);
-                if (lane >= d) off += o;
-            }
-            const int total = __shfl(off, kWave - 1, kWave);
-            if (total > kNatWaveCand) return -2;
-            off -= c;
-            if (pre) {
-#pragma unroll
-                for (int u = 0; u < kNatPre; ++u)
-                    if (pn[u] >= 0) {
-                        L.cand[off] = pn[u];
-                        L.cx[off] = pxs[u];
-                        L.cy[off] = pys[u];
-                        ++off;
-                    }
-            } else {
-                for (int s = s0; s < s1; ++s) {
-                    const int p = a.bnode[s];
-                    if (p == q || a.bad[p]) continue;
-                    const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
-                    if (reaches(px, py)) {
-                        L.cand[off] = p;
-                        L.cx[off] = px;
-                        L.cy[off] = py;
-                        ++off;
-                    }
-                }
-            }
-            wave_sync_lds();
-            for (int ci = 0; ci < total; ++ci) {
-                const int p = L.cand[ci];
-                const double px = L.cx[ci], py = L.cy[ci];
-                if (px * px + py * py >= 4.0 * rho * rho) continue;   // the cell shrank since
-                if (!reaches(px, py)) continue;
-                const Line M{px, py, 0.5 * (px * px + py * py)};
-                const int m0 = m;
-                m = nat_wclip(a, Q, B, L, cur, m, M, p, lane);
-                if (m <= 0) return m == -2 ? -2 : -1;
-                if (m != m0 || L.tag[cur][1] == p) shape();
-            }
-            wave_sync_lds();
         }
+        // the first kNatPre nodes of each bucket with every load in flight, in registers;
+        // nodes past them (dense buckets) by a loop
+        int pn[kNatLaneB][kNatPre];
+        double pxs[kNatLaneB][kNatPre], pys[kNatLaneB][kNatPre];
+#pragma unroll
+        for (int j = 0; j < kNatLaneB; ++j)
+#pragma unroll
+            for (int u = 0; u < kNatPre; ++u) pn[j][u] = s0[j] + u < s1[j] ? a.bnode[s0[j] + u] : -1;
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < kNatLaneB; ++j)
+#pragma unroll
+            for (int u = 0; u < kNatPre; ++u) {
+                const int p = pn[j][u] < 0 ? q : pn[j][u];
+                const double2 P = a.xy[p];
+                const int bd = a.bad[p];
+                pxs[j][u] = P.x - Q.x;
+                pys[j][u] = P.y - Q.y;
+                if (p == q || bd || !cuts(pxs[j][u], pys[j][u])) pn[j][u] = -1;
+                c += pn[j][u] >= 0;
+            }
+        auto rest = [&](auto &&take) {
+            for (int j = 0; j < kNatLaneB; ++j)
+                for (int sx = s0[j] + kNatPre; sx < s1[j]; ++sx) {
+                    const int p = a.bnode[sx];
+                    if (p == q || a.bad[p]) continue;
+                    const double px = a.xy[p].x - Q.x, py = a.xy[p].y - Q.y;
+                    if (cuts(px, py)) take(p, px, py);
+                }
+        };
+        rest([&](int, double, double) { ++c; });
+        int off = c;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {   // inclusive scan over lanes
+            const int o = __shfl_up(off, d, kWave);
+            if (lane >= d) off += o;
+        }
+        const int total = __shfl(off, kWave - 1, kWave);
+        if (total > kNatWaveCand) return -2;
+        off -= c;
+        auto put = [&](int p, double px, double py) {
+            L.cand[off] = p;
+            L.cx[off] = px;
+            L.cy[off] = py;
+            ++off;
+        };
+#pragma unroll
+        for (int j = 0; j < kNatLaneB; ++j)
+#pragma unroll
+            for (int u = 0; u < kNatPre; ++u)
+                if (pn[j][u] >= 0) put(pn[j][u], pxs[j][u], pys[j][u]);
+        rest(put);
+        wave_sync_lds();
+        for (int ci = 0; ci < total; ++ci) {
+            const int p = L.cand[ci];
+            const double px = L.cx[ci], py = L.cy[ci];
+            if (px * px + py * py >= 4.0 * rho * rho) continue;   // the cell shrank since
+            if (!reaches(px, py)) continue;
+            const Line M{px, py, 0.5 * (px * px + py * py)};
+            const int m0 = m;
+            m = nat_wclip(a, Q, B, L, cur, m, M, p, lane);
+            if (m <= 0) return m == -2 ? -2 : -1;
+            if (m != m0 || L.tag[cur][1] == p) shape();
+        }
+        wave_sync_lds();
     }
     // a cell still touching the initial square could not be formed
     bool open = false;
