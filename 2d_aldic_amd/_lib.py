@@ -39,7 +39,8 @@ class FrameOpts(C.Structure):
                 ("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
                 ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
                 ("no_subpb1_fit", C.c_int), ("stage_times", C.c_int), ("smooth", C.c_int),
-                ("beta_sweep", C.c_int), ("admm_tol", C.c_double), ("outlier_thr", C.c_double)]
+                ("beta_sweep", C.c_int), ("admm_tol", C.c_double), ("outlier_thr", C.c_double),
+                ("learned_start", C.c_int)]
 
 
 class QFrameOpts(C.Structure):
@@ -525,7 +526,8 @@ class Context:
 
     def frame_setup(self, ws, step, s, grid, tol=1e-2, mu=1e-3, beta=None, n_admm=3,
                     cg_maxit=2000, cg_rtol=1e-10, subpb1_fit=True, stage_times=True,
-                    smooth=False, beta_sweep=False, admm_tol=0.0, outlier_thr=100.0):
+                    smooth=False, beta_sweep=False, admm_tol=0.0, outlier_thr=100.0,
+                    learned_start=False):
         o = FrameOpts()
         o.ws, o.step = int(ws), int(step)
         o.s[0], o.s[1] = (int(s), int(s)) if np.isscalar(s) else (int(s[0]), int(s[1]))
@@ -543,6 +545,7 @@ class Context:
         o.beta_sweep = 1 if beta_sweep else 0
         o.admm_tol = float(admm_tol)
         o.outlier_thr = float(outlier_thr)
+        o.learned_start = 1 if learned_start else 0
         n = C.c_int(0)
         self._ck(self.L.aldic_frame_setup(self.h, C.byref(o), C.byref(n)), "aldic_frame_setup")
         self.frame_nodes = n.value
@@ -585,9 +588,11 @@ class Context:
     def frame_counters(self):
         """{gridfit iteration budget of the next frame, frames re-run with polled
         gridfits, most gridfit iterations in the last frame, bad-node count of each ADMM
-        step's Subpb1 (-1: step not run)}."""
-        v = np.zeros(11, np.int32)
-        self._ck(self.L.aldic_frame_counters(self.h, _p(v), 11), "aldic_frame_counters")
+        step's Subpb1 (-1: step not run), subsets that fell back from the learned start
+        (-1: the frame ran the search)}."""
+        v = np.zeros(12, np.int32)
+        self._ck(self.L.aldic_frame_counters(self.h, _p(v), 12), "aldic_frame_counters")
         d = dict(zip(("gf_budget", "gf_reruns", "gf_most"), v[:3].tolist()))
-        d["subpb1_bad"] = v[3:].tolist()
+        d["subpb1_bad"] = v[3:11].tolist()
+        d["learned_fallback"] = int(v[11])
         return d

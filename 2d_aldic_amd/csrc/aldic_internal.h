@@ -155,6 +155,14 @@ struct QFrameState {
 // entries, main_ALDIC.m:174; the first is LocalICGN's).
 constexpr int kMaxSteps = 8;
 
+// The learned start's launch plan for a node set (cnn.hip): the lattice path when every
+// node shares one 4-pixel cell lattice.
+struct CnnPlan {
+    bool lattice = false;
+    int px0 = 0, py0 = 0;
+    long nx1 = 0, ny1 = 0;
+};
+
 struct FrameState {
     bool ready = false;
     aldic_frame_opts o{};
@@ -168,6 +176,10 @@ struct FrameState {
     DBuf nat_err;   // refills of the frame whose cell could not be formed (device int)
     DBuf nbad;      // bad-node count of each ADMM step's Subpb1 (device int[kMaxSteps], -1: not run)
     int n_outliers = 0;  // nodes the last searched frame's outlier test flagged
+    CnnPlan plan;        // learned_start: the network's plan for the node grid
+    DBuf P6, F0, lflag;  // learned_start: warps (6S), F0 (4S), fallback flags, list, count
+    int n_fallback = -1; // subsets of the last frame that fell back to the search (-1: none run)
+    int *nfb_h = nullptr;  // pinned copy of the fallback count
     int *nat_err_h = nullptr;  // pinned copy, read after the frame's wait
     int f2cur = 0;  // which of F2 / F2b holds the last global step's F
     double *gfh = nullptr;  // pinned: {converged, iterations, residual} per frame gridfit
@@ -333,11 +345,6 @@ hipError_t cnn_set_weights(CnnWork &c, const float *w_host, hipStream_t st);  //
 hipError_t cnn_get_weights(CnnWork &c, float *w_host, hipStream_t st);        // waits
 // How cnn_forward runs on a node set (host arithmetic on the nodes, 2 per node, 1-based):
 // on one 4-pixel lattice of px0, py0 and nx1 x ny1 E1 cells, or per patch
-struct CnnPlan {
-    bool lattice = false;
-    int px0 = 0, py0 = 0;
-    long nx1 = 0, ny1 = 0;
-};
 CnnPlan cnn_plan(const double *xy_host, int S);
 // P (6 per node: u_x, v_x, u_y, v_y, u, v) at nodes xy (2 per node, 1-based) from f, g
 hipError_t cnn_forward(CnnWork &c, const float *f, const float *g, int nx, int ny,

@@ -282,6 +282,8 @@ void aldic_destroy(aldic_ctx *ctx)
     mg_work_free(ctx->mg);
     ctx->mg = nullptr;
     F.seed.release();
+    for (DBuf *bb : {&F.P6, &F.F0, &F.lflag}) bb->release();
+    if (F.nfb_h) (void)hipHostFree(F.nfb_h);
     ctx->qf.nat_err.release();
     if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
     if (ctx->qf.nat_err_h) (void)hipHostFree(ctx->qf.nat_err_h);
@@ -1548,6 +1550,18 @@ int aldic_frame_setup(aldic_ctx *ctx, const aldic_frame_opts *o, int *n_nodes)
     F.beta_used = o->beta;
     F.admm_steps = 0;
     F.upd = F.upd2 = NAN;
+    F.n_fallback = -1;
+    if (o->learned_start) {
+        // the network's warps, F0, and the fallback list (flags, list, count, run counts)
+        if (int r = cnn_ready(ctx, "frame_setup: learned_start", true)) return r;
+        if (!search_fits(ctx, false, o->ws, o->s[0], o->s[1]))
+            return fail(ctx, ALDIC_ERR_ARG, "frame_setup: fallback search window too large for LDS");
+        F.plan = cnn_plan(F.hcoords.data(), S);
+        CK(F.P6.ensure(6 * sd));
+        CK(F.F0.ensure(4 * sd));
+        CK(F.lflag.ensure(sizeof(int) * (2 * (size_t)S + 2 + ((size_t)S + 4095) / 4096)));
+        if (!F.nfb_h) CK(hipHostMalloc((void **)&F.nfb_h, sizeof(int), hipHostMallocDefault));
+    }
     CK(F.F2b.ensure(sizeof(double) * 4 * S));
     CK(F.tmp.ensure(2 * sd));
     CK(ctx->scratch[5].ensure(2 * sd + 64));
@@ -1628,15 +1642,23 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
     sa.status = reinterpret_cast<int *>(s4 + 4 * (size_t)S);
     sa.U0 = F.U0.as<double>();
     sa.ev_kernel = F.ev_skern;
+    const SearchArgs sa32 = sa;   // the FP32 search (learned start's fallback list)
     search_raw8(ctx, sa);
-    if (!seeded) {
+    // learned start: the network's warp of every node in place of the search
+    const bool learned = o.learned_start && !seeded;
+    if (learned) {
+        CK(cnn_forward(*ctx->cnn, ctx->f.as<float>(), ctx->g.as<float>(), ctx->nx, ctx->ny,
+                       F.coords.as<double>(), F.plan, S, F.P6.as<double>(), st));
+        CK(launch_split_start(F.P6.as<double>(), S, F.U0.as<double>(), F.F0.as<double>(), st));
+        CK(hipEventRecord(F.ev_skern, st));
+    } else if (!seeded) {
         CK(launch_search(sa, st));
     } else {
         // main_ALDIC.m:136-137: the previous frame's U is the initial guess (no search)
         CK(hipEventRecord(F.ev_skern, st));
         CK(hipMemcpyAsync(F.U0.p, F.seed.p, 16 * (size_t)S, hipMemcpyDeviceToDevice, st));
     }
-    if (!seeded && o.outlier_thr > 0) {
+    if (!seeded && !learned && o.outlier_thr > 0) {
         // IntegerSearch.m:68-76 (mask holes NaN), :114 (funRemoveOutliers at Thr0) and
         // Init.m:13: flagged and NaN nodes in-painted by springs in one solve
         if (ctx->have_mask)
@@ -1668,6 +1690,7 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
     a.ws_all = o.ws;
     a.tol = o.tol;
     a.U0 = F.U0.as<double>();
+    a.F0 = learned ? F.F0.as<double>() : nullptr;
     a.U = F.U6.as<double>();
     a.F = F.F6.as<double>();
     a.H21 = F.H21.as<double>();
@@ -1693,6 +1716,41 @@ static int frame_pass(aldic_ctx *ctx, bool gf_poll, bool seeded)
         ar.nlist = S;
         ar.nlist_dev = rl + S;
         CK(launch_icgn6(ar, ppl_big, st));
+    }
+    if (learned) {
+        // fallback: the subsets whose IC-GN missed status 1..20 or ended with ZNCC below
+        // 0.9 are searched (FP32, that list only: its length comes to the host so the
+        // search launches one workgroup per listed subset) and run again from the peak
+        int *flag = F.lflag.as<int>(), *list = flag + S, *cnt = list + S, *runs = cnt + 2;
+        CK(launch_flag_fallback(F.it6.as<int>(), F.zncc.as<double>(), S, 20, 0.9, flag, st));
+        CK(launch_list_eq(flag, S, 1, list, cnt, runs, st));
+        CK(hipMemcpyAsync(F.nfb_h, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        const int nfb = *F.nfb_h;
+        F.n_fallback = nfb;
+        if (nfb > 0) {
+            SearchArgs sl = sa32;
+            sl.ev_kernel = nullptr;
+            sl.list = list;
+            sl.nlist_dev = cnt;
+            CK(launch_search_list(sl, nfb, st));
+            IcgnArgs af = a;
+            af.F0 = nullptr;
+            af.list = list;
+            af.nlist = nfb;
+            CK(launch_icgn6(af, ppl, st));
+            if (relaunch) {
+                IcgnArgs ar2 = af;
+                int *rl = F.rl.as<int>();
+                CK(launch_list_eq(F.it6.as<int>(), S, -2, rl, rl + S, rl + S + 1, st));
+                ar2.list = rl;
+                ar2.nlist = S;
+                ar2.nlist_dev = rl + S;
+                CK(launch_icgn6(ar2, ppl_big, st));
+            }
+        }
+    } else {
+        F.n_fallback = -1;
     }
     const int e_icgn6 = mark();
     span(2, e_search, e_icgn6);
@@ -2211,9 +2269,10 @@ int aldic_qframe_get(aldic_ctx *ctx, double *U, double *F, int *iters6, int *ite
 
 int aldic_frame_counters(aldic_ctx *ctx, int *out, int n)
 {
-    if (!ctx || !out || n < 0 || n > 3 + kMaxSteps) return ALDIC_ERR_ARG;
-    int v[3 + kMaxSteps] = {ctx->fr.gf_budget, ctx->fr.gf_reruns, ctx->fr.gf_most};
+    if (!ctx || !out || n < 0 || n > 4 + kMaxSteps) return ALDIC_ERR_ARG;
+    int v[4 + kMaxSteps] = {ctx->fr.gf_budget, ctx->fr.gf_reruns, ctx->fr.gf_most};
     for (int k = 0; k < kMaxSteps; ++k) v[3 + k] = -1;
+    v[3 + kMaxSteps] = ctx->fr.n_fallback;
     if (n > 3 && ctx->fr.ready && ctx->fr.nbad.p) {
         CK(hipSetDevice(ctx->device));
         CK(hipStreamSynchronize(ctx->stream));

@@ -525,6 +525,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         o.beta_sweep = (int)mxGetScalar(prhs[12]);
         o.admm_tol = mxGetScalar(prhs[13]);
         o.outlier_thr = nrhs > 14 ? mxGetScalar(prhs[14]) : 100.0;
+        o.learned_start = nrhs > 15 ? (int)mxGetScalar(prhs[15]) : 0;
         int S = 0;
         check(aldic_frame_setup(g_ctx, &o, &S), "frame_setup");
         plhs[0] = mxCreateDoubleScalar(S);

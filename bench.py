@@ -200,9 +200,13 @@ def main():
         smooth_ms = (time.perf_counter() - t0) * 1e3 / nsm
         log(f"[bench] smoothing on: {smooth_ms:.3f} ms/frame")
     learned = None
+    frame_learned = None
     if args.workload == 3 and not args.no_learned:
         learned = learned_start(ctx, pkg, pair, grid, ws, step, srad, args.steps)
         log(f"[bench] learned start: {learned}")
+        frame_learned = learned_frame(ctx, pair, out, ms_per_step, ws, step, srad, grid, mu,
+                                      beta, n_admm, args.steps, args.warmup)
+        log(f"[bench] frame with the learned start: {frame_learned}")
     mgs = None
     if not args.no_mg:
         mgs = multigrid_search(ctx, pair, grid, ws, step, args.steps)
@@ -304,7 +308,8 @@ def main():
                    "stage_ms_from": "separate untimed frames with an event at every stage boundary",
                    "gridfit": counters,
                    "icgn6_subsets_per_s": round(conv / t_i, 1),
-                   "learned_start": learned, "multigrid_search": mgs},
+                   "learned_start": learned, "frame_learned_start": frame_learned,
+                   "multigrid_search": mgs},
         "roofline": roof,
         extra_key: extra_roof,
         "cpu_baseline": cpu,
@@ -357,6 +362,42 @@ def learned_start(ctx, pkg, pair, grid, ws, step, srad, reps):
     out["note"] = ("LocalICGN stage only (start + 6-DOF IC-GN over 247,009 subsets); the "
                    "search start is the exact 12-bit integer search")
     return out
+
+
+def learned_frame(ctx, pair, out_s, ms_s, ws, step, srad, grid, mu, beta, n_admm, steps, warmup):
+    """The same device frame with LocalICGN started from the learned warp (frame option
+    learned_start; the search runs only on the fallback list): wall time per frame as the
+    headline's, converged subsets, fallback count, and its final U against the searched
+    frame's and against the analytic field."""
+    ctx.frame_stage(pair.f, pair.g)
+    ctx.frame_setup(ws, step, srad, grid, tol=1e-2, mu=mu, beta=beta, n_admm=n_admm,
+                    cg_maxit=2000, cg_rtol=1e-10, stage_times=False, learned_start=True)
+    for _ in range(max(1, warmup)):
+        ctx.frame_run()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.frame_run()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    r = ctx.frame_get()
+    fb = ctx.frame_counters()["learned_fallback"]
+    it6 = r["iters6"]
+    conv = int(np.sum((it6 >= 1) & (it6 <= 100)))
+    S = it6.size
+    x1, y1, Nx, Ny = ctx.search_grid(grid, ws, step, [srad, srad])
+    X, Y = np.meshgrid(x1 + np.arange(Nx) * step + 0.5 * ws, y1 + np.arange(Ny) * step + 0.5 * ws)
+    xy = np.column_stack([X.ravel(), Y.ravel()])
+    d = (r["U"] - out_s["U"]).reshape(-1, 2)
+    res = dict(frame_ms=round(ms, 4), converged=conv, converged_subsets_per_s=round(conv / (ms / 1e3), 1),
+               fallback=fb, vs_searched_frame_rms_px=float(np.sqrt(np.mean(np.sum(d ** 2, 1)))),
+               vs_searched_frame_max_px=float(np.abs(d).max()),
+               speedup_vs_searched_frame=round(ms_s / ms, 3))
+    ua, va = pair.ufun(xy[:, 0], xy[:, 1])
+    for name, UU in (("rms_vs_field_px", r["U"]), ("searched_frame_rms_vs_field_px", out_s["U"])):
+        U = UU.reshape(-1, 2)
+        res[name] = float(np.sqrt(np.mean((U[:, 0] - ua) ** 2 + (U[:, 1] - va) ** 2)))
+    res["note"] = (f"{S} subsets; the search runs only for the fallback list; outlier stage off "
+                   "(no search field to test)")
+    return res
 
 
 def multigrid_search(ctx, pair, grid, ws, step, reps):

@@ -299,6 +299,13 @@ typedef struct {
                         * are in-painted by springs (inpaint_nans method 4; the same solve as
                         * funRemoveOutliers' followed by Init.m:13's) before LocalICGN;
                         * 0: the search's field goes to LocalICGN as it is */
+    int learned_start; /* 1: the learned initial guess (aldic_cnn_*, weights set) starts
+                        * LocalICGN's 6-DOF IC-GN from its full affine warp in place of the
+                        * integer search; subsets whose IC-GN misses status 1..20 or ends
+                        * with ZNCC below 0.9 are searched (FP32 search on that list) and
+                        * run again from the search's peak, so converged IC-GN still decides
+                        * every displacement; the search's outlier stage does not run.
+                        * Not for aldic_frame_run_seeded, which starts from the caller's U. */
 } aldic_frame_opts;
 
 int aldic_frame_stage(aldic_ctx *ctx, const void *f, const void *g, int nx, int ny, int dtype,
@@ -328,6 +335,8 @@ int aldic_frame_times(aldic_ctx *ctx, double *ms, int n);
  * LocalICGN's count (from iters6, LocalICGN.m:117-122) is ALSub1BadPtNum
  * (main_ALDIC.m:180,365,456). n <= 11. */
 int aldic_frame_counters(aldic_ctx *ctx, int *out, int n);
+/* (out[11], n = 12: subsets of the last frame that fell back from the learned start to
+ * the search, -1 when the frame did not use the learned start.) */
 /* Scalars of the last frame (no counterpart in the reference's outputs; main_ALDIC.m
  * prints them): {beta used, ADMM steps run, RMS change of U2 in the last step, RMS change
  * of Subpb1's U in the last step (both measured only with admm_tol > 0, else NaN)}.

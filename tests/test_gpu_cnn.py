@@ -188,3 +188,40 @@ def test_local_icgn_learned_start_with_shipped_weights(gpu_ctx, speckle):
     same = (rf["iters"] == r0["iters"])
     assert same.mean() > 0.99
     assert np.max(np.abs(rf["U"] - r0["U"]).reshape(-1, 2)[same & ok0]) < 1e-3
+
+
+def test_frame_with_learned_start(gpu_ctx, speckle):
+    """The device frame (aldic_frame_run) with learned_start: LocalICGN starts from the
+    shipped network's warp instead of the integer search (FP32 search and IC-GN again on
+    the subsets that miss status 1..20 or ZNCC 0.9), then the same ADMM steps.  On the
+    12-bit 1024^2 pair it converges the same subsets as the searched frame, LocalICGN's U
+    agrees to IC-GN's stop tolerance (1e-3 here) and the final U and F to about that."""
+    import importlib
+    cnn = importlib.import_module("2d_aldic_amd.cnn")
+    p = speckle.config3(seed=3, n=1024, n_dots=37_500)
+    gpu_ctx.set_images(p.f, p.g)
+    cnn.load_weights(gpu_ctx)
+    gpu_ctx.frame_stage(p.f, p.g)
+    grid, ws, step, s = [1, 1024, 1, 1024], 32, 8, [24, 24]
+    res = {}
+    for learned in (False, True):
+        _flush(f"frame learned_start={learned}")
+        gpu_ctx.frame_setup(ws, step, s, grid, tol=1e-3, learned_start=learned)
+        gpu_ctx.frame_run()
+        res[learned] = gpu_ctx.frame_get()
+        res[learned]["fallback"] = gpu_ctx.frame_counters()["learned_fallback"]
+    r0, r1 = res[False], res[True]
+    assert r0["fallback"] == -1 and 0 <= r1["fallback"] <= 0.02 * r0["iters6"].size
+    ok0 = (r0["iters6"] >= 1) & (r0["iters6"] <= 100)
+    ok1 = (r1["iters6"] >= 1) & (r1["iters6"] <= 100)
+    both = ok0 & ok1
+    d6 = np.abs(r1["U6"] - r0["U6"]).reshape(-1, 2)[both]
+    dU = np.abs(r1["U"] - r0["U"]).reshape(-1, 2)
+    dF = np.abs(r1["F"] - r0["F"])
+    _flush(f"converged {ok0.mean():.4f} / {ok1.mean():.4f}, fallback {r1['fallback']}, "
+           f"max |dU6| {d6.max():.2e}, max |dU| {dU.max():.2e}, rms {np.sqrt(np.mean(dU ** 2)):.2e}, "
+           f"max |dF| {dF.max():.2e}")
+    assert ok1.mean() >= ok0.mean() - 1e-3
+    assert d6.max() < 5e-3
+    assert dU.max() < 2e-2 and np.sqrt(np.mean(dU ** 2)) < 2e-3
+    assert dF.max() < 1e-3
