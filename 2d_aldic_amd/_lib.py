@@ -30,7 +30,7 @@ EXPORTS = (
     "aldic_remove_outliers", "aldic_icgn6_start", "aldic_cnn_nparams", "aldic_cnn_set_weights",
     "aldic_cnn_get_weights", "aldic_cnn_predict", "aldic_cnn_add_samples",
     "aldic_cnn_clear_samples", "aldic_cnn_train", "aldic_cnn_grad", "aldic_local_icgn",
-    "aldic_integer_search_mg",
+    "aldic_integer_search_mg", "aldic_smooth_strain_quadtree",
 )
 
 
@@ -72,6 +72,7 @@ def load():
     L.aldic_get_images.argtypes = [P, vp, vp, vp, vp]
     L.aldic_search_grid.argtypes = [ip, ip, vp, ip, ip, vp, vp]
     L.aldic_integer_search.argtypes = [P, vp, ip, ip, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.aldic_smooth_strain_quadtree.argtypes = [P, ip, vp, dp, dp, dp, ip, vp, vp]
     L.aldic_integer_search_mg.argtypes = [P, vp, ip, ip, C.c_long, vp, vp, vp, vp, vp, vp,
                                           C.c_long, vp]
     L.aldic_icgn6.argtypes = [P, ip, vp, vp, vp, ip, dp, vp, vp, vp, vp, vp]
@@ -211,6 +212,19 @@ class Context:
         x, y, u, v, cm = [o.reshape(Ny, Nx) for o in outs]
         return dict(x=x, y=y, u=u, v=v, ccmax=cm, qf=qf, status=st.reshape(Ny, Nx), x1=x1,
                     y1=y1, Nx=Nx, Ny=Ny)
+
+    def smooth_strain_quadtree(self, coords, F, h, smoothness=1e-4, filter_std=0.5, filter_size=0):
+        """funSmoothStrainQuadtree(F, DICmesh, DICpara) on the device: (smoothed F, CG
+        iterations of the two passes)."""
+        coords = _c(coords, np.float64).reshape(-1, 2)
+        n = coords.shape[0]
+        F = _c(F, np.float64).copy()
+        assert F.size == 4 * n
+        it = np.zeros(2, np.int32)
+        self._ck(self.L.aldic_smooth_strain_quadtree(self.h, n, _p(coords), float(h), float(smoothness),
+                                                     float(filter_std), int(filter_size), _p(F), _p(it)),
+                 "aldic_smooth_strain_quadtree")
+        return F, it
 
     def integer_search_mg(self, grid, ws, step):
         """funIntegerSearchMg over grid = (x1, x2, y1, y2): dict(x, y, u, v, ccmax) on the

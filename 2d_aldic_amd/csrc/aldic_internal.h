@@ -125,6 +125,10 @@ hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const doub
                              int np, const double *px, const double *py, const double2 *pd, int npair,
                              double *U, double rtol, int maxit, DBuf *work, double *stats_dev,
                              hipStream_t st);
+// funSmoothStrainQuadtree's two passes on F (4 per node, device, in place): nn nodes on
+// the nx x ny grid at grid indices gidx (device); w[0..1] work; iters[2] CG iterations
+hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double smoothness, double sigma, double rtol,
+                            int maxit, double *F, DBuf *w, int *iters, hipStream_t st);
 // nanconv(., fspecial('gaussian', 3, sigma), 'edge') of nc values per node on the gx x gy
 // node grid (out-of-place)
 hipError_t launch_blur3(const double *in, double *out, int gx, int gy, int nc, double sigma,
@@ -223,6 +227,7 @@ struct aldic_ctx {
     aldic::DBuf boxp;                // window box-sum planes of the exact search
     aldic::DBuf scratch[9];          // per-call work buffers (8: IC-GN start gradients)
     aldic::DBuf lic[8];              // aldic_local_icgn work buffers (kept between calls)
+    aldic::DBuf smq[4];              // aldic_smooth_strain_quadtree work buffers
     aldic::MeshDev mesh;
     bool have_mesh = false;
     aldic::QuadDev quad;             // aldic_set_quadtree / aldic_subpb2_quadtree

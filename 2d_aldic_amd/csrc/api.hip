@@ -281,6 +281,7 @@ void aldic_destroy(aldic_ctx *ctx)
     ctx->cnn = nullptr;
     mg_work_free(ctx->mg);
     ctx->mg = nullptr;
+    for (DBuf &bb : ctx->smq) bb.release();
     F.seed.release();
     for (DBuf *bb : {&F.P6, &F.F0, &F.lflag}) bb->release();
     if (F.nfb_h) (void)hipHostFree(F.nfb_h);
@@ -465,6 +466,52 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
     if (qf) CK(down(qf, a.qf, 2 * (size_t)S, ctx->stream));
     CK(down(status, a.status, S, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
+    return ALDIC_OK;
+}
+
+int aldic_smooth_strain_quadtree(aldic_ctx *ctx, int nn, const double *coords, double h, double smoothness,
+                                 double filter_std, int filter_size, double *F, int *iters)
+{
+    if (!ctx) return ALDIC_ERR_ARG;
+    if (nn < 1 || !coords || !F || !(h > 0) || !(smoothness > 0) || filter_std < 0 || filter_size < 0)
+        return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: bad arguments");
+    // funSmoothStrainQuadtree.m:40-61: std 0 -> 0.5, size 0 -> 2 ceil(2 std) + 1
+    const double sd = filter_std == 0 ? 0.5 : filter_std;
+    const int fs = filter_size == 0 ? 2 * (int)std::ceil(2.0 * sd) + 1 : filter_size;
+    if (fs != 3) return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: only the 3 x 3 filter is built");
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int i = 0; i < nn; ++i) {
+        x0 = std::min(x0, coords[2 * i]); x1 = std::max(x1, coords[2 * i]);
+        y0 = std::min(y0, coords[2 * i + 1]); y1 = std::max(y1, coords[2 * i + 1]);
+    }
+    if (!std::isfinite(x0 + x1 + y0 + y1)) return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: bad coordinates");
+    // the grid min:h:max of each axis; every node must sit on it (the reference's find)
+    const long nxl = (long)std::floor((x1 - x0) / h + 1e-9) + 1, nyl = (long)std::floor((y1 - y0) / h + 1e-9) + 1;
+    if (nxl < 3 || nyl < 3 || nxl * nyl > (1L << 26))
+        return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: grid must be 3..2^26 points with >= 3 per axis");
+    const int nx = (int)nxl, ny = (int)nyl;
+    std::vector<int> gidx(nn);
+    for (int i = 0; i < nn; ++i) {
+        const long ix = std::lround((coords[2 * i] - x0) / h), iy = std::lround((coords[2 * i + 1] - y0) / h);
+        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || x0 + ix * h != coords[2 * i] ||
+            y0 + iy * h != coords[2 * i + 1])
+            return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: a node is off the grid min:h:max");
+        gidx[i] = (int)(iy * nx + ix);
+    }
+    CK(hipSetDevice(ctx->device));
+    DBuf *w = ctx->smq;
+    CK(w[2].ensure(sizeof(int) * (size_t)nn));
+    CK(w[3].ensure(sizeof(double) * 4 * (size_t)nn));
+    CK(hipMemcpyAsync(w[2].p, gidx.data(), sizeof(int) * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(w[3].p, F, sizeof(double) * 4 * (size_t)nn, hipMemcpyHostToDevice, ctx->stream));
+    int it[2] = {0, 0};
+    const hipError_t e = smooth_strain_qt(nn, w[2].as<int>(), nx, ny, smoothness, sd, 1e-12, 20000,
+                                          w[3].as<double>(), w, it, ctx->stream);
+    if (e == hipErrorUnknown) return fail(ctx, ALDIC_ERR_NUMERIC, "smooth_strain_quadtree: non-finite residual");
+    CK(e);
+    CK(down(F, w[3].as<double>(), 4 * (size_t)nn, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (iters) { iters[0] = it[0]; iters[1] = it[1]; }
     return ALDIC_OK;
 }
 

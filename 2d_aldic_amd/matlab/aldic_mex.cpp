@@ -471,6 +471,17 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         check(aldic_natural_fill(g_ctx, (int)nn, xy.data(), bad.data(), mxGetDoubles(plhs[0]),
                                  withF ? mxGetDoubles(plhs[1]) : nullptr),
               "natural_fill");
+    } else if (cmd == "smooth_strain_quadtree") {
+        // F = aldic_mex('smooth_strain_quadtree', F, coords, h, smoothness, filter_std, filter_size)
+        need(nrhs, 7, cmd.c_str());
+        const std::vector<double> xy = coords2(prhs[2]);
+        const size_t nn = xy.size() / 2;
+        if (numel(prhs[1]) != 4 * nn) mexErrMsgIdAndTxt("aldic:args", "F must hold 4 values per node");
+        plhs[0] = mxDuplicateArray(prhs[1]);
+        check(aldic_smooth_strain_quadtree(g_ctx, (int)nn, xy.data(), mxGetScalar(prhs[3]), mxGetScalar(prhs[4]),
+                                           mxGetScalar(prhs[5]), (int)mxGetScalar(prhs[6]),
+                                           mxGetDoubles(plhs[0]), nullptr),
+              "smooth_strain_quadtree");
     } else if (cmd == "bicubic") {  // Z = aldic_mex('bicubic', X, Y): ba_interp2 of the deformed image
         need(nrhs, 3, cmd.c_str());
         const size_t n = numel(prhs[1]);

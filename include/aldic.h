@@ -105,6 +105,18 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
                          double *x, double *y, double *u, double *v, double *ccmax, double *qf,
                          int *status);
 
+/* funSmoothStrainQuadtree.m:27-104 (called by main_ALDIC_Quadtree.m:172,221,283 with
+ * StrainSmoothness 1e-4): two passes of regularizeNd (regularizeNd.m, 'linear', 'normal')
+ * of each F component over the grid min:h:max (h = DICmesh.elementMinSize), the
+ * nanconv(., fspecial('gaussian', size, std), 'edge', 'nanout') blur and the read-back at
+ * the nodes.  nn nodes (nn x 2 coordinates, every one on that grid), F (4 nn, the
+ * reference's interleaving) smoothed in place.  filter_std 0 -> 0.5 and filter_size 0 ->
+ * 2 ceil(2 std) + 1 as the reference's defaults; only the 3 x 3 filter is built.  The
+ * normal equations are solved by FP64 Jacobi-CG to a relative residual of 1e-12;
+ * iters (nullable, 2) = its iterations in each pass. */
+int aldic_smooth_strain_quadtree(aldic_ctx *ctx, int nn, const double *coords, double h, double smoothness,
+                                 double filter_std, int filter_size, double *F, int *iters);
+
 /* funIntegerSearchMg.m:3-285 (IntegerSearch.m:120, the multigrid search) on the staged
  * images, grid = {x1, x2, y1, y2} (gridxROIRange, gridyROIRange), winsize ws, winstepsize
  * step.  Level 1 correlates the whole ROI (normxcorr2 in full, FFT); each later level
