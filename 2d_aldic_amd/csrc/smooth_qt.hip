@@ -134,7 +134,8 @@ struct SqVec {
     int n, nb;
 };
 
-// r = b, z = M^-1 r, p = z, x = 0; partials of r.z and b.b
+// from the starting x (zero in the first pass, the first pass's solution in the second):
+// r = b - N x, z = M^-1 r, p = z; partials of r.z and b.b
 __global__ __launch_bounds__(kSqT) void k_sq_init(SqOp o, SqVec v)
 {
     double rz = 0.0, bb = 0.0;
@@ -143,13 +144,13 @@ __global__ __launch_bounds__(kSqT) void k_sq_init(SqOp o, SqVec v)
         const double dg = o.D[g] + o.cx * ltl_diag(ix, o.nx) + o.cy * ltl_diag(iy, o.ny);
         const double di = 1.0 / dg;
         v.dinv[g] = di;
-        const double4 b = v.b[g];
-        const double4 z = make_double4(b.x * di, b.y * di, b.z * di, b.w * di);
-        v.x[g] = make_double4(0, 0, 0, 0);
-        v.r[g] = b;
+        const double4 b = v.b[g], nx = sq_apply(o, v.x, g);
+        const double4 r = make_double4(b.x - nx.x, b.y - nx.y, b.z - nx.z, b.w - nx.w);
+        const double4 z = make_double4(r.x * di, r.y * di, r.z * di, r.w * di);
+        v.r[g] = r;
         v.z[g] = z;
         v.p[g] = z;
-        rz += dot4(b, z);
+        rz += dot4(r, z);
         bb += dot4(b, b);
     }
     sq_block(rz, bb, 0.0, v.prz[0], v.pbb, nullptr);
@@ -253,6 +254,7 @@ hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double 
     v.n = n;
     v.nb = nb;
     double hs[2];
+    if ((e = hipMemsetAsync(v.x, 0, v4, st))) return e;
     for (int pass = 0; pass < 2; ++pass) {
         if ((e = hipMemsetAsync(D, 0, sizeof(double) * n, st)) || (e = hipMemsetAsync(bv, 0, v4, st))) return e;
         hipLaunchKernelGGL(k_sq_scatter, dim3((nn + 255) / 256), dim3(256), 0, st, nn, gidx_dev,
