@@ -10,9 +10,14 @@
 // (1, -2, 1) along d at the interior points and s_d = smoothness sqrt(n_nodes / n_eq_d)
 // (n_d - 1)^2 (regularizeNd.m:666-672 with secondDerivativeWeights on a uniform grid:
 // (range_d / h)^2 = (n_d - 1)^2).  The operator is applied matrix-free (a 13-point
-// stencil) and the four components are solved together by Jacobi-preconditioned CG in
-// FP64 (one shared recurrence over the stacked vector).
+// stencil) and the four components are solved together by preconditioned CG in FP64 (one
+// shared recurrence over the stacked vector).  The preconditioner is the operator with D
+// replaced by its mean d: M = d I + s_x^2 (T_x (+) I) + s_y^2 (I (+) T_y), T = L^T L,
+// inverted exactly by fast diagonalisation (T's eigenvectors from a host Jacobi
+// eigensolver, cached per grid size; four 16 x 16-tiled FP64 products per application).
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "aldic_internal.h"
@@ -50,17 +55,6 @@ __device__ __forceinline__ double4 ltl(int j, int n, V v)
         y.w += w * (l.w - 2.0 * m.w + r.w);
     }
     return y;
-}
-
-__device__ __forceinline__ double ltl_diag(int j, int n)
-{
-    double d = 0.0;
-    const double c2[3] = {1.0, 4.0, 1.0};
-    for (int a = -1; a <= 1; ++a) {
-        const int i = j + a;
-        if (i >= 1 && i <= n - 2) d += c2[j - i + 1];
-    }
-    return d;
 }
 
 __device__ __forceinline__ double4 sq_apply(const SqOp &o, const double4 *__restrict__ x, int g)
@@ -128,32 +122,22 @@ __global__ void k_sq_scatter(int nn, const int *__restrict__ gidx, const double4
 struct SqVec {
     double4 *x, *r, *z, *p, *q;
     const double4 *b;
-    double *dinv;
     double *prz[2], *ppq, *prr, *pbb;   // per-block partials
     double *stat;                        // {rr, bb} after the last update
     int n, nb;
 };
 
 // from the starting x (zero in the first pass, the first pass's solution in the second):
-// r = b - N x, z = M^-1 r, p = z; partials of r.z and b.b
+// r = b - N x; partials of b.b
 __global__ __launch_bounds__(kSqT) void k_sq_init(SqOp o, SqVec v)
 {
-    double rz = 0.0, bb = 0.0;
+    double bb = 0.0;
     for (int g = blockIdx.x * kSqT + threadIdx.x; g < v.n; g += gridDim.x * kSqT) {
-        const int ix = g % o.nx, iy = g / o.nx;
-        const double dg = o.D[g] + o.cx * ltl_diag(ix, o.nx) + o.cy * ltl_diag(iy, o.ny);
-        const double di = 1.0 / dg;
-        v.dinv[g] = di;
         const double4 b = v.b[g], nx = sq_apply(o, v.x, g);
-        const double4 r = make_double4(b.x - nx.x, b.y - nx.y, b.z - nx.z, b.w - nx.w);
-        const double4 z = make_double4(r.x * di, r.y * di, r.z * di, r.w * di);
-        v.r[g] = r;
-        v.z[g] = z;
-        v.p[g] = z;
-        rz += dot4(r, z);
+        v.r[g] = make_double4(b.x - nx.x, b.y - nx.y, b.z - nx.z, b.w - nx.w);
         bb += dot4(b, b);
     }
-    sq_block(rz, bb, 0.0, v.prz[0], v.pbb, nullptr);
+    sq_block(bb, 0.0, 0.0, v.pbb, nullptr, nullptr);
 }
 
 // q = N p, partials of p.q
@@ -168,26 +152,96 @@ __global__ __launch_bounds__(kSqT) void k_sq_apply(SqOp o, SqVec v)
     sq_block(pq, 0.0, 0.0, v.ppq, nullptr, nullptr);
 }
 
-// alpha = r.z / p.q; x += alpha p, r -= alpha q, z = M^-1 r; partials of the new r.z, r.r
+// alpha = r.z / p.q; x += alpha p, r -= alpha q; partials of r.r
 __global__ __launch_bounds__(kSqT) void k_sq_update(SqVec v, int par)
 {
     const double rz = sq_total(v.prz[par], v.nb), pq = sq_total(v.ppq, v.nb);
     const double alpha = (pq > 0.0 && isfinite(pq)) ? rz / pq : 0.0;
-    double rzn = 0.0, rr = 0.0;
+    double rr = 0.0;
     for (int g = blockIdx.x * kSqT + threadIdx.x; g < v.n; g += gridDim.x * kSqT) {
         const double4 p = v.p[g], q = v.q[g];
         double4 x = v.x[g], r = v.r[g];
         x.x += alpha * p.x; x.y += alpha * p.y; x.z += alpha * p.z; x.w += alpha * p.w;
         r.x -= alpha * q.x; r.y -= alpha * q.y; r.z -= alpha * q.z; r.w -= alpha * q.w;
-        const double di = v.dinv[g];
-        const double4 z = make_double4(r.x * di, r.y * di, r.z * di, r.w * di);
         v.x[g] = x;
         v.r[g] = r;
-        v.z[g] = z;
-        rzn += dot4(r, z);
         rr += dot4(r, r);
     }
-    sq_block(rzn, rr, 0.0, v.prz[par ^ 1], v.prr, nullptr);
+    sq_block(rr, 0.0, 0.0, v.prr, nullptr, nullptr);
+}
+
+// partials of r.z (z = M^-1 r, just formed) into prz[par]; INIT: p = z
+template <bool INIT>
+__global__ __launch_bounds__(kSqT) void k_sq_rz(SqVec v, int par)
+{
+    double rz = 0.0;
+    for (int g = blockIdx.x * kSqT + threadIdx.x; g < v.n; g += gridDim.x * kSqT) {
+        const double4 z = v.z[g];
+        rz += dot4(v.r[g], z);
+        if (INIT) v.p[g] = z;
+    }
+    sq_block(rz, 0.0, 0.0, v.prz[par], nullptr, nullptr);
+}
+
+// Fast-diagonalisation products on the ny x nx grid of double4 values (x fastest), 16 x 16
+// tiles through LDS.  Right: Y[r][c] = sum_k X[r][k] V'(k, c), V' = V or V^T (V n x n,
+// row-major), X rows x n.  Left: Y[r][c] = sum_k V'(r, k) X[k][c], X n x cols; SCALE
+// divides by d + cx lx[c] + cy ly[r] (the eigenvalues of M).
+template <bool TR>
+__global__ __launch_bounds__(256) void k_fd_right(const double4 *__restrict__ X, const double *__restrict__ V,
+                                                  double4 *__restrict__ Y, int rows, int n)
+{
+    __shared__ double4 xs[16][17];
+    __shared__ double vs[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int r = blockIdx.y * 16 + ty, c = blockIdx.x * 16 + tx;
+    double4 acc = make_double4(0, 0, 0, 0);
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        xs[ty][tx] = (r < rows && k0 + tx < n) ? X[(size_t)r * n + k0 + tx] : make_double4(0, 0, 0, 0);
+        const int kk = k0 + ty, cc = blockIdx.x * 16 + tx;
+        vs[ty][tx] = (kk < n && cc < n) ? (TR ? V[(size_t)cc * n + kk] : V[(size_t)kk * n + cc]) : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double4 a = xs[ty][j];
+            const double w = vs[j][tx];
+            acc.x += a.x * w; acc.y += a.y * w; acc.z += a.z * w; acc.w += a.w * w;
+        }
+        __syncthreads();
+    }
+    if (r < rows && c < n) Y[(size_t)r * n + c] = acc;
+}
+
+template <bool TR, bool SCALE>
+__global__ __launch_bounds__(256) void k_fd_left(const double *__restrict__ V, const double4 *__restrict__ X,
+                                                 double4 *__restrict__ Y, int n, int cols, const double *lx,
+                                                 const double *ly, double d, double cx, double cy)
+{
+    __shared__ double vs[16][17];
+    __shared__ double4 xs[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int r = blockIdx.y * 16 + ty, c = blockIdx.x * 16 + tx;
+    double4 acc = make_double4(0, 0, 0, 0);
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        const int k = k0 + tx;
+        vs[ty][tx] = (r < n && k < n) ? (TR ? V[(size_t)k * n + r] : V[(size_t)r * n + k]) : 0.0;
+        xs[ty][tx] = (k0 + ty < n && c < cols) ? X[(size_t)(k0 + ty) * cols + c] : make_double4(0, 0, 0, 0);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double w = vs[ty][j];
+            const double4 a = xs[j][tx];
+            acc.x += w * a.x; acc.y += w * a.y; acc.z += w * a.z; acc.w += w * a.w;
+        }
+        __syncthreads();
+    }
+    if (r < n && c < cols) {
+        if (SCALE) {
+            const double iv = 1.0 / (d + cx * lx[c] + cy * ly[r]);
+            acc.x *= iv; acc.y *= iv; acc.z *= iv; acc.w *= iv;
+        }
+        Y[(size_t)r * cols + c] = acc;
+    }
 }
 
 // beta = r.z (new) / r.z (old); p = z + beta p; block 0 records {r.r, b.b}
@@ -216,6 +270,64 @@ __global__ void k_sq_gather(int nn, const int *__restrict__ gidx, const double4 
     if (i < nn) F[i] = G[gidx[i]];
 }
 
+// T = L^T L for n points (L: second differences (1, -2, 1) at the interior points) and
+// its eigen-decomposition by cyclic Jacobi rotations: V (row-major, column j the
+// eigenvector of lam[j]).  Cached per n (host).
+struct FdAxis {
+    std::vector<double> V, lam;
+};
+
+const FdAxis &fd_axis(int n)
+{
+    static std::mutex mu;
+    static std::map<int, FdAxis> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
+    std::vector<double> A((size_t)n * n, 0.0), V((size_t)n * n, 0.0);
+    const double c[3] = {1.0, -2.0, 1.0};
+    for (int i = 1; i + 1 < n; ++i)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) A[(size_t)(i - 1 + a) * n + (i - 1 + b)] += c[a] * c[b];
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    double norm = 0.0;
+    for (double x : A) norm += x * x;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += A[(size_t)p * n + q] * A[(size_t)p * n + q];
+        if (off <= 1e-30 * norm) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[(size_t)p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double th = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < n; ++k) {   // columns p, q
+                    const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = cs * akp - sn * akq;
+                    A[(size_t)k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; ++k) {   // rows p, q
+                    const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = cs * apk - sn * aqk;
+                    A[(size_t)q * n + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = cs * vkp - sn * vkq;
+                    V[(size_t)k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    FdAxis f;
+    f.V = std::move(V);
+    f.lam.resize(n);
+    for (int i = 0; i < n; ++i) f.lam[i] = std::max(A[(size_t)i * n + i], 0.0);
+    return cache.emplace(n, std::move(f)).first->second;
+}
+
 }  // namespace
 
 // F (4 nn, in/out, on the device): one call = both passes.  iters[0..1] = CG iterations.
@@ -226,11 +338,20 @@ hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double 
     hipError_t e;
     const int nb = std::min(kSqMaxB, (n + kSqT - 1) / kSqT);
     const size_t v4 = sizeof(double4) * (size_t)n;
-    if ((e = w[0].ensure(7 * v4)) || (e = w[1].ensure(sizeof(double) * (2 * (size_t)n + 6 * kSqMaxB + 4))))
+    const FdAxis &ax = fd_axis(nx), &ay = fd_axis(ny);
+    const size_t nfd = (size_t)nx * nx + (size_t)ny * ny + nx + ny;
+    if ((e = w[0].ensure(7 * v4)) ||
+        (e = w[1].ensure(sizeof(double) * ((size_t)n + 6 * kSqMaxB + 4 + nfd))))
         return e;
     double4 *base = w[0].as<double4>();
     double4 *bv = base + 5 * (size_t)n, *blur = base + 6 * (size_t)n;
-    double *D = w[1].as<double>(), *dinv = D + n, *parts = dinv + n;
+    double *D = w[1].as<double>(), *parts = D + n, *Vx = parts + 6 * kSqMaxB + 4;
+    double *Vy = Vx + (size_t)nx * nx, *lx = Vy + (size_t)ny * ny, *ly = lx + nx;
+    if ((e = hipMemcpyAsync(Vx, ax.V.data(), sizeof(double) * ax.V.size(), hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(Vy, ay.V.data(), sizeof(double) * ay.V.size(), hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(lx, ax.lam.data(), sizeof(double) * nx, hipMemcpyHostToDevice, st)) ||
+        (e = hipMemcpyAsync(ly, ay.lam.data(), sizeof(double) * ny, hipMemcpyHostToDevice, st)))
+        return e;
     SqOp o{nx, ny, 0.0, 0.0, D};
     const double neq_x = (double)(nx - 2) * ny, neq_y = (double)(ny - 2) * nx;
     const double sx = smoothness * std::sqrt(nn / neq_x) * (double)(nx - 1) * (nx - 1);
@@ -244,7 +365,6 @@ hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double 
     v.p = base + 3 * (size_t)n;
     v.q = base + 4 * (size_t)n;
     v.b = bv;
-    v.dinv = dinv;
     v.prz[0] = parts;
     v.prz[1] = parts + kSqMaxB;
     v.ppq = parts + 2 * kSqMaxB;
@@ -253,6 +373,20 @@ hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double 
     v.stat = parts + 5 * kSqMaxB;
     v.n = n;
     v.nb = nb;
+    // z = M^-1 r: R Vx, Vy^T (.) scaled, Vy (.), (.) Vx^T; q and blur as temporaries
+    const double dmean = (double)nn / (double)n;
+    const dim3 gfd((nx + 15) / 16, (ny + 15) / 16);
+    auto precond = [&]() {
+        hipLaunchKernelGGL(k_fd_right<false>, gfd, dim3(256), 0, st, (const double4 *)v.r, (const double *)Vx, v.q,
+                           ny, nx);
+        hipLaunchKernelGGL((k_fd_left<true, true>), gfd, dim3(256), 0, st, (const double *)Vy, (const double4 *)v.q,
+                           blur, ny, nx, (const double *)lx, (const double *)ly, dmean, o.cx, o.cy);
+        hipLaunchKernelGGL((k_fd_left<false, false>), gfd, dim3(256), 0, st, (const double *)Vy,
+                           (const double4 *)blur, v.q, ny, nx, (const double *)lx, (const double *)ly, dmean, o.cx,
+                           o.cy);
+        hipLaunchKernelGGL(k_fd_right<true>, gfd, dim3(256), 0, st, (const double4 *)v.q, (const double *)Vx, v.z,
+                           ny, nx);
+    };
     double hs[2];
     if ((e = hipMemsetAsync(v.x, 0, v4, st))) return e;
     for (int pass = 0; pass < 2; ++pass) {
@@ -260,13 +394,17 @@ hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double 
         hipLaunchKernelGGL(k_sq_scatter, dim3((nn + 255) / 256), dim3(256), 0, st, nn, gidx_dev,
                            (const double4 *)F, D, bv);
         hipLaunchKernelGGL(k_sq_init, dim3(nb), dim3(kSqT), 0, st, o, v);
+        precond();
+        hipLaunchKernelGGL(k_sq_rz<true>, dim3(nb), dim3(kSqT), 0, st, v, 0);
         int it = 0;
         bool done = false;
         while (!done && it < maxit) {
-            const int chunk = std::min(16, maxit - it);
+            const int chunk = std::min(8, maxit - it);
             for (int k = 0; k < chunk; ++k, ++it) {
                 hipLaunchKernelGGL(k_sq_apply, dim3(nb), dim3(kSqT), 0, st, o, v);
                 hipLaunchKernelGGL(k_sq_update, dim3(nb), dim3(kSqT), 0, st, v, it & 1);
+                precond();
+                hipLaunchKernelGGL(k_sq_rz<false>, dim3(nb), dim3(kSqT), 0, st, v, (it & 1) ^ 1);
                 hipLaunchKernelGGL(k_sq_dir, dim3(nb), dim3(kSqT), 0, st, v, it & 1);
             }
             if ((e = hipGetLastError())) return e;

@@ -112,8 +112,8 @@ int aldic_integer_search(aldic_ctx *ctx, const int *grid, int ws, int step, cons
  * the nodes.  nn nodes (nn x 2 coordinates, every one on that grid), F (4 nn, the
  * reference's interleaving) smoothed in place.  filter_std 0 -> 0.5 and filter_size 0 ->
  * 2 ceil(2 std) + 1 as the reference's defaults; only the 3 x 3 filter is built.  The
- * normal equations are solved by FP64 Jacobi-CG to a relative residual of 1e-12;
- * iters (nullable, 2) = its iterations in each pass. */
+ * normal equations are solved by FP64 CG (fast-diagonalisation preconditioner) to a
+ * relative residual of 1e-12; iters (nullable, 2) = its iterations in each pass. */
 int aldic_smooth_strain_quadtree(aldic_ctx *ctx, int nn, const double *coords, double h, double smoothness,
                                  double filter_std, int filter_size, double *F, int *iters);
 

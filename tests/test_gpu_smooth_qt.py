@@ -1,7 +1,7 @@
 """funSmoothStrainQuadtree on the device (aldic_smooth_strain_quadtree: csrc/smooth_qt.hip)
 against the CPU restatement (oracle/smooth_oracle.py: regularizeNd's normal equations by a
 sparse direct solve, nanconv, two passes) on quadtree meshes with a hole and on a full
-grid; FP64 CG to a relative residual of 1e-12 against the direct solve."""
+grid; FP64 preconditioned CG to a relative residual of 1e-12 against the direct solve."""
 import os
 import sys
 
