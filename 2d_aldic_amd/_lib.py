@@ -45,7 +45,8 @@ class FrameOpts(C.Structure):
 
 class QFrameOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("mu", C.c_double), ("beta", C.c_double),
-                ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double)]
+                ("n_admm", C.c_int), ("cg_maxit", C.c_int), ("cg_rtol", C.c_double),
+                ("smooth_h", C.c_double), ("smoothness", C.c_double)]
 
 
 _lib = None
@@ -421,7 +422,7 @@ class Context:
         return F, Fgp, xy
 
     def qframe_setup(self, ws, tri, w, tol=1e-2, mu=1e-3, beta=None, n_admm=3, step=None,
-                     cg_maxit=5000, cg_rtol=1e-10):
+                     cg_maxit=5000, cg_rtol=1e-10, smooth_h=0.0, smoothness=1e-4):
         """Quadtree frame on the mesh of set_quadtree: per-node windows ws (LocalICGN and
         Subpb1), Gauss-point weights tri / w; beta defaults to 1e-3 step^2 mu
         (main_ALDIC_Quadtree.m:202) with step = mean(ws) / 2."""
@@ -430,7 +431,7 @@ class Context:
             st = float(np.mean(ws)) / 2.0 if step is None else float(step)
             beta = 1e-3 * st ** 2 * mu
         o = QFrameOpts(float(tol), float(mu), float(beta), int(n_admm), int(cg_maxit),
-                       float(cg_rtol))
+                       float(cg_rtol), float(smooth_h), float(smoothness))
         self._ck(self.L.aldic_qframe_setup(self.h, C.byref(o), _p(ws),
                                            _p(_c(tri, np.int32)), _p(_c(w, np.float64))),
                  "aldic_qframe_setup")

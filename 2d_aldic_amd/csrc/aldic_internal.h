@@ -125,6 +125,11 @@ hipError_t gridfit_scattered(GridfitDev &g, const double *xnodes_dev, const doub
                              int np, const double *px, const double *py, const double2 *pd, int npair,
                              double *U, double rtol, int maxit, DBuf *work, double *stats_dev,
                              hipStream_t st);
+// F2 = 0.1 F2 + 0.9 F1 (main_ALDIC_Quadtree.m:221,283), n values
+hipError_t launch_blend_f(double *F2, const double *F1, long n, hipStream_t st);
+// the grid min:h:max of node coordinates (host, nn x 2): nx, ny and each node's grid index
+// iy * nx + ix; false if a node is off the grid or the grid is too small or too large
+bool smooth_grid(int nn, const double *coords, double h, int &nx, int &ny, std::vector<int> &gidx);
 // funSmoothStrainQuadtree's two passes on F (4 per node, device, in place): nn nodes on
 // the nx x ny grid at grid indices gidx (device); w[0..1] work; iters[2] CG iterations
 hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double smoothness, double sigma, double rtol,
@@ -147,6 +152,9 @@ struct QFrameState {
     int pcg_budget = 64;              // PCG iterations enqueued per global step unpolled
     int pcg_reruns = 0;               // frames re-run with polled PCG
     int pcg_most = 0;                 // most PCG iterations of one global step, last frame
+    DBuf sgidx, sw[2];                // strain smoothing: node -> grid index, work
+    int snx = 0, sny = 0;             // its grid
+    int smooth_iters = 0;             // its CG iterations in the last frame
     DBuf pstat;                       // {converged, iterations, residual} per global step
     DBuf nat_err;                     // refills whose cell could not be formed (device int)
     int *nat_err_h = nullptr;         // pinned copy

@@ -283,7 +283,7 @@ void aldic_destroy(aldic_ctx *ctx)
     ctx->mg = nullptr;
     for (DBuf &bb : ctx->smq) bb.release();
     F.seed.release();
-    for (DBuf *bb : {&F.P6, &F.F0, &F.lflag}) bb->release();
+    for (DBuf *bb : {&F.P6, &F.F0, &F.lflag, &ctx->qf.sgidx, &ctx->qf.sw[0], &ctx->qf.sw[1]}) bb->release();
     if (F.nfb_h) (void)hipHostFree(F.nfb_h);
     ctx->qf.nat_err.release();
     if (F.nat_err_h) (void)hipHostFree(F.nat_err_h);
@@ -479,25 +479,12 @@ int aldic_smooth_strain_quadtree(aldic_ctx *ctx, int nn, const double *coords, d
     const double sd = filter_std == 0 ? 0.5 : filter_std;
     const int fs = filter_size == 0 ? 2 * (int)std::ceil(2.0 * sd) + 1 : filter_size;
     if (fs != 3) return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: only the 3 x 3 filter is built");
-    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
-    for (int i = 0; i < nn; ++i) {
-        x0 = std::min(x0, coords[2 * i]); x1 = std::max(x1, coords[2 * i]);
-        y0 = std::min(y0, coords[2 * i + 1]); y1 = std::max(y1, coords[2 * i + 1]);
-    }
-    if (!std::isfinite(x0 + x1 + y0 + y1)) return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: bad coordinates");
     // the grid min:h:max of each axis; every node must sit on it (the reference's find)
-    const long nxl = (long)std::floor((x1 - x0) / h + 1e-9) + 1, nyl = (long)std::floor((y1 - y0) / h + 1e-9) + 1;
-    if (nxl < 3 || nyl < 3 || nxl * nyl > (1L << 26))
-        return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: grid must be 3..2^26 points with >= 3 per axis");
-    const int nx = (int)nxl, ny = (int)nyl;
-    std::vector<int> gidx(nn);
-    for (int i = 0; i < nn; ++i) {
-        const long ix = std::lround((coords[2 * i] - x0) / h), iy = std::lround((coords[2 * i + 1] - y0) / h);
-        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || x0 + ix * h != coords[2 * i] ||
-            y0 + iy * h != coords[2 * i + 1])
-            return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: a node is off the grid min:h:max");
-        gidx[i] = (int)(iy * nx + ix);
-    }
+    int nx = 0, ny = 0;
+    std::vector<int> gidx;
+    if (!smooth_grid(nn, coords, h, nx, ny, gidx))
+        return fail(ctx, ALDIC_ERR_ARG, "smooth_strain_quadtree: a node is off the grid min:h:max, or the grid "
+                                        "has fewer than 3 points per axis or more than 2^26");
     CK(hipSetDevice(ctx->device));
     DBuf *w = ctx->smq;
     CK(w[2].ensure(sizeof(int) * (size_t)nn));
@@ -2104,6 +2091,21 @@ int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int 
     CK(aldic::quad_assemble(q, opts->beta, opts->mu, st));
     CK(hipStreamSynchronize(st));
     Q.o = *opts;
+    Q.snx = Q.sny = 0;
+    if (opts->smooth_h > 0) {
+        // funSmoothStrainQuadtree's grid min:h:max and each node's place on it
+        if (!(opts->smoothness > 0)) return fail(ctx, ALDIC_ERR_ARG, "qframe_setup: smoothness must be > 0");
+        std::vector<double> xy(2 * (size_t)S);
+        CK(hipMemcpy(xy.data(), q.m.coords.p, sizeof(double) * 2 * (size_t)S, hipMemcpyDeviceToHost));
+        std::vector<int> gidx;
+        int nx = 0, ny = 0;
+        if (!smooth_grid(S, xy.data(), opts->smooth_h, nx, ny, gidx))
+            return fail(ctx, ALDIC_ERR_ARG, "qframe_setup: a node is off the smoothing grid min:smooth_h:max");
+        CK(up(Q.sgidx, gidx.data(), gidx.size(), ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));   // gidx is a local
+        Q.snx = nx;
+        Q.sny = ny;
+    }
     Q.S = S;
     Q.ready = true;
     return ALDIC_OK;
@@ -2171,6 +2173,17 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     CK(natural_fill(q.nat, q.m.coords.as<double>(), Q.bad.as<int>(), U1, F1, st,
                     Q.nat_err.as<int>()));
     CK(hipEventRecord(Q.ev[1], st));
+    // funSmoothStrainQuadtree (main_ALDIC_Quadtree.m:170-172, 221, 283-284), when on
+    const bool smooth = o.smooth_h > 0 && Q.snx > 0;
+    Q.smooth_iters = 0;
+    auto smooth_F = [&](double *Fs) -> hipError_t {
+        int it[2] = {0, 0};
+        const hipError_t e = smooth_strain_qt(S, Q.sgidx.as<int>(), Q.snx, Q.sny, o.smoothness, 0.5, 1e-12, 20000,
+                                              Fs, Q.sw, it, st);
+        Q.smooth_iters += it[0] + it[1];
+        return e;
+    };
+    if (smooth) CK(smooth_F(F1));
 
     double *F2buf[2] = {Q.F2a.as<double>(), Q.F2b.as<double>()};
     int cur = 0;
@@ -2178,7 +2191,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
     static thread_local DBuf work[8];
     int nglob = 0;
     float t_glob = 0.f;
-    auto global_step = [&]() -> int {
+    auto global_step = [&](bool first) -> int {
         CK(hipEventRecord(Q.ev[2], st));
         CK(aldic::quad_rhs(q, o.beta, o.mu, U1, F1p, Q.b.as<double>(), st));
         double *rec = Q.pstat.as<double>() + 3 * nglob++;
@@ -2193,6 +2206,12 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
                                  F2buf[cur], st));
         // hole-edge nodes keep Subpb1's F (main_ALDIC_Quadtree.m:222,225,286,288)
         CK(aldic::quad_copy_dirichlet(q, F1p, F2buf[cur], st));
+        if (smooth) {
+            // F2 = smooth(0.1 F2 + 0.9 F1), the hole edge copied again in the ADMM loop
+            CK(launch_blend_f(F2buf[cur], F1p, 4 * (long)S, st));
+            CK(smooth_F(F2buf[cur]));
+            if (!first) CK(aldic::quad_copy_dirichlet(q, F1p, F2buf[cur], st));
+        }
         CK(launch_sub(F2buf[cur], F1p, Q.udual.as<double>(), 4 * (long)S, st));
         CK(launch_sub(U2, U1, Q.vdual.as<double>(), 2 * (long)S, st));
         CK(hipEventRecord(Q.ev[3], st));
@@ -2204,7 +2223,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
         }
         return ALDIC_OK;
     };
-    int r = global_step();
+    int r = global_step(true);
     if (r) return r;
     for (int step = 0; step < o.n_admm; ++step) {
         // Subpb1Quadtree from (U2, F2) with the LocalICGN Hessian, on the same windows
@@ -2243,7 +2262,7 @@ static int qframe_pass(aldic_ctx *ctx, const double *U0, bool polled)
                         Q.nat_err.as<int>()));
         F1p = F2buf[cur];  // FSubpb1 = FSubpb2 (main_ALDIC_Quadtree.m:258)
         cur ^= 1;
-        if ((r = global_step())) return r;
+        if ((r = global_step(false))) return r;
     }
     Q.cur_F2 = cur;
     // one wait per frame: the PCG records

@@ -270,6 +270,12 @@ __global__ void k_sq_gather(int nn, const int *__restrict__ gidx, const double4 
     if (i < nn) F[i] = G[gidx[i]];
 }
 
+__global__ void k_blend_f(double *__restrict__ F2, const double *__restrict__ F1, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) F2[i] = 0.1 * F2[i] + 0.9 * F1[i];
+}
+
 // T = L^T L for n points (L: second differences (1, -2, 1) at the interior points) and
 // its eigen-decomposition by cyclic Jacobi rotations: V (row-major, column j the
 // eigenvector of lam[j]).  Cached per n (host).
@@ -329,6 +335,35 @@ const FdAxis &fd_axis(int n)
 }
 
 }  // namespace
+
+hipError_t launch_blend_f(double *F2, const double *F1, long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_blend_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, F2, F1, n);
+    return hipGetLastError();
+}
+
+bool smooth_grid(int nn, const double *coords, double h, int &nx, int &ny, std::vector<int> &gidx)
+{
+    if (nn < 1 || !(h > 0)) return false;
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int i = 0; i < nn; ++i) {
+        x0 = std::min(x0, coords[2 * i]); x1 = std::max(x1, coords[2 * i]);
+        y0 = std::min(y0, coords[2 * i + 1]); y1 = std::max(y1, coords[2 * i + 1]);
+    }
+    if (!std::isfinite(x0 + x1 + y0 + y1)) return false;
+    const long nxl = (long)std::floor((x1 - x0) / h + 1e-9) + 1, nyl = (long)std::floor((y1 - y0) / h + 1e-9) + 1;
+    if (nxl < 3 || nyl < 3 || nxl * nyl > (1L << 26)) return false;
+    nx = (int)nxl;
+    ny = (int)nyl;
+    gidx.resize(nn);
+    for (int i = 0; i < nn; ++i) {
+        const long ix = std::lround((coords[2 * i] - x0) / h), iy = std::lround((coords[2 * i + 1] - y0) / h);
+        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || x0 + ix * h != coords[2 * i] || y0 + iy * h != coords[2 * i + 1])
+            return false;
+        gidx[i] = (int)(iy * nx + ix);
+    }
+    return true;
+}
 
 // F (4 nn, in/out, on the device): one call = both passes.  iters[0..1] = CG iterations.
 hipError_t smooth_strain_qt(int nn, const int *gidx_dev, int nx, int ny, double smoothness, double sigma, double rtol,

@@ -567,7 +567,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         check(aldic_frame_scalars(g_ctx, mxGetDoubles(plhs[0]), 4), "frame_scalars");
     } else if (cmd == "qframe_setup") {
         // aldic_mex('qframe_setup', winsizeList, tri (1-based, nn x 3), w, tol, mu, beta,
-        //           n_admm, cg_maxit, cg_rtol)
+        //           n_admm, cg_maxit, cg_rtol [, elementMinSize (0: no strain smoothing),
+        //           StrainSmoothness (1e-4)])
         need(nrhs, 10, cmd.c_str());
         const std::vector<int> ws = ints(prhs[1], 0);
         const size_t nn = ws.size();
@@ -586,6 +587,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         o.n_admm = (int)mxGetScalar(prhs[7]);
         o.cg_maxit = (int)mxGetScalar(prhs[8]);
         o.cg_rtol = mxGetScalar(prhs[9]);
+        o.smooth_h = nrhs > 10 ? mxGetScalar(prhs[10]) : 0.0;
+        o.smoothness = nrhs > 11 ? mxGetScalar(prhs[11]) : 1e-4;
         check(aldic_qframe_setup(g_ctx, &o, ws.data(), tri.data(), w.data()), "qframe_setup");
     } else if (cmd == "qframe_run") {  // aldic_mex('qframe_run', U0)
         need(nrhs, 2, cmd.c_str());

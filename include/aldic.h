@@ -364,8 +364,8 @@ int aldic_frame_scalars(aldic_ctx *ctx, double *out, int n);
  *   Subpb1Quadtree (2-DOF on the same per-node windows, its winsizeList of
  *   Subpb1Quadtree.m:50, with LocalICGN's Hessian; the max(mean + 0.15 std, 20) rule;
  *   F1 = F2), Subpb2Quadtree, nodal strain, hole-edge F, duals (:251-312).
- * Bad nodes are refilled from their mesh neighbours (inverse-distance average, sweeps
- * outward; the reference uses scatteredInterpolant 'natural').  The nodal strain uses
+ * Bad nodes are refilled by natural-neighbour (Sibson) interpolation of the good nodes,
+ * as the reference's scatteredInterpolant 'natural' (see aldic_natural_fill).  The nodal strain uses
  * the Gauss-point interpolation weights tri / w (nn x 3) as in
  * aldic_nodal_strain_quadtree. */
 typedef struct {
@@ -374,6 +374,12 @@ typedef struct {
     int n_admm;        /* ADMM steps after step 1 (3) */
     int cg_maxit;      /* PCG iterations per global step */
     double cg_rtol;    /* PCG relative residual */
+    double smooth_h;   /* > 0: funSmoothStrainQuadtree as main_ALDIC_Quadtree.m:170-172,
+                        * 221, 283-284 call it (on F1 after LocalICGN, on 0.1 F2 + 0.9 F1
+                        * after every global step, hole-edge F copied before and, in the
+                        * ADMM loop, after), with elementMinSize smooth_h (every node on
+                        * the grid min:h:max) and the 3 x 3 default filter; 0: off */
+    double smoothness; /* StrainSmoothness (the reference sets 1e-4) */
 } aldic_qframe_opts;
 int aldic_qframe_setup(aldic_ctx *ctx, const aldic_qframe_opts *opts, const int *ws,
                        const int *tri, const double *w);

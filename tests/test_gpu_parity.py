@@ -826,21 +826,31 @@ def test_mirror_chain_config1(cfg1, orc):
     assert np.max(np.abs(U1 - U1s)) <= 1e-3
 
 
-def _qframe_oracle(orc, f, g, gx, gy, xy, el, dirichlet, ws, tri, w, U0, beta, mu, n_admm):
-    """main_ALDIC_Quadtree.m:159-312 restated from the oracle's pieces (smoothing off)."""
+def _qframe_oracle(orc, f, g, gx, gy, xy, el, dirichlet, ws, tri, w, U0, beta, mu, n_admm,
+                   smooth_h=0.0, smoothness=1e-4):
+    """main_ALDIC_Quadtree.m:159-312 restated from the oracle's pieces; smooth_h > 0: with
+    funSmoothStrainQuadtree at :170-172, :221, :283-284 (elementMinSize smooth_h)."""
+    import smooth_oracle as so
     U6, F6, H, it6, _ = orc.icgn6(f, g, gx, gy, xy, U0, ws)
     bad = orc.badpoint(it6, 1)
     U1 = orc.natural_fill(xy, np.where(np.repeat(bad, 2), np.nan, U6), 2, bad)
     F1 = orc.natural_fill(xy, np.where(np.repeat(bad, 4), np.nan, F6), 4, bad)
+    if smooth_h > 0:
+        F1 = so.smooth_strain_quadtree(F1, xy, smooth_h, smoothness)
 
-    def global_step(U1, F1):
+    def global_step(U1, F1, first):
         U2 = orc.subpb2_quadtree(xy, el, dirichlet, beta, mu, U1, F1)
         Fgp, _ = orc.gauss_strain_quadtree(xy, el, U2)
         F2 = orc.gather_weights(Fgp, tri, w).reshape(-1, 4)
         F2[dirichlet] = F1.reshape(-1, 4)[dirichlet]
+        if smooth_h > 0:
+            F2 = so.smooth_strain_quadtree(0.1 * F2.ravel() + 0.9 * F1, xy, smooth_h,
+                                           smoothness).reshape(-1, 4)
+            if not first:
+                F2[dirichlet] = F1.reshape(-1, 4)[dirichlet]
         return U2, F2.ravel()
 
-    U2, F2 = global_step(U1, F1)
+    U2, F2 = global_step(U1, F1, True)
     it2 = None
     for _ in range(n_admm):
         udual, vdual = F2 - F1, U2 - U1
@@ -848,13 +858,15 @@ def _qframe_oracle(orc, f, g, gx, gy, xy, el, dirichlet, ws, tri, w, U0, beta, m
         bad = orc.badpoint(it2, 2)
         U1 = orc.natural_fill(xy, np.where(np.repeat(bad, 2), np.nan, U1), 2, bad)
         F1 = F2
-        U2, F2 = global_step(U1, F1)
+        U2, F2 = global_step(U1, F1, False)
     return dict(U=U2, F=F2, iters6=it6, iters2=it2)
 
 
-def test_qframe_plate_with_hole_matches_oracle(gpu_ctx, speckle, orc):
+@pytest.mark.parametrize("smooth_h", [0.0, 8.0])
+def test_qframe_plate_with_hole_matches_oracle(gpu_ctx, speckle, orc, smooth_h):
     """The quadtree frame on a small plate with a hole (uneven windows 16-64 px, masked
-    hole, hanging nodes, hole-edge Dirichlet nodes) against the oracle's restatement."""
+    hole, hanging nodes, hole-edge Dirichlet nodes) against the oracle's restatement,
+    without and with the reference's strain smoothing (smooth_h = elementMinSize 8)."""
     import importlib
     qt = importlib.import_module("2d_aldic_amd.quadtree")
     n, a = 384, 60.0
@@ -874,13 +886,13 @@ def test_qframe_plate_with_hole_matches_oracle(gpu_ctx, speckle, orc):
     tri, w = qt.linear_weights(gxy, xy)
     mu = 1e-3
     beta = 1e-3 * 16 ** 2 * mu
-    gpu_ctx.qframe_setup(ws, tri, w, mu=mu, beta=beta, n_admm=3, cg_rtol=1e-12)
+    gpu_ctx.qframe_setup(ws, tri, w, mu=mu, beta=beta, n_admm=3, cg_rtol=1e-12, smooth_h=smooth_h)
     gpu_ctx.qframe_run(U0)
     r = gpu_ctx.qframe_get()
     f, _ = orc.normalize(p.f.astype(np.float64))
     g, _ = orc.normalize(p.g.astype(np.float64))
     gx, gy = orc.gradient(f, mask)
-    o = _qframe_oracle(orc, f, g, gx, gy, xy, el, dr, ws, tri, w, U0, beta, mu, 3)
+    o = _qframe_oracle(orc, f, g, gx, gy, xy, el, dr, ws, tri, w, U0, beta, mu, 3, smooth_h=smooth_h)
     assert np.mean(r["iters6"] == o["iters6"]) >= 0.97
     assert np.mean(r["iters2"] == o["iters2"]) >= 0.97
     dU = np.abs(r["U"] - o["U"])
