@@ -8,7 +8,7 @@ cost volumes over displacements of up to 5 cells (20 px) for the four quadrants 
 central 24 x 24 pixels, a soft-argmax of each and a dense 10 -> 6 head.  IC-GN then
 starts from that warp (aldic_icgn6_start); the plain integer search is the fallback for
 every subset whose IC-GN did not converge within 20 iterations or ends with ZNCC below
-0.9, and converged IC-GN stays the judge of every displacement.
+0.9, and converged IC-GN decides every displacement.
 
 Training data come from the synthetic speckle warper (speckle.make_pair): many small
 image pairs, each with its own random translation, displacement gradient and smooth
